@@ -1,0 +1,1620 @@
+// deb_oracle.cpp -- CPU FP64 restatement of DISCO-EB's linear-perturbation path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP product path: it is
+// built into oracle/_build/libdeb_oracle.so and loaded (through oracle/oracle.py) only by
+// tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  Nothing in the
+// product package links or calls it.
+//
+// What it restates (reference = lawrenceberry/DISCO-EB, paths relative to its repo root):
+//   * natural cubic spline ............. src/discoeb/spline_interpolation.py:8-276
+//   * neutrino momentum bins / table .... src/discoeb/background.py:14-69, util.py:83-124
+//   * background set-up, H(a) ........... src/discoeb/background.py:72-189
+//   * RECFAST thermal history ........... src/discoeb/thermodynamics_recfast.py:124-492
+//   * GRKT4 Rosenbrock step ............. src/discoeb/ode_integrators_stiff.py:166-281
+//   * Romberg quadrature ................ jax_cosmo.scipy.integrate.romb (divmax=6), used at
+//                                         background.py:183 and thermodynamics_recfast.py:488
+//   * evolve_background tail ............ src/discoeb/background.py:191-345
+//   * synchronous-gauge RHS ............. src/discoeb/perturbations.py:88-525 (+ approximations.py)
+//   * adiabatic ICs / start time ........ src/discoeb/perturbations.py:680-835
+//   * Rodas5 (transformed) step ......... src/discoeb/ode_integrators_stiff.py:678-892
+//   * PI controller, filtered RMS norm,
+//     SaveAt(ts)+linear interpolation ... perturbations.py:844-925,1059-1080 and diffrax 0.7
+//                                         (PIDController, LocalLinearInterpolation, _clip_to_end)
+//   * RSA output projection ............. src/discoeb/approximations.py:84-176
+//   * output conversion ................. src/discoeb/perturbations.py:528-677
+//
+// Derivatives that the reference takes with JAX forward-mode AD (Jacobian column of the
+// scale factor, df/dtau, the 3x3 RECFAST Jacobian, d Saha_HeII / da) are taken here by the
+// complex-step method, f'(x) = Im f(x + i h) / h, which is exact to rounding for the
+// piecewise-analytic functions involved.  Jacobian columns of the other state entries are
+// read off the linear operator by evaluating the right-hand side on unit vectors.  The
+// linear solves use dense LU with partial pivoting (LAPACK getrf semantics), as the
+// reference's jax.scipy.linalg.lu_factor does.
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef std::complex<double> cplx;
+
+namespace {
+
+// ----------------------------------------------------------------------------------------
+// scalar helpers usable for double and complex<double>
+// ----------------------------------------------------------------------------------------
+inline double re(double x) { return x; }
+inline double re(const cplx &x) { return x.real(); }
+inline double im(double) { return 0.0; }
+inline double im(const cplx &x) { return x.imag(); }
+template <class T> inline T absr(const T &x) { return re(x) < 0 ? -x : x; }       // |x| along the real line
+template <class T> inline T maxr(const T &x, double b) { return re(x) > b ? x : T(b); }
+template <class T> inline T minr(const T &x, double b) { return re(x) < b ? x : T(b); }
+inline double powr(double x, double p) { return std::pow(x, p); }
+inline cplx powr(const cplx &x, double p) { return std::exp(p * std::log(x)); }
+
+// ----------------------------------------------------------------------------------------
+// Natural cubic spline: spline_interpolation.py:8-276
+// ----------------------------------------------------------------------------------------
+struct Spline {
+  int n = 0;
+  bool uniform = false;
+  std::vector<double> x, y, S, Icum;
+  double x0 = 0, dxinv = 0;
+
+  void build(const double *xin, const double *yin, int nn, bool unif) {
+    n = nn;
+    uniform = unif;
+    x.assign(xin, xin + n);
+    y.assign(yin, yin + n);
+    // NaN forward fill (spline_interpolation.py:33-40): y by factor 1, x by factor 1.01
+    for (int i = 1; i < n; ++i) {
+      if (std::isnan(y[i])) y[i] = y[i - 1];
+      if (std::isnan(x[i])) x[i] = x[i - 1] * 1.01;
+    }
+    std::vector<double> h(n - 1);
+    for (int i = 0; i < n - 1; ++i) h[i] = x[i + 1] - x[i];
+    S.assign(n, 0.0);
+    int m = n - 2;
+    if (m > 0) {  // Thomas algorithm, spline_interpolation.py:50-85
+      std::vector<double> cp(m), dp(m);
+      auto bd = [&](int i) { return 2.0 * (h[i] + h[i + 1]); };
+      auto dd = [&](int i) { return 6.0 * ((y[i + 2] - y[i + 1]) / h[i + 1] - (y[i + 1] - y[i]) / h[i]); };
+      cp[0] = h[1] / bd(0);
+      dp[0] = dd(0) / bd(0);
+      for (int i = 1; i < m; ++i) {
+        double denom = bd(i) - h[i] * cp[i - 1];
+        cp[i] = (i < m - 1) ? h[i + 1] / denom : 0.0;
+        dp[i] = (dd(i) - h[i] * dp[i - 1]) / denom;
+      }
+      std::vector<double> Si(m);
+      Si[m - 1] = dp[m - 1];
+      for (int i = m - 2; i >= 0; --i) Si[i] = dp[i] - cp[i] * Si[i + 1];
+      for (int i = 0; i < m; ++i) S[i + 1] = Si[i];
+    }
+    Icum.assign(n, 0.0);
+    for (int i = 0; i < n - 1; ++i) {
+      double a = y[i];
+      double b = (y[i + 1] - y[i]) / h[i] - h[i] * (S[i + 1] + 2 * S[i]) / 6.0;
+      double c = S[i] / 2.0;
+      double d = (S[i + 1] - S[i]) / (6.0 * h[i]);
+      double I = a * h[i] + b * h[i] * h[i] / 2 + c * h[i] * h[i] * h[i] / 3 + d * h[i] * h[i] * h[i] * h[i] / 4;
+      Icum[i + 1] = Icum[i] + I;
+    }
+    x0 = x[0];
+    dxinv = 1.0 / (x[1] - x[0]);
+  }
+
+  // spline_interpolation.py:132-145
+  int index(double xq) const {
+    int idx;
+    if (uniform) {
+      double f = std::floor((xq - x0) * dxinv);
+      if (!(f > 0)) f = 0;  // clip (NaN -> 0)
+      if (f > n - 2) f = n - 2;
+      idx = (int)f;
+    } else {
+      // number of knots strictly below xq, minus one
+      int cnt = (int)(std::lower_bound(x.begin(), x.end(), xq) - x.begin());
+      idx = cnt - 1;
+      if (idx < 0) idx = 0;
+      if (idx > n - 2) idx = n - 2;
+    }
+    return idx;
+  }
+
+  template <class T> T eval(const T &xq) const {  // :147-169
+    int i = index(re(xq));
+    double hl = x[i + 1] - x[i];
+    T t = (xq - x[i]) / hl;
+    T A = 1.0 - t, B = t;
+    return A * y[i] + B * y[i + 1] + ((A * A * A - A) * S[i] + (B * B * B - B) * S[i + 1]) * (hl * hl) / 6.0;
+  }
+  template <class T> T deriv(const T &xq) const {  // :207-228
+    int i = index(re(xq));
+    double hl = x[i + 1] - x[i];
+    T d = xq - x[i];
+    double b = (y[i + 1] - y[i]) / hl - hl * (S[i + 1] + 2 * S[i]) / 6.0;
+    double c = S[i] / 2.0;
+    double dd = (S[i + 1] - S[i]) / (6.0 * hl);
+    return b + 2.0 * c * d + 3.0 * dd * d * d;
+  }
+  double deriv2(double xq) const {  // :230-251
+    int i = index(xq);
+    double hl = x[i + 1] - x[i];
+    double d = xq - x[i];
+    double c = S[i] / 2.0;
+    double dd = (S[i + 1] - S[i]) / (6.0 * hl);
+    return 2 * c + 6 * dd * d;
+  }
+  double integral(double xq) const {  // :171-205 (integrate_from_start=True)
+    int i = index(xq);
+    double hl = x[i + 1] - x[i];
+    double d = xq - x[i];
+    double a = y[i];
+    double b = (y[i + 1] - y[i]) / hl - hl * (S[i + 1] + 2 * S[i]) / 6.0;
+    double c = S[i] / 2.0;
+    double dd = (S[i + 1] - S[i]) / (6.0 * hl);
+    return Icum[i] + a * d + b * d * d / 2 + c * d * d * d / 3 + dd * d * d * d * d / 4;
+  }
+};
+
+// ----------------------------------------------------------------------------------------
+// linspace/geomspace in the form jax.numpy uses (start*(1-i/div) + stop*(i/div), then 10**)
+// ----------------------------------------------------------------------------------------
+void geomspace(double start, double stop, int num, bool endpoint, double *out) {
+  double l0 = std::log10(start), l1 = std::log10(stop);
+  int div = endpoint ? num - 1 : num;
+  if (num == 1) { out[0] = start; return; }
+  for (int i = 0; i < num; ++i) {
+    double s = (double)i / (double)div;
+    double v = (endpoint && i == num - 1) ? l1 : l0 * (1.0 - s) + l1 * s;
+    out[i] = std::pow(10.0, v);
+  }
+}
+void linspace(double start, double stop, int num, bool endpoint, double *out) {
+  int div = endpoint ? num - 1 : num;
+  if (num == 1) { out[0] = start; return; }
+  for (int i = 0; i < num; ++i) {
+    double s = (double)i / (double)div;
+    out[i] = (endpoint && i == num - 1) ? stop : start * (1.0 - s) + stop * s;
+  }
+}
+
+// ----------------------------------------------------------------------------------------
+// Neutrino momentum bins: background.py:14-45; Golub-Welsch: util.py:83-124
+// ----------------------------------------------------------------------------------------
+void sym_eig_jacobi(int n, double *A, double *V) {  // cyclic Jacobi, A n x n symmetric (destroyed)
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) V[i * n + j] = (i == j) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) off += A[p * n + q] * A[p * n + q];
+    if (off < 1e-300) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double apq = A[p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        double app = A[p * n + p], aqq = A[q * n + q];
+        double theta = (aqq - app) / (2 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+        double c = 1 / std::sqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - s * akq;
+          A[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - s * aqk;
+          A[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          double vkp = V[k * n + p], vkq = V[k * n + q];
+          V[k * n + p] = c * vkp - s * vkq;
+          V[k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+}
+
+void nu_bins(int nq, double *q, double *w) {
+  const double fdc = 5.682196976983475;
+  if (nq == 3 || nq == 4 || nq == 5) {
+    static const double q3[] = {0.913201, 3.37517, 7.79184}, w3[] = {0.0687359, 3.31435, 2.29911};
+    static const double q4[] = {0.7, 2.62814, 5.90428, 12.0}, w4[] = {0.0200251, 1.84539, 3.52736, 0.289427};
+    static const double q5[] = {0.583165, 2.0, 4.0, 7.26582, 13.0}, w5[] = {0.0081201, 0.689407, 2.8063, 2.05156, 0.12681};
+    const double *qq = nq == 3 ? q3 : nq == 4 ? q4 : q5, *ww = nq == 3 ? w3 : nq == 4 ? w4 : w5;
+    for (int i = 0; i < nq; ++i) {
+      q[i] = qq[i];
+      double dlfdlq = -q[i] / (1 + std::exp(-q[i]));
+      w[i] = ww[i] / (-0.25 * dlfdlq) / fdc;
+    }
+    return;
+  }
+  // generalized Gauss-Laguerre, alpha = 1: Jacobi matrix diag 2i-1+alpha, offdiag sqrt(i(i+alpha))
+  std::vector<double> A(nq * nq, 0.0), V(nq * nq);
+  for (int i = 1; i <= nq; ++i) A[(i - 1) * nq + (i - 1)] = 2.0 * i;
+  for (int i = 1; i < nq; ++i) {
+    double b = std::sqrt((double)i * (i + 1.0));
+    A[(i - 1) * nq + i] = b;
+    A[i * nq + (i - 1)] = b;
+  }
+  sym_eig_jacobi(nq, A.data(), V.data());
+  std::vector<std::pair<double, double>> ev(nq);
+  for (int j = 0; j < nq; ++j) ev[j] = {A[j * nq + j], V[0 * nq + j] * V[0 * nq + j]};  // weight * Gamma(2)=1
+  std::sort(ev.begin(), ev.end());
+  for (int j = 0; j < nq; ++j) {
+    q[j] = ev[j].first;
+    double ww = ev[j].second;
+    ww *= q[j] * q[j] * q[j] / (1 + std::exp(-q[j])) / q[j];
+    w[j] = ww / fdc;
+  }
+}
+
+// ----------------------------------------------------------------------------------------
+// cosmology parameter vector layout (shared with oracle.py)
+// ----------------------------------------------------------------------------------------
+enum {
+  P_OMEGAM, P_OMEGAB, P_H0, P_TCMB, P_YHE, P_NEFF, P_NMNU, P_MNU, P_W0, P_WA, P_CS2DE, P_OMEGAK, P_AS, P_NS, P_KP,
+  P_USE_TCA, P_TCA_H, P_TCA_K, P_USE_RSA, P_RSA_C, P_RSA_K, P_USE_UFA, P_UFA_K, NPARAM
+};
+
+struct Cosmo {
+  double Omegam, Omegab, H0, Tcmb, YHe, Neff, Nmnu, mnu, w0, wa, cs2DE, Omegak, A_s, n_s, k_p;
+  bool use_tca, use_rsa, use_ufa;
+  double tca_h, tca_k, rsa_c, rsa_k, ufa_k;
+  // derived (background.py:141-189)
+  double amin, amax, grhom, grhog, grhor, adotrad, amnu, Omegamnu, OmegaDE, taumin, taumax, fHe;
+  bool lambda_de;  // w0=-1, wa=0: dark-energy perturbations identically zero
+};
+
+struct Bg {
+  Cosmo c;
+  int num_thermo;
+  std::vector<double> nu_a;                          // 512-point a grid for the neutrino tables
+  Spline logrhonu, logpnu, logppnu;                  // uniform in log a
+  std::vector<double> aexp, tau, xe, xeHI, xeHeI, xeHeII, cs2, Tm, mu, xeprime_recf;
+  Spline tau_of_a, a_of_tau, xe_of_tau, cs2a_of_tau, tempba_of_tau, xe_of_loga, cs2a_of_loga;
+  std::vector<double> xe_vis, xeprime, opac, optical_depth, gvis, gvisprime, gvispprime;
+  Spline opacspline;
+  double tau0, tau_maxvis, optical_depth_today;
+  int recfast_maxstep_hits = 0;
+};
+
+// rhonu/rhonu0 etc. for one massive flavour, background.py:48-69 (always nq = 8)
+void nu_background(double a, double amnu, const double *q, const double *w, double *rho, double *p, double *pp) {
+  double r = 0, ps = 0, pps = 0;
+  for (int i = 0; i < 8; ++i) {
+    double aq = a * amnu / q[i];
+    double v = 1.0 / std::sqrt(1 + aq * aq);
+    r += w[i] / v;
+    ps += w[i] * v / 3;
+    pps += w[i] * v * v * v / 3;
+  }
+  *rho = r; *p = ps; *pp = pps;
+}
+
+// conformal Hubble rate a'/a, background.py:101-123
+template <class T> T aprimeoa(const Bg &bg, const T &a) {
+  const Cosmo &c = bg.c;
+  T rhonu = std::exp(bg.logrhonu.eval(T(std::log(a))));
+  T rho_Q = powr(a, -3 * (1 + c.w0 + c.wa)) * std::exp(3.0 * (a - 1.0) * c.wa);
+  T grho = c.grhom * c.Omegam / a + (c.grhog + c.grhor * (c.Neff + c.Nmnu * rhonu)) / (a * a) +
+           c.grhom * c.OmegaDE * rho_Q * a * a + c.grhom * c.Omegak;
+  return std::sqrt(grho / 3.0);
+}
+// dtau/da, background.py:72-81
+double dtauda(const Bg &bg, double a) {
+  const Cosmo &c = bg.c;
+  double rhonu = std::exp(bg.logrhonu.eval(std::log(a)));
+  double rho_DE = std::pow(a, -3 * (1 + c.w0 + c.wa)) * std::exp(3 * (a - 1) * c.wa);
+  double grho2 = c.grhom * c.Omegam * a + (c.grhog + c.grhor * (c.Neff + c.Nmnu * rhonu)) +
+                 c.grhom * c.OmegaDE * rho_DE * a * a * a * a + c.grhom * c.Omegak * a * a;
+  return std::sqrt(3.0 / grho2);
+}
+
+// jax_cosmo.scipy.integrate.romb with divmax = 6: Romberg R(6,6) on 64 panels
+template <class F> double romb(F f, double a, double b) {
+  const int divmax = 6;
+  double range = b - a;
+  double ordsum = 0.5 * (f(a) + f(b));
+  double prev[divmax + 1], cur[divmax + 1];
+  prev[0] = range * ordsum;
+  for (int k = 1; k <= divmax; ++k) prev[k] = prev[0];
+  for (int i = 1; i <= divmax; ++i) {
+    int n = 1 << i;
+    int numtosum = n / 2;
+    double h = range / numtosum;
+    double lox = a + 0.5 * h;
+    double s = 0;
+    for (int j = 0; j < numtosum; ++j) s += f(lox + h * j);
+    ordsum += s;
+    cur[0] = range * ordsum / n;
+    for (int k = 0; k < divmax; ++k) {
+      double t4 = std::pow(4.0, k + 1);
+      cur[k + 1] = (t4 * cur[k] - prev[k]) / (t4 - 1.0);
+    }
+    std::memcpy(prev, cur, sizeof(prev));
+  }
+  return prev[divmax];
+}
+
+// ----------------------------------------------------------------------------------------
+// RECFAST: thermodynamics_recfast.py:37-492
+// ----------------------------------------------------------------------------------------
+namespace rf {
+const double G = 6.67430e-11, mH = 1.67353284e-27, me = 9.1093837015e-31, mHe_mH = 3.97146570884;
+const double c = 2.99792458e+08, h = 6.62607015e-34, kB = 1.380649e-23, sigma = 6.6524587321e-29;
+const double arad = 4 * 5.670374419e-8 / c;
+const double bigH = 3.2407792902755102e-18, dens_fac = 11.223810928601939;
+const double EionHe12s = 6.314878282674e5;
+const double Lambda = 8.2245809, Lambda_He = 51.3;
+const double L_H_ion = 1.096787737e7, L_H_alpha = 8.225916453e6, L_He1_ion = 1.98310772e7, L_He2_ion = 4.389088863e7;
+const double L_He_2s = 1.66277434e7, L_He_2p = 1.71134891e7;
+const double A2P_s = 1.798287e9, A2P_t = 177.58, L_He_2Pt = 1.690871466e7, L_He_2St = 1.5985597526e7;
+const double L_He2St_ion = 3.8454693845e6, sigma_He_2Ps = 1.436289e-22, sigma_He_2Pt = 1.484872e-22;
+const double AGauss1 = -0.14, AGauss2 = 0.05, zGauss1 = 7.28, zGauss2 = 6.75, wGauss1 = 0.18, wGauss2 = 0.33;
+const double H_frac = 1.e-3;
+const double PI = 3.141592653589793;
+const double Lalpha = 1.0 / L_H_alpha, Lalpha_He = 1.0 / L_He_2p;
+const double DeltaB = h * c * (L_H_ion - L_H_alpha), CDB = DeltaB / kB;
+const double DeltaB_He = h * c * (L_He1_ion - L_He_2s), CDB_He = DeltaB_He / kB;
+const double CB1 = h * c * L_H_ion / kB, CB1_He1 = h * c * L_He1_ion / kB, CB1_He2 = h * c * L_He2_ion / kB;
+const double CR = 2.0 * PI * (me / h) * (kB / h);
+const double CK = Lalpha * Lalpha * Lalpha / (8.0 * PI), CK_He = Lalpha_He * Lalpha_He * Lalpha_He / (8.0 * PI);
+const double CL = c * h / (kB * Lalpha), CL_He = c * h / (kB / L_He_2s);
+const double CT = (8.0 / 3.0) * (sigma / (me * c)) * arad;
+const double Bfact = h * c * (L_He_2p - L_He_2s) / kB;
+const double CL_PSt = h * c * (L_He_2Pt - L_He_2St) / kB;
+const double hcL2St_kB = h * c * L_He_2St / kB;
+const double a_PPB = 4.309, b_PPB = -0.6166, c_PPB = 0.6703, d_PPB = 0.5300;
+const double a_VF = std::pow(10., -16.744), b_VF = 0.711, T_0 = std::pow(10., 0.477121), T_1 = std::pow(10., 5.114);
+const double a_trip = std::pow(10., -16.306), b_trip = 0.761;
+}  // namespace rf
+
+// ionization RHS d(x_H, x_He, T_m)/da, thermodynamics_recfast.py:124-281
+template <class T> void ionization(const Bg &bg, const T &a, const T *y, T *f) {
+  using namespace rf;
+  const Cosmo &cp = bg.c;
+  T z = 1.0 / a - 1.0;
+  double H = cp.H0 / 100.0, HO = H * bigH, fu = 1.105;
+  double mu_H = 1.0 / (1.0 - cp.YHe);
+  double fHe = cp.YHe / (mHe_mH * (1.0 - cp.YHe));
+  double Nnow = dens_fac * H * H * cp.Omegab / mu_H;
+  T x_H = y[0], x_He = y[1];
+  T x = x_H + fHe * x_He;
+  T Tmat = absr(y[2]);
+  T zt = 1.0 + z;
+  T n = Nnow * zt * zt * zt;
+  T n_He = fHe * n;
+  T Trad = cp.Tcmb * zt;
+  T Hz = (1e-5 * aprimeoa<T>(bg, a)) / a * c * bigH;
+  T Tmat_1e4 = Tmat / 1e4;
+  T CR_Tmat_15 = powr(CR * Tmat, 1.5);
+  T Rdown = 1e-19 * a_PPB * powr(Tmat_1e4, b_PPB) / (1.0 + c_PPB * powr(Tmat_1e4, d_PPB));
+  T Rup = Rdown * CR_Tmat_15 * std::exp(-CDB / Tmat);
+  T sq_0 = std::sqrt(Tmat / T_0), sq_1 = std::sqrt(Tmat / T_1);
+  T sq_0_term = 1.0 + sq_0, sq_1_term = 1.0 + sq_1;
+  T Rdown_He = a_VF / (sq_0 * powr(sq_0_term, 1 - b_VF) * powr(sq_1_term, 1 + b_VF));
+  T Rup_He = 4.0 * Rdown_He * CR_Tmat_15 * std::exp(-CDB_He / Tmat);
+  T He_Boltz = std::exp(minr(Bfact / Tmat, 680.0));
+  T Rdown_trip = a_trip / (sq_0 * powr(sq_0_term, 1 - b_trip) * powr(sq_1_term, 1 + b_trip));
+  T Rup_trip = Rdown_trip * std::exp(-h * c * L_He2St_ion / (kB * Tmat)) * CR_Tmat_15 * (4.0 / 3.0);
+  T lz = std::log(zt);
+  T g1 = (lz - zGauss1) / wGauss1, g2 = (lz - zGauss2) / wGauss2;
+  T K_gaussian = 1.0 + AGauss1 * std::exp(-g1 * g1) + AGauss2 * std::exp(-g2 * g2);
+  T K = CK / Hz * K_gaussian;
+  T omxHe = 1.0 - x_He, omxH = 1.0 - x_H;
+  T nHe1mx = n_He * omxHe;
+  T tauHe_s = A2P_s * CK_He * 3.0 * nHe1mx / Hz;
+  T pHe_s = (1.0 - std::exp(-tauHe_s)) / tauHe_s;
+  T Doppler_term = std::sqrt(2.0 * kB * Tmat / (mH * mHe_mH * c * c));
+  T Doppler_2p = c * L_He_2p * Doppler_term;
+  T Doppler_2Pt = c * L_He_2Pt * Doppler_term;
+  T gamma_2Ps = (3.0 * A2P_s * fHe * omxHe * c * c) /
+                (std::sqrt(PI) * sigma_He_2Ps * 8.0 * PI * Doppler_2p * omxH * ((c * L_He_2p) * (c * L_He_2p)));
+  const double pb = 0.36, qb = 0.86;
+  T AHcon = A2P_s / (1.0 + pb * powr(gamma_2Ps, qb));
+  bool K_He_cond1 = re(x_He) < 5e-9 || re(x_He) > 0.98;
+  bool K_He_cond2 = re(x_H) < 0.9999999;
+  T K_He = K_He_cond1 ? T(CK_He / Hz)
+                      : (K_He_cond2 ? T(1.0 / ((A2P_s * pHe_s + AHcon) * 3.0 * nHe1mx)) : T(1.0 / (A2P_s * pHe_s * 3.0 * nHe1mx)));
+  T tauHe_t = A2P_t * nHe1mx * 3.0 / (8.0 * PI * Hz * (L_He_2Pt * L_He_2Pt * L_He_2Pt));
+  T pHe_t = (1.0 - std::exp(-tauHe_t)) / tauHe_t;
+  T gamma_2Pt = (3.0 * A2P_t * fHe * omxHe * c * c) /
+                (std::sqrt(PI) * sigma_He_2Pt * 8.0 * PI * Doppler_2Pt * omxH * ((c * L_He_2Pt) * (c * L_He_2Pt)));
+  const double pb_t = 0.66, qb_t = 0.9;
+  T AHcon_t = A2P_t / (1.0 + pb_t * powr(gamma_2Pt, qb_t)) / 3.0;
+  T exp_CL_PSt = std::exp(-CL_PSt / Tmat);
+  T CfHe_t = re(x_H) > 0.99999 ? T(A2P_t * pHe_t * exp_CL_PSt) : T((A2P_t * pHe_t + AHcon_t) * exp_CL_PSt);
+  CfHe_t = CfHe_t / (Rup_trip + CfHe_t);
+  T Trad2 = Trad * Trad;
+  T timeTh = (1.0 / (CT * Trad2 * Trad2)) * (1.0 + x + fHe) / x;
+  T timeH = 2.0 / (3.0 * HO * powr(zt, 1.5));
+  T Hz_zt = Hz * zt;
+  T rate_diff = x * x_H * n * Rdown - Rup * omxH * std::exp(-CL / Tmat);
+  T KLn = K * Lambda * n * omxH;
+  T denom_term = 1.0 / fu + KLn / fu + K * Rup * n * omxH;
+  T f0;
+  if (re(x_H) > 0.99) f0 = 0.0;
+  else if (re(x_H) > 0.985) f0 = rate_diff / Hz_zt;
+  else f0 = (rate_diff * (1.0 + KLn)) / (Hz_zt * denom_term);
+  T rate_diff_He = x * x_He * n * Rdown_He - Rup_He * omxHe * std::exp(-CL_He / Tmat);
+  T KHe_term = K_He * Lambda_He * nHe1mx * He_Boltz;
+  T He_denom = 1.0 + K_He * (Lambda_He + Rup_He) * nHe1mx * He_Boltz;
+  T f1_main = (rate_diff_He * (1.0 + KHe_term)) / (Hz_zt * He_denom);
+  T trip_rate_diff = x * x_He * n * Rdown_trip - omxHe * 3.0 * Rup_trip * std::exp(-hcL2St_kB / Tmat);
+  T trip_contrib = trip_rate_diff * CfHe_t / Hz_zt;
+  bool trip_cond = re(x_He) < 5e-9 || re(x_He) > 0.98;
+  T f1 = re(x_He) < 1e-8 ? T(0.0) : T(f1_main + (trip_cond ? T(0.0) : trip_contrib));
+  T f2;
+  if (re(timeTh) < H_frac * re(timeH)) f2 = Tmat / zt;
+  else f2 = CT * (Trad2 * Trad2) * x / (1.0 + x + fHe) * (Tmat - Trad) / Hz_zt + 2.0 * Tmat / zt;
+  T dzda = -1.0 / (a * a);
+  f[0] = f0 * dzda;
+  f[1] = f1 * dzda;
+  f[2] = f2 * dzda;
+}
+
+// GRKT4 (GRK4A coefficients) step: ode_integrators_stiff.py:166-281.  All stages use the
+// vector field at t0 (the reference reuses the name t1 = t0 for the stage times).
+void grkt4_step(const Bg &bg, double t0, double dt, const double *y0, double *y1, double *yerr) {
+  const double gamma = 0.395, g21 = -0.767672395484, g31 = -0.851675323742, g32 = 0.522967289188;
+  const double g41 = 0.288463109545, g42 = 0.880214273381e-1, g43 = -0.337389840627;
+  const double a21 = 0.438, a31 = 0.796920457938, a32 = 0.730795420615e-1;
+  const double ch1 = 0.346325833758, ch2 = 0.285693175712, ch3 = 0.367980990530;
+  const double c1 = 0.199293275701, c2 = 0.482645235674, c3 = 0.680614886256e-1, c4 = 0.25;
+  auto vf = [&](const double *y, double *f) {
+    ionization<double>(bg, t0, y, f);
+    for (int i = 0; i < 3; ++i) f[i] *= dt;
+  };
+  // J = d(dt f)/dy by complex step
+  double J[3][3];
+  for (int j = 0; j < 3; ++j) {
+    cplx yc[3], fc[3];
+    for (int i = 0; i < 3; ++i) yc[i] = y0[i];
+    double hstep = 1e-30 * std::max(1.0, std::fabs(y0[j]));
+    yc[j] += cplx(0, hstep);
+    ionization<cplx>(bg, cplx(t0), yc, fc);
+    for (int i = 0; i < 3; ++i) J[i][j] = dt * fc[i].imag() / hstep;
+  }
+  double A[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) A[i][j] = (i == j ? 1.0 : 0.0) - gamma * J[i][j];
+  // 3x3 LU without pivoting (:77-113)
+  double u00 = A[0][0], u01 = A[0][1], u02 = A[0][2];
+  double l10 = A[1][0] / u00, u11 = A[1][1] - l10 * u01, u12 = A[1][2] - l10 * u02;
+  double l20 = A[2][0] / u00, r21 = A[2][1] - l20 * u01, r22 = A[2][2] - l20 * u02;
+  double l21 = r21 / u11, u22 = r22 - l21 * u12;
+  auto solve = [&](const double *b, double *x) {
+    double z0 = b[0], z1 = b[1] - l10 * z0, z2 = b[2] - l20 * z0 - l21 * z1;
+    x[2] = z2 / u22;
+    x[1] = (z1 - u12 * x[2]) / u11;
+    x[0] = (z0 - u01 * x[1] - u02 * x[2]) / u00;
+  };
+  auto Jmv = [&](const double *v, double *o) {
+    for (int i = 0; i < 3; ++i) o[i] = J[i][0] * v[0] + J[i][1] * v[1] + J[i][2] * v[2];
+  };
+  double b[3], k1[3], k2[3], k3[3], k4[3], yy[3], f[3], Jk1[3], Jk2[3], Jk3[3];
+  vf(y0, b);
+  solve(b, k1);
+  for (int i = 0; i < 3; ++i) yy[i] = y0[i] + a21 * k1[i];
+  vf(yy, f);
+  Jmv(k1, Jk1);
+  for (int i = 0; i < 3; ++i) b[i] = f[i] + g21 * Jk1[i];
+  solve(b, k2);
+  for (int i = 0; i < 3; ++i) yy[i] = y0[i] + a31 * k1[i] + a32 * k2[i];
+  vf(yy, f);
+  Jmv(k2, Jk2);
+  for (int i = 0; i < 3; ++i) b[i] = f[i] + g31 * Jk1[i] + g32 * Jk2[i];
+  solve(b, k3);
+  Jmv(k3, Jk3);
+  for (int i = 0; i < 3; ++i) b[i] = f[i] + g41 * Jk1[i] + g42 * Jk2[i] + g43 * Jk3[i];
+  solve(b, k4);
+  for (int i = 0; i < 3; ++i) {
+    y1[i] = y0[i] + c1 * k1[i] + c2 * k2[i] + c3 * k3[i] + c4 * k4[i];
+    yerr[i] = (c1 - ch1) * k1[i] + (c2 - ch2) * k2[i] + (c3 - ch3) * k3[i] + c4 * k4[i];
+  }
+}
+
+// diffeqsolve(GRKT4, PIDController(rtol=1e-3, atol=1e-6), dt0=|t0|*1e-3, max_steps=128,
+// throw=False, SaveAt(t1=True)): thermodynamics_recfast.py:283-301.  Default controller:
+// I-controller, safety 0.9, factormin 0.2, factormax 10, error order 3, RMS norm.
+bool solve_ionization(const Bg &bg, double astart, double aend, const double *ystart, double *yend) {
+  const double rtol = 1e-3, atol = 1e-6;
+  double y[3] = {ystart[0], ystart[1], ystart[2]};
+  double tprev = astart, t1 = aend;
+  double tnext = std::min(astart + std::fabs(astart * 1e-3), t1);
+  int nsteps = 0;
+  while (tprev < t1 && nsteps < 128) {
+    double y1[3], err[3];
+    grkt4_step(bg, tprev, tnext - tprev, y, y1, err);
+    double s = 0;
+    bool nan = std::isnan(y1[0]) || std::isnan(y1[1]) || std::isnan(y1[2]);
+    for (int i = 0; i < 3; ++i) {
+      double yy = std::max(std::fabs(y[i]), std::fabs(nan ? y[i] : y1[i]));
+      double e = err[i] / (atol + yy * rtol);
+      s += e * e;
+    }
+    double en = std::sqrt(s / 3.0);
+    bool keep = en < 1.0;
+    double inv = 1.0 / en;
+    double factor = 0.9 * std::pow(inv, 1.0 / 3.0);
+    double fmin = keep ? 1.0 : 0.2;
+    factor = std::min(std::max(factor, fmin), 10.0);
+    double dt = (tnext - tprev) * factor;
+    double next_t0 = keep ? tnext : tprev;
+    if (keep) { y[0] = y1[0]; y[1] = y1[1]; y[2] = y1[2]; }
+    tprev = std::min(next_t0, t1);
+    double tn = next_t0 + dt;
+    tnext = (tn > t1 - 1e-10) ? t1 : tn;
+    ++nsteps;
+  }
+  yend[0] = y[0]; yend[1] = y[1]; yend[2] = y[2];
+  return tprev >= t1;
+}
+
+// Saha_HeII, thermodynamics_recfast.py:303-322
+template <class T> T saha_HeII(const Cosmo &cp, const T &a) {
+  using namespace rf;
+  double Hfac = 1 / (1.0e+06 * 3.0856775807e+13);
+  double mu_H = 1 / (1 - cp.YHe);
+  double rho_c = 3 * (cp.H0 * Hfac) * (cp.H0 * Hfac) / (8 * PI * G);
+  T NH = rho_c * cp.Omegab / (mH * mu_H) / (a * a * a);
+  T Temp = cp.Tcmb / a;
+  T betaE = EionHe12s / Temp;
+  double A = 1 + cp.fHe, B = 1 + 2 * cp.fHe;
+  T R = powr(2 * PI * me * kB / (h * h) * Temp, 1.5) / NH * std::exp(-betaE);
+  if (re(R) > 1e5) return cp.fHe * (1.0 - B / R + (1 + 5 * cp.fHe + 6 * cp.fHe * cp.fHe) / (R * R));
+  return -(R - A) / 2.0 + std::sqrt((R - A) * (R - A) / 4.0 + R * B) - A;
+}
+
+// _get_adaptive_sampling, thermodynamics_recfast.py:325-361
+std::vector<double> adaptive_sampling(double a0, double a1, int N) {
+  int n1 = std::max(8, (int)(N * 0.05));
+  int n2 = std::max(8, (int)(N * 0.10));
+  int n3 = (int)(N * 0.50);
+  int n4 = N - n1 - n2 - n3;
+  double ab1 = std::max(1.0 / (1.0 + 10000.0), a0), ab2 = 1.0 / (1.0 + 1400.0), ab3 = 1.0 / (1.0 + 600.0);
+  std::vector<double> a(N);
+  geomspace(a0, ab1, n1, false, a.data());
+  geomspace(ab1, ab2, n2, false, a.data() + n1);
+  geomspace(ab2, ab3, n3, false, a.data() + n1 + n2);
+  geomspace(ab3, a1, n4, true, a.data() + n1 + n2 + n3);
+  return a;
+}
+
+void setup_cosmo(const double *p, Cosmo &c) {
+  c.Omegam = p[P_OMEGAM]; c.Omegab = p[P_OMEGAB]; c.H0 = p[P_H0]; c.Tcmb = p[P_TCMB]; c.YHe = p[P_YHE];
+  c.Neff = p[P_NEFF]; c.Nmnu = p[P_NMNU]; c.mnu = p[P_MNU]; c.w0 = p[P_W0]; c.wa = p[P_WA]; c.cs2DE = p[P_CS2DE];
+  c.Omegak = p[P_OMEGAK]; c.A_s = p[P_AS]; c.n_s = p[P_NS]; c.k_p = p[P_KP];
+  c.use_tca = p[P_USE_TCA] != 0; c.tca_h = p[P_TCA_H]; c.tca_k = p[P_TCA_K];
+  c.use_rsa = p[P_USE_RSA] != 0; c.rsa_c = p[P_RSA_C]; c.rsa_k = p[P_RSA_K];
+  c.use_ufa = p[P_USE_UFA] != 0; c.ufa_k = p[P_UFA_K];
+  c.lambda_de = (c.w0 == -1.0 && c.wa == 0.0);
+}
+
+// evolve_background(thermo_module='RECFAST'): background.py:141-345 + recfast:363-492
+void build_background(Bg &bg, const double *p, int num_thermo) {
+  Cosmo &c = bg.c;
+  setup_cosmo(p, c);
+  bg.num_thermo = num_thermo;
+  c.amin = 1e-9; c.amax = 1.01;
+  const double c2ok = 1.62581581e4;
+  c.grhom = 3.33795017e-11 * c.H0 * c.H0;
+  c.grhog = 1.49594245e-13 * std::pow(c.Tcmb, 4);
+  c.grhor = 3.39739477e-14 * std::pow(c.Tcmb, 4);
+  c.adotrad = std::sqrt((c.grhog + c.grhor * (c.Neff + c.Nmnu)) / 3.0);
+  c.amnu = c.mnu * c2ok / c.Tcmb;
+  const int nnu = 512;
+  bg.nu_a.resize(nnu);
+  geomspace(c.amin * 0.9, c.amax * 1.1, nnu, true, bg.nu_a.data());
+  std::vector<double> loga(nnu), lr(nnu), lp(nnu), lpp(nnu);
+  double q8[8], w8[8];
+  nu_bins(8, q8, w8);
+  for (int i = 0; i < nnu; ++i) {
+    double r, pp, ppp;
+    nu_background(bg.nu_a[i], c.amnu, q8, w8, &r, &pp, &ppp);
+    loga[i] = std::log(bg.nu_a[i]);
+    lr[i] = std::log(r); lp[i] = std::log(pp); lpp[i] = std::log(ppp);
+  }
+  bg.logrhonu.build(loga.data(), lr.data(), nnu, true);
+  bg.logpnu.build(loga.data(), lp.data(), nnu, true);
+  bg.logppnu.build(loga.data(), lpp.data(), nnu, true);
+  double rhonu1 = std::exp(bg.logrhonu.eval(0.0));
+  c.Omegamnu = c.grhor * rhonu1 / c.grhom;
+  double Omegar = (c.Neff + c.Nmnu * rhonu1) * c.grhor / c.grhom;
+  double Omegag = c.grhog / c.grhom;
+  c.OmegaDE = 1.0 - c.Omegak - Omegar - Omegag - c.Omegam;
+  c.taumin = c.amin / c.adotrad;
+  c.taumax = c.taumin + romb([&](double a) { return dtauda(bg, a); }, c.amin, c.amax);
+  c.fHe = c.YHe / (rf::mHe_mH * (1.0 - c.YHe));
+
+  // compute_thermal_history (recfast :363-453)
+  const int N = num_thermo;
+  std::vector<double> agrid = adaptive_sampling(c.amin, c.amax, N + 1);
+  std::vector<double> Y(6 * N);
+  double Tcmb = c.Tcmb, Tcmb2 = Tcmb * Tcmb, Tcmb3 = Tcmb2 * Tcmb;
+  double HH = c.H0 / 100.0, HO = HH * rf::bigH, mu_H = 1.0 / (1.0 - c.YHe);
+  double Nnow = 3.0 * HO * HO * c.Omegab / (8.0 * rf::PI * rf::G * mu_H * rf::mH);
+  double fHe = c.fHe;
+  using rf::CR; using rf::CB1; using rf::CB1_He1;
+  bg.recfast_maxstep_hits = 0;
+  for (int i = 0; i < N; ++i) {
+    double astart = agrid[i], zstart = 1.0 / astart - 1.0, aend = agrid[i + 1], zend = 1.0 / aend - 1.0;
+    double dzda = -1.0 / (aend * aend);
+    double yprev[6];
+    if (i > 0) for (int j = 0; j < 6; ++j) yprev[j] = Y[j * N + i - 1];
+    else { yprev[0] = 1; yprev[1] = 1; yprev[2] = Tcmb * (1 + zstart); yprev[3] = 0; yprev[4] = 0; yprev[5] = -Tcmb * (1 + zstart); }
+    double out[6];
+    if (zend > 3500.0) {
+      out[0] = 1; out[1] = 1; out[2] = Tcmb * (1 + zend); out[3] = 0; out[4] = 0; out[5] = -Tcmb * (1 + zend);
+    } else if (i > 0 && yprev[1] > 0.99) {
+      double rhs = std::exp(1.5 * std::log(CR * Tcmb / (1 + zend)) - CB1_He1 / (Tcmb * (1 + zend))) / Nnow * 4.0;
+      double xHe0 = 0.5 * (std::sqrt((rhs - 1) * (rhs - 1) + 4 * (1 + fHe) * rhs) - (rhs - 1));
+      double zz = 1 + zend;
+      double q = -((CB1_He1 - CR * Tcmb2) / (Tcmb * zz));
+      double sq = std::sqrt(Nnow * Nnow + (16 * std::pow(-CB1_He1 + CR * Tcmb2, 3)) / (Tcmb3 * zz * zz * zz) +
+                            8 * (1 + 2 * fHe) * Nnow * std::pow(q, 1.5));
+      double dxHeIdz = (-3 * std::pow(-(CB1_He1 / Tcmb) + CR * Tcmb, 1.5) *
+                        (Nnow + 2 * fHe * Nnow + 4 * std::pow(q, 1.5) - sq)) /
+                       (Nnow * std::pow(zz, 2.5) * sq);
+      out[0] = 1.0; out[1] = (xHe0 - 1.0) / fHe; out[2] = Tcmb * zz; out[3] = 0; out[4] = dxHeIdz * dzda; out[5] = -Tcmb * zz;
+    } else {
+      double ys[3], ye[3];
+      ys[0] = yprev[0]; ys[1] = yprev[1]; ys[2] = yprev[2];
+      if (!solve_ionization(bg, astart, aend, ys, ye)) bg.recfast_maxstep_hits++;
+      double f[3];
+      ionization<double>(bg, aend, ye, f);
+      out[0] = ye[0]; out[1] = ye[1]; out[2] = ye[2]; out[3] = f[0]; out[4] = f[1]; out[5] = f[2];
+      if (i > 0 && yprev[0] > 0.99) {  // H Saha branch (f_case3)
+        double zz = 1 + zend;
+        double rhs = std::exp(1.5 * std::log(CR * Tcmb / zz) - CB1 / (Tcmb * zz)) / Nnow;
+        double xH0 = 0.5 * (std::sqrt(rhs * rhs + 4.0 * rhs) - rhs);
+        double q = -((CB1 - CR * Tcmb2) / (Tcmb * zz));
+        double sq = std::sqrt(q);
+        double dxHdz = ((3 * ((2 * std::pow(q, 1.5)) / zz +
+                              ((CB1 - CR * Tcmb2) * (2 * CB1 * CB1 - 4 * CB1 * CR * Tcmb2 +
+                                                     Tcmb2 * (2 * CR * CR * Tcmb2 + Nnow * zz * zz * sq))) /
+                                  (std::pow(Tcmb, 1.5) * std::pow(zz, 2.5) *
+                                   std::sqrt((-CB1 + CR * Tcmb2) *
+                                             (CB1 * CB1 - 2 * CB1 * CR * Tcmb2 + Tcmb2 * (CR * CR * Tcmb2 + Nnow * zz * zz * sq)))))) /
+                        (2. * Nnow));
+        out[0] = xH0; out[3] = dxHdz * dzda;
+      }
+    }
+    for (int j = 0; j < 6; ++j) Y[j * N + i] = out[j];
+  }
+  // evaluate_thermo (:456-492)
+  bg.aexp.assign(agrid.begin() + 1, agrid.end());
+  bg.xeHI.resize(N); bg.xeHeI.resize(N); bg.xeHeII.resize(N); bg.xe.resize(N); bg.mu.resize(N);
+  bg.Tm.resize(N); bg.cs2.resize(N); bg.xeprime_recf.resize(N); bg.tau.resize(N);
+  for (int i = 0; i < N; ++i) {
+    double a = bg.aexp[i];
+    bg.xeHI[i] = Y[0 * N + i];
+    bg.xeHeI[i] = Y[1 * N + i];
+    bg.xeHeII[i] = saha_HeII<double>(c, a);
+    bg.xe[i] = bg.xeHI[i] + fHe * bg.xeHeI[i] + bg.xeHeII[i];
+    bg.mu[i] = 1 / (1 + (1 / rf::mHe_mH - 1) * c.YHe + (1 - c.YHe) * bg.xe[i]);
+    bg.Tm[i] = Y[2 * N + i];
+    double hh = 1e-20 * a;
+    double dHeII = saha_HeII<cplx>(c, cplx(a, hh)).imag() / hh;
+    double daTmda = bg.Tm[i] + a * Y[5 * N + i];
+    bg.cs2[i] = rf::kB / rf::mH / (rf::c * rf::c) / bg.mu[i] * bg.Tm[i] * (4 - daTmda / bg.Tm[i]) / 3;
+    bg.xeprime_recf[i] = (Y[3 * N + i] + fHe * Y[4 * N + i] + dHeII) * (1.0 / dtauda(bg, a));
+  }
+  // tau = taumin + concatenate([0], cumsum(romb increments)) (recfast :488-490)
+  {
+    double cum = 0.0;
+    bg.tau[0] = c.taumin;
+    for (int i = 1; i < N; ++i) {
+      cum += romb([&](double a) { return dtauda(bg, a); }, bg.aexp[i - 1], bg.aexp[i]);
+      bg.tau[i] = c.taumin + cum;
+    }
+  }
+  std::vector<double> tmp(N), la(N);
+  for (int i = 0; i < N; ++i) la[i] = std::log(bg.aexp[i]);
+  bg.tau_of_a.build(bg.aexp.data(), bg.tau.data(), N, false);
+  bg.a_of_tau.build(bg.tau.data(), bg.aexp.data(), N, false);
+  bg.xe_of_tau.build(bg.tau.data(), bg.xe.data(), N, false);
+  for (int i = 0; i < N; ++i) tmp[i] = bg.aexp[i] * bg.cs2[i];
+  bg.cs2a_of_tau.build(bg.tau.data(), tmp.data(), N, false);
+  bg.cs2a_of_loga.build(la.data(), tmp.data(), N, false);
+  for (int i = 0; i < N; ++i) tmp[i] = bg.aexp[i] * bg.Tm[i];
+  bg.tempba_of_tau.build(bg.tau.data(), tmp.data(), N, false);
+  bg.xe_of_loga.build(la.data(), bg.xe.data(), N, false);
+
+  // optical depth and visibility, background.py:308-338
+  double akthom = 2.3038921003709498e-9 * (1.0 - c.YHe) * c.Omegab * c.H0 * c.H0;
+  double tau_pre = bg.tau_of_a.eval(1e-4);
+  double xe_full = 1 + c.YHe / (1 - c.YHe);
+  bg.xe_vis.resize(N); bg.xeprime.resize(N); bg.opac.resize(N);
+  for (int i = 0; i < N; ++i) {
+    bool pre = bg.tau[i] <= tau_pre;
+    bg.xe_vis[i] = pre ? xe_full : bg.xe_of_tau.eval(bg.tau[i]);
+    bg.xeprime[i] = pre ? 0.0 : bg.xe_of_tau.deriv(bg.tau[i]);
+    bg.opac[i] = bg.xe_vis[i] * akthom / (bg.aexp[i] * bg.aexp[i]);
+  }
+  bg.opacspline.build(bg.tau.data(), bg.opac.data(), N, false);
+  bg.tau0 = bg.tau_of_a.eval(1.0);
+  bg.optical_depth_today = bg.opacspline.integral(bg.tau0);
+  bg.optical_depth.resize(N); bg.gvis.resize(N); bg.gvisprime.resize(N); bg.gvispprime.resize(N);
+  int imax = 0;
+  for (int i = 0; i < N; ++i) {
+    double op = bg.opac[i];
+    double o1 = bg.opacspline.deriv(bg.tau[i]), o2 = bg.opacspline.deriv2(bg.tau[i]);
+    bg.optical_depth[i] = bg.optical_depth_today - bg.opacspline.integral(bg.tau[i]);
+    double em = std::exp(-bg.optical_depth[i]);
+    bg.gvis[i] = op * em;
+    bg.gvisprime[i] = (o1 + op * op) * em;
+    bg.gvispprime[i] = (o2 + 3 * op * o1 + op * op * op) * em;
+    if (bg.gvis[i] > bg.gvis[imax]) imax = i;  // jnp.argmax: first maximum
+  }
+  bg.tau_maxvis = bg.tau[imax];
+}
+
+// ----------------------------------------------------------------------------------------
+// Perturbations
+// ----------------------------------------------------------------------------------------
+struct Layout {
+  int lmaxg, lmaxgp, lmaxr, lmaxnu, nq, n;
+  int idxg, idxgp, idxr, iq0;
+  double q[32], w[32], dlfdlq[32];
+  void init(int lg, int lgp, int lr, int lnu, int nqq) {
+    lmaxg = lg; lmaxgp = lgp; lmaxr = lr; lmaxnu = lnu; nq = nqq;
+    n = 7 + (lg + 1) + (lgp + 1) + (lr + 1) + nqq * (lnu + 1) + 2;
+    idxg = 7; idxgp = 7 + (lg + 1); idxr = 9 + lg + lgp; iq0 = 10 + lg + lgp + lr;
+    nu_bins(nq, q, w);
+    for (int i = 0; i < nq; ++i) dlfdlq[i] = -q[i] / (1.0 + std::exp(-q[i]));
+  }
+};
+
+// model_synchronous, perturbations.py:88-525.  T is the scalar type of (tau, a); the other
+// state entries are real.  y[0] is not read (a is passed separately).
+template <class T>
+void rhs(const Bg &bg, const Layout &L, const T &tau, const T &a, const double *y, double k, T *f) {
+  const Cosmo &c = bg.c;
+  const int lmaxg = L.lmaxg, lmaxgp = L.lmaxgp, lmaxr = L.lmaxr, lmaxnu = L.lmaxnu, nq = L.nq, n = L.n;
+  const int iq0 = L.iq0, iq1 = iq0 + nq, iq2 = iq1 + nq, iq3 = iq2 + nq;
+  const double Omegac = c.Omegam - c.Omegab;
+  const double s2_squared = 1.0, s_l2 = 1.0, s_l3 = 1.0;
+  T loga = std::log(a);
+  double eta = y[2], deltac = y[3], thetac = y[4], deltab = y[5], thetab = y[6];
+  double deltag = y[7], thetag = y[8], shearg = y[9] / 2.0;
+  double deltar = y[9 + lmaxg + lmaxgp], thetar = y[10 + lmaxg + lmaxgp], shearr = y[11 + lmaxg + lmaxgp] / 2.0;
+  double deltaq = y[n - 2], thetaq = y[n - 1];
+  T cs2 = bg.cs2a_of_loga.eval(loga) / a;
+  T xe = bg.xe_of_loga.eval(loga);
+  T photbar = c.grhog / (c.grhom * c.Omegab * a);
+  T pb43 = 4.0 / 3.0 * photbar;
+  T pnu = std::exp(bg.logpnu.eval(loga));
+  double cs2_Q = c.cs2DE;
+  T w_Q = c.w0 + c.wa * (1.0 - a);
+  T rho_Q = powr(a, -3 * (1 + c.w0 + c.wa)) * std::exp(3.0 * (a - 1.0) * c.wa);
+  T rho_plus_p_theta_Q = (1.0 + w_Q) * rho_Q * c.grhom * c.OmegaDE * thetaq * a * a;
+  T aH = aprimeoa<T>(bg, a);
+  T xeprime = bg.xe_of_loga.deriv(loga) * aH;
+  T gpres = ((c.grhog + c.grhor * c.Neff) / 3.0 + c.grhor * c.Nmnu * pnu) / (a * a) + w_Q * c.grhom * c.OmegaDE * rho_Q * a * a;
+  T aHH = 0.5 * (aH * aH - gpres);
+  T w_Q_prime = -c.wa * aH * a;
+  T ca2_Q = w_Q - w_Q_prime / 3.0 / ((1.0 + w_Q) + 1e-6) / aH;
+  double akthom = 2.3038921003709498e-9 * (1.0 - c.YHe) * c.Omegab * c.H0 * c.H0;
+  T opac = xe * akthom / (a * a);
+  T tauc = 1.0 / maxr(opac, 1e-30);
+  T taucprime = tauc * (2.0 * aH - xeprime / maxr(xe, 1e-30));
+  T tauh = 1.0 / maxr(aH, 1e-30);
+  double tauk = 1.0 / std::max(k, 1e-30);
+  // approximation switches (approximations.py:50-75)
+  bool rsa = c.use_rsa && (re(tauc) / std::max(re(tau), 1e-30) > c.rsa_c) && (re(tau) * k > c.rsa_k);
+  bool ufa = !rsa && c.use_ufa && (re(tau) * k > c.ufa_k);
+  bool uncoupled = (re(tauc) / std::max(tauk, 1e-30) > c.tca_k) ||
+                   ((re(tauc) / std::max(re(tauh), 1e-30) > c.tca_h) && (re(tauc) / std::max(tauk, 1e-30) > 0.1 * c.tca_k));
+  bool tca = !rsa && c.use_tca && !uncoupled;
+
+  f[0] = aH * a;
+  // nu_perturb (perturbations.py:29-57)
+  T drhonu = 0.0, dpnu = 0.0, fnu = 0.0, shearnu = 0.0;
+  T v[32];
+  for (int i = 0; i < nq; ++i) {
+    T aq = a * c.amnu / L.q[i];
+    v[i] = 1.0 / std::sqrt(1.0 + aq * aq);
+    drhonu += L.w[i] * y[iq0 + i] / v[i];
+    dpnu += L.w[i] * y[iq0 + i] * v[i];
+    fnu += L.w[i] * y[iq1 + i];
+    shearnu += L.w[i] * y[iq2 + i] * v[i];
+  }
+  dpnu = dpnu / 3.0;
+  shearnu = shearnu * 2.0 / 3.0;
+  T dgrho = c.grhom * (Omegac * deltac + c.Omegab * deltab) / a +
+            (c.grhog * deltag + c.grhor * (c.Neff * deltar + c.Nmnu * drhonu)) / (a * a) +
+            c.grhom * c.OmegaDE * deltaq * rho_Q * a * a;
+  T dgrho_wo_rel = c.grhom * (Omegac * deltac + c.Omegab * deltab) / a + c.grhor * c.Nmnu * drhonu / (a * a) +
+                   c.grhom * c.OmegaDE * deltaq * rho_Q * a * a;
+  T hprime_rsa = (2.0 * k * k * eta + dgrho_wo_rel) / maxr(aH, 1e-30);
+  // compute_fields_rsa (approximations.py:28-47)
+  double sk2 = std::max(k * k, 1e-30);
+  T stc = maxr(tauc, 1e-30);
+  T dg_rsa = 4.0 / sk2 * (aH * hprime_rsa - sk2 * eta) + 4.0 / (sk2 * stc) * (thetab + 0.5 * hprime_rsa);
+  T tg_rsa = -0.5 * hprime_rsa + 3.0 / (sk2 * stc) * (-taucprime / stc * (thetab + 0.5 * hprime_rsa) +
+                                                        (-aH * thetab + cs2 * sk2 * deltab + sk2 * eta));
+  T dr_rsa = 4.0 / sk2 * (aH * hprime_rsa - sk2 * eta);
+  T tr_rsa = -0.5 * hprime_rsa;
+  T deltag_eff = rsa ? dg_rsa : T(deltag), thetag_eff = rsa ? tg_rsa : T(thetag), shearg_eff = rsa ? T(0.0) : T(shearg);
+  T deltar_eff = rsa ? dr_rsa : T(deltar), thetar_eff = rsa ? tr_rsa : T(thetar), shearr_eff = rsa ? T(0.0) : T(shearr);
+  T dgpres = (c.grhog * deltag_eff + c.grhor * c.Neff * deltar_eff) / (a * a) / 3.0 + c.grhor * c.Nmnu * dpnu / (a * a) +
+             (cs2_Q * c.grhom * c.OmegaDE * deltaq * rho_Q * a * a + (cs2_Q - ca2_Q) * (3.0 * aH * rho_plus_p_theta_Q / (k * k)));
+  T dgtheta = c.grhom * (Omegac * thetac + c.Omegab * thetab) / a +
+              4.0 / 3.0 * (c.grhog * thetag_eff + c.Neff * c.grhor * thetar_eff) / (a * a) +
+              c.Nmnu * c.grhor * k * fnu / (a * a) + rho_plus_p_theta_Q;
+  T dgshear = 4.0 / 3.0 * (c.grhog * shearg_eff + c.Neff * c.grhor * shearr_eff) / (a * a) + c.Nmnu * c.grhor * shearnu / (a * a);
+  (void)dgshear;
+  f[1] = -(dgrho + 3.0 * dgpres) * a;
+  T hprime_full = (2.0 * k * k * eta + dgrho) / aH;
+  T hprime = rsa ? hprime_rsa : hprime_full;
+  T etaprime = 0.5 * dgtheta / (k * k);
+  T alpha = (hprime + 6. * etaprime) / 2. / (k * k);
+  f[2] = etaprime;
+  f[3] = -thetac - 0.5 * hprime;
+  f[4] = -aH * thetac;
+  f[5] = -thetab - 0.5 * hprime;
+  f[6] = -aH * thetab + k * k * cs2 * deltab + pb43 * opac * (thetag_eff - thetab);
+  const int idxg = 7, idxgp = 7 + (lmaxg + 1);
+  if (rsa) {
+    for (int i = idxg; i <= idxg + lmaxg; ++i) f[i] = 0.0;
+    for (int i = idxgp; i <= idxgp + lmaxgp; ++i) f[i] = 0.0;
+  } else if (tca) {
+    T dbp = -thetab - 0.5 * hprime;
+    T tbp = (-aH * thetab + k * k * cs2 * deltab + k * k * pb43 * (0.25 * deltag - s2_squared * shearg)) / (1.0 + pb43);
+    T dgp = 4.0 / 3.0 * (-thetag - 0.5 * hprime);
+    T slip = 2.0 * pb43 / (1.0 + pb43) * aH * (thetab - thetag) +
+             tauc * (-aHH * thetab - 0.5 * aH * k * k * deltag + k * k * (cs2 * dbp - 0.25 * dgp)) / (1.0 + pb43);
+    tbp = tbp + pb43 / (1.0 + pb43) * slip;
+    T tgp = (-tbp - aH * thetab + k * k * cs2 * deltab) / maxr(pb43, 1e-30) + k * k * (0.25 * deltag - s2_squared * shearg);
+    f[5] = dbp; f[6] = tbp; f[idxg] = dgp; f[idxg + 1] = tgp;
+    for (int i = idxg + 2; i <= idxg + lmaxg; ++i) f[i] = 0.0;
+    for (int i = idxgp; i <= idxgp + lmaxgp; ++i) f[i] = 0.0;
+  } else {
+    double polter = y[idxg + 2] + y[idxgp + 0] + y[idxgp + 2];
+    f[idxg + 0] = 4.0 / 3.0 * (-thetag_eff - 0.5 * hprime);
+    f[idxg + 1] = k * k * (0.25 * deltag_eff - s2_squared * shearg_eff) - opac * (thetag_eff - thetab);
+    f[idxg + 2] = 8. / 15. * (thetag_eff + k * k * alpha) - 3. / 5. * k * s_l3 / s_l2 * y[idxg + 3] - opac * (y[idxg + 2] - 0.1 * s_l2 * polter);
+    for (int l = 3; l < lmaxg; ++l)
+      f[idxg + l] = k / (2 * l + 1) * (l * y[idxg + l - 1] - (l + 1) * y[idxg + l + 1]) - opac * y[idxg + l];
+    f[idxg + lmaxg] = k * y[idxg + lmaxg - 1] - (double)(lmaxg + 1) / tau * y[idxg + lmaxg] - opac * y[idxg + lmaxg];
+    for (int l = 0; l < lmaxgp; ++l) {
+      double ym1 = (l == 0) ? 0.0 : y[idxgp + l - 1];  // l*y[l-1] with l=0 vanishes
+      f[idxgp + l] = k / (2 * l + 1) * (l * ym1 - (l + 1) * y[idxgp + l + 1]) - opac * y[idxgp + l];
+    }
+    f[idxgp + 0] += opac * polter / 2.0;
+    f[idxgp + 2] += opac * polter / 10.0;
+    f[idxgp + lmaxgp] = k * y[idxgp + lmaxgp - 1] - (double)(lmaxgp + 1) / tau * y[idxgp + lmaxgp] - opac * y[idxgp + lmaxgp];
+  }
+  const int idxr = 9 + lmaxg + lmaxgp;
+  if (rsa) {
+    for (int i = idxr; i <= idxr + lmaxr; ++i) f[i] = 0.0;
+  } else {
+    f[idxr + 0] = 4.0 / 3.0 * (-thetar_eff - 0.5 * hprime);
+    f[idxr + 1] = k * k * (0.25 * deltar_eff - shearr_eff);
+    if (ufa) {
+      f[idxr + 2] = -3.0 / maxr(tau, 1e-30) * shearr_eff + 4.0 / 15.0 * (thetar_eff + 0.5 * hprime);
+      for (int i = idxr + 3; i <= idxr + lmaxr; ++i) f[i] = 0.0;
+    } else {
+      f[idxr + 2] = 8. / 15. * (thetar_eff + k * k * alpha) - 0.6 * k * y[idxr + 3];
+      for (int l = 3; l < lmaxr; ++l) f[idxr + l] = k / (2 * l + 1) * (l * y[idxr + l - 1] - (l + 1) * y[idxr + l + 1]);
+      f[idxr + lmaxr] = k * y[idxr + lmaxr - 1] - (double)(lmaxr + 1) / tau * y[idxr + lmaxr];
+    }
+  }
+  // massive neutrinos
+  for (int i = 0; i < nq; ++i) {
+    double dl = L.dlfdlq[i];
+    f[iq0 + i] = -k * v[i] * y[iq1 + i] + hprime * dl / 6.0;
+    f[iq1 + i] = k * v[i] * (y[iq0 + i] - 2.0 * y[iq2 + i]) / 3.0;
+    f[iq2 + i] = k * v[i] * (2 * y[iq1 + i] - 3 * y[iq3 + i]) / 5.0 - (hprime / 15.0 + 2.0 / 5.0 * etaprime) * dl;
+    for (int l = 3; l < lmaxnu; ++l)
+      f[iq0 + l * nq + i] = k * v[i] / (double)(2 * l + 1) * ((double)l * y[iq0 + (l - 1) * nq + i] - (double)(l + 1) * y[iq0 + (l + 1) * nq + i]);
+    f[iq0 + lmaxnu * nq + i] = k * v[i] * y[iq0 + (lmaxnu - 1) * nq + i] - (double)(lmaxnu + 1) / tau * y[iq0 + lmaxnu * nq + i];
+  }
+  // dark energy fluid, BL10 (3.5)-(3.6)
+  if (c.lambda_de) {
+    f[n - 2] = 0.0; f[n - 1] = 0.0;
+  } else {
+    f[n - 2] = -(1.0 + w_Q) * (thetaq + 0.5 * hprime) - 3.0 * (cs2_Q - w_Q) * aH * deltaq -
+               9.0 * (1.0 + w_Q) * (cs2_Q - ca2_Q) * aH * aH / (k * k) * thetaq;
+    f[n - 1] = -(1.0 - 3.0 * cs2_Q) * aH * thetaq + cs2_Q / (1.0 + w_Q) * k * k * deltaq;
+  }
+}
+
+// adiabatic_ics_one_mode, perturbations.py:680-781
+void initial_conditions(const Bg &bg, const Layout &L, double tau, double k, double *y) {
+  const Cosmo &c = bg.c;
+  const int n = L.n, nq = L.nq, iq0 = L.iq0, iq1 = iq0 + nq, iq2 = iq1 + nq;
+  for (int i = 0; i < n; ++i) y[i] = 0.0;
+  double a = bg.a_of_tau.eval(tau);
+  double rnu = std::exp(bg.logrhonu.eval(std::log(a)));
+  double rhom = c.grhom * c.Omegam / (a * a * a);
+  double rhor = (c.grhog + c.grhor * (c.Neff + c.Nmnu * rnu)) / (a * a * a * a);
+  double rhonu = c.grhor * (c.Neff + c.Nmnu * rnu) / (a * a * a * a);
+  double fracb = c.Omegab / c.Omegam, fracnu = rhonu / rhor;
+  double om = a * rhom / std::sqrt(rhor);
+  double ci = -1.0, s2 = 1.0;
+  double kt = k * tau;
+  double deltag = -kt * kt / 3 * (1 - om * tau / 5) * ci * s2;
+  double thetag = -kt * kt * kt / tau / 36 * (1 - 3 * (1 + 5 * fracb - fracnu) / 20 / (1 - fracnu) * om * tau) * ci * s2;
+  double deltab = 0.75 * deltag, thetab = thetag, deltac = 0.75 * deltag, thetac = 0.0;
+  double deltar = deltag;
+  double thetar = -kt * kt * kt * kt / tau / 36 / (4 * fracnu + 15) *
+                  (4 * fracnu + 11 + 12 - 3 * (8 * fracnu * fracnu + 50 * fracnu + 275) / 20 / (2 * fracnu + 15) * tau * om) * ci;
+  double shearr = kt * kt / (45 + 12 * fracnu) * (3 * s2 - 1) * (1 + (4 * fracnu - 5) / 4 / (2 * fracnu + 15) * tau * om) * ci;
+  double cs2_Q = c.cs2DE, w_Q = c.w0 + c.wa * (1.0 - a);
+  double deltaq = kt * kt / 4 * (1 + w_Q) * (4 - 3 * cs2_Q) / (4 - 6 * w_Q + 3 * cs2_Q) * ci * s2;
+  double thetaq = kt * kt * kt * kt / tau / 4 * cs2_Q / (4 - 6 * w_Q + 3 * cs2_Q) * ci * s2;
+  if (c.lambda_de) { deltaq = 0.0; thetaq = 0.0; }
+  double eta = ci * (1 - kt * kt / 12 / (15 + 4 * fracnu) *
+                             (5 + 4 * s2 * fracnu - (16 * fracnu * fracnu + 280 * fracnu + 325) / 10 / (2 * fracnu + 15) * tau * om));
+  y[0] = a; y[1] = 0.0; y[2] = eta; y[3] = deltac; y[4] = thetac; y[5] = deltab; y[6] = thetab; y[7] = deltag; y[8] = thetag;
+  y[9 + L.lmaxg + L.lmaxgp] = deltar;
+  y[10 + L.lmaxg + L.lmaxgp] = thetar;
+  y[11 + L.lmaxg + L.lmaxgp] = shearr * 2.0;
+  for (int i = 0; i < nq; ++i) {
+    double aq = a * c.amnu / L.q[i];
+    double v = 1 / std::sqrt(1 + aq * aq);
+    double dl = L.dlfdlq[i];
+    y[iq0 + i] = -0.25 * dl * deltar;
+    y[iq1 + i] = -dl * thetar / v / k / 3.0;
+    y[iq2 + i] = -0.5 * dl * shearr;
+  }
+  y[n - 2] = deltaq;
+  y[n - 1] = thetaq;
+}
+
+// determine_starting_time, perturbations.py:784-835 (+ util.root_find_bisect :249-280)
+double starting_time(const Bg &bg, double k) {
+  const Cosmo &c = bg.c;
+  const double small_k = 0.0004, large_k = 0.07;
+  double tau0 = c.taumin, tau1 = bg.tau_of_a.eval(0.1);
+  double tau_k = 1.0 / k;
+  double akthom = 2.3048e-9 * (1.0 - c.YHe) * c.Omegab * c.H0 * c.H0;
+  auto cond_small = [&](double lt) {
+    double t = std::exp(lt);
+    double xe = bg.xe_of_tau.eval(t), a = bg.a_of_tau.eval(t);
+    double opac = xe * akthom / (a * a);
+    double aH = aprimeoa<double>(bg, a);
+    return (1.0 / opac) / (1.0 / aH) / small_k - 1.0;
+  };
+  auto cond_large = [&](double lt) {
+    double t = std::exp(lt);
+    double a = bg.a_of_tau.eval(t);
+    double aH = aprimeoa<double>(bg, a);
+    return (1.0 / aH) / tau_k / large_k - 1.0;
+  };
+  auto bisect = [&](auto fn) {
+    double xl = std::log(tau0), xr = std::log(tau1);
+    for (int it = 0; it < 7; ++it) {
+      double xm = 0.5 * (xl + xr);
+      if (fn(xm) * fn(xl) > 0) xl = xm; else xr = xm;
+    }
+    return 0.5 * (xl + xr);
+  };
+  double lt_large = bisect(cond_large), lt_small = bisect(cond_small);
+  return std::exp(std::min(lt_small, lt_large));
+}
+
+// RSA output projection (approximations.py:84-176)
+void rsa_projection(const Bg &bg, const Layout &L, double tau, double k, double *y) {
+  const Cosmo &c = bg.c;
+  if (!c.use_rsa) return;
+  double a = y[0], loga = std::log(a);
+  double eta = y[2], deltab = y[5], thetab = y[6];
+  double aH = aprimeoa<double>(bg, a);
+  double cs2 = bg.cs2a_of_loga.eval(loga) / a;
+  double xe = bg.xe_of_loga.eval(loga);
+  double xeprime = bg.xe_of_loga.deriv(loga) * aH;
+  double akthom = 2.3038921003709498e-9 * (1.0 - c.YHe) * c.Omegab * c.H0 * c.H0;
+  double opac = xe * akthom / (a * a);
+  double tau_c = 1.0 / std::max(opac, 1e-30);
+  double tau_c_prime = tau_c * (2.0 * aH - xeprime / std::max(xe, 1e-30));
+  bool do_rsa = (tau_c / std::max(tau, 1e-30) > c.rsa_c) && (tau * k > c.rsa_k);
+  if (!do_rsa) return;
+  double Omegac = c.Omegam - c.Omegab;
+  const int nq = L.nq, iq0 = L.iq0;
+  double drhonu = 0;
+  for (int i = 0; i < nq; ++i) {
+    double aq = a * c.amnu / L.q[i];
+    double v = 1 / std::sqrt(1 + aq * aq);
+    drhonu += L.w[i] * y[iq0 + i] / v;
+  }
+  double rho_Q = std::pow(a, -3 * (1 + c.w0 + c.wa)) * std::exp(3 * (a - 1) * c.wa);
+  double deltaq = y[L.n - 2];
+  double dgrho_wo_rel = c.grhom * (Omegac * y[3] + c.Omegab * deltab) / a + c.grhor * c.Nmnu * drhonu / (a * a) +
+                        c.grhom * c.OmegaDE * deltaq * rho_Q * a * a;
+  double hprime = (2.0 * k * k * eta + dgrho_wo_rel) / std::max(aH, 1e-30);
+  double sk2 = std::max(k * k, 1e-30), stc = std::max(tau_c, 1e-30);
+  double dg = 4.0 / sk2 * (aH * hprime - sk2 * eta) + 4.0 / (sk2 * stc) * (thetab + 0.5 * hprime);
+  double tg = -0.5 * hprime + 3.0 / (sk2 * stc) * (-tau_c_prime / stc * (thetab + 0.5 * hprime) + (-aH * thetab + cs2 * sk2 * deltab + sk2 * eta));
+  double dr = 4.0 / sk2 * (aH * hprime - sk2 * eta), tr = -0.5 * hprime;
+  const int idxg = 7, idxgp = 7 + (L.lmaxg + 1), idxr = 9 + L.lmaxg + L.lmaxgp;
+  y[idxg] = dg; y[idxg + 1] = tg; y[idxg + 2] = 0.0;
+  for (int i = idxg + 3; i <= idxg + L.lmaxg; ++i) y[i] = 0.0;
+  for (int i = idxgp; i <= idxgp + L.lmaxgp; ++i) y[i] = 0.0;
+  y[idxr] = dr; y[idxr + 1] = tr; y[idxr + 2] = 0.0;
+  for (int i = idxr + 3; i <= idxr + L.lmaxr; ++i) y[i] = 0.0;
+}
+
+// convert_to_output_variables, perturbations.py:528-677
+void convert_output(const Bg &bg, const Layout &L, double k, const double *y, double *out) {
+  const Cosmo &c = bg.c;
+  const int nq = L.nq, iq0 = L.iq0, iq1 = iq0 + nq;
+  double Omegac = c.Omegam - c.Omegab;
+  double a = y[0], eta = y[2], deltac = y[3], thetac = y[4], deltab = y[5], thetab = y[6], deltag = y[7], thetag = y[8];
+  double deltar = y[9 + L.lmaxg + L.lmaxgp], thetar = y[10 + L.lmaxg + L.lmaxgp];
+  double rhonu = std::exp(bg.logrhonu.eval(std::log(a)));
+  double pnu = std::exp(bg.logpnu.eval(std::log(a)));
+  double rho_plus_p = rhonu + pnu;
+  double drhonu = 0, fnu = 0;
+  for (int i = 0; i < nq; ++i) {
+    double aq = a * c.amnu / L.q[i];
+    double v = 1 / std::sqrt(1 + aq * aq);
+    drhonu += L.w[i] * y[iq0 + i] / v;
+    fnu += L.w[i] * y[iq1 + i];
+  }
+  double deltanu = drhonu / rhonu, thetanu = k * fnu / rho_plus_p;
+  double deltaq = y[L.n - 2], thetaq = y[L.n - 1];
+  double w_Q = c.w0 + c.wa * (1.0 - a);
+  double rho_Q = std::pow(a, -3 * (1 + c.w0 + c.wa)) * std::exp(3 * (a - 1) * c.wa);
+  double rpptq = (1 + w_Q) * rho_Q * c.grhom * c.OmegaDE * thetaq * a * a;
+  double grho = c.grhom * c.Omegam / a + (c.grhog + c.grhor * (c.Neff + c.Nmnu * rhonu)) / (a * a) + c.grhom * c.OmegaDE * rho_Q * a * a +
+                c.grhom * c.Omegak;
+  double aH = std::sqrt(grho / 3.0);
+  double dgrho = c.grhom * (Omegac * deltac + c.Omegab * deltab) / a +
+                 (c.grhog * deltag + c.grhor * (c.Neff * deltar + c.Nmnu * drhonu)) / (a * a) + c.grhom * c.OmegaDE * deltaq * rho_Q * a * a;
+  double dgtheta = c.grhom * (Omegac * thetac + c.Omegab * thetab) / a + 4.0 / 3.0 * (c.grhog * thetag + c.Neff * c.grhor * thetar) / (a * a) +
+                   c.Nmnu * c.grhor * k * fnu / (a * a) + rpptq;
+  double hprime = (2.0 * k * k * eta + dgrho) / aH;
+  double etaprime = 0.5 * dgtheta / (k * k);
+  double alpha = (hprime + 6. * etaprime) / 2. / (k * k);
+  double deltam = (c.grhom * (Omegac * deltac + c.Omegab * deltab) / a + (c.grhor * c.Nmnu * drhonu) / (a * a)) /
+                  (c.grhom * c.Omegam / a + (c.grhor * c.Nmnu * rhonu) / (a * a));
+  double thetam = (c.grhom * (Omegac * thetac + c.Omegab * thetab) / a + c.Nmnu * c.grhor * k * fnu / (a * a)) /
+                  (3.0 * (c.grhom * c.Omegam / a + c.grhor * c.Nmnu * rhonu / (a * a)));
+  double deltabc = (c.grhom * (Omegac * deltac + c.Omegab * deltab) / a) / (c.grhom * c.Omegam / a);
+  double thetabc = (c.grhom * (Omegac * thetac + c.Omegab * thetab) / a) / (3.0 * (c.grhom * c.Omegam / a) / (a * a));
+  thetam += alpha * k * k;
+  thetabc += alpha * k * k;
+  double o[20] = {eta, etaprime, hprime, alpha, deltam, thetam / aH, deltabc, thetabc / aH, deltac, thetac / aH,
+                  deltab, thetab / aH, deltag, thetag / aH, deltar, thetar / aH, deltanu, thetanu / aH, deltaq, thetaq / aH};
+  std::memcpy(out, o, sizeof(o));
+}
+
+// ----------------------------------------------------------------------------------------
+// Rodas5 (transformed), ode_integrators_stiff.py:678-892
+// ----------------------------------------------------------------------------------------
+namespace r5 {
+const double gamma = 0.19;
+const double a21 = 2.0, a31 = 3.040894194418781, a32 = 1.041747909077569, a41 = 2.576417536461461, a42 = 1.622083060776640,
+             a43 = -0.9089668560264532, a51 = 2.760842080225597, a52 = 1.446624659844071, a53 = -0.3036980084553738,
+             a54 = 0.2877498600325443, a61 = -14.09640773051259, a62 = 6.925207756232704, a63 = -41.47510893210728,
+             a64 = 2.343771018586405, a65 = 24.13215229196062;
+const double C21 = -10.31323885133993, C31 = -21.04823117650003, C32 = -7.234992135176716, C41 = 32.22751541853323,
+             C42 = -4.943732386540191, C43 = 19.44922031041879, C51 = -20.69865579590063, C52 = -8.816374604402768,
+             C53 = 1.260436877740897, C54 = -0.7495647613787146, C61 = -46.22004352711257, C62 = -17.49534862857472,
+             C63 = -289.6389582892057, C64 = 93.60855400400906, C65 = 318.3822534212147, C71 = 34.20013733472935,
+             C72 = -14.15535402717690, C73 = 57.82335640988400, C74 = 25.83362985412365, C75 = 1.408950972071624,
+             C76 = -6.551835421242162, C81 = 42.57076742291101, C82 = -13.80770672017997, C83 = 93.98938432427124,
+             C84 = 18.77919633714503, C85 = -31.58359187223370, C86 = -6.685968952921985, C87 = -5.810979938412932;
+const double c2 = 0.38, c3 = 0.3878509998321533, c4 = 0.4839718937873840, c5 = 0.4570477008819580;
+const double d1 = gamma, d2 = -0.1823079225333714636, d3 = -0.319231832186874912, d4 = 0.3449828624725343, d5 = -0.377417564392089818;
+}  // namespace r5
+
+struct Workspace {
+  std::vector<double> J, LU, dT, k[8], u, f, rhs, ytmp;
+  std::vector<int> piv;
+  void init(int n) {
+    J.assign(n * n, 0); LU.assign(n * n, 0); dT.assign(n, 0);
+    for (int i = 0; i < 8; ++i) k[i].assign(n, 0);
+    u.assign(n, 0); f.assign(n, 0); rhs.assign(n, 0); ytmp.assign(n, 0); piv.assign(n, 0);
+  }
+};
+
+void rhs_real(const Bg &bg, const Layout &L, double tau, const double *y, double k, double *f) {
+  rhs<double>(bg, L, tau, y[0], y, k, f);
+}
+
+// exact Jacobian J = df/dy and dT = df/dtau at (t0, y0)
+void jacobian(const Bg &bg, const Layout &L, double t0, const double *y0, double k, Workspace &ws) {
+  const int n = L.n;
+  std::vector<cplx> fc(n);
+  double h = 1e-20 * std::fabs(y0[0]);
+  rhs<cplx>(bg, L, cplx(t0), cplx(y0[0], h), y0, k, fc.data());
+  for (int i = 0; i < n; ++i) ws.J[i * n + 0] = fc[i].imag() / h;
+  double ht = 1e-20 * std::fabs(t0);
+  rhs<cplx>(bg, L, cplx(t0, ht), cplx(y0[0]), y0, k, fc.data());
+  for (int i = 0; i < n; ++i) ws.dT[i] = fc[i].imag() / ht;
+  // linear in y[1:]: columns from unit vectors
+  std::vector<double> e(n, 0.0), f(n);
+  for (int j = 1; j < n; ++j) {
+    e[j] = 1.0;
+    rhs<double>(bg, L, t0, y0[0], e.data(), k, f.data());
+    e[j] = 0.0;
+    ws.J[0 * n + j] = 0.0;
+    for (int i = 1; i < n; ++i) ws.J[i * n + j] = f[i];
+  }
+}
+
+void lu_factor(int n, double *A, int *piv) {  // getrf: partial pivoting, first maximum
+  for (int kcol = 0; kcol < n; ++kcol) {
+    int p = kcol;
+    double best = std::fabs(A[kcol * n + kcol]);
+    for (int i = kcol + 1; i < n; ++i) {
+      double v = std::fabs(A[i * n + kcol]);
+      if (v > best) { best = v; p = i; }
+    }
+    piv[kcol] = p;
+    if (p != kcol)
+      for (int j = 0; j < n; ++j) std::swap(A[kcol * n + j], A[p * n + j]);
+    double d = A[kcol * n + kcol];
+    if (d != 0.0) {
+      for (int i = kcol + 1; i < n; ++i) A[i * n + kcol] /= d;
+    }
+    for (int i = kcol + 1; i < n; ++i) {
+      double l = A[i * n + kcol];
+      if (l == 0.0) continue;
+      for (int j = kcol + 1; j < n; ++j) A[i * n + j] -= l * A[kcol * n + j];
+    }
+  }
+}
+void lu_solve(int n, const double *A, const int *piv, double *b) {
+  for (int kk = 0; kk < n; ++kk)
+    if (piv[kk] != kk) std::swap(b[kk], b[piv[kk]]);
+  for (int i = 1; i < n; ++i) {
+    double s = b[i];
+    for (int j = 0; j < i; ++j) s -= A[i * n + j] * b[j];
+    b[i] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= A[i * n + j] * b[j];
+    b[i] = s / A[i * n + i];
+  }
+}
+
+// One Rodas5 step from (t0, y0) with step dt; returns y1 and the error estimate k8.
+void rodas5_step(const Bg &bg, const Layout &L, double kmode, double t0, double dt, const double *y0, double *y1, double *err,
+                 Workspace &ws) {
+  using namespace r5;
+  const int n = L.n;
+  jacobian(bg, L, t0, y0, kmode, ws);
+  const double dtgamma = dt * gamma;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) ws.LU[i * n + j] = (i == j ? 1.0 / dtgamma : 0.0) - ws.J[i * n + j];
+  lu_factor(n, ws.LU.data(), ws.piv.data());
+  double *k1 = ws.k[0].data(), *k2 = ws.k[1].data(), *k3 = ws.k[2].data(), *k4 = ws.k[3].data();
+  double *k5 = ws.k[4].data(), *k6 = ws.k[5].data(), *k7 = ws.k[6].data(), *k8 = ws.k[7].data();
+  double *u = ws.u.data(), *du = ws.f.data(), *dT = ws.dT.data();
+  const double dtd1 = dt * d1, dtd2 = dt * d2, dtd3 = dt * d3, dtd4 = dt * d4, dtd5 = dt * d5;
+  auto solve_into = [&](double *kout) { lu_solve(n, ws.LU.data(), ws.piv.data(), kout); };
+
+  rhs_real(bg, L, t0, y0, kmode, du);
+  for (int i = 0; i < n; ++i) k1[i] = du[i] + dtd1 * dT[i];
+  solve_into(k1);
+
+  for (int i = 0; i < n; ++i) u[i] = y0[i] + a21 * k1[i];
+  rhs_real(bg, L, t0 + c2 * dt, u, kmode, du);
+  for (int i = 0; i < n; ++i) k2[i] = du[i] + dtd2 * dT[i] + (C21 / dt) * k1[i];
+  solve_into(k2);
+
+  for (int i = 0; i < n; ++i) u[i] = y0[i] + a31 * k1[i] + a32 * k2[i];
+  rhs_real(bg, L, t0 + c3 * dt, u, kmode, du);
+  for (int i = 0; i < n; ++i) k3[i] = du[i] + dtd3 * dT[i] + ((C31 / dt) * k1[i] + (C32 / dt) * k2[i]);
+  solve_into(k3);
+
+  for (int i = 0; i < n; ++i) u[i] = y0[i] + a41 * k1[i] + a42 * k2[i] + a43 * k3[i];
+  rhs_real(bg, L, t0 + c4 * dt, u, kmode, du);
+  for (int i = 0; i < n; ++i) k4[i] = du[i] + dtd4 * dT[i] + ((C41 / dt) * k1[i] + (C42 / dt) * k2[i] + (C43 / dt) * k3[i]);
+  solve_into(k4);
+
+  for (int i = 0; i < n; ++i) u[i] = y0[i] + a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i];
+  rhs_real(bg, L, t0 + c5 * dt, u, kmode, du);
+  for (int i = 0; i < n; ++i)
+    k5[i] = du[i] + dtd5 * dT[i] + ((C51 / dt) * k1[i] + (C52 / dt) * k2[i] + (C53 / dt) * k3[i] + (C54 / dt) * k4[i]);
+  solve_into(k5);
+
+  for (int i = 0; i < n; ++i) u[i] = y0[i] + a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i];
+  rhs_real(bg, L, t0 + dt, u, kmode, du);
+  for (int i = 0; i < n; ++i)
+    k6[i] = du[i] + ((C61 / dt) * k1[i] + (C62 / dt) * k2[i] + (C63 / dt) * k3[i] + (C64 / dt) * k4[i] + (C65 / dt) * k5[i]);
+  solve_into(k6);
+
+  for (int i = 0; i < n; ++i) u[i] = u[i] + k6[i];
+  rhs_real(bg, L, t0 + dt, u, kmode, du);
+  for (int i = 0; i < n; ++i)
+    k7[i] = du[i] + ((C71 / dt) * k1[i] + (C72 / dt) * k2[i] + (C73 / dt) * k3[i] + (C74 / dt) * k4[i] + (C75 / dt) * k5[i] +
+                     (C76 / dt) * k6[i]);
+  solve_into(k7);
+
+  for (int i = 0; i < n; ++i) u[i] = u[i] + k7[i];
+  rhs_real(bg, L, t0 + dt, u, kmode, du);
+  for (int i = 0; i < n; ++i)
+    k8[i] = du[i] + ((C81 / dt) * k1[i] + (C82 / dt) * k2[i] + (C83 / dt) * k3[i] + (C84 / dt) * k4[i] + (C85 / dt) * k5[i] +
+                     (C86 / dt) * k6[i] + (C87 / dt) * k7[i]);
+  solve_into(k8);
+  for (int i = 0; i < n; ++i) {
+    y1[i] = u[i] + k8[i];
+    err[i] = k8[i];
+  }
+}
+
+struct SolverCfg {
+  double rtol, atol, pcoeff, icoeff, dcoeff, factormax, factormin;
+  int max_steps;
+};
+
+// squared contributions of the filtered, weighted error (perturbations.py:855-859)
+bool any_nan(const double *y, int n) {
+  for (int i = 0; i < n; ++i)
+    if (std::isnan(y[i])) return true;
+  return false;
+}
+double filtered_sq(const double *y0, const double *y1, const double *err, double k, const SolverCfg &s, bool nan) {
+  static const int idx[6] = {0, 2, 3, 5, 6, 7};
+  double wts[6] = {1.0, k * k, 1.0, 1.0, 1.0 / (k * k), 1.0};
+  double acc = 0;
+  for (int j = 0; j < 6; ++j) {
+    int i = idx[j];
+    double yy = std::max(std::fabs(y0[i]), std::fabs(nan ? y0[i] : y1[i]));
+    double e = err[i] / (s.atol + yy * s.rtol) * wts[j];
+    acc += e * e;
+  }
+  return acc;
+}
+
+// diffrax PIDController.adapt_step_size (error order 5 for Rodas5, safety 0.9)
+struct Pid {
+  double prev_inv = 1.0, prev_prev_inv = 1.0;
+  // returns the factor; updates the history according to keep
+  double adapt(double en, bool keep, const SolverCfg &s) {
+    const double order = 5.0;
+    double coeff1 = (s.icoeff + s.pcoeff + s.dcoeff) / order;
+    double coeff2 = -(s.pcoeff + 2 * s.dcoeff) / order;
+    double coeff3 = s.dcoeff / order;
+    double inv = 1.0 / en;
+    double f1 = coeff1 == 0 ? 1.0 : std::pow(inv, coeff1);
+    double f2 = coeff2 == 0 ? 1.0 : std::pow(prev_inv, coeff2);
+    double f3 = coeff3 == 0 ? 1.0 : std::pow(prev_prev_inv, coeff3);
+    double fmin = keep ? 1.0 : s.factormin;
+    double factor = 0.9 * f1 * f2 * f3;
+    factor = std::min(std::max(factor, fmin), s.factormax);
+    if (inv == 0.0 || std::isinf(inv)) inv = 1.0;
+    if (keep) { prev_prev_inv = prev_inv; prev_inv = inv; }
+    return factor;
+  }
+};
+
+// evolve_one_mode (perturbations.py:882-949): per-k adaptive Rodas5 with SaveAt(ts) linear
+// interpolation.  yout receives nout x nvar projected states (or nout x 20 converted fields).
+int evolve_one(const Bg &bg, const Layout &L, double kmode, const double *tau_out, int nout, const SolverCfg &s, bool full,
+               double *yout, int *nacc, int *nrej) {
+  const int n = L.n;
+  double tmin = tau_out[0];
+  for (int i = 1; i < nout; ++i) tmin = std::min(tmin, tau_out[i]);
+  double t1 = tau_out[0];
+  for (int i = 1; i < nout; ++i) t1 = std::max(t1, tau_out[i]);
+  double ts = starting_time(bg, kmode);
+  double t0 = 0.99 * std::min(tmin, ts);
+  Workspace ws;
+  ws.init(n);
+  std::vector<double> y(n), y1(n), err(n), tmp(n);
+  initial_conditions(bg, L, t0, kmode, y.data());
+  double tprev = t0;
+  double dt0 = std::min(t0 / 4, 0.5 * (t1 - t0));
+  double tnext = std::min(t0 + dt0, t1);
+  Pid pid;
+  int isave = 0, steps = 0, na = 0, nr = 0, status = 0;
+  while (tprev < t1) {
+    if (steps >= s.max_steps) { status = 1; break; }
+    double dt = tnext - tprev;
+    rodas5_step(bg, L, kmode, tprev, dt, y.data(), y1.data(), err.data(), ws);
+    double en = std::sqrt(filtered_sq(y.data(), y1.data(), err.data(), kmode, s, any_nan(y1.data(), n)) / 6.0);
+    if (!std::isfinite(en)) { status = 2; break; }
+    bool keep = en < 1.0;
+    double factor = pid.adapt(en, keep, s);
+    double newdt = dt * factor;
+    if (keep) {
+      while (isave < nout && tau_out[isave] <= tnext) {
+        double th = (tau_out[isave] - tprev) / (tnext - tprev);
+        for (int i = 0; i < n; ++i) tmp[i] = y[i] + th * (y1[i] - y[i]);
+        rsa_projection(bg, L, tau_out[isave], kmode, tmp.data());
+        if (full) std::memcpy(yout + (size_t)isave * n, tmp.data(), sizeof(double) * n);
+        else convert_output(bg, L, kmode, tmp.data(), yout + (size_t)isave * 20);
+        ++isave;
+      }
+      y.swap(y1);
+      ++na;
+    } else {
+      ++nr;
+    }
+    double next_t0 = keep ? tnext : tprev;
+    tprev = std::min(next_t0, t1);
+    double tn = next_t0 + newdt;
+    tnext = (tn > t1 - 1e-10) ? t1 : tn;
+    ++steps;
+  }
+  if (nacc) *nacc = na;
+  if (nrej) *nrej = nr;
+  if (status != 0) {
+    int w = full ? n : 20;
+    for (int i = isave; i < nout; ++i)
+      for (int j = 0; j < w; ++j) yout[(size_t)i * w + j] = NAN;
+  }
+  return status;
+}
+
+// evolve_modes_batched (perturbations.py:1002-1120) with Rodas5Batched (ode_integrators_stiff.py:928-1059):
+// B modes share one step size, one start time (the batch minimum) and one RMS norm over B x 6 entries.
+int evolve_batch(const Bg &bg, const Layout &L, const double *kmodes, int B, const double *tau_out, int nout, const SolverCfg &s,
+                 bool full, double *yout, int *nacc, int *nrej) {
+  const int n = L.n;
+  double tmin = tau_out[0], t1 = tau_out[0];
+  for (int i = 1; i < nout; ++i) { tmin = std::min(tmin, tau_out[i]); t1 = std::max(t1, tau_out[i]); }
+  double t0 = 1e300;
+  for (int b = 0; b < B; ++b) t0 = std::min(t0, 0.99 * std::min(tmin, starting_time(bg, kmodes[b])));
+  std::vector<Workspace> ws(B);
+  std::vector<std::vector<double>> y(B, std::vector<double>(n)), y1(B, std::vector<double>(n)), err(B, std::vector<double>(n));
+  std::vector<double> tmp(n);
+  for (int b = 0; b < B; ++b) { ws[b].init(n); initial_conditions(bg, L, t0, kmodes[b], y[b].data()); }
+  double tprev = t0;
+  double dt0 = std::min(t0 / 4, 0.5 * (t1 - t0));
+  double tnext = std::min(t0 + dt0, t1);
+  Pid pid;
+  int isave = 0, steps = 0, na = 0, nr = 0, status = 0;
+  int w = full ? n : 20;
+  while (tprev < t1) {
+    if (steps >= s.max_steps) { status = 1; break; }
+    double dt = tnext - tprev;
+    double acc = 0;
+    bool nan = false;
+    for (int b = 0; b < B; ++b) {
+      rodas5_step(bg, L, kmodes[b], tprev, dt, y[b].data(), y1[b].data(), err[b].data(), ws[b]);
+      nan = nan || any_nan(y1[b].data(), n);
+    }
+    for (int b = 0; b < B; ++b) acc += filtered_sq(y[b].data(), y1[b].data(), err[b].data(), kmodes[b], s, nan);
+    double en = std::sqrt(acc / (6.0 * B));
+    if (!std::isfinite(en)) { status = 2; break; }
+    bool keep = en < 1.0;
+    double factor = pid.adapt(en, keep, s);
+    double newdt = dt * factor;
+    if (keep) {
+      while (isave < nout && tau_out[isave] <= tnext) {
+        double th = (tau_out[isave] - tprev) / (tnext - tprev);
+        for (int b = 0; b < B; ++b) {
+          for (int i = 0; i < n; ++i) tmp[i] = y[b][i] + th * (y1[b][i] - y[b][i]);
+          rsa_projection(bg, L, tau_out[isave], kmodes[b], tmp.data());
+          double *dst = yout + ((size_t)b * nout + isave) * w;
+          if (full) std::memcpy(dst, tmp.data(), sizeof(double) * n);
+          else convert_output(bg, L, kmodes[b], tmp.data(), dst);
+        }
+        ++isave;
+      }
+      for (int b = 0; b < B; ++b) y[b].swap(y1[b]);
+      ++na;
+    } else {
+      ++nr;
+    }
+    double next_t0 = keep ? tnext : tprev;
+    tprev = std::min(next_t0, t1);
+    double tn = next_t0 + newdt;
+    tnext = (tn > t1 - 1e-10) ? t1 : tn;
+    ++steps;
+  }
+  for (int b = 0; b < B; ++b) { nacc[b] = na; nrej[b] = nr; }
+  if (status != 0)
+    for (int b = 0; b < B; ++b)
+      for (int i = isave; i < nout; ++i)
+        for (int j = 0; j < w; ++j) yout[((size_t)b * nout + i) * w + j] = NAN;
+  return status;
+}
+
+const Spline *find_spline(const Bg &bg, const std::string &name) {
+  if (name == "tau_of_a") return &bg.tau_of_a;
+  if (name == "a_of_tau") return &bg.a_of_tau;
+  if (name == "xe_of_tau") return &bg.xe_of_tau;
+  if (name == "cs2a_of_tau") return &bg.cs2a_of_tau;
+  if (name == "tempba_of_tau") return &bg.tempba_of_tau;
+  if (name == "xe_of_loga") return &bg.xe_of_loga;
+  if (name == "cs2a_of_loga") return &bg.cs2a_of_loga;
+  if (name == "logrhonu_of_loga") return &bg.logrhonu;
+  if (name == "logpnu_of_loga") return &bg.logpnu;
+  if (name == "logppseudonu_of_loga") return &bg.logppnu;
+  if (name == "opac") return &bg.opacspline;
+  return nullptr;
+}
+
+}  // namespace
+
+// ========================================================================================
+// C ABI for oracle.py (test-side ctypes)
+// ========================================================================================
+extern "C" {
+
+int orc_nparam() { return NPARAM; }
+
+void *orc_bg_new(const double *params, int num_thermo) {
+  Bg *bg = new Bg();
+  build_background(*bg, params, num_thermo);
+  return bg;
+}
+void orc_bg_free(void *h) { delete (Bg *)h; }
+
+double orc_bg_scalar(void *h, const char *name) {
+  const Bg &bg = *(Bg *)h;
+  const Cosmo &c = bg.c;
+  std::string s(name);
+  if (s == "grhom") return c.grhom;
+  if (s == "grhog") return c.grhog;
+  if (s == "grhor") return c.grhor;
+  if (s == "adotrad") return c.adotrad;
+  if (s == "amnu") return c.amnu;
+  if (s == "Omegamnu") return c.Omegamnu;
+  if (s == "OmegaDE") return c.OmegaDE;
+  if (s == "taumin") return c.taumin;
+  if (s == "taumax") return c.taumax;
+  if (s == "fHe") return c.fHe;
+  if (s == "tau0") return bg.tau0;
+  if (s == "tau_maxvis") return bg.tau_maxvis;
+  if (s == "optical_depth_today") return bg.optical_depth_today;
+  if (s == "recfast_maxstep_hits") return bg.recfast_maxstep_hits;
+  return NAN;
+}
+
+int orc_bg_array(void *h, const char *name, double *out) {
+  const Bg &bg = *(Bg *)h;
+  std::string s(name);
+  const std::vector<double> *v = nullptr;
+  if (s == "aexp") v = &bg.aexp;
+  else if (s == "tau") v = &bg.tau;
+  else if (s == "xe") v = &bg.xe;
+  else if (s == "xeHI") v = &bg.xeHI;
+  else if (s == "xeHeI") v = &bg.xeHeI;
+  else if (s == "xeHeII") v = &bg.xeHeII;
+  else if (s == "cs2") v = &bg.cs2;
+  else if (s == "Tm") v = &bg.Tm;
+  else if (s == "mu") v = &bg.mu;
+  else if (s == "xeprime_recf") v = &bg.xeprime_recf;
+  else if (s == "xeprime") v = &bg.xeprime;
+  else if (s == "opac") v = &bg.opac;
+  else if (s == "optical_depth") v = &bg.optical_depth;
+  else if (s == "gvis") v = &bg.gvis;
+  else if (s == "gvisprime") v = &bg.gvisprime;
+  else if (s == "gvispprime") v = &bg.gvispprime;
+  else if (s == "nu_a") v = &bg.nu_a;
+  if (!v) return -1;
+  if (out) std::memcpy(out, v->data(), sizeof(double) * v->size());
+  return (int)v->size();
+}
+
+// kind: 0 evaluate, 1 derivative, 2 second derivative, 3 integral from the first knot
+int orc_bg_spline(void *h, const char *name, int kind, const double *x, int nx, double *out) {
+  const Bg &bg = *(Bg *)h;
+  const Spline *sp = find_spline(bg, name);
+  if (!sp) return -1;
+  for (int i = 0; i < nx; ++i) {
+    if (kind == 0) out[i] = sp->eval(x[i]);
+    else if (kind == 1) out[i] = sp->deriv(x[i]);
+    else if (kind == 2) out[i] = sp->deriv2(x[i]);
+    else out[i] = sp->integral(x[i]);
+  }
+  return 0;
+}
+
+// standalone natural cubic spline (for checks against scipy)
+int orc_spline_eval(const double *x, const double *y, int n, int uniform, int kind, const double *xq, int nq, double *out) {
+  Spline sp;
+  sp.build(x, y, n, uniform != 0);
+  for (int i = 0; i < nq; ++i) {
+    if (kind == 0) out[i] = sp.eval(xq[i]);
+    else if (kind == 1) out[i] = sp.deriv(xq[i]);
+    else if (kind == 2) out[i] = sp.deriv2(xq[i]);
+    else out[i] = sp.integral(xq[i]);
+  }
+  return 0;
+}
+
+void orc_nu_bins(int nq, double *q, double *w) { nu_bins(nq, q, w); }
+
+double orc_romb_test(int which, double a, double b) {  // romb of simple integrands for checks
+  if (which == 0) return romb([](double x) { return std::exp(x); }, a, b);
+  if (which == 1) return romb([](double x) { return 1.0 / (1.0 + x * x); }, a, b);
+  return romb([](double x) { return x * x * x * x * x * x * x * x * x * x * x; }, a, b);
+}
+
+int orc_layout(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
+  Layout L;
+  L.init(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax);
+  return L.n;
+}
+
+// right-hand side and exact Jacobian at (tau, y) for one mode
+int orc_rhs(void *h, const int *lmax, double tau, const double *y, double k, double *f) {
+  const Bg &bg = *(Bg *)h;
+  Layout L;
+  L.init(lmax[0], lmax[1], lmax[2], lmax[3], lmax[4]);
+  rhs_real(bg, L, tau, y, k, f);
+  return L.n;
+}
+int orc_jacobian(void *h, const int *lmax, double tau, const double *y, double k, double *J, double *dT) {
+  const Bg &bg = *(Bg *)h;
+  Layout L;
+  L.init(lmax[0], lmax[1], lmax[2], lmax[3], lmax[4]);
+  Workspace ws;
+  ws.init(L.n);
+  jacobian(bg, L, tau, y, k, ws);
+  std::memcpy(J, ws.J.data(), sizeof(double) * L.n * L.n);
+  std::memcpy(dT, ws.dT.data(), sizeof(double) * L.n);
+  return L.n;
+}
+int orc_ics(void *h, const int *lmax, double tau, double k, double *y) {
+  const Bg &bg = *(Bg *)h;
+  Layout L;
+  L.init(lmax[0], lmax[1], lmax[2], lmax[3], lmax[4]);
+  initial_conditions(bg, L, tau, k, y);
+  return L.n;
+}
+double orc_start_time(void *h, double k) { return starting_time(*(Bg *)h, k); }
+int orc_convert(void *h, const int *lmax, double k, const double *y, double *out) {
+  const Bg &bg = *(Bg *)h;
+  Layout L;
+  L.init(lmax[0], lmax[1], lmax[2], lmax[3], lmax[4]);
+  convert_output(bg, L, k, y, out);
+  return 20;
+}
+
+// One Rodas5 step (for step-level checks)
+int orc_rodas5_step(void *h, const int *lmax, double k, double t0, double dt, const double *y0, double *y1, double *err) {
+  const Bg &bg = *(Bg *)h;
+  Layout L;
+  L.init(lmax[0], lmax[1], lmax[2], lmax[3], lmax[4]);
+  Workspace ws;
+  ws.init(L.n);
+  rodas5_step(bg, L, k, t0, dt, y0, y1, err, ws);
+  return L.n;
+}
+
+// evolve_perturbations / evolve_perturbations_batched for given k-modes and output times.
+// lmax = {lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax}; solver = {rtol, atol, pcoeff, icoeff, dcoeff,
+// factormax, factormin}; batch_size 0 = independent per-mode stepping.
+int orc_evolve(void *h, const double *kmodes, int nk, const double *tau_out, int nout, const int *lmax, const double *solver,
+               int max_steps, int return_full, int batch_size, int nthreads, double *y_out, int *status, int *n_acc, int *n_rej) {
+  const Bg &bg = *(Bg *)h;
+  Layout L;
+  L.init(lmax[0], lmax[1], lmax[2], lmax[3], lmax[4]);
+  SolverCfg s{solver[0], solver[1], solver[2], solver[3], solver[4], solver[5], solver[6], max_steps};
+  int w = return_full ? L.n : 20;
+  if (batch_size <= 0) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+    for (int i = 0; i < nk; ++i)
+      status[i] = evolve_one(bg, L, kmodes[i], tau_out, nout, s, return_full != 0, y_out + (size_t)i * nout * w, n_acc + i, n_rej + i);
+  } else {
+    if (nk % batch_size != 0) return -1;
+    int nb = nk / batch_size;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+    for (int b = 0; b < nb; ++b) {
+      int st = evolve_batch(bg, L, kmodes + b * batch_size, batch_size, tau_out, nout, s, return_full != 0,
+                            y_out + (size_t)b * batch_size * nout * w, n_acc + b * batch_size, n_rej + b * batch_size);
+      for (int j = 0; j < batch_size; ++j) status[b * batch_size + j] = st;
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,232 @@
+"""ctypes front-end of the CPU oracle (oracle/deb_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg as the checker of the HIP path.  The product package never imports it.
+
+The parameter dictionary follows the reference's ``param`` keys
+(src/discoeb/background.py:141-189, src/discoeb/approximations.py:9-20).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libdeb_oracle.so")
+
+PARAM_KEYS = [
+    "Omegam", "Omegab", "H0", "Tcmb", "YHe", "Neff", "Nmnu", "mnu", "w_DE_0", "w_DE_a", "cs2_DE", "Omegak",
+    "A_s", "n_s", "k_p",
+    "use_tca", "tca_tau_c_over_tau_h_trigger", "tca_tau_c_over_tau_k_trigger",
+    "use_rsa", "rsa_tau_c_over_tau_trigger", "rsa_tau_over_tau_k_trigger",
+    "use_ur_fluid", "ur_fluid_tau_over_tau_k_trigger",
+]
+# approximations.py:9-20 defaults
+_DEFAULTS = {
+    "Omegak": 0.0, "k_p": 0.05, "cs2_DE": 1.0,
+    "use_tca": False, "tca_tau_c_over_tau_h_trigger": 0.015, "tca_tau_c_over_tau_k_trigger": 0.010,
+    "use_rsa": True, "rsa_tau_c_over_tau_trigger": 10.0, "rsa_tau_over_tau_k_trigger": 80.0,
+    "use_ur_fluid": True, "ur_fluid_tau_over_tau_k_trigger": 120.0,
+}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
+    lib = ctypes.CDLL(_LIB_PATH)
+    lib.orc_bg_new.restype = ctypes.c_void_p
+    lib.orc_bg_new.argtypes = [_dp, ctypes.c_int]
+    lib.orc_bg_free.argtypes = [ctypes.c_void_p]
+    lib.orc_bg_scalar.restype = ctypes.c_double
+    lib.orc_bg_scalar.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    lib.orc_bg_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp]
+    lib.orc_bg_spline.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _dp, ctypes.c_int, _dp]
+    lib.orc_spline_eval.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp]
+    lib.orc_nu_bins.argtypes = [ctypes.c_int, _dp, _dp]
+    lib.orc_romb_test.restype = ctypes.c_double
+    lib.orc_romb_test.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    lib.orc_layout.argtypes = [ctypes.c_int] * 5
+    lib.orc_rhs.argtypes = [ctypes.c_void_p, _ip, ctypes.c_double, _dp, ctypes.c_double, _dp]
+    lib.orc_jacobian.argtypes = [ctypes.c_void_p, _ip, ctypes.c_double, _dp, ctypes.c_double, _dp, _dp]
+    lib.orc_ics.argtypes = [ctypes.c_void_p, _ip, ctypes.c_double, ctypes.c_double, _dp]
+    lib.orc_start_time.restype = ctypes.c_double
+    lib.orc_start_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    lib.orc_convert.argtypes = [ctypes.c_void_p, _ip, ctypes.c_double, _dp, _dp]
+    lib.orc_rodas5_step.argtypes = [ctypes.c_void_p, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+    lib.orc_evolve.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, ctypes.c_int, _dp, _ip, _ip, _ip]
+    _lib = lib
+    return lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def param_vector(param):
+    vals = []
+    for k in PARAM_KEYS:
+        v = param.get(k, _DEFAULTS.get(k))
+        if v is None:
+            raise KeyError(f"missing cosmological parameter {k!r}")
+        vals.append(float(v))
+    return np.array(vals, dtype=np.float64)
+
+
+def geomspace(start, stop, num, endpoint=True):
+    """jax.numpy.geomspace arithmetic: 10**(log10(start)*(1-s) + log10(stop)*s)."""
+    l0, l1 = np.log10(start), np.log10(stop)
+    div = (num - 1) if endpoint else num
+    s = np.arange(num, dtype=np.float64) / div
+    lin = l0 * (1.0 - s) + l1 * s
+    if endpoint:
+        lin[-1] = l1
+    return 10.0 ** lin
+
+
+class Background:
+    """Oracle counterpart of evolve_background(thermo_module='RECFAST') (background.py:191-345)."""
+
+    def __init__(self, param, num_thermo=256):
+        self._lib = _load()
+        self.param = dict(param)
+        self.num_thermo = int(num_thermo)
+        pv, pp = _d(param_vector(param))
+        self._h = self._lib.orc_bg_new(pp, self.num_thermo)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.orc_bg_free(h)
+            self._h = None
+
+    def scalar(self, name):
+        return self._lib.orc_bg_scalar(self._h, name.encode())
+
+    def array(self, name):
+        n = self._lib.orc_bg_array(self._h, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n)
+        self._lib.orc_bg_array(self._h, name.encode(), out.ctypes.data_as(_dp))
+        return out
+
+    def spline(self, name, x, kind=0):
+        x, xp = _d(np.atleast_1d(x))
+        out = np.empty_like(x)
+        if self._lib.orc_bg_spline(self._h, name.encode(), int(kind), xp, x.size, out.ctypes.data_as(_dp)) != 0:
+            raise KeyError(name)
+        return out
+
+    # --- per-mode pieces ------------------------------------------------------------------
+    def rhs(self, lmax, tau, y, k):
+        lm, lp = _i(lmax)
+        y, yp = _d(y)
+        f = np.empty_like(y)
+        self._lib.orc_rhs(self._h, lp, float(tau), yp, float(k), f.ctypes.data_as(_dp))
+        return f
+
+    def jacobian(self, lmax, tau, y, k):
+        lm, lp = _i(lmax)
+        y, yp = _d(y)
+        n = y.size
+        J = np.empty((n, n))
+        dT = np.empty(n)
+        self._lib.orc_jacobian(self._h, lp, float(tau), yp, float(k), J.ctypes.data_as(_dp), dT.ctypes.data_as(_dp))
+        return J, dT
+
+    def ics(self, lmax, tau, k):
+        lm, lp = _i(lmax)
+        n = self._lib.orc_layout(*[int(v) for v in lmax])
+        y = np.empty(n)
+        self._lib.orc_ics(self._h, lp, float(tau), float(k), y.ctypes.data_as(_dp))
+        return y
+
+    def start_time(self, k):
+        return self._lib.orc_start_time(self._h, float(k))
+
+    def convert(self, lmax, k, y):
+        lm, lp = _i(lmax)
+        y, yp = _d(y)
+        out = np.empty(20)
+        self._lib.orc_convert(self._h, lp, float(k), yp, out.ctypes.data_as(_dp))
+        return out
+
+    def rodas5_step(self, lmax, k, t0, dt, y0):
+        lm, lp = _i(lmax)
+        y0, yp = _d(y0)
+        y1 = np.empty_like(y0)
+        err = np.empty_like(y0)
+        self._lib.orc_rodas5_step(self._h, lp, float(k), float(t0), float(dt), yp, y1.ctypes.data_as(_dp),
+                                  err.ctypes.data_as(_dp))
+        return y1, err
+
+    def evolve(self, kmodes, aexp_out, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4,
+               pcoeff=0.25, icoeff=0.80, dcoeff=0.0, factormax=20.0, factormin=0.3, max_steps=2048,
+               return_full=False, batch_size=0, nthreads=None):
+        """evolve_perturbations (batch_size=0) / evolve_perturbations_batched on given k-modes.
+
+        Returns (y[num_k, nout, 20 or nvar], tau_out, status, n_accepted, n_rejected)."""
+        kmodes, kp = _d(kmodes)
+        aexp_out = np.atleast_1d(np.asarray(aexp_out, dtype=np.float64))
+        tau_out, tp = _d(self.spline("tau_of_a", aexp_out))
+        lmax = [lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax]
+        lm, lp = _i(lmax)
+        n = self._lib.orc_layout(*lmax)
+        w = n if return_full else 20
+        sv, sp = _d([rtol, atol, pcoeff, icoeff, dcoeff, factormax, factormin])
+        nk = kmodes.size
+        y = np.empty((nk, tau_out.size, w))
+        st = np.zeros(nk, dtype=np.int32)
+        na = np.zeros(nk, dtype=np.int32)
+        nr = np.zeros(nk, dtype=np.int32)
+        if nthreads is None:
+            nthreads = os.cpu_count() or 1
+        rc = self._lib.orc_evolve(self._h, kp, nk, tp, tau_out.size, lp, sp, int(max_steps), int(bool(return_full)),
+                                  int(batch_size), int(nthreads), y.ctypes.data_as(_dp), st.ctypes.data_as(_ip),
+                                  na.ctypes.data_as(_ip), nr.ctypes.data_as(_ip))
+        if rc != 0:
+            raise ValueError("num_k must be a multiple of batch_size")
+        return y, tau_out, st, na, nr
+
+
+def nvar(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax):
+    return _load().orc_layout(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax)
+
+
+def nu_bins(nq):
+    q = np.empty(nq)
+    w = np.empty(nq)
+    _load().orc_nu_bins(nq, q.ctypes.data_as(_dp), w.ctypes.data_as(_dp))
+    return q, w
+
+
+def spline_eval(x, y, xq, kind=0, uniform=False):
+    x, xp = _d(x)
+    y, yp = _d(y)
+    xq, xqp = _d(np.atleast_1d(xq))
+    out = np.empty_like(xq)
+    _load().orc_spline_eval(xp, yp, x.size, int(uniform), int(kind), xqp, xq.size, out.ctypes.data_as(_dp))
+    return out
+
+
+def romb_test(which, a, b):
+    return _load().orc_romb_test(which, a, b)
+
+
+def get_power(k, y, idx, param):
+    """P(k) = 2 pi^2 A_s (k/k_p)^(n_s-1) k^-3 y[..., idx]^2 (perturbations.py:1446-1468)."""
+    return 2 * np.pi ** 2 * param["A_s"] * (k / param.get("k_p", 0.05)) ** (param["n_s"] - 1) * k ** (-3.0) * y[..., idx] ** 2
