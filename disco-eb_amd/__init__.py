@@ -1,0 +1,12 @@
+"""MI355X-native linear perturbation solver with the call surface of DISCO-EB's
+background -> evolve_perturbations -> P(k) path (lawrenceberry/DISCO-EB).
+
+All compute runs in hand-written gfx950 HIP kernels (libdiscoeb_amd.so, C ABI in
+include/discoeb_amd.h); this package is the Python host side.
+"""
+from . import _lib
+from .background import DeviceSpline, attach_background, background_tables, compute_background_quantities, evolve_background
+from .perturbations import compute_time_derivatives, evolve_perturbations, evolve_units, get_power, k_grid, nvar
+
+__all__ = ["evolve_background", "background_tables", "attach_background", "compute_background_quantities", "DeviceSpline",
+           "evolve_perturbations", "evolve_units", "get_power", "compute_time_derivatives", "k_grid", "nvar"]

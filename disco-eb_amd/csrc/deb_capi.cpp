@@ -1,0 +1,115 @@
+// deb_capi.cpp -- C ABI entry points (include/discoeb_amd.h): argument checks, error strings,
+// kernel launches on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/discoeb_amd.h"
+
+extern "C" int deb_launch_background(const double *, int, int, double *, hipStream_t);
+extern "C" int deb_launch_nu_bins(int, double *, hipStream_t);
+extern "C" int deb_launch_perturb(const double *, int, const int *, const double *, int, const double *, int, const double *,
+                                  const DebProblem *, const DebSolverCfg *, double *, int *, int *, int *, hipStream_t);
+extern "C" long deb_perturb_lds_bytes(int, int, int, int, int);
+extern "C" int deb_launch_spline_eval(const double *, int, int, int, const double *, int, double *, hipStream_t);
+extern "C" int deb_launch_power(const double *, int, const double *, long, int, int, double, double, double, double *, hipStream_t);
+extern "C" int deb_launch_time_deriv(const double *, const double *, const double *, int, const double *, int, const double *,
+                                     const DebProblem *, double *, hipStream_t);
+
+namespace {
+thread_local std::string g_err;
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+int hip_rc(int rc, const char *what) {
+  if (rc != 0) {
+    g_err = std::string(what) + ": " + hipGetErrorString((hipError_t)rc);
+    return rc;
+  }
+  g_err.clear();
+  return 0;
+}
+int check_problem(const DebProblem *p) {
+  if (!p) return fail(-1, "problem descriptor is NULL");
+  if (p->lmaxg < 3 || p->lmaxgp < 3 || p->lmaxr < 3 || p->lmaxnu < 3)
+    return fail(-1, "hierarchy truncations lmaxg, lmaxgp, lmaxr, lmaxnu must be >= 3");
+  if (p->lmaxg > 250 || p->lmaxgp > 250 || p->lmaxr > 250 || p->lmaxnu > 250) return fail(-1, "hierarchy truncations must be <= 250");
+  if (p->nqmax < 1 || p->nqmax > 16) return fail(-1, "nqmax must be in [1, 16]");
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+const char *deb_version(void) { return "discoeb_amd 0.1 (gfx950)"; }
+const char *deb_last_error(void) { return g_err.c_str(); }
+long deb_tables_size(int num_thermo) { return deb_table_doubles(num_thermo); }
+long deb_table_offset(int slot, int num_thermo) { return deb_array_offset(slot, num_thermo); }
+long deb_evolve_workspace_doubles(void) { return 32; }
+
+int deb_nvar(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
+  return 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + nqmax * (lmaxnu + 1) + 2;
+}
+
+int deb_background(const double *d_params, int n_cosmo, int num_thermo, double *d_tables, void *stream) {
+  if (n_cosmo < 0) return fail(-1, "n_cosmo must be >= 0");
+  if (num_thermo < 16) return fail(-1, "num_thermo must be >= 16");
+  if (n_cosmo == 0) return 0;
+  if (!d_params || !d_tables) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_background(d_params, n_cosmo, num_thermo, d_tables, (hipStream_t)stream), "deb_background");
+}
+
+int deb_spline_eval(const double *d_tables, int num_thermo, int cosmo, int spline_id, int kind, const double *d_x, int nx,
+                    double *d_out, void *stream) {
+  if (spline_id < 0 || spline_id >= DEB_SPL_COUNT) return fail(-1, "unknown spline id");
+  if (kind < 0 || kind > 3) return fail(-1, "kind must be 0 (value), 1, 2 (derivatives) or 3 (integral)");
+  if (kind == 3 && spline_id != DEB_SPL_OPAC) return fail(-1, "integrals are tabulated for the opacity spline only");
+  if (cosmo < 0) return fail(-1, "cosmology index must be >= 0");
+  const double *t = d_tables + (long)cosmo * deb_table_doubles(num_thermo);
+  return hip_rc(deb_launch_spline_eval(t, num_thermo, spline_id, kind, d_x, nx, d_out, (hipStream_t)stream), "deb_spline_eval");
+}
+
+int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
+               const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
+               int *d_n_accept, int *d_n_reject, double *d_work, void *stream) {
+  int rc = check_problem(prob);
+  if (rc) return rc;
+  if (!cfg) return fail(-1, "solver descriptor is NULL");
+  if (!(cfg->rtol > 0) || !(cfg->atol >= 0)) return fail(-1, "rtol must be > 0 and atol >= 0");
+  if (cfg->max_steps < 1) return fail(-1, "max_steps must be >= 1");
+  if (!(cfg->factormin > 0) || !(cfg->factormax >= 1)) return fail(-1, "factormin must be > 0 and factormax >= 1");
+  if (nout < 1) return fail(-1, "at least one output time is required");
+  if (n_units < 0) return fail(-1, "n_units must be >= 0");
+  if (n_units == 0) return 0;
+  if (!d_tables || !d_unit_cosmo || !d_unit_k || !d_tau_out || !d_y_out || !d_status || !d_n_accept || !d_n_reject || !d_work)
+    return fail(-1, "NULL device pointer");
+  long lds = deb_perturb_lds_bytes(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
+  if (lds > 160 * 1024) return fail(-1, "state too large for the 160 KiB LDS of one CU (lower the hierarchy truncations)");
+  rc = deb_launch_nu_bins(prob->nqmax, d_work, (hipStream_t)stream);
+  if (rc) return hip_rc(rc, "deb_evolve (momentum bins)");
+  rc = deb_launch_perturb(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, d_work, prob, cfg, d_y_out, d_status,
+                          d_n_accept, d_n_reject, (hipStream_t)stream);
+  if (rc == -2) return fail(-1, "state too large for LDS");
+  return hip_rc(rc, "deb_evolve");
+}
+
+int deb_power(const double *d_k, int k_repeat, const double *d_y, long n, int ystride, int idx, double A_s, double n_s, double k_p,
+              double *d_pk, void *stream) {
+  if (k_repeat < 1 || ystride < 1 || idx < 0 || idx >= ystride) return fail(-1, "invalid stride, repeat or field index");
+  return hip_rc(deb_launch_power(d_k, k_repeat, d_y, n, ystride, idx, A_s, n_s, k_p, d_pk, (hipStream_t)stream), "deb_power");
+}
+
+int deb_time_derivatives(const double *d_tables, int num_thermo, int cosmo, const double *d_y, const double *d_tau, int n_tau,
+                         const double *d_k, int n_k, const DebProblem *prob, double *d_dy, double *d_work, void *stream) {
+  int rc = check_problem(prob);
+  if (rc) return rc;
+  if (n_k <= 0 || n_tau <= 0) return 0;
+  const double *t = d_tables + (long)cosmo * deb_table_doubles(num_thermo);
+  rc = deb_launch_nu_bins(prob->nqmax, d_work, (hipStream_t)stream);
+  if (rc) return hip_rc(rc, "deb_time_derivatives (momentum bins)");
+  return hip_rc(deb_launch_time_deriv(t, d_y, d_tau, n_tau, d_k, n_k, d_work, prob, d_dy, (hipStream_t)stream), "deb_time_derivatives");
+}
+
+}  // extern "C"

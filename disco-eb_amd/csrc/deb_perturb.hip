@@ -1,0 +1,997 @@
+// deb_perturb.hip -- synchronous-gauge perturbations of one k-mode per wavefront, integrated
+// with the linearly implicit Rodas5 method under a PI step-size controller.
+//
+// Follows the reference's evolve_one_mode / evolve_modes_batched (src/discoeb/perturbations.py:
+// 882-1120), model_synchronous (:88-525), approximations.py, Rodas5Transformed
+// (ode_integrators_stiff.py:678-892) and diffrax's PIDController + SaveAt(ts) with linear
+// interpolation between accepted steps.
+//
+// Solve of W = I/(gamma h) - J, exact up to rounding, exploiting the structure of J:
+//   * row 0 (the scale factor) depends on a only, so x_0 = b_0 / W_00 is solved first and the
+//     column of a is moved to the right-hand side;
+//   * every Boltzmann hierarchy tail (l >= 3: photon temperature, photon polarisation,
+//     massless neutrinos, each massive-neutrino momentum bin) is a tridiagonal chain that
+//     touches the rest of the system only through its l = 2 variable; a bottom-up sweep
+//     eliminates each chain into the dense block;
+//   * the dense block D (metric, CDM, baryons, l <= 2 moments, dark energy: 17 + 3 nq
+//     unknowns) receives one diagonal correction per chain (Schur complement) and is
+//     inverted by Gauss-Jordan with partial pivoting in LDS; each of the 8 stage solves is
+//     then a mat-vec plus two chain sweeps.
+// The Jacobian column of a and df/dtau come from dual-number evaluations of the RHS; the
+// dense block's columns from lane-parallel evaluation of the (linear) RHS rows on unit
+// vectors; chain entries are the hierarchy coefficients themselves.
+#include <hip/hip_runtime.h>
+
+#include "deb_cosmo.hpp"
+#include "deb_perturb.h"
+
+namespace deb {
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int NQMAX = 16;
+
+// Rodas5 tableau (transformed form), ode_integrators_stiff.py:680-745
+namespace r5 {
+constexpr double gamma = 0.19;
+constexpr double a21 = 2.0, a31 = 3.040894194418781, a32 = 1.041747909077569, a41 = 2.576417536461461,
+                 a42 = 1.622083060776640, a43 = -0.9089668560264532, a51 = 2.760842080225597, a52 = 1.446624659844071,
+                 a53 = -0.3036980084553738, a54 = 0.2877498600325443, a61 = -14.09640773051259, a62 = 6.925207756232704,
+                 a63 = -41.47510893210728, a64 = 2.343771018586405, a65 = 24.13215229196062;
+constexpr double C21 = -10.31323885133993, C31 = -21.04823117650003, C32 = -7.234992135176716, C41 = 32.22751541853323,
+                 C42 = -4.943732386540191, C43 = 19.44922031041879, C51 = -20.69865579590063, C52 = -8.816374604402768,
+                 C53 = 1.260436877740897, C54 = -0.7495647613787146, C61 = -46.22004352711257, C62 = -17.49534862857472,
+                 C63 = -289.6389582892057, C64 = 93.60855400400906, C65 = 318.3822534212147, C71 = 34.20013733472935,
+                 C72 = -14.15535402717690, C73 = 57.82335640988400, C74 = 25.83362985412365, C75 = 1.408950972071624,
+                 C76 = -6.551835421242162, C81 = 42.57076742291101, C82 = -13.80770672017997, C83 = 93.98938432427124,
+                 C84 = 18.77919633714503, C85 = -31.58359187223370, C86 = -6.685968952921985, C87 = -5.810979938412932;
+constexpr double c2 = 0.38, c3 = 0.3878509998321533, c4 = 0.4839718937873840, c5 = 0.4570477008819580;
+constexpr double d1 = gamma, d2 = -0.1823079225333714636, d3 = -0.319231832186874912, d4 = 0.3449828624725343,
+                 d5 = -0.377417564392089818;
+}  // namespace r5
+
+// ---------------------------------------------------------------------------------------
+// state layout (perturbations.py:134-186, 893)
+// ---------------------------------------------------------------------------------------
+struct PLay {
+  int lmaxg, lmaxgp, lmaxr, lmaxnu, nq, n, idxg, idxgp, idxr, iq0, m, nch;
+  const double *q, *w, *dlfdlq;  // momentum bins (LDS)
+};
+
+// neutrino momentum bins (background.py:14-45): CAMB tables for 3-5 bins, else generalized
+// Gauss-Laguerre (alpha = 1) reweighted to the Fermi-Dirac integrand; qw[0..nq) nodes,
+// qw[NQMAX..NQMAX+nq) weights
+__global__ void nu_bins_kernel(int nq, double *qw) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double fdc = 5.682196976983475;
+  if (nq >= 3 && nq <= 5) {
+    const double q3[] = {0.913201, 3.37517, 7.79184}, w3[] = {0.0687359, 3.31435, 2.29911};
+    const double q4[] = {0.7, 2.62814, 5.90428, 12.0}, w4[] = {0.0200251, 1.84539, 3.52736, 0.289427};
+    const double q5[] = {0.583165, 2.0, 4.0, 7.26582, 13.0}, w5[] = {0.0081201, 0.689407, 2.8063, 2.05156, 0.12681};
+    const double *qq = nq == 3 ? q3 : nq == 4 ? q4 : q5;
+    const double *ww = nq == 3 ? w3 : nq == 4 ? w4 : w5;
+    for (int i = 0; i < nq; ++i) {
+      double dl = -qq[i] / (1.0 + exp(-qq[i]));
+      qw[i] = qq[i];
+      qw[NQMAX + i] = ww[i] / (-0.25 * dl) / fdc;
+    }
+    return;
+  }
+  gauss_laguerre_fd(nq, qw, qw + NQMAX);
+}
+
+// ---------------------------------------------------------------------------------------
+// background and thermodynamics at (tau, a) for one mode
+// ---------------------------------------------------------------------------------------
+template <class T> struct PBg {
+  T a, tau, aH, aHH, cs2, xe, xeprime, opac, tauc, taucprime, pb43, rho_Q, w_Q, ca2_Q;
+  T v[NQMAX];
+  bool rsa, ufa, tca;
+};
+
+template <class T>
+__device__ void pbg_eval(const BgView &b, const PLay &L, double k, const T &tau, const T &a, PBg<T> &g) {
+  g.a = a;
+  g.tau = tau;
+  T loga = dlog(a);
+  int i = spl_index(b.loga, b.N, val(loga));
+  g.cs2 = spl_eval_at(b.loga, b.cs2a, b.S_cs2a_of_loga, i, loga) / a;
+  g.xe = spl_eval_at(b.loga, b.xe, b.S_xe_of_loga, i, loga);
+  T dxe = spl_deriv_at(b.loga, b.xe, b.S_xe_of_loga, i, loga);
+  g.pb43 = 4.0 / 3.0 * (b.grhog / (b.grhom * b.Omegab * a));
+  int j = spl_index_uniform(b.nu_x0, b.nu_dxinv, DEB_NNU, val(loga));
+  T rhonu = dexp(spl_eval_at(b.nu_loga, b.nu_lr, b.nu_lrS, j, loga));
+  T pnu = dexp(spl_eval_at(b.nu_loga, b.nu_lp, b.nu_lpS, j, loga));
+  g.w_Q = b.w0 + b.wa * (1.0 - a);
+  g.rho_Q = b.rho_Q(a);
+  T a2 = a * a;
+  T grho = b.grhom * b.Omegam / a + (b.grhog + b.grhor * (b.Neff + b.Nmnu * rhonu)) / a2 + b.grhom * b.OmegaDE * g.rho_Q * a2 +
+           b.grhom * b.Omegak;
+  g.aH = dsqrt(grho / 3.0);
+  g.xeprime = dxe * g.aH;
+  T gpres = ((b.grhog + b.grhor * b.Neff) / 3.0 + b.grhor * b.Nmnu * pnu) / a2 + g.w_Q * b.grhom * b.OmegaDE * g.rho_Q * a2;
+  g.aHH = 0.5 * (g.aH * g.aH - gpres);
+  T wprime = -b.wa * g.aH * a;
+  g.ca2_Q = g.w_Q - wprime / 3.0 / ((1.0 + g.w_Q) + 1e-6) / g.aH;
+  g.opac = g.xe * b.akthom / a2;
+  g.tauc = 1.0 / dmax(g.opac, 1e-30);
+  g.taucprime = g.tauc * (2.0 * g.aH - g.xeprime / dmax(g.xe, 1e-30));
+  // approximation switches (approximations.py:50-75)
+  double tv = val(tau), tc = val(g.tauc), th = 1.0 / fmax(val(g.aH), 1e-30), tk = 1.0 / fmax(k, 1e-30);
+  g.rsa = b.use_rsa && (tc / fmax(tv, 1e-30) > b.rsa_c) && (tv * k > b.rsa_k);
+  g.ufa = !g.rsa && b.use_ufa && (tv * k > b.ufa_k);
+  bool uncoupled = (tc / fmax(tk, 1e-30) > b.tca_k) || ((tc / fmax(th, 1e-30) > b.tca_h) && (tc / fmax(tk, 1e-30) > 0.1 * b.tca_k));
+  g.tca = !g.rsa && b.use_tca && !uncoupled;
+  for (int q = 0; q < L.nq; ++q) {
+    T aq = a * b.amnu / L.q[q];
+    g.v[q] = 1.0 / dsqrt(1.0 + aq * aq);
+  }
+}
+
+// metric perturbations and effective (RSA/TCA) fields, perturbations.py:292-351,407-427
+template <class T> struct Met {
+  T hprime, etaprime, alpha, dgrho, dgpres, dg, tg, sg, dr, tr, sr, dbp_tca, tbp_tca, dgp_tca, tgp_tca;
+};
+
+template <class T, class Y>
+__device__ void metric(const BgView &b, const PLay &L, double k, const PBg<T> &g, const Y &y, Met<T> &m) {
+  const int n = L.n, nq = L.nq, iq0 = L.iq0;
+  const double Omegac = b.Omegam - b.Omegab;
+  const double eta = y(2), deltac = y(3), thetac = y(4), deltab = y(5), thetab = y(6);
+  const double deltag = y(7), thetag = y(8), shearg = y(9) / 2.0;
+  const double deltar = y(L.idxr), thetar = y(L.idxr + 1), shearr = y(L.idxr + 2) / 2.0;
+  const double deltaq = y(n - 2), thetaq = y(n - 1);
+  T drhonu(0.0), dpnu(0.0), fnu(0.0), shearnu(0.0);
+  for (int q = 0; q < nq; ++q) {
+    double p0 = y(iq0 + q), p1 = y(iq0 + nq + q), p2 = y(iq0 + 2 * nq + q);
+    drhonu += L.w[q] * p0 / g.v[q];
+    dpnu += L.w[q] * p0 * g.v[q];
+    fnu += T(L.w[q] * p1);
+    shearnu += L.w[q] * p2 * g.v[q];
+  }
+  dpnu = dpnu / 3.0;
+  shearnu = shearnu * 2.0 / 3.0;
+  const T a = g.a, a2 = g.a * g.a;
+  const double cs2_Q = b.cs2DE;
+  T rpptq = (1.0 + g.w_Q) * g.rho_Q * b.grhom * b.OmegaDE * thetaq * a2;
+  T de_rho = b.grhom * b.OmegaDE * deltaq * g.rho_Q * a2;
+  T dgrho = b.grhom * (Omegac * deltac + b.Omegab * deltab) / a + (b.grhog * deltag + b.grhor * (b.Neff * deltar + b.Nmnu * drhonu)) / a2 +
+            de_rho;
+  T dgrho_wo = b.grhom * (Omegac * deltac + b.Omegab * deltab) / a + b.grhor * b.Nmnu * drhonu / a2 + de_rho;
+  T hp_rsa = (2.0 * k * k * eta + dgrho_wo) / dmax(g.aH, 1e-30);
+  if (g.rsa) {  // compute_fields_rsa (approximations.py:28-47)
+    const double sk2 = fmax(k * k, 1e-30);
+    T stc = dmax(g.tauc, 1e-30);
+    m.dg = 4.0 / sk2 * (g.aH * hp_rsa - sk2 * eta) + 4.0 / (sk2 * stc) * (thetab + 0.5 * hp_rsa);
+    m.tg = -0.5 * hp_rsa + 3.0 / (sk2 * stc) * (-g.taucprime / stc * (thetab + 0.5 * hp_rsa) + (-g.aH * thetab + g.cs2 * sk2 * deltab + sk2 * eta));
+    m.sg = T(0.0);
+    m.dr = 4.0 / sk2 * (g.aH * hp_rsa - sk2 * eta);
+    m.tr = -0.5 * hp_rsa;
+    m.sr = T(0.0);
+  } else {
+    m.dg = T(deltag); m.tg = T(thetag); m.sg = T(shearg);
+    m.dr = T(deltar); m.tr = T(thetar); m.sr = T(shearr);
+  }
+  m.dgpres = (b.grhog * m.dg + b.grhor * b.Neff * m.dr) / a2 / 3.0 + b.grhor * b.Nmnu * dpnu / a2 +
+             (cs2_Q * de_rho + (cs2_Q - g.ca2_Q) * (3.0 * g.aH * rpptq / (k * k)));
+  T dgtheta = b.grhom * (Omegac * thetac + b.Omegab * thetab) / a + 4.0 / 3.0 * (b.grhog * m.tg + b.Neff * b.grhor * m.tr) / a2 +
+              b.Nmnu * b.grhor * k * fnu / a2 + rpptq;
+  m.dgrho = dgrho;
+  m.hprime = g.rsa ? hp_rsa : (2.0 * k * k * eta + dgrho) / g.aH;
+  m.etaprime = 0.5 * dgtheta / (k * k);
+  m.alpha = (m.hprime + 6.0 * m.etaprime) / 2.0 / (k * k);
+  if (g.tca) {  // _photon_tca (perturbations.py:407-435)
+    const double k2 = k * k;
+    m.dbp_tca = -thetab - 0.5 * m.hprime;
+    T tbp = (-g.aH * thetab + k2 * g.cs2 * deltab + k2 * g.pb43 * (0.25 * deltag - shearg)) / (1.0 + g.pb43);
+    m.dgp_tca = 4.0 / 3.0 * (-thetag - 0.5 * m.hprime);
+    T slip = 2.0 * g.pb43 / (1.0 + g.pb43) * g.aH * (thetab - thetag) +
+             g.tauc * (-g.aHH * thetab - 0.5 * g.aH * k2 * deltag + k2 * (g.cs2 * m.dbp_tca - 0.25 * m.dgp_tca)) / (1.0 + g.pb43);
+    tbp = tbp + g.pb43 / (1.0 + g.pb43) * slip;
+    m.tbp_tca = tbp;
+    m.tgp_tca = (-tbp - g.aH * thetab + k2 * g.cs2 * deltab) / dmax(g.pb43, 1e-30) + k2 * (0.25 * deltag - shearg);
+  }
+}
+
+// RHS row i for i outside the l >= 3 tails (row 0 included)
+template <class T, class Y>
+__device__ T core_row(int i, const BgView &b, const PLay &L, double k, const PBg<T> &g, const Met<T> &m, const Y &y) {
+  const int n = L.n, nq = L.nq, iq0 = L.iq0, idxgp = L.idxgp, idxr = L.idxr;
+  const double k2 = k * k;
+  if (i == 0) return g.aH * g.a;
+  if (i == 1) return -(m.dgrho + 3.0 * m.dgpres) * g.a;
+  if (i == 2) return m.etaprime;
+  if (i == 3) return -y(4) - 0.5 * m.hprime;
+  if (i == 4) return -g.aH * y(4);
+  if (i == 5) return -y(6) - 0.5 * m.hprime;
+  if (i == 6) {
+    if (g.tca) return m.tbp_tca;
+    return -g.aH * y(6) + k2 * g.cs2 * y(5) + g.pb43 * g.opac * (m.tg - y(6));
+  }
+  if (i <= 9) {  // photon l = 0, 1, 2
+    if (g.rsa) return T(0.0);
+    if (g.tca) return (i == 7) ? m.dgp_tca : (i == 8 ? m.tgp_tca : T(0.0));
+    if (i == 7) return 4.0 / 3.0 * (-m.tg - 0.5 * m.hprime);
+    if (i == 8) return k2 * (0.25 * m.dg - m.sg) - g.opac * (m.tg - y(6));
+    double polter = y(9) + y(idxgp) + y(idxgp + 2);
+    return 8.0 / 15.0 * (m.tg + k2 * m.alpha) - 3.0 / 5.0 * k * y(10) - g.opac * (y(9) - 0.1 * polter);
+  }
+  if (i >= idxgp && i <= idxgp + 2) {  // polarisation l = 0, 1, 2
+    if (g.rsa || g.tca) return T(0.0);
+    double polter = y(9) + y(idxgp) + y(idxgp + 2);
+    int l = i - idxgp;
+    if (l == 0) return -k * y(idxgp + 1) - g.opac * y(idxgp) + g.opac * polter / 2.0;
+    if (l == 1) return k / 3.0 * (y(idxgp) - 2.0 * y(idxgp + 2)) - g.opac * y(idxgp + 1);
+    return k / 5.0 * (2.0 * y(idxgp + 1) - 3.0 * y(idxgp + 3)) - g.opac * y(idxgp + 2) + g.opac * polter / 10.0;
+  }
+  if (i >= idxr && i <= idxr + 2) {  // massless neutrinos l = 0, 1, 2
+    if (g.rsa) return T(0.0);
+    int l = i - idxr;
+    if (l == 0) return 4.0 / 3.0 * (-m.tr - 0.5 * m.hprime);
+    if (l == 1) return k2 * (0.25 * m.dr - m.sr);
+    if (g.ufa) return -3.0 / dmax(g.tau, 1e-30) * m.sr + 4.0 / 15.0 * (m.tr + 0.5 * m.hprime);
+    return 8.0 / 15.0 * (m.tr + k2 * m.alpha) - 0.6 * k * y(idxr + 3);
+  }
+  if (i >= iq0 && i < iq0 + 3 * nq) {  // massive neutrinos l = 0, 1, 2
+    int l = (i - iq0) / nq, q = (i - iq0) - l * nq;
+    T kv = k * g.v[q];
+    double dl = L.dlfdlq[q];
+    if (l == 0) return -kv * y(iq0 + nq + q) + m.hprime * dl / 6.0;
+    if (l == 1) return kv * (y(iq0 + q) - 2.0 * y(iq0 + 2 * nq + q)) / 3.0;
+    return kv * (2.0 * y(iq0 + nq + q) - 3.0 * y(iq0 + 3 * nq + q)) / 5.0 - (m.hprime / 15.0 + 2.0 / 5.0 * m.etaprime) * dl;
+  }
+  if (b.lambda_de) return T(0.0);
+  const double cs2_Q = b.cs2DE;
+  if (i == n - 2)  // BL10 (3.5)
+    return -(1.0 + g.w_Q) * (y(n - 1) + 0.5 * m.hprime) - 3.0 * (cs2_Q - g.w_Q) * g.aH * y(n - 2) -
+           9.0 * (1.0 + g.w_Q) * (cs2_Q - g.ca2_Q) * g.aH * g.aH / k2 * y(n - 1);
+  return -(1.0 - 3.0 * cs2_Q) * g.aH * y(n - 1) + cs2_Q / (1.0 + g.w_Q) * k2 * y(n - 2);  // BL10 (3.6)
+}
+
+// tail chain coefficients: f_l = lo y_{l-1} + di y_l + up y_{l+1}, l = 3..lmax
+template <class T>
+__device__ void tail_coef(int ch, int l, const PLay &L, double k, const PBg<T> &g, T &lo, T &di, T &up) {
+  int lmax;
+  bool off;
+  T kv(k), damp(0.0);
+  if (ch == 0) { lmax = L.lmaxg; off = g.rsa || g.tca; damp = g.opac; }
+  else if (ch == 1) { lmax = L.lmaxgp; off = g.rsa || g.tca; damp = g.opac; }
+  else if (ch == 2) { lmax = L.lmaxr; off = g.rsa || g.ufa; }
+  else { lmax = L.lmaxnu; off = false; kv = k * g.v[ch - 3]; }
+  if (off) { lo = T(0.0); di = T(0.0); up = T(0.0); return; }
+  if (l < lmax) {
+    double inv = 1.0 / (double)(2 * l + 1);
+    lo = kv * ((double)l * inv);
+    up = kv * (-(double)(l + 1) * inv);
+    di = -damp;
+  } else {
+    lo = kv;
+    up = T(0.0);
+    di = -(double)(lmax + 1) / g.tau - damp;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// wave helpers
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_or(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+  return v;
+}
+
+// per-wave LDS workspace
+struct Ws {
+  double *y0, *y1, *u, *f, *J0, *dT, *K, *S, *Jh, *tlo, *tdi, *tup, *tmul, *td, *xd;
+  int *dmap, *rk, *chead, *cstride, *clen, *cpar;
+};
+
+
+// rows of the tail chains: rk[i] = (chain << 8) | l, -1 for rows in the dense block or row 0
+__device__ void setup_maps(const PLay &L, Ws &W, int lane) {
+  const int n = L.n, nq = L.nq;
+  if (lane == 0) {
+    int r = 0;
+    for (int i = 1; i <= 9; ++i) W.dmap[r++] = i;
+    for (int i = 0; i < 3; ++i) W.dmap[r++] = L.idxgp + i;
+    for (int i = 0; i < 3; ++i) W.dmap[r++] = L.idxr + i;
+    for (int i = 0; i < 3 * nq; ++i) W.dmap[r++] = L.iq0 + i;
+    W.dmap[r++] = n - 2;
+    W.dmap[r++] = n - 1;
+    // chains: head index, stride, length, dense-block row of the l = 2 parent
+    W.chead[0] = L.idxg + 3; W.cstride[0] = 1; W.clen[0] = L.lmaxg - 2; W.cpar[0] = 8;
+    W.chead[1] = L.idxgp + 3; W.cstride[1] = 1; W.clen[1] = L.lmaxgp - 2; W.cpar[1] = 11;
+    W.chead[2] = L.idxr + 3; W.cstride[2] = 1; W.clen[2] = L.lmaxr - 2; W.cpar[2] = 14;
+    for (int q = 0; q < nq; ++q) {
+      W.chead[3 + q] = L.iq0 + 3 * nq + q; W.cstride[3 + q] = nq; W.clen[3 + q] = L.lmaxnu - 2; W.cpar[3 + q] = 15 + 2 * nq + q;
+    }
+  }
+  for (int i = lane; i < n; i += WAVE) W.rk[i] = -1;
+  __syncthreads();
+  for (int c = 0; c < L.nch; ++c)
+    for (int j = lane; j < W.clen[c]; j += WAVE) W.rk[W.chead[c] + j * W.cstride[c]] = (c << 8) | (3 + j);
+  __syncthreads();
+}
+
+// full RHS f(tau, u) into out[] (lanes over rows); u in LDS
+__device__ void rhs_eval(const BgView &b, const PLay &L, double k, double tau, const double *u, double *out, Ws &W, int lane) {
+  PBg<double> g;
+  pbg_eval<double>(b, L, k, tau, u[0], g);
+  auto yv = [&](int i) { return u[i]; };
+  Met<double> m;
+  metric<double>(b, L, k, g, yv, m);
+  for (int i = lane; i < L.n; i += WAVE) {
+    int rk = W.rk[i];
+    double r;
+    if (rk < 0) {
+      r = core_row<double>(i, b, L, k, g, m, yv);
+    } else {
+      int ch = rk >> 8, l = rk & 255;
+      double lo, di, up;
+      tail_coef<double>(ch, l, L, k, g, lo, di, up);
+      int s = W.cstride[ch];
+      double yn = (up != 0.0) ? u[i + s] : 0.0;
+      r = lo * u[i - s] + di * u[i] + up * yn;
+    }
+    out[i] = r;
+  }
+}
+
+// d f / d(seed) at (tau, y) with seed = a (which = 0) or tau (which = 1)
+__device__ void rhs_dual(const BgView &b, const PLay &L, double k, double tau, const double *y, double *out, Ws &W, int lane,
+                         int which) {
+  Dual td(tau), ad(y[0]);
+  if (which == 0) ad.d[0] = 1.0; else td.d[0] = 1.0;
+  PBg<Dual> g;
+  pbg_eval<Dual>(b, L, k, td, ad, g);
+  auto yv = [&](int i) { return y[i]; };
+  Met<Dual> m;
+  metric<Dual>(b, L, k, g, yv, m);
+  for (int i = lane; i < L.n; i += WAVE) {
+    int rk = W.rk[i];
+    Dual r;
+    if (rk < 0) {
+      r = core_row<Dual>(i, b, L, k, g, m, yv);
+    } else {
+      int ch = rk >> 8, l = rk & 255;
+      Dual lo, di, up;
+      tail_coef<Dual>(ch, l, L, k, g, lo, di, up);
+      int s = W.cstride[ch];
+      double yn = (up.v != 0.0 || up.d[0] != 0.0) ? y[i + s] : 0.0;
+      r = lo * y[i - s] + di * y[i] + up * yn;
+    }
+    out[i] = r.d[0];
+  }
+}
+
+// Jacobian pieces at (t0, y0) and the structured factorisation of W = I/(gamma dt) - J
+__device__ void factorise(const BgView &b, const PLay &L, double k, double t0, double dt, Ws &W, int lane, double &W00) {
+  const int n = L.n, m = L.m, nch = L.nch;
+  const double gi = 1.0 / (dt * r5::gamma);
+  rhs_dual(b, L, k, t0, W.y0, W.J0, W, lane, 0);
+  rhs_dual(b, L, k, t0, W.y0, W.dT, W, lane, 1);
+  PBg<double> g;
+  pbg_eval<double>(b, L, k, t0, W.y0[0], g);
+  // dense-block columns (lanes 0..m-1) and chain-head columns (lanes m..m+nch-1)
+  for (int col = lane; col < m + nch; col += WAVE) {
+    int s = (col < m) ? W.dmap[col] : W.chead[col - m];
+    auto ev = [&](int i) { return i == s ? 1.0 : 0.0; };
+    Met<double> mt;
+    metric<double>(b, L, k, g, ev, mt);
+    for (int r = 0; r < m; ++r) {
+      double v = core_row<double>(W.dmap[r], b, L, k, g, mt, ev);
+      if (col < m) W.S[r * m + col] = ((r == col) ? gi : 0.0) - v;  // W_DD = I/(gamma h) - J_DD
+      else W.Jh[(col - m) * m + r] = v;
+    }
+  }
+  // tail coefficients
+  for (int i = lane; i < n; i += WAVE) {
+    int rk = W.rk[i];
+    if (rk >= 0) {
+      double lo, di, up;
+      tail_coef<double>(rk >> 8, rk & 255, L, k, g, lo, di, up);
+      W.tlo[i] = lo; W.tdi[i] = di; W.tup[i] = up;
+    }
+  }
+  __syncthreads();
+  // bottom-up (UL) elimination of each chain: d_l and multipliers
+  for (int c = lane; c < nch; c += WAVE) {
+    int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
+    int i = h + (len - 1) * s;
+    W.td[i] = gi - W.tdi[i];
+    for (int j = len - 2; j >= 0; --j) {
+      int ii = h + j * s;
+      double mul = (-W.tup[ii]) / W.td[ii + s];
+      W.tmul[ii] = mul;
+      W.td[ii] = (gi - W.tdi[ii]) - mul * (-W.tlo[ii + s]);
+    }
+  }
+  __syncthreads();
+  // Schur correction of the parent diagonal column: S[:, p_c] -= Jh_c * g_c * lo_1c
+  for (int r = lane; r < m; r += WAVE) {
+    for (int c = 0; c < nch; ++c) {
+      int h = W.chead[c];
+      double gcoef = W.tlo[h] / W.td[h];
+      W.S[r * m + W.cpar[c]] -= W.Jh[c * m + r] * gcoef;
+    }
+  }
+  W00 = gi - W.J0[0];
+  __syncthreads();
+  // Gauss-Jordan inversion with partial pivoting; row swaps recorded in xd (as ints) for the
+  // final column unscrambling
+  int *perm = (int *)W.xd;
+  for (int p = 0; p < m; ++p) {
+    double best = -1.0;
+    int bi = p;
+    for (int r = p + lane; r < m; r += WAVE) {
+      double v = fabs(W.S[r * m + p]);
+      if (v > best) { best = v; bi = r; }
+    }
+    // wave argmax (first maximum)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      double ob = __shfl_xor(best, o);
+      int oi = __shfl_xor(bi, o);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    if (lane == 0) perm[p] = bi;
+    if (bi != p)
+      for (int j = lane; j < m; j += WAVE) {
+        double t = W.S[p * m + j];
+        W.S[p * m + j] = W.S[bi * m + j];
+        W.S[bi * m + j] = t;
+      }
+    __syncthreads();
+    double piv = W.S[p * m + p];
+    double ip = 1.0 / piv;
+    __syncthreads();
+    for (int j = lane; j < m; j += WAVE) W.S[p * m + j] = (j == p) ? ip : W.S[p * m + j] * ip;
+    __syncthreads();
+    for (int r = lane; r < m; r += WAVE) {
+      if (r == p) continue;
+      double fct = W.S[r * m + p];
+      if (fct == 0.0) continue;
+      for (int j = 0; j < m; ++j) {
+        double pj = W.S[p * m + j];
+        W.S[r * m + j] = (j == p) ? -fct * pj : W.S[r * m + j] - fct * pj;
+      }
+    }
+    __syncthreads();
+  }
+  for (int p = m - 1; p >= 0; --p) {
+    int bi = perm[p];
+    if (bi != p)
+      for (int r = lane; r < m; r += WAVE) {
+        double t = W.S[r * m + p];
+        W.S[r * m + p] = W.S[r * m + bi];
+        W.S[r * m + bi] = t;
+      }
+    __syncthreads();
+  }
+}
+
+// solve W x = v in place (v in LDS, length n)
+__device__ void wsolve(const PLay &L, Ws &W, double W00, double *v, int lane) {
+  const int n = L.n, m = L.m, nch = L.nch;
+  double x0 = v[0] / W00;
+  __syncthreads();
+  for (int i = lane; i < n; i += WAVE) v[i] = (i == 0) ? x0 : v[i] + W.J0[i] * x0;
+  __syncthreads();
+  for (int c = lane; c < nch; c += WAVE) {  // bottom-up sweep
+    int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
+    for (int j = len - 2; j >= 0; --j) {
+      int ii = h + j * s;
+      v[ii] = v[ii] - W.tmul[ii] * v[ii + s];
+    }
+  }
+  __syncthreads();
+  for (int r = lane; r < m; r += WAVE) {
+    double acc = v[W.dmap[r]];
+    for (int c = 0; c < nch; ++c) {
+      int h = W.chead[c];
+      acc += W.Jh[c * m + r] * (v[h] / W.td[h]);
+    }
+    W.xd[r] = acc;
+  }
+  __syncthreads();
+  for (int r = lane; r < m; r += WAVE) {
+    double acc = 0.0;
+    const double *row = W.S + r * m;
+    for (int j = 0; j < m; ++j) acc += row[j] * W.xd[j];
+    v[W.dmap[r]] = acc;
+  }
+  __syncthreads();
+  for (int c = lane; c < nch; c += WAVE) {  // top-down sweep
+    int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
+    double xp = v[W.dmap[W.cpar[c]]];
+    double x = (v[h] + W.tlo[h] * xp) / W.td[h];
+    v[h] = x;
+    for (int j = 1; j < len; ++j) {
+      int ii = h + j * s;
+      x = (v[ii] + W.tlo[ii] * x) / W.td[ii];
+      v[ii] = x;
+    }
+  }
+  __syncthreads();
+}
+
+// adiabatic initial conditions (perturbations.py:680-781), written by lane 0
+__device__ void initial_conditions(const BgView &b, const PLay &L, double tau, double k, double *y) {
+  const int n = L.n, nq = L.nq, iq0 = L.iq0;
+  for (int i = 0; i < n; ++i) y[i] = 0.0;
+  double a = b.a_of_tau(tau);
+  double rnu = exp(b.logrhonu(log(a)));
+  double rhom = b.grhom * b.Omegam / (a * a * a);
+  double rhor = (b.grhog + b.grhor * (b.Neff + b.Nmnu * rnu)) / (a * a * a * a);
+  double rhonu = b.grhor * (b.Neff + b.Nmnu * rnu) / (a * a * a * a);
+  double fracb = b.Omegab / b.Omegam, fracnu = rhonu / rhor;
+  double om = a * rhom / sqrt(rhor);
+  const double ci = -1.0, s2 = 1.0;
+  double kt = k * tau;
+  double deltag = -kt * kt / 3 * (1 - om * tau / 5) * ci * s2;
+  double thetag = -kt * kt * kt / tau / 36 * (1 - 3 * (1 + 5 * fracb - fracnu) / 20 / (1 - fracnu) * om * tau) * ci * s2;
+  double deltar = deltag;
+  double thetar = -kt * kt * kt * kt / tau / 36 / (4 * fracnu + 15) *
+                  (4 * fracnu + 11 + 12 - 3 * (8 * fracnu * fracnu + 50 * fracnu + 275) / 20 / (2 * fracnu + 15) * tau * om) * ci;
+  double shearr = kt * kt / (45 + 12 * fracnu) * (3 * s2 - 1) * (1 + (4 * fracnu - 5) / 4 / (2 * fracnu + 15) * tau * om) * ci;
+  double cs2_Q = b.cs2DE, w_Q = b.w0 + b.wa * (1.0 - a);
+  double deltaq = kt * kt / 4 * (1 + w_Q) * (4 - 3 * cs2_Q) / (4 - 6 * w_Q + 3 * cs2_Q) * ci * s2;
+  double thetaq = kt * kt * kt * kt / tau / 4 * cs2_Q / (4 - 6 * w_Q + 3 * cs2_Q) * ci * s2;
+  if (b.lambda_de) { deltaq = 0.0; thetaq = 0.0; }
+  double eta = ci * (1 - kt * kt / 12 / (15 + 4 * fracnu) *
+                             (5 + 4 * s2 * fracnu - (16 * fracnu * fracnu + 280 * fracnu + 325) / 10 / (2 * fracnu + 15) * tau * om));
+  y[0] = a; y[1] = 0.0; y[2] = eta;
+  y[3] = 0.75 * deltag; y[4] = 0.0; y[5] = 0.75 * deltag; y[6] = thetag; y[7] = deltag; y[8] = thetag;
+  y[L.idxr] = deltar; y[L.idxr + 1] = thetar; y[L.idxr + 2] = shearr * 2.0;
+  for (int q = 0; q < nq; ++q) {
+    double aq = a * b.amnu / L.q[q];
+    double v = 1.0 / sqrt(1.0 + aq * aq);
+    double dl = L.dlfdlq[q];
+    y[iq0 + q] = -0.25 * dl * deltar;
+    y[iq0 + nq + q] = -dl * thetar / v / k / 3.0;
+    y[iq0 + 2 * nq + q] = -0.5 * dl * shearr;
+  }
+  y[n - 2] = deltaq;
+  y[n - 1] = thetaq;
+}
+
+// determine_starting_time (perturbations.py:784-835): two 7-step bisections in log tau
+__device__ double starting_time(const BgView &b, double k) {
+  const double small_k = 0.0004, large_k = 0.07;
+  double tau0 = b.taumin, tau1 = b.tau_of_a(0.1);
+  double tau_k = 1.0 / k;
+  double akthom2 = 2.3048e-9 * (1.0 - b.YHe) * b.Omegab * b.H0 * b.H0;
+  double lt[2];
+  for (int which = 0; which < 2; ++which) {
+    auto fn = [&](double x) {
+      double tt = exp(x);
+      double a = b.a_of_tau(tt);
+      double aH = b.aprimeoa(a);
+      if (which == 0) return (1.0 / aH) / tau_k / large_k - 1.0;
+      double xe = b.xe_of_tau(tt);
+      double opac = xe * akthom2 / (a * a);
+      return (1.0 / opac) / (1.0 / aH) / small_k - 1.0;
+    };
+    double xl = log(tau0), xr = log(tau1);
+    for (int it = 0; it < 7; ++it) {
+      double xm = 0.5 * (xl + xr);
+      if (fn(xm) * fn(xl) > 0) xl = xm; else xr = xm;
+    }
+    lt[which] = 0.5 * (xl + xr);
+  }
+  return exp(fmin(lt[1], lt[0]));
+}
+
+// RSA output projection (approximations.py:84-176) and output conversion
+// (perturbations.py:528-677) of the interpolated state y (LDS), written to dst
+__device__ void emit_output(const BgView &b, const PLay &L, double k, double tau, double *y, double *dst, int full, int lane) {
+  const int n = L.n, nq = L.nq, iq0 = L.iq0;
+  const double a = y[0], loga = log(a);
+  if (b.use_rsa) {
+    double aH = b.aprimeoa(a);
+    int i = spl_index(b.loga, b.N, loga);
+    double cs2 = spl_eval_at(b.loga, b.cs2a, b.S_cs2a_of_loga, i, loga) / a;
+    double xe = spl_eval_at(b.loga, b.xe, b.S_xe_of_loga, i, loga);
+    double xep = spl_deriv_at(b.loga, b.xe, b.S_xe_of_loga, i, loga) * aH;
+    double opac = xe * b.akthom / (a * a);
+    double tc = 1.0 / fmax(opac, 1e-30);
+    double tcp = tc * (2.0 * aH - xep / fmax(xe, 1e-30));
+    bool do_rsa = (tc / fmax(tau, 1e-30) > b.rsa_c) && (tau * k > b.rsa_k);
+    if (do_rsa) {
+      double drhonu = 0.0;
+      for (int q = 0; q < nq; ++q) {
+        double aq = a * b.amnu / L.q[q];
+        drhonu += L.w[q] * y[iq0 + q] * sqrt(1.0 + aq * aq);
+      }
+      double rho_Q = pow(a, -3.0 * (1.0 + b.w0 + b.wa)) * exp(3.0 * (a - 1.0) * b.wa);
+      double eta = y[2], deltab = y[5], thetab = y[6];
+      double dgw = b.grhom * ((b.Omegam - b.Omegab) * y[3] + b.Omegab * deltab) / a + b.grhor * b.Nmnu * drhonu / (a * a) +
+                   b.grhom * b.OmegaDE * y[n - 2] * rho_Q * a * a;
+      double hp = (2.0 * k * k * eta + dgw) / fmax(aH, 1e-30);
+      double sk2 = fmax(k * k, 1e-30), stc = fmax(tc, 1e-30);
+      double dg = 4.0 / sk2 * (aH * hp - sk2 * eta) + 4.0 / (sk2 * stc) * (thetab + 0.5 * hp);
+      double tg = -0.5 * hp + 3.0 / (sk2 * stc) * (-tcp / stc * (thetab + 0.5 * hp) + (-aH * thetab + cs2 * sk2 * deltab + sk2 * eta));
+      double dr = 4.0 / sk2 * (aH * hp - sk2 * eta), tr = -0.5 * hp;
+      __syncthreads();
+      for (int j = lane; j < n; j += WAVE) {
+        double v = y[j];
+        if (j >= L.idxg && j <= L.idxg + L.lmaxg) v = (j == L.idxg) ? dg : (j == L.idxg + 1 ? tg : 0.0);
+        else if (j >= L.idxgp && j <= L.idxgp + L.lmaxgp) v = 0.0;
+        else if (j >= L.idxr && j <= L.idxr + L.lmaxr) v = (j == L.idxr) ? dr : (j == L.idxr + 1 ? tr : 0.0);
+        y[j] = v;
+      }
+      __syncthreads();
+    }
+  }
+  if (full) {
+    for (int j = lane; j < n; j += WAVE) dst[j] = y[j];
+    return;
+  }
+  // 20 output fields
+  const double Omegac = b.Omegam - b.Omegab;
+  double eta = y[2], deltac = y[3], thetac = y[4], deltab = y[5], thetab = y[6], deltag = y[7], thetag = y[8];
+  double deltar = y[L.idxr], thetar = y[L.idxr + 1];
+  double rhonu = exp(b.logrhonu(loga)), pnu = exp(b.logpnu(loga));
+  double drhonu = 0.0, fnu = 0.0;
+  for (int q = 0; q < nq; ++q) {
+    double aq = a * b.amnu / L.q[q];
+    double v = 1.0 / sqrt(1.0 + aq * aq);
+    drhonu += L.w[q] * y[iq0 + q] / v;
+    fnu += L.w[q] * y[iq0 + nq + q];
+  }
+  double deltaq = y[n - 2], thetaq = y[n - 1];
+  double w_Q = b.w0 + b.wa * (1.0 - a);
+  double rho_Q = pow(a, -3.0 * (1.0 + b.w0 + b.wa)) * exp(3.0 * (a - 1.0) * b.wa);
+  double rpptq = (1 + w_Q) * rho_Q * b.grhom * b.OmegaDE * thetaq * a * a;
+  double grho = b.grhom * b.Omegam / a + (b.grhog + b.grhor * (b.Neff + b.Nmnu * rhonu)) / (a * a) + b.grhom * b.OmegaDE * rho_Q * a * a +
+                b.grhom * b.Omegak;
+  double aH = sqrt(grho / 3.0);
+  double dgrho = b.grhom * (Omegac * deltac + b.Omegab * deltab) / a +
+                 (b.grhog * deltag + b.grhor * (b.Neff * deltar + b.Nmnu * drhonu)) / (a * a) + b.grhom * b.OmegaDE * deltaq * rho_Q * a * a;
+  double dgtheta = b.grhom * (Omegac * thetac + b.Omegab * thetab) / a + 4.0 / 3.0 * (b.grhog * thetag + b.Neff * b.grhor * thetar) / (a * a) +
+                   b.Nmnu * b.grhor * k * fnu / (a * a) + rpptq;
+  double hprime = (2.0 * k * k * eta + dgrho) / aH;
+  double etaprime = 0.5 * dgtheta / (k * k);
+  double alpha = (hprime + 6. * etaprime) / 2. / (k * k);
+  double mden = b.grhom * b.Omegam / a + (b.grhor * b.Nmnu * rhonu) / (a * a);
+  double deltam = (b.grhom * (Omegac * deltac + b.Omegab * deltab) / a + (b.grhor * b.Nmnu * drhonu) / (a * a)) / mden;
+  double thetam = (b.grhom * (Omegac * thetac + b.Omegab * thetab) / a + b.Nmnu * b.grhor * k * fnu / (a * a)) /
+                  (3.0 * (b.grhom * b.Omegam / a + b.grhor * b.Nmnu * rhonu / (a * a)));
+  double deltabc = (b.grhom * (Omegac * deltac + b.Omegab * deltab) / a) / (b.grhom * b.Omegam / a);
+  double thetabc = (b.grhom * (Omegac * thetac + b.Omegab * thetab) / a) / (3.0 * (b.grhom * b.Omegam / a) / (a * a));
+  thetam += alpha * k * k;
+  thetabc += alpha * k * k;
+  double deltanu = drhonu / rhonu, thetanu = k * fnu / (rhonu + pnu);
+  if (lane < 20) {
+    double v;
+    switch (lane) {
+      case 0: v = eta; break;
+      case 1: v = etaprime; break;
+      case 2: v = hprime; break;
+      case 3: v = alpha; break;
+      case 4: v = deltam; break;
+      case 5: v = thetam / aH; break;
+      case 6: v = deltabc; break;
+      case 7: v = thetabc / aH; break;
+      case 8: v = deltac; break;
+      case 9: v = thetac / aH; break;
+      case 10: v = deltab; break;
+      case 11: v = thetab / aH; break;
+      case 12: v = deltag; break;
+      case 13: v = thetag / aH; break;
+      case 14: v = deltar; break;
+      case 15: v = thetar / aH; break;
+      case 16: v = deltanu; break;
+      case 17: v = thetanu / aH; break;
+      case 18: v = deltaq; break;
+      default: v = thetaq / aH; break;
+    }
+    dst[lane] = v;
+  }
+}
+
+// diffrax PIDController factor (error order 5, safety 0.9)
+struct Pid {
+  double prev_inv, prev_prev_inv;
+  __device__ double adapt(double en, bool keep, const DebSolverCfg &s) {
+    const double order = 5.0;
+    double c1 = (s.icoeff + s.pcoeff + s.dcoeff) / order;
+    double c2 = -(s.pcoeff + 2.0 * s.dcoeff) / order;
+    double c3 = s.dcoeff / order;
+    double inv = 1.0 / en;
+    double f1 = (c1 == 0.0) ? 1.0 : pow(inv, c1);
+    double f2 = (c2 == 0.0) ? 1.0 : pow(prev_inv, c2);
+    double f3 = (c3 == 0.0) ? 1.0 : pow(prev_prev_inv, c3);
+    double factor = fmin(fmax(0.9 * f1 * f2 * f3, keep ? 1.0 : s.factormin), s.factormax);
+    if (inv == 0.0 || isinf(inv)) inv = 1.0;
+    if (keep) { prev_prev_inv = prev_inv; prev_inv = inv; }
+    return factor;
+  }
+};
+
+__global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
+                                                       const double *__restrict__ unit_k, int n_units, const double *__restrict__ tau_out,
+                                                       int nout, const double *__restrict__ nu_qw, DebProblem prob, DebSolverCfg cfg,
+                                                       long lds_bytes, double *__restrict__ y_out,
+                                                       int *__restrict__ status, int *__restrict__ n_acc, int *__restrict__ n_rej) {
+  const int unit = blockIdx.x;
+  if (unit >= n_units) return;
+  const int lane = threadIdx.x;
+  const int cosmo = unit_cosmo[unit];
+  const double k = unit_k[unit];
+  BgView b;
+  b.init(tables + (long)cosmo * tstride);
+  PLay L;
+  L.lmaxg = prob.lmaxg; L.lmaxgp = prob.lmaxgp; L.lmaxr = prob.lmaxr; L.lmaxnu = prob.lmaxnu; L.nq = prob.nqmax;
+  L.n = 7 + (L.lmaxg + 1) + (L.lmaxgp + 1) + (L.lmaxr + 1) + L.nq * (L.lmaxnu + 1) + 2;
+  L.idxg = 7; L.idxgp = 7 + (L.lmaxg + 1); L.idxr = 9 + L.lmaxg + L.lmaxgp; L.iq0 = 10 + L.lmaxg + L.lmaxgp + L.lmaxr;
+  L.m = 17 + 3 * L.nq;
+  L.nch = 3 + L.nq;
+  const int n = L.n, m = L.m, nch = L.nch;
+
+  extern __shared__ double smem[];
+  Ws W;
+  double *p = smem;
+  // carve-up must match deb_perturb_lds_bytes(): 19 n + m^2 + nch m + m + 3 NQMAX doubles, then
+  // m + n + 4 nch ints
+  W.y0 = p; p += n; W.y1 = p; p += n; W.u = p; p += n; W.f = p; p += n; W.J0 = p; p += n; W.dT = p; p += n;
+  W.K = p; p += 8 * n; W.tlo = p; p += n; W.tdi = p; p += n; W.tup = p; p += n; W.tmul = p; p += n; W.td = p; p += n;
+  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m;
+  double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
+  for (int q = lane; q < L.nq; q += WAVE) {
+    sq[q] = nu_qw[q];
+    sw[q] = nu_qw[NQMAX + q];
+    sdl[q] = -nu_qw[q] / (1.0 + exp(-nu_qw[q]));
+  }
+  L.q = sq; L.w = sw; L.dlfdlq = sdl;
+  int *ip = (int *)p;
+  W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
+  W.cpar = ip; ip += nch;
+  if ((long)((char *)ip - (char *)smem) > lds_bytes) {  // launch/carve-up mismatch: refuse to run
+    if (lane == 0) { status[unit] = 3; n_acc[unit] = 0; n_rej[unit] = 0; }
+    return;
+  }
+
+  setup_maps(L, W, lane);
+
+  // output times of this cosmology
+  const double *tout = tau_out + (long)cosmo * nout;
+  double tmin = tout[0], t1 = tout[0];
+  for (int j = 1; j < nout; ++j) { tmin = fmin(tmin, tout[j]); t1 = fmax(t1, tout[j]); }
+  double t0 = 0.99 * fmin(tmin, starting_time(b, k));
+  if (lane == 0) initial_conditions(b, L, t0, k, W.y0);
+  __syncthreads();
+
+  const int wout = prob.return_full ? n : 20;
+  double *yo = y_out + (long)unit * nout * wout;
+  double tprev = t0;
+  double tnext = fmin(t0 + fmin(t0 / 4.0, 0.5 * (t1 - t0)), t1);
+  Pid pid;
+  pid.prev_inv = 1.0;
+  pid.prev_prev_inv = 1.0;
+  int isave = 0, steps = 0, na = 0, nr = 0, st = 0;
+  double *K = W.K;
+  double *u = W.u, *f = W.f, *y0 = W.y0, *dT = W.dT;
+  while (tprev < t1) {
+    if (steps >= cfg.max_steps) { st = 1; break; }
+    const double dt = tnext - tprev;
+    double W00;
+    factorise(b, L, k, tprev, dt, W, lane, W00);
+    using namespace r5;
+    const double dtd[5] = {dt * d1, dt * d2, dt * d3, dt * d4, dt * d5};
+    const double cs[5] = {0.0, c2, c3, c4, c5};
+    // stages 1-5 (with the df/dtau term), 6-8 at t0 + dt
+    for (int s = 0; s < 8; ++s) {
+      double *ks = K + s * n;
+      // stage input u
+      for (int i = lane; i < n; i += WAVE) {
+        double v;
+        switch (s) {
+          case 0: v = y0[i]; break;
+          case 1: v = y0[i] + a21 * K[i]; break;
+          case 2: v = y0[i] + a31 * K[i] + a32 * K[n + i]; break;
+          case 3: v = y0[i] + a41 * K[i] + a42 * K[n + i] + a43 * K[2 * n + i]; break;
+          case 4: v = y0[i] + a51 * K[i] + a52 * K[n + i] + a53 * K[2 * n + i] + a54 * K[3 * n + i]; break;
+          case 5: v = y0[i] + a61 * K[i] + a62 * K[n + i] + a63 * K[2 * n + i] + a64 * K[3 * n + i] + a65 * K[4 * n + i]; break;
+          case 6: v = u[i] + K[5 * n + i]; break;
+          default: v = u[i] + K[6 * n + i]; break;
+        }
+        u[i] = v;
+      }
+      __syncthreads();
+      double ts = (s < 5) ? tprev + cs[s] * dt : tprev + dt;
+      rhs_eval(b, L, k, ts, u, f, W, lane);
+      __syncthreads();
+      for (int i = lane; i < n; i += WAVE) {
+        double r = f[i];
+        switch (s) {
+          case 0: r = r + dtd[0] * dT[i]; break;
+          case 1: r = r + dtd[1] * dT[i] + (C21 / dt) * K[i]; break;
+          case 2: r = r + dtd[2] * dT[i] + ((C31 / dt) * K[i] + (C32 / dt) * K[n + i]); break;
+          case 3: r = r + dtd[3] * dT[i] + ((C41 / dt) * K[i] + (C42 / dt) * K[n + i] + (C43 / dt) * K[2 * n + i]); break;
+          case 4:
+            r = r + dtd[4] * dT[i] + ((C51 / dt) * K[i] + (C52 / dt) * K[n + i] + (C53 / dt) * K[2 * n + i] + (C54 / dt) * K[3 * n + i]);
+            break;
+          case 5:
+            r = r + ((C61 / dt) * K[i] + (C62 / dt) * K[n + i] + (C63 / dt) * K[2 * n + i] + (C64 / dt) * K[3 * n + i] +
+                     (C65 / dt) * K[4 * n + i]);
+            break;
+          case 6:
+            r = r + ((C71 / dt) * K[i] + (C72 / dt) * K[n + i] + (C73 / dt) * K[2 * n + i] + (C74 / dt) * K[3 * n + i] +
+                     (C75 / dt) * K[4 * n + i] + (C76 / dt) * K[5 * n + i]);
+            break;
+          default:
+            r = r + ((C81 / dt) * K[i] + (C82 / dt) * K[n + i] + (C83 / dt) * K[2 * n + i] + (C84 / dt) * K[3 * n + i] +
+                     (C85 / dt) * K[4 * n + i] + (C86 / dt) * K[5 * n + i] + (C87 / dt) * K[6 * n + i]);
+            break;
+        }
+        ks[i] = r;
+      }
+      __syncthreads();
+      wsolve(L, W, W00, ks, lane);
+    }
+    // candidate y1 = u + k8 and the filtered error norm (perturbations.py:855-859)
+    int nanflag = 0;
+    for (int i = lane; i < n; i += WAVE) {
+      double v = u[i] + K[7 * n + i];
+      W.y1[i] = v;
+      nanflag |= (v != v);
+    }
+    nanflag = wave_or(nanflag);
+    __syncthreads();
+    double acc = 0.0;
+    {
+      const int idx[6] = {0, 2, 3, 5, 6, 7};
+      const double wts[6] = {1.0, k * k, 1.0, 1.0, 1.0 / (k * k), 1.0};
+      for (int j = 0; j < 6; ++j) {
+        int i = idx[j];
+        double yy = fmax(fabs(y0[i]), fabs(nanflag ? y0[i] : W.y1[i]));
+        double e = K[7 * n + i] / (cfg.atol + yy * cfg.rtol) * wts[j];
+        acc += e * e;
+      }
+    }
+    double en = sqrt(acc / 6.0);
+    if (!isfinite(en)) { st = 2; break; }
+    bool keep = en < 1.0;
+    double factor = pid.adapt(en, keep, cfg);
+    if (keep) {
+      while (isave < nout && tout[isave] <= tnext) {
+        double th = (tout[isave] - tprev) / (tnext - tprev);
+        for (int i = lane; i < n; i += WAVE) u[i] = y0[i] + th * (W.y1[i] - y0[i]);
+        __syncthreads();
+        emit_output(b, L, k, tout[isave], u, yo + (long)isave * wout, prob.return_full, lane);
+        __syncthreads();
+        ++isave;
+      }
+      for (int i = lane; i < n; i += WAVE) y0[i] = W.y1[i];
+      ++na;
+    } else {
+      ++nr;
+    }
+    __syncthreads();
+    double nt0 = keep ? tnext : tprev;
+    tprev = fmin(nt0, t1);
+    double tn = nt0 + dt * factor;
+    tnext = (tn > t1 - 1e-10) ? t1 : tn;
+    ++steps;
+  }
+  if (st != 0) {
+    for (int j = isave; j < nout; ++j)
+      for (int i = lane; i < wout; i += WAVE) yo[(long)j * wout + i] = __builtin_nan("");
+  }
+  if (lane == 0) {
+    status[unit] = st;
+    n_acc[unit] = na;
+    n_rej[unit] = nr;
+  }
+}
+
+// dy/dtau at saved states (compute_time_derivatives, perturbations.py:1339-1406): one wave per
+// k-mode, looping over the output times
+__global__ __launch_bounds__(WAVE) void time_deriv_kernel(const double *__restrict__ tables, const double *__restrict__ yin,
+                                                          const double *__restrict__ tau, int n_tau, const double *__restrict__ kk,
+                                                          const double *__restrict__ nu_qw, DebProblem prob, double *__restrict__ dy) {
+  const int ik = blockIdx.x, lane = threadIdx.x;
+  const double k = kk[ik];
+  BgView b;
+  b.init(tables);
+  PLay L;
+  L.lmaxg = prob.lmaxg; L.lmaxgp = prob.lmaxgp; L.lmaxr = prob.lmaxr; L.lmaxnu = prob.lmaxnu; L.nq = prob.nqmax;
+  L.n = 7 + (L.lmaxg + 1) + (L.lmaxgp + 1) + (L.lmaxr + 1) + L.nq * (L.lmaxnu + 1) + 2;
+  L.idxg = 7; L.idxgp = 7 + (L.lmaxg + 1); L.idxr = 9 + L.lmaxg + L.lmaxgp; L.iq0 = 10 + L.lmaxg + L.lmaxgp + L.lmaxr;
+  L.m = 17 + 3 * L.nq;
+  L.nch = 3 + L.nq;
+  const int n = L.n, m = L.m, nch = L.nch;
+  extern __shared__ double smem[];
+  Ws W;
+  double *p = smem;
+  W.u = p; p += n; W.f = p; p += n;
+  double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
+  for (int q = lane; q < L.nq; q += WAVE) {
+    sq[q] = nu_qw[q];
+    sw[q] = nu_qw[NQMAX + q];
+    sdl[q] = -nu_qw[q] / (1.0 + exp(-nu_qw[q]));
+  }
+  L.q = sq; L.w = sw; L.dlfdlq = sdl;
+  int *ip = (int *)p;
+  W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
+  W.cpar = ip; ip += nch;
+  setup_maps(L, W, lane);
+  for (int it = 0; it < n_tau; ++it) {
+    const double *src = yin + ((long)ik * n_tau + it) * n;
+    for (int i = lane; i < n; i += WAVE) W.u[i] = src[i];
+    __syncthreads();
+    rhs_eval(b, L, k, tau[it], W.u, W.f, W, lane);
+    __syncthreads();
+    double *dst = dy + ((long)ik * n_tau + it) * n;
+    for (int i = lane; i < n; i += WAVE) dst[i] = W.f[i];
+    __syncthreads();
+  }
+}
+
+// tau_out[c, j] = tau_of_a(aexp_out[j]) for every cosmology
+__global__ void tau_out_kernel(const double *__restrict__ tables, long tstride, int n_cosmo, const double *__restrict__ aexp_out,
+                               int nout, double *__restrict__ tau_out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_cosmo * nout) return;
+  int c = i / nout, j = i - c * nout;
+  BgView b;
+  b.init(tables + (long)c * tstride);
+  tau_out[i] = b.tau_of_a(aexp_out[j]);
+}
+
+}  // namespace
+}  // namespace deb
+
+extern "C" long deb_perturb_lds_bytes(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
+  long n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + (long)nqmax * (lmaxnu + 1) + 2;
+  long m = 17 + 3 * (long)nqmax, nch = 3 + nqmax;
+  long dbl = n * 19 + m * m + nch * m + m + 3 * deb::NQMAX;
+  long ints = m + n + 4 * nch;
+  return dbl * 8 + ints * 4;
+}
+
+extern "C" int deb_launch_nu_bins(int nqmax, double *d_qw, hipStream_t stream) {
+  hipLaunchKernelGGL(deb::nu_bins_kernel, dim3(1), dim3(64), 0, stream, nqmax, d_qw);
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_time_deriv(const double *d_tables, const double *d_y, const double *d_tau, int n_tau, const double *d_k,
+                                     int n_k, const double *d_nu_qw, const DebProblem *prob, double *d_dy, hipStream_t stream) {
+  long n = 7 + (prob->lmaxg + 1) + (prob->lmaxgp + 1) + (prob->lmaxr + 1) + (long)prob->nqmax * (prob->lmaxnu + 1) + 2;
+  long m = 17 + 3 * (long)prob->nqmax, nch = 3 + prob->nqmax;
+  long lds = (2 * n + 3 * deb::NQMAX) * 8 + (m + n + 4 * nch) * 4;
+  hipLaunchKernelGGL(deb::time_deriv_kernel, dim3(n_k), dim3(deb::WAVE), (size_t)lds, stream, d_tables, d_y, d_tau, n_tau, d_k, d_nu_qw,
+                     *prob, d_dy);
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_tau_out(const double *d_tables, int num_thermo, int n_cosmo, const double *d_aexp_out, int nout,
+                                  double *d_tau_out, hipStream_t stream) {
+  long stride = deb_table_doubles(num_thermo);
+  int total = n_cosmo * nout;
+  hipLaunchKernelGGL(deb::tau_out_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, d_tables, stride, n_cosmo, d_aexp_out, nout,
+                     d_tau_out);
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_perturb(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
+                                  const double *d_tau_out, int nout, const double *d_nu_qw, const DebProblem *prob,
+                                  const DebSolverCfg *cfg, double *d_y_out,
+                                  int *d_status, int *d_nacc, int *d_nrej, hipStream_t stream) {
+  long stride = deb_table_doubles(num_thermo);
+  long lds = deb_perturb_lds_bytes(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
+  if (lds > 160 * 1024) return -2;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)deb::perturb_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return (int)e;
+  }
+  hipLaunchKernelGGL(deb::perturb_kernel, dim3(n_units), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k,
+                     n_units, d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_y_out, d_status, d_nacc, d_nrej);
+  return (int)hipGetLastError();
+}

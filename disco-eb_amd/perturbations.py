@@ -1,0 +1,167 @@
+"""evolve_perturbations, get_power and compute_time_derivatives on the GPU.
+
+Mirrors src/discoeb/perturbations.py:1124-1226 (evolve_perturbations), :1446-1468 (get_power)
+and :1339-1406 (compute_time_derivatives): same keyword arguments, same output layout
+(num_k, nout, 20 fields or nvar), same param-dict annotations and error behaviour.  Each k-mode
+is integrated independently (the reference's evolve_one_mode) by one wavefront of the HIP
+kernel behind deb_evolve.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _geomspace(start, stop, num, endpoint=True):
+    # jax.numpy.geomspace arithmetic: 10**(log10(start)*(1-s) + log10(stop)*s)
+    l0, l1 = np.log10(start), np.log10(stop)
+    div = (num - 1) if endpoint else num
+    if num == 1:
+        return np.array([start], dtype=np.float64)
+    s = np.arange(num, dtype=np.float64) / div
+    lin = l0 * (1.0 - s) + l1 * s
+    if endpoint:
+        lin[-1] = l1
+    return 10.0 ** lin
+
+
+def _linspace(start, stop, num, endpoint=True):
+    div = (num - 1) if endpoint else num
+    if num == 1:
+        return np.array([start], dtype=np.float64)
+    s = np.arange(num, dtype=np.float64) / div
+    out = start * (1.0 - s) + stop * s
+    if endpoint:
+        out[-1] = stop
+    return out
+
+
+def k_grid(param, kmin, kmax, num_k, k_sampling_method="camb"):
+    """k-sampling of perturbations.py:1177-1199 ('log', 'linear' or the CAMB-like hybrid)."""
+    if k_sampling_method == "log":
+        return _geomspace(kmin, kmax, num_k)
+    if k_sampling_method == "linear":
+        return _linspace(kmin, kmax, num_k)
+    if k_sampling_method == "camb":
+        if "tau_maxvis" not in param:
+            raise ValueError("param dictionary must contain 'tau_maxvis' for 'camb' k-sampling.")
+        taurst = float(param["tau_maxvis"])
+        q1, q2 = 8.0 / taurst, 30.0 / taurst
+        n1, n2 = int(num_k * 0.2), int(num_k * 0.4)
+        n3 = num_k - n1 - n2
+        return np.concatenate([_geomspace(kmin, q1, n1, endpoint=False), _linspace(q1, q2, n2, endpoint=False),
+                               _geomspace(q2, kmax, n3)])
+    raise ValueError(f"Unknown k_sampling_method: {k_sampling_method}")
+
+
+def nvar(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax):
+    return 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + nqmax * (lmaxnu + 1) + 2
+
+
+def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3,
+                 rtol=1e-4, atol=1e-4, pcoeff=0.25, icoeff=0.80, dcoeff=0.0, factormax=20.0, factormin=0.3, max_steps=2048,
+                 return_full=False, out=None):
+    """Low-level launch: independent adaptive solves of (cosmology, k) units.
+
+    tables: (n_cosmo, size) device tensor from background_tables; unit_cosmo (int32) and unit_k
+    (float64) device tensors of length n_units; tau_out (n_cosmo, nout) device tensor.
+    Returns (y, status, n_accept, n_reject) device tensors; asynchronous on the current stream."""
+    L = _lib.lib()
+    dev = tables.device
+    n_units = int(unit_k.numel())
+    nout = int(tau_out.shape[-1])
+    nv = nvar(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax)
+    w = nv if return_full else 20
+    y = out if out is not None else torch.empty((n_units, nout, w), dtype=torch.float64, device=dev)
+    status = torch.empty(n_units, dtype=torch.int32, device=dev)
+    nacc = torch.empty(n_units, dtype=torch.int32, device=dev)
+    nrej = torch.empty(n_units, dtype=torch.int32, device=dev)
+    work = torch.empty(L.deb_evolve_workspace_doubles(), dtype=torch.float64, device=dev)
+    prob = _lib.DebProblem(int(lmaxg), int(lmaxgp), int(lmaxr), int(lmaxnu), int(nqmax), int(bool(return_full)))
+    cfg = _lib.DebSolverCfg(float(rtol), float(atol), float(pcoeff), float(icoeff), float(dcoeff), float(factormax),
+                            float(factormin), int(max_steps))
+    uc = unit_cosmo.to(device=dev, dtype=torch.int32).contiguous()
+    uk = unit_k.to(device=dev, dtype=torch.float64).contiguous()
+    to = tau_out.to(device=dev, dtype=torch.float64).contiguous()
+    _lib.check(L.deb_evolve(_lib.ptr(tables), int(num_thermo), _lib.ptr(uc), _lib.ptr(uk), n_units, _lib.ptr(to), nout,
+                            ctypes.byref(prob), ctypes.byref(cfg), _lib.ptr(y), _lib.ptr(status), _lib.ptr(nacc),
+                            _lib.ptr(nrej), _lib.ptr(work), _lib.stream()))
+    return y, status, nacc, nrej
+
+
+def _raise_on_status(status):
+    st = status.cpu()
+    if (st == 1).any():
+        raise RuntimeError("The maximum number of solver steps was reached. Try increasing `max_steps`.")
+    if (st == 2).any():
+        raise RuntimeError("Non-finite error estimate in the perturbation solve.")
+    if (st == 3).any():
+        raise RuntimeError("internal error: LDS carve-up exceeds the launch size")
+
+
+def evolve_perturbations(*, param, aexp_out, kmin: float, kmax: float, num_k: int, lmaxg: int = 11, lmaxgp: int = 11,
+                         lmaxr: int = 11, lmaxnu: int = 8, nqmax: int = 3, rtol: float = 1e-4, atol: float = 1e-4,
+                         pcoeff: float = 0.25, icoeff: float = 0.80, dcoeff: float = 0.0, factormax: float = 20.0,
+                         factormin: float = 0.3, max_steps: int = 2048, return_full: bool = False,
+                         k_sampling_method: str = "camb"):
+    """Synchronous-gauge perturbations on a k-grid (perturbations.py:1124-1226).
+
+    Returns (y[num_k, nout, 20 or nvar], kmodes, param) as float64 device tensors; annotates
+    param with lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax, nout and tau_out as the reference does."""
+    kmodes = k_grid(param, kmin, kmax, num_k, k_sampling_method)
+    tables = param["_deb_tables"]
+    dev = tables.device
+    aexp = torch.as_tensor(np.atleast_1d(np.asarray(aexp_out, dtype=np.float64)) if not torch.is_tensor(aexp_out) else aexp_out,
+                           dtype=torch.float64, device=dev).reshape(-1)
+    tau_out = param["tau_of_a_spline"].evaluate(aexp)
+    cosmo = int(param.get("_deb_cosmo", 0))
+    n_cosmo = tables.shape[0] if tables.ndim == 2 else 1
+    tau_all = torch.zeros((n_cosmo, tau_out.numel()), dtype=torch.float64, device=dev)
+    tau_all[cosmo] = tau_out
+    kt = torch.as_tensor(kmodes, dtype=torch.float64, device=dev)
+    uc = torch.full((kt.numel(),), cosmo, dtype=torch.int32, device=dev)
+    y, status, nacc, nrej = evolve_units(tables, param["_deb_num_thermo"], uc, kt, tau_all, lmaxg=lmaxg, lmaxgp=lmaxgp,
+                                         lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax, rtol=rtol, atol=atol, pcoeff=pcoeff,
+                                         icoeff=icoeff, dcoeff=dcoeff, factormax=factormax, factormin=factormin,
+                                         max_steps=max_steps, return_full=return_full)
+    _raise_on_status(status)
+    param["lmaxg"], param["lmaxgp"], param["lmaxr"], param["lmaxnu"], param["nqmax"] = lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax
+    param["nout"] = int(aexp.numel())
+    param["tau_out"] = tau_out
+    param["_deb_steps"] = (nacc, nrej)
+    return y, kt, param
+
+
+def get_power(*, k, y, idx: int, param):
+    """P(k) = 2 pi^2 A_s (k/k_p)^(n_s-1) k^-3 y[..., idx]^2 (perturbations.py:1446-1468), on the GPU."""
+    yt = torch.as_tensor(y, dtype=torch.float64).contiguous()
+    dev = yt.device
+    kt = torch.as_tensor(k, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+    nf = yt.shape[-1]
+    rows = yt.numel() // nf
+    krep = max(rows // max(kt.numel(), 1), 1)
+    out = torch.empty(rows, dtype=torch.float64, device=dev)
+    _lib.check(_lib.lib().deb_power(_lib.ptr(kt), krep, _lib.ptr(yt), rows, nf, int(idx), float(param["A_s"]),
+                                    float(param["n_s"]), float(param.get("k_p", 0.05)), _lib.ptr(out), _lib.stream()))
+    return out.reshape(yt.shape[:-1])
+
+
+def compute_time_derivatives(yout, tau, kmodes, param, lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax):
+    """dy/dtau re-evaluated at saved states (perturbations.py:1339-1406)."""
+    L = _lib.lib()
+    tables = param["_deb_tables"]
+    dev = tables.device
+    y = torch.as_tensor(yout, dtype=torch.float64, device=dev).contiguous()
+    tt = torch.as_tensor(tau, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+    kt = torch.as_tensor(kmodes, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+    if y.shape != (kt.numel(), tt.numel(), nvar(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax)):
+        raise ValueError("yout must have shape (n_kmodes, n_times, nvar)")
+    dy = torch.empty_like(y)
+    work = torch.empty(L.deb_evolve_workspace_doubles(), dtype=torch.float64, device=dev)
+    prob = _lib.DebProblem(int(lmaxg), int(lmaxgp), int(lmaxr), int(lmaxnu), int(nqmax), 1)
+    _lib.check(L.deb_time_derivatives(_lib.ptr(tables), int(param["_deb_num_thermo"]), int(param.get("_deb_cosmo", 0)),
+                                      _lib.ptr(y), _lib.ptr(tt), tt.numel(), _lib.ptr(kt), kt.numel(), ctypes.byref(prob),
+                                      _lib.ptr(dy), _lib.ptr(work), _lib.stream()))
+    return dy

@@ -1,0 +1,84 @@
+/* discoeb_amd.h -- C ABI of the MI355X (gfx950) linear-perturbation solver.
+ *
+ * Every pointer argument named d_* is a device pointer (HBM) owned by the caller; the library
+ * never allocates, frees or retains caller memory.  Work is enqueued on the HIP stream passed
+ * as `stream` (a hipStream_t; NULL = default stream) and is asynchronous unless stated.
+ * Return codes: 0 success, < 0 invalid argument (see deb_last_error()), > 0 hipError_t.
+ *
+ * Each entry point names the function of the reference (lawrenceberry/DISCO-EB, paths relative
+ * to its repository root) whose role it takes over.
+ */
+#ifndef DISCOEB_AMD_H
+#define DISCOEB_AMD_H
+
+#include "../disco-eb_amd/csrc/deb_layout.h"
+#include "../disco-eb_amd/csrc/deb_perturb.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library version string. */
+const char *deb_version(void);
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char *deb_last_error(void);
+
+/* Doubles per cosmology in the background table block (deb_layout.h) for num_thermo knots,
+ * and the offset of one array slot inside a block. */
+long deb_tables_size(int num_thermo);
+long deb_table_offset(int slot, int num_thermo);
+
+/* Background evolution and RECFAST thermal history for n_cosmo cosmologies.
+ * d_params: n_cosmo x DEB_NPARAM doubles (deb_layout.h order).  d_tables: n_cosmo x
+ * deb_tables_size(num_thermo) doubles, written.
+ * Replaces evolve_background(param=..., thermo_module='RECFAST', num_thermo=...)
+ * (src/discoeb/background.py:191-345, thermodynamics_recfast.py:456-492). */
+int deb_background(const double *d_params, int n_cosmo, int num_thermo, double *d_tables, void *stream);
+
+/* Spline queries on the background tables of one cosmology: kind 0 value, 1 first derivative,
+ * 2 second derivative, 3 integral from the first knot (opacity spline only).
+ * Replaces spline_interpolation.evaluate/derivative/derivative2/integral
+ * (src/discoeb/spline_interpolation.py:147-276) on the param[...] splines. */
+enum {
+  DEB_SPL_TAU_OF_A, DEB_SPL_A_OF_TAU, DEB_SPL_XE_OF_TAU, DEB_SPL_CS2A_OF_TAU, DEB_SPL_TEMPBA_OF_TAU, DEB_SPL_XE_OF_LOGA,
+  DEB_SPL_CS2A_OF_LOGA, DEB_SPL_LOGRHONU_OF_LOGA, DEB_SPL_LOGPNU_OF_LOGA, DEB_SPL_LOGPPNU_OF_LOGA, DEB_SPL_OPAC, DEB_SPL_COUNT
+};
+int deb_spline_eval(const double *d_tables, int num_thermo, int cosmo, int spline_id, int kind, const double *d_x, int nx,
+                    double *d_out, void *stream);
+
+/* Doubles of device workspace deb_evolve needs (neutrino momentum bins). */
+long deb_evolve_workspace_doubles(void);
+
+/* Adaptive Rodas5 evolution of n_units independent (cosmology, k) modes.
+ * d_unit_cosmo[u], d_unit_k[u]: cosmology index and wavenumber [1/Mpc] of unit u.
+ * d_tau_out: n_cosmo x nout conformal output times (ascending per cosmology).
+ * d_y_out: n_units x nout x (return_full ? nvar : 20) doubles, written.
+ * d_status[u]: 0 ok, 1 max_steps exceeded, 2 non-finite error norm (outputs NaN from there).
+ * d_n_accept / d_n_reject: accepted / rejected step counts per unit.
+ * d_work: deb_evolve_workspace_doubles() doubles.
+ * Replaces evolve_perturbations (src/discoeb/perturbations.py:1124-1226) with
+ * evolve_one_mode (:882-949) per unit. */
+int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
+               const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
+               int *d_n_accept, int *d_n_reject, double *d_work, void *stream);
+
+/* Number of state variables for the given truncations (perturbations.py:893). */
+int deb_nvar(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax);
+
+/* P(k) = 2 pi^2 A_s (k/k_p)^(n_s-1) k^-3 y[idx]^2 for n wavenumbers; y row stride `ystride`
+ * doubles, k of row r is d_k[r / k_repeat].
+ * Replaces get_power (src/discoeb/perturbations.py:1446-1468). */
+int deb_power(const double *d_k, int k_repeat, const double *d_y, long n, int ystride, int idx, double A_s, double n_s, double k_p,
+              double *d_pk, void *stream);
+
+/* dy/dtau re-evaluated at saved states: d_y is n_k x n_tau x nvar, d_tau (n_tau), d_k (n_k);
+ * d_dy has the same shape.  Replaces compute_time_derivatives
+ * (src/discoeb/perturbations.py:1339-1406). */
+int deb_time_derivatives(const double *d_tables, int num_thermo, int cosmo, const double *d_y, const double *d_tau, int n_tau,
+                         const double *d_k, int n_k, const DebProblem *prob, double *d_dy, double *d_work, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

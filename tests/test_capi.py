@@ -1,0 +1,61 @@
+"""C ABI checks that need no GPU: the library loads, exports what include/discoeb_amd.h
+declares, and its host-side layout helpers agree with the Python mirror."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT, load_package
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    return load_package()
+
+
+def test_library_exports_header_symbols(pkg):
+    L = pkg._lib.lib()
+    header = open(os.path.join(ROOT, "include", "discoeb_amd.h")).read()
+    declared = set(re.findall(r"\b(deb_[a-z_0-9]+)\s*\(", header))
+    assert declared == set(pkg._lib.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def test_layout_helpers(pkg):
+    L = pkg._lib.lib()
+    assert L.deb_version().decode().startswith("discoeb_amd")
+    for N in (256, 1024):
+        size = L.deb_tables_size(N)
+        assert size == 64 + 7 * 512 + (len(pkg._lib.ARRAY_SLOTS) - 7) * max(N, 512)
+        assert L.deb_table_offset(pkg._lib.ARRAY_SLOTS.index("aexp"), N) == 64 + 7 * 512
+    assert L.deb_nvar(12, 12, 12, 17, 8) == 192 == pkg.nvar(12, 12, 12, 17, 8)
+    assert L.deb_nvar(12, 12, 12, 8, 3) == 75
+    assert L.deb_nvar(24, 24, 12, 8, 3) == 99
+
+
+def test_argument_errors_without_gpu(pkg):
+    """Invalid descriptors are rejected on the host before any launch."""
+    L = pkg._lib.lib()
+    prob = pkg._lib.DebProblem(2, 11, 11, 8, 3, 0)
+    cfg = pkg._lib.DebSolverCfg(1e-4, 1e-4, 0.25, 0.8, 0.0, 20.0, 0.3, 2048)
+    rc = L.deb_evolve(None, 256, None, None, 4, None, 1, ctypes.byref(prob), ctypes.byref(cfg), None, None, None, None, None, None)
+    assert rc < 0 and b">= 3" in L.deb_last_error()
+    prob = pkg._lib.DebProblem(11, 11, 11, 8, 17, 0)
+    rc = L.deb_evolve(None, 256, None, None, 4, None, 1, ctypes.byref(prob), ctypes.byref(cfg), None, None, None, None, None, None)
+    assert rc < 0 and b"nqmax" in L.deb_last_error()
+    assert L.deb_background(None, 1, 4, None, None) < 0
+
+
+def test_k_grid_matches_reference_rules(pkg, oracle):
+    k = pkg.k_grid({}, 1e-4, 1.0, 64, "log")
+    assert abs(k[0] - 1e-4) / 1e-4 < 1e-14 and abs(k[-1] - 1.0) < 1e-14
+    with pytest.raises(ValueError):
+        pkg.k_grid({}, 1e-4, 1.0, 64, "camb")
+    with pytest.raises(ValueError):
+        pkg.k_grid({}, 1e-4, 1.0, 64, "cubic")
+    kc = pkg.k_grid({"tau_maxvis": 280.0}, 1e-4, 1.0, 100, "camb")
+    assert kc.size == 100 and (abs(kc[20] - 8.0 / 280.0) < 1e-15) and abs(kc[60] - 30.0 / 280.0) < 1e-15
+    import numpy as np
+    np.testing.assert_allclose(k, oracle.geomspace(1e-4, 1.0, 64), rtol=0, atol=0)

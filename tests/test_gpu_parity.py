@@ -1,0 +1,158 @@
+"""HIP path vs the CPU oracle on identical inputs (run on an MI355X: pytest -m gpu).
+
+Tolerances: the north-star bar is 1e-5 relative for transfer functions and P(k); both sides
+integrate the same Rodas5 step sequence in FP64, so agreement is normally far tighter and the
+measured error is printed for each case.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import disco_notebook, load_package
+
+pytestmark = pytest.mark.gpu
+
+TOL_TRANSFER = 1e-5  # relative, north-star tolerance for transfer functions and P(k)
+
+
+@pytest.fixture(scope="module")
+def deb():
+    if not torch.cuda.is_available():
+        pytest.skip("no ROCm device")
+    return load_package()
+
+
+def cfg0_param():
+    """Flat LCDM with massless neutrinos (Neff = 3.046): omega_c h^2 = 0.120, omega_b h^2 = 0.0224, h = 0.674."""
+    h = 0.674
+    return dict(Omegam=(0.120 + 0.0224) / h ** 2, Omegab=0.0224 / h ** 2, H0=100 * h, Tcmb=2.7255, YHe=0.248, Neff=3.046,
+                Nmnu=0.0, mnu=0.0, w_DE_0=-1.0, w_DE_a=0.0, cs2_DE=1.0, Omegak=0.0, A_s=2.1e-9, n_s=0.965, k_p=0.05)
+
+
+def cfg1_param():
+    """cfg0 plus one massive neutrino (0.06 eV) and a w0wa fluid (w0 = -0.95, wa = 0.1)."""
+    p = cfg0_param()
+    p.update(Neff=2.046, Nmnu=1.0, mnu=0.06, w_DE_0=-0.95, w_DE_a=0.1)
+    return p
+
+
+def _scaled_err(a, b):
+    """max |a - b| / max|b| per output field, then the max over fields."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    scale = np.abs(b).reshape(-1, b.shape[-1]).max(axis=0) + 1e-300
+    return (np.abs(a - b).reshape(-1, b.shape[-1]) / scale).max(axis=0)
+
+
+@pytest.mark.parametrize("num_thermo", [256, 1024])
+def test_background_tables_match_oracle(deb, oracle, golden, num_thermo):
+    print(f"case: background DISCO-Notebook num_thermo={num_thermo}", flush=True)
+    P = disco_notebook()
+    p = deb.evolve_background(param=dict(P), num_thermo=num_thermo)
+    bg = oracle.Background(P, num_thermo)
+    for name in ["aexp", "tau", "xe", "Tm", "cs2", "xeHeII", "xeprime", "opac", "optical_depth", "gvis", "gvisprime"]:
+        got = p[name].cpu().numpy()
+        ref = bg.array(name)
+        err = np.max(np.abs(got - ref) / (np.abs(ref).max() + 1e-300))
+        print(f"  {name:14s} max scaled err {err:.2e}")
+        assert err < 1e-9, name
+    for s in ["tau0", "tau_maxvis", "taumin", "OmegaDE"]:
+        assert p[s] == pytest.approx(bg.scalar(s), rel=1e-10), s
+    if num_thermo == 1024:  # the reference's own fixture
+        g = golden("evolve_background")
+        tau = p["tau_of_a_spline"].evaluate(g["a"]).cpu().numpy()
+        np.testing.assert_allclose(tau, g["tau"], rtol=1e-11)
+        xe = torch.exp(p["xe_of_loga_spline"].evaluate(np.log(g["a"]))).cpu().numpy()
+        np.testing.assert_allclose(xe, g["exp_xe"], rtol=1e-9)
+
+
+@pytest.mark.parametrize("which", ["cfg0", "cfg1"])
+def test_background_config_cosmologies(deb, oracle, which):
+    P = cfg0_param() if which == "cfg0" else cfg1_param()
+    print(f"case: background {which}", flush=True)
+    p = deb.evolve_background(param=dict(P), num_thermo=256)
+    bg = oracle.Background(P, 256)
+    for name in ["tau", "xe", "cs2", "gvis"]:
+        ref = bg.array(name)
+        err = np.max(np.abs(p[name].cpu().numpy() - ref) / np.abs(ref).max())
+        assert err < 1e-9, (name, err)
+    a = np.geomspace(1e-8, 1.0, 50)
+    np.testing.assert_allclose(p["logrhonu_of_loga_spline"].evaluate(np.log(a)).cpu().numpy(),
+                               bg.spline("logrhonu_of_loga", np.log(a)), rtol=1e-12, atol=1e-14)
+
+
+def _compare_evolve(deb, oracle, P, k, aout, lmax, rtol, atol, full=False, label=""):
+    print(f"case: evolve {label} nk={len(k)} nout={len(aout)} lmax={lmax} rtol={rtol} full={full}", flush=True)
+    p = deb.evolve_background(param=dict(P), num_thermo=256)
+    tables = p["_deb_tables"]
+    tau = p["tau_of_a_spline"].evaluate(np.asarray(aout, dtype=np.float64)).reshape(1, -1)
+    kt = torch.as_tensor(k, dtype=torch.float64, device=tables.device)
+    uc = torch.zeros(len(k), dtype=torch.int32, device=tables.device)
+    y, st, na, nr = deb.evolve_units(tables, 256, uc, kt, tau, lmaxg=lmax[0], lmaxgp=lmax[1], lmaxr=lmax[2], lmaxnu=lmax[3],
+                                     nqmax=lmax[4], rtol=rtol, atol=atol, return_full=full)
+    torch.cuda.synchronize()
+    bg = oracle.Background(P, 256)
+    yo, tau_o, sto, nao, nro = bg.evolve(k, aout, lmaxg=lmax[0], lmaxgp=lmax[1], lmaxr=lmax[2], lmaxnu=lmax[3], nqmax=lmax[4],
+                                         rtol=rtol, atol=atol, return_full=full)
+    assert (st.cpu().numpy() == 0).all() and (sto == 0).all()
+    na, nr = na.cpu().numpy(), nr.cpu().numpy()
+    print("  accepted gpu", na.tolist(), "oracle", nao.tolist())
+    print("  rejected gpu", nr.tolist(), "oracle", nro.tolist())
+    np.testing.assert_allclose(tau.cpu().numpy()[0], tau_o, rtol=1e-12)
+    yg = y.cpu().numpy()
+    err = _scaled_err(yg, yo)
+    print("  max scaled err per field", np.array2string(err, precision=1))
+    return yg, yo, err, na, nao, bg
+
+
+def test_evolve_cfg0_lcdm_matches_oracle(deb, oracle):
+    """Plain LCDM, massless neutrinos, lmax 12, rtol 1e-5 / atol 1e-10 from a = 1e-7."""
+    k = oracle.geomspace(1e-4, 1.0, 6)
+    aout = oracle.geomspace(1e-7, 1.0, 16)
+    yg, yo, err, na, nao, _ = _compare_evolve(deb, oracle, cfg0_param(), k, aout, [12, 12, 12, 8, 3], 1e-5, 1e-10, label="cfg0")
+    assert err.max() < TOL_TRANSFER
+    assert np.abs(na - nao).max() <= 2
+
+
+def test_evolve_cfg1_massive_nu_w0wa_matches_oracle(deb, oracle):
+    """One massive neutrino on an 8-point momentum grid (lmax_nu 17) and a w0wa fluid."""
+    k = oracle.geomspace(1e-5, 5.0, 6)
+    aout = [1.0 / 3.0, 0.5, 1.0]
+    P = cfg1_param()
+    yg, yo, err, na, nao, _ = _compare_evolve(deb, oracle, P, k, aout, [12, 12, 12, 17, 8], 1e-5, 1e-10, label="cfg1")
+    assert err.max() < TOL_TRANSFER
+    pk_g = deb.get_power(k=torch.as_tensor(k, device="cuda"), y=torch.as_tensor(yg[:, -1, :], device="cuda"), idx=4,
+                         param=P).cpu().numpy()
+    pk_o = oracle.get_power(k, yo[:, -1, :], 4, P)
+    rel = np.abs(pk_g / pk_o - 1).max()
+    print(f"  P(k, z=0) max rel err {rel:.2e}")
+    assert rel < TOL_TRANSFER
+
+
+def test_return_full_state_and_time_derivatives(deb, oracle):
+    """Projected full state (RSA projection included) and compute_time_derivatives vs the oracle RHS."""
+    P = disco_notebook()
+    k = np.array([0.02, 0.3])
+    aout = [2e-3, 0.02, 0.5]
+    lmax = [11, 11, 11, 8, 3]
+    yg, yo, err, na, nao, bg = _compare_evolve(deb, oracle, P, k, aout, lmax, 1e-4, 1e-4, full=True, label="full")
+    assert err.max() < TOL_TRANSFER
+    p = deb.evolve_background(param=dict(P), num_thermo=256)
+    tau = p["tau_of_a_spline"].evaluate(np.asarray(aout))
+    dy = deb.compute_time_derivatives(torch.as_tensor(yo, device="cuda"), tau, torch.as_tensor(k, device="cuda"), p, *lmax).cpu().numpy()
+    for i in range(len(k)):
+        for j in range(len(aout)):
+            ref = bg.rhs(lmax, float(tau[j]), yo[i, j], k[i])
+            np.testing.assert_allclose(dy[i, j], ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def test_reference_surface_and_errors(deb):
+    P = disco_notebook()
+    p = deb.evolve_background(param=dict(P))
+    y, kmodes, p2 = deb.evolve_perturbations(param=p, aexp_out=[0.01], kmin=1e-3, kmax=0.1, num_k=4, k_sampling_method="log")
+    assert y.shape == (4, 1, 20) and kmodes.shape == (4,)
+    assert p2["nout"] == 1 and p2["lmaxg"] == 11 and p2["tau_out"].shape == (1,)
+    with pytest.raises(RuntimeError, match="max_steps"):
+        deb.evolve_perturbations(param=p, aexp_out=[1.0], kmin=0.1, kmax=0.2, num_k=2, max_steps=5, k_sampling_method="log")
+    with pytest.raises(ValueError):
+        deb.evolve_perturbations(param=p, aexp_out=[1.0], kmin=0.1, kmax=0.2, num_k=2, lmaxg=2, k_sampling_method="log")
