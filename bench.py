@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=24, help="k-modes timed on the CPU oracle (0: skip)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (a known-size kernel for scaling HBM counters)")
     args = ap.parse_args()
     rank = int(os.environ.get("RANK", "0"))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -143,6 +145,12 @@ def main():
                      "flop_per_step": W},
         "cpu_baseline": None,
     }
+    if args.copy_calibration_mib > 0:
+        src = torch.ones(args.copy_calibration_mib * (1 << 20) // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = src.numel() * 8
     if args.cpu_sample > 0 and rank == 0:
         out["cpu_baseline"] = cpu_baseline(kmodes, args.cpu_sample)
     print(json.dumps(out))

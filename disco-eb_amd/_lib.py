@@ -83,7 +83,7 @@ def lib():
     L.deb_background.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
     L.deb_spline_eval.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
     L.deb_evolve.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.POINTER(DebProblem),
-                             ctypes.POINTER(DebSolverCfg), _vp, _vp, _vp, _vp, _vp, _vp]
+                             ctypes.POINTER(DebSolverCfg), _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.deb_power.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                             ctypes.c_double, ctypes.c_double, _vp, _vp]
     L.deb_time_derivatives.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int,

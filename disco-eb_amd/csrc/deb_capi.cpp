@@ -10,7 +10,7 @@
 extern "C" int deb_launch_background(const double *, int, int, double *, hipStream_t);
 extern "C" int deb_launch_nu_bins(int, double *, hipStream_t);
 extern "C" int deb_launch_perturb(const double *, int, const int *, const double *, int, const double *, int, const double *,
-                                  const DebProblem *, const DebSolverCfg *, double *, int *, int *, int *, hipStream_t);
+                                  const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, hipStream_t);
 extern "C" long deb_perturb_lds_bytes(int, int, int, int, int);
 extern "C" int deb_launch_spline_eval(const double *, int, int, int, const double *, int, double *, hipStream_t);
 extern "C" int deb_launch_power(const double *, int, const double *, long, int, int, double, double, double, double *, hipStream_t);
@@ -73,7 +73,7 @@ int deb_spline_eval(const double *d_tables, int num_thermo, int cosmo, int splin
 
 int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
                const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
-               int *d_n_accept, int *d_n_reject, double *d_work, void *stream) {
+               int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, void *stream) {
   int rc = check_problem(prob);
   if (rc) return rc;
   if (!cfg) return fail(-1, "solver descriptor is NULL");
@@ -89,8 +89,8 @@ int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, 
   if (lds > 160 * 1024) return fail(-1, "state too large for the 160 KiB LDS of one CU (lower the hierarchy truncations)");
   rc = deb_launch_nu_bins(prob->nqmax, d_work, (hipStream_t)stream);
   if (rc) return hip_rc(rc, "deb_evolve (momentum bins)");
-  rc = deb_launch_perturb(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, d_work, prob, cfg, d_y_out, d_status,
-                          d_n_accept, d_n_reject, (hipStream_t)stream);
+  rc = deb_launch_perturb(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, d_work, prob, cfg, d_step_log, d_y_out,
+                          d_status, d_n_accept, d_n_reject, (hipStream_t)stream);
   if (rc == -2) return fail(-1, "state too large for LDS");
   return hip_rc(rc, "deb_evolve");
 }

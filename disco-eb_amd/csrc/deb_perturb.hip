@@ -717,7 +717,7 @@ struct Pid {
 __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
                                                        const double *__restrict__ unit_k, int n_units, const double *__restrict__ tau_out,
                                                        int nout, const double *__restrict__ nu_qw, DebProblem prob, DebSolverCfg cfg,
-                                                       long lds_bytes, double *__restrict__ y_out,
+                                                       long lds_bytes, double *__restrict__ step_log, double *__restrict__ y_out,
                                                        int *__restrict__ status, int *__restrict__ n_acc, int *__restrict__ n_rej) {
   const int unit = blockIdx.x;
   if (unit >= n_units) return;
@@ -856,6 +856,12 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
       }
     }
     double en = sqrt(acc / 6.0);
+    if (step_log != nullptr && lane == 0) {  // (t_prev, t_next, error norm) of every attempted step
+      double *lg = step_log + ((long)unit * cfg.max_steps + steps) * 3;
+      lg[0] = tprev;
+      lg[1] = tnext;
+      lg[2] = en;
+    }
     if (!isfinite(en)) { st = 2; break; }
     bool keep = en < 1.0;
     double factor = pid.adapt(en, keep, cfg);
@@ -982,7 +988,7 @@ extern "C" int deb_launch_tau_out(const double *d_tables, int num_thermo, int n_
 
 extern "C" int deb_launch_perturb(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
                                   const double *d_tau_out, int nout, const double *d_nu_qw, const DebProblem *prob,
-                                  const DebSolverCfg *cfg, double *d_y_out,
+                                  const DebSolverCfg *cfg, double *d_step_log, double *d_y_out,
                                   int *d_status, int *d_nacc, int *d_nrej, hipStream_t stream) {
   long stride = deb_table_doubles(num_thermo);
   long lds = deb_perturb_lds_bytes(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
@@ -992,6 +998,6 @@ extern "C" int deb_launch_perturb(const double *d_tables, int num_thermo, const 
     if (e != hipSuccess) return (int)e;
   }
   hipLaunchKernelGGL(deb::perturb_kernel, dim3(n_units), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k,
-                     n_units, d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_y_out, d_status, d_nacc, d_nrej);
+                     n_units, d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej);
   return (int)hipGetLastError();
 }

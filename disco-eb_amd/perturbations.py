@@ -62,12 +62,14 @@ def nvar(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax):
 
 def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3,
                  rtol=1e-4, atol=1e-4, pcoeff=0.25, icoeff=0.80, dcoeff=0.0, factormax=20.0, factormin=0.3, max_steps=2048,
-                 return_full=False, out=None):
+                 return_full=False, out=None, step_log=False):
     """Low-level launch: independent adaptive solves of (cosmology, k) units.
 
     tables: (n_cosmo, size) device tensor from background_tables; unit_cosmo (int32) and unit_k
     (float64) device tensors of length n_units; tau_out (n_cosmo, nout) device tensor.
-    Returns (y, status, n_accept, n_reject) device tensors; asynchronous on the current stream."""
+    Returns (y, status, n_accept, n_reject) device tensors, plus the step mesh
+    (n_units, max_steps, 3: t_prev, t_next, error norm) when step_log is set; asynchronous on
+    the current stream."""
     L = _lib.lib()
     dev = tables.device
     n_units = int(unit_k.numel())
@@ -85,9 +87,12 @@ def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, l
     uc = unit_cosmo.to(device=dev, dtype=torch.int32).contiguous()
     uk = unit_k.to(device=dev, dtype=torch.float64).contiguous()
     to = tau_out.to(device=dev, dtype=torch.float64).contiguous()
+    log = torch.full((n_units, int(max_steps), 3), float("nan"), dtype=torch.float64, device=dev) if step_log else None
     _lib.check(L.deb_evolve(_lib.ptr(tables), int(num_thermo), _lib.ptr(uc), _lib.ptr(uk), n_units, _lib.ptr(to), nout,
                             ctypes.byref(prob), ctypes.byref(cfg), _lib.ptr(y), _lib.ptr(status), _lib.ptr(nacc),
-                            _lib.ptr(nrej), _lib.ptr(work), _lib.stream()))
+                            _lib.ptr(nrej), _lib.ptr(work), _lib.ptr(log) if log is not None else None, _lib.stream()))
+    if step_log:
+        return y, status, nacc, nrej, log
     return y, status, nacc, nrej
 
 

@@ -57,11 +57,13 @@ long deb_evolve_workspace_doubles(void);
  * d_status[u]: 0 ok, 1 max_steps exceeded, 2 non-finite error norm (outputs NaN from there).
  * d_n_accept / d_n_reject: accepted / rejected step counts per unit.
  * d_work: deb_evolve_workspace_doubles() doubles.
+ * d_step_log: NULL, or n_units x max_steps x 3 doubles receiving (t_prev, t_next, error norm)
+ * of every attempted step (the step mesh, for replay checks and frozen-mesh derivatives).
  * Replaces evolve_perturbations (src/discoeb/perturbations.py:1124-1226) with
  * evolve_one_mode (:882-949) per unit. */
 int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
                const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
-               int *d_n_accept, int *d_n_reject, double *d_work, void *stream);
+               int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, void *stream);
 
 /* Number of state variables for the given truncations (perturbations.py:893). */
 int deb_nvar(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax);

@@ -111,6 +111,24 @@ struct Spline {
     dxinv = 1.0 / (x[1] - x[0]);
   }
 
+  // adopt knots, values and second derivatives as given (no Thomas solve)
+  void adopt(const double *xin, const double *yin, const double *Sin, int nn, bool unif) {
+    n = nn;
+    uniform = unif;
+    x.assign(xin, xin + n);
+    y.assign(yin, yin + n);
+    S.assign(Sin, Sin + n);
+    Icum.assign(n, 0.0);
+    for (int i = 0; i < n - 1; ++i) {
+      double h = x[i + 1] - x[i];
+      double b = (y[i + 1] - y[i]) / h - h * (S[i + 1] + 2 * S[i]) / 6.0;
+      double c = S[i] / 2.0, d = (S[i + 1] - S[i]) / (6.0 * h);
+      Icum[i + 1] = Icum[i] + (y[i] * h + b * h * h / 2 + c * h * h * h / 3 + d * h * h * h * h / 4);
+    }
+    x0 = x[0];
+    dxinv = 1.0 / (x[1] - x[0]);
+  }
+
   // spline_interpolation.py:132-145
   int index(double xq) const {
     int idx;
@@ -1356,6 +1374,39 @@ int evolve_one(const Bg &bg, const Layout &L, double kmode, const double *tau_ou
   return status;
 }
 
+// Replay of a prescribed step mesh (t_prev, t_next, accepted) from the same start: every step is
+// taken with dt = t_next - t_prev; outputs as in evolve_one.  en_out receives this solver's own
+// error norm of each step, so the caller can compare the two solvers step by step.
+int evolve_replay(const Bg &bg, const Layout &L, double kmode, const double *tau_out, int nout, const SolverCfg &s, bool full,
+                  const double *mesh, const int *keep, int nsteps, double *yout, double *en_out) {
+  const int n = L.n;
+  double tmin = tau_out[0];
+  for (int i = 1; i < nout; ++i) tmin = std::min(tmin, tau_out[i]);
+  double t0 = 0.99 * std::min(tmin, starting_time(bg, kmode));
+  Workspace ws;
+  ws.init(n);
+  std::vector<double> y(n), y1(n), err(n), tmp(n);
+  initial_conditions(bg, L, t0, kmode, y.data());
+  int isave = 0;
+  for (int st = 0; st < nsteps; ++st) {
+    double tprev = mesh[2 * st], tnext = mesh[2 * st + 1];
+    rodas5_step(bg, L, kmode, tprev, tnext - tprev, y.data(), y1.data(), err.data(), ws);
+    en_out[st] = std::sqrt(filtered_sq(y.data(), y1.data(), err.data(), kmode, s, any_nan(y1.data(), n)) / 6.0);
+    if (keep[st]) {
+      while (isave < nout && tau_out[isave] <= tnext) {
+        double th = (tau_out[isave] - tprev) / (tnext - tprev);
+        for (int i = 0; i < n; ++i) tmp[i] = y[i] + th * (y1[i] - y[i]);
+        rsa_projection(bg, L, tau_out[isave], kmode, tmp.data());
+        if (full) std::memcpy(yout + (size_t)isave * n, tmp.data(), sizeof(double) * n);
+        else convert_output(bg, L, kmode, tmp.data(), yout + (size_t)isave * 20);
+        ++isave;
+      }
+      y.swap(y1);
+    }
+  }
+  return isave == nout ? 0 : 1;
+}
+
 // evolve_modes_batched (perturbations.py:1002-1120) with Rodas5Batched (ode_integrators_stiff.py:928-1059):
 // B modes share one step size, one start time (the batch minimum) and one RMS norm over B x 6 entries.
 int evolve_batch(const Bg &bg, const Layout &L, const double *kmodes, int B, const double *tau_out, int nout, const SolverCfg &s,
@@ -1421,6 +1472,43 @@ int evolve_batch(const Bg &bg, const Layout &L, const double *kmodes, int B, con
   return status;
 }
 
+// Background from a table block written by the HIP background kernel (layout of
+// disco-eb_amd/csrc/deb_layout.h, offsets passed in by the caller).  Lets the tests compare the
+// perturbation solvers on bit-identical background tables.
+void background_from_tables(Bg &bg, const double *t, int N, const long *off) {
+  Cosmo &c = bg.c;
+  setup_cosmo(t, c);
+  bg.num_thermo = N;
+  c.amin = 1e-9; c.amax = 1.01;
+  c.grhom = t[24]; c.grhog = t[25]; c.grhor = t[26]; c.adotrad = t[27]; c.amnu = t[28]; c.Omegamnu = t[29];
+  c.OmegaDE = t[30]; c.taumin = t[31]; c.taumax = t[32]; c.fHe = t[33];
+  bg.tau0 = t[34]; bg.tau_maxvis = t[35]; bg.optical_depth_today = t[36];
+  enum { NU_LOGA, NU_LR, NU_LRS, NU_LP, NU_LPS, NU_LPP, NU_LPPS, AEXP, LOGA, TAU, XE, CS2A, TEMPBA, S_TAU_OF_A, S_A_OF_TAU,
+         S_XE_OF_TAU, S_CS2A_OF_TAU, S_TEMPBA_OF_TAU, S_XE_OF_LOGA, S_CS2A_OF_LOGA, NOFF };
+  const int nnu = 512;
+  bg.logrhonu.adopt(t + off[NU_LOGA], t + off[NU_LR], t + off[NU_LRS], nnu, true);
+  bg.logpnu.adopt(t + off[NU_LOGA], t + off[NU_LP], t + off[NU_LPS], nnu, true);
+  bg.logppnu.adopt(t + off[NU_LOGA], t + off[NU_LPP], t + off[NU_LPPS], nnu, true);
+  bg.logrhonu.x0 = bg.logppnu.x0 = bg.logpnu.x0 = t[41];
+  bg.logrhonu.dxinv = bg.logppnu.dxinv = bg.logpnu.dxinv = t[42];
+  bg.aexp.assign(t + off[AEXP], t + off[AEXP] + N);
+  bg.tau.assign(t + off[TAU], t + off[TAU] + N);
+  bg.xe.assign(t + off[XE], t + off[XE] + N);
+  bg.tau_of_a.adopt(t + off[AEXP], t + off[TAU], t + off[S_TAU_OF_A], N, false);
+  bg.a_of_tau.adopt(t + off[TAU], t + off[AEXP], t + off[S_A_OF_TAU], N, false);
+  bg.xe_of_tau.adopt(t + off[TAU], t + off[XE], t + off[S_XE_OF_TAU], N, false);
+  bg.cs2a_of_tau.adopt(t + off[TAU], t + off[CS2A], t + off[S_CS2A_OF_TAU], N, false);
+  bg.tempba_of_tau.adopt(t + off[TAU], t + off[TEMPBA], t + off[S_TEMPBA_OF_TAU], N, false);
+  bg.xe_of_loga.adopt(t + off[LOGA], t + off[XE], t + off[S_XE_OF_LOGA], N, false);
+  bg.cs2a_of_loga.adopt(t + off[LOGA], t + off[CS2A], t + off[S_CS2A_OF_LOGA], N, false);
+}
+
+double start_tau(const Bg &bg, double k, const double *tau_out, int nout) {
+  double tmin = tau_out[0];
+  for (int i = 1; i < nout; ++i) tmin = std::min(tmin, tau_out[i]);
+  return 0.99 * std::min(tmin, starting_time(bg, k));
+}
+
 const Spline *find_spline(const Bg &bg, const std::string &name) {
   if (name == "tau_of_a") return &bg.tau_of_a;
   if (name == "a_of_tau") return &bg.a_of_tau;
@@ -1451,6 +1539,12 @@ void *orc_bg_new(const double *params, int num_thermo) {
   return bg;
 }
 void orc_bg_free(void *h) { delete (Bg *)h; }
+
+void *orc_bg_from_tables(const double *block, int num_thermo, const long *offsets) {
+  Bg *bg = new Bg();
+  background_from_tables(*bg, block, num_thermo, offsets);
+  return bg;
+}
 
 double orc_bg_scalar(void *h, const char *name) {
   const Bg &bg = *(Bg *)h;
@@ -1616,5 +1710,16 @@ int orc_evolve(void *h, const double *kmodes, int nk, const double *tau_out, int
   }
   return 0;
 }
+
+int orc_evolve_replay(void *h, double k, const double *tau_out, int nout, const int *lmax, const double *solver, int return_full,
+                      const double *mesh, const int *keep, int nsteps, double *y_out, double *en_out) {
+  const Bg &bg = *(Bg *)h;
+  Layout L;
+  L.init(lmax[0], lmax[1], lmax[2], lmax[3], lmax[4]);
+  SolverCfg s{solver[0], solver[1], solver[2], solver[3], solver[4], solver[5], solver[6], nsteps};
+  return evolve_replay(bg, L, k, tau_out, nout, s, return_full != 0, mesh, keep, nsteps, y_out, en_out);
+}
+
+double orc_start_tau(void *h, double k, const double *tau_out, int nout) { return start_tau(*(Bg *)h, k, tau_out, nout); }
 
 }  // extern "C"

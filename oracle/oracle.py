@@ -44,6 +44,8 @@ def _load():
     lib.orc_bg_new.restype = ctypes.c_void_p
     lib.orc_bg_new.argtypes = [_dp, ctypes.c_int]
     lib.orc_bg_free.argtypes = [ctypes.c_void_p]
+    lib.orc_bg_from_tables.restype = ctypes.c_void_p
+    lib.orc_bg_from_tables.argtypes = [_dp, ctypes.c_int, ctypes.POINTER(ctypes.c_long)]
     lib.orc_bg_scalar.restype = ctypes.c_double
     lib.orc_bg_scalar.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
     lib.orc_bg_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp]
@@ -62,6 +64,10 @@ def _load():
     lib.orc_rodas5_step.argtypes = [ctypes.c_void_p, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
     lib.orc_evolve.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, ctypes.c_int, _dp, _ip, _ip, _ip]
+    lib.orc_evolve_replay.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, _ip,
+                                      ctypes.c_int, _dp, _dp]
+    lib.orc_start_tau.restype = ctypes.c_double
+    lib.orc_start_tau.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, ctypes.c_int]
     _lib = lib
     return lib
 
@@ -106,6 +112,21 @@ class Background:
         self.num_thermo = int(num_thermo)
         pv, pp = _d(param_vector(param))
         self._h = self._lib.orc_bg_new(pp, self.num_thermo)
+
+    @classmethod
+    def from_tables(cls, block, num_thermo, offsets, param=None):
+        """Background taken verbatim from a HIP table block (numpy float64 array) whose array
+        offsets (in doubles) are given for: nu_loga, nu_logrho, nu_logrho_S, nu_logp, nu_logp_S,
+        nu_logpp, nu_logpp_S, aexp, loga, tau, xe, cs2a, tempba, S_tau_of_a, S_a_of_tau,
+        S_xe_of_tau, S_cs2a_of_tau, S_tempba_of_tau, S_xe_of_loga, S_cs2a_of_loga."""
+        self = cls.__new__(cls)
+        self._lib = _load()
+        self.param = dict(param or {})
+        self.num_thermo = int(num_thermo)
+        self._block = np.ascontiguousarray(block, dtype=np.float64)
+        offs = (ctypes.c_long * len(offsets))(*[int(o) for o in offsets])
+        self._h = self._lib.orc_bg_from_tables(self._block.ctypes.data_as(_dp), self.num_thermo, offs)
+        return self
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -201,6 +222,30 @@ class Background:
         if rc != 0:
             raise ValueError("num_k must be a multiple of batch_size")
         return y, tau_out, st, na, nr
+
+
+def _replay(self, k, aexp_out, mesh, keep, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4,
+            pcoeff=0.25, icoeff=0.80, dcoeff=0.0, factormax=20.0, factormin=0.3, return_full=False):
+    """Take exactly the steps of `mesh` (nsteps x 2: t_prev, t_next; `keep` accepted flags) for one
+    mode; returns (y[nout, 20 or nvar], own error norm per step)."""
+    lib = _load()
+    aexp_out = np.atleast_1d(np.asarray(aexp_out, dtype=np.float64))
+    tau_out, tp = _d(self.spline("tau_of_a", aexp_out))
+    lmax = [lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax]
+    lm, lp = _i(lmax)
+    n = lib.orc_layout(*lmax)
+    w = n if return_full else 20
+    mesh, mp = _d(mesh)
+    keep, kp = _i(keep)
+    sv, sp = _d([rtol, atol, pcoeff, icoeff, dcoeff, factormax, factormin])
+    y = np.full((tau_out.size, w), np.nan)
+    en = np.empty(keep.size)
+    lib.orc_evolve_replay(self._h, float(k), tp, tau_out.size, lp, sp, int(bool(return_full)), mp, kp, keep.size,
+                          y.ctypes.data_as(_dp), en.ctypes.data_as(_dp))
+    return y, en
+
+
+Background.replay = _replay
 
 
 def nvar(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax):
