@@ -112,9 +112,9 @@ def main():
 
     for _ in range(args.warmup):
         st, na, nr, pk = one_step(False)
-    torch.cuda.synchronize()
-    if (st != 0).any():
-        raise RuntimeError(f"solver failures in warm-up: status {st.cpu().numpy().tolist()}")
+        torch.cuda.synchronize()
+        if (st != 0).any():
+            raise RuntimeError(f"solver failures in warm-up: status {st.cpu().numpy().tolist()}")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         st, na, nr, pk = one_step(True)

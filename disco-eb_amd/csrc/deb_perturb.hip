@@ -89,19 +89,96 @@ template <class T> struct PBg {
   bool rsa, ufa, tca;
 };
 
+// Per-wave cache of the spline intervals the RHS reads: the thermal-history splines in log a
+// (x_e and c_s^2 a share knots) and the neutrino tables.  Successive evaluations move a little
+// in a, so the interval and its coefficients are reused; a miss falls back to the full search,
+// and the interval chosen is always the one spl_index would return.
+struct SplCache {
+  int i, j;
+  double x[2], ye[2], Se[2], yc[2], Sc[2];      // log-a knots, x_e, c_s^2 a
+  double nx[2], nr[2], nSr[2], np[2], nSp[2];   // neutrino tables
+};
+
+// coefficients of the current intervals, in registers of every lane
+struct SplCoef {
+  double x[2], ye[2], Se[2], yc[2], Sc[2], nx[2], nr[2], nSr[2], np[2], nSp[2];
+};
+
+// Interval coefficients for log a = la.  A hit reads the LDS entry; a miss loads from the
+// tables on every lane, lane 0 republishes the entry and a barrier orders that write before
+// any later read.  The hit/miss decision is uniform across the wave (same inputs), so the
+// barrier is reached by all lanes.
+__device__ __forceinline__ void cache_lookup(const BgView &b, SplCache *c, double la, int lane, SplCoef &k) {
+  const int N = b.N;
+  bool refreshed = false;
+  int i = c->i;
+  bool hit = (i >= 0) && (i == 0 || c->x[0] < la) && (i == N - 2 || la <= c->x[1]);
+  if (hit) {
+    k.x[0] = c->x[0]; k.x[1] = c->x[1]; k.ye[0] = c->ye[0]; k.ye[1] = c->ye[1]; k.Se[0] = c->Se[0]; k.Se[1] = c->Se[1];
+    k.yc[0] = c->yc[0]; k.yc[1] = c->yc[1]; k.Sc[0] = c->Sc[0]; k.Sc[1] = c->Sc[1];
+  } else {
+    i = spl_index(b.loga, N, la);
+    k.x[0] = b.loga[i]; k.x[1] = b.loga[i + 1];
+    k.ye[0] = b.xe[i]; k.ye[1] = b.xe[i + 1]; k.Se[0] = b.S_xe_of_loga[i]; k.Se[1] = b.S_xe_of_loga[i + 1];
+    k.yc[0] = b.cs2a[i]; k.yc[1] = b.cs2a[i + 1]; k.Sc[0] = b.S_cs2a_of_loga[i]; k.Sc[1] = b.S_cs2a_of_loga[i + 1];
+    refreshed = true;
+  }
+  int j = spl_index_uniform(b.nu_x0, b.nu_dxinv, DEB_NNU, la);
+  bool nhit = (j == c->j);
+  if (nhit) {
+    k.nx[0] = c->nx[0]; k.nx[1] = c->nx[1]; k.nr[0] = c->nr[0]; k.nr[1] = c->nr[1]; k.nSr[0] = c->nSr[0]; k.nSr[1] = c->nSr[1];
+    k.np[0] = c->np[0]; k.np[1] = c->np[1]; k.nSp[0] = c->nSp[0]; k.nSp[1] = c->nSp[1];
+  } else {
+    k.nx[0] = b.nu_loga[j]; k.nx[1] = b.nu_loga[j + 1];
+    k.nr[0] = b.nu_lr[j]; k.nr[1] = b.nu_lr[j + 1]; k.nSr[0] = b.nu_lrS[j]; k.nSr[1] = b.nu_lrS[j + 1];
+    k.np[0] = b.nu_lp[j]; k.np[1] = b.nu_lp[j + 1]; k.nSp[0] = b.nu_lpS[j]; k.nSp[1] = b.nu_lpS[j + 1];
+    refreshed = true;
+  }
+  if (refreshed) {
+    __syncthreads();  // every lane has read the old entry before lane 0 overwrites it
+    if (lane == 0) {
+      if (!hit) {
+        c->i = i; c->x[0] = k.x[0]; c->x[1] = k.x[1]; c->ye[0] = k.ye[0]; c->ye[1] = k.ye[1]; c->Se[0] = k.Se[0];
+        c->Se[1] = k.Se[1]; c->yc[0] = k.yc[0]; c->yc[1] = k.yc[1]; c->Sc[0] = k.Sc[0]; c->Sc[1] = k.Sc[1];
+      }
+      if (!nhit) {
+        c->j = j; c->nx[0] = k.nx[0]; c->nx[1] = k.nx[1]; c->nr[0] = k.nr[0]; c->nr[1] = k.nr[1]; c->nSr[0] = k.nSr[0];
+        c->nSr[1] = k.nSr[1]; c->np[0] = k.np[0]; c->np[1] = k.np[1]; c->nSp[0] = k.nSp[0]; c->nSp[1] = k.nSp[1];
+      }
+    }
+    __syncthreads();  // the new entry is visible before any later lookup
+  }
+}
+
 template <class T>
-__device__ void pbg_eval(const BgView &b, const PLay &L, double k, const T &tau, const T &a, PBg<T> &g) {
+__device__ __forceinline__ T cubic_at(double x0, double x1, double y0, double y1, double S0, double S1, const T &xq) {
+  double hl = x1 - x0;
+  T t = (xq - x0) / hl;
+  T A = 1.0 - t;
+  T B = t;
+  return A * y0 + B * y1 + ((A * A * A - A) * S0 + (B * B * B - B) * S1) * (hl * hl / 6.0);
+}
+template <class T>
+__device__ __forceinline__ T cubic_deriv_at(double x0, double x1, double y0, double y1, double S0, double S1, const T &xq) {
+  double hl = x1 - x0;
+  T d = xq - x0;
+  double bb = (y1 - y0) / hl - hl * (S1 + 2.0 * S0) / 6.0;
+  return bb + 2.0 * (S0 / 2.0) * d + 3.0 * ((S1 - S0) / (6.0 * hl)) * d * d;
+}
+
+template <class T>
+__device__ __forceinline__ void pbg_eval(const BgView &b, const PLay &L, double k, const T &tau, const T &a, PBg<T> &g, SplCache *sc, int lane) {
   g.a = a;
   g.tau = tau;
   T loga = dlog(a);
-  int i = spl_index(b.loga, b.N, val(loga));
-  g.cs2 = spl_eval_at(b.loga, b.cs2a, b.S_cs2a_of_loga, i, loga) / a;
-  g.xe = spl_eval_at(b.loga, b.xe, b.S_xe_of_loga, i, loga);
-  T dxe = spl_deriv_at(b.loga, b.xe, b.S_xe_of_loga, i, loga);
+  SplCoef c;
+  cache_lookup(b, sc, val(loga), lane, c);
+  g.cs2 = cubic_at(c.x[0], c.x[1], c.yc[0], c.yc[1], c.Sc[0], c.Sc[1], loga) / a;
+  g.xe = cubic_at(c.x[0], c.x[1], c.ye[0], c.ye[1], c.Se[0], c.Se[1], loga);
+  T dxe = cubic_deriv_at(c.x[0], c.x[1], c.ye[0], c.ye[1], c.Se[0], c.Se[1], loga);
   g.pb43 = 4.0 / 3.0 * (b.grhog / (b.grhom * b.Omegab * a));
-  int j = spl_index_uniform(b.nu_x0, b.nu_dxinv, DEB_NNU, val(loga));
-  T rhonu = dexp(spl_eval_at(b.nu_loga, b.nu_lr, b.nu_lrS, j, loga));
-  T pnu = dexp(spl_eval_at(b.nu_loga, b.nu_lp, b.nu_lpS, j, loga));
+  T rhonu = dexp(cubic_at(c.nx[0], c.nx[1], c.nr[0], c.nr[1], c.nSr[0], c.nSr[1], loga));
+  T pnu = dexp(cubic_at(c.nx[0], c.nx[1], c.np[0], c.np[1], c.nSp[0], c.nSp[1], loga));
   g.w_Q = b.w0 + b.wa * (1.0 - a);
   g.rho_Q = b.rho_Q(a);
   T a2 = a * a;
@@ -134,7 +211,7 @@ template <class T> struct Met {
 };
 
 template <class T, class Y>
-__device__ void metric(const BgView &b, const PLay &L, double k, const PBg<T> &g, const Y &y, Met<T> &m) {
+__device__ __forceinline__ void metric(const BgView &b, const PLay &L, double k, const PBg<T> &g, const Y &y, Met<T> &m) {
   const int n = L.n, nq = L.nq, iq0 = L.iq0;
   const double Omegac = b.Omegam - b.Omegab;
   const double eta = y(2), deltac = y(3), thetac = y(4), deltab = y(5), thetab = y(6);
@@ -195,7 +272,7 @@ __device__ void metric(const BgView &b, const PLay &L, double k, const PBg<T> &g
 
 // RHS row i for i outside the l >= 3 tails (row 0 included)
 template <class T, class Y>
-__device__ T core_row(int i, const BgView &b, const PLay &L, double k, const PBg<T> &g, const Met<T> &m, const Y &y) {
+__device__ __forceinline__ T core_row(int i, const BgView &b, const PLay &L, double k, const PBg<T> &g, const Met<T> &m, const Y &y) {
   const int n = L.n, nq = L.nq, iq0 = L.iq0, idxgp = L.idxgp, idxr = L.idxr;
   const double k2 = k * k;
   if (i == 0) return g.aH * g.a;
@@ -250,7 +327,7 @@ __device__ T core_row(int i, const BgView &b, const PLay &L, double k, const PBg
 
 // tail chain coefficients: f_l = lo y_{l-1} + di y_l + up y_{l+1}, l = 3..lmax
 template <class T>
-__device__ void tail_coef(int ch, int l, const PLay &L, double k, const PBg<T> &g, T &lo, T &di, T &up) {
+__device__ __forceinline__ void tail_coef(int ch, int l, const PLay &L, double k, const PBg<T> &g, T &lo, T &di, T &up) {
   int lmax;
   bool off;
   T kv(k), damp(0.0);
@@ -288,12 +365,13 @@ __device__ __forceinline__ int wave_or(int v) {
 // per-wave LDS workspace
 struct Ws {
   double *y0, *y1, *u, *f, *J0, *dT, *K, *S, *Jh, *tlo, *tdi, *tup, *tmul, *td, *xd;
-  int *dmap, *rk, *chead, *cstride, *clen, *cpar;
+  int *dmap, *rk, *chead, *cstride, *clen, *cpar, *piv, *pi;
+  SplCache *sc;
 };
 
 
 // rows of the tail chains: rk[i] = (chain << 8) | l, -1 for rows in the dense block or row 0
-__device__ void setup_maps(const PLay &L, Ws &W, int lane) {
+__device__ __forceinline__ void setup_maps(const PLay &L, Ws &W, int lane) {
   const int n = L.n, nq = L.nq;
   if (lane == 0) {
     int r = 0;
@@ -319,9 +397,9 @@ __device__ void setup_maps(const PLay &L, Ws &W, int lane) {
 }
 
 // full RHS f(tau, u) into out[] (lanes over rows); u in LDS
-__device__ void rhs_eval(const BgView &b, const PLay &L, double k, double tau, const double *u, double *out, Ws &W, int lane) {
+__device__ __forceinline__ void rhs_eval(const BgView &b, const PLay &L, double k, double tau, const double *u, double *out, Ws &W, int lane) {
   PBg<double> g;
-  pbg_eval<double>(b, L, k, tau, u[0], g);
+  pbg_eval<double>(b, L, k, tau, u[0], g, W.sc, lane);
   auto yv = [&](int i) { return u[i]; };
   Met<double> m;
   metric<double>(b, L, k, g, yv, m);
@@ -342,41 +420,100 @@ __device__ void rhs_eval(const BgView &b, const PLay &L, double k, double tau, c
   }
 }
 
-// d f / d(seed) at (tau, y) with seed = a (which = 0) or tau (which = 1)
-__device__ void rhs_dual(const BgView &b, const PLay &L, double k, double tau, const double *y, double *out, Ws &W, int lane,
-                         int which) {
-  Dual td(tau), ad(y[0]);
-  if (which == 0) ad.d[0] = 1.0; else td.d[0] = 1.0;
-  PBg<Dual> g;
-  pbg_eval<Dual>(b, L, k, td, ad, g);
+// d f / da (into outa) and d f / dtau (into outt) at (tau, y): one pass with two dual directions
+__device__ __forceinline__ void rhs_dual(const BgView &b, const PLay &L, double k, double tau, const double *y, double *outa, double *outt, Ws &W,
+                         int lane) {
+  typedef Dn<2> D2;
+  D2 td(tau), ad(y[0]);
+  ad.d[0] = 1.0;
+  td.d[1] = 1.0;
+  PBg<D2> g;
+  pbg_eval<D2>(b, L, k, td, ad, g, W.sc, lane);
   auto yv = [&](int i) { return y[i]; };
-  Met<Dual> m;
-  metric<Dual>(b, L, k, g, yv, m);
+  Met<D2> m;
+  metric<D2>(b, L, k, g, yv, m);
   for (int i = lane; i < L.n; i += WAVE) {
     int rk = W.rk[i];
-    Dual r;
+    D2 r;
     if (rk < 0) {
-      r = core_row<Dual>(i, b, L, k, g, m, yv);
+      r = core_row<D2>(i, b, L, k, g, m, yv);
     } else {
       int ch = rk >> 8, l = rk & 255;
-      Dual lo, di, up;
-      tail_coef<Dual>(ch, l, L, k, g, lo, di, up);
+      D2 lo, di, up;
+      tail_coef<D2>(ch, l, L, k, g, lo, di, up);
       int s = W.cstride[ch];
-      double yn = (up.v != 0.0 || up.d[0] != 0.0) ? y[i + s] : 0.0;
+      double yn = (up.v != 0.0 || up.d[0] != 0.0 || up.d[1] != 0.0) ? y[i + s] : 0.0;
       r = lo * y[i - s] + di * y[i] + up * yn;
     }
-    out[i] = r.d[0];
+    outa[i] = r.d[0];
+    outt[i] = r.d[1];
   }
 }
 
+}  // namespace
+}  // namespace deb
+// Wave-wide unsigned 64-bit maximum from the ROCm device library (DPP reduction).
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(unsigned long long);
+namespace deb {
+namespace {
+
+// In-place Gauss-Jordan inversion of the MC x MC row-major matrix S (LDS), MC <= 64 known at
+// compile time, with partial pivoting.  The pivot search is one wave reduction over a 64-bit key
+// (|S[r][p]| with its six lowest mantissa bits replaced by 63 - r, so ties and near-ties go to
+// the first row); row interchanges leave the inverse with permuted columns, and the composite
+// permutation pi (inverse[:, j] = S[:, pi[j]]) is built across lanes with readlane.
+template <int MC>
+__device__ __forceinline__ void gauss_jordan_fixed(double *S, int *piv, int *pi, double *prow, double *colp, int lane) {
+  static_assert(MC <= WAVE, "one pivot-column entry per lane");
+  constexpr int NE = (MC * MC + WAVE - 1) / WAVE;
+  for (int p = 0; p < MC; ++p) {
+    unsigned long long key = 0;
+    if (lane >= p && lane < MC) {
+      const double v = fabs(S[lane * MC + p]);
+      key = ((unsigned long long)__double_as_longlong(v) & ~63ull) | (unsigned long long)(63 - lane);
+    }
+    key = __ockl_wfred_max_u64(key);  // all 64 lanes active here
+    const int bi = 63 - (int)(key & 63ull);
+    if (lane == 0) piv[p] = bi;
+    const double ip = 1.0 / S[bi * MC + p];
+    if (lane < MC) {
+      const int j = lane;
+      const double vp = S[p * MC + j], vb = S[bi * MC + j];
+      prow[j] = (j == p) ? ip : vb * ip;
+      S[bi * MC + j] = vp;                                       // old row p moves to the pivot slot
+      colp[j] = (j == bi) ? S[p * MC + p] : S[j * MC + p];       // column p, after the interchange
+    }
+    __syncthreads();
+#pragma unroll 9
+    for (int i = 0; i < NE; ++i) {
+      const int e = lane + i * WAVE;
+      if (e < MC * MC) {
+        const int r = e / MC, c = e - r * MC;
+        const double cr = colp[r], pc = prow[c];
+        double v = (c == p) ? -cr * ip : S[e] - cr * pc;
+        S[e] = (r == p) ? pc : v;
+      }
+    }
+    __syncthreads();
+  }
+  int cur = lane;  // pi[lane]
+  for (int p = MC - 1; p >= 0; --p) {
+    const int q = __builtin_amdgcn_readfirstlane(piv[p]);
+    const int at_q = __builtin_amdgcn_readlane(cur, q), at_p = __builtin_amdgcn_readlane(cur, p);
+    cur = (lane == p) ? at_q : ((lane == q) ? at_p : cur);
+  }
+  if (lane < MC) pi[lane] = cur;
+  __syncthreads();
+}
+
 // Jacobian pieces at (t0, y0) and the structured factorisation of W = I/(gamma dt) - J
-__device__ void factorise(const BgView &b, const PLay &L, double k, double t0, double dt, Ws &W, int lane, double &W00) {
+template <int MC>
+__device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double k, double t0, double dt, Ws &W, int lane, double &W00) {
   const int n = L.n, m = L.m, nch = L.nch;
   const double gi = 1.0 / (dt * r5::gamma);
-  rhs_dual(b, L, k, t0, W.y0, W.J0, W, lane, 0);
-  rhs_dual(b, L, k, t0, W.y0, W.dT, W, lane, 1);
+  rhs_dual(b, L, k, t0, W.y0, W.J0, W.dT, W, lane);
   PBg<double> g;
-  pbg_eval<double>(b, L, k, t0, W.y0[0], g);
+  pbg_eval<double>(b, L, k, t0, W.y0[0], g, W.sc, lane);
   // dense-block columns (lanes 0..m-1) and chain-head columns (lanes m..m+nch-1)
   for (int col = lane; col < m + nch; col += WAVE) {
     int s = (col < m) ? W.dmap[col] : W.chead[col - m];
@@ -422,61 +559,59 @@ __device__ void factorise(const BgView &b, const PLay &L, double k, double t0, d
   }
   W00 = gi - W.J0[0];
   __syncthreads();
-  // Gauss-Jordan inversion with partial pivoting; row swaps recorded in xd (as ints) for the
-  // final column unscrambling
-  int *perm = (int *)W.xd;
-  for (int p = 0; p < m; ++p) {
-    double best = -1.0;
-    int bi = p;
-    for (int r = p + lane; r < m; r += WAVE) {
-      double v = fabs(W.S[r * m + p]);
-      if (v > best) { best = v; bi = r; }
-    }
-    // wave argmax (first maximum)
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      double ob = __shfl_xor(best, o);
-      int oi = __shfl_xor(bi, o);
-      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-    }
-    if (lane == 0) perm[p] = bi;
-    if (bi != p)
+  // In-place Gauss-Jordan inversion with partial pivoting (first maximum).  Per pivot: one pass
+  // moves row p to the pivot row's slot and snapshots the scaled pivot row and column p, one
+  // element-parallel pass updates all m^2 entries.  Row interchanges leave the inverse with
+  // permuted columns; the composite permutation pi is applied to the right-hand side in wsolve.
+  if constexpr (MC > 0) {
+    gauss_jordan_fixed<MC>(W.S, W.piv, W.pi, W.u, W.f, lane);
+  } else {
+    double *prow = W.u, *colp = W.f;  // free during the factorisation
+    for (int p = 0; p < m; ++p) {
+      double best = -1.0;
+      int bi = p;
+      for (int r = p + lane; r < m; r += WAVE) {
+        double v = fabs(W.S[r * m + p]);
+        if (v > best) { best = v; bi = r; }
+      }
+  #pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        double ob = __shfl_xor(best, o);
+        int oi = __shfl_xor(bi, o);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      if (lane == 0) W.piv[p] = bi;
+      const double ip = 1.0 / W.S[bi * m + p];
       for (int j = lane; j < m; j += WAVE) {
-        double t = W.S[p * m + j];
-        W.S[p * m + j] = W.S[bi * m + j];
-        W.S[bi * m + j] = t;
+        double vp = W.S[p * m + j], vb = W.S[bi * m + j];
+        prow[j] = (j == p) ? ip : vb * ip;
+        W.S[bi * m + j] = vp;                                   // old row p moves to the pivot slot
+        colp[j] = (j == bi) ? W.S[p * m + p] : W.S[j * m + p];  // column p, after the interchange
       }
-    __syncthreads();
-    double piv = W.S[p * m + p];
-    double ip = 1.0 / piv;
-    __syncthreads();
-    for (int j = lane; j < m; j += WAVE) W.S[p * m + j] = (j == p) ? ip : W.S[p * m + j] * ip;
-    __syncthreads();
-    for (int r = lane; r < m; r += WAVE) {
-      if (r == p) continue;
-      double fct = W.S[r * m + p];
-      if (fct == 0.0) continue;
-      for (int j = 0; j < m; ++j) {
-        double pj = W.S[p * m + j];
-        W.S[r * m + j] = (j == p) ? -fct * pj : W.S[r * m + j] - fct * pj;
+      __syncthreads();
+      for (int e = lane; e < m * m; e += WAVE) {
+        int r = e / m, c = e - r * m;
+        double v;
+        if (r == p) v = prow[c];
+        else if (c == p) v = -colp[r] * ip;
+        else v = W.S[e] - colp[r] * prow[c];
+        W.S[e] = v;
+      }
+      __syncthreads();
+    }
+    if (lane == 0) {  // inverse[:, j] = S[:, pi[j]]
+      for (int j = 0; j < m; ++j) W.pi[j] = j;
+      for (int p = m - 1; p >= 0; --p) {
+        int q = W.piv[p];
+        int t = W.pi[p]; W.pi[p] = W.pi[q]; W.pi[q] = t;
       }
     }
-    __syncthreads();
-  }
-  for (int p = m - 1; p >= 0; --p) {
-    int bi = perm[p];
-    if (bi != p)
-      for (int r = lane; r < m; r += WAVE) {
-        double t = W.S[r * m + p];
-        W.S[r * m + p] = W.S[r * m + bi];
-        W.S[r * m + bi] = t;
-      }
     __syncthreads();
   }
 }
 
 // solve W x = v in place (v in LDS, length n)
-__device__ void wsolve(const PLay &L, Ws &W, double W00, double *v, int lane) {
+__device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double *v, int lane) {
   const int n = L.n, m = L.m, nch = L.nch;
   double x0 = v[0] / W00;
   __syncthreads();
@@ -496,7 +631,7 @@ __device__ void wsolve(const PLay &L, Ws &W, double W00, double *v, int lane) {
       int h = W.chead[c];
       acc += W.Jh[c * m + r] * (v[h] / W.td[h]);
     }
-    W.xd[r] = acc;
+    W.xd[W.pi[r]] = acc;
   }
   __syncthreads();
   for (int r = lane; r < m; r += WAVE) {
@@ -521,7 +656,7 @@ __device__ void wsolve(const PLay &L, Ws &W, double W00, double *v, int lane) {
 }
 
 // adiabatic initial conditions (perturbations.py:680-781), written by lane 0
-__device__ void initial_conditions(const BgView &b, const PLay &L, double tau, double k, double *y) {
+__device__ __forceinline__ void initial_conditions(const BgView &b, const PLay &L, double tau, double k, double *y) {
   const int n = L.n, nq = L.nq, iq0 = L.iq0;
   for (int i = 0; i < n; ++i) y[i] = 0.0;
   double a = b.a_of_tau(tau);
@@ -561,7 +696,7 @@ __device__ void initial_conditions(const BgView &b, const PLay &L, double tau, d
 }
 
 // determine_starting_time (perturbations.py:784-835): two 7-step bisections in log tau
-__device__ double starting_time(const BgView &b, double k) {
+__device__ __forceinline__ double starting_time(const BgView &b, double k) {
   const double small_k = 0.0004, large_k = 0.07;
   double tau0 = b.taumin, tau1 = b.tau_of_a(0.1);
   double tau_k = 1.0 / k;
@@ -589,7 +724,7 @@ __device__ double starting_time(const BgView &b, double k) {
 
 // RSA output projection (approximations.py:84-176) and output conversion
 // (perturbations.py:528-677) of the interpolated state y (LDS), written to dst
-__device__ void emit_output(const BgView &b, const PLay &L, double k, double tau, double *y, double *dst, int full, int lane) {
+__device__ __forceinline__ void emit_output(const BgView &b, const PLay &L, double k, double tau, double *y, double *dst, int full, int lane) {
   const int n = L.n, nq = L.nq, iq0 = L.iq0;
   const double a = y[0], loga = log(a);
   if (b.use_rsa) {
@@ -714,6 +849,7 @@ struct Pid {
   }
 };
 
+template <int NQ>
 __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
                                                        const double *__restrict__ unit_k, int n_units, const double *__restrict__ tau_out,
                                                        int nout, const double *__restrict__ nu_qw, DebProblem prob, DebSolverCfg cfg,
@@ -727,7 +863,8 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
   BgView b;
   b.init(tables + (long)cosmo * tstride);
   PLay L;
-  L.lmaxg = prob.lmaxg; L.lmaxgp = prob.lmaxgp; L.lmaxr = prob.lmaxr; L.lmaxnu = prob.lmaxnu; L.nq = prob.nqmax;
+  L.lmaxg = prob.lmaxg; L.lmaxgp = prob.lmaxgp; L.lmaxr = prob.lmaxr; L.lmaxnu = prob.lmaxnu;
+  L.nq = (NQ > 0) ? NQ : prob.nqmax;  // compile-time for the common bin counts
   L.n = 7 + (L.lmaxg + 1) + (L.lmaxgp + 1) + (L.lmaxr + 1) + L.nq * (L.lmaxnu + 1) + 2;
   L.idxg = 7; L.idxgp = 7 + (L.lmaxg + 1); L.idxr = 9 + L.lmaxg + L.lmaxgp; L.iq0 = 10 + L.lmaxg + L.lmaxgp + L.lmaxr;
   L.m = 17 + 3 * L.nq;
@@ -751,12 +888,15 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
   L.q = sq; L.w = sw; L.dlfdlq = sdl;
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
-  W.cpar = ip; ip += nch;
+  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m;
+  ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;  // 8-byte alignment for the cache
+  W.sc = (SplCache *)ip; ip += sizeof(SplCache) / 4;
   if ((long)((char *)ip - (char *)smem) > lds_bytes) {  // launch/carve-up mismatch: refuse to run
     if (lane == 0) { status[unit] = 3; n_acc[unit] = 0; n_rej[unit] = 0; }
     return;
   }
 
+  if (lane == 0) { W.sc->i = -1; W.sc->j = -1; }
   setup_maps(L, W, lane);
 
   // output times of this cosmology
@@ -781,7 +921,7 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
     if (steps >= cfg.max_steps) { st = 1; break; }
     const double dt = tnext - tprev;
     double W00;
-    factorise(b, L, k, tprev, dt, W, lane, W00);
+    factorise<(NQ > 0 ? 17 + 3 * NQ : 0)>(b, L, k, tprev, dt, W, lane, W00);
     using namespace r5;
     const double dtd[5] = {dt * d1, dt * d2, dt * d3, dt * d4, dt * d5};
     const double cs[5] = {0.0, c2, c3, c4, c5};
@@ -927,6 +1067,9 @@ __global__ __launch_bounds__(WAVE) void time_deriv_kernel(const double *__restri
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
   W.cpar = ip; ip += nch;
+  ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;
+  W.sc = (SplCache *)ip;
+  if (lane == 0) { W.sc->i = -1; W.sc->j = -1; }
   setup_maps(L, W, lane);
   for (int it = 0; it < n_tau; ++it) {
     const double *src = yin + ((long)ik * n_tau + it) * n;
@@ -958,8 +1101,8 @@ extern "C" long deb_perturb_lds_bytes(int lmaxg, int lmaxgp, int lmaxr, int lmax
   long n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + (long)nqmax * (lmaxnu + 1) + 2;
   long m = 17 + 3 * (long)nqmax, nch = 3 + nqmax;
   long dbl = n * 19 + m * m + nch * m + m + 3 * deb::NQMAX;
-  long ints = m + n + 4 * nch;
-  return dbl * 8 + ints * 4;
+  long ints = m + n + 4 * nch + 2 * m + 1;
+  return dbl * 8 + ints * 4 + (long)sizeof(deb::SplCache);
 }
 
 extern "C" int deb_launch_nu_bins(int nqmax, double *d_qw, hipStream_t stream) {
@@ -971,7 +1114,7 @@ extern "C" int deb_launch_time_deriv(const double *d_tables, const double *d_y, 
                                      int n_k, const double *d_nu_qw, const DebProblem *prob, double *d_dy, hipStream_t stream) {
   long n = 7 + (prob->lmaxg + 1) + (prob->lmaxgp + 1) + (prob->lmaxr + 1) + (long)prob->nqmax * (prob->lmaxnu + 1) + 2;
   long m = 17 + 3 * (long)prob->nqmax, nch = 3 + prob->nqmax;
-  long lds = (2 * n + 3 * deb::NQMAX) * 8 + (m + n + 4 * nch) * 4;
+  long lds = (2 * n + 3 * deb::NQMAX) * 8 + (m + n + 4 * nch + 1) * 4 + (long)sizeof(deb::SplCache);
   hipLaunchKernelGGL(deb::time_deriv_kernel, dim3(n_k), dim3(deb::WAVE), (size_t)lds, stream, d_tables, d_y, d_tau, n_tau, d_k, d_nu_qw,
                      *prob, d_dy);
   return (int)hipGetLastError();
@@ -993,11 +1136,18 @@ extern "C" int deb_launch_perturb(const double *d_tables, int num_thermo, const 
   long stride = deb_table_doubles(num_thermo);
   long lds = deb_perturb_lds_bytes(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
   if (lds > 160 * 1024) return -2;
+  auto kern = deb::perturb_kernel<0>;
+  switch (prob->nqmax) {
+    case 3: kern = deb::perturb_kernel<3>; break;
+    case 5: kern = deb::perturb_kernel<5>; break;
+    case 8: kern = deb::perturb_kernel<8>; break;
+    default: break;
+  }
   if (lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void *)deb::perturb_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return (int)e;
   }
-  hipLaunchKernelGGL(deb::perturb_kernel, dim3(n_units), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k,
-                     n_units, d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej);
+  hipLaunchKernelGGL(kern, dim3(n_units), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units,
+                     d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej);
   return (int)hipGetLastError();
 }
