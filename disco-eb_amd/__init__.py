@@ -7,6 +7,8 @@ include/discoeb_amd.h); this package is the Python host side.
 from . import _lib
 from .background import DeviceSpline, attach_background, background_tables, compute_background_quantities, evolve_background
 from .perturbations import compute_time_derivatives, evolve_perturbations, evolve_units, get_power, k_grid, nvar
+from .batch import PkGraph, autotune_schedule, evolve_many, power_units, tau_out_all
 
 __all__ = ["evolve_background", "background_tables", "attach_background", "compute_background_quantities", "DeviceSpline",
-           "evolve_perturbations", "evolve_units", "get_power", "compute_time_derivatives", "k_grid", "nvar"]
+           "evolve_perturbations", "evolve_units", "get_power", "compute_time_derivatives", "k_grid", "nvar", "evolve_many",
+           "power_units", "tau_out_all", "autotune_schedule", "PkGraph"]

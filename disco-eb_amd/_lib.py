@@ -45,7 +45,8 @@ SPLINES = ["tau_of_a", "a_of_tau", "xe_of_tau", "cs2a_of_tau", "tempba_of_tau", 
 
 # symbols declared in include/discoeb_amd.h
 EXPORTS = ["deb_version", "deb_last_error", "deb_tables_size", "deb_table_offset", "deb_background", "deb_spline_eval",
-           "deb_evolve_workspace_doubles", "deb_evolve", "deb_nvar", "deb_power", "deb_time_derivatives"]
+           "deb_evolve_workspace_doubles", "deb_evolve", "deb_tau_out", "deb_nvar", "deb_power", "deb_power_units",
+           "deb_time_derivatives"]
 
 
 class DebProblem(ctypes.Structure):
@@ -57,6 +58,11 @@ class DebSolverCfg(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("pcoeff", ctypes.c_double),
                 ("icoeff", ctypes.c_double), ("dcoeff", ctypes.c_double), ("factormax", ctypes.c_double),
                 ("factormin", ctypes.c_double), ("max_steps", ctypes.c_int)]
+
+
+class DebSchedule(ctypes.Structure):
+    _fields_ = [("persistent", ctypes.c_int), ("chunk", ctypes.c_int), ("n_workgroups", ctypes.c_int),
+                ("reserved", ctypes.c_int), ("d_order", ctypes.c_void_p)]
 
 
 _lib = None
@@ -83,7 +89,9 @@ def lib():
     L.deb_background.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
     L.deb_spline_eval.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
     L.deb_evolve.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.POINTER(DebProblem),
-                             ctypes.POINTER(DebSolverCfg), _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+                             ctypes.POINTER(DebSolverCfg), _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(DebSchedule), _vp]
+    L.deb_tau_out.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
+    L.deb_power_units.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
     L.deb_power.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                             ctypes.c_double, ctypes.c_double, _vp, _vp]
     L.deb_time_derivatives.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int,

@@ -9,8 +9,12 @@
 
 extern "C" int deb_launch_background(const double *, int, int, double *, hipStream_t);
 extern "C" int deb_launch_nu_bins(int, double *, hipStream_t);
+extern "C" int deb_launch_tau_out(const double *, int, int, const double *, int, double *, hipStream_t);
+extern "C" int deb_launch_power_units(const double *, int, const int *, const double *, int, int, const double *, int, int, double *,
+                                      hipStream_t);
 extern "C" int deb_launch_perturb(const double *, int, const int *, const double *, int, const double *, int, const double *,
-                                  const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, hipStream_t);
+                                  const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, int *, const int *, int,
+                                  int, hipStream_t);
 extern "C" long deb_perturb_lds_bytes(int, int, int, int, int);
 extern "C" int deb_launch_spline_eval(const double *, int, int, int, const double *, int, double *, hipStream_t);
 extern "C" int deb_launch_power(const double *, int, const double *, long, int, int, double, double, double, double *, hipStream_t);
@@ -47,7 +51,7 @@ const char *deb_version(void) { return "discoeb_amd 0.1 (gfx950)"; }
 const char *deb_last_error(void) { return g_err.c_str(); }
 long deb_tables_size(int num_thermo) { return deb_table_doubles(num_thermo); }
 long deb_table_offset(int slot, int num_thermo) { return deb_array_offset(slot, num_thermo); }
-long deb_evolve_workspace_doubles(void) { return 32; }
+long deb_evolve_workspace_doubles(void) { return 64; }  // momentum bins (32) + work-queue counter
 
 int deb_nvar(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
   return 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + nqmax * (lmaxnu + 1) + 2;
@@ -73,7 +77,7 @@ int deb_spline_eval(const double *d_tables, int num_thermo, int cosmo, int splin
 
 int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
                const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
-               int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, void *stream) {
+               int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const DebSchedule *sched, void *stream) {
   int rc = check_problem(prob);
   if (rc) return rc;
   if (!cfg) return fail(-1, "solver descriptor is NULL");
@@ -82,6 +86,8 @@ int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, 
   if (!(cfg->factormin > 0) || !(cfg->factormax >= 1)) return fail(-1, "factormin must be > 0 and factormax >= 1");
   if (nout < 1) return fail(-1, "at least one output time is required");
   if (n_units < 0) return fail(-1, "n_units must be >= 0");
+  if (sched && sched->persistent && sched->chunk < 1) return fail(-1, "schedule chunk must be >= 1");
+  if (sched && sched->persistent && sched->n_workgroups < 0) return fail(-1, "schedule n_workgroups must be >= 0");
   if (n_units == 0) return 0;
   if (!d_tables || !d_unit_cosmo || !d_unit_k || !d_tau_out || !d_y_out || !d_status || !d_n_accept || !d_n_reject || !d_work)
     return fail(-1, "NULL device pointer");
@@ -89,16 +95,43 @@ int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, 
   if (lds > 160 * 1024) return fail(-1, "state too large for the 160 KiB LDS of one CU (lower the hierarchy truncations)");
   rc = deb_launch_nu_bins(prob->nqmax, d_work, (hipStream_t)stream);
   if (rc) return hip_rc(rc, "deb_evolve (momentum bins)");
+  int *queue = nullptr;
+  const int *order = nullptr;
+  int chunk = 1, nwg = 0;
+  if (sched && sched->persistent) {
+    queue = (int *)(d_work + 32);
+    order = sched->d_order;
+    chunk = sched->chunk;
+    nwg = sched->n_workgroups;
+  }
   rc = deb_launch_perturb(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, d_work, prob, cfg, d_step_log, d_y_out,
-                          d_status, d_n_accept, d_n_reject, (hipStream_t)stream);
+                          d_status, d_n_accept, d_n_reject, queue, order, chunk, nwg, (hipStream_t)stream);
   if (rc == -2) return fail(-1, "state too large for LDS");
   return hip_rc(rc, "deb_evolve");
+}
+
+int deb_tau_out(const double *d_tables, int num_thermo, int n_cosmo, const double *d_aexp_out, int nout, double *d_tau_out, void *stream) {
+  if (n_cosmo < 0 || nout < 0) return fail(-1, "n_cosmo and nout must be >= 0");
+  if (n_cosmo == 0 || nout == 0) return 0;
+  if (!d_tables || !d_aexp_out || !d_tau_out) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_tau_out(d_tables, num_thermo, n_cosmo, d_aexp_out, nout, d_tau_out, (hipStream_t)stream), "deb_tau_out");
 }
 
 int deb_power(const double *d_k, int k_repeat, const double *d_y, long n, int ystride, int idx, double A_s, double n_s, double k_p,
               double *d_pk, void *stream) {
   if (k_repeat < 1 || ystride < 1 || idx < 0 || idx >= ystride) return fail(-1, "invalid stride, repeat or field index");
   return hip_rc(deb_launch_power(d_k, k_repeat, d_y, n, ystride, idx, A_s, n_s, k_p, d_pk, (hipStream_t)stream), "deb_power");
+}
+
+int deb_power_units(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units, int nout,
+                    const double *d_y, int ystride, int idx, double *d_pk, void *stream) {
+  if (n_units < 0 || nout < 0) return fail(-1, "n_units and nout must be >= 0");
+  if (ystride < 1 || idx < 0 || idx >= ystride) return fail(-1, "invalid stride or field index");
+  if ((long)n_units * nout == 0) return 0;
+  if (!d_tables || !d_unit_cosmo || !d_unit_k || !d_y || !d_pk) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_power_units(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, nout, d_y, ystride, idx, d_pk,
+                                       (hipStream_t)stream),
+                "deb_power_units");
 }
 
 int deb_time_derivatives(const double *d_tables, int num_thermo, int cosmo, const double *d_y, const double *d_tau, int n_tau,

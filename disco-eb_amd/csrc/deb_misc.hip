@@ -39,8 +39,32 @@ __global__ void power_kernel(const double *__restrict__ k, int krep, const doubl
   pk[i] = 2.0 * 3.141592653589793 * 3.141592653589793 * A_s * pow(kv / k_p, n_s - 1.0) * pow(kv, -3.0) * d * d;
 }
 
+// P(k) of every (unit, output time) row, A_s, n_s and k_p taken from the unit's cosmology
+// (the parameter copy at the head of its table block)
+__global__ void power_units_kernel(const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
+                                   const double *__restrict__ unit_k, long n_rows, int nout, const double *__restrict__ y, int ystride,
+                                   int idx, double *__restrict__ pk) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_rows) return;
+  long u = i / nout;
+  const double *hdr = tables + (long)unit_cosmo[u] * tstride;
+  double kv = unit_k[u];
+  double d = y[i * ystride + idx];
+  pk[i] = 2.0 * 3.141592653589793 * 3.141592653589793 * hdr[DEB_P_AS] * pow(kv / hdr[DEB_P_KP], hdr[DEB_P_NS] - 1.0) * pow(kv, -3.0) * d *
+          d;
+}
+
 }  // namespace
 }  // namespace deb
+
+extern "C" int deb_launch_power_units(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
+                                      int nout, const double *d_y, int ystride, int idx, double *d_pk, hipStream_t stream) {
+  long rows = (long)n_units * nout;
+  if (rows <= 0) return 0;
+  hipLaunchKernelGGL(deb::power_units_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, stream, d_tables,
+                     deb_table_doubles(num_thermo), d_unit_cosmo, d_unit_k, rows, nout, d_y, ystride, idx, d_pk);
+  return (int)hipGetLastError();
+}
 
 extern "C" int deb_launch_spline_eval(const double *d_tables, int N, int sid, int kind, const double *d_x, int nx, double *d_out,
                                       hipStream_t stream) {

@@ -13,6 +13,15 @@ typedef struct DebSolverCfg {
   double rtol, atol, pcoeff, icoeff, dcoeff, factormax, factormin;
   int max_steps;
 } DebSolverCfg;
+
+/* How units are spread over workgroups (one wavefront each).  persistent = 0: one workgroup
+ * per unit.  persistent = 1: n_workgroups resident workgroups (0: as many as fit on the
+ * device) claim `chunk` units at a time from a device counter, in the order d_order
+ * (device array of n_units unit indices, NULL: 0..n_units-1). */
+typedef struct DebSchedule {
+  int persistent, chunk, n_workgroups, reserved;
+  const int *d_order;
+} DebSchedule;
 #ifdef __cplusplus
 }
 #endif

@@ -849,55 +849,32 @@ struct Pid {
   }
 };
 
+// One (cosmology, k) unit from its initial conditions to the last output time.  The LDS
+// workspace W and the layout L (with its momentum-bin tables) are set up by the caller.
 template <int NQ>
-__global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
-                                                       const double *__restrict__ unit_k, int n_units, const double *__restrict__ tau_out,
-                                                       int nout, const double *__restrict__ nu_qw, DebProblem prob, DebSolverCfg cfg,
-                                                       long lds_bytes, double *__restrict__ step_log, double *__restrict__ y_out,
-                                                       int *__restrict__ status, int *__restrict__ n_acc, int *__restrict__ n_rej) {
-  const int unit = blockIdx.x;
-  if (unit >= n_units) return;
-  const int lane = threadIdx.x;
+__device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
+                                           const double *__restrict__ unit_k, const double *__restrict__ tau_out, int nout,
+                                           const DebProblem &prob, const DebSolverCfg &cfg, double *__restrict__ step_log,
+                                           double *__restrict__ y_out, int *__restrict__ status, int *__restrict__ n_acc,
+                                           int *__restrict__ n_rej, const PLay &L, Ws &W, int lane) {
   const int cosmo = unit_cosmo[unit];
   const double k = unit_k[unit];
+  const int n = L.n;
   BgView b;
   b.init(tables + (long)cosmo * tstride);
-  PLay L;
-  L.lmaxg = prob.lmaxg; L.lmaxgp = prob.lmaxgp; L.lmaxr = prob.lmaxr; L.lmaxnu = prob.lmaxnu;
-  L.nq = (NQ > 0) ? NQ : prob.nqmax;  // compile-time for the common bin counts
-  L.n = 7 + (L.lmaxg + 1) + (L.lmaxgp + 1) + (L.lmaxr + 1) + L.nq * (L.lmaxnu + 1) + 2;
-  L.idxg = 7; L.idxgp = 7 + (L.lmaxg + 1); L.idxr = 9 + L.lmaxg + L.lmaxgp; L.iq0 = 10 + L.lmaxg + L.lmaxgp + L.lmaxr;
-  L.m = 17 + 3 * L.nq;
-  L.nch = 3 + L.nq;
-  const int n = L.n, m = L.m, nch = L.nch;
-
-  extern __shared__ double smem[];
-  Ws W;
-  double *p = smem;
-  // carve-up must match deb_perturb_lds_bytes(): 19 n + m^2 + nch m + m + 3 NQMAX doubles, then
-  // m + n + 4 nch ints
-  W.y0 = p; p += n; W.y1 = p; p += n; W.u = p; p += n; W.f = p; p += n; W.J0 = p; p += n; W.dT = p; p += n;
-  W.K = p; p += 8 * n; W.tlo = p; p += n; W.tdi = p; p += n; W.tup = p; p += n; W.tmul = p; p += n; W.td = p; p += n;
-  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m;
-  double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
-  for (int q = lane; q < L.nq; q += WAVE) {
-    sq[q] = nu_qw[q];
-    sw[q] = nu_qw[NQMAX + q];
-    sdl[q] = -nu_qw[q] / (1.0 + exp(-nu_qw[q]));
-  }
-  L.q = sq; L.w = sw; L.dlfdlq = sdl;
-  int *ip = (int *)p;
-  W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
-  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m;
-  ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;  // 8-byte alignment for the cache
-  W.sc = (SplCache *)ip; ip += sizeof(SplCache) / 4;
-  if ((long)((char *)ip - (char *)smem) > lds_bytes) {  // launch/carve-up mismatch: refuse to run
-    if (lane == 0) { status[unit] = 3; n_acc[unit] = 0; n_rej[unit] = 0; }
+  __syncthreads();  // the previous unit's last LDS reads are done
+  if (lane == 0) { W.sc->i = -1; W.sc->j = -1; }
+  __syncthreads();
+  const int wout = prob.return_full ? n : 20;
+  double *yo = y_out + (long)unit * nout * wout;
+  // w(a) = w0 + wa (1 - a) reaching -1 for some a in (0, 1] (the phantom divide) makes the
+  // fluid equations singular (their 1 / (1 + w) terms): such units are not integrated
+  // (status 4, NaN outputs).  w0 = -1, wa = 0 is the cosmological constant, handled exactly.
+  if (!b.lambda_de && (1.0 + b.w0) * (1.0 + b.w0 + b.wa) <= 0.0) {
+    for (int i = lane; i < nout * wout; i += WAVE) yo[i] = __builtin_nan("");
+    if (lane == 0) { status[unit] = 4; n_acc[unit] = 0; n_rej[unit] = 0; }
     return;
   }
-
-  if (lane == 0) { W.sc->i = -1; W.sc->j = -1; }
-  setup_maps(L, W, lane);
 
   // output times of this cosmology
   const double *tout = tau_out + (long)cosmo * nout;
@@ -907,8 +884,6 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
   if (lane == 0) initial_conditions(b, L, t0, k, W.y0);
   __syncthreads();
 
-  const int wout = prob.return_full ? n : 20;
-  double *yo = y_out + (long)unit * nout * wout;
   double tprev = t0;
   double tnext = fmin(t0 + fmin(t0 / 4.0, 0.5 * (t1 - t0)), t1);
   Pid pid;
@@ -1037,6 +1012,78 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
   }
 }
 
+// Work distribution: without a queue, workgroup g solves unit g (the grid covers n_units).
+// With a queue (a device counter set to 0 before the launch), each workgroup repeatedly claims
+// the next `chunk` positions of the unit order with one atomic add until the order is
+// exhausted; order[] (optional) lists units most expensive first so the tail stays short.
+template <int NQ, bool QUEUE>
+__global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
+                                                       const double *__restrict__ unit_k, int n_units, const double *__restrict__ tau_out,
+                                                       int nout, const double *__restrict__ nu_qw, DebProblem prob, DebSolverCfg cfg,
+                                                       long lds_bytes, double *__restrict__ step_log, double *__restrict__ y_out,
+                                                       int *__restrict__ status, int *__restrict__ n_acc, int *__restrict__ n_rej,
+                                                       int *__restrict__ queue, const int *__restrict__ order, int chunk) {
+  if (!QUEUE && (int)blockIdx.x >= n_units) return;
+  const int lane = threadIdx.x;
+  PLay L;
+  L.lmaxg = prob.lmaxg; L.lmaxgp = prob.lmaxgp; L.lmaxr = prob.lmaxr; L.lmaxnu = prob.lmaxnu;
+  L.nq = (NQ > 0) ? NQ : prob.nqmax;  // compile-time for the common bin counts
+  L.n = 7 + (L.lmaxg + 1) + (L.lmaxgp + 1) + (L.lmaxr + 1) + L.nq * (L.lmaxnu + 1) + 2;
+  L.idxg = 7; L.idxgp = 7 + (L.lmaxg + 1); L.idxr = 9 + L.lmaxg + L.lmaxgp; L.iq0 = 10 + L.lmaxg + L.lmaxgp + L.lmaxr;
+  L.m = 17 + 3 * L.nq;
+  L.nch = 3 + L.nq;
+  const int n = L.n, m = L.m, nch = L.nch;
+
+  extern __shared__ double smem[];
+  Ws W;
+  double *p = smem;
+  // carve-up must match deb_perturb_lds_bytes(): 19 n + m^2 + nch m + m + 3 NQMAX doubles, then
+  // m + n + 4 nch ints
+  W.y0 = p; p += n; W.y1 = p; p += n; W.u = p; p += n; W.f = p; p += n; W.J0 = p; p += n; W.dT = p; p += n;
+  W.K = p; p += 8 * n; W.tlo = p; p += n; W.tdi = p; p += n; W.tup = p; p += n; W.tmul = p; p += n; W.td = p; p += n;
+  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m;
+  double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
+  for (int q = lane; q < L.nq; q += WAVE) {
+    sq[q] = nu_qw[q];
+    sw[q] = nu_qw[NQMAX + q];
+    sdl[q] = -nu_qw[q] / (1.0 + exp(-nu_qw[q]));
+  }
+  L.q = sq; L.w = sw; L.dlfdlq = sdl;
+  int *ip = (int *)p;
+  W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
+  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m;
+  ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;  // 8-byte alignment for the cache
+  W.sc = (SplCache *)ip; ip += sizeof(SplCache) / 4;
+  if ((long)((char *)ip - (char *)smem) > lds_bytes) {  // launch/carve-up mismatch: refuse to run
+    for (int u = blockIdx.x * WAVE + lane; u < n_units; u += gridDim.x * WAVE) { status[u] = 3; n_acc[u] = 0; n_rej[u] = 0; }
+    return;
+  }
+
+  setup_maps(L, W, lane);
+
+  if constexpr (!QUEUE) {
+    solve_unit<NQ>((int)blockIdx.x, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, y_out, status, n_acc, n_rej, L,
+                   W, lane);
+    return;
+  }
+  for (;;) {
+    int first = 0;
+    if (lane == 0) first = atomicAdd(queue, chunk);
+    first = __shfl(first, 0);
+    if (first >= n_units) break;  // every workgroup ends here once the counter passes n_units
+    const int last = min(first + chunk, n_units);
+    for (int pos = first; pos < last; ++pos) {
+      const int unit = order ? order[pos] : pos;
+      solve_unit<NQ>(unit, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, y_out, status, n_acc, n_rej, L, W,
+                     lane);
+    }
+  }
+}
+
+__global__ void queue_reset_kernel(int *queue) {
+  if (threadIdx.x == 0) *queue = 0;
+}
+
 // dy/dtau at saved states (compute_time_derivatives, perturbations.py:1339-1406): one wave per
 // k-mode, looping over the output times
 __global__ __launch_bounds__(WAVE) void time_deriv_kernel(const double *__restrict__ tables, const double *__restrict__ yin,
@@ -1131,23 +1178,40 @@ extern "C" int deb_launch_tau_out(const double *d_tables, int num_thermo, int n_
 
 extern "C" int deb_launch_perturb(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
                                   const double *d_tau_out, int nout, const double *d_nu_qw, const DebProblem *prob,
-                                  const DebSolverCfg *cfg, double *d_step_log, double *d_y_out,
-                                  int *d_status, int *d_nacc, int *d_nrej, hipStream_t stream) {
+                                  const DebSolverCfg *cfg, double *d_step_log, double *d_y_out, int *d_status, int *d_nacc, int *d_nrej,
+                                  int *d_queue, const int *d_order, int chunk, int n_workgroups, hipStream_t stream) {
   long stride = deb_table_doubles(num_thermo);
   long lds = deb_perturb_lds_bytes(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
   if (lds > 160 * 1024) return -2;
-  auto kern = deb::perturb_kernel<0>;
+  // the static and the queue form are separate kernels, each with one inlined copy of the unit
+  // solve (one copy per kernel keeps register allocation and code size those of a single solve)
+  const bool q = d_queue != nullptr;
+  auto kern = q ? deb::perturb_kernel<0, true> : deb::perturb_kernel<0, false>;
   switch (prob->nqmax) {
-    case 3: kern = deb::perturb_kernel<3>; break;
-    case 5: kern = deb::perturb_kernel<5>; break;
-    case 8: kern = deb::perturb_kernel<8>; break;
+    case 3: kern = q ? deb::perturb_kernel<3, true> : deb::perturb_kernel<3, false>; break;
+    case 5: kern = q ? deb::perturb_kernel<5, true> : deb::perturb_kernel<5, false>; break;
+    case 8: kern = q ? deb::perturb_kernel<8, true> : deb::perturb_kernel<8, false>; break;
     default: break;
   }
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return (int)e;
   }
-  hipLaunchKernelGGL(kern, dim3(n_units), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units,
-                     d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej);
+  int grid = n_units;
+  if (d_queue) {
+    if (n_workgroups <= 0) {  // one resident wave per slot the register and LDS budgets allow
+      int dev = 0, ncu = 0, per_cu = 0;
+      hipError_t e = hipGetDevice(&dev);
+      if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+      if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kern, deb::WAVE, (size_t)lds);
+      if (e != hipSuccess) return (int)e;
+      n_workgroups = ncu * (per_cu > 0 ? per_cu : 1);
+    }
+    grid = n_workgroups < n_units ? n_workgroups : n_units;
+    hipLaunchKernelGGL(deb::queue_reset_kernel, dim3(1), dim3(1), 0, stream, d_queue);
+  }
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units,
+                     d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej, d_queue, d_order,
+                     chunk > 0 ? chunk : 1);
   return (int)hipGetLastError();
 }

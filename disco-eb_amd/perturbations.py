@@ -62,11 +62,14 @@ def nvar(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax):
 
 def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3,
                  rtol=1e-4, atol=1e-4, pcoeff=0.25, icoeff=0.80, dcoeff=0.0, factormax=20.0, factormin=0.3, max_steps=2048,
-                 return_full=False, out=None, step_log=False):
+                 return_full=False, out=None, step_log=False, schedule=None):
     """Low-level launch: independent adaptive solves of (cosmology, k) units.
 
     tables: (n_cosmo, size) device tensor from background_tables; unit_cosmo (int32) and unit_k
     (float64) device tensors of length n_units; tau_out (n_cosmo, nout) device tensor.
+    schedule: None (one workgroup per unit) or a dict with keys chunk (units claimed per
+    queue access, default 1), n_workgroups (persistent workgroups, 0 = as many as fit) and
+    order (None, "k_desc" = largest k first, or an int32 device tensor of unit indices).
     Returns (y, status, n_accept, n_reject) device tensors, plus the step mesh
     (n_units, max_steps, 3: t_prev, t_next, error norm) when step_log is set; asynchronous on
     the current stream."""
@@ -88,22 +91,53 @@ def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, l
     uk = unit_k.to(device=dev, dtype=torch.float64).contiguous()
     to = tau_out.to(device=dev, dtype=torch.float64).contiguous()
     log = torch.full((n_units, int(max_steps), 3), float("nan"), dtype=torch.float64, device=dev) if step_log else None
+    sched, order = _schedule(schedule, uk)
     _lib.check(L.deb_evolve(_lib.ptr(tables), int(num_thermo), _lib.ptr(uc), _lib.ptr(uk), n_units, _lib.ptr(to), nout,
                             ctypes.byref(prob), ctypes.byref(cfg), _lib.ptr(y), _lib.ptr(status), _lib.ptr(nacc),
-                            _lib.ptr(nrej), _lib.ptr(work), _lib.ptr(log) if log is not None else None, _lib.stream()))
+                            _lib.ptr(nrej), _lib.ptr(work), _lib.ptr(log) if log is not None else None,
+                            ctypes.byref(sched) if sched is not None else None, _lib.stream()))
+    del order  # stream-ordered: the allocator reuses it only after the launch has run
     if step_log:
         return y, status, nacc, nrej, log
     return y, status, nacc, nrej
 
 
-def _raise_on_status(status):
+def _schedule(schedule, uk):
+    """DebSchedule for a schedule dict (see evolve_units) and the order tensor it points to."""
+    if schedule is None:
+        return None, None
+    order = schedule.get("order")
+    if isinstance(order, str):
+        if order != "k_desc":
+            raise ValueError(f"unknown unit order {order!r} (expected 'k_desc', None or a tensor)")
+        order = torch.argsort(uk, descending=True, stable=True).to(torch.int32)
+    elif order is not None:
+        order = torch.as_tensor(order, dtype=torch.int32, device=uk.device).contiguous()
+        if order.numel() != uk.numel():
+            raise ValueError("order must list every unit once")
+        if not torch.cuda.is_current_stream_capturing():  # indices reach the kernel unchecked
+            srt = torch.sort(order.long()).values.cpu()
+            if not torch.equal(srt, torch.arange(uk.numel())):
+                raise ValueError("order must be a permutation of 0..n_units-1")
+    chunk, nwg = int(schedule.get("chunk", 1)), int(schedule.get("n_workgroups", 0))
+    if chunk < 1 or nwg < 0:
+        raise ValueError("schedule chunk must be >= 1 and n_workgroups >= 0")
+    return _lib.DebSchedule(1, chunk, nwg, 0, order.data_ptr() if order is not None else None), order
+
+
+def _raise_on_status(status, allow_phantom=False):
     st = status.cpu()
+    if allow_phantom:
+        st = torch.where(st == 4, torch.zeros_like(st), st)
     if (st == 1).any():
         raise RuntimeError("The maximum number of solver steps was reached. Try increasing `max_steps`.")
     if (st == 2).any():
         raise RuntimeError("Non-finite error estimate in the perturbation solve.")
     if (st == 3).any():
         raise RuntimeError("internal error: LDS carve-up exceeds the launch size")
+    if (st == 4).any():
+        raise ValueError("w(a) = w_DE_0 + w_DE_a (1 - a) crosses -1 in (0, 1]: the dark-energy fluid equations are singular "
+                         "at the phantom divide (use w_DE_0 = -1, w_DE_a = 0 for a cosmological constant)")
 
 
 def evolve_perturbations(*, param, aexp_out, kmin: float, kmax: float, num_k: int, lmaxg: int = 11, lmaxgp: int = 11,

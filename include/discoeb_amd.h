@@ -47,23 +47,33 @@ enum {
 int deb_spline_eval(const double *d_tables, int num_thermo, int cosmo, int spline_id, int kind, const double *d_x, int nx,
                     double *d_out, void *stream);
 
-/* Doubles of device workspace deb_evolve needs (neutrino momentum bins). */
+/* Doubles of device workspace deb_evolve needs (neutrino momentum bins and the work-queue
+ * counter). */
 long deb_evolve_workspace_doubles(void);
 
 /* Adaptive Rodas5 evolution of n_units independent (cosmology, k) modes.
  * d_unit_cosmo[u], d_unit_k[u]: cosmology index and wavenumber [1/Mpc] of unit u.
  * d_tau_out: n_cosmo x nout conformal output times (ascending per cosmology).
  * d_y_out: n_units x nout x (return_full ? nvar : 20) doubles, written.
- * d_status[u]: 0 ok, 1 max_steps exceeded, 2 non-finite error norm (outputs NaN from there).
+ * d_status[u]: 0 ok, 1 max_steps exceeded, 2 non-finite error norm (outputs NaN from there),
+ * 4 w(a) = w0 + wa (1 - a) crosses -1 in (0, 1] (singular fluid equations; not integrated, NaN outputs).
  * d_n_accept / d_n_reject: accepted / rejected step counts per unit.
  * d_work: deb_evolve_workspace_doubles() doubles.
  * d_step_log: NULL, or n_units x max_steps x 3 doubles receiving (t_prev, t_next, error norm)
  * of every attempted step (the step mesh, for replay checks and frozen-mesh derivatives).
+ * sched: NULL (one workgroup per unit) or a DebSchedule (deb_perturb.h): persistent
+ * workgroups fed by a device work queue, optionally in a caller-given cost order.  Results
+ * do not depend on the schedule.
  * Replaces evolve_perturbations (src/discoeb/perturbations.py:1124-1226) with
  * evolve_one_mode (:882-949) per unit. */
 int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
                const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
-               int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, void *stream);
+               int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const DebSchedule *sched, void *stream);
+
+/* Conformal output times tau_out[c, j] = tau(a = d_aexp_out[j]) of every cosmology c, from its
+ * tau_of_a spline; d_tau_out: n_cosmo x nout doubles.
+ * Replaces tau_out = param['tau_of_a_spline'].evaluate(aexp_out) (perturbations.py:1200-1206). */
+int deb_tau_out(const double *d_tables, int num_thermo, int n_cosmo, const double *d_aexp_out, int nout, double *d_tau_out, void *stream);
 
 /* Number of state variables for the given truncations (perturbations.py:893). */
 int deb_nvar(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax);
@@ -73,6 +83,13 @@ int deb_nvar(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax);
  * Replaces get_power (src/discoeb/perturbations.py:1446-1468). */
 int deb_power(const double *d_k, int k_repeat, const double *d_y, long n, int ystride, int idx, double A_s, double n_s, double k_p,
               double *d_pk, void *stream);
+
+/* P(k) for the output of deb_evolve over many cosmologies: d_pk[u, j] from d_y[u, j, idx]
+ * (row stride ystride) with k = d_unit_k[u] and A_s, n_s, k_p of cosmology d_unit_cosmo[u]
+ * read from its table block, so a captured graph stays valid when parameters change.
+ * Replaces get_power (src/discoeb/perturbations.py:1446-1468) applied per cosmology. */
+int deb_power_units(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units, int nout,
+                    const double *d_y, int ystride, int idx, double *d_pk, void *stream);
 
 /* dy/dtau re-evaluated at saved states: d_y is n_k x n_tau x nvar, d_tau (n_tau), d_k (n_k);
  * d_dy has the same shape.  Replaces compute_time_derivatives

@@ -40,12 +40,18 @@ def test_argument_errors_without_gpu(pkg):
     L = pkg._lib.lib()
     prob = pkg._lib.DebProblem(2, 11, 11, 8, 3, 0)
     cfg = pkg._lib.DebSolverCfg(1e-4, 1e-4, 0.25, 0.8, 0.0, 20.0, 0.3, 2048)
-    rc = L.deb_evolve(None, 256, None, None, 4, None, 1, ctypes.byref(prob), ctypes.byref(cfg), None, None, None, None, None, None, None)
+    rc = L.deb_evolve(None, 256, None, None, 4, None, 1, ctypes.byref(prob), ctypes.byref(cfg), None, None, None, None, None, None, None, None)
     assert rc < 0 and b">= 3" in L.deb_last_error()
     prob = pkg._lib.DebProblem(11, 11, 11, 8, 17, 0)
-    rc = L.deb_evolve(None, 256, None, None, 4, None, 1, ctypes.byref(prob), ctypes.byref(cfg), None, None, None, None, None, None, None)
+    rc = L.deb_evolve(None, 256, None, None, 4, None, 1, ctypes.byref(prob), ctypes.byref(cfg), None, None, None, None, None, None, None, None)
     assert rc < 0 and b"nqmax" in L.deb_last_error()
+    prob = pkg._lib.DebProblem(11, 11, 11, 8, 3, 0)
+    sched = pkg._lib.DebSchedule(1, 0, 0, 0, None)
+    rc = L.deb_evolve(None, 256, None, None, 4, None, 1, ctypes.byref(prob), ctypes.byref(cfg), None, None, None, None, None, None,
+                      ctypes.byref(sched), None)
+    assert rc < 0 and b"chunk" in L.deb_last_error()
     assert L.deb_background(None, 1, 4, None, None) < 0
+    assert L.deb_tau_out(None, 256, 1, None, 3, None, None) < 0
 
 
 def test_k_grid_matches_reference_rules(pkg, oracle):

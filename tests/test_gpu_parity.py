@@ -1,6 +1,6 @@
 """HIP path vs the CPU oracle on identical inputs (run on an MI355X: pytest -m gpu).
 
-Tolerances: the north-star bar is 1e-5 relative for transfer functions and P(k); both sides
+Tolerances: the bar is 1e-5 relative for transfer functions and P(k); both sides
 integrate the same Rodas5 step sequence in FP64, so agreement is normally far tighter and the
 measured error is printed for each case.
 """
@@ -12,7 +12,7 @@ from conftest import disco_notebook, load_package
 
 pytestmark = pytest.mark.gpu
 
-TOL_TRANSFER = 1e-5  # relative, north-star tolerance for transfer functions and P(k)
+TOL_TRANSFER = 1e-5  # relative tolerance for transfer functions and P(k)
 
 
 @pytest.fixture(scope="module")
@@ -22,16 +22,16 @@ def deb():
     return load_package()
 
 
-def cfg0_param():
+def lcdm_param():
     """Flat LCDM with massless neutrinos (Neff = 3.046): omega_c h^2 = 0.120, omega_b h^2 = 0.0224, h = 0.674."""
     h = 0.674
     return dict(Omegam=(0.120 + 0.0224) / h ** 2, Omegab=0.0224 / h ** 2, H0=100 * h, Tcmb=2.7255, YHe=0.248, Neff=3.046,
                 Nmnu=0.0, mnu=0.0, w_DE_0=-1.0, w_DE_a=0.0, cs2_DE=1.0, Omegak=0.0, A_s=2.1e-9, n_s=0.965, k_p=0.05)
 
 
-def cfg1_param():
-    """cfg0 plus one massive neutrino (0.06 eV) and a w0wa fluid (w0 = -0.95, wa = 0.1)."""
-    p = cfg0_param()
+def nu_w0wa_param():
+    """The flat LCDM cosmology plus one massive neutrino (0.06 eV) and a w0wa fluid (w0 = -0.95, wa = 0.1)."""
+    p = lcdm_param()
     p.update(Neff=2.046, Nmnu=1.0, mnu=0.06, w_DE_0=-0.95, w_DE_a=0.1)
     return p
 
@@ -87,9 +87,9 @@ def test_background_tables_match_oracle(deb, oracle, golden, num_thermo):
         np.testing.assert_allclose(xe, g["exp_xe"], rtol=1e-9)
 
 
-@pytest.mark.parametrize("which", ["cfg0", "cfg1"])
-def test_background_config_cosmologies(deb, oracle, which):
-    P = cfg0_param() if which == "cfg0" else cfg1_param()
+@pytest.mark.parametrize("which", ["lcdm", "nu_w0wa"])
+def test_background_two_cosmologies(deb, oracle, which):
+    P = lcdm_param() if which == "lcdm" else nu_w0wa_param()
     print(f"case: background {which}", flush=True)
     p = deb.evolve_background(param=dict(P), num_thermo=256)
     bg = oracle.Background(P, 256)
@@ -145,15 +145,15 @@ def _check_replay(bg, k, aout, lmax, rtol, atol, yg, log, na, nr, full=False, to
     assert worst < tol
 
 
-def test_evolve_cfg0_lcdm_matches_oracle(deb, oracle):
+def test_evolve_lcdm_matches_oracle(deb, oracle):
     """Plain LCDM, massless neutrinos, lmax 12, rtol 1e-5 / atol 1e-10, outputs from a = 1e-7."""
     k = oracle.geomspace(1e-4, 1.0, 6)
     aout = oracle.geomspace(1e-7, 1.0, 16)
     lmax = [12, 12, 12, 8, 3]
-    print("case: evolve cfg0 (LCDM) nk=6 nout=16", flush=True)
-    p, yg, na, nr, log = _gpu_run(deb, cfg0_param(), k, aout, lmax, 1e-5, 1e-10)
+    print("case: evolve LCDM nk=6 nout=16", flush=True)
+    p, yg, na, nr, log = _gpu_run(deb, lcdm_param(), k, aout, lmax, 1e-5, 1e-10)
     bg = oracle_on_gpu_tables(deb, oracle, p)
-    _check_replay(bg, k, aout, lmax, 1e-5, 1e-10, yg, log, na, nr, label="cfg0")
+    _check_replay(bg, k, aout, lmax, 1e-5, 1e-10, yg, log, na, nr, label="lcdm")
     yo, _, sto, nao, nro = bg.evolve(k, aout, lmaxg=12, lmaxgp=12, lmaxr=12, lmaxnu=8, nqmax=3, rtol=1e-5, atol=1e-10)
     print("  free-running accepted gpu", na.tolist(), "oracle", nao.tolist())
     end = np.abs(yg[:, -1, DOMINANT] - yo[:, -1, DOMINANT]) / np.abs(yo[:, -1, DOMINANT])
@@ -162,16 +162,16 @@ def test_evolve_cfg0_lcdm_matches_oracle(deb, oracle):
     assert np.all(np.abs(na - nao) <= 0.02 * nao + 3)
 
 
-def test_evolve_cfg1_massive_nu_w0wa_matches_oracle(deb, oracle):
+def test_evolve_massive_nu_w0wa_matches_oracle(deb, oracle):
     """One massive neutrino on an 8-point momentum grid (lmax_nu 17) and a w0wa fluid; P(k) at z=0."""
     k = oracle.geomspace(1e-5, 5.0, 6)
     aout = [1.0 / 3.0, 0.5, 1.0]
     lmax = [12, 12, 12, 17, 8]
-    P = cfg1_param()
-    print("case: evolve cfg1 (massive nu + w0wa) nk=6 nout=3", flush=True)
+    P = nu_w0wa_param()
+    print("case: evolve massive nu + w0wa nk=6 nout=3", flush=True)
     p, yg, na, nr, log = _gpu_run(deb, P, k, aout, lmax, 1e-5, 1e-10)
     bg = oracle_on_gpu_tables(deb, oracle, p)
-    _check_replay(bg, k, aout, lmax, 1e-5, 1e-10, yg, log, na, nr, label="cfg1")
+    _check_replay(bg, k, aout, lmax, 1e-5, 1e-10, yg, log, na, nr, label="nu_w0wa")
     yo, _, sto, nao, nro = bg.evolve(k, aout, lmaxg=12, lmaxgp=12, lmaxr=12, lmaxnu=17, nqmax=8, rtol=1e-5, atol=1e-10)
     print("  free-running accepted gpu", na.tolist(), "oracle", nao.tolist())
     pk_g = deb.get_power(k=torch.as_tensor(k, device="cuda"), y=torch.as_tensor(yg[:, -1, :], device="cuda"), idx=4,
