@@ -58,6 +58,22 @@ def main():
     # tests/test_cmb/test_benchmark_compute_Dell.csv: D_ell for C_ell = 1 (closed form)
     df = pd.read_csv(f"{REF}/test_cmb/test_benchmark_compute_Dell.csv")
     np.savez_compressed(f"{HERE}/compute_dell.npz", ell=df["ell"].values, Dell=df["Dell"].values)
+
+    # CMB chain fixtures of tests/test_cmb.py:93-240 (DISCO-Notebook, num_thermo=1024, output
+    # a = geomspace(3e-4, 5e-3, 256, endpoint=False) ++ geomspace(5e-3, 1, 64)).  The
+    # visibility functions depend on the background only; the others come from the reference's
+    # batched perturbation run (128 camb-sampled k, rtol = atol = 1e-4), sampled at k0/k63/k127.
+    df = pd.read_csv(f"{REF}/test_cmb/test_benchmark_compute_visibility_functions.csv")
+    np.savez_compressed(f"{HERE}/visibility_functions.npz", gvis=df["gvis"].values, optical_depth=df["optical_depth"].values,
+                        gvisprime=df["gvisprime"].values)
+    out = {}
+    for name in ["compute_theta_ell", "compute_Cell", "compute_source_term_sachs_wolfe", "compute_source_term_doppler",
+                 "compute_source_term_polarization", "compute_metric_perturbations", "compute_polarization_terms",
+                 "compute_neutrino_perturbations"]:
+        df = pd.read_csv(f"{REF}/test_cmb/test_benchmark_{name}.csv")
+        for col in df.columns[1:]:
+            out[col] = df[col].values
+    np.savez_compressed(f"{HERE}/cmb_chain.npz", **out)
     print("written:", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
 
 
