@@ -1,0 +1,155 @@
+"""CMB source functions, line of sight and C_ell on the GPU (reference: src/discoeb/cmb.py).
+
+Same names, arguments and return layouts as the reference where the reference's function
+takes arrays (compute_visibility_functions, compute_theta_ell, compute_Cell, compute_Dell,
+compute_Cell_spectrum_from_cosmo_params).  The reference's chain of dict-passing helpers
+(extract_perturbations .. compute_source_function, cmb.py:17-633) runs as one fused kernel
+here, compute_source_function_from_states, which returns the same S1..S4 / S planes.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .background import evolve_background
+from .perturbations import _geomspace, compute_time_derivatives, evolve_perturbations
+
+
+def _dev(x, dev):
+    return torch.as_tensor(x, dtype=torch.float64, device=dev).contiguous()
+
+
+def extract_perturbations(yout, youtprime, lmaxg, lmaxgp, lmaxr):
+    """Named views of the state and its time derivative (cmb.py:17-89); pure indexing."""
+    idxb, idxg = 5, 7
+    idxgp = 7 + (lmaxg + 1)
+    idxr = 9 + lmaxg + lmaxgp
+    y, yp = yout, youtprime
+    return {"idxb": idxb, "idxg": idxg, "idxgp": idxgp, "idxr": idxr, "iq0": 10 + lmaxg + lmaxgp + lmaxr,
+            "eta": y[..., 2], "deltac": y[..., 3], "thetac": y[..., 4], "deltab": y[..., idxb], "thetab": y[..., idxb + 1],
+            "deltabprime": yp[..., idxb], "thetabprime": yp[..., idxb + 1],
+            "deltag": y[..., idxg], "thetag": y[..., idxg + 1], "shearg": y[..., idxg + 2] / 2.0,
+            "deltagprime": yp[..., idxg], "thetagprime": yp[..., idxg + 1], "sheargprime": yp[..., idxg + 2] / 2.0,
+            "deltar": y[..., idxr], "thetar": y[..., idxr + 1], "shearr": y[..., idxr + 2] / 2.0,
+            "deltarprime": yp[..., idxr], "thetarprime": yp[..., idxr + 1], "shearrprime": yp[..., idxr + 2] / 2.0,
+            "deltaq": y[..., -2], "thetaq": y[..., -1]}
+
+
+def _tables(param):
+    t = param["_deb_tables"]
+    return t, int(param["_deb_num_thermo"]), int(param.get("_deb_cosmo", 0))
+
+
+def compute_visibility_functions(tau, param, aexp=None):
+    """Opacity, its derivative, g, g', g'' and the optical depth at conformal times tau, from
+    splines in log tau of the thermal tables (cmb.py:92-166); also returns aprimeoa, rhonu,
+    rho_Q and w_Q at the matching scale factors (a from the a_of_tau spline unless given)."""
+    t, N, c = _tables(param)
+    dev = t.device
+    tt = _dev(tau, dev).reshape(-1)
+    a = param["a_of_tau_spline"].evaluate(tt) if aexp is None else _dev(aexp, dev).reshape(-1)
+    out = torch.empty((10, tt.numel()), dtype=torch.float64, device=dev)
+    work = torch.empty(12 * N, dtype=torch.float64, device=dev)
+    _lib.check(_lib.lib().deb_visibility(_lib.ptr(t), N, c, _lib.ptr(tt), _lib.ptr(a), tt.numel(), _lib.ptr(out), _lib.ptr(work),
+                                         _lib.stream()))
+    names = ["opac", "opacprime", "gvis", "gvisprime", "gvispprime", "optical_depth", "aprimeoa", "rhonu", "rho_Q", "w_Q"]
+    return {nm: out[i] for i, nm in enumerate(names)}
+
+
+def compute_source_function_from_states(yout, yprime, kmodes, aexp_out, tau, param, lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax):
+    """S1 (ISW), S2 (Sachs-Wolfe), S3 (Doppler), S4 (polarisation), S and the metric alpha,
+    alpha' for states yout and their time derivatives yprime (n_k, n_tau, nvar) -- the chain of
+    compute_Cell_spectrum_from_cosmo_params steps 6-10 (cmb.py:1050-1073)."""
+    t, N, c = _tables(param)
+    dev = t.device
+    y, yp = _dev(yout, dev), _dev(yprime, dev)
+    kk, aa, tt = _dev(kmodes, dev).reshape(-1), _dev(aexp_out, dev).reshape(-1), _dev(tau, dev).reshape(-1)
+    nk, nt = kk.numel(), tt.numel()
+    nv = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + nqmax * (lmaxnu + 1) + 2
+    if tuple(y.shape) != (nk, nt, nv) or y.shape != yp.shape:
+        raise ValueError(f"yout and yprime must have shape (n_k, n_tau, nvar) = ({nk}, {nt}, {nv})")
+    L = _lib.lib()
+    out = torch.empty((7, nk, nt), dtype=torch.float64, device=dev)
+    work = torch.empty(L.deb_cmb_sources_workspace_doubles(N, nk, nt), dtype=torch.float64, device=dev)
+    prob = _lib.DebProblem(int(lmaxg), int(lmaxgp), int(lmaxr), int(lmaxnu), int(nqmax), 1)
+    _lib.check(L.deb_cmb_sources(_lib.ptr(t), N, c, _lib.ptr(y), _lib.ptr(yp), _lib.ptr(kk), nk, _lib.ptr(aa), _lib.ptr(tt), nt,
+                                 ctypes.byref(prob), _lib.ptr(out), _lib.ptr(work), _lib.stream()))
+    return {"S1": out[0], "S2": out[1], "S3": out[2], "S4": out[3], "S": out[4], "alpha": out[5], "alphaprime": out[6]}
+
+
+def compute_theta_ell(ellmax, kmodes, tau, S, tau0, n_fftlog=16384, n_k_dense=0):
+    """Theta_l(k) by FFTLog line-of-sight integration (cmb.py:680-835).  Returns (theta_ell
+    (n_k, ellmax + 1), kmodes used) as device tensors."""
+    S = torch.as_tensor(S, dtype=torch.float64)
+    dev = S.device if S.is_cuda else _lib.device()
+    S = S.to(dev).contiguous()
+    kk, tt = _dev(kmodes, dev).reshape(-1), _dev(tau, dev).reshape(-1)
+    nk, nt = kk.numel(), tt.numel()
+    if tuple(S.shape) != (nk, nt):
+        raise ValueError("S must have shape (n_kmodes, n_times)")
+    nku = int(n_k_dense) if n_k_dense > 0 else nk
+    L = _lib.lib()
+    theta = torch.empty((nku, int(ellmax) + 1), dtype=torch.float64, device=dev)
+    kout = torch.empty(nku, dtype=torch.float64, device=dev)
+    work = torch.empty(L.deb_los_workspace_doubles(nk, nt, int(ellmax), int(n_fftlog), int(n_k_dense)), dtype=torch.float64, device=dev)
+    _lib.check(L.deb_los_theta_ell(_lib.ptr(kk), nk, _lib.ptr(tt), nt, _lib.ptr(S), float(tau0), int(ellmax), int(n_fftlog),
+                                   int(n_k_dense), _lib.ptr(theta), _lib.ptr(kout), _lib.ptr(work), _lib.stream()))
+    return theta, kout
+
+
+def compute_Cell(theta_ell, kmodes, n_s, k_p):
+    """C_l = trapezoid over ln k of (k/k_p)^(n_s-1) Theta_l^2 (cmb.py:838-873)."""
+    th = torch.as_tensor(theta_ell, dtype=torch.float64)
+    dev = th.device if th.is_cuda else _lib.device()
+    th = th.to(dev).contiguous()
+    kk = _dev(kmodes, dev).reshape(-1)
+    cl = torch.empty(th.shape[1], dtype=torch.float64, device=dev)
+    _lib.check(_lib.lib().deb_cell(_lib.ptr(th), _lib.ptr(kk), kk.numel(), th.shape[1], float(n_s), float(k_p), _lib.ptr(cl),
+                                   _lib.stream()))
+    return cl
+
+
+def compute_Dell(Cell, A_s, Tcmb, ellmax=None):
+    """(ell, D_l = l (l + 1) C_l A_s 2 T_cmb^2 1e12) for l >= 2 (cmb.py:876-927)."""
+    cl = torch.as_tensor(Cell, dtype=torch.float64)
+    dev = cl.device if cl.is_cuda else _lib.device()
+    cl = cl.to(dev).contiguous()
+    if ellmax is None:
+        ellmax = cl.numel() - 1
+    ell = torch.empty(ellmax - 1, dtype=torch.float64, device=dev)
+    dl = torch.empty(ellmax - 1, dtype=torch.float64, device=dev)
+    _lib.check(_lib.lib().deb_dell(_lib.ptr(cl), int(ellmax), float(A_s), float(Tcmb), _lib.ptr(ell), _lib.ptr(dl), _lib.stream()))
+    return ell.to(torch.int64), dl
+
+
+def cmb_aexp_out(n_early=32, n_pre_recomb=64, n_recomb=128, n_post=96):
+    """Output scale factors of the CMB driver: log-spaced pieces split at z = 3000, 1400, 600
+    (cmb.py:999-1014)."""
+    a1, a2, a3 = 1.0 / 3001.0, 1.0 / 1401.0, 1.0 / 601.0
+    return np.concatenate([_geomspace(1e-4, a1, n_early, endpoint=False), _geomspace(a1, a2, n_pre_recomb, endpoint=False),
+                           _geomspace(a2, a3, n_recomb, endpoint=False), _geomspace(a3, 1.0, n_post)])
+
+
+def compute_Cell_spectrum_from_cosmo_params(param_dict, ellmax=2500, nmodes=128, kmin=1e-4, kmax=1.0, n_k_dense=8192, n_fftlog=16384,
+                                            lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3):
+    """TT C_l from cosmological parameters (cmb.py:930-1090): background (num_thermo 1024),
+    perturbations on the camb k-sampling at rtol = atol = 1e-4 with full states, time
+    derivatives, source function, line of sight, C_l.  Returns (Cell, param).  The
+    perturbations are integrated per mode with adaptive steps (the reference batches 16 modes
+    on one shared step)."""
+    param = evolve_background(param=dict(param_dict), thermo_module="RECFAST", num_thermo=1024)
+    aexp_out = cmb_aexp_out()
+    lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
+    yout, kmodes, param = evolve_perturbations(param=param, aexp_out=aexp_out, kmin=kmin, kmax=kmax, num_k=nmodes, rtol=1e-4, atol=1e-4,
+                                               return_full=True, k_sampling_method="camb", **lm)
+    tau = param["tau_out"]
+    yprime = compute_time_derivatives(yout, tau, kmodes, param, lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax)
+    src = compute_source_function_from_states(yout, yprime, kmodes, aexp_out, tau, param, **lm)
+    tau0 = float(param["tau_of_a_spline"].evaluate(np.array([1.0])).cpu()[0])
+    theta, kk = compute_theta_ell(ellmax, kmodes, tau, src["S"], tau0, n_fftlog=n_fftlog, n_k_dense=n_k_dense)
+    return compute_Cell(theta, kk, param["n_s"], param.get("k_p", 0.05)), param
+
+
+__all__ = ["extract_perturbations", "compute_visibility_functions", "compute_source_function_from_states", "compute_theta_ell",
+           "compute_Cell", "compute_Dell", "cmb_aexp_out", "compute_Cell_spectrum_from_cosmo_params"]

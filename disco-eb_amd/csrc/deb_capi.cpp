@@ -9,6 +9,17 @@
 
 extern "C" int deb_launch_background(const double *, int, int, double *, hipStream_t);
 extern "C" int deb_launch_nu_bins(int, double *, hipStream_t);
+extern "C" long deb_cmb_sources_work_doubles(int, int, int);
+extern "C" int deb_launch_cmb_sources(const double *, int, const double *, const double *, const double *, int, const double *, const double *,
+                                      int, const DebProblem *, double *, double *, hipStream_t);
+extern "C" int deb_launch_visibility(const double *, int, const double *, const double *, int, double *, double *, hipStream_t);
+extern "C" long deb_los_work_doubles(int, int, int, int, int);
+extern "C" int deb_launch_los(const double *, int, const double *, int, const double *, double, int, int, int, double *, double *, double *,
+                              hipStream_t);
+extern "C" int deb_launch_contract(const double *, const double *, int, int, long, double *, hipStream_t);
+extern "C" int deb_launch_cell(const double *, const double *, int, int, double, double, double *, hipStream_t);
+extern "C" int deb_launch_dell(const double *, int, double, double, double *, double *, hipStream_t);
+
 extern "C" int deb_launch_tau_out(const double *, int, int, const double *, int, double *, hipStream_t);
 extern "C" int deb_launch_power_units(const double *, int, const int *, const double *, int, int, const double *, int, int, double *,
                                       hipStream_t);
@@ -143,6 +154,65 @@ int deb_time_derivatives(const double *d_tables, int num_thermo, int cosmo, cons
   rc = deb_launch_nu_bins(prob->nqmax, d_work, (hipStream_t)stream);
   if (rc) return hip_rc(rc, "deb_time_derivatives (momentum bins)");
   return hip_rc(deb_launch_time_deriv(t, d_y, d_tau, n_tau, d_k, n_k, d_work, prob, d_dy, (hipStream_t)stream), "deb_time_derivatives");
+}
+
+// ------------------------------------------------------------------ CMB chain
+long deb_cmb_sources_workspace_doubles(int num_thermo, int n_k, int n_tau) { return deb_cmb_sources_work_doubles(num_thermo, n_k, n_tau); }
+
+int deb_cmb_sources(const double *d_tables, int num_thermo, int cosmo, const double *d_y, const double *d_yprime, const double *d_k, int n_k,
+                    const double *d_aexp, const double *d_tau, int n_tau, const DebProblem *prob, double *d_out, double *d_work, void *stream) {
+  int rc = check_problem(prob);
+  if (rc) return rc;
+  if (n_k < 1 || n_tau < 3) return fail(-1, "need n_k >= 1 and n_tau >= 3");
+  if (cosmo < 0) return fail(-1, "cosmology index must be >= 0");
+  if (!d_tables || !d_y || !d_yprime || !d_k || !d_aexp || !d_tau || !d_out || !d_work) return fail(-1, "NULL device pointer");
+  const double *t = d_tables + (long)cosmo * deb_table_doubles(num_thermo);
+  return hip_rc(deb_launch_cmb_sources(t, num_thermo, d_y, d_yprime, d_k, n_k, d_aexp, d_tau, n_tau, prob, d_out, d_work, (hipStream_t)stream),
+                "deb_cmb_sources");
+}
+
+int deb_visibility(const double *d_tables, int num_thermo, int cosmo, const double *d_tau, const double *d_aexp, int n_tau, double *d_out,
+                   double *d_work, void *stream) {
+  if (n_tau < 1) return 0;
+  if (cosmo < 0) return fail(-1, "cosmology index must be >= 0");
+  if (!d_tables || !d_tau || !d_aexp || !d_out || !d_work) return fail(-1, "NULL device pointer");
+  const double *t = d_tables + (long)cosmo * deb_table_doubles(num_thermo);
+  return hip_rc(deb_launch_visibility(t, num_thermo, d_tau, d_aexp, n_tau, d_out, d_work, (hipStream_t)stream), "deb_visibility");
+}
+
+long deb_los_workspace_doubles(int n_k, int n_tau, int ellmax, int n_fftlog, int n_k_dense) {
+  return deb_los_work_doubles(n_k, n_tau, ellmax + 1, n_fftlog, n_k_dense);
+}
+
+int deb_los_theta_ell(const double *d_k, int n_k, const double *d_tau, int n_tau, const double *d_S, double tau0, int ellmax, int n_fftlog,
+                      int n_k_dense, double *d_theta, double *d_k_out, double *d_work, void *stream) {
+  if (ellmax < 0) return fail(-1, "ellmax must be >= 0");
+  if (n_fftlog < 128 || n_fftlog > 16384 || (n_fftlog & (n_fftlog - 1))) return fail(-1, "n_fftlog must be a power of two in [128, 16384]");
+  if (n_k < 2 || n_tau < 3) return fail(-1, "need n_k >= 2 and n_tau >= 3");
+  if (n_k_dense < 0 || n_k_dense == 1) return fail(-1, "n_k_dense must be 0 or >= 2");
+  if (!d_k || !d_tau || !d_S || !d_theta || !d_k_out || !d_work) return fail(-1, "NULL device pointer");
+  int rc = deb_launch_los(d_k, n_k, d_tau, n_tau, d_S, tau0, ellmax, n_fftlog, n_k_dense, d_theta, d_k_out, d_work, (hipStream_t)stream);
+  if (rc == -2) return fail(-1, "n_fftlog must be a power of two in [128, 16384]");
+  return hip_rc(rc, "deb_los_theta_ell");
+}
+
+int deb_contract(const double *d_A, const double *d_B, int m, int n, long k, double *d_C, void *stream) {
+  if (m < 0 || n < 0 || k < 0 || (k % 16)) return fail(-1, "m, n >= 0 and k a non-negative multiple of 16");
+  if ((long)m * n == 0) return 0;
+  if (!d_A || !d_B || !d_C) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_contract(d_A, d_B, m, n, k, d_C, (hipStream_t)stream), "deb_contract");
+}
+
+int deb_cell(const double *d_theta, const double *d_k, int n_k, int n_ell, double n_s, double k_p, double *d_cell, void *stream) {
+  if (n_k < 2 || n_ell < 1) return fail(-1, "need n_k >= 2 and n_ell >= 1");
+  if (!d_theta || !d_k || !d_cell) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_cell(d_theta, d_k, n_k, n_ell, n_s, k_p, d_cell, (hipStream_t)stream), "deb_cell");
+}
+
+int deb_dell(const double *d_cell, int ellmax, double A_s, double Tcmb, double *d_ell, double *d_dell, void *stream) {
+  if (ellmax < 2) return fail(-1, "ellmax must be >= 2");
+  if (!d_cell || !d_ell || !d_dell) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_dell(d_cell, ellmax, A_s, Tcmb, d_ell, d_dell, (hipStream_t)stream), "deb_dell");
 }
 
 }  // extern "C"

@@ -1,0 +1,678 @@
+// deb_cmb.hip -- CMB source functions and the FFTLog line-of-sight projection.
+//
+// Follows the reference's src/discoeb/cmb.py: compute_visibility_functions (:92-166),
+// compute_neutrino_perturbations (:169-240), compute_metric_perturbations (:243-345),
+// compute_polarization_terms (:349-395), the source terms S1..S4 (:398-633),
+// _fftlog_bessel_integral_coefficients (:636-677) with util.py:13-45, compute_theta_ell
+// (:680-835), compute_Cell (:838-873) and compute_Dell (:876-927).
+//
+// Kernels:
+//   * vis_build / vis_eval: natural splines in log tau of the thermal tables (one thread per
+//     spline), evaluated with the background quantities at every output time;
+//   * source_point: one thread per (k, tau) -- neutrino moments, metric, polarisation terms,
+//     S2, S3, S4 and the ISW potential eta + alpha';
+//   * source_isw: one thread per k -- natural spline of the potential in tau, its derivative
+//     at the knots times exp(-tau_c), and the total S;
+//   * densify_build / densify_eval: S interpolated in log k onto the dense log-uniform grid;
+//   * fftlog: one workgroup per k-row -- the row's spline in log chi, the biased samples on the
+//     FFTLog grid written bit-reversed into LDS, an in-place radix-2 FFT of N/2 complex points
+//     (N/2 * 16 bytes of LDS), the real-FFT split and the phase / k-power epilogue, written as
+//     the K-contiguous left operand of the contraction;
+//   * bessel_coef: M_l(p_n) by the Lanczos complex log-gamma, the right operand;
+//   * contract: Theta = A B^T by FP64 MFMA (v_mfma_f64_16x16x4f64), 128 x 128 output tiles;
+//   * cell / dell: trapezoid over ln k, and the D_l scaling.
+#include <hip/hip_runtime.h>
+
+#include "deb_cosmo.hpp"
+#include "deb_perturb.h"
+
+namespace deb {
+namespace {
+
+constexpr double kPi = 3.141592653589793;
+
+// ---------------------------------------------------------------------------------------------
+// natural cubic splines (spline_interpolation.py:41-82): cp = modified upper diagonal of the
+// interior system, a function of the knots only
+// ---------------------------------------------------------------------------------------------
+__device__ void spline_cp(const double *__restrict__ x, int n, double *__restrict__ cp) {
+  const int m = n - 2;
+  for (int i = 0; i < m; ++i) {
+    double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
+    double b = 2.0 * (h0 + h1);
+    double den = (i == 0) ? b : b - h0 * cp[i - 1];
+    cp[i] = (i < m - 1) ? h1 / den : 0.0;
+  }
+}
+
+// second derivatives of the natural spline through (x[i], y[i * ys]) into S[i * ss]; the
+// forward sweep keeps dp in S, the back substitution overwrites it
+__device__ void spline_build(const double *__restrict__ x, const double *__restrict__ y, long ys, int n, const double *__restrict__ cp,
+                             double *__restrict__ S, long ss) {
+  S[0] = 0.0;
+  if (n > 1) S[(long)(n - 1) * ss] = 0.0;
+  const int m = n - 2;
+  if (m <= 0) return;
+  double dprev = 0.0;
+  for (int i = 0; i < m; ++i) {
+    double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
+    double y0 = y[(long)i * ys], y1 = y[(long)(i + 1) * ys], y2 = y[(long)(i + 2) * ys];
+    double d = 6.0 * ((y2 - y1) / h1 - (y1 - y0) / h0);
+    double b = 2.0 * (h0 + h1);
+    double dp = (i == 0) ? d / b : (d - h0 * dprev) / (b - h0 * cp[i - 1]);
+    S[(long)(i + 1) * ss] = dp;
+    dprev = dp;
+  }
+  double nxt = S[(long)m * ss];
+  for (int i = m - 2; i >= 0; --i) {
+    double v = S[(long)(i + 1) * ss] - cp[i] * nxt;
+    S[(long)(i + 1) * ss] = v;
+    nxt = v;
+  }
+}
+
+__device__ __forceinline__ double spl_value(const double *x, const double *y, long ys, const double *S, long ss, int i, double xq) {
+  double h = x[i + 1] - x[i];
+  double t = (xq - x[i]) / h;
+  double A = 1.0 - t, B = t;
+  return A * y[(long)i * ys] + B * y[(long)(i + 1) * ys] +
+         ((A * A * A - A) * S[(long)i * ss] + (B * B * B - B) * S[(long)(i + 1) * ss]) * (h * h / 6.0);
+}
+
+__device__ __forceinline__ double spl_slope(const double *x, const double *y, long ys, const double *S, long ss, int i, double xq) {
+  double h = x[i + 1] - x[i];
+  double d = xq - x[i];
+  double S0 = S[(long)i * ss], S1 = S[(long)(i + 1) * ss];
+  double b = (y[(long)(i + 1) * ys] - y[(long)i * ys]) / h - h * (S1 + 2.0 * S0) / 6.0;
+  return b + 2.0 * (S0 / 2.0) * d + 3.0 * ((S1 - S0) / (6.0 * h)) * d * d;
+}
+
+// ---------------------------------------------------------------------------------------------
+// visibility functions and background quantities at the output times
+// ---------------------------------------------------------------------------------------------
+// workspace layout (doubles): [0, N) log tau knots, [N, 2N) cp, then for spline s = 0..4
+// (opac, gvis, gvis', gvis'', log max(tau_c, 1e-10)) its values at [(2 + 2 s) N, ...) and
+// second derivatives at [(3 + 2 s) N, ...)
+__global__ void vis_build_kernel(const double *__restrict__ t, double *__restrict__ w) {
+  const int N = (int)t[DEB_HDR_N];
+  if (threadIdx.x == 0) {
+    const double *tau = t + deb_array_offset(DEB_A_TAU, N);
+    for (int i = 0; i < N; ++i) w[i] = log(tau[i]);
+    spline_cp(w, N, w + N);
+  }
+  __syncthreads();
+  const int s = threadIdx.x;
+  if (s >= 5) return;
+  const int slot[5] = {DEB_A_OPAC, DEB_A_GVIS, DEB_A_GVISP, DEB_A_GVISPP, DEB_A_OPTDEPTH};
+  const double *src = t + deb_array_offset(slot[s], N);
+  double *Y = w + (long)(2 + 2 * s) * N, *S = w + (long)(3 + 2 * s) * N;
+  double last = src[0];
+  for (int i = 0; i < N; ++i) {  // NaN entries take the previous value (spline_interpolation.py:32-40)
+    double v = src[i];
+    if (s == 4 && !isnan(v)) v = log(v > 1e-10 ? v : 1e-10);
+    if (isnan(v)) v = last;
+    Y[i] = v;
+    last = v;
+  }
+  spline_build(w, Y, 1, N, w + N, S, 1);
+}
+
+// per output time j: [0] opac [1] opac' [2] g [3] g' [4] g'' [5] tau_c [6] aH [7] rho_nu
+// [8] rho_Q [9] w_Q, each a row of n_tau
+constexpr int NV = 10;
+__global__ void vis_eval_kernel(const double *__restrict__ t, const double *__restrict__ w, const double *__restrict__ tau_q,
+                                const double *__restrict__ aexp, int n_tau, double *__restrict__ V) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_tau) return;
+  BgView b;
+  b.init(t);
+  const int N = b.N;
+  const double lq = log(tau_q[j]);
+  const int i = spl_index(w, N, lq);
+  double v[6];
+  for (int s = 0; s < 5; ++s) v[s < 1 ? 0 : s + 1] = spl_value(w, w + (long)(2 + 2 * s) * N, 1, w + (long)(3 + 2 * s) * N, 1, i, lq);
+  // the loop filled v[0] (opac), v[2..5]; reorder the log optical depth into place
+  double od = exp(v[5]);
+  double g = v[2], g1 = v[3], g2 = v[4];
+  double opacp = spl_slope(w, w + 2L * N, 1, w + 3L * N, 1, i, lq) / tau_q[j];
+  const double a = aexp[j];
+  V[0L * n_tau + j] = v[0];
+  V[1L * n_tau + j] = opacp;
+  V[2L * n_tau + j] = g;
+  V[3L * n_tau + j] = g1;
+  V[4L * n_tau + j] = g2;
+  V[5L * n_tau + j] = od;
+  V[6L * n_tau + j] = b.aprimeoa(a);
+  V[7L * n_tau + j] = exp(b.logrhonu(log(a)));
+  V[8L * n_tau + j] = b.rho_Q(a);
+  V[9L * n_tau + j] = b.w0 + b.wa * (1.0 - a);
+}
+
+// ---------------------------------------------------------------------------------------------
+// source terms
+// ---------------------------------------------------------------------------------------------
+// outputs (each n_k x n_tau, row-major by k): 0 S1, 1 S2, 2 S3, 3 S4, 4 S, 5 alpha, 6 alpha';
+// pot: n_tau x n_k (transposed) potential eta + alpha' for the ISW spline
+__global__ void source_point_kernel(const double *__restrict__ t, const double *__restrict__ y, const double *__restrict__ yp,
+                                    const double *__restrict__ kk, int n_k, int n_tau, const double *__restrict__ aexp,
+                                    const double *__restrict__ V, const double *__restrict__ qw, DebProblem prob,
+                                    double *__restrict__ out, double *__restrict__ pot) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n_k * n_tau) return;
+  const int j = (int)(idx % n_tau), ik = (int)(idx / n_tau);
+  BgView b;
+  b.init(t);
+  const int lmaxg = prob.lmaxg, lmaxgp = prob.lmaxgp, lmaxr = prob.lmaxr, nq = prob.nqmax;
+  const int n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + nq * (prob.lmaxnu + 1) + 2;
+  const int idxg = 7, idxgp = 8 + lmaxg, idxr = 9 + lmaxg + lmaxgp, iq0 = 10 + lmaxg + lmaxgp + lmaxr;
+  const double *Y = y + idx * n, *P = yp + idx * n;
+  const double k = kk[ik], k2 = k * k, a = aexp[j], a2 = a * a;
+  const double opac = V[0L * n_tau + j], opacp = V[1L * n_tau + j], g = V[2L * n_tau + j], g1 = V[3L * n_tau + j],
+               g2 = V[4L * n_tau + j];
+  const double aH = V[6L * n_tau + j], rhonu = V[7L * n_tau + j], rho_Q = V[8L * n_tau + j], w_Q = V[9L * n_tau + j];
+  // massive neutrinos (perturbations.py:29-85)
+  double drhonu = 0.0, fnu = 0.0, shearnu = 0.0, rnp = 0.0, snp = 0.0;
+  for (int q = 0; q < nq; ++q) {
+    const double qq = qw[q], wq = qw[16 + q];
+    const double aq = a * b.amnu / qq;
+    const double v = 1.0 / sqrt(1.0 + aq * aq);
+    const double vp = -0.5 * aq * aH * v * v * v;
+    const double p0 = Y[iq0 + q], p1 = Y[iq0 + nq + q], p2 = Y[iq0 + 2 * nq + q];
+    const double p0p = P[iq0 + q], p2p = P[iq0 + 2 * nq + q];
+    drhonu += wq * p0 / v;
+    fnu += wq * p1;
+    shearnu += wq * p2 * v;
+    rnp += wq * (p0p / v - p0 / (v * v) * vp);
+    snp += wq * (p2p * v + p2 * vp);
+  }
+  shearnu = shearnu * 2.0 / 3.0;
+  snp = snp * 2.0 / 3.0;
+  // metric (cmb.py:243-345)
+  const double Omegac = b.Omegam - b.Omegab;
+  const double eta = Y[2], deltac = Y[3], thetac = Y[4], deltab = Y[5], thetab = Y[6], thetabp = P[6];
+  const double deltag = Y[idxg], thetag = Y[idxg + 1], shearg = Y[idxg + 2] / 2.0, thetagp = P[idxg + 1], sheargp = P[idxg + 2] / 2.0;
+  const double deltar = Y[idxr], thetar = Y[idxr + 1], shearr = Y[idxr + 2] / 2.0, shearrp = P[idxr + 2] / 2.0;
+  const double deltaq = Y[n - 2], thetaq = Y[n - 1];
+  const double rpptq = (1.0 + w_Q) * rho_Q * b.grhom * b.OmegaDE * thetaq * a2;
+  const double grhoprime =
+      (-b.grhom * b.Omegam / a - 2.0 * (b.grhog + b.grhor * (b.Neff + b.Nmnu * rhonu)) / a2 +
+       b.grhom * b.OmegaDE * (-3.0 * (1.0 + b.w0 + (1.0 - a) * b.wa) * rho_Q * a2 + 2.0 * rho_Q * a2)) *
+          aH +
+      b.grhor * b.Nmnu * rnp / a2;
+  const double aHp = grhoprime / (6.0 * aH);
+  const double dgrho = b.grhom * (Omegac * deltac + b.Omegab * deltab) / a +
+                       (b.grhog * deltag + b.grhor * (b.Neff * deltar + b.Nmnu * drhonu)) / a2 + b.grhom * b.OmegaDE * deltaq * rho_Q * a2;
+  const double dgtheta = b.grhom * (Omegac * thetac + b.Omegab * thetab) / a +
+                         4.0 / 3.0 * (b.grhog * thetag + b.Neff * b.grhor * thetar) / a2 + b.Nmnu * b.grhor * k * fnu / a2 + rpptq;
+  const double dgshear = 4.0 / 3.0 * (b.grhog * shearg + b.Neff * b.grhor * shearr) / a2 + b.Nmnu * b.grhor * shearnu / a2;
+  const double dgshearp =
+      4.0 / 3.0 * (b.grhog * sheargp + b.Neff * b.grhor * shearrp) / a2 + b.Nmnu * b.grhor * snp / a2 - 2.0 * aH * dgshear;
+  const double hprime = (2.0 * k2 * eta + dgrho) / aH;
+  const double etaprime = 0.5 * dgtheta / k2;
+  const double alpha = (hprime + 6.0 * etaprime) / 2.0 / k2;
+  const double alphap = -3.0 * dgshear / (2.0 * k2) + eta - 2.0 * aH * alpha;
+  (void)etaprime;
+  // polarisation terms (cmb.py:349-395)
+  const double couplp = 8.0 * (thetagp + k2 * alphap) / 15.0 - k * 0.6 * (P[idxg + 3] + P[idxgp + 1] + P[idxgp + 3]);
+  const double pol = Y[idxg + 2] + Y[idxgp] + Y[idxgp + 2];
+  const double polp = P[idxg + 2] + P[idxgp] + P[idxgp + 2];
+  const double polpp = couplp - 0.3 * (opacp * pol + opac * polp);
+  (void)aHp;
+  (void)dgshearp;
+  // S2, S3, S4 (cmb.py:455-554)
+  const double S2 = g * (0.25 * deltag + 2.0 * alphap + pol / 16.0 + (thetabp + 3.0 / 16.0 * polpp) / k2);
+  const double S3 = g1 * (alpha + (thetab + 3.0 / 8.0 * polp) / k2);
+  const double S4 = g2 * 3.0 / 16.0 * pol / k2;
+  const long plane = (long)n_k * n_tau;
+  out[1 * plane + idx] = S2;
+  out[2 * plane + idx] = S3;
+  out[3 * plane + idx] = S4;
+  out[5 * plane + idx] = alpha;
+  out[6 * plane + idx] = alphap;
+  pot[(long)j * n_k + ik] = eta + alphap;
+}
+
+// S1 = exp(-tau_c) d(eta + alpha')/dtau from the natural spline of the potential in tau
+// (cmb.py:398-452), then S = S1 + S2 + S3 + S4.  One thread per k; pot and its second
+// derivatives are stored tau-major (n_tau x n_k) so neighbouring threads touch neighbouring words.
+__global__ void source_isw_kernel(const double *__restrict__ tau, int n_tau, int n_k, const double *__restrict__ cp,
+                                  const double *__restrict__ pot, double *__restrict__ Spot, const double *__restrict__ V,
+                                  double *__restrict__ out) {
+  const int ik = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ik >= n_k) return;
+  spline_build(tau, pot + ik, n_k, n_tau, cp, Spot + ik, n_k);
+  const long plane = (long)n_k * n_tau;
+  for (int j = 0; j < n_tau; ++j) {
+    const int i = spl_index(tau, n_tau, tau[j]);
+    const double d = spl_slope(tau, pot + ik, n_k, Spot + ik, n_k, i, tau[j]);
+    const long o = (long)ik * n_tau + j;
+    const double S1 = exp(-V[5L * n_tau + j]) * d;
+    out[0 * plane + o] = S1;
+    out[4 * plane + o] = S1 + out[1 * plane + o] + out[2 * plane + o] + out[3 * plane + o];
+  }
+}
+
+__global__ void spline_cp_kernel(const double *__restrict__ x, int n, double *__restrict__ cp) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) spline_cp(x, n, cp);
+}
+
+// ---------------------------------------------------------------------------------------------
+// line of sight
+// ---------------------------------------------------------------------------------------------
+// logs of the input wavenumbers and the dense log-uniform grid (jax.numpy.linspace arithmetic:
+// start (1 - s) + stop s, s = i / (n - 1), last point exactly stop)
+__global__ void log_k_kernel(const double *__restrict__ k, int n_k, int n_kd, double *__restrict__ lk, double *__restrict__ lkd,
+                             double *__restrict__ kd) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_k) lk[i] = log(k[i]);
+  if (i < n_kd) {
+    double l0 = log(k[0]), l1 = log(k[n_k - 1]);
+    double s = (double)i / (double)(n_kd - 1);
+    double v = (i == n_kd - 1) ? l1 : l0 * (1.0 - s) + l1 * s;
+    lkd[i] = v;
+    kd[i] = exp(v);
+  }
+}
+
+// one thread per time slice: spline of S[:, j] in log k (S is n_k x n_tau)
+__global__ void densify_build_kernel(const double *__restrict__ lk, int n_k, int n_tau, const double *__restrict__ cp,
+                                     const double *__restrict__ S, double *__restrict__ Sk) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_tau) return;
+  spline_build(lk, S + j, n_tau, n_k, cp, Sk + j, n_tau);
+}
+
+__global__ void densify_eval_kernel(const double *__restrict__ lk, int n_k, int n_tau, const double *__restrict__ S,
+                                    const double *__restrict__ Sk, const double *__restrict__ lkd, int n_kd, double *__restrict__ Sd) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n_kd * n_tau) return;
+  const int j = (int)(idx % n_tau), id = (int)(idx / n_tau);
+  const double xq = lkd[id];
+  const int i = spl_index(lk, n_k, xq);
+  Sd[idx] = spl_value(lk, S + j, n_tau, Sk + j, n_tau, i, xq);
+}
+
+// knots of the log-chi splines: chi_i = tau0 - tau[n - 1 - i], log max(chi, 1e-10)
+__global__ void chi_knots_kernel(const double *__restrict__ tau, int n_tau, double tau0, double *__restrict__ lco) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_tau) return;
+  double chi = tau0 - tau[n_tau - 1 - i];
+  lco[i] = log(chi > 1e-10 ? chi : 1e-10);
+}
+
+// FFTLog grid of cmb.py:753-775: chi = tau0 - tau (increasing after reversal), log chi_min =
+// log max(chi_0, 1) - 2, log chi_max = log chi_last + 2, N points of width dl
+__device__ __forceinline__ void fft_grid(const double *__restrict__ tau, int n_tau, double tau0, int N, double &lmin, double &dl) {
+  const double c0 = tau0 - tau[n_tau - 1], c1 = tau0 - tau[0];
+  lmin = log(c0 > 1.0 ? c0 : 1.0) - 2.0;
+  const double lmax = log(c1) + 2.0;
+  dl = (lmax - lmin) / (double)N;
+}
+
+// e^{-2 pi i t / N} for t < N / 2
+__global__ void twiddle_kernel(int N, double2 *__restrict__ tw) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N / 2) return;
+  double s, c;
+  sincospi(-2.0 * (double)t / (double)N, &s, &c);
+  tw[t] = make_double2(c, s);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+// FFTLog coefficients of one k-row per workgroup (cmb.py:753-803).  The row's spline in log chi
+// (knots lco) is built by one lane into Schi; samples f_m = S(chi_m) chi_m^1.1 on the grid
+// log chi_m = lmin + m dl are packed as z_j = f_2j + i f_2j+1 at bit-reversed LDS slots; a
+// radix-2 FFT of M = N/2 points gives Z, the real-FFT split gives X_n = rfft(f)_n for n <= N/2,
+// and ck_n = X_n e^{-i eta_n lmin} / N * k^{-(p_n + 1)} is written as A[row, 2n] = w_n Re ck,
+// A[row, 2n + 1] = -w_n Im ck (w = 1 for n = 0, N/2, else 2), zero beyond 2 (N/2 + 1).
+template <int LOGM>
+__global__ __launch_bounds__(256) void fftlog_kernel(const double *__restrict__ S, int n_tau, const double *__restrict__ lco,
+                                                     const double *__restrict__ cpchi, double *__restrict__ Schi,
+                                                     const double *__restrict__ kk, const double *__restrict__ tau, double tau0,
+                                                     const double2 *__restrict__ tw, double *__restrict__ A, long Kp) {
+  constexpr int M = 1 << LOGM, N = 2 * M;
+  extern __shared__ double2 z[];
+  double lmin, dl;
+  fft_grid(tau, n_tau, tau0, N, lmin, dl);
+  const int row = blockIdx.x, tid = threadIdx.x;
+  const double *Sr = S + (long)row * n_tau;
+  double *Sc = Schi + (long)row * n_tau;
+  if (tid == 0) spline_build(lco, Sr + (n_tau - 1), -1, n_tau, cpchi, Sc, 1);  // the row reversed (increasing chi)
+  __syncthreads();
+  const double *Yr = Sr + (n_tau - 1);  // Yr[-i] = value at knot i
+  for (int j = tid; j < M; j += blockDim.x) {
+    double f[2];
+    for (int h = 0; h < 2; ++h) {
+      const int m = 2 * j + h;
+      const double lx = lmin + (double)m * dl;
+      const int i = spl_index(lco, n_tau, lx);
+      const double sv = spl_value(lco, Yr, -1, Sc, 1, i, lx);
+      f[h] = sv * exp(1.1 * lx);  // chi^(-bias), bias = -1.1
+    }
+    const int r = (int)(__brev((unsigned)j) >> (32 - LOGM));
+    z[r] = make_double2(f[0], f[1]);
+  }
+  __syncthreads();
+  for (int s = 1; s <= LOGM; ++s) {  // in-place decimation in time
+    const int half = 1 << (s - 1);
+    const int stride = N >> s;  // twiddle index step: e^{-2 pi i j / 2^s} = tw[j N / 2^s]
+    for (int bfly = tid; bfly < M / 2; bfly += blockDim.x) {
+      const int j = bfly & (half - 1);
+      const int i0 = ((bfly >> (s - 1)) << s) | j, i1 = i0 + half;
+      const double2 w = tw[j * stride];
+      const double2 u = z[i0], v = cmul(z[i1], w);
+      z[i0] = make_double2(u.x + v.x, u.y + v.y);
+      z[i1] = make_double2(u.x - v.x, u.y - v.y);
+    }
+    __syncthreads();
+  }
+  const double lk = log(kk[row]);
+  const double kpow = exp(-(-1.1 + 1.0) * lk);
+  double *Ar = A + (long)row * Kp;
+  for (int nn = tid; nn <= M; nn += blockDim.x) {
+    const double2 Zn = z[nn & (M - 1)];
+    const double2 Zm = z[(M - nn) & (M - 1)];
+    const double2 Zc = make_double2(Zm.x, -Zm.y);
+    const double2 E = make_double2(0.5 * (Zn.x + Zc.x), 0.5 * (Zn.y + Zc.y));
+    const double2 D = make_double2(0.5 * (Zn.x - Zc.x), 0.5 * (Zn.y - Zc.y));
+    const double2 W = (nn < M) ? tw[nn] : make_double2(-1.0, 0.0);
+    const double2 O = cmul(W, make_double2(D.y, -D.x));  // e^{-2 pi i n / N} (-i) D
+    double2 X = make_double2(E.x + O.x, E.y + O.y);
+    const double eta = 2.0 * kPi * (double)nn / ((double)N * dl);
+    double s1, c1, s2, c2;
+    sincos(-eta * lmin, &s1, &c1);
+    sincos(-eta * lk, &s2, &c2);
+    X = cmul(X, make_double2(c1 / (double)N, s1 / (double)N));
+    X = cmul(X, make_double2(kpow * c2, kpow * s2));
+    const double wn = (nn == 0 || nn == M) ? 1.0 : 2.0;
+    Ar[2 * nn] = wn * X.x;
+    Ar[2 * nn + 1] = -wn * X.y;
+  }
+  for (long c = (long)N + 2 + tid; c < Kp; c += blockDim.x) Ar[c] = 0.0;
+}
+
+// complex helpers for the log-gamma
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 clog(double2 a) { return make_double2(log(hypot(a.x, a.y)), atan2(a.y, a.x)); }
+
+// log Gamma(z): Lanczos approximation, g = 7, 9 terms, reflection for Re z <= 1/2 (util.py:13-45)
+__device__ double2 lngamma_complex(double2 z) {
+  const double c[9] = {0.99999999999980993227684700473478, 676.520368121885098567009190444019,
+                       -1259.13921672240287047156078755283, 771.3234287776530788486528258894,
+                       -176.61502916214059906584551354, 12.507343278686904814458936853,
+                       -0.13857109526572011689554707, 9.984369578019570859563e-6, 1.50563273514931155834e-7};
+  const bool refl = z.x <= 0.5;
+  double2 zz = refl ? make_double2(1.0 - z.x, -z.y) : z;
+  zz.x -= 1.0;
+  double2 Ag = make_double2(c[0], 0.0);
+  for (int j = 1; j < 9; ++j) {
+    const double2 t = make_double2(zz.x + j, zz.y);
+    const double n2 = t.x * t.x + t.y * t.y;
+    Ag.x += c[j] / n2 * t.x;
+    Ag.y += c[j] / n2 * (-t.y);
+  }
+  const double2 zh = make_double2(zz.x + 0.5, zz.y), z75 = make_double2(zz.x + 7.5, zz.y);
+  double2 r = cmul(zh, clog(z75));
+  r = csub(r, z75);
+  r.x += 0.9189385332046727418;
+  r = cadd(r, clog(Ag));
+  if (!refl) return r;
+  // ln pi - ln sin(pi z) - lnGamma(1 - z); sin(pi (x + i y)) = sin(pi x) cosh(pi y) + i cos(pi x) sinh(pi y)
+  double sx, cx;
+  sincospi(z.x, &sx, &cx);
+  const double2 sn = make_double2(sx * cosh(kPi * z.y), cx * sinh(kPi * z.y));
+  double2 out = make_double2(1.14472988584940017414342735135, 0.0);
+  out = csub(out, clog(sn));
+  return csub(out, r);
+}
+
+// B[l, 2n] = Re M_l(p_n), B[l, 2n + 1] = Im M_l(p_n), p_n = -1.1 + i eta_n (cmb.py:636-677)
+__global__ void bessel_coef_kernel(int n_ell, int N, const double *__restrict__ tau, int n_tau, double tau0, double *__restrict__ B,
+                                   long Kp) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double lmin, dl;
+  fft_grid(tau, n_tau, tau0, N, lmin, dl);
+  const int nf = N / 2 + 1;
+  if (idx >= (long)n_ell * Kp / 2) return;
+  const int l = (int)(idx / (Kp / 2)), nn = (int)(idx % (Kp / 2));
+  double2 M = make_double2(0.0, 0.0);
+  if (nn < nf) {
+    const double eta = 2.0 * kPi * (double)nn / ((double)N * dl);
+    const double2 p = make_double2(-1.1, eta);
+    const double2 a1 = make_double2((1.0 + l + p.x) / 2.0, p.y / 2.0), a2 = make_double2((2.0 + l - p.x) / 2.0, -p.y / 2.0);
+    double2 lg = make_double2((p.x - 1.0) * log(2.0), p.y * log(2.0));
+    lg.x += 0.5 * log(kPi);
+    lg = cadd(lg, lngamma_complex(a1));
+    lg = csub(lg, lngamma_complex(a2));
+    const double e = exp(lg.x);
+    double s, c;
+    sincos(lg.y, &s, &c);
+    M = make_double2(e * c, e * s);
+    if (e == 0.0) M = make_double2(0.0, 0.0);  // the reflection branch overflowed: Gamma -> 0
+  }
+  B[(long)l * Kp + 2 * nn] = M.x;
+  B[(long)l * Kp + 2 * nn + 1] = M.y;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Theta = A B^T with FP64 MFMA.  A: M x K, B: N x K (both K-contiguous, K a multiple of 16),
+// C: M x N row-major.  Workgroup: 256 lanes (4 waves in 2 x 2), tile 128 x 128, each wave a
+// 64 x 64 block of 4 x 4 v_mfma_f64_16x16x4f64 accumulators; K staged through LDS 16 at a time.
+// Operand layout of the 16x16x4 f64 MFMA: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16];
+// the accumulator's element v of lane l is C[4 v + l / 16][l % 16] (checked on the device).
+// ---------------------------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int TM = 128, TN = 128, TK = 16, TKP = TK + 1;  // +1: LDS rows padded against bank conflicts
+
+__global__ __launch_bounds__(256) void contract_kernel(const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C,
+                                                       int Mr, int Nc, long K) {
+  __shared__ double As[TM * TKP], Bs[TN * TKP];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const long m0 = (long)blockIdx.y * TM, n0 = (long)blockIdx.x * TN;
+  v4d acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+  const int lr = lane & 15, lk = lane >> 4;
+  for (long k0 = 0; k0 < K; k0 += TK) {
+    // stage: 128 rows x 16 columns of each operand, 8 doubles per thread
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int e = tid + t * 256;
+      const int r = e >> 4, c = e & 15;
+      const long ga = m0 + r, gb = n0 + r;
+      As[r * TKP + c] = (ga < Mr) ? A[ga * K + k0 + c] : 0.0;
+      Bs[r * TKP + c] = (gb < Nc) ? B[gb * K + k0 + c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[(wm + i * 16 + lr) * TKP + kk + lk];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[(wn + j * 16 + lr) * TKP + kk + lk];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long r = m0 + wm + i * 16 + 4 * v + lk, c = n0 + wn + j * 16 + lr;
+        if (r < Mr && c < Nc) C[r * Nc + c] = acc[i][j][v];
+      }
+}
+
+// C_l = sum_i (ln k_{i+1} - ln k_i) (f_i + f_{i+1}) / 2, f = (k / k_p)^(n_s - 1) Theta_l(k)^2
+__global__ void cell_kernel(const double *__restrict__ th, const double *__restrict__ kk, int n_k, int n_ell, double n_s, double k_p,
+                            double *__restrict__ cl) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n_ell) return;
+  double acc = 0.0, fprev = 0.0, lkprev = 0.0;
+  for (int i = 0; i < n_k; ++i) {
+    const double k = kk[i], t = th[(long)i * n_ell + l];
+    const double f = pow(k / k_p, n_s - 1.0) * t * t, lk = log(k);
+    if (i > 0) acc += (lk - lkprev) * (f + fprev) / 2.0;
+    fprev = f;
+    lkprev = lk;
+  }
+  cl[l] = acc;
+}
+
+__global__ void dell_kernel(const double *__restrict__ cl, int ellmax, double A_s, double Tcmb, double *__restrict__ ell,
+                            double *__restrict__ dl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > ellmax - 2) return;
+  const double l = (double)(i + 2);
+  ell[i] = l;
+  dl[i] = l * (l + 1.0) * cl[i + 2] * A_s * 2.0 * Tcmb * Tcmb * 1e12;
+}
+
+}  // namespace
+}  // namespace deb
+
+// ---------------------------------------------------------------------------------------------
+// launchers (stream-ordered, no allocation)
+// ---------------------------------------------------------------------------------------------
+extern "C" int deb_launch_nu_bins(int nqmax, double *d_qw, hipStream_t stream);
+
+extern "C" long deb_cmb_sources_work_doubles(int num_thermo, int n_k, int n_tau) {
+  return 12L * num_thermo + (long)deb::NV * n_tau + 2L * n_k * n_tau + n_tau + 64;
+}
+
+extern "C" int deb_launch_cmb_sources(const double *d_t, int num_thermo, const double *d_y, const double *d_yp, const double *d_k, int n_k,
+                                      const double *d_aexp, const double *d_tau, int n_tau, const DebProblem *prob, double *d_out,
+                                      double *d_work, hipStream_t stream) {
+  double *vw = d_work, *V = vw + 12L * num_thermo, *pot = V + (long)deb::NV * n_tau, *Spot = pot + (long)n_k * n_tau;
+  double *cp = Spot + (long)n_k * n_tau, *qw = cp + n_tau;
+  int rc = deb_launch_nu_bins(prob->nqmax, qw, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(deb::vis_build_kernel, dim3(1), dim3(64), 0, stream, d_t, vw);
+  hipLaunchKernelGGL(deb::vis_eval_kernel, dim3((n_tau + 127) / 128), dim3(128), 0, stream, d_t, vw, d_tau, d_aexp, n_tau, V);
+  long pts = (long)n_k * n_tau;
+  hipLaunchKernelGGL(deb::source_point_kernel, dim3((unsigned)((pts + 127) / 128)), dim3(128), 0, stream, d_t, d_y, d_yp, d_k, n_k, n_tau,
+                     d_aexp, V, qw, *prob, d_out, pot);
+  hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(1), 0, stream, d_tau, n_tau, cp);
+  hipLaunchKernelGGL(deb::source_isw_kernel, dim3((n_k + 63) / 64), dim3(64), 0, stream, d_tau, n_tau, n_k, cp, pot, Spot, V, d_out);
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_visibility(const double *d_t, int num_thermo, const double *d_tau, const double *d_aexp, int n_tau, double *d_V,
+                                     double *d_work, hipStream_t stream) {
+  hipLaunchKernelGGL(deb::vis_build_kernel, dim3(1), dim3(64), 0, stream, d_t, d_work);
+  hipLaunchKernelGGL(deb::vis_eval_kernel, dim3((n_tau + 127) / 128), dim3(128), 0, stream, d_t, d_work, d_tau, d_aexp, n_tau, d_V);
+  return (int)hipGetLastError();
+}
+
+static long los_kp(int n_fftlog) { return (((long)n_fftlog + 2 + 15) / 16) * 16; }
+
+extern "C" long deb_los_work_doubles(int n_k, int n_tau, int n_ell, int n_fftlog, int n_k_dense) {
+  const long nk = n_k_dense > 0 ? n_k_dense : n_k, Kp = los_kp(n_fftlog);
+  long w = (long)n_fftlog + 3L * n_k + 2L * n_tau;  // twiddles, log k / cp_k, lco / cp_chi
+  if (n_k_dense > 0) w += (long)n_k * n_tau + 2L * n_k_dense + 2L * n_k_dense * n_tau;
+  w += nk * n_tau + nk * Kp + (long)n_ell * Kp;
+  return w + 64;
+}
+
+extern "C" int deb_launch_los(const double *d_k, int n_k, const double *d_tau, int n_tau, const double *d_S, double tau0, int ellmax,
+                              int n_fftlog, int n_k_dense, double *d_theta, double *d_kout, double *d_work, hipStream_t stream) {
+  const int n_ell = ellmax + 1;
+  const long Kp = los_kp(n_fftlog);
+  int logm = 0;
+  while ((2 << logm) < n_fftlog) ++logm;
+  if ((2 << logm) != n_fftlog || logm < 6 || logm > 13) return -2;  // power of two, 128 .. 16384
+  double *w = d_work;
+  double2 *tw = (double2 *)w;
+  w += n_fftlog;
+  double *lk = w; w += n_k;
+  double *cpk = w; w += 2L * n_k;
+  double *lco = w; w += n_tau;
+  double *cpchi = w; w += n_tau;
+  const double *S = d_S;
+  const double *kuse = d_k;
+  int nk = n_k;
+  double *lkd = nullptr;
+  hipLaunchKernelGGL(deb::twiddle_kernel, dim3((n_fftlog / 2 + 255) / 256), dim3(256), 0, stream, n_fftlog, tw);
+  if (n_k_dense > 0) {
+    double *Sk = w; w += (long)n_k * n_tau;
+    lkd = w; w += n_k_dense;
+    double *kd = w; w += n_k_dense;
+    double *Sd = w; w += 2L * n_k_dense * n_tau;  // densified S (the second half is slack)
+    int mx = n_k > n_k_dense ? n_k : n_k_dense;
+    hipLaunchKernelGGL(deb::log_k_kernel, dim3((mx + 255) / 256), dim3(256), 0, stream, d_k, n_k, n_k_dense, lk, lkd, kd);
+    hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(1), 0, stream, lk, n_k, cpk);
+    hipLaunchKernelGGL(deb::densify_build_kernel, dim3((n_tau + 127) / 128), dim3(128), 0, stream, lk, n_k, n_tau, cpk, d_S, Sk);
+    long pts = (long)n_k_dense * n_tau;
+    hipLaunchKernelGGL(deb::densify_eval_kernel, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, stream, lk, n_k, n_tau, d_S, Sk, lkd,
+                       n_k_dense, Sd);
+    S = Sd;
+    kuse = kd;
+    nk = n_k_dense;
+    hipError_t e = hipMemcpyAsync(d_kout, kd, sizeof(double) * n_k_dense, hipMemcpyDeviceToDevice, stream);
+    if (e != hipSuccess) return (int)e;
+  } else {
+    hipError_t e = hipMemcpyAsync(d_kout, d_k, sizeof(double) * n_k, hipMemcpyDeviceToDevice, stream);
+    if (e != hipSuccess) return (int)e;
+  }
+  double *Schi = w; w += (long)nk * n_tau;
+  double *A = w; w += (long)nk * Kp;
+  double *B = w; w += (long)n_ell * Kp;
+  hipLaunchKernelGGL(deb::chi_knots_kernel, dim3((n_tau + 255) / 256), dim3(256), 0, stream, d_tau, n_tau, tau0, lco);
+  hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(1), 0, stream, lco, n_tau, cpchi);
+  const size_t lds = sizeof(double2) * (size_t)(n_fftlog / 2);
+  auto launch_fft = [&](auto kern) -> int {
+    if (lds > 64 * 1024) {
+      hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL(kern, dim3(nk), dim3(256), lds, stream, S, n_tau, lco, cpchi, Schi, kuse, d_tau, tau0, tw, A, Kp);
+    return (int)hipGetLastError();
+  };
+  int rc = 0;
+  switch (logm) {
+    case 6: rc = launch_fft(deb::fftlog_kernel<6>); break;
+    case 7: rc = launch_fft(deb::fftlog_kernel<7>); break;
+    case 8: rc = launch_fft(deb::fftlog_kernel<8>); break;
+    case 9: rc = launch_fft(deb::fftlog_kernel<9>); break;
+    case 10: rc = launch_fft(deb::fftlog_kernel<10>); break;
+    case 11: rc = launch_fft(deb::fftlog_kernel<11>); break;
+    case 12: rc = launch_fft(deb::fftlog_kernel<12>); break;
+    default: rc = launch_fft(deb::fftlog_kernel<13>); break;
+  }
+  if (rc) return rc;
+  long cells = (long)n_ell * (Kp / 2);
+  hipLaunchKernelGGL(deb::bessel_coef_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, stream, n_ell, n_fftlog, d_tau, n_tau,
+                     tau0, B, Kp);
+  dim3 grid((n_ell + deb::TN - 1) / deb::TN, (nk + deb::TM - 1) / deb::TM);
+  hipLaunchKernelGGL(deb::contract_kernel, grid, dim3(256), 0, stream, A, B, d_theta, nk, n_ell, Kp);
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_contract(const double *d_A, const double *d_B, int m, int n, long k, double *d_C, hipStream_t stream) {
+  dim3 grid((n + deb::TN - 1) / deb::TN, (m + deb::TM - 1) / deb::TM);
+  hipLaunchKernelGGL(deb::contract_kernel, grid, dim3(256), 0, stream, d_A, d_B, d_C, m, n, k);
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_cell(const double *d_theta, const double *d_k, int n_k, int n_ell, double n_s, double k_p, double *d_cl,
+                               hipStream_t stream) {
+  hipLaunchKernelGGL(deb::cell_kernel, dim3((n_ell + 127) / 128), dim3(128), 0, stream, d_theta, d_k, n_k, n_ell, n_s, k_p, d_cl);
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_dell(const double *d_cl, int ellmax, double A_s, double Tcmb, double *d_ell, double *d_dl, hipStream_t stream) {
+  if (ellmax < 2) return 0;
+  hipLaunchKernelGGL(deb::dell_kernel, dim3((ellmax + 127) / 128), dim3(128), 0, stream, d_cl, ellmax, A_s, Tcmb, d_ell, d_dl);
+  return (int)hipGetLastError();
+}

@@ -97,6 +97,54 @@ int deb_power_units(const double *d_tables, int num_thermo, const int *d_unit_co
 int deb_time_derivatives(const double *d_tables, int num_thermo, int cosmo, const double *d_y, const double *d_tau, int n_tau,
                          const double *d_k, int n_k, const DebProblem *prob, double *d_dy, double *d_work, void *stream);
 
+/* ---------------------------------------------------------------- CMB chain ---------------- */
+
+/* Visibility functions and background quantities at conformal times d_tau (scale factors
+ * d_aexp), n_tau of each: d_out is 10 x n_tau doubles, rows opac, opac', g, g', g'', tau_c,
+ * aH, rho_nu/rho_nu0, rho_Q/rho_Q(1), w_Q.  d_work: 12 num_thermo doubles.
+ * Replaces compute_visibility_functions (src/discoeb/cmb.py:92-166) and
+ * compute_background_quantities (src/discoeb/background.py:348-406). */
+int deb_visibility(const double *d_tables, int num_thermo, int cosmo, const double *d_tau, const double *d_aexp, int n_tau, double *d_out,
+                   double *d_work, void *stream);
+
+/* Doubles of workspace deb_cmb_sources needs. */
+long deb_cmb_sources_workspace_doubles(int num_thermo, int n_k, int n_tau);
+
+/* CMB temperature source function from full states d_y and their time derivatives d_yprime
+ * (both n_k x n_tau x nvar, as deb_evolve with return_full and deb_time_derivatives write
+ * them) at scale factors d_aexp / conformal times d_tau (n_tau, increasing).  d_out: 7 x n_k x
+ * n_tau doubles, planes S1 (ISW), S2 (Sachs-Wolfe), S3 (Doppler), S4 (polarisation), S, alpha,
+ * alpha'.  Replaces extract_perturbations .. compute_source_function
+ * (src/discoeb/cmb.py:17-633) as chained in compute_Cell_spectrum_from_cosmo_params
+ * (:1050-1073). */
+int deb_cmb_sources(const double *d_tables, int num_thermo, int cosmo, const double *d_y, const double *d_yprime, const double *d_k, int n_k,
+                    const double *d_aexp, const double *d_tau, int n_tau, const DebProblem *prob, double *d_out, double *d_work, void *stream);
+
+/* Doubles of workspace deb_los_theta_ell needs. */
+long deb_los_workspace_doubles(int n_k, int n_tau, int ellmax, int n_fftlog, int n_k_dense);
+
+/* Theta_l(k), l = 0..ellmax, by FFTLog line-of-sight integration of the source d_S (n_k x n_tau)
+ * on wavenumbers d_k and increasing conformal times d_tau, tau0 = conformal time today;
+ * n_fftlog a power of two in [128, 16384]; n_k_dense > 0 first interpolates S onto that many
+ * log-uniform wavenumbers.  d_theta: n_k_used x (ellmax + 1); d_k_out: the n_k_used wavenumbers.
+ * Replaces compute_theta_ell (src/discoeb/cmb.py:680-835) with
+ * _fftlog_bessel_integral_coefficients (:636-677). */
+int deb_los_theta_ell(const double *d_k, int n_k, const double *d_tau, int n_tau, const double *d_S, double tau0, int ellmax, int n_fftlog,
+                      int n_k_dense, double *d_theta, double *d_k_out, double *d_work, void *stream);
+
+/* C = A B^T in FP64 on the matrix cores: A m x k, B n x k (row-major, k a multiple of 16), C
+ * m x n.  The contraction of compute_theta_ell (src/discoeb/cmb.py:805-833, the einsums over
+ * the FFTLog frequencies), exposed on its own. */
+int deb_contract(const double *d_A, const double *d_B, int m, int n, long k, double *d_C, void *stream);
+
+/* C_l = trapezoid over ln k of (k/k_p)^(n_s-1) Theta_l(k)^2: d_theta n_k x n_ell, d_cell n_ell.
+ * Replaces compute_Cell (src/discoeb/cmb.py:838-873). */
+int deb_cell(const double *d_theta, const double *d_k, int n_k, int n_ell, double n_s, double k_p, double *d_cell, void *stream);
+
+/* D_l = l (l + 1) C_l A_s 2 T_cmb^2 1e12 for l = 2..ellmax: d_ell, d_dell (ellmax - 1 each).
+ * Replaces compute_Dell (src/discoeb/cmb.py:876-927). */
+int deb_dell(const double *d_cell, int ellmax, double A_s, double Tcmb, double *d_ell, double *d_dell, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

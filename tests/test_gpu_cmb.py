@@ -1,0 +1,169 @@
+"""CMB chain on the GPU: sources, line of sight, C_l, D_l.
+
+Each stage is compared with the oracle (oracle/cmb_oracle.py) on identical inputs -- the GPU's
+own background tables, states and time derivatives -- so the comparisons are at rounding level.
+The end-to-end check against the reference's fixtures is loose: those came from the
+reference's batched (shared-step) run, whose step sequence this solver does not reproduce.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import disco_notebook, load_package
+from oracle import cmb_oracle as C
+from test_gpu_parity import oracle_on_gpu_tables
+
+pytestmark = pytest.mark.gpu
+LM = dict(lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3)
+
+
+@pytest.fixture(scope="module")
+def deb():
+    if not torch.cuda.is_available():
+        pytest.skip("no ROCm device")
+    return load_package()
+
+
+def _np(t):
+    return t.detach().cpu().numpy() if torch.is_tensor(t) else np.asarray(t)
+
+
+def _bgd(deb, oracle, p, aexp):
+    """Background pieces for the oracle chain, taken from the GPU's own tables."""
+    bg = oracle_on_gpu_tables(deb, oracle, p)
+    q, w = oracle.nu_bins(p.get("nqmax", 3))
+    d = {k: float(p[k]) for k in ["grhom", "grhog", "grhor", "OmegaDE", "amnu"]}
+    d.update({k: float(p[k]) for k in ["Omegam", "Omegab", "Neff", "Nmnu", "w_DE_0", "w_DE_a"]})
+    d["Omegak"] = float(p.get("Omegak", 0.0))
+    d.update(rhonu=np.exp(bg.spline("logrhonu_of_loga", np.log(aexp))), q=q, w=w)
+    for name, key in [("tau", "tau_tab"), ("opac", "opac"), ("gvis", "gvis"), ("gvisprime", "gvisprime"), ("gvispprime", "gvispprime"),
+                      ("optical_depth", "optical_depth")]:
+        d[key] = _np(p[name])
+    return d
+
+
+def test_contract_matches_numpy(deb):
+    print("case: deb_contract m,n,k = (150,133,80) (3,300,16) (257,129,512)", flush=True)
+    L = deb._lib.lib()
+    rng = np.random.default_rng(1)
+    for m, n, k in [(150, 133, 80), (3, 300, 16), (257, 129, 512)]:
+        A, B = rng.standard_normal((m, k)), rng.standard_normal((n, k))
+        Ad, Bd = torch.tensor(A, device="cuda"), torch.tensor(B, device="cuda")
+        Cd = torch.empty((m, n), dtype=torch.float64, device="cuda")
+        deb._lib.check(L.deb_contract(deb._lib.ptr(Ad), deb._lib.ptr(Bd), m, n, k, deb._lib.ptr(Cd), deb._lib.stream()))
+        ref = A @ B.T
+        err = np.abs(_np(Cd) - ref).max() / np.abs(ref).max()
+        print(f"  {m}x{n}x{k} max scaled err {err:.2e}")
+        assert err < 1e-13
+    assert L.deb_contract(None, None, 4, 4, 17, None, None) < 0
+
+
+def _cmb_grid(deb):
+    a = np.concatenate([deb.perturbations._geomspace(3e-4, 5e-3, 256, endpoint=False), deb.perturbations._geomspace(5e-3, 1.0, 64)])
+    return a
+
+
+def test_visibility_matches_oracle_and_fixture(deb, oracle, golden):
+    print("case: visibility DISCO-Notebook num_thermo=1024, 320 output times", flush=True)
+    p = deb.evolve_background(param=disco_notebook(), num_thermo=1024)
+    aexp = _cmb_grid(deb)
+    tau = p["tau_of_a_spline"].evaluate(aexp)
+    v = deb.compute_visibility_functions(tau, p, aexp=aexp)
+    d = _bgd(deb, oracle, p, aexp)
+    o = C.visibility_functions(_np(tau), d["tau_tab"], d["opac"], d["gvis"], d["gvisprime"], d["gvispprime"], d["optical_depth"])
+    o["aprimeoa"] = C.aprimeoa(aexp, d, d["rhonu"])
+    o["rhonu"] = d["rhonu"]
+    for name in ["opac", "opacprime", "gvis", "gvisprime", "gvispprime", "aprimeoa", "rhonu"]:
+        err = np.abs(_np(v[name]) - o[name]).max() / np.abs(o[name]).max()
+        print(f"  {name:10s} max scaled err {err:.2e}")
+        assert err < 1e-11, name
+    assert np.abs(np.exp(-_np(v["optical_depth"])) - np.exp(-o["optical_depth"])).max() < 1e-12
+    g = golden("visibility_functions")
+    for name in ["gvis", "gvisprime"]:
+        ref = g[name]
+        err = np.abs(_np(v[name]) - ref) / np.maximum(np.abs(ref), 1e-6 * np.abs(ref).max())
+        assert err.max() < 1e-5, name
+    assert np.abs(np.exp(-_np(v["optical_depth"])) - np.exp(-g["optical_depth"])).max() < 1e-6
+
+
+@pytest.fixture(scope="module")
+def small_cmb_run(deb):
+    """DISCO-Notebook, 6 log-spaced k in 1e-3..0.3, 90 output times 1e-3..1, full states."""
+    p = deb.evolve_background(param=disco_notebook(), num_thermo=1024)
+    aexp = deb.perturbations._geomspace(1e-3, 1.0, 90)
+    y, k, p = deb.evolve_perturbations(param=p, aexp_out=aexp, kmin=1e-3, kmax=0.3, num_k=6, rtol=1e-4, atol=1e-4, return_full=True,
+                                       k_sampling_method="log", **LM)
+    tau = p["tau_out"]
+    yp = deb.compute_time_derivatives(y, tau, k, p, *LM.values())
+    return p, aexp, y, yp, k, tau
+
+
+def test_sources_match_oracle(deb, oracle, small_cmb_run):
+    print("case: sources 6 k x 90 tau (DISCO-Notebook, massive nu)", flush=True)
+    p, aexp, y, yp, k, tau = small_cmb_run
+    src = deb.compute_source_function_from_states(y, yp, k, aexp, tau, p, **LM)
+    d = _bgd(deb, oracle, p, aexp)
+    o = C.sources_from_states(_np(y), _np(yp), _np(k), aexp, _np(tau), d, 11, 11, 11, 3)
+    o["alpha"], o["alphaprime"] = o["metric"]["alpha"], o["metric"]["alphaprime"]
+    for name in ["S1", "S2", "S3", "S4", "S", "alpha", "alphaprime"]:
+        ref = o[name]
+        err = np.abs(_np(src[name]) - ref).max() / np.abs(ref).max()
+        print(f"  {name:10s} max scaled err {err:.2e}")
+        assert err < 1e-9, name
+
+
+@pytest.mark.parametrize("n_k_dense", [0, 40])
+def test_theta_ell_cell_dell_match_oracle(deb, golden, small_cmb_run, n_k_dense):
+    print(f"case: line of sight ellmax=300 n_fftlog=4096 n_k_dense={n_k_dense}", flush=True)
+    p, aexp, y, yp, k, tau = small_cmb_run
+    src = deb.compute_source_function_from_states(y, yp, k, aexp, tau, p, **LM)
+    tau0 = float(p["tau0"])
+    th, kk = deb.compute_theta_ell(300, k, tau, src["S"], tau0, n_fftlog=4096, n_k_dense=n_k_dense)
+    tho, kko = C.compute_theta_ell(300, _np(k), _np(tau), _np(src["S"]), tau0, n_fftlog=4096, n_k_dense=n_k_dense)
+    assert np.allclose(_np(kk), kko, rtol=1e-14, atol=0)
+    err = np.abs(_np(th) - tho).max() / np.abs(tho).max()
+    print(f"  Theta_l max scaled err {err:.2e}")
+    assert err < 1e-10
+    cl = deb.compute_Cell(th, kk, p["n_s"], p["k_p"])
+    clo = C.compute_Cell(_np(th), _np(kk), p["n_s"], p["k_p"])
+    assert np.abs(_np(cl) - clo).max() / np.abs(clo).max() < 1e-12
+    ell, dl = deb.compute_Dell(torch.ones(2501, dtype=torch.float64, device="cuda"), 2.1064e-09, 2.7255, ellmax=2500)
+    g = golden("compute_dell")
+    np.testing.assert_array_equal(_np(ell), g["ell"])
+    np.testing.assert_allclose(_np(dl), g["Dell"], rtol=1e-14)
+
+
+def test_chain_against_reference_fixtures(deb, golden):
+    """The reference's CMB test configuration (tests/test_cmb.py:93-240): DISCO-Notebook,
+    num_thermo 1024, 128 k in 1e-4..1, 320 output times, rtol = atol = 1e-4, full states;
+    Theta_l and C_l for l <= 100 (n_fftlog 16384, no densification).  The fixtures were made with
+    log-spaced k (their k63 reproduces only with that sampling) by the reference's batched
+    shared-step solver, so this is a regression check at the reference's own tolerance (rtol
+    0.2), not parity: alpha = (h' + 6 eta') / 2k^2 turns solver-level differences into large
+    differences at small k.  Against a converged run (rtol 1e-8) this solver's C_l at rtol 1e-4
+    is off by a median 6e-3, the fixture by 1.2e-1."""
+    print("case: reference CMB fixture configuration (128 log k x 320 tau), ellmax 100", flush=True)
+    p = deb.evolve_background(param=disco_notebook(), num_thermo=1024)
+    aexp = _cmb_grid(deb)
+    y, k, p = deb.evolve_perturbations(param=p, aexp_out=aexp, kmin=1e-4, kmax=1.0, num_k=128, rtol=1e-4, atol=1e-4, return_full=True,
+                                       k_sampling_method="log", **LM)
+    tau = p["tau_out"]
+    yp = deb.compute_time_derivatives(y, tau, k, p, *LM.values())
+    src = deb.compute_source_function_from_states(y, yp, k, aexp, tau, p, **LM)
+    th, kk = deb.compute_theta_ell(100, k, tau, src["S"], float(p["tau0"]))
+    cl = _np(deb.compute_Cell(th, kk, p["n_s"], p["k_p"]))
+    g = golden("cmb_chain")
+    rel = np.abs(cl[2:] / g["Cell"][2:] - 1)
+    print(f"  C_l (l = 2..100) vs reference: median rel {np.median(rel):.2e}, max rel {rel.max():.2e}")
+    assert np.median(rel) < 0.2
+    for name, plane, ik in [("alpha_k63", "alpha", 63), ("alpha_k127", "alpha", 127), ("source_sw_k63", "S2", 63),
+                            ("source_sw_k127", "S2", 127), ("source_doppler_k63", "S3", 63), ("source_doppler_k127", "S3", 127),
+                            ("source_polarization_k63", "S4", 63), ("source_polarization_k127", "S4", 127)]:
+        e = np.abs(_np(src[plane][ik]) - g[name]).max() / np.abs(g[name]).max()
+        print(f"  {name}: max scaled diff {e:.2e}")
+        assert e < 0.2, name
+    # Theta_l(k = 1) for l <= 100 (k chi >> l) sits at the FFTLog noise floor (~4e-7): reported only
+    e = np.abs(_np(th[127]) - g["theta_ell_k_max"]).max() / np.abs(g["theta_ell_k_max"]).max()
+    print(f"  theta_ell_k_max: max scaled diff {e:.2e}")
