@@ -25,7 +25,7 @@ extern "C" int deb_launch_power_units(const double *, int, const int *, const do
                                       hipStream_t);
 extern "C" int deb_launch_perturb(const double *, int, const int *, const double *, int, const double *, int, const double *,
                                   const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, int *, const int *, int,
-                                  int, hipStream_t);
+                                  int, const double *, long, const int *, hipStream_t);
 extern "C" long deb_perturb_lds_bytes(int, int, int, int, int);
 extern "C" int deb_launch_spline_eval(const double *, int, int, int, const double *, int, double *, hipStream_t);
 extern "C" int deb_launch_power(const double *, int, const double *, long, int, int, double, double, double, double *, hipStream_t);
@@ -86,9 +86,10 @@ int deb_spline_eval(const double *d_tables, int num_thermo, int cosmo, int splin
   return hip_rc(deb_launch_spline_eval(t, num_thermo, spline_id, kind, d_x, nx, d_out, (hipStream_t)stream), "deb_spline_eval");
 }
 
-int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
-               const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
-               int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const DebSchedule *sched, void *stream) {
+static int evolve_common(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
+                         const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
+                         int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const DebSchedule *sched,
+                         const double *d_replay, long replay_stride, const int *d_replay_len, void *stream) {
   int rc = check_problem(prob);
   if (rc) return rc;
   if (!cfg) return fail(-1, "solver descriptor is NULL");
@@ -116,9 +117,27 @@ int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, 
     nwg = sched->n_workgroups;
   }
   rc = deb_launch_perturb(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, d_work, prob, cfg, d_step_log, d_y_out,
-                          d_status, d_n_accept, d_n_reject, queue, order, chunk, nwg, (hipStream_t)stream);
+                          d_status, d_n_accept, d_n_reject, queue, order, chunk, nwg, d_replay, replay_stride, d_replay_len,
+                          (hipStream_t)stream);
   if (rc == -2) return fail(-1, "state too large for LDS");
   return hip_rc(rc, "deb_evolve");
+}
+
+int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units, const double *d_tau_out,
+               int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status, int *d_n_accept, int *d_n_reject,
+               double *d_work, double *d_step_log, const DebSchedule *sched, void *stream) {
+  return evolve_common(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, prob, cfg, d_y_out, d_status, d_n_accept,
+                       d_n_reject, d_work, d_step_log, sched, nullptr, 0, nullptr, stream);
+}
+
+int deb_evolve_replay(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
+                      const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, const double *d_mesh, long mesh_stride,
+                      const int *d_mesh_len, double *d_y_out, int *d_status, int *d_n_accept, int *d_n_reject, double *d_work,
+                      const DebSchedule *sched, void *stream) {
+  if (n_units > 0 && (!d_mesh || !d_mesh_len)) return fail(-1, "NULL device pointer");
+  if (mesh_stride < 1) return fail(-1, "mesh_stride must be >= 1");
+  return evolve_common(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, prob, cfg, d_y_out, d_status, d_n_accept,
+                       d_n_reject, d_work, nullptr, sched, d_mesh, mesh_stride, d_mesh_len, stream);
 }
 
 int deb_tau_out(const double *d_tables, int num_thermo, int n_cosmo, const double *d_aexp_out, int nout, double *d_tau_out, void *stream) {

@@ -856,7 +856,8 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
                                            const double *__restrict__ unit_k, const double *__restrict__ tau_out, int nout,
                                            const DebProblem &prob, const DebSolverCfg &cfg, double *__restrict__ step_log,
                                            double *__restrict__ y_out, int *__restrict__ status, int *__restrict__ n_acc,
-                                           int *__restrict__ n_rej, const PLay &L, Ws &W, int lane) {
+                                           int *__restrict__ n_rej, const double *__restrict__ replay, long replay_stride,
+                                           const int *__restrict__ replay_len, const PLay &L, Ws &W, int lane) {
   const int cosmo = unit_cosmo[unit];
   const double k = unit_k[unit];
   const int n = L.n;
@@ -886,6 +887,13 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
 
   double tprev = t0;
   double tnext = fmin(t0 + fmin(t0 / 4.0, 0.5 * (t1 - t0)), t1);
+  // replay: the accepted step ends are given as s = (t - t0) / (t1 - t0), increasing to 1; the
+  // steps are taken as they are, with no error control (the frozen-mesh map)
+  const double *rs = (replay != nullptr) ? replay + (long)unit * replay_stride : nullptr;
+  const int rlen = (replay != nullptr) ? replay_len[unit] : 0;
+  int ridx = 0;
+  auto mesh_t = [&](int i) { return (rs[i] >= 1.0) ? t1 : t0 + rs[i] * (t1 - t0); };
+  if (rs != nullptr) tnext = (rlen > 0) ? mesh_t(0) : t1;
   Pid pid;
   pid.prev_inv = 1.0;
   pid.prev_prev_inv = 1.0;
@@ -893,7 +901,7 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   double *K = W.K;
   double *u = W.u, *f = W.f, *y0 = W.y0, *dT = W.dT;
   while (tprev < t1) {
-    if (steps >= cfg.max_steps) { st = 1; break; }
+    if (steps >= cfg.max_steps || (rs != nullptr && ridx >= rlen)) { st = 1; break; }
     const double dt = tnext - tprev;
     double W00;
     factorise<(NQ > 0 ? 17 + 3 * NQ : 0)>(b, L, k, tprev, dt, W, lane, W00);
@@ -978,8 +986,8 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
       lg[2] = en;
     }
     if (!isfinite(en)) { st = 2; break; }
-    bool keep = en < 1.0;
-    double factor = pid.adapt(en, keep, cfg);
+    bool keep = (rs != nullptr) || en < 1.0;
+    double factor = (rs != nullptr) ? 1.0 : pid.adapt(en, keep, cfg);
     if (keep) {
       while (isave < nout && tout[isave] <= tnext) {
         double th = (tout[isave] - tprev) / (tnext - tprev);
@@ -999,6 +1007,10 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
     tprev = fmin(nt0, t1);
     double tn = nt0 + dt * factor;
     tnext = (tn > t1 - 1e-10) ? t1 : tn;
+    if (rs != nullptr) {
+      ++ridx;
+      tnext = (ridx < rlen) ? mesh_t(ridx) : t1;
+    }
     ++steps;
   }
   if (st != 0) {
@@ -1022,7 +1034,9 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
                                                        int nout, const double *__restrict__ nu_qw, DebProblem prob, DebSolverCfg cfg,
                                                        long lds_bytes, double *__restrict__ step_log, double *__restrict__ y_out,
                                                        int *__restrict__ status, int *__restrict__ n_acc, int *__restrict__ n_rej,
-                                                       int *__restrict__ queue, const int *__restrict__ order, int chunk) {
+                                                       int *__restrict__ queue, const int *__restrict__ order, int chunk,
+                                                       const double *__restrict__ replay, long replay_stride,
+                                                       const int *__restrict__ replay_len) {
   if (!QUEUE && (int)blockIdx.x >= n_units) return;
   const int lane = threadIdx.x;
   PLay L;
@@ -1062,8 +1076,8 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
   setup_maps(L, W, lane);
 
   if constexpr (!QUEUE) {
-    solve_unit<NQ>((int)blockIdx.x, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, y_out, status, n_acc, n_rej, L,
-                   W, lane);
+    solve_unit<NQ>((int)blockIdx.x, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, y_out, status, n_acc, n_rej,
+                   replay, replay_stride, replay_len, L, W, lane);
     return;
   }
   for (;;) {
@@ -1074,8 +1088,8 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
     const int last = min(first + chunk, n_units);
     for (int pos = first; pos < last; ++pos) {
       const int unit = order ? order[pos] : pos;
-      solve_unit<NQ>(unit, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, y_out, status, n_acc, n_rej, L, W,
-                     lane);
+      solve_unit<NQ>(unit, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, y_out, status, n_acc, n_rej, replay,
+                     replay_stride, replay_len, L, W, lane);
     }
   }
 }
@@ -1179,7 +1193,8 @@ extern "C" int deb_launch_tau_out(const double *d_tables, int num_thermo, int n_
 extern "C" int deb_launch_perturb(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
                                   const double *d_tau_out, int nout, const double *d_nu_qw, const DebProblem *prob,
                                   const DebSolverCfg *cfg, double *d_step_log, double *d_y_out, int *d_status, int *d_nacc, int *d_nrej,
-                                  int *d_queue, const int *d_order, int chunk, int n_workgroups, hipStream_t stream) {
+                                  int *d_queue, const int *d_order, int chunk, int n_workgroups, const double *d_replay,
+                                  long replay_stride, const int *d_replay_len, hipStream_t stream) {
   long stride = deb_table_doubles(num_thermo);
   long lds = deb_perturb_lds_bytes(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
   if (lds > 160 * 1024) return -2;
@@ -1212,6 +1227,6 @@ extern "C" int deb_launch_perturb(const double *d_tables, int num_thermo, const 
   }
   hipLaunchKernelGGL(kern, dim3(grid), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units,
                      d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej, d_queue, d_order,
-                     chunk > 0 ? chunk : 1);
+                     chunk > 0 ? chunk : 1, d_replay, replay_stride, d_replay_len);
   return (int)hipGetLastError();
 }

@@ -1,0 +1,137 @@
+"""P(k) and its derivatives with respect to nine sampling parameters.
+
+Parameters theta = (omega_c, omega_b, h, n_s, ln(1e10 A_s), m_nu, w0, wa, tau_reion), mapped
+to the model's Omega_m = (omega_c + omega_b) / h^2, Omega_b = omega_b / h^2, H0 = 100 h,
+A_s = exp(theta_4) / 1e10.  The reference differentiates its whole pipeline with jax.jacfwd
+(notebooks; DirectAdjoint in perturbations.py:919,1079).  Here the derivative is that of the
+frozen-mesh map: the adaptive base solve fixes each mode's accepted step ends (as fractions of
+its integration range), and the six parameters that enter the background and perturbations
+(omega_c, omega_b, h, m_nu, w0, wa) are differentiated by central differences of that map --
+all twelve shifted cosmologies in one replay launch.  n_s and A_s enter P(k) only through the
+primordial spectrum (analytic columns), and tau_reion does not enter the reference's model
+(zero column).
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .background import background_tables
+from .batch import power_units, tau_out_all, units_for
+from .perturbations import _raise_on_status, evolve_units
+
+PARAM9 = ("omega_c", "omega_b", "h", "n_s", "ln10As", "m_nu", "w0", "wa", "tau_reion")
+ODE_PARAMS = (0, 1, 2, 5, 6, 7)
+# Relative central-difference step.  The frozen-mesh map still holds discrete decisions (the
+# approximation switches, the thermal-history solver's own adaptive steps); where one flips
+# inside +-step the difference quotient carries the jump.  Steps of 1e-5 and 3e-5 agree to
+# about 4e-5 relative; larger steps cross more such decisions.
+REL_STEP = 3e-5
+
+
+def theta_to_param(theta, base):
+    """Model parameter dict for theta (9 values), other keys from base."""
+    oc, ob, h, ns, lnA, mnu, w0, wa, _tr = [float(v) for v in theta]
+    p = dict(base)
+    p.update(Omegam=(oc + ob) / h ** 2, Omegab=ob / h ** 2, H0=100.0 * h, n_s=ns, A_s=math.exp(lnA) / 1e10, mnu=mnu, w_DE_0=w0,
+             w_DE_a=wa)
+    return p
+
+
+def param_to_theta(param, tau_reion=0.054):
+    h = param["H0"] / 100.0
+    ob = param["Omegab"] * h ** 2
+    oc = param["Omegam"] * h ** 2 - ob
+    return np.array([oc, ob, h, param["n_s"], math.log(1e10 * param["A_s"]), param["mnu"], param["w_DE_0"], param["w_DE_a"],
+                     tau_reion])
+
+
+def _accepted_mesh(log, nacc, t1):
+    """Accepted step ends of every unit as fractions s = (t - t0) / (t1 - t0), the last exactly 1."""
+    lg = log.cpu().numpy()
+    na = nacc.cpu().numpy()
+    n_units = lg.shape[0]
+    L = int(na.max())
+    mesh = np.ones((n_units, L))
+    for u in range(n_units):
+        rows = lg[u]
+        ok = np.isfinite(rows[:, 2]) & (rows[:, 2] < 1.0)
+        tn = rows[ok, 1]
+        t0 = rows[0, 0]
+        s = (tn - t0) / (t1[u] - t0)
+        s[-1] = 1.0
+        mesh[u, :s.size] = s
+    return torch.tensor(mesh), torch.tensor(na, dtype=torch.int32)
+
+
+def pk_jacobian(theta, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, num_thermo=256, schedule=None, lmaxg=12, lmaxgp=12, lmaxr=12,
+                lmaxnu=17, nqmax=8, **solver):
+    """P(k) at aexp_out and its Jacobian with respect to theta.
+
+    Returns (pk (n_k, nout), J (n_k, nout, 9)) device tensors."""
+    theta = np.asarray(theta, dtype=np.float64)
+    p0 = theta_to_param(theta, base)
+    dev = _lib.device()
+    lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
+    kk = torch.as_tensor(np.asarray(kmodes, dtype=np.float64), device=dev)
+    nk = kk.numel()
+    aexp = np.atleast_1d(np.asarray(aexp_out, dtype=np.float64))
+    # base solve with its step log
+    t0 = background_tables([p0], num_thermo)
+    tau0 = tau_out_all(t0, num_thermo, aexp)
+    uc0 = torch.zeros(nk, dtype=torch.int32, device=dev)
+    y, st, na, nr, log = evolve_units(t0, num_thermo, uc0, kk, tau0, **lm, **solver, step_log=True)
+    _raise_on_status(st)
+    pk = power_units(t0, num_thermo, uc0, kk, y, idx)
+    t1 = np.full(nk, float(tau0.max().cpu()))
+    mesh, mlen = _accepted_mesh(log, na, t1)
+    # twelve shifted cosmologies, replayed on the base mesh in one launch
+    steps, params = [], []
+    for j in ODE_PARAMS:
+        d = rel_step * max(abs(theta[j]), 1e-2)
+        steps.append(d)
+        for sgn in (1.0, -1.0):
+            th = theta.copy()
+            th[j] += sgn * d
+            params.append(theta_to_param(th, base))
+    tab = background_tables(params, num_thermo)
+    tau = tau_out_all(tab, num_thermo, aexp)
+    uc, uk, _ = units_for([kk.cpu().numpy()] * len(params), dev)
+    reps = len(params)
+    yr, str_, _, _ = evolve_units(tab, num_thermo, uc, uk, tau, **lm, **solver, schedule=schedule,
+                                  replay=(mesh.repeat(reps, 1), mlen.repeat(reps)))
+    _raise_on_status(str_)
+    pkr = power_units(tab, num_thermo, uc, uk, yr, idx).reshape(reps, nk, -1)
+    J = torch.zeros((nk, pk.shape[1], 9), dtype=torch.float64, device=dev)
+    for i, j in enumerate(ODE_PARAMS):
+        J[:, :, j] = (pkr[2 * i] - pkr[2 * i + 1]) / (2.0 * steps[i])
+    J[:, :, 3] = pk * torch.log(kk / p0.get("k_p", 0.05))[:, None]
+    J[:, :, 4] = pk
+    return pk, J
+
+
+class PkFunction(torch.autograd.Function):
+    """P(k, a_out) as a differentiable function of theta (9,): backward returns J^T g."""
+
+    @staticmethod
+    def forward(ctx, theta, kmodes, base, opts):
+        pk, J = pk_jacobian(theta.detach().cpu().numpy(), base, kmodes, **opts)
+        ctx.save_for_backward(J)
+        return pk.to(theta.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        (J,) = ctx.saved_tensors
+        gt = torch.einsum("kon,ko->n", J, g.to(J.device))
+        return gt.to(g.device), None, None, None
+
+
+def pk_and_vjp(theta, base, kmodes, cotangent, **opts):
+    """(P(k), d<cotangent, P>/d theta): P at aexp_out (default a = 1) and the 9-parameter VJP."""
+    pk, J = pk_jacobian(theta, base, kmodes, **opts)
+    c = torch.as_tensor(np.asarray(cotangent, dtype=np.float64), device=J.device).reshape(pk.shape)
+    return pk, torch.einsum("kon,ko->n", J, c)
+
+
+__all__ = ["PARAM9", "theta_to_param", "param_to_theta", "pk_jacobian", "PkFunction", "pk_and_vjp"]

@@ -62,7 +62,7 @@ def nvar(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax):
 
 def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3,
                  rtol=1e-4, atol=1e-4, pcoeff=0.25, icoeff=0.80, dcoeff=0.0, factormax=20.0, factormin=0.3, max_steps=2048,
-                 return_full=False, out=None, step_log=False, schedule=None):
+                 return_full=False, out=None, step_log=False, schedule=None, replay=None):
     """Low-level launch: independent adaptive solves of (cosmology, k) units.
 
     tables: (n_cosmo, size) device tensor from background_tables; unit_cosmo (int32) and unit_k
@@ -70,6 +70,9 @@ def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, l
     schedule: None (one workgroup per unit) or a dict with keys chunk (units claimed per
     queue access, default 1), n_workgroups (persistent workgroups, 0 = as many as fit) and
     order (None, "k_desc" = largest k first, or an int32 device tensor of unit indices).
+    replay: None, or (mesh (n_units, L) float64, mesh_len (n_units,) int32) device tensors: each
+    unit then takes exactly its accepted steps, ending at t0 + s (t1 - t0) for the listed s
+    (the frozen-mesh map, see deb_evolve_replay).
     Returns (y, status, n_accept, n_reject) device tensors, plus the step mesh
     (n_units, max_steps, 3: t_prev, t_next, error norm) when step_log is set; asynchronous on
     the current stream."""
@@ -92,10 +95,21 @@ def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, l
     to = tau_out.to(device=dev, dtype=torch.float64).contiguous()
     log = torch.full((n_units, int(max_steps), 3), float("nan"), dtype=torch.float64, device=dev) if step_log else None
     sched, order = _schedule(schedule, uk)
-    _lib.check(L.deb_evolve(_lib.ptr(tables), int(num_thermo), _lib.ptr(uc), _lib.ptr(uk), n_units, _lib.ptr(to), nout,
-                            ctypes.byref(prob), ctypes.byref(cfg), _lib.ptr(y), _lib.ptr(status), _lib.ptr(nacc),
-                            _lib.ptr(nrej), _lib.ptr(work), _lib.ptr(log) if log is not None else None,
-                            ctypes.byref(sched) if sched is not None else None, _lib.stream()))
+    sref = ctypes.byref(sched) if sched is not None else None
+    if replay is not None:
+        if step_log:
+            raise ValueError("step_log and replay are exclusive")
+        mesh = replay[0].to(device=dev, dtype=torch.float64).contiguous()
+        mlen = replay[1].to(device=dev, dtype=torch.int32).contiguous()
+        if mesh.ndim != 2 or mesh.shape[0] != n_units or mlen.numel() != n_units:
+            raise ValueError("replay mesh must be (n_units, L) with one length per unit")
+        _lib.check(L.deb_evolve_replay(_lib.ptr(tables), int(num_thermo), _lib.ptr(uc), _lib.ptr(uk), n_units, _lib.ptr(to), nout,
+                                       ctypes.byref(prob), ctypes.byref(cfg), _lib.ptr(mesh), mesh.shape[1], _lib.ptr(mlen), _lib.ptr(y),
+                                       _lib.ptr(status), _lib.ptr(nacc), _lib.ptr(nrej), _lib.ptr(work), sref, _lib.stream()))
+    else:
+        _lib.check(L.deb_evolve(_lib.ptr(tables), int(num_thermo), _lib.ptr(uc), _lib.ptr(uk), n_units, _lib.ptr(to), nout,
+                                ctypes.byref(prob), ctypes.byref(cfg), _lib.ptr(y), _lib.ptr(status), _lib.ptr(nacc),
+                                _lib.ptr(nrej), _lib.ptr(work), _lib.ptr(log) if log is not None else None, sref, _lib.stream()))
     del order  # stream-ordered: the allocator reuses it only after the launch has run
     if step_log:
         return y, status, nacc, nrej, log

@@ -70,6 +70,17 @@ int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, 
                const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
                int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const DebSchedule *sched, void *stream);
 
+/* deb_evolve on a frozen step mesh: unit u takes exactly d_mesh_len[u] steps whose ends are
+ * t0 + s (t1 - t0) for the increasing s = d_mesh[u * mesh_stride + i] (the last one 1), with
+ * no error control or rejection; t0 and t1 are the unit's own start and final output times.
+ * The map from parameters to outputs with the mesh held fixed is what gradients differentiate.
+ * Replaces, for differentiation, the forward-mode pass of jax.jacfwd through diffeqsolve
+ * (src/discoeb/perturbations.py:904-925 with DirectAdjoint), with step sizes held fixed. */
+int deb_evolve_replay(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
+                      const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, const double *d_mesh, long mesh_stride,
+                      const int *d_mesh_len, double *d_y_out, int *d_status, int *d_n_accept, int *d_n_reject, double *d_work,
+                      const DebSchedule *sched, void *stream);
+
 /* Conformal output times tau_out[c, j] = tau(a = d_aexp_out[j]) of every cosmology c, from its
  * tau_of_a spline; d_tau_out: n_cosmo x nout doubles.
  * Replaces tau_out = param['tau_of_a_spline'].evaluate(aexp_out) (perturbations.py:1200-1206). */

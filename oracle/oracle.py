@@ -179,6 +179,11 @@ class Background:
     def start_time(self, k):
         return self._lib.orc_start_time(self._h, float(k))
 
+    def start_tau(self, k, aexp_out):
+        """Initial conformal time of one mode: 0.99 min(min tau_out, start_time(k)) (perturbations.py:897)."""
+        tau, tp = _d(self.spline("tau_of_a", np.atleast_1d(np.asarray(aexp_out, dtype=np.float64))))
+        return self._lib.orc_start_tau(self._h, float(k), tp, tau.size)
+
     def convert(self, lmax, k, y):
         lm, lp = _i(lmax)
         y, yp = _d(y)

@@ -167,3 +167,20 @@ def test_chain_against_reference_fixtures(deb, golden):
     # Theta_l(k = 1) for l <= 100 (k chi >> l) sits at the FFTLog noise floor (~4e-7): reported only
     e = np.abs(_np(th[127]) - g["theta_ell_k_max"]).max() / np.abs(g["theta_ell_k_max"]).max()
     print(f"  theta_ell_k_max: max scaled diff {e:.2e}")
+
+
+def test_cell_spectrum_driver_composes_the_stages(deb):
+    """compute_Cell_spectrum_from_cosmo_params (cmb.py:930-1090) equals the stages called one by one."""
+    print("case: C_l driver nmodes=24 ellmax=200 n_k_dense=128 n_fftlog=4096", flush=True)
+    kw = dict(ellmax=200, nmodes=24, kmin=1e-4, kmax=0.5, n_k_dense=128, n_fftlog=4096)
+    cl, p = deb.compute_Cell_spectrum_from_cosmo_params(disco_notebook(), **kw)
+    q = deb.evolve_background(param=disco_notebook(), num_thermo=1024)
+    aexp = deb.cmb_aexp_out()
+    y, k, q = deb.evolve_perturbations(param=q, aexp_out=aexp, kmin=1e-4, kmax=0.5, num_k=24, rtol=1e-4, atol=1e-4, return_full=True,
+                                       k_sampling_method="camb", **LM)
+    yp = deb.compute_time_derivatives(y, q["tau_out"], k, q, *LM.values())
+    src = deb.compute_source_function_from_states(y, yp, k, aexp, q["tau_out"], q, **LM)
+    th, kk = deb.compute_theta_ell(200, k, q["tau_out"], src["S"], float(q["tau0"]), n_fftlog=4096, n_k_dense=128)
+    cl2 = deb.compute_Cell(th, kk, q["n_s"], q["k_p"])
+    assert torch.equal(cl, cl2)
+    assert p["nout"] == 320 and (cl[2:] > 0).all()
