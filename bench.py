@@ -7,7 +7,12 @@ momentum grid with lmax_nu 17, w0 = -0.95, wa = 0.1), then P(k) at z = 2, 1, 0.
 
 Prints one JSON line: k-mode ODE solves/s, the ODE kernel's roofline position (FP64 FLOP/s
 from the kernel's own step counters, timed with HIP events on the launch stream) and the
-CPU oracle timed on a bounded sample of the same modes.
+CPU oracle timed on a bounded sample of the same modes.  After the timed steps (outside
+them) it also measures, unless switched off:
+  * gradients: P(k, z = 0) and its 9-parameter VJP for 256 k-modes log-spaced in 1e-4..1/Mpc
+    (same cosmology and solver settings), evaluations/s;
+  * line of sight: Theta_l(k) for 3000 k x 2500 conformal times, l = 0..2500, 16384-point
+    FFTLog (synthetic smooth sources), and its FP64 matrix-core contraction on its own.
 """
 import argparse
 import json
@@ -70,12 +75,74 @@ def cpu_baseline(kmodes, sample):
                       f"included, oracle C++ dense-LU Rodas5, OpenMP over modes, {dt:.1f} s"}
 
 
+def measure_gradient(deb, nk, repeats=2):
+    """P(k, z=0) plus its 9-parameter VJP (seeded standard-normal cotangent) per second."""
+    import torch
+    P = workload_param()
+    theta = deb.param_to_theta(P)
+    k = deb.k_grid({}, 1e-4, 1.0, nk, "log")
+    cot = np.random.default_rng(0).standard_normal(nk)
+    opts = dict(**LMAX, **SOLVER, aexp_out=(1.0,), schedule={"order": "k_desc"})
+    deb.pk_and_vjp(theta, P, k, cot, **opts)  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(repeats):
+        pk, g = deb.pk_and_vjp(theta, P, k, cot, **opts)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / repeats
+    return {"gradients_per_s": 1.0 / dt, "ms_per_gradient": 1e3 * dt, "k_modes": nk, "params": list(deb.PARAM9),
+            "method": "frozen-mesh central differences, 12 shifted cosmologies replayed in one launch; n_s, A_s closed form",
+            "vjp": [float(v) for v in g.cpu().numpy()]}
+
+
+def measure_los(deb, nk=3000, nt=2500, ellmax=2500, nfft=16384):
+    """Line-of-sight projection at the CMB shape, and the contraction kernel alone."""
+    import torch
+    dev = torch.device("cuda", 0)
+    P = dict(workload_param(), Neff=3.046, Nmnu=0.0, mnu=0.0, w_DE_0=-1.0, w_DE_a=0.0)
+    p = deb.evolve_background(param=P, num_thermo=1024)
+    aexp = np.geomspace(1e-4, 1.0, nt)
+    tau = p["tau_of_a_spline"].evaluate(aexp)
+    k = torch.as_tensor(np.geomspace(1e-5, 0.5, nk), device=dev)
+    tv = tau.reshape(1, -1)
+    S = torch.exp(-0.5 * ((tv - 280.0) / 20.0) ** 2) * torch.cos(k.reshape(-1, 1) * tv) * 1e-2  # smooth synthetic sources
+    tau0 = float(p["tau0"])
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    deb.compute_theta_ell(ellmax, k, tau, S, tau0, n_fftlog=nfft)
+    torch.cuda.synchronize()
+    e0.record()
+    th, kk = deb.compute_theta_ell(ellmax, k, tau, S, tau0, n_fftlog=nfft)
+    e1.record()
+    torch.cuda.synchronize()
+    los_ms = e0.elapsed_time(e1)
+    K = ((nfft + 2 + 15) // 16) * 16
+    A = torch.randn((nk, K), dtype=torch.float64, device=dev)
+    B = torch.randn((ellmax + 1, K), dtype=torch.float64, device=dev)
+    C = torch.empty((nk, ellmax + 1), dtype=torch.float64, device=dev)
+    L = deb._lib.lib()
+    L.deb_contract(deb._lib.ptr(A), deb._lib.ptr(B), nk, ellmax + 1, K, deb._lib.ptr(C), deb._lib.stream())
+    torch.cuda.synchronize()
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        deb._lib.check(L.deb_contract(deb._lib.ptr(A), deb._lib.ptr(B), nk, ellmax + 1, K, deb._lib.ptr(C), deb._lib.stream()))
+    e1.record()
+    torch.cuda.synchronize()
+    cms = e0.elapsed_time(e1) / reps
+    flop = 2.0 * nk * (ellmax + 1) * K
+    tf = flop / (cms * 1e-3) / 1e12
+    return {"shape": {"k_modes": nk, "n_tau": nt, "ellmax": ellmax, "n_fftlog": nfft}, "theta_ell_ms": los_ms,
+            "contract": {"ms": cms, "flop": flop, "achieved_tflops": tf, "peak_tflops": FP64_PEAK_TFLOPS, "frac": tf / FP64_PEAK_TFLOPS}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=24, help="k-modes timed on the CPU oracle (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=256, help="k-modes timed on the CPU oracle (0: skip)")
+    ap.add_argument("--grad-modes", type=int, default=256, help="k-modes of the gradient measurement (0: skip)")
+    ap.add_argument("--los", type=int, default=1, help="measure the line-of-sight projection (0: skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a known-size kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -151,6 +218,11 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         out["copy_calibration_bytes"] = src.numel() * 8
+    if args.grad_modes > 0:
+        out["gradient"] = measure_gradient(deb, args.grad_modes)
+        out["config"]["gradients_per_s"] = out["gradient"]["gradients_per_s"]
+    if args.los:
+        out["line_of_sight"] = measure_los(deb)
     if args.cpu_sample > 0 and rank == 0:
         out["cpu_baseline"] = cpu_baseline(kmodes, args.cpu_sample)
     print(json.dumps(out))

@@ -62,8 +62,7 @@ class DebSolverCfg(ctypes.Structure):
 
 
 class DebSchedule(ctypes.Structure):
-    _fields_ = [("persistent", ctypes.c_int), ("chunk", ctypes.c_int), ("n_workgroups", ctypes.c_int),
-                ("reserved", ctypes.c_int), ("d_order", ctypes.c_void_p)]
+    _fields_ = [("d_order", ctypes.c_void_p)]
 
 
 _lib = None

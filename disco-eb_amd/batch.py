@@ -1,6 +1,6 @@
-"""Many cosmologies at once: one background launch for all of them, one persistent
-work-queue launch for every (cosmology, k) unit, P(k) for all units; the schedule autotuner;
-graph capture of the whole P(k) pipeline.
+"""Many cosmologies at once: one background launch for all of them, one launch for every
+(cosmology, k) unit in cost order, P(k) for all units; the launch-order autotuner; graph
+capture of the whole P(k) pipeline.
 
 The reference runs one cosmology per call (evolve_background -> evolve_perturbations ->
 get_power, src/discoeb/background.py:191-345, perturbations.py:1124-1226, 1446-1468) and
@@ -53,8 +53,8 @@ def evolve_many(params, kgrids, aexp_out, *, num_thermo=256, idx=4, schedule="au
     """Perturbations and P(k) of several cosmologies, each on its own k-grid, in one launch.
 
     params: list of parameter dicts; kgrids: list of 1-d arrays (one per cosmology);
-    schedule: "auto" (autotuned work queue), None (one workgroup per unit) or a schedule
-    dict (see evolve_units).  Returns (ys, pks, status): lists of per-cosmology device tensors
+    schedule: "auto" (autotuned launch order), None (index order) or a schedule dict (see
+    evolve_units).  Returns (ys, pks, status): lists of per-cosmology device tensors
     (n_k, nout, 20 or nvar) and (n_k, nout), views into single batch arrays, and the per-unit
     status.  Cosmologies whose w(a) crosses -1 do not stop the batch: their units carry
     status 4 and NaN outputs; any other failure raises."""
@@ -102,14 +102,9 @@ def _store_cache():
 
 
 def schedule_candidates(n_units):
-    """Schedules the autotuner tries: static (one workgroup per unit), and persistent queues over
-    the resident workgroups in largest-k-first order with 1, 2 or 4 units per claim, or in
-    input order with 1."""
-    cands = [None, {"chunk": 1, "n_workgroups": 0, "order": None}]
-    for chunk in (1, 2, 4):
-        if chunk == 1 or n_units >= 8 * chunk:
-            cands.append({"chunk": chunk, "n_workgroups": 0, "order": "k_desc"})
-    return cands
+    """Launch orders the autotuner tries: index order (units grouped by cosmology, k ascending)
+    and largest k first across the whole batch."""
+    return [None, {"order": "k_desc"}]
 
 
 def autotune_schedule(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, sample=1024, repeats=1, **kw):

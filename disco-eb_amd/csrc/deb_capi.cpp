@@ -24,8 +24,8 @@ extern "C" int deb_launch_tau_out(const double *, int, int, const double *, int,
 extern "C" int deb_launch_power_units(const double *, int, const int *, const double *, int, int, const double *, int, int, double *,
                                       hipStream_t);
 extern "C" int deb_launch_perturb(const double *, int, const int *, const double *, int, const double *, int, const double *,
-                                  const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, int *, const int *, int,
-                                  int, const double *, long, const int *, hipStream_t);
+                                  const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, const int *,
+                                  const double *, long, const int *, hipStream_t);
 extern "C" long deb_perturb_lds_bytes(int, int, int, int, int);
 extern "C" int deb_launch_spline_eval(const double *, int, int, int, const double *, int, double *, hipStream_t);
 extern "C" int deb_launch_power(const double *, int, const double *, long, int, int, double, double, double, double *, hipStream_t);
@@ -62,7 +62,7 @@ const char *deb_version(void) { return "discoeb_amd 0.1 (gfx950)"; }
 const char *deb_last_error(void) { return g_err.c_str(); }
 long deb_tables_size(int num_thermo) { return deb_table_doubles(num_thermo); }
 long deb_table_offset(int slot, int num_thermo) { return deb_array_offset(slot, num_thermo); }
-long deb_evolve_workspace_doubles(void) { return 64; }  // momentum bins (32) + work-queue counter
+long deb_evolve_workspace_doubles(void) { return 32; }  // momentum bins: 16 nodes, 16 weights
 
 int deb_nvar(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
   return 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + nqmax * (lmaxnu + 1) + 2;
@@ -98,8 +98,7 @@ static int evolve_common(const double *d_tables, int num_thermo, const int *d_un
   if (!(cfg->factormin > 0) || !(cfg->factormax >= 1)) return fail(-1, "factormin must be > 0 and factormax >= 1");
   if (nout < 1) return fail(-1, "at least one output time is required");
   if (n_units < 0) return fail(-1, "n_units must be >= 0");
-  if (sched && sched->persistent && sched->chunk < 1) return fail(-1, "schedule chunk must be >= 1");
-  if (sched && sched->persistent && sched->n_workgroups < 0) return fail(-1, "schedule n_workgroups must be >= 0");
+
   if (n_units == 0) return 0;
   if (!d_tables || !d_unit_cosmo || !d_unit_k || !d_tau_out || !d_y_out || !d_status || !d_n_accept || !d_n_reject || !d_work)
     return fail(-1, "NULL device pointer");
@@ -107,18 +106,9 @@ static int evolve_common(const double *d_tables, int num_thermo, const int *d_un
   if (lds > 160 * 1024) return fail(-1, "state too large for the 160 KiB LDS of one CU (lower the hierarchy truncations)");
   rc = deb_launch_nu_bins(prob->nqmax, d_work, (hipStream_t)stream);
   if (rc) return hip_rc(rc, "deb_evolve (momentum bins)");
-  int *queue = nullptr;
-  const int *order = nullptr;
-  int chunk = 1, nwg = 0;
-  if (sched && sched->persistent) {
-    queue = (int *)(d_work + 32);
-    order = sched->d_order;
-    chunk = sched->chunk;
-    nwg = sched->n_workgroups;
-  }
+  const int *order = sched ? sched->d_order : nullptr;
   rc = deb_launch_perturb(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, d_work, prob, cfg, d_step_log, d_y_out,
-                          d_status, d_n_accept, d_n_reject, queue, order, chunk, nwg, d_replay, replay_stride, d_replay_len,
-                          (hipStream_t)stream);
+                          d_status, d_n_accept, d_n_reject, order, d_replay, replay_stride, d_replay_len, (hipStream_t)stream);
   if (rc == -2) return fail(-1, "state too large for LDS");
   return hip_rc(rc, "deb_evolve");
 }

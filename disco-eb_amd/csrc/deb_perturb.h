@@ -14,12 +14,11 @@ typedef struct DebSolverCfg {
   int max_steps;
 } DebSolverCfg;
 
-/* How units are spread over workgroups (one wavefront each).  persistent = 0: one workgroup
- * per unit.  persistent = 1: n_workgroups resident workgroups (0: as many as fit on the
- * device) claim `chunk` units at a time from a device counter, in the order d_order
- * (device array of n_units unit indices, NULL: 0..n_units-1). */
+/* Launch order of the units: one workgroup (one wavefront) per unit, workgroup g solving unit
+ * d_order[g] (device array of n_units distinct unit indices; NULL: 0..n_units-1).  Workgroups
+ * start in index order as compute slots free up, so listing the most expensive units first
+ * (largest k) shortens the tail of a launch with more units than resident slots. */
 typedef struct DebSchedule {
-  int persistent, chunk, n_workgroups, reserved;
   const int *d_order;
 } DebSchedule;
 #ifdef __cplusplus

@@ -1024,20 +1024,18 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   }
 }
 
-// Work distribution: without a queue, workgroup g solves unit g (the grid covers n_units).
-// With a queue (a device counter set to 0 before the launch), each workgroup repeatedly claims
-// the next `chunk` positions of the unit order with one atomic add until the order is
-// exhausted; order[] (optional) lists units most expensive first so the tail stays short.
-template <int NQ, bool QUEUE>
+// Work distribution: workgroup g solves unit order[g] (g when order is NULL); the grid covers
+// n_units.  The dispatcher starts workgroups in index order as slots free up, so an order that
+// lists the most expensive units first keeps the tail short.
+template <int NQ>
 __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
                                                        const double *__restrict__ unit_k, int n_units, const double *__restrict__ tau_out,
                                                        int nout, const double *__restrict__ nu_qw, DebProblem prob, DebSolverCfg cfg,
                                                        long lds_bytes, double *__restrict__ step_log, double *__restrict__ y_out,
                                                        int *__restrict__ status, int *__restrict__ n_acc, int *__restrict__ n_rej,
-                                                       int *__restrict__ queue, const int *__restrict__ order, int chunk,
-                                                       const double *__restrict__ replay, long replay_stride,
-                                                       const int *__restrict__ replay_len) {
-  if (!QUEUE && (int)blockIdx.x >= n_units) return;
+                                                       const int *__restrict__ order, const double *__restrict__ replay,
+                                                       long replay_stride, const int *__restrict__ replay_len) {
+  if ((int)blockIdx.x >= n_units) return;
   const int lane = threadIdx.x;
   PLay L;
   L.lmaxg = prob.lmaxg; L.lmaxgp = prob.lmaxgp; L.lmaxr = prob.lmaxr; L.lmaxnu = prob.lmaxnu;
@@ -1075,28 +1073,11 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
 
   setup_maps(L, W, lane);
 
-  if constexpr (!QUEUE) {
-    solve_unit<NQ>((int)blockIdx.x, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, y_out, status, n_acc, n_rej,
-                   replay, replay_stride, replay_len, L, W, lane);
-    return;
-  }
-  for (;;) {
-    int first = 0;
-    if (lane == 0) first = atomicAdd(queue, chunk);
-    first = __shfl(first, 0);
-    if (first >= n_units) break;  // every workgroup ends here once the counter passes n_units
-    const int last = min(first + chunk, n_units);
-    for (int pos = first; pos < last; ++pos) {
-      const int unit = order ? order[pos] : pos;
-      solve_unit<NQ>(unit, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, y_out, status, n_acc, n_rej, replay,
-                     replay_stride, replay_len, L, W, lane);
-    }
-  }
+  const int unit = order ? order[blockIdx.x] : (int)blockIdx.x;
+  solve_unit<NQ>(unit, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, y_out, status, n_acc, n_rej, replay,
+                 replay_stride, replay_len, L, W, lane);
 }
 
-__global__ void queue_reset_kernel(int *queue) {
-  if (threadIdx.x == 0) *queue = 0;
-}
 
 // dy/dtau at saved states (compute_time_derivatives, perturbations.py:1339-1406): one wave per
 // k-mode, looping over the output times
@@ -1193,40 +1174,24 @@ extern "C" int deb_launch_tau_out(const double *d_tables, int num_thermo, int n_
 extern "C" int deb_launch_perturb(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
                                   const double *d_tau_out, int nout, const double *d_nu_qw, const DebProblem *prob,
                                   const DebSolverCfg *cfg, double *d_step_log, double *d_y_out, int *d_status, int *d_nacc, int *d_nrej,
-                                  int *d_queue, const int *d_order, int chunk, int n_workgroups, const double *d_replay,
-                                  long replay_stride, const int *d_replay_len, hipStream_t stream) {
+                                  const int *d_order, const double *d_replay, long replay_stride, const int *d_replay_len,
+                                  hipStream_t stream) {
   long stride = deb_table_doubles(num_thermo);
   long lds = deb_perturb_lds_bytes(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
   if (lds > 160 * 1024) return -2;
-  // the static and the queue form are separate kernels, each with one inlined copy of the unit
-  // solve (one copy per kernel keeps register allocation and code size those of a single solve)
-  const bool q = d_queue != nullptr;
-  auto kern = q ? deb::perturb_kernel<0, true> : deb::perturb_kernel<0, false>;
+  auto kern = deb::perturb_kernel<0>;
   switch (prob->nqmax) {
-    case 3: kern = q ? deb::perturb_kernel<3, true> : deb::perturb_kernel<3, false>; break;
-    case 5: kern = q ? deb::perturb_kernel<5, true> : deb::perturb_kernel<5, false>; break;
-    case 8: kern = q ? deb::perturb_kernel<8, true> : deb::perturb_kernel<8, false>; break;
+    case 3: kern = deb::perturb_kernel<3>; break;
+    case 5: kern = deb::perturb_kernel<5>; break;
+    case 8: kern = deb::perturb_kernel<8>; break;
     default: break;
   }
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return (int)e;
   }
-  int grid = n_units;
-  if (d_queue) {
-    if (n_workgroups <= 0) {  // one resident wave per slot the register and LDS budgets allow
-      int dev = 0, ncu = 0, per_cu = 0;
-      hipError_t e = hipGetDevice(&dev);
-      if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-      if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kern, deb::WAVE, (size_t)lds);
-      if (e != hipSuccess) return (int)e;
-      n_workgroups = ncu * (per_cu > 0 ? per_cu : 1);
-    }
-    grid = n_workgroups < n_units ? n_workgroups : n_units;
-    hipLaunchKernelGGL(deb::queue_reset_kernel, dim3(1), dim3(1), 0, stream, d_queue);
-  }
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units,
-                     d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej, d_queue, d_order,
-                     chunk > 0 ? chunk : 1, d_replay, replay_stride, d_replay_len);
+  hipLaunchKernelGGL(kern, dim3(n_units), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units,
+                     d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej, d_order, d_replay,
+                     replay_stride, d_replay_len);
   return (int)hipGetLastError();
 }

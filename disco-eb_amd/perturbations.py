@@ -67,9 +67,8 @@ def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, l
 
     tables: (n_cosmo, size) device tensor from background_tables; unit_cosmo (int32) and unit_k
     (float64) device tensors of length n_units; tau_out (n_cosmo, nout) device tensor.
-    schedule: None (one workgroup per unit) or a dict with keys chunk (units claimed per
-    queue access, default 1), n_workgroups (persistent workgroups, 0 = as many as fit) and
-    order (None, "k_desc" = largest k first, or an int32 device tensor of unit indices).
+    schedule: None (units launched in index order, one workgroup each) or a dict with key
+    order: None, "k_desc" (largest k first) or an int32 device tensor permuting the units.
     replay: None, or (mesh (n_units, L) float64, mesh_len (n_units,) int32) device tensors: each
     unit then takes exactly its accepted steps, ending at t0 + s (t1 - t0) for the listed s
     (the frozen-mesh map, see deb_evolve_replay).
@@ -133,10 +132,9 @@ def _schedule(schedule, uk):
             srt = torch.sort(order.long()).values.cpu()
             if not torch.equal(srt, torch.arange(uk.numel())):
                 raise ValueError("order must be a permutation of 0..n_units-1")
-    chunk, nwg = int(schedule.get("chunk", 1)), int(schedule.get("n_workgroups", 0))
-    if chunk < 1 or nwg < 0:
-        raise ValueError("schedule chunk must be >= 1 and n_workgroups >= 0")
-    return _lib.DebSchedule(1, chunk, nwg, 0, order.data_ptr() if order is not None else None), order
+    if order is None:
+        return None, None
+    return _lib.DebSchedule(order.data_ptr()), order
 
 
 def _raise_on_status(status, allow_phantom=False):

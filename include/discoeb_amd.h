@@ -47,8 +47,7 @@ enum {
 int deb_spline_eval(const double *d_tables, int num_thermo, int cosmo, int spline_id, int kind, const double *d_x, int nx,
                     double *d_out, void *stream);
 
-/* Doubles of device workspace deb_evolve needs (neutrino momentum bins and the work-queue
- * counter). */
+/* Doubles of device workspace deb_evolve needs (neutrino momentum bins). */
 long deb_evolve_workspace_doubles(void);
 
 /* Adaptive Rodas5 evolution of n_units independent (cosmology, k) modes.
@@ -61,9 +60,9 @@ long deb_evolve_workspace_doubles(void);
  * d_work: deb_evolve_workspace_doubles() doubles.
  * d_step_log: NULL, or n_units x max_steps x 3 doubles receiving (t_prev, t_next, error norm)
  * of every attempted step (the step mesh, for replay checks and frozen-mesh derivatives).
- * sched: NULL (one workgroup per unit) or a DebSchedule (deb_perturb.h): persistent
- * workgroups fed by a device work queue, optionally in a caller-given cost order.  Results
- * do not depend on the schedule.
+ * sched: NULL (units in index order) or a DebSchedule (deb_perturb.h) giving the launch
+ * order of the units, one workgroup each (most expensive first shortens the tail).  Results
+ * do not depend on the order.
  * Replaces evolve_perturbations (src/discoeb/perturbations.py:1124-1226) with
  * evolve_one_mode (:882-949) per unit. */
 int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,

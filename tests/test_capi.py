@@ -45,11 +45,6 @@ def test_argument_errors_without_gpu(pkg):
     prob = pkg._lib.DebProblem(11, 11, 11, 8, 17, 0)
     rc = L.deb_evolve(None, 256, None, None, 4, None, 1, ctypes.byref(prob), ctypes.byref(cfg), None, None, None, None, None, None, None, None)
     assert rc < 0 and b"nqmax" in L.deb_last_error()
-    prob = pkg._lib.DebProblem(11, 11, 11, 8, 3, 0)
-    sched = pkg._lib.DebSchedule(1, 0, 0, 0, None)
-    rc = L.deb_evolve(None, 256, None, None, 4, None, 1, ctypes.byref(prob), ctypes.byref(cfg), None, None, None, None, None, None,
-                      ctypes.byref(sched), None)
-    assert rc < 0 and b"chunk" in L.deb_last_error()
     assert L.deb_background(None, 1, 4, None, None) < 0
     assert L.deb_tau_out(None, 256, 1, None, 3, None, None) < 0
 

@@ -48,7 +48,7 @@ def _same(a, b):
 
 
 def test_schedules_give_identical_results(deb):
-    print("case: schedules static / queue chunk 1,2,3 / orders", flush=True)
+    print("case: launch orders index / largest k first / random permutation", flush=True)
     params = [lcdm(), nu_w0wa()]
     kgrids = [np.geomspace(1e-4, 0.5, 5), np.geomspace(3e-4, 2.0, 7)]
     tables = deb.background_tables(params, 256)
@@ -57,8 +57,7 @@ def test_schedules_give_identical_results(deb):
     ref = deb.evolve_units(tables, 256, uc, uk, tau, **LMAX, **SOLVER)
     assert int(ref[1].abs().max()) == 0
     perm = torch.randperm(uk.numel(), generator=torch.Generator().manual_seed(5)).to(torch.int32).cuda()
-    for sch in [{"chunk": 1, "n_workgroups": 3, "order": "k_desc"}, {"chunk": 2, "n_workgroups": 0, "order": None},
-                {"chunk": 3, "n_workgroups": 5, "order": perm}, {"chunk": 64, "n_workgroups": 1, "order": None}]:
+    for sch in [{"order": "k_desc"}, {"order": None}, {"order": perm}]:
         got = deb.evolve_units(tables, 256, uc, uk, tau, **LMAX, **SOLVER, schedule=sch)
         for a, b in zip(got, ref):
             assert _same(a, b), sch
@@ -94,7 +93,7 @@ def test_graph_replays_match_eager(deb):
     print("case: PkGraph capture + 3 replays with parameter changes", flush=True)
     params = [lcdm(), nu_w0wa()]
     kgrids = [np.geomspace(1e-4, 0.5, 4), np.geomspace(1e-4, 1.0, 5)]
-    g = deb.PkGraph(params, kgrids, AOUT, **LMAX, **SOLVER, schedule={"chunk": 1, "n_workgroups": 0, "order": "k_desc"})
+    g = deb.PkGraph(params, kgrids, AOUT, **LMAX, **SOLVER, schedule={"order": "k_desc"})
     variants = [params, [dict(params[0], H0=68.5, A_s=2.3e-9), dict(params[1], mnu=0.08, n_s=0.95)], params]
     for it, ps in enumerate(variants):
         g.set_params(ps)
@@ -114,10 +113,18 @@ def test_graph_replays_match_eager(deb):
     assert not torch.equal(pk0, pk1)
 
 
+@pytest.mark.skip(reason="the persistent work-queue kernel is removed: its 8-momentum-bin instantiation took a GPU memory "
+                         "aperture violation (256 units) and gave wrong results on replay, with no cause found in the source; "
+                         "units are launched one workgroup each in cost order instead")
 def test_queue_matches_static_for_eight_momentum_bins(deb):
-    """The largest dense block (41 unknowns, 8 momentum bins, lmax_nu 17) through the static and
-    the work-queue kernels, k up to 5/Mpc (radiation-streaming and fluid approximations active)."""
-    print("case: nq=8 lmax_nu=17 static vs queue, 6 k 1e-5..5", flush=True)
+    """Work-queue kernel versus one workgroup per unit, 8 momentum bins (kept as a record)."""
+
+
+def test_cost_order_matches_index_order_for_eight_momentum_bins(deb):
+    """The largest dense block (41 unknowns, 8 momentum bins, lmax_nu 17), k up to 5/Mpc
+    (radiation-streaming and fluid approximations active), launched in index order and largest-k
+    first: bitwise the same results."""
+    print("case: nq=8 lmax_nu=17 index order vs largest k first, 6 k 1e-5..5", flush=True)
     p = deb.evolve_background(param=dict(nu_w0wa()), num_thermo=256)
     tau = p["tau_of_a_spline"].evaluate(np.asarray([1 / 3, 0.5, 1.0])).reshape(1, -1)
     kt = torch.as_tensor(deb.k_grid({}, 1e-5, 5.0, 6, "log"), device="cuda")
@@ -125,6 +132,6 @@ def test_queue_matches_static_for_eight_momentum_bins(deb):
     kw = dict(lmaxg=12, lmaxgp=12, lmaxr=12, lmaxnu=17, nqmax=8, rtol=1e-5, atol=1e-10)
     ref = deb.evolve_units(p["_deb_tables"], 256, uc, kt, tau, **kw)
     assert int(ref[1].abs().max()) == 0
-    got = deb.evolve_units(p["_deb_tables"], 256, uc, kt, tau, **kw, schedule={"chunk": 1, "n_workgroups": 2, "order": "k_desc"})
+    got = deb.evolve_units(p["_deb_tables"], 256, uc, kt, tau, **kw, schedule={"order": "k_desc"})
     for a, b in zip(got, ref):
         assert _same(a, b)
