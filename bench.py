@@ -59,6 +59,17 @@ def work_per_step(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax):
     return 2 * m ** 3 + 16 * m ** 2 + 12 * m * (m + nch) + 260 * n, n, m
 
 
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE passes, scaled by a device copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        entry = json.load(f).get(key)
+    return None if entry is None else float(entry["bytes_per_launch"])
+
+
 def cpu_baseline(kmodes, sample):
     sys.path.insert(0, ROOT)
     from oracle import oracle as O
@@ -132,7 +143,8 @@ def measure_los(deb, nk=3000, nt=2500, ellmax=2500, nfft=16384):
     flop = 2.0 * nk * (ellmax + 1) * K
     tf = flop / (cms * 1e-3) / 1e12
     return {"shape": {"k_modes": nk, "n_tau": nt, "ellmax": ellmax, "n_fftlog": nfft}, "theta_ell_ms": los_ms,
-            "contract": {"ms": cms, "flop": flop, "achieved_tflops": tf, "peak_tflops": FP64_PEAK_TFLOPS, "frac": tf / FP64_PEAK_TFLOPS}}
+            "contract": {"ms": cms, "flop": flop, "achieved_tflops": tf, "peak_tflops": FP64_PEAK_TFLOPS, "frac": tf / FP64_PEAK_TFLOPS,
+                         "traffic": measured_traffic(f"contract_kernel {nk} x {ellmax + 1} x {K}")}}
 
 
 def main():
@@ -207,7 +219,8 @@ def main():
                    "k_modes": NK, "nvar": n, "dense_block": m, "num_thermo": 256, "solver": "Rodas5, per-mode adaptive",
                    "all_modes_ok": ok, "gradients_per_s": None},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic("perturb_kernel<8> 512 k-modes nq 8 lmax_nu 17 rtol 1e-5"),
                      "kernel": "perturb_kernel", "kernel_ms": kmean, "attempted_steps_per_launch": float(np.mean(steps_attempted)),
                      "flop_per_step": W},
         "cpu_baseline": None,
