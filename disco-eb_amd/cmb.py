@@ -13,7 +13,7 @@ import torch
 
 from . import _lib
 from .background import evolve_background
-from .perturbations import _geomspace, compute_time_derivatives, evolve_perturbations
+from .perturbations import _geomspace, compute_time_derivatives, evolve_perturbations_batched
 
 
 def _dev(x, dev):
@@ -134,15 +134,14 @@ def cmb_aexp_out(n_early=32, n_pre_recomb=64, n_recomb=128, n_post=96):
 def compute_Cell_spectrum_from_cosmo_params(param_dict, ellmax=2500, nmodes=128, kmin=1e-4, kmax=1.0, n_k_dense=8192, n_fftlog=16384,
                                             lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3):
     """TT C_l from cosmological parameters (cmb.py:930-1090): background (num_thermo 1024),
-    perturbations on the camb k-sampling at rtol = atol = 1e-4 with full states, time
-    derivatives, source function, line of sight, C_l.  Returns (Cell, param).  The
-    perturbations are integrated per mode with adaptive steps (the reference batches 16 modes
-    on one shared step)."""
+    perturbations on the camb k-sampling at rtol = atol = 1e-4 with full states, integrated in
+    batches of 16 modes on one shared step sequence as the reference does (nmodes a multiple of
+    16), time derivatives, source function, line of sight, C_l.  Returns (Cell, param)."""
     param = evolve_background(param=dict(param_dict), thermo_module="RECFAST", num_thermo=1024)
     aexp_out = cmb_aexp_out()
     lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
-    yout, kmodes, param = evolve_perturbations(param=param, aexp_out=aexp_out, kmin=kmin, kmax=kmax, num_k=nmodes, rtol=1e-4, atol=1e-4,
-                                               return_full=True, k_sampling_method="camb", **lm)
+    yout, kmodes, param = evolve_perturbations_batched(param=param, aexp_out=aexp_out, kmin=kmin, kmax=kmax, num_k=nmodes, rtol=1e-4,
+                                                       atol=1e-4, return_full=True, k_sampling_method="camb", **lm)
     tau = param["tau_out"]
     yprime = compute_time_derivatives(yout, tau, kmodes, param, lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax)
     src = compute_source_function_from_states(yout, yprime, kmodes, aexp_out, tau, param, **lm)

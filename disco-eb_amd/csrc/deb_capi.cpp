@@ -23,6 +23,9 @@ extern "C" int deb_launch_dell(const double *, int, double, double, double *, do
 extern "C" int deb_launch_tau_out(const double *, int, int, const double *, int, double *, hipStream_t);
 extern "C" int deb_launch_power_units(const double *, int, const int *, const double *, int, int, const double *, int, int, double *,
                                       hipStream_t);
+extern "C" int deb_launch_perturb_batched(const double *, int, const int *, const double *, int, int, const double *, int, const double *,
+                                          const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, double *,
+                                          hipStream_t);
 extern "C" int deb_launch_perturb(const double *, int, const int *, const double *, int, const double *, int, const double *,
                                   const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, const int *,
                                   const double *, long, const int *, hipStream_t);
@@ -86,10 +89,10 @@ int deb_spline_eval(const double *d_tables, int num_thermo, int cosmo, int splin
   return hip_rc(deb_launch_spline_eval(t, num_thermo, spline_id, kind, d_x, nx, d_out, (hipStream_t)stream), "deb_spline_eval");
 }
 
-static int evolve_common(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
-                         const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
-                         int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const DebSchedule *sched,
-                         const double *d_replay, long replay_stride, const int *d_replay_len, void *stream) {
+// Argument checks shared by the evolution entry points (0 = fine).
+static int check_solve(const double *d_tables, const int *d_unit_cosmo, const double *d_unit_k, int n_units, const double *d_tau_out,
+                       int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status, int *d_n_accept,
+                       int *d_n_reject, double *d_work) {
   int rc = check_problem(prob);
   if (rc) return rc;
   if (!cfg) return fail(-1, "solver descriptor is NULL");
@@ -98,12 +101,21 @@ static int evolve_common(const double *d_tables, int num_thermo, const int *d_un
   if (!(cfg->factormin > 0) || !(cfg->factormax >= 1)) return fail(-1, "factormin must be > 0 and factormax >= 1");
   if (nout < 1) return fail(-1, "at least one output time is required");
   if (n_units < 0) return fail(-1, "n_units must be >= 0");
-
   if (n_units == 0) return 0;
   if (!d_tables || !d_unit_cosmo || !d_unit_k || !d_tau_out || !d_y_out || !d_status || !d_n_accept || !d_n_reject || !d_work)
     return fail(-1, "NULL device pointer");
   long lds = deb_perturb_lds_bytes(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
   if (lds > 160 * 1024) return fail(-1, "state too large for the 160 KiB LDS of one CU (lower the hierarchy truncations)");
+  return 0;
+}
+
+static int evolve_common(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
+                         const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
+                         int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const DebSchedule *sched,
+                         const double *d_replay, long replay_stride, const int *d_replay_len, void *stream) {
+  int rc = check_solve(d_tables, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, prob, cfg, d_y_out, d_status, d_n_accept, d_n_reject,
+                       d_work);
+  if (rc || n_units == 0) return rc;
   rc = deb_launch_nu_bins(prob->nqmax, d_work, (hipStream_t)stream);
   if (rc) return hip_rc(rc, "deb_evolve (momentum bins)");
   const int *order = sched ? sched->d_order : nullptr;
@@ -128,6 +140,28 @@ int deb_evolve_replay(const double *d_tables, int num_thermo, const int *d_unit_
   if (mesh_stride < 1) return fail(-1, "mesh_stride must be >= 1");
   return evolve_common(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, prob, cfg, d_y_out, d_status, d_n_accept,
                        d_n_reject, d_work, nullptr, sched, d_mesh, mesh_stride, d_mesh_len, stream);
+}
+
+long deb_evolve_batched_workspace_doubles(int n_units, const DebProblem *prob) {
+  if (!prob || n_units < 0) return -1;
+  return 32 + 2L * deb_nvar(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax) * n_units;
+}
+
+int deb_evolve_batched(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units, int batch_size,
+                       const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
+                       int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, void *stream) {
+  if (batch_size < 1) return fail(-1, "batch_size must be >= 1");
+  if (n_units % batch_size != 0) return fail(-1, "the number of units must be a multiple of batch_size");
+  int rc = check_solve(d_tables, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, prob, cfg, d_y_out, d_status, d_n_accept, d_n_reject,
+                       d_work);
+  if (rc || n_units == 0) return rc;
+  rc = deb_launch_nu_bins(prob->nqmax, d_work, (hipStream_t)stream);
+  if (rc) return hip_rc(rc, "deb_evolve_batched (momentum bins)");
+  rc = deb_launch_perturb_batched(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, batch_size, d_tau_out, nout, d_work, prob, cfg,
+                                  d_step_log, d_y_out, d_status, d_n_accept, d_n_reject, d_work + 32, (hipStream_t)stream);
+  if (rc == -2) return fail(-1, "state too large for LDS");
+  if (rc == -3) return fail(-1, "the number of units must be a multiple of batch_size");
+  return hip_rc(rc, "deb_evolve_batched");
 }
 
 int deb_tau_out(const double *d_tables, int num_thermo, int n_cosmo, const double *d_aexp_out, int nout, double *d_tau_out, void *stream) {

@@ -849,6 +849,98 @@ struct Pid {
   }
 };
 
+// One Rodas5 step attempt of the mode with wavenumber k from W.y0 at tprev over dt: factorises
+// W = 1 / (gamma dt) - J, runs the eight stages and writes the candidate y1 = u + k8 to W.y1
+// (K + 7 n holds the error estimate k8).  Returns 1 on every lane if y1 has a NaN.
+template <int NQ>
+__device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, double k, double tprev, double dt, Ws &W, int lane) {
+  const int n = L.n;
+  double *K = W.K;
+  double *u = W.u, *f = W.f, *y0 = W.y0, *dT = W.dT;
+  double W00;
+  factorise<(NQ > 0 ? 17 + 3 * NQ : 0)>(b, L, k, tprev, dt, W, lane, W00);
+  using namespace r5;
+  const double dtd[5] = {dt * d1, dt * d2, dt * d3, dt * d4, dt * d5};
+  const double cs[5] = {0.0, c2, c3, c4, c5};
+  // stages 1-5 (with the df/dtau term), 6-8 at t0 + dt
+  for (int s = 0; s < 8; ++s) {
+    double *ks = K + s * n;
+    // stage input u
+    for (int i = lane; i < n; i += WAVE) {
+      double v;
+      switch (s) {
+        case 0: v = y0[i]; break;
+        case 1: v = y0[i] + a21 * K[i]; break;
+        case 2: v = y0[i] + a31 * K[i] + a32 * K[n + i]; break;
+        case 3: v = y0[i] + a41 * K[i] + a42 * K[n + i] + a43 * K[2 * n + i]; break;
+        case 4: v = y0[i] + a51 * K[i] + a52 * K[n + i] + a53 * K[2 * n + i] + a54 * K[3 * n + i]; break;
+        case 5: v = y0[i] + a61 * K[i] + a62 * K[n + i] + a63 * K[2 * n + i] + a64 * K[3 * n + i] + a65 * K[4 * n + i]; break;
+        case 6: v = u[i] + K[5 * n + i]; break;
+        default: v = u[i] + K[6 * n + i]; break;
+      }
+      u[i] = v;
+    }
+    __syncthreads();
+    double ts = (s < 5) ? tprev + cs[s] * dt : tprev + dt;
+    rhs_eval(b, L, k, ts, u, f, W, lane);
+    __syncthreads();
+    for (int i = lane; i < n; i += WAVE) {
+      double r = f[i];
+      switch (s) {
+        case 0: r = r + dtd[0] * dT[i]; break;
+        case 1: r = r + dtd[1] * dT[i] + (C21 / dt) * K[i]; break;
+        case 2: r = r + dtd[2] * dT[i] + ((C31 / dt) * K[i] + (C32 / dt) * K[n + i]); break;
+        case 3: r = r + dtd[3] * dT[i] + ((C41 / dt) * K[i] + (C42 / dt) * K[n + i] + (C43 / dt) * K[2 * n + i]); break;
+        case 4:
+          r = r + dtd[4] * dT[i] + ((C51 / dt) * K[i] + (C52 / dt) * K[n + i] + (C53 / dt) * K[2 * n + i] + (C54 / dt) * K[3 * n + i]);
+          break;
+        case 5:
+          r = r + ((C61 / dt) * K[i] + (C62 / dt) * K[n + i] + (C63 / dt) * K[2 * n + i] + (C64 / dt) * K[3 * n + i] +
+                   (C65 / dt) * K[4 * n + i]);
+          break;
+        case 6:
+          r = r + ((C71 / dt) * K[i] + (C72 / dt) * K[n + i] + (C73 / dt) * K[2 * n + i] + (C74 / dt) * K[3 * n + i] +
+                   (C75 / dt) * K[4 * n + i] + (C76 / dt) * K[5 * n + i]);
+          break;
+        default:
+          r = r + ((C81 / dt) * K[i] + (C82 / dt) * K[n + i] + (C83 / dt) * K[2 * n + i] + (C84 / dt) * K[3 * n + i] +
+                   (C85 / dt) * K[4 * n + i] + (C86 / dt) * K[5 * n + i] + (C87 / dt) * K[6 * n + i]);
+          break;
+      }
+      ks[i] = r;
+    }
+    __syncthreads();
+    wsolve(L, W, W00, ks, lane);
+  }
+  // candidate y1 = u + k8 (perturbations.py:855-859)
+  int nanflag = 0;
+  for (int i = lane; i < n; i += WAVE) {
+    double v = u[i] + K[7 * n + i];
+    W.y1[i] = v;
+    nanflag |= (v != v);
+  }
+  nanflag = wave_or(nanflag);
+  __syncthreads();
+  return nanflag;
+}
+
+// Sum of squares of the scaled error over the six filtered entries (0, 2, 3, 5, 6, 7) with
+// weights (1, k^2, 1, 1, 1 / k^2, 1) (rms_norm_filtered, perturbations.py:855-859); the scale
+// uses |y0| alone when the candidate has a NaN.
+__device__ __forceinline__ double filtered_sq(double k, const double *y0, const double *y1, const double *err, int nanflag,
+                                              const DebSolverCfg &cfg) {
+  const int idx[6] = {0, 2, 3, 5, 6, 7};
+  const double wts[6] = {1.0, k * k, 1.0, 1.0, 1.0 / (k * k), 1.0};
+  double acc = 0.0;
+  for (int j = 0; j < 6; ++j) {
+    int i = idx[j];
+    double yy = fmax(fabs(y0[i]), fabs(nanflag ? y0[i] : y1[i]));
+    double e = err[i] / (cfg.atol + yy * cfg.rtol) * wts[j];
+    acc += e * e;
+  }
+  return acc;
+}
+
 // One (cosmology, k) unit from its initial conditions to the last output time.  The LDS
 // workspace W and the layout L (with its momentum-bin tables) are set up by the caller.
 template <int NQ>
@@ -899,86 +991,12 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   pid.prev_prev_inv = 1.0;
   int isave = 0, steps = 0, na = 0, nr = 0, st = 0;
   double *K = W.K;
-  double *u = W.u, *f = W.f, *y0 = W.y0, *dT = W.dT;
+  double *u = W.u, *y0 = W.y0;
   while (tprev < t1) {
     if (steps >= cfg.max_steps || (rs != nullptr && ridx >= rlen)) { st = 1; break; }
     const double dt = tnext - tprev;
-    double W00;
-    factorise<(NQ > 0 ? 17 + 3 * NQ : 0)>(b, L, k, tprev, dt, W, lane, W00);
-    using namespace r5;
-    const double dtd[5] = {dt * d1, dt * d2, dt * d3, dt * d4, dt * d5};
-    const double cs[5] = {0.0, c2, c3, c4, c5};
-    // stages 1-5 (with the df/dtau term), 6-8 at t0 + dt
-    for (int s = 0; s < 8; ++s) {
-      double *ks = K + s * n;
-      // stage input u
-      for (int i = lane; i < n; i += WAVE) {
-        double v;
-        switch (s) {
-          case 0: v = y0[i]; break;
-          case 1: v = y0[i] + a21 * K[i]; break;
-          case 2: v = y0[i] + a31 * K[i] + a32 * K[n + i]; break;
-          case 3: v = y0[i] + a41 * K[i] + a42 * K[n + i] + a43 * K[2 * n + i]; break;
-          case 4: v = y0[i] + a51 * K[i] + a52 * K[n + i] + a53 * K[2 * n + i] + a54 * K[3 * n + i]; break;
-          case 5: v = y0[i] + a61 * K[i] + a62 * K[n + i] + a63 * K[2 * n + i] + a64 * K[3 * n + i] + a65 * K[4 * n + i]; break;
-          case 6: v = u[i] + K[5 * n + i]; break;
-          default: v = u[i] + K[6 * n + i]; break;
-        }
-        u[i] = v;
-      }
-      __syncthreads();
-      double ts = (s < 5) ? tprev + cs[s] * dt : tprev + dt;
-      rhs_eval(b, L, k, ts, u, f, W, lane);
-      __syncthreads();
-      for (int i = lane; i < n; i += WAVE) {
-        double r = f[i];
-        switch (s) {
-          case 0: r = r + dtd[0] * dT[i]; break;
-          case 1: r = r + dtd[1] * dT[i] + (C21 / dt) * K[i]; break;
-          case 2: r = r + dtd[2] * dT[i] + ((C31 / dt) * K[i] + (C32 / dt) * K[n + i]); break;
-          case 3: r = r + dtd[3] * dT[i] + ((C41 / dt) * K[i] + (C42 / dt) * K[n + i] + (C43 / dt) * K[2 * n + i]); break;
-          case 4:
-            r = r + dtd[4] * dT[i] + ((C51 / dt) * K[i] + (C52 / dt) * K[n + i] + (C53 / dt) * K[2 * n + i] + (C54 / dt) * K[3 * n + i]);
-            break;
-          case 5:
-            r = r + ((C61 / dt) * K[i] + (C62 / dt) * K[n + i] + (C63 / dt) * K[2 * n + i] + (C64 / dt) * K[3 * n + i] +
-                     (C65 / dt) * K[4 * n + i]);
-            break;
-          case 6:
-            r = r + ((C71 / dt) * K[i] + (C72 / dt) * K[n + i] + (C73 / dt) * K[2 * n + i] + (C74 / dt) * K[3 * n + i] +
-                     (C75 / dt) * K[4 * n + i] + (C76 / dt) * K[5 * n + i]);
-            break;
-          default:
-            r = r + ((C81 / dt) * K[i] + (C82 / dt) * K[n + i] + (C83 / dt) * K[2 * n + i] + (C84 / dt) * K[3 * n + i] +
-                     (C85 / dt) * K[4 * n + i] + (C86 / dt) * K[5 * n + i] + (C87 / dt) * K[6 * n + i]);
-            break;
-        }
-        ks[i] = r;
-      }
-      __syncthreads();
-      wsolve(L, W, W00, ks, lane);
-    }
-    // candidate y1 = u + k8 and the filtered error norm (perturbations.py:855-859)
-    int nanflag = 0;
-    for (int i = lane; i < n; i += WAVE) {
-      double v = u[i] + K[7 * n + i];
-      W.y1[i] = v;
-      nanflag |= (v != v);
-    }
-    nanflag = wave_or(nanflag);
-    __syncthreads();
-    double acc = 0.0;
-    {
-      const int idx[6] = {0, 2, 3, 5, 6, 7};
-      const double wts[6] = {1.0, k * k, 1.0, 1.0, 1.0 / (k * k), 1.0};
-      for (int j = 0; j < 6; ++j) {
-        int i = idx[j];
-        double yy = fmax(fabs(y0[i]), fabs(nanflag ? y0[i] : W.y1[i]));
-        double e = K[7 * n + i] / (cfg.atol + yy * cfg.rtol) * wts[j];
-        acc += e * e;
-      }
-    }
-    double en = sqrt(acc / 6.0);
+    const int nanflag = rodas5_attempt<NQ>(b, L, k, tprev, dt, W, lane);
+    const double en = sqrt(filtered_sq(k, W.y0, W.y1, K + 7 * n, nanflag, cfg) / 6.0);
     if (step_log != nullptr && lane == 0) {  // (t_prev, t_next, error norm) of every attempted step
       double *lg = step_log + ((long)unit * cfg.max_steps + steps) * 3;
       lg[0] = tprev;
@@ -1024,6 +1042,42 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   }
 }
 
+// Layout of the state vector for the truncations in prob (NQ > 0: compile-time bin count) and
+// the carve-up of the dynamic LDS into the solver workspace, which must match
+// deb_perturb_lds_bytes(): 19 n + m^2 + nch m + m + 3 NQMAX doubles, then m + n + 4 nch + 2 m
+// ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
+template <int NQ>
+__device__ __forceinline__ bool setup_solver(const DebProblem &prob, const double *__restrict__ nu_qw, long lds_bytes, PLay &L, Ws &W,
+                                             int lane) {
+  L.lmaxg = prob.lmaxg; L.lmaxgp = prob.lmaxgp; L.lmaxr = prob.lmaxr; L.lmaxnu = prob.lmaxnu;
+  L.nq = (NQ > 0) ? NQ : prob.nqmax;
+  L.n = 7 + (L.lmaxg + 1) + (L.lmaxgp + 1) + (L.lmaxr + 1) + L.nq * (L.lmaxnu + 1) + 2;
+  L.idxg = 7; L.idxgp = 7 + (L.lmaxg + 1); L.idxr = 9 + L.lmaxg + L.lmaxgp; L.iq0 = 10 + L.lmaxg + L.lmaxgp + L.lmaxr;
+  L.m = 17 + 3 * L.nq;
+  L.nch = 3 + L.nq;
+  const int n = L.n, m = L.m, nch = L.nch;
+  extern __shared__ double smem[];
+  double *p = smem;
+  W.y0 = p; p += n; W.y1 = p; p += n; W.u = p; p += n; W.f = p; p += n; W.J0 = p; p += n; W.dT = p; p += n;
+  W.K = p; p += 8 * n; W.tlo = p; p += n; W.tdi = p; p += n; W.tup = p; p += n; W.tmul = p; p += n; W.td = p; p += n;
+  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m;
+  double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
+  int *ip = (int *)p;
+  W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
+  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m;
+  ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;  // 8-byte alignment for the cache
+  W.sc = (SplCache *)ip; ip += sizeof(SplCache) / 4;
+  if ((long)((char *)ip - (char *)smem) > lds_bytes) return false;
+  for (int q = lane; q < L.nq; q += WAVE) {
+    sq[q] = nu_qw[q];
+    sw[q] = nu_qw[NQMAX + q];
+    sdl[q] = -nu_qw[q] / (1.0 + exp(-nu_qw[q]));
+  }
+  L.q = sq; L.w = sw; L.dlfdlq = sdl;
+  setup_maps(L, W, lane);
+  return true;
+}
+
 // Work distribution: workgroup g solves unit order[g] (g when order is NULL); the grid covers
 // n_units.  The dispatcher starts workgroups in index order as slots free up, so an order that
 // lists the most expensive units first keeps the tail short.
@@ -1038,44 +1092,151 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
   if ((int)blockIdx.x >= n_units) return;
   const int lane = threadIdx.x;
   PLay L;
-  L.lmaxg = prob.lmaxg; L.lmaxgp = prob.lmaxgp; L.lmaxr = prob.lmaxr; L.lmaxnu = prob.lmaxnu;
-  L.nq = (NQ > 0) ? NQ : prob.nqmax;  // compile-time for the common bin counts
-  L.n = 7 + (L.lmaxg + 1) + (L.lmaxgp + 1) + (L.lmaxr + 1) + L.nq * (L.lmaxnu + 1) + 2;
-  L.idxg = 7; L.idxgp = 7 + (L.lmaxg + 1); L.idxr = 9 + L.lmaxg + L.lmaxgp; L.iq0 = 10 + L.lmaxg + L.lmaxgp + L.lmaxr;
-  L.m = 17 + 3 * L.nq;
-  L.nch = 3 + L.nq;
-  const int n = L.n, m = L.m, nch = L.nch;
-
-  extern __shared__ double smem[];
   Ws W;
-  double *p = smem;
-  // carve-up must match deb_perturb_lds_bytes(): 19 n + m^2 + nch m + m + 3 NQMAX doubles, then
-  // m + n + 4 nch ints
-  W.y0 = p; p += n; W.y1 = p; p += n; W.u = p; p += n; W.f = p; p += n; W.J0 = p; p += n; W.dT = p; p += n;
-  W.K = p; p += 8 * n; W.tlo = p; p += n; W.tdi = p; p += n; W.tup = p; p += n; W.tmul = p; p += n; W.td = p; p += n;
-  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m;
-  double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
-  for (int q = lane; q < L.nq; q += WAVE) {
-    sq[q] = nu_qw[q];
-    sw[q] = nu_qw[NQMAX + q];
-    sdl[q] = -nu_qw[q] / (1.0 + exp(-nu_qw[q]));
-  }
-  L.q = sq; L.w = sw; L.dlfdlq = sdl;
-  int *ip = (int *)p;
-  W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
-  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m;
-  ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;  // 8-byte alignment for the cache
-  W.sc = (SplCache *)ip; ip += sizeof(SplCache) / 4;
-  if ((long)((char *)ip - (char *)smem) > lds_bytes) {  // launch/carve-up mismatch: refuse to run
+  if (!setup_solver<NQ>(prob, nu_qw, lds_bytes, L, W, lane)) {  // launch/carve-up mismatch: refuse to run
     for (int u = blockIdx.x * WAVE + lane; u < n_units; u += gridDim.x * WAVE) { status[u] = 3; n_acc[u] = 0; n_rej[u] = 0; }
     return;
   }
-
-  setup_maps(L, W, lane);
-
   const int unit = order ? order[blockIdx.x] : (int)blockIdx.x;
   solve_unit<NQ>(unit, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, y_out, status, n_acc, n_rej, replay,
                  replay_stride, replay_len, L, W, lane);
+}
+
+
+// Batched shared-step evolution (evolve_modes_batched, perturbations.py:1002-1120, with
+// Rodas5Batched): workgroup g integrates the `batch` consecutive units g * batch .. (g + 1) *
+// batch - 1, which must belong to one cosmology, on ONE step sequence: a common start time
+// (the smallest of the units' own), one step size, and one accept decision from the RMS of the
+// filtered errors over all batch x 6 entries.  Each attempt walks the units in turn through the
+// per-mode Rodas5 stages in LDS; their states live in `state` (2 x nvar doubles per unit, the
+// current state and the candidate, swapped on accept), which stays in L2 between visits.
+// step_log, when given, receives (t_prev, t_next, error norm) of every attempt of batch g at
+// step_log[(g * max_steps + step) * 3].  Status 5: the batch's units have different cosmologies.
+template <int NQ>
+__global__ __launch_bounds__(WAVE) void perturb_batched_kernel(const double *__restrict__ tables, long tstride,
+                                                               const int *__restrict__ unit_cosmo, const double *__restrict__ unit_k,
+                                                               int n_units, int batch, const double *__restrict__ tau_out, int nout,
+                                                               const double *__restrict__ nu_qw, DebProblem prob, DebSolverCfg cfg,
+                                                               long lds_bytes, double *__restrict__ step_log, double *__restrict__ y_out,
+                                                               int *__restrict__ status, int *__restrict__ n_acc, int *__restrict__ n_rej,
+                                                               double *__restrict__ state) {
+  const int g = blockIdx.x, lane = threadIdx.x;
+  const int u0 = g * batch;
+  if (u0 + batch > n_units) return;
+  PLay L;
+  Ws W;
+  if (!setup_solver<NQ>(prob, nu_qw, lds_bytes, L, W, lane)) {
+    for (int j = lane; j < batch; j += WAVE) { status[u0 + j] = 3; n_acc[u0 + j] = 0; n_rej[u0 + j] = 0; }
+    return;
+  }
+  const int n = L.n;
+  const int cosmo = unit_cosmo[u0];
+  BgView b;
+  b.init(tables + (long)cosmo * tstride);
+  if (lane == 0) { W.sc->i = -1; W.sc->j = -1; }
+  __syncthreads();
+  const int wout = prob.return_full ? n : 20;
+  double *yo = y_out + (long)u0 * nout * wout;  // unit j's outputs at yo + j nout wout
+  int mixed = 0;
+  for (int j = lane; j < batch; j += WAVE) mixed |= (unit_cosmo[u0 + j] != cosmo);
+  mixed = wave_or(mixed);
+  const bool phantom = !b.lambda_de && (1.0 + b.w0) * (1.0 + b.w0 + b.wa) <= 0.0;
+  if (mixed || phantom) {  // not integrated: NaN outputs (w crossing -1 as in the per-mode kernel)
+    for (long i = lane; i < (long)batch * nout * wout; i += WAVE) yo[i] = __builtin_nan("");
+    for (int j = lane; j < batch; j += WAVE) { status[u0 + j] = mixed ? 5 : 4; n_acc[u0 + j] = 0; n_rej[u0 + j] = 0; }
+    return;
+  }
+
+  const double *tout = tau_out + (long)cosmo * nout;
+  double tmin = tout[0], t1 = tout[0];
+  for (int j = 1; j < nout; ++j) { tmin = fmin(tmin, tout[j]); t1 = fmax(t1, tout[j]); }
+  // determine_start_times_per_batch (perturbations.py:954-999): the earliest start of the batch
+  double t0 = 1e300;
+  for (int j = 0; j < batch; ++j) t0 = fmin(t0, 0.99 * fmin(tmin, starting_time(b, unit_k[u0 + j])));
+  double *st0 = state + (long)u0 * 2 * n;  // unit j: st0 + 2 n j + {cur, 1 - cur} n
+  for (int j = 0; j < batch; ++j) {
+    if (lane == 0) initial_conditions(b, L, t0, unit_k[u0 + j], W.y0);
+    __syncthreads();
+    for (int i = lane; i < n; i += WAVE) st0[2L * n * j + i] = W.y0[i];
+    __syncthreads();
+  }
+
+  double tprev = t0;
+  double tnext = fmin(t0 + fmin(t0 / 4.0, 0.5 * (t1 - t0)), t1);
+  Pid pid;
+  pid.prev_inv = 1.0;
+  pid.prev_prev_inv = 1.0;
+  int isave = 0, steps = 0, na = 0, nr = 0, st = 0, cur = 0;
+  while (tprev < t1) {
+    if (steps >= cfg.max_steps) { st = 1; break; }
+    const double dt = tnext - tprev;
+    double acc = 0.0, acc_nan = 0.0;  // scaled with max(|y0|, |y1|), and with |y0| alone
+    int nanany = 0;
+    for (int j = 0; j < batch; ++j) {
+      const double k = unit_k[u0 + j];
+      const double *ys = st0 + 2L * n * j + (long)cur * n;
+      double *yd = st0 + 2L * n * j + (long)(1 - cur) * n;
+      for (int i = lane; i < n; i += WAVE) W.y0[i] = ys[i];
+      __syncthreads();
+      const int nanflag = rodas5_attempt<NQ>(b, L, k, tprev, dt, W, lane);
+      for (int i = lane; i < n; i += WAVE) yd[i] = W.y1[i];
+      acc += filtered_sq(k, W.y0, W.y1, W.K + 7 * n, 0, cfg);
+      acc_nan += filtered_sq(k, W.y0, W.y1, W.K + 7 * n, 1, cfg);
+      nanany |= nanflag;
+      __syncthreads();
+    }
+    const double en = sqrt((nanany ? acc_nan : acc) / (6.0 * batch));
+    if (step_log != nullptr && lane == 0) {
+      double *lg = step_log + ((long)g * cfg.max_steps + steps) * 3;
+      lg[0] = tprev;
+      lg[1] = tnext;
+      lg[2] = en;
+    }
+    if (!isfinite(en)) { st = 2; break; }
+    const bool keep = en < 1.0;
+    const double factor = pid.adapt(en, keep, cfg);
+    if (keep) {
+      if (isave < nout && tout[isave] <= tnext) {  // outputs in (tprev, tnext]: interpolate every unit
+        int isv = isave;
+        for (int j = 0; j < batch; ++j) {
+          const double k = unit_k[u0 + j];
+          const double *ya = st0 + 2L * n * j + (long)cur * n;
+          const double *yb = st0 + 2L * n * j + (long)(1 - cur) * n;
+          for (int i = lane; i < n; i += WAVE) { W.y0[i] = ya[i]; W.y1[i] = yb[i]; }
+          __syncthreads();
+          isv = isave;
+          while (isv < nout && tout[isv] <= tnext) {
+            double th = (tout[isv] - tprev) / (tnext - tprev);
+            for (int i = lane; i < n; i += WAVE) W.u[i] = W.y0[i] + th * (W.y1[i] - W.y0[i]);
+            __syncthreads();
+            emit_output(b, L, k, tout[isv], W.u, yo + ((long)j * nout + isv) * wout, prob.return_full, lane);
+            __syncthreads();
+            ++isv;
+          }
+        }
+        isave = isv;
+      }
+      cur = 1 - cur;
+      ++na;
+    } else {
+      ++nr;
+    }
+    double nt0 = keep ? tnext : tprev;
+    tprev = fmin(nt0, t1);
+    double tn = nt0 + dt * factor;
+    tnext = (tn > t1 - 1e-10) ? t1 : tn;
+    ++steps;
+  }
+  if (st != 0) {
+    for (int j = 0; j < batch; ++j)
+      for (int r = isave; r < nout; ++r)
+        for (int i = lane; i < wout; i += WAVE) yo[((long)j * nout + r) * wout + i] = __builtin_nan("");
+  }
+  for (int j = lane; j < batch; j += WAVE) {
+    status[u0 + j] = st;
+    n_acc[u0 + j] = na;
+    n_rej[u0 + j] = nr;
+  }
 }
 
 
@@ -1193,5 +1354,29 @@ extern "C" int deb_launch_perturb(const double *d_tables, int num_thermo, const 
   hipLaunchKernelGGL(kern, dim3(n_units), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units,
                      d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej, d_order, d_replay,
                      replay_stride, d_replay_len);
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_perturb_batched(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k,
+                                          int n_units, int batch, const double *d_tau_out, int nout, const double *d_nu_qw,
+                                          const DebProblem *prob, const DebSolverCfg *cfg, double *d_step_log, double *d_y_out,
+                                          int *d_status, int *d_nacc, int *d_nrej, double *d_state, hipStream_t stream) {
+  long stride = deb_table_doubles(num_thermo);
+  long lds = deb_perturb_lds_bytes(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
+  if (lds > 160 * 1024) return -2;
+  if (batch <= 0 || n_units % batch != 0) return -3;
+  auto kern = deb::perturb_batched_kernel<0>;
+  switch (prob->nqmax) {
+    case 3: kern = deb::perturb_batched_kernel<3>; break;
+    case 5: kern = deb::perturb_batched_kernel<5>; break;
+    case 8: kern = deb::perturb_batched_kernel<8>; break;
+    default: break;
+  }
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return (int)e;
+  }
+  hipLaunchKernelGGL(kern, dim3(n_units / batch), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units,
+                     batch, d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej, d_state);
   return (int)hipGetLastError();
 }

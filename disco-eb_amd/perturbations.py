@@ -115,6 +115,39 @@ def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, l
     return y, status, nacc, nrej
 
 
+def evolve_units_batched(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, batch_size=16, lmaxg=11, lmaxgp=11, lmaxr=11,
+                         lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4, pcoeff=0.25, icoeff=0.80, dcoeff=0.0, factormax=20.0,
+                         factormin=0.3, max_steps=2048, return_full=False, step_log=False):
+    """Low-level launch of the batched shared-step solve (deb_evolve_batched): the units
+    g * batch_size .. (g + 1) * batch_size - 1, which must share a cosmology, step together.
+    Arguments as evolve_units.  Returns (y, status, n_accept, n_reject) device tensors, plus the
+    per-batch step mesh (n_units / batch_size, max_steps, 3) when step_log is set."""
+    L = _lib.lib()
+    dev = tables.device
+    n_units = int(unit_k.numel())
+    if batch_size < 1 or n_units % batch_size != 0:
+        raise ValueError(f"num_k = {n_units} must be a multiple of batch_size = {batch_size}")
+    nout = int(tau_out.shape[-1])
+    nv = nvar(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax)
+    y = torch.empty((n_units, nout, nv if return_full else 20), dtype=torch.float64, device=dev)
+    status, nacc, nrej = (torch.empty(n_units, dtype=torch.int32, device=dev) for _ in range(3))
+    prob = _lib.DebProblem(int(lmaxg), int(lmaxgp), int(lmaxr), int(lmaxnu), int(nqmax), int(bool(return_full)))
+    cfg = _lib.DebSolverCfg(float(rtol), float(atol), float(pcoeff), float(icoeff), float(dcoeff), float(factormax),
+                            float(factormin), int(max_steps))
+    work = torch.empty(L.deb_evolve_batched_workspace_doubles(n_units, ctypes.byref(prob)), dtype=torch.float64, device=dev)
+    uc = unit_cosmo.to(device=dev, dtype=torch.int32).contiguous()
+    uk = unit_k.to(device=dev, dtype=torch.float64).contiguous()
+    to = tau_out.to(device=dev, dtype=torch.float64).contiguous()
+    log = (torch.full((n_units // batch_size, int(max_steps), 3), float("nan"), dtype=torch.float64, device=dev)
+           if step_log else None)
+    _lib.check(L.deb_evolve_batched(_lib.ptr(tables), int(num_thermo), _lib.ptr(uc), _lib.ptr(uk), n_units, int(batch_size), _lib.ptr(to),
+                                    nout, ctypes.byref(prob), ctypes.byref(cfg), _lib.ptr(y), _lib.ptr(status), _lib.ptr(nacc),
+                                    _lib.ptr(nrej), _lib.ptr(work), _lib.ptr(log) if log is not None else None, _lib.stream()))
+    if step_log:
+        return y, status, nacc, nrej, log
+    return y, status, nacc, nrej
+
+
 def _schedule(schedule, uk):
     """DebSchedule for a schedule dict (see evolve_units) and the order tensor it points to."""
     if schedule is None:
@@ -147,6 +180,8 @@ def _raise_on_status(status, allow_phantom=False):
         raise RuntimeError("Non-finite error estimate in the perturbation solve.")
     if (st == 3).any():
         raise RuntimeError("internal error: LDS carve-up exceeds the launch size")
+    if (st == 5).any():
+        raise ValueError("the units of one batch must belong to one cosmology")
     if (st == 4).any():
         raise ValueError("w(a) = w_DE_0 + w_DE_a (1 - a) crosses -1 in (0, 1]: the dark-energy fluid equations are singular "
                          "at the phantom divide (use w_DE_0 = -1, w_DE_a = 0 for a cosmological constant)")
@@ -177,6 +212,40 @@ def evolve_perturbations(*, param, aexp_out, kmin: float, kmax: float, num_k: in
                                          lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax, rtol=rtol, atol=atol, pcoeff=pcoeff,
                                          icoeff=icoeff, dcoeff=dcoeff, factormax=factormax, factormin=factormin,
                                          max_steps=max_steps, return_full=return_full)
+    _raise_on_status(status)
+    param["lmaxg"], param["lmaxgp"], param["lmaxr"], param["lmaxnu"], param["nqmax"] = lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax
+    param["nout"] = int(aexp.numel())
+    param["tau_out"] = tau_out
+    param["_deb_steps"] = (nacc, nrej)
+    return y, kt, param
+
+
+def evolve_perturbations_batched(*, param, aexp_out, kmin: float, kmax: float, num_k: int, lmaxg: int = 11, lmaxgp: int = 11,
+                                 lmaxr: int = 11, lmaxnu: int = 8, nqmax: int = 3, rtol: float = 1e-4, atol: float = 1e-4,
+                                 pcoeff: float = 0.25, icoeff: float = 0.80, dcoeff: float = 0.0, factormax: float = 20.0,
+                                 factormin: float = 0.3, max_steps: int = 2048, batch_size: int = 16, return_full: bool = False,
+                                 k_sampling_method: str = "camb"):
+    """Perturbations with batch_size consecutive k-modes on one shared step sequence
+    (perturbations.py:1228-1337); num_k must be a multiple of batch_size.  Same returns and
+    param annotations as evolve_perturbations."""
+    kmodes = k_grid(param, kmin, kmax, num_k, k_sampling_method)
+    if batch_size < 1 or num_k % batch_size != 0:
+        raise ValueError(f"num_k = {num_k} must be a multiple of batch_size = {batch_size}")
+    tables = param["_deb_tables"]
+    dev = tables.device
+    aexp = torch.as_tensor(np.atleast_1d(np.asarray(aexp_out, dtype=np.float64)) if not torch.is_tensor(aexp_out) else aexp_out,
+                           dtype=torch.float64, device=dev).reshape(-1)
+    tau_out = param["tau_of_a_spline"].evaluate(aexp)
+    cosmo = int(param.get("_deb_cosmo", 0))
+    n_cosmo = tables.shape[0] if tables.ndim == 2 else 1
+    tau_all = torch.zeros((n_cosmo, tau_out.numel()), dtype=torch.float64, device=dev)
+    tau_all[cosmo] = tau_out
+    kt = torch.as_tensor(kmodes, dtype=torch.float64, device=dev)
+    uc = torch.full((kt.numel(),), cosmo, dtype=torch.int32, device=dev)
+    y, status, nacc, nrej = evolve_units_batched(tables, param["_deb_num_thermo"], uc, kt, tau_all, batch_size=batch_size, lmaxg=lmaxg,
+                                                 lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax, rtol=rtol, atol=atol,
+                                                 pcoeff=pcoeff, icoeff=icoeff, dcoeff=dcoeff, factormax=factormax,
+                                                 factormin=factormin, max_steps=max_steps, return_full=return_full)
     _raise_on_status(status)
     param["lmaxg"], param["lmaxgp"], param["lmaxr"], param["lmaxnu"], param["nqmax"] = lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax
     param["nout"] = int(aexp.numel())

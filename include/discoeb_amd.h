@@ -69,6 +69,26 @@ int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, 
                const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
                int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const DebSchedule *sched, void *stream);
 
+/* Doubles of device workspace deb_evolve_batched needs for n_units units: the momentum bins
+ * plus two state vectors per unit. */
+long deb_evolve_batched_workspace_doubles(int n_units, const DebProblem *prob);
+
+/* Batched shared-step Rodas5 evolution: the units g * batch_size .. (g + 1) * batch_size - 1
+ * (n_units a multiple of batch_size, each batch within one cosmology) are integrated together
+ * from the earliest of their start times with one step size and one accept decision, taken on
+ * the RMS of the filtered error over all batch_size x 6 entries.  Outputs, status and step
+ * counts as deb_evolve (every unit of a batch has the batch's counts); status 5 marks a batch
+ * whose units have different cosmologies (not integrated, NaN outputs).
+ * d_work: deb_evolve_batched_workspace_doubles(n_units, prob) doubles.
+ * d_step_log: NULL, or (n_units / batch_size) x max_steps x 3 doubles receiving (t_prev,
+ * t_next, error norm) of every attempted step of each batch.
+ * Replaces evolve_perturbations_batched (src/discoeb/perturbations.py:1228-1337) with
+ * evolve_modes_batched (:1001-1120), determine_start_times_per_batch (:954-999) and
+ * Rodas5Batched (src/discoeb/ode_integrators_stiff.py:904-1059). */
+int deb_evolve_batched(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units, int batch_size,
+                       const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
+                       int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, void *stream);
+
 /* deb_evolve on a frozen step mesh: unit u takes exactly d_mesh_len[u] steps whose ends are
  * t0 + s (t1 - t0) for the increasing s = d_mesh[u * mesh_stride + i] (the last one 1), with
  * no error control or rejection; t0 and t1 are the unit's own start and final output times.

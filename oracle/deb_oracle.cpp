@@ -1374,15 +1374,17 @@ int evolve_one(const Bg &bg, const Layout &L, double kmode, const double *tau_ou
   return status;
 }
 
-// Replay of a prescribed step mesh (t_prev, t_next, accepted) from the same start: every step is
-// taken with dt = t_next - t_prev; outputs as in evolve_one.  en_out receives this solver's own
+// Replay of a prescribed step mesh (t_prev, t_next, accepted), starting from the initial
+// conditions at the first step's t_prev: every step is taken with dt = t_next - t_prev; outputs
+// as in evolve_one.  en_out receives this solver's own
 // error norm of each step, so the caller can compare the two solvers step by step.
 int evolve_replay(const Bg &bg, const Layout &L, double kmode, const double *tau_out, int nout, const SolverCfg &s, bool full,
                   const double *mesh, const int *keep, int nsteps, double *yout, double *en_out) {
   const int n = L.n;
   double tmin = tau_out[0];
   for (int i = 1; i < nout; ++i) tmin = std::min(tmin, tau_out[i]);
-  double t0 = 0.99 * std::min(tmin, starting_time(bg, kmode));
+  // the mesh's first step starts at the initial time: the mode's own start, or a batch's
+  double t0 = nsteps > 0 ? mesh[0] : 0.99 * std::min(tmin, starting_time(bg, kmode));
   Workspace ws;
   ws.init(n);
   std::vector<double> y(n), y1(n), err(n), tmp(n);

@@ -232,7 +232,8 @@ class Background:
 def _replay(self, k, aexp_out, mesh, keep, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4,
             pcoeff=0.25, icoeff=0.80, dcoeff=0.0, factormax=20.0, factormin=0.3, return_full=False):
     """Take exactly the steps of `mesh` (nsteps x 2: t_prev, t_next; `keep` accepted flags) for one
-    mode; returns (y[nout, 20 or nvar], own error norm per step)."""
+    mode, from its initial conditions at mesh[0, 0]; returns (y[nout, 20 or nvar], own error norm
+    per step)."""
     lib = _load()
     aexp_out = np.atleast_1d(np.asarray(aexp_out, dtype=np.float64))
     tau_out, tp = _d(self.spline("tau_of_a", aexp_out))

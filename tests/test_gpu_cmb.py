@@ -135,20 +135,12 @@ def test_theta_ell_cell_dell_match_oracle(deb, golden, small_cmb_run, n_k_dense)
     np.testing.assert_allclose(_np(dl), g["Dell"], rtol=1e-14)
 
 
-def test_chain_against_reference_fixtures(deb, golden):
-    """The reference's CMB test configuration (tests/test_cmb.py:93-240): DISCO-Notebook,
-    num_thermo 1024, 128 k in 1e-4..1, 320 output times, rtol = atol = 1e-4, full states;
-    Theta_l and C_l for l <= 100 (n_fftlog 16384, no densification).  The fixtures were made with
-    log-spaced k (their k63 reproduces only with that sampling) by the reference's batched
-    shared-step solver, so this is a regression check at the reference's own tolerance (rtol
-    0.2), not parity: alpha = (h' + 6 eta') / 2k^2 turns solver-level differences into large
-    differences at small k.  Against a converged run (rtol 1e-8) this solver's C_l at rtol 1e-4
-    is off by a median 6e-3, the fixture by 1.2e-1."""
-    print("case: reference CMB fixture configuration (128 log k x 320 tau), ellmax 100", flush=True)
+def _fixture_chain(deb, golden, batched):
     p = deb.evolve_background(param=disco_notebook(), num_thermo=1024)
     aexp = _cmb_grid(deb)
-    y, k, p = deb.evolve_perturbations(param=p, aexp_out=aexp, kmin=1e-4, kmax=1.0, num_k=128, rtol=1e-4, atol=1e-4, return_full=True,
-                                       k_sampling_method="log", **LM)
+    kw = dict(param=p, aexp_out=aexp, kmin=1e-4, kmax=1.0, num_k=128, rtol=1e-4, atol=1e-4, return_full=True, k_sampling_method="log",
+              **LM)
+    y, k, p = deb.evolve_perturbations_batched(batch_size=16, **kw) if batched else deb.evolve_perturbations(**kw)
     tau = p["tau_out"]
     yp = deb.compute_time_derivatives(y, tau, k, p, *LM.values())
     src = deb.compute_source_function_from_states(y, yp, k, aexp, tau, p, **LM)
@@ -157,27 +149,53 @@ def test_chain_against_reference_fixtures(deb, golden):
     g = golden("cmb_chain")
     rel = np.abs(cl[2:] / g["Cell"][2:] - 1)
     print(f"  C_l (l = 2..100) vs reference: median rel {np.median(rel):.2e}, max rel {rel.max():.2e}")
-    assert np.median(rel) < 0.2
+    errs = {}
     for name, plane, ik in [("alpha_k63", "alpha", 63), ("alpha_k127", "alpha", 127), ("source_sw_k63", "S2", 63),
                             ("source_sw_k127", "S2", 127), ("source_doppler_k63", "S3", 63), ("source_doppler_k127", "S3", 127),
                             ("source_polarization_k63", "S4", 63), ("source_polarization_k127", "S4", 127)]:
-        e = np.abs(_np(src[plane][ik]) - g[name]).max() / np.abs(g[name]).max()
-        print(f"  {name}: max scaled diff {e:.2e}")
-        assert e < 0.2, name
+        errs[name] = np.abs(_np(src[plane][ik]) - g[name]).max() / np.abs(g[name]).max()
+        print(f"  {name}: max scaled diff {errs[name]:.2e}")
     # Theta_l(k = 1) for l <= 100 (k chi >> l) sits at the FFTLog noise floor (~4e-7): reported only
     e = np.abs(_np(th[127]) - g["theta_ell_k_max"]).max() / np.abs(g["theta_ell_k_max"]).max()
     print(f"  theta_ell_k_max: max scaled diff {e:.2e}")
+    return rel, errs
+
+
+def test_chain_against_reference_fixtures(deb, golden):
+    """The reference's CMB test configuration (tests/test_cmb.py:93-240): DISCO-Notebook,
+    num_thermo 1024, 128 k in 1e-4..1, 320 output times, rtol = atol = 1e-4, full states;
+    Theta_l and C_l for l <= 100 (n_fftlog 16384, no densification), with the per-mode solver.
+    The fixtures were made with log-spaced k (their k63 reproduces only with that sampling) by
+    the reference's batched shared-step solver, so this is a regression check at the
+    reference's own tolerance (rtol 0.2), not parity: alpha = (h' + 6 eta') / 2k^2 turns
+    solver-level differences into large differences at small k.  Against a converged run (rtol
+    1e-8) this solver's C_l at rtol 1e-4 is off by a median 6e-3, the fixture by 1.2e-1."""
+    print("case: reference CMB fixture configuration (128 log k x 320 tau), ellmax 100, per-mode steps", flush=True)
+    rel, errs = _fixture_chain(deb, golden, batched=False)
+    assert np.median(rel) < 0.2
+    for name, e in errs.items():
+        assert e < 0.2, name
+
+
+def test_batched_chain_against_reference_fixtures(deb, golden):
+    """The same configuration through the batched shared-step solver (16 modes per batch), the
+    solver the fixtures were made with."""
+    print("case: reference CMB fixture configuration (128 log k x 320 tau), ellmax 100, batches of 16", flush=True)
+    rel, errs = _fixture_chain(deb, golden, batched=True)
+    assert np.median(rel) < 0.2
+    for name, e in errs.items():
+        assert e < 0.2, name
 
 
 def test_cell_spectrum_driver_composes_the_stages(deb):
     """compute_Cell_spectrum_from_cosmo_params (cmb.py:930-1090) equals the stages called one by one."""
-    print("case: C_l driver nmodes=24 ellmax=200 n_k_dense=128 n_fftlog=4096", flush=True)
-    kw = dict(ellmax=200, nmodes=24, kmin=1e-4, kmax=0.5, n_k_dense=128, n_fftlog=4096)
+    print("case: C_l driver nmodes=32 ellmax=200 n_k_dense=128 n_fftlog=4096", flush=True)
+    kw = dict(ellmax=200, nmodes=32, kmin=1e-4, kmax=0.5, n_k_dense=128, n_fftlog=4096)
     cl, p = deb.compute_Cell_spectrum_from_cosmo_params(disco_notebook(), **kw)
     q = deb.evolve_background(param=disco_notebook(), num_thermo=1024)
     aexp = deb.cmb_aexp_out()
-    y, k, q = deb.evolve_perturbations(param=q, aexp_out=aexp, kmin=1e-4, kmax=0.5, num_k=24, rtol=1e-4, atol=1e-4, return_full=True,
-                                       k_sampling_method="camb", **LM)
+    y, k, q = deb.evolve_perturbations_batched(param=q, aexp_out=aexp, kmin=1e-4, kmax=0.5, num_k=32, rtol=1e-4, atol=1e-4,
+                                               return_full=True, batch_size=16, k_sampling_method="camb", **LM)
     yp = deb.compute_time_derivatives(y, q["tau_out"], k, q, *LM.values())
     src = deb.compute_source_function_from_states(y, yp, k, aexp, q["tau_out"], q, **LM)
     th, kk = deb.compute_theta_ell(200, k, q["tau_out"], src["S"], float(q["tau0"]), n_fftlog=4096, n_k_dense=128)
