@@ -183,7 +183,7 @@ def main():
         tau = p["tau_of_a_spline"].evaluate(aout).reshape(1, -1)
         if record:
             ev0.record()
-        y, st, na, nr = deb.evolve_units(tables, 256, uc, kt, tau, **LMAX, **SOLVER)
+        y, st, na, nr = deb.evolve_units(tables, 256, uc, kt, tau, **LMAX, **SOLVER, schedule=deb.perturbations.DEFAULT_SCHEDULE)
         if record:
             ev1.record()
         pk = deb.get_power(k=kt, y=y, idx=4, param=P)
@@ -217,6 +217,7 @@ def main():
         "config": {"workload": "512 k-modes log 1e-5..5/Mpc, LCDM + 1 massive nu (0.06 eV, nq 8, lmax_nu 17) + w0wa "
                                "(-0.95, 0.1), lmax 12, rtol 1e-5 atol 1e-10, P(k) at z=2,1,0",
                    "k_modes": NK, "nvar": n, "dense_block": m, "num_thermo": 256, "solver": "Rodas5, per-mode adaptive",
+                   "launch": {"order": "largest k first", "threads_per_unit": deb._lib.lib().deb_default_threads_per_unit()},
                    "all_modes_ok": ok, "gradients_per_s": None},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,

@@ -102,9 +102,10 @@ def _store_cache():
 
 
 def schedule_candidates(n_units):
-    """Launch orders the autotuner tries: index order (units grouped by cosmology, k ascending)
-    and largest k first across the whole batch."""
-    return [None, {"order": "k_desc"}]
+    """Schedules the autotuner tries: index order (units grouped by cosmology, k ascending) or
+    largest k first across the whole batch, each with 64, 128 or 256 threads per unit (more
+    threads shorten every step of a unit; fewer keep more units in flight)."""
+    return [{"order": order, "threads": nt} for order in (None, "k_desc") for nt in (64, 128, 256)]
 
 
 def autotune_schedule(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, sample=1024, repeats=1, **kw):

@@ -24,10 +24,10 @@ extern "C" int deb_launch_tau_out(const double *, int, int, const double *, int,
 extern "C" int deb_launch_power_units(const double *, int, const int *, const double *, int, int, const double *, int, int, double *,
                                       hipStream_t);
 extern "C" int deb_launch_perturb_batched(const double *, int, const int *, const double *, int, int, const double *, int, const double *,
-                                          const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, double *,
+                                          const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, double *, int,
                                           hipStream_t);
 extern "C" int deb_launch_perturb(const double *, int, const int *, const double *, int, const double *, int, const double *,
-                                  const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, const int *,
+                                  const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, const int *, int,
                                   const double *, long, const int *, hipStream_t);
 extern "C" long deb_perturb_lds_bytes(int, int, int, int, int);
 extern "C" int deb_launch_spline_eval(const double *, int, int, int, const double *, int, double *, hipStream_t);
@@ -65,6 +65,8 @@ const char *deb_version(void) { return "discoeb_amd 0.1 (gfx950)"; }
 const char *deb_last_error(void) { return g_err.c_str(); }
 long deb_tables_size(int num_thermo) { return deb_table_doubles(num_thermo); }
 long deb_table_offset(int slot, int num_thermo) { return deb_array_offset(slot, num_thermo); }
+int deb_default_threads_per_unit(void) { return DEB_DEFAULT_THREADS_PER_UNIT; }
+
 long deb_evolve_workspace_doubles(void) { return 32; }  // momentum bins: 16 nodes, 16 weights
 
 int deb_nvar(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
@@ -119,9 +121,11 @@ static int evolve_common(const double *d_tables, int num_thermo, const int *d_un
   rc = deb_launch_nu_bins(prob->nqmax, d_work, (hipStream_t)stream);
   if (rc) return hip_rc(rc, "deb_evolve (momentum bins)");
   const int *order = sched ? sched->d_order : nullptr;
+  const int nt = sched ? sched->threads_per_unit : 0;
   rc = deb_launch_perturb(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, d_work, prob, cfg, d_step_log, d_y_out,
-                          d_status, d_n_accept, d_n_reject, order, d_replay, replay_stride, d_replay_len, (hipStream_t)stream);
+                          d_status, d_n_accept, d_n_reject, order, nt, d_replay, replay_stride, d_replay_len, (hipStream_t)stream);
   if (rc == -2) return fail(-1, "state too large for LDS");
+  if (rc == -4) return fail(-1, "threads_per_unit must be 0, 64, 128 or 256");
   return hip_rc(rc, "deb_evolve");
 }
 
@@ -149,8 +153,9 @@ long deb_evolve_batched_workspace_doubles(int n_units, const DebProblem *prob) {
 
 int deb_evolve_batched(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units, int batch_size,
                        const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
-                       int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, void *stream) {
+                       int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const DebSchedule *sched, void *stream) {
   if (batch_size < 1) return fail(-1, "batch_size must be >= 1");
+  if (sched && sched->d_order) return fail(-1, "batches are consecutive units: the schedule's order must be NULL");
   if (n_units % batch_size != 0) return fail(-1, "the number of units must be a multiple of batch_size");
   int rc = check_solve(d_tables, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, prob, cfg, d_y_out, d_status, d_n_accept, d_n_reject,
                        d_work);
@@ -158,8 +163,10 @@ int deb_evolve_batched(const double *d_tables, int num_thermo, const int *d_unit
   rc = deb_launch_nu_bins(prob->nqmax, d_work, (hipStream_t)stream);
   if (rc) return hip_rc(rc, "deb_evolve_batched (momentum bins)");
   rc = deb_launch_perturb_batched(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, batch_size, d_tau_out, nout, d_work, prob, cfg,
-                                  d_step_log, d_y_out, d_status, d_n_accept, d_n_reject, d_work + 32, (hipStream_t)stream);
+                                  d_step_log, d_y_out, d_status, d_n_accept, d_n_reject, d_work + 32, sched ? sched->threads_per_unit : 0,
+                                  (hipStream_t)stream);
   if (rc == -2) return fail(-1, "state too large for LDS");
+  if (rc == -4) return fail(-1, "threads_per_unit must be 0, 64, 128 or 256");
   if (rc == -3) return fail(-1, "the number of units must be a multiple of batch_size");
   return hip_rc(rc, "deb_evolve_batched");
 }

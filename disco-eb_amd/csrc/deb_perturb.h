@@ -14,12 +14,18 @@ typedef struct DebSolverCfg {
   int max_steps;
 } DebSolverCfg;
 
-/* Launch order of the units: one workgroup (one wavefront) per unit, workgroup g solving unit
- * d_order[g] (device array of n_units distinct unit indices; NULL: 0..n_units-1).  Workgroups
- * start in index order as compute slots free up, so listing the most expensive units first
- * (largest k) shortens the tail of a launch with more units than resident slots. */
+/* Threads of the workgroup that solves one unit when a schedule does not say (see below). */
+#define DEB_DEFAULT_THREADS_PER_UNIT 256
+
+/* Launch order and width: one workgroup per unit, workgroup g solving unit d_order[g] (device
+ * array of n_units distinct unit indices; NULL: 0..n_units-1).  Workgroups start in index order
+ * as compute slots free up, so listing the most expensive units first (largest k) shortens the
+ * tail of a launch with more units than resident slots.  threads_per_unit: 64, 128 or 256
+ * threads (1, 2 or 4 wavefronts) share one unit's rows, matrix entries and Jacobian columns,
+ * shortening each step at the cost of fewer units in flight; 0 = the library default. */
 typedef struct DebSchedule {
   const int *d_order;
+  int threads_per_unit;
 } DebSchedule;
 #ifdef __cplusplus
 }

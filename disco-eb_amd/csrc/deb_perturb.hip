@@ -85,8 +85,14 @@ __global__ void nu_bins_kernel(int nq, double *qw) {
 // ---------------------------------------------------------------------------------------
 template <class T> struct PBg {
   T a, tau, aH, aHH, cs2, xe, xeprime, opac, tauc, taucprime, pb43, rho_Q, w_Q, ca2_Q;
-  T v[NQMAX];
+  T amnu_a;  // a m_nu / T_nu (in units of the momentum grid)
   bool rsa, ufa, tca;
+  // velocity factor q / eps of momentum bin q, recomputed where used: an array indexed by a
+  // run-time bin would live in scratch memory
+  __device__ __forceinline__ T vq(const PLay &L, int q) const {
+    T aq = amnu_a / L.q[q];
+    return 1.0 / dsqrt(1.0 + aq * aq);
+  }
 };
 
 // Per-wave cache of the spline intervals the RHS reads: the thermal-history splines in log a
@@ -99,16 +105,16 @@ struct SplCache {
   double nx[2], nr[2], nSr[2], np[2], nSp[2];   // neutrino tables
 };
 
-// coefficients of the current intervals, in registers of every lane
+// coefficients of the current intervals, in registers of every thread
 struct SplCoef {
   double x[2], ye[2], Se[2], yc[2], Sc[2], nx[2], nr[2], nSr[2], np[2], nSp[2];
 };
 
 // Interval coefficients for log a = la.  A hit reads the LDS entry; a miss loads from the
-// tables on every lane, lane 0 republishes the entry and a barrier orders that write before
-// any later read.  The hit/miss decision is uniform across the wave (same inputs), so the
-// barrier is reached by all lanes.
-__device__ __forceinline__ void cache_lookup(const BgView &b, SplCache *c, double la, int lane, SplCoef &k) {
+// tables on every thread, thread 0 republishes the entry and a barrier orders that write before
+// any later read.  The hit/miss decision is uniform across the workgroup (same inputs, same
+// entry), so the barriers are reached by all threads.
+__device__ __forceinline__ void cache_lookup(const BgView &b, SplCache *c, double la, int tid, SplCoef &k) {
   const int N = b.N;
   bool refreshed = false;
   int i = c->i;
@@ -135,8 +141,8 @@ __device__ __forceinline__ void cache_lookup(const BgView &b, SplCache *c, doubl
     refreshed = true;
   }
   if (refreshed) {
-    __syncthreads();  // every lane has read the old entry before lane 0 overwrites it
-    if (lane == 0) {
+    __syncthreads();  // every thread has read the old entry before thread 0 overwrites it
+    if (tid == 0) {
       if (!hit) {
         c->i = i; c->x[0] = k.x[0]; c->x[1] = k.x[1]; c->ye[0] = k.ye[0]; c->ye[1] = k.ye[1]; c->Se[0] = k.Se[0];
         c->Se[1] = k.Se[1]; c->yc[0] = k.yc[0]; c->yc[1] = k.yc[1]; c->Sc[0] = k.Sc[0]; c->Sc[1] = k.Sc[1];
@@ -167,12 +173,12 @@ __device__ __forceinline__ T cubic_deriv_at(double x0, double x1, double y0, dou
 }
 
 template <class T>
-__device__ __forceinline__ void pbg_eval(const BgView &b, const PLay &L, double k, const T &tau, const T &a, PBg<T> &g, SplCache *sc, int lane) {
+__device__ __forceinline__ void pbg_eval(const BgView &b, const PLay &L, double k, const T &tau, const T &a, PBg<T> &g, SplCache *sc, int tid) {
   g.a = a;
   g.tau = tau;
   T loga = dlog(a);
   SplCoef c;
-  cache_lookup(b, sc, val(loga), lane, c);
+  cache_lookup(b, sc, val(loga), tid, c);
   g.cs2 = cubic_at(c.x[0], c.x[1], c.yc[0], c.yc[1], c.Sc[0], c.Sc[1], loga) / a;
   g.xe = cubic_at(c.x[0], c.x[1], c.ye[0], c.ye[1], c.Se[0], c.Se[1], loga);
   T dxe = cubic_deriv_at(c.x[0], c.x[1], c.ye[0], c.ye[1], c.Se[0], c.Se[1], loga);
@@ -199,10 +205,7 @@ __device__ __forceinline__ void pbg_eval(const BgView &b, const PLay &L, double 
   g.ufa = !g.rsa && b.use_ufa && (tv * k > b.ufa_k);
   bool uncoupled = (tc / fmax(tk, 1e-30) > b.tca_k) || ((tc / fmax(th, 1e-30) > b.tca_h) && (tc / fmax(tk, 1e-30) > 0.1 * b.tca_k));
   g.tca = !g.rsa && b.use_tca && !uncoupled;
-  for (int q = 0; q < L.nq; ++q) {
-    T aq = a * b.amnu / L.q[q];
-    g.v[q] = 1.0 / dsqrt(1.0 + aq * aq);
-  }
+  g.amnu_a = a * b.amnu;
 }
 
 // metric perturbations and effective (RSA/TCA) fields, perturbations.py:292-351,407-427
@@ -221,10 +224,11 @@ __device__ __forceinline__ void metric(const BgView &b, const PLay &L, double k,
   T drhonu(0.0), dpnu(0.0), fnu(0.0), shearnu(0.0);
   for (int q = 0; q < nq; ++q) {
     double p0 = y(iq0 + q), p1 = y(iq0 + nq + q), p2 = y(iq0 + 2 * nq + q);
-    drhonu += L.w[q] * p0 / g.v[q];
-    dpnu += L.w[q] * p0 * g.v[q];
+    const T v = g.vq(L, q);
+    drhonu += L.w[q] * p0 / v;
+    dpnu += L.w[q] * p0 * v;
     fnu += T(L.w[q] * p1);
-    shearnu += L.w[q] * p2 * g.v[q];
+    shearnu += L.w[q] * p2 * v;
   }
   dpnu = dpnu / 3.0;
   shearnu = shearnu * 2.0 / 3.0;
@@ -311,7 +315,7 @@ __device__ __forceinline__ T core_row(int i, const BgView &b, const PLay &L, dou
   }
   if (i >= iq0 && i < iq0 + 3 * nq) {  // massive neutrinos l = 0, 1, 2
     int l = (i - iq0) / nq, q = (i - iq0) - l * nq;
-    T kv = k * g.v[q];
+    T kv = k * g.vq(L, q);
     double dl = L.dlfdlq[q];
     if (l == 0) return -kv * y(iq0 + nq + q) + m.hprime * dl / 6.0;
     if (l == 1) return kv * (y(iq0 + q) - 2.0 * y(iq0 + 2 * nq + q)) / 3.0;
@@ -334,7 +338,7 @@ __device__ __forceinline__ void tail_coef(int ch, int l, const PLay &L, double k
   if (ch == 0) { lmax = L.lmaxg; off = g.rsa || g.tca; damp = g.opac; }
   else if (ch == 1) { lmax = L.lmaxgp; off = g.rsa || g.tca; damp = g.opac; }
   else if (ch == 2) { lmax = L.lmaxr; off = g.rsa || g.ufa; }
-  else { lmax = L.lmaxnu; off = false; kv = k * g.v[ch - 3]; }
+  else { lmax = L.lmaxnu; off = false; kv = k * g.vq(L, ch - 3); }
   if (off) { lo = T(0.0); di = T(0.0); up = T(0.0); return; }
   if (l < lmax) {
     double inv = 1.0 / (double)(2 * l + 1);
@@ -351,18 +355,7 @@ __device__ __forceinline__ void tail_coef(int ch, int l, const PLay &L, double k
 // ---------------------------------------------------------------------------------------
 // wave helpers
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-}
-__device__ __forceinline__ int wave_or(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
-  return v;
-}
-
-// per-wave LDS workspace
+// per-unit LDS workspace (shared by the workgroup's threads)
 struct Ws {
   double *y0, *y1, *u, *f, *J0, *dT, *K, *S, *Jh, *tlo, *tdi, *tup, *tmul, *td, *xd;
   int *dmap, *rk, *chead, *cstride, *clen, *cpar, *piv, *pi;
@@ -371,9 +364,10 @@ struct Ws {
 
 
 // rows of the tail chains: rk[i] = (chain << 8) | l, -1 for rows in the dense block or row 0
-__device__ __forceinline__ void setup_maps(const PLay &L, Ws &W, int lane) {
+template <int NT>
+__device__ __forceinline__ void setup_maps(const PLay &L, Ws &W, int tid) {
   const int n = L.n, nq = L.nq;
-  if (lane == 0) {
+  if (tid == 0) {
     int r = 0;
     for (int i = 1; i <= 9; ++i) W.dmap[r++] = i;
     for (int i = 0; i < 3; ++i) W.dmap[r++] = L.idxgp + i;
@@ -389,21 +383,22 @@ __device__ __forceinline__ void setup_maps(const PLay &L, Ws &W, int lane) {
       W.chead[3 + q] = L.iq0 + 3 * nq + q; W.cstride[3 + q] = nq; W.clen[3 + q] = L.lmaxnu - 2; W.cpar[3 + q] = 15 + 2 * nq + q;
     }
   }
-  for (int i = lane; i < n; i += WAVE) W.rk[i] = -1;
+  for (int i = tid; i < n; i += NT) W.rk[i] = -1;
   __syncthreads();
   for (int c = 0; c < L.nch; ++c)
-    for (int j = lane; j < W.clen[c]; j += WAVE) W.rk[W.chead[c] + j * W.cstride[c]] = (c << 8) | (3 + j);
+    for (int j = tid; j < W.clen[c]; j += NT) W.rk[W.chead[c] + j * W.cstride[c]] = (c << 8) | (3 + j);
   __syncthreads();
 }
 
 // full RHS f(tau, u) into out[] (lanes over rows); u in LDS
-__device__ __forceinline__ void rhs_eval(const BgView &b, const PLay &L, double k, double tau, const double *u, double *out, Ws &W, int lane) {
+template <int NT>
+__device__ __forceinline__ void rhs_eval(const BgView &b, const PLay &L, double k, double tau, const double *u, double *out, Ws &W, int tid) {
   PBg<double> g;
-  pbg_eval<double>(b, L, k, tau, u[0], g, W.sc, lane);
+  pbg_eval<double>(b, L, k, tau, u[0], g, W.sc, tid);
   auto yv = [&](int i) { return u[i]; };
   Met<double> m;
   metric<double>(b, L, k, g, yv, m);
-  for (int i = lane; i < L.n; i += WAVE) {
+  for (int i = tid; i < L.n; i += NT) {
     int rk = W.rk[i];
     double r;
     if (rk < 0) {
@@ -421,18 +416,19 @@ __device__ __forceinline__ void rhs_eval(const BgView &b, const PLay &L, double 
 }
 
 // d f / da (into outa) and d f / dtau (into outt) at (tau, y): one pass with two dual directions
+template <int NT>
 __device__ __forceinline__ void rhs_dual(const BgView &b, const PLay &L, double k, double tau, const double *y, double *outa, double *outt, Ws &W,
-                         int lane) {
+                         int tid) {
   typedef Dn<2> D2;
   D2 td(tau), ad(y[0]);
   ad.d[0] = 1.0;
   td.d[1] = 1.0;
   PBg<D2> g;
-  pbg_eval<D2>(b, L, k, td, ad, g, W.sc, lane);
+  pbg_eval<D2>(b, L, k, td, ad, g, W.sc, tid);
   auto yv = [&](int i) { return y[i]; };
   Met<D2> m;
   metric<D2>(b, L, k, g, yv, m);
-  for (int i = lane; i < L.n; i += WAVE) {
+  for (int i = tid; i < L.n; i += NT) {
     int rk = W.rk[i];
     D2 r;
     if (rk < 0) {
@@ -458,35 +454,42 @@ namespace deb {
 namespace {
 
 // In-place Gauss-Jordan inversion of the MC x MC row-major matrix S (LDS), MC <= 64 known at
-// compile time, with partial pivoting.  The pivot search is one wave reduction over a 64-bit key
-// (|S[r][p]| with its six lowest mantissa bits replaced by 63 - r, so ties and near-ties go to
-// the first row); row interchanges leave the inverse with permuted columns, and the composite
-// permutation pi (inverse[:, j] = S[:, pi[j]]) is built across lanes with readlane.
-template <int MC>
-__device__ __forceinline__ void gauss_jordan_fixed(double *S, int *piv, int *pi, double *prow, double *colp, int lane) {
+// compile time, with partial pivoting, by a workgroup of NT threads.  Per pivot, wave 0 finds
+// the pivot row (one wave reduction over a 64-bit key: |S[r][p]| with its six lowest mantissa
+// bits replaced by 63 - r, so ties and near-ties go to the first row), moves row p to the pivot
+// row's slot and snapshots the scaled pivot row and column p; then every thread updates its
+// share of the MC^2 entries.  Row interchanges leave the inverse with permuted columns; the
+// composite permutation pi (inverse[:, j] = S[:, pi[j]]) is built across lanes with readlane.
+template <int MC, int NT>
+__device__ __forceinline__ void gauss_jordan_fixed(double *S, int *piv, int *pi, double *prow, double *colp, int tid) {
   static_assert(MC <= WAVE, "one pivot-column entry per lane");
-  constexpr int NE = (MC * MC + WAVE - 1) / WAVE;
+  constexpr int NE = (MC * MC + NT - 1) / NT;
+  const int lane = tid & (WAVE - 1);
   for (int p = 0; p < MC; ++p) {
-    unsigned long long key = 0;
-    if (lane >= p && lane < MC) {
-      const double v = fabs(S[lane * MC + p]);
-      key = ((unsigned long long)__double_as_longlong(v) & ~63ull) | (unsigned long long)(63 - lane);
-    }
-    key = __ockl_wfred_max_u64(key);  // all 64 lanes active here
-    const int bi = 63 - (int)(key & 63ull);
-    if (lane == 0) piv[p] = bi;
-    const double ip = 1.0 / S[bi * MC + p];
-    if (lane < MC) {
-      const int j = lane;
-      const double vp = S[p * MC + j], vb = S[bi * MC + j];
-      prow[j] = (j == p) ? ip : vb * ip;
-      S[bi * MC + j] = vp;                                       // old row p moves to the pivot slot
-      colp[j] = (j == bi) ? S[p * MC + p] : S[j * MC + p];       // column p, after the interchange
+    double ip = 0.0;
+    if (NT == WAVE || tid < WAVE) {  // wave 0
+      unsigned long long key = 0;
+      if (lane >= p && lane < MC) {
+        const double v = fabs(S[lane * MC + p]);
+        key = ((unsigned long long)__double_as_longlong(v) & ~63ull) | (unsigned long long)(63 - lane);
+      }
+      key = __ockl_wfred_max_u64(key);  // all 64 lanes of the wave active here
+      const int bi = 63 - (int)(key & 63ull);
+      if (lane == 0) piv[p] = bi;
+      ip = 1.0 / S[bi * MC + p];
+      if (lane < MC) {
+        const int j = lane;
+        const double vp = S[p * MC + j], vb = S[bi * MC + j];
+        prow[j] = (j == p) ? ip : vb * ip;
+        S[bi * MC + j] = vp;                                       // old row p moves to the pivot slot
+        colp[j] = (j == bi) ? S[p * MC + p] : S[j * MC + p];       // column p, after the interchange
+      }
     }
     __syncthreads();
+    if (NT > WAVE) ip = prow[p];
 #pragma unroll 9
     for (int i = 0; i < NE; ++i) {
-      const int e = lane + i * WAVE;
+      const int e = tid + i * NT;
       if (e < MC * MC) {
         const int r = e / MC, c = e - r * MC;
         const double cr = colp[r], pc = prow[c];
@@ -496,38 +499,42 @@ __device__ __forceinline__ void gauss_jordan_fixed(double *S, int *piv, int *pi,
     }
     __syncthreads();
   }
-  int cur = lane;  // pi[lane]
-  for (int p = MC - 1; p >= 0; --p) {
-    const int q = __builtin_amdgcn_readfirstlane(piv[p]);
-    const int at_q = __builtin_amdgcn_readlane(cur, q), at_p = __builtin_amdgcn_readlane(cur, p);
-    cur = (lane == p) ? at_q : ((lane == q) ? at_p : cur);
+  if (NT == WAVE || tid < WAVE) {  // wave 0
+    int cur = lane;  // pi[lane]
+    for (int p = MC - 1; p >= 0; --p) {
+      const int q = __builtin_amdgcn_readfirstlane(piv[p]);
+      const int at_q = __builtin_amdgcn_readlane(cur, q), at_p = __builtin_amdgcn_readlane(cur, p);
+      cur = (lane == p) ? at_q : ((lane == q) ? at_p : cur);
+    }
+    if (lane < MC) pi[lane] = cur;
   }
-  if (lane < MC) pi[lane] = cur;
   __syncthreads();
 }
 
 // Jacobian pieces at (t0, y0) and the structured factorisation of W = I/(gamma dt) - J
-template <int MC>
-__device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double k, double t0, double dt, Ws &W, int lane, double &W00) {
+template <int MC, int NT>
+__device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double k, double t0, double dt, Ws &W, int tid, double &W00) {
   const int n = L.n, m = L.m, nch = L.nch;
   const double gi = 1.0 / (dt * r5::gamma);
-  rhs_dual(b, L, k, t0, W.y0, W.J0, W.dT, W, lane);
+  rhs_dual<NT>(b, L, k, t0, W.y0, W.J0, W.dT, W, tid);
   PBg<double> g;
-  pbg_eval<double>(b, L, k, t0, W.y0[0], g, W.sc, lane);
-  // dense-block columns (lanes 0..m-1) and chain-head columns (lanes m..m+nch-1)
-  for (int col = lane; col < m + nch; col += WAVE) {
+  pbg_eval<double>(b, L, k, t0, W.y0[0], g, W.sc, tid);
+  // dense-block columns 0..m-1 and chain-head columns m..m+nch-1, each split into NR row groups
+  constexpr int NR = NT / WAVE;
+  for (int it = tid; it < (m + nch) * NR; it += NT) {
+    const int col = it % (m + nch), rg = it / (m + nch);
     int s = (col < m) ? W.dmap[col] : W.chead[col - m];
     auto ev = [&](int i) { return i == s ? 1.0 : 0.0; };
     Met<double> mt;
     metric<double>(b, L, k, g, ev, mt);
-    for (int r = 0; r < m; ++r) {
+    for (int r = rg; r < m; r += NR) {
       double v = core_row<double>(W.dmap[r], b, L, k, g, mt, ev);
       if (col < m) W.S[r * m + col] = ((r == col) ? gi : 0.0) - v;  // W_DD = I/(gamma h) - J_DD
       else W.Jh[(col - m) * m + r] = v;
     }
   }
   // tail coefficients
-  for (int i = lane; i < n; i += WAVE) {
+  for (int i = tid; i < n; i += NT) {
     int rk = W.rk[i];
     if (rk >= 0) {
       double lo, di, up;
@@ -537,7 +544,7 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
   }
   __syncthreads();
   // bottom-up (UL) elimination of each chain: d_l and multipliers
-  for (int c = lane; c < nch; c += WAVE) {
+  for (int c = tid; c < nch; c += NT) {
     int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
     int i = h + (len - 1) * s;
     W.td[i] = gi - W.tdi[i];
@@ -550,7 +557,7 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
   }
   __syncthreads();
   // Schur correction of the parent diagonal column: S[:, p_c] -= Jh_c * g_c * lo_1c
-  for (int r = lane; r < m; r += WAVE) {
+  for (int r = tid; r < m; r += NT) {
     for (int c = 0; c < nch; ++c) {
       int h = W.chead[c];
       double gcoef = W.tlo[h] / W.td[h];
@@ -559,37 +566,42 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
   }
   W00 = gi - W.J0[0];
   __syncthreads();
-  // In-place Gauss-Jordan inversion with partial pivoting (first maximum).  Per pivot: one pass
-  // moves row p to the pivot row's slot and snapshots the scaled pivot row and column p, one
-  // element-parallel pass updates all m^2 entries.  Row interchanges leave the inverse with
+  // In-place Gauss-Jordan inversion with partial pivoting (first maximum).  Per pivot: wave 0
+  // moves row p to the pivot row's slot and snapshots the scaled pivot row and column p, then
+  // one element-parallel pass updates all m^2 entries.  Row interchanges leave the inverse with
   // permuted columns; the composite permutation pi is applied to the right-hand side in wsolve.
   if constexpr (MC > 0) {
-    gauss_jordan_fixed<MC>(W.S, W.piv, W.pi, W.u, W.f, lane);
+    gauss_jordan_fixed<MC, NT>(W.S, W.piv, W.pi, W.u, W.f, tid);
   } else {
     double *prow = W.u, *colp = W.f;  // free during the factorisation
+    const int lane = tid & (WAVE - 1);
     for (int p = 0; p < m; ++p) {
-      double best = -1.0;
-      int bi = p;
-      for (int r = p + lane; r < m; r += WAVE) {
-        double v = fabs(W.S[r * m + p]);
-        if (v > best) { best = v; bi = r; }
-      }
+      double ip = 0.0;
+      if (NT == WAVE || tid < WAVE) {  // wave 0: pivot search, interchange, snapshots
+        double best = -1.0;
+        int bi = p;
+        for (int r = p + lane; r < m; r += WAVE) {
+          double v = fabs(W.S[r * m + p]);
+          if (v > best) { best = v; bi = r; }
+        }
   #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        double ob = __shfl_xor(best, o);
-        int oi = __shfl_xor(bi, o);
-        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-      }
-      if (lane == 0) W.piv[p] = bi;
-      const double ip = 1.0 / W.S[bi * m + p];
-      for (int j = lane; j < m; j += WAVE) {
-        double vp = W.S[p * m + j], vb = W.S[bi * m + j];
-        prow[j] = (j == p) ? ip : vb * ip;
-        W.S[bi * m + j] = vp;                                   // old row p moves to the pivot slot
-        colp[j] = (j == bi) ? W.S[p * m + p] : W.S[j * m + p];  // column p, after the interchange
+        for (int o = 32; o > 0; o >>= 1) {
+          double ob = __shfl_xor(best, o);
+          int oi = __shfl_xor(bi, o);
+          if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        if (lane == 0) W.piv[p] = bi;
+        ip = 1.0 / W.S[bi * m + p];
+        for (int j = lane; j < m; j += WAVE) {
+          double vp = W.S[p * m + j], vb = W.S[bi * m + j];
+          prow[j] = (j == p) ? ip : vb * ip;
+          W.S[bi * m + j] = vp;                                   // old row p moves to the pivot slot
+          colp[j] = (j == bi) ? W.S[p * m + p] : W.S[j * m + p];  // column p, after the interchange
+        }
       }
       __syncthreads();
-      for (int e = lane; e < m * m; e += WAVE) {
+      if (NT > WAVE) ip = prow[p];
+      for (int e = tid; e < m * m; e += NT) {
         int r = e / m, c = e - r * m;
         double v;
         if (r == p) v = prow[c];
@@ -599,7 +611,7 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
       }
       __syncthreads();
     }
-    if (lane == 0) {  // inverse[:, j] = S[:, pi[j]]
+    if (tid == 0) {  // inverse[:, j] = S[:, pi[j]]
       for (int j = 0; j < m; ++j) W.pi[j] = j;
       for (int p = m - 1; p >= 0; --p) {
         int q = W.piv[p];
@@ -611,13 +623,14 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
 }
 
 // solve W x = v in place (v in LDS, length n)
-__device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double *v, int lane) {
+template <int NT>
+__device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double *v, int tid) {
   const int n = L.n, m = L.m, nch = L.nch;
   double x0 = v[0] / W00;
   __syncthreads();
-  for (int i = lane; i < n; i += WAVE) v[i] = (i == 0) ? x0 : v[i] + W.J0[i] * x0;
+  for (int i = tid; i < n; i += NT) v[i] = (i == 0) ? x0 : v[i] + W.J0[i] * x0;
   __syncthreads();
-  for (int c = lane; c < nch; c += WAVE) {  // bottom-up sweep
+  for (int c = tid; c < nch; c += NT) {  // bottom-up sweep
     int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
     for (int j = len - 2; j >= 0; --j) {
       int ii = h + j * s;
@@ -625,7 +638,7 @@ __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double 
     }
   }
   __syncthreads();
-  for (int r = lane; r < m; r += WAVE) {
+  for (int r = tid; r < m; r += NT) {
     double acc = v[W.dmap[r]];
     for (int c = 0; c < nch; ++c) {
       int h = W.chead[c];
@@ -634,14 +647,14 @@ __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double 
     W.xd[W.pi[r]] = acc;
   }
   __syncthreads();
-  for (int r = lane; r < m; r += WAVE) {
+  for (int r = tid; r < m; r += NT) {
     double acc = 0.0;
     const double *row = W.S + r * m;
     for (int j = 0; j < m; ++j) acc += row[j] * W.xd[j];
     v[W.dmap[r]] = acc;
   }
   __syncthreads();
-  for (int c = lane; c < nch; c += WAVE) {  // top-down sweep
+  for (int c = tid; c < nch; c += NT) {  // top-down sweep
     int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
     double xp = v[W.dmap[W.cpar[c]]];
     double x = (v[h] + W.tlo[h] * xp) / W.td[h];
@@ -655,7 +668,7 @@ __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double 
   __syncthreads();
 }
 
-// adiabatic initial conditions (perturbations.py:680-781), written by lane 0
+// adiabatic initial conditions (perturbations.py:680-781), written by one thread
 __device__ __forceinline__ void initial_conditions(const BgView &b, const PLay &L, double tau, double k, double *y) {
   const int n = L.n, nq = L.nq, iq0 = L.iq0;
   for (int i = 0; i < n; ++i) y[i] = 0.0;
@@ -724,7 +737,8 @@ __device__ __forceinline__ double starting_time(const BgView &b, double k) {
 
 // RSA output projection (approximations.py:84-176) and output conversion
 // (perturbations.py:528-677) of the interpolated state y (LDS), written to dst
-__device__ __forceinline__ void emit_output(const BgView &b, const PLay &L, double k, double tau, double *y, double *dst, int full, int lane) {
+template <int NT>
+__device__ __forceinline__ void emit_output(const BgView &b, const PLay &L, double k, double tau, double *y, double *dst, int full, int tid) {
   const int n = L.n, nq = L.nq, iq0 = L.iq0;
   const double a = y[0], loga = log(a);
   if (b.use_rsa) {
@@ -753,7 +767,7 @@ __device__ __forceinline__ void emit_output(const BgView &b, const PLay &L, doub
       double tg = -0.5 * hp + 3.0 / (sk2 * stc) * (-tcp / stc * (thetab + 0.5 * hp) + (-aH * thetab + cs2 * sk2 * deltab + sk2 * eta));
       double dr = 4.0 / sk2 * (aH * hp - sk2 * eta), tr = -0.5 * hp;
       __syncthreads();
-      for (int j = lane; j < n; j += WAVE) {
+      for (int j = tid; j < n; j += NT) {
         double v = y[j];
         if (j >= L.idxg && j <= L.idxg + L.lmaxg) v = (j == L.idxg) ? dg : (j == L.idxg + 1 ? tg : 0.0);
         else if (j >= L.idxgp && j <= L.idxgp + L.lmaxgp) v = 0.0;
@@ -764,7 +778,7 @@ __device__ __forceinline__ void emit_output(const BgView &b, const PLay &L, doub
     }
   }
   if (full) {
-    for (int j = lane; j < n; j += WAVE) dst[j] = y[j];
+    for (int j = tid; j < n; j += NT) dst[j] = y[j];
     return;
   }
   // 20 output fields
@@ -802,9 +816,9 @@ __device__ __forceinline__ void emit_output(const BgView &b, const PLay &L, doub
   thetam += alpha * k * k;
   thetabc += alpha * k * k;
   double deltanu = drhonu / rhonu, thetanu = k * fnu / (rhonu + pnu);
-  if (lane < 20) {
+  if (tid < 20) {
     double v;
-    switch (lane) {
+    switch (tid) {
       case 0: v = eta; break;
       case 1: v = etaprime; break;
       case 2: v = hprime; break;
@@ -826,7 +840,7 @@ __device__ __forceinline__ void emit_output(const BgView &b, const PLay &L, doub
       case 18: v = deltaq; break;
       default: v = thetaq / aH; break;
     }
-    dst[lane] = v;
+    dst[tid] = v;
   }
 }
 
@@ -851,22 +865,20 @@ struct Pid {
 
 // One Rodas5 step attempt of the mode with wavenumber k from W.y0 at tprev over dt: factorises
 // W = 1 / (gamma dt) - J, runs the eight stages and writes the candidate y1 = u + k8 to W.y1
-// (K + 7 n holds the error estimate k8).  Returns 1 on every lane if y1 has a NaN.
-template <int NQ>
-__device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, double k, double tprev, double dt, Ws &W, int lane) {
+// (K + 7 n holds the error estimate k8).  Returns 1 on every thread if y1 has a NaN.
+template <int NQ, int NT>
+__device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, double k, double tprev, double dt, Ws &W, int tid) {
   const int n = L.n;
   double *K = W.K;
   double *u = W.u, *f = W.f, *y0 = W.y0, *dT = W.dT;
   double W00;
-  factorise<(NQ > 0 ? 17 + 3 * NQ : 0)>(b, L, k, tprev, dt, W, lane, W00);
+  factorise<(NQ > 0 ? 17 + 3 * NQ : 0), NT>(b, L, k, tprev, dt, W, tid, W00);
   using namespace r5;
-  const double dtd[5] = {dt * d1, dt * d2, dt * d3, dt * d4, dt * d5};
-  const double cs[5] = {0.0, c2, c3, c4, c5};
   // stages 1-5 (with the df/dtau term), 6-8 at t0 + dt
   for (int s = 0; s < 8; ++s) {
     double *ks = K + s * n;
     // stage input u
-    for (int i = lane; i < n; i += WAVE) {
+    for (int i = tid; i < n; i += NT) {
       double v;
       switch (s) {
         case 0: v = y0[i]; break;
@@ -881,18 +893,21 @@ __device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, do
       u[i] = v;
     }
     __syncthreads();
-    double ts = (s < 5) ? tprev + cs[s] * dt : tprev + dt;
-    rhs_eval(b, L, k, ts, u, f, W, lane);
+    // stage time and df/dtau weight, selected without a run-time-indexed array (scratch memory)
+    const double cs = (s == 0) ? 0.0 : (s == 1) ? c2 : (s == 2) ? c3 : (s == 3) ? c4 : c5;
+    const double dts = dt * ((s == 0) ? d1 : (s == 1) ? d2 : (s == 2) ? d3 : (s == 3) ? d4 : d5);
+    double ts = (s < 5) ? tprev + cs * dt : tprev + dt;
+    rhs_eval<NT>(b, L, k, ts, u, f, W, tid);
     __syncthreads();
-    for (int i = lane; i < n; i += WAVE) {
+    for (int i = tid; i < n; i += NT) {
       double r = f[i];
       switch (s) {
-        case 0: r = r + dtd[0] * dT[i]; break;
-        case 1: r = r + dtd[1] * dT[i] + (C21 / dt) * K[i]; break;
-        case 2: r = r + dtd[2] * dT[i] + ((C31 / dt) * K[i] + (C32 / dt) * K[n + i]); break;
-        case 3: r = r + dtd[3] * dT[i] + ((C41 / dt) * K[i] + (C42 / dt) * K[n + i] + (C43 / dt) * K[2 * n + i]); break;
+        case 0: r = r + dts * dT[i]; break;
+        case 1: r = r + dts * dT[i] + (C21 / dt) * K[i]; break;
+        case 2: r = r + dts * dT[i] + ((C31 / dt) * K[i] + (C32 / dt) * K[n + i]); break;
+        case 3: r = r + dts * dT[i] + ((C41 / dt) * K[i] + (C42 / dt) * K[n + i] + (C43 / dt) * K[2 * n + i]); break;
         case 4:
-          r = r + dtd[4] * dT[i] + ((C51 / dt) * K[i] + (C52 / dt) * K[n + i] + (C53 / dt) * K[2 * n + i] + (C54 / dt) * K[3 * n + i]);
+          r = r + dts * dT[i] + ((C51 / dt) * K[i] + (C52 / dt) * K[n + i] + (C53 / dt) * K[2 * n + i] + (C54 / dt) * K[3 * n + i]);
           break;
         case 5:
           r = r + ((C61 / dt) * K[i] + (C62 / dt) * K[n + i] + (C63 / dt) * K[2 * n + i] + (C64 / dt) * K[3 * n + i] +
@@ -910,16 +925,16 @@ __device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, do
       ks[i] = r;
     }
     __syncthreads();
-    wsolve(L, W, W00, ks, lane);
+    wsolve<NT>(L, W, W00, ks, tid);
   }
   // candidate y1 = u + k8 (perturbations.py:855-859)
   int nanflag = 0;
-  for (int i = lane; i < n; i += WAVE) {
+  for (int i = tid; i < n; i += NT) {
     double v = u[i] + K[7 * n + i];
     W.y1[i] = v;
     nanflag |= (v != v);
   }
-  nanflag = wave_or(nanflag);
+  nanflag = __syncthreads_or(nanflag);
   __syncthreads();
   return nanflag;
 }
@@ -943,20 +958,20 @@ __device__ __forceinline__ double filtered_sq(double k, const double *y0, const 
 
 // One (cosmology, k) unit from its initial conditions to the last output time.  The LDS
 // workspace W and the layout L (with its momentum-bin tables) are set up by the caller.
-template <int NQ>
+template <int NQ, int NT>
 __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
                                            const double *__restrict__ unit_k, const double *__restrict__ tau_out, int nout,
                                            const DebProblem &prob, const DebSolverCfg &cfg, double *__restrict__ step_log,
                                            double *__restrict__ y_out, int *__restrict__ status, int *__restrict__ n_acc,
                                            int *__restrict__ n_rej, const double *__restrict__ replay, long replay_stride,
-                                           const int *__restrict__ replay_len, const PLay &L, Ws &W, int lane) {
+                                           const int *__restrict__ replay_len, const PLay &L, Ws &W, int tid) {
   const int cosmo = unit_cosmo[unit];
   const double k = unit_k[unit];
   const int n = L.n;
   BgView b;
   b.init(tables + (long)cosmo * tstride);
   __syncthreads();  // the previous unit's last LDS reads are done
-  if (lane == 0) { W.sc->i = -1; W.sc->j = -1; }
+  if (tid == 0) { W.sc->i = -1; W.sc->j = -1; }
   __syncthreads();
   const int wout = prob.return_full ? n : 20;
   double *yo = y_out + (long)unit * nout * wout;
@@ -964,8 +979,8 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   // fluid equations singular (their 1 / (1 + w) terms): such units are not integrated
   // (status 4, NaN outputs).  w0 = -1, wa = 0 is the cosmological constant, handled exactly.
   if (!b.lambda_de && (1.0 + b.w0) * (1.0 + b.w0 + b.wa) <= 0.0) {
-    for (int i = lane; i < nout * wout; i += WAVE) yo[i] = __builtin_nan("");
-    if (lane == 0) { status[unit] = 4; n_acc[unit] = 0; n_rej[unit] = 0; }
+    for (int i = tid; i < nout * wout; i += NT) yo[i] = __builtin_nan("");
+    if (tid == 0) { status[unit] = 4; n_acc[unit] = 0; n_rej[unit] = 0; }
     return;
   }
 
@@ -974,7 +989,7 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   double tmin = tout[0], t1 = tout[0];
   for (int j = 1; j < nout; ++j) { tmin = fmin(tmin, tout[j]); t1 = fmax(t1, tout[j]); }
   double t0 = 0.99 * fmin(tmin, starting_time(b, k));
-  if (lane == 0) initial_conditions(b, L, t0, k, W.y0);
+  if (tid == 0) initial_conditions(b, L, t0, k, W.y0);
   __syncthreads();
 
   double tprev = t0;
@@ -995,9 +1010,9 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   while (tprev < t1) {
     if (steps >= cfg.max_steps || (rs != nullptr && ridx >= rlen)) { st = 1; break; }
     const double dt = tnext - tprev;
-    const int nanflag = rodas5_attempt<NQ>(b, L, k, tprev, dt, W, lane);
+    const int nanflag = rodas5_attempt<NQ, NT>(b, L, k, tprev, dt, W, tid);
     const double en = sqrt(filtered_sq(k, W.y0, W.y1, K + 7 * n, nanflag, cfg) / 6.0);
-    if (step_log != nullptr && lane == 0) {  // (t_prev, t_next, error norm) of every attempted step
+    if (step_log != nullptr && tid == 0) {  // (t_prev, t_next, error norm) of every attempted step
       double *lg = step_log + ((long)unit * cfg.max_steps + steps) * 3;
       lg[0] = tprev;
       lg[1] = tnext;
@@ -1009,13 +1024,13 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
     if (keep) {
       while (isave < nout && tout[isave] <= tnext) {
         double th = (tout[isave] - tprev) / (tnext - tprev);
-        for (int i = lane; i < n; i += WAVE) u[i] = y0[i] + th * (W.y1[i] - y0[i]);
+        for (int i = tid; i < n; i += NT) u[i] = y0[i] + th * (W.y1[i] - y0[i]);
         __syncthreads();
-        emit_output(b, L, k, tout[isave], u, yo + (long)isave * wout, prob.return_full, lane);
+        emit_output<NT>(b, L, k, tout[isave], u, yo + (long)isave * wout, prob.return_full, tid);
         __syncthreads();
         ++isave;
       }
-      for (int i = lane; i < n; i += WAVE) y0[i] = W.y1[i];
+      for (int i = tid; i < n; i += NT) y0[i] = W.y1[i];
       ++na;
     } else {
       ++nr;
@@ -1033,9 +1048,9 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   }
   if (st != 0) {
     for (int j = isave; j < nout; ++j)
-      for (int i = lane; i < wout; i += WAVE) yo[(long)j * wout + i] = __builtin_nan("");
+      for (int i = tid; i < wout; i += NT) yo[(long)j * wout + i] = __builtin_nan("");
   }
-  if (lane == 0) {
+  if (tid == 0) {
     status[unit] = st;
     n_acc[unit] = na;
     n_rej[unit] = nr;
@@ -1046,9 +1061,9 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
 // the carve-up of the dynamic LDS into the solver workspace, which must match
 // deb_perturb_lds_bytes(): 19 n + m^2 + nch m + m + 3 NQMAX doubles, then m + n + 4 nch + 2 m
 // ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
-template <int NQ>
+template <int NQ, int NT>
 __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const double *__restrict__ nu_qw, long lds_bytes, PLay &L, Ws &W,
-                                             int lane) {
+                                             int tid) {
   L.lmaxg = prob.lmaxg; L.lmaxgp = prob.lmaxgp; L.lmaxr = prob.lmaxr; L.lmaxnu = prob.lmaxnu;
   L.nq = (NQ > 0) ? NQ : prob.nqmax;
   L.n = 7 + (L.lmaxg + 1) + (L.lmaxgp + 1) + (L.lmaxr + 1) + L.nq * (L.lmaxnu + 1) + 2;
@@ -1068,21 +1083,21 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;  // 8-byte alignment for the cache
   W.sc = (SplCache *)ip; ip += sizeof(SplCache) / 4;
   if ((long)((char *)ip - (char *)smem) > lds_bytes) return false;
-  for (int q = lane; q < L.nq; q += WAVE) {
+  for (int q = tid; q < L.nq; q += NT) {
     sq[q] = nu_qw[q];
     sw[q] = nu_qw[NQMAX + q];
     sdl[q] = -nu_qw[q] / (1.0 + exp(-nu_qw[q]));
   }
   L.q = sq; L.w = sw; L.dlfdlq = sdl;
-  setup_maps(L, W, lane);
+  setup_maps<NT>(L, W, tid);
   return true;
 }
 
 // Work distribution: workgroup g solves unit order[g] (g when order is NULL); the grid covers
 // n_units.  The dispatcher starts workgroups in index order as slots free up, so an order that
 // lists the most expensive units first keeps the tail short.
-template <int NQ>
-__global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
+template <int NQ, int NT>
+__global__ __launch_bounds__(NT) void perturb_kernel(const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
                                                        const double *__restrict__ unit_k, int n_units, const double *__restrict__ tau_out,
                                                        int nout, const double *__restrict__ nu_qw, DebProblem prob, DebSolverCfg cfg,
                                                        long lds_bytes, double *__restrict__ step_log, double *__restrict__ y_out,
@@ -1090,16 +1105,16 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
                                                        const int *__restrict__ order, const double *__restrict__ replay,
                                                        long replay_stride, const int *__restrict__ replay_len) {
   if ((int)blockIdx.x >= n_units) return;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   PLay L;
   Ws W;
-  if (!setup_solver<NQ>(prob, nu_qw, lds_bytes, L, W, lane)) {  // launch/carve-up mismatch: refuse to run
-    for (int u = blockIdx.x * WAVE + lane; u < n_units; u += gridDim.x * WAVE) { status[u] = 3; n_acc[u] = 0; n_rej[u] = 0; }
+  if (!setup_solver<NQ, NT>(prob, nu_qw, lds_bytes, L, W, tid)) {  // launch/carve-up mismatch: refuse to run
+    for (int u = blockIdx.x * NT + tid; u < n_units; u += gridDim.x * NT) { status[u] = 3; n_acc[u] = 0; n_rej[u] = 0; }
     return;
   }
   const int unit = order ? order[blockIdx.x] : (int)blockIdx.x;
-  solve_unit<NQ>(unit, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, y_out, status, n_acc, n_rej, replay,
-                 replay_stride, replay_len, L, W, lane);
+  solve_unit<NQ, NT>(unit, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, y_out, status, n_acc, n_rej, replay,
+                 replay_stride, replay_len, L, W, tid);
 }
 
 
@@ -1112,38 +1127,38 @@ __global__ __launch_bounds__(WAVE) void perturb_kernel(const double *__restrict_
 // current state and the candidate, swapped on accept), which stays in L2 between visits.
 // step_log, when given, receives (t_prev, t_next, error norm) of every attempt of batch g at
 // step_log[(g * max_steps + step) * 3].  Status 5: the batch's units have different cosmologies.
-template <int NQ>
-__global__ __launch_bounds__(WAVE) void perturb_batched_kernel(const double *__restrict__ tables, long tstride,
+template <int NQ, int NT>
+__global__ __launch_bounds__(NT) void perturb_batched_kernel(const double *__restrict__ tables, long tstride,
                                                                const int *__restrict__ unit_cosmo, const double *__restrict__ unit_k,
                                                                int n_units, int batch, const double *__restrict__ tau_out, int nout,
                                                                const double *__restrict__ nu_qw, DebProblem prob, DebSolverCfg cfg,
                                                                long lds_bytes, double *__restrict__ step_log, double *__restrict__ y_out,
                                                                int *__restrict__ status, int *__restrict__ n_acc, int *__restrict__ n_rej,
                                                                double *__restrict__ state) {
-  const int g = blockIdx.x, lane = threadIdx.x;
+  const int g = blockIdx.x, tid = threadIdx.x;
   const int u0 = g * batch;
   if (u0 + batch > n_units) return;
   PLay L;
   Ws W;
-  if (!setup_solver<NQ>(prob, nu_qw, lds_bytes, L, W, lane)) {
-    for (int j = lane; j < batch; j += WAVE) { status[u0 + j] = 3; n_acc[u0 + j] = 0; n_rej[u0 + j] = 0; }
+  if (!setup_solver<NQ, NT>(prob, nu_qw, lds_bytes, L, W, tid)) {
+    for (int j = tid; j < batch; j += NT) { status[u0 + j] = 3; n_acc[u0 + j] = 0; n_rej[u0 + j] = 0; }
     return;
   }
   const int n = L.n;
   const int cosmo = unit_cosmo[u0];
   BgView b;
   b.init(tables + (long)cosmo * tstride);
-  if (lane == 0) { W.sc->i = -1; W.sc->j = -1; }
+  if (tid == 0) { W.sc->i = -1; W.sc->j = -1; }
   __syncthreads();
   const int wout = prob.return_full ? n : 20;
   double *yo = y_out + (long)u0 * nout * wout;  // unit j's outputs at yo + j nout wout
   int mixed = 0;
-  for (int j = lane; j < batch; j += WAVE) mixed |= (unit_cosmo[u0 + j] != cosmo);
-  mixed = wave_or(mixed);
+  for (int j = tid; j < batch; j += NT) mixed |= (unit_cosmo[u0 + j] != cosmo);
+  mixed = __syncthreads_or(mixed);
   const bool phantom = !b.lambda_de && (1.0 + b.w0) * (1.0 + b.w0 + b.wa) <= 0.0;
   if (mixed || phantom) {  // not integrated: NaN outputs (w crossing -1 as in the per-mode kernel)
-    for (long i = lane; i < (long)batch * nout * wout; i += WAVE) yo[i] = __builtin_nan("");
-    for (int j = lane; j < batch; j += WAVE) { status[u0 + j] = mixed ? 5 : 4; n_acc[u0 + j] = 0; n_rej[u0 + j] = 0; }
+    for (long i = tid; i < (long)batch * nout * wout; i += NT) yo[i] = __builtin_nan("");
+    for (int j = tid; j < batch; j += NT) { status[u0 + j] = mixed ? 5 : 4; n_acc[u0 + j] = 0; n_rej[u0 + j] = 0; }
     return;
   }
 
@@ -1155,9 +1170,9 @@ __global__ __launch_bounds__(WAVE) void perturb_batched_kernel(const double *__r
   for (int j = 0; j < batch; ++j) t0 = fmin(t0, 0.99 * fmin(tmin, starting_time(b, unit_k[u0 + j])));
   double *st0 = state + (long)u0 * 2 * n;  // unit j: st0 + 2 n j + {cur, 1 - cur} n
   for (int j = 0; j < batch; ++j) {
-    if (lane == 0) initial_conditions(b, L, t0, unit_k[u0 + j], W.y0);
+    if (tid == 0) initial_conditions(b, L, t0, unit_k[u0 + j], W.y0);
     __syncthreads();
-    for (int i = lane; i < n; i += WAVE) st0[2L * n * j + i] = W.y0[i];
+    for (int i = tid; i < n; i += NT) st0[2L * n * j + i] = W.y0[i];
     __syncthreads();
   }
 
@@ -1176,17 +1191,17 @@ __global__ __launch_bounds__(WAVE) void perturb_batched_kernel(const double *__r
       const double k = unit_k[u0 + j];
       const double *ys = st0 + 2L * n * j + (long)cur * n;
       double *yd = st0 + 2L * n * j + (long)(1 - cur) * n;
-      for (int i = lane; i < n; i += WAVE) W.y0[i] = ys[i];
+      for (int i = tid; i < n; i += NT) W.y0[i] = ys[i];
       __syncthreads();
-      const int nanflag = rodas5_attempt<NQ>(b, L, k, tprev, dt, W, lane);
-      for (int i = lane; i < n; i += WAVE) yd[i] = W.y1[i];
+      const int nanflag = rodas5_attempt<NQ, NT>(b, L, k, tprev, dt, W, tid);
+      for (int i = tid; i < n; i += NT) yd[i] = W.y1[i];
       acc += filtered_sq(k, W.y0, W.y1, W.K + 7 * n, 0, cfg);
       acc_nan += filtered_sq(k, W.y0, W.y1, W.K + 7 * n, 1, cfg);
       nanany |= nanflag;
       __syncthreads();
     }
     const double en = sqrt((nanany ? acc_nan : acc) / (6.0 * batch));
-    if (step_log != nullptr && lane == 0) {
+    if (step_log != nullptr && tid == 0) {
       double *lg = step_log + ((long)g * cfg.max_steps + steps) * 3;
       lg[0] = tprev;
       lg[1] = tnext;
@@ -1202,14 +1217,14 @@ __global__ __launch_bounds__(WAVE) void perturb_batched_kernel(const double *__r
           const double k = unit_k[u0 + j];
           const double *ya = st0 + 2L * n * j + (long)cur * n;
           const double *yb = st0 + 2L * n * j + (long)(1 - cur) * n;
-          for (int i = lane; i < n; i += WAVE) { W.y0[i] = ya[i]; W.y1[i] = yb[i]; }
+          for (int i = tid; i < n; i += NT) { W.y0[i] = ya[i]; W.y1[i] = yb[i]; }
           __syncthreads();
           isv = isave;
           while (isv < nout && tout[isv] <= tnext) {
             double th = (tout[isv] - tprev) / (tnext - tprev);
-            for (int i = lane; i < n; i += WAVE) W.u[i] = W.y0[i] + th * (W.y1[i] - W.y0[i]);
+            for (int i = tid; i < n; i += NT) W.u[i] = W.y0[i] + th * (W.y1[i] - W.y0[i]);
             __syncthreads();
-            emit_output(b, L, k, tout[isv], W.u, yo + ((long)j * nout + isv) * wout, prob.return_full, lane);
+            emit_output<NT>(b, L, k, tout[isv], W.u, yo + ((long)j * nout + isv) * wout, prob.return_full, tid);
             __syncthreads();
             ++isv;
           }
@@ -1230,9 +1245,9 @@ __global__ __launch_bounds__(WAVE) void perturb_batched_kernel(const double *__r
   if (st != 0) {
     for (int j = 0; j < batch; ++j)
       for (int r = isave; r < nout; ++r)
-        for (int i = lane; i < wout; i += WAVE) yo[((long)j * nout + r) * wout + i] = __builtin_nan("");
+        for (int i = tid; i < wout; i += NT) yo[((long)j * nout + r) * wout + i] = __builtin_nan("");
   }
-  for (int j = lane; j < batch; j += WAVE) {
+  for (int j = tid; j < batch; j += NT) {
     status[u0 + j] = st;
     n_acc[u0 + j] = na;
     n_rej[u0 + j] = nr;
@@ -1273,12 +1288,12 @@ __global__ __launch_bounds__(WAVE) void time_deriv_kernel(const double *__restri
   ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;
   W.sc = (SplCache *)ip;
   if (lane == 0) { W.sc->i = -1; W.sc->j = -1; }
-  setup_maps(L, W, lane);
+  setup_maps<WAVE>(L, W, lane);
   for (int it = 0; it < n_tau; ++it) {
     const double *src = yin + ((long)ik * n_tau + it) * n;
     for (int i = lane; i < n; i += WAVE) W.u[i] = src[i];
     __syncthreads();
-    rhs_eval(b, L, k, tau[it], W.u, W.f, W, lane);
+    rhs_eval<WAVE>(b, L, k, tau[it], W.u, W.f, W, lane);
     __syncthreads();
     double *dst = dy + ((long)ik * n_tau + it) * n;
     for (int i = lane; i < n; i += WAVE) dst[i] = W.f[i];
@@ -1332,51 +1347,75 @@ extern "C" int deb_launch_tau_out(const double *d_tables, int num_thermo, int n_
   return (int)hipGetLastError();
 }
 
+namespace {
+typedef decltype(&deb::perturb_kernel<0, deb::WAVE>) PerturbFn;
+typedef decltype(&deb::perturb_batched_kernel<0, deb::WAVE>) PerturbBatchedFn;
+
+// Instantiation for nq momentum bins (3, 5 and 8 at compile time, others generic) and
+// threads_per_unit threads (64, 128 or 256) per unit.
+template <int NQ> PerturbFn pick_nt(int nt) {
+  return nt == 256 ? deb::perturb_kernel<NQ, 256> : (nt == 128 ? deb::perturb_kernel<NQ, 128> : deb::perturb_kernel<NQ, 64>);
+}
+template <int NQ> PerturbBatchedFn pick_nt_batched(int nt) {
+  return nt == 256 ? deb::perturb_batched_kernel<NQ, 256>
+                   : (nt == 128 ? deb::perturb_batched_kernel<NQ, 128> : deb::perturb_batched_kernel<NQ, 64>);
+}
+PerturbFn pick_perturb(int nq, int nt) {
+  switch (nq) {
+    case 3: return pick_nt<3>(nt);
+    case 5: return pick_nt<5>(nt);
+    case 8: return pick_nt<8>(nt);
+    default: return pick_nt<0>(nt);
+  }
+}
+PerturbBatchedFn pick_perturb_batched(int nq, int nt) {
+  switch (nq) {
+    case 3: return pick_nt_batched<3>(nt);
+    case 5: return pick_nt_batched<5>(nt);
+    case 8: return pick_nt_batched<8>(nt);
+    default: return pick_nt_batched<0>(nt);
+  }
+}
+}  // namespace
+
 extern "C" int deb_launch_perturb(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
                                   const double *d_tau_out, int nout, const double *d_nu_qw, const DebProblem *prob,
                                   const DebSolverCfg *cfg, double *d_step_log, double *d_y_out, int *d_status, int *d_nacc, int *d_nrej,
-                                  const int *d_order, const double *d_replay, long replay_stride, const int *d_replay_len,
-                                  hipStream_t stream) {
+                                  const int *d_order, int threads_per_unit, const double *d_replay, long replay_stride,
+                                  const int *d_replay_len, hipStream_t stream) {
   long stride = deb_table_doubles(num_thermo);
   long lds = deb_perturb_lds_bytes(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
   if (lds > 160 * 1024) return -2;
-  auto kern = deb::perturb_kernel<0>;
-  switch (prob->nqmax) {
-    case 3: kern = deb::perturb_kernel<3>; break;
-    case 5: kern = deb::perturb_kernel<5>; break;
-    case 8: kern = deb::perturb_kernel<8>; break;
-    default: break;
-  }
+  const int nt = threads_per_unit > 0 ? threads_per_unit : DEB_DEFAULT_THREADS_PER_UNIT;
+  if (nt != 64 && nt != 128 && nt != 256) return -4;
+  PerturbFn kern = pick_perturb(prob->nqmax, nt);
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return (int)e;
   }
-  hipLaunchKernelGGL(kern, dim3(n_units), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units,
-                     d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej, d_order, d_replay,
-                     replay_stride, d_replay_len);
+  hipLaunchKernelGGL(kern, dim3(n_units), dim3(nt), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units, d_tau_out,
+                     nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej, d_order, d_replay, replay_stride,
+                     d_replay_len);
   return (int)hipGetLastError();
 }
 
 extern "C" int deb_launch_perturb_batched(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k,
                                           int n_units, int batch, const double *d_tau_out, int nout, const double *d_nu_qw,
                                           const DebProblem *prob, const DebSolverCfg *cfg, double *d_step_log, double *d_y_out,
-                                          int *d_status, int *d_nacc, int *d_nrej, double *d_state, hipStream_t stream) {
+                                          int *d_status, int *d_nacc, int *d_nrej, double *d_state, int threads_per_unit,
+                                          hipStream_t stream) {
   long stride = deb_table_doubles(num_thermo);
   long lds = deb_perturb_lds_bytes(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
   if (lds > 160 * 1024) return -2;
   if (batch <= 0 || n_units % batch != 0) return -3;
-  auto kern = deb::perturb_batched_kernel<0>;
-  switch (prob->nqmax) {
-    case 3: kern = deb::perturb_batched_kernel<3>; break;
-    case 5: kern = deb::perturb_batched_kernel<5>; break;
-    case 8: kern = deb::perturb_batched_kernel<8>; break;
-    default: break;
-  }
+  const int nt = threads_per_unit > 0 ? threads_per_unit : DEB_DEFAULT_THREADS_PER_UNIT;
+  if (nt != 64 && nt != 128 && nt != 256) return -4;
+  PerturbBatchedFn kern = pick_perturb_batched(prob->nqmax, nt);
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return (int)e;
   }
-  hipLaunchKernelGGL(kern, dim3(n_units / batch), dim3(deb::WAVE), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units,
+  hipLaunchKernelGGL(kern, dim3(n_units / batch), dim3(nt), (size_t)lds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units,
                      batch, d_tau_out, nout, d_nu_qw, *prob, *cfg, lds, d_step_log, d_y_out, d_status, d_nacc, d_nrej, d_state);
   return (int)hipGetLastError();
 }

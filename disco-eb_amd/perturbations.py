@@ -67,8 +67,9 @@ def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, l
 
     tables: (n_cosmo, size) device tensor from background_tables; unit_cosmo (int32) and unit_k
     (float64) device tensors of length n_units; tau_out (n_cosmo, nout) device tensor.
-    schedule: None (units launched in index order, one workgroup each) or a dict with key
-    order: None, "k_desc" (largest k first) or an int32 device tensor permuting the units.
+    schedule: None (units launched in index order, one workgroup each) or a dict with keys
+    order: None, "k_desc" (largest k first) or an int32 device tensor permuting the units, and
+    threads: 64, 128 or 256 threads per unit (absent or 0: the library default).
     replay: None, or (mesh (n_units, L) float64, mesh_len (n_units,) int32) device tensors: each
     unit then takes exactly its accepted steps, ending at t0 + s (t1 - t0) for the listed s
     (the frozen-mesh map, see deb_evolve_replay).
@@ -117,10 +118,10 @@ def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, l
 
 def evolve_units_batched(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, batch_size=16, lmaxg=11, lmaxgp=11, lmaxr=11,
                          lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4, pcoeff=0.25, icoeff=0.80, dcoeff=0.0, factormax=20.0,
-                         factormin=0.3, max_steps=2048, return_full=False, step_log=False):
+                         factormin=0.3, max_steps=2048, return_full=False, step_log=False, threads=0):
     """Low-level launch of the batched shared-step solve (deb_evolve_batched): the units
     g * batch_size .. (g + 1) * batch_size - 1, which must share a cosmology, step together.
-    Arguments as evolve_units.  Returns (y, status, n_accept, n_reject) device tensors, plus the
+    Arguments as evolve_units; threads: 64, 128 or 256 threads per batch (0: library default).  Returns (y, status, n_accept, n_reject) device tensors, plus the
     per-batch step mesh (n_units / batch_size, max_steps, 3) when step_log is set."""
     L = _lib.lib()
     dev = tables.device
@@ -140,9 +141,11 @@ def evolve_units_batched(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, bat
     to = tau_out.to(device=dev, dtype=torch.float64).contiguous()
     log = (torch.full((n_units // batch_size, int(max_steps), 3), float("nan"), dtype=torch.float64, device=dev)
            if step_log else None)
+    sched, _ = _schedule({"threads": threads}, uk)
     _lib.check(L.deb_evolve_batched(_lib.ptr(tables), int(num_thermo), _lib.ptr(uc), _lib.ptr(uk), n_units, int(batch_size), _lib.ptr(to),
                                     nout, ctypes.byref(prob), ctypes.byref(cfg), _lib.ptr(y), _lib.ptr(status), _lib.ptr(nacc),
-                                    _lib.ptr(nrej), _lib.ptr(work), _lib.ptr(log) if log is not None else None, _lib.stream()))
+                                    _lib.ptr(nrej), _lib.ptr(work), _lib.ptr(log) if log is not None else None,
+                                    ctypes.byref(sched) if sched is not None else None, _lib.stream()))
     if step_log:
         return y, status, nacc, nrej, log
     return y, status, nacc, nrej
@@ -152,6 +155,9 @@ def _schedule(schedule, uk):
     """DebSchedule for a schedule dict (see evolve_units) and the order tensor it points to."""
     if schedule is None:
         return None, None
+    threads = int(schedule.get("threads", 0) or 0)
+    if threads not in (0, 64, 128, 256):
+        raise ValueError(f"threads per unit must be 64, 128 or 256 (got {threads})")
     order = schedule.get("order")
     if isinstance(order, str):
         if order != "k_desc":
@@ -166,8 +172,8 @@ def _schedule(schedule, uk):
             if not torch.equal(srt, torch.arange(uk.numel())):
                 raise ValueError("order must be a permutation of 0..n_units-1")
     if order is None:
-        return None, None
-    return _lib.DebSchedule(order.data_ptr()), order
+        return (_lib.DebSchedule(None, threads) if threads else None), None
+    return _lib.DebSchedule(order.data_ptr(), threads), order
 
 
 def _raise_on_status(status, allow_phantom=False):
@@ -185,6 +191,11 @@ def _raise_on_status(status, allow_phantom=False):
     if (st == 4).any():
         raise ValueError("w(a) = w_DE_0 + w_DE_a (1 - a) crosses -1 in (0, 1]: the dark-energy fluid equations are singular "
                          "at the phantom divide (use w_DE_0 = -1, w_DE_a = 0 for a cosmological constant)")
+
+
+# Launch order of evolve_perturbations: largest k (most steps) first, so the longest solves
+# start first when there are more units than resident workgroups.  Results do not depend on it.
+DEFAULT_SCHEDULE = {"order": "k_desc"}
 
 
 def evolve_perturbations(*, param, aexp_out, kmin: float, kmax: float, num_k: int, lmaxg: int = 11, lmaxgp: int = 11,
@@ -211,7 +222,7 @@ def evolve_perturbations(*, param, aexp_out, kmin: float, kmax: float, num_k: in
     y, status, nacc, nrej = evolve_units(tables, param["_deb_num_thermo"], uc, kt, tau_all, lmaxg=lmaxg, lmaxgp=lmaxgp,
                                          lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax, rtol=rtol, atol=atol, pcoeff=pcoeff,
                                          icoeff=icoeff, dcoeff=dcoeff, factormax=factormax, factormin=factormin,
-                                         max_steps=max_steps, return_full=return_full)
+                                         max_steps=max_steps, return_full=return_full, schedule=DEFAULT_SCHEDULE)
     _raise_on_status(status)
     param["lmaxg"], param["lmaxgp"], param["lmaxr"], param["lmaxnu"], param["nqmax"] = lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax
     param["nout"] = int(aexp.numel())

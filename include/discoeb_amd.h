@@ -47,6 +47,9 @@ enum {
 int deb_spline_eval(const double *d_tables, int num_thermo, int cosmo, int spline_id, int kind, const double *d_x, int nx,
                     double *d_out, void *stream);
 
+/* Threads per unit deb_evolve and deb_evolve_batched use when no schedule sets them. */
+int deb_default_threads_per_unit(void);
+
 /* Doubles of device workspace deb_evolve needs (neutrino momentum bins). */
 long deb_evolve_workspace_doubles(void);
 
@@ -60,9 +63,10 @@ long deb_evolve_workspace_doubles(void);
  * d_work: deb_evolve_workspace_doubles() doubles.
  * d_step_log: NULL, or n_units x max_steps x 3 doubles receiving (t_prev, t_next, error norm)
  * of every attempted step (the step mesh, for replay checks and frozen-mesh derivatives).
- * sched: NULL (units in index order) or a DebSchedule (deb_perturb.h) giving the launch
- * order of the units, one workgroup each (most expensive first shortens the tail).  Results
- * do not depend on the order.
+ * sched: NULL (units in index order, library-default width) or a DebSchedule (deb_perturb.h)
+ * giving the launch order of the units, one workgroup each (most expensive first shortens the
+ * tail), and the workgroup width (threads_per_unit).  Results depend on neither: every thread
+ * count performs the same operations on each entry.
  * Replaces evolve_perturbations (src/discoeb/perturbations.py:1124-1226) with
  * evolve_one_mode (:882-949) per unit. */
 int deb_evolve(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
@@ -82,12 +86,14 @@ long deb_evolve_batched_workspace_doubles(int n_units, const DebProblem *prob);
  * d_work: deb_evolve_batched_workspace_doubles(n_units, prob) doubles.
  * d_step_log: NULL, or (n_units / batch_size) x max_steps x 3 doubles receiving (t_prev,
  * t_next, error norm) of every attempted step of each batch.
+ * sched: NULL or a DebSchedule with d_order NULL; its threads_per_unit sets the width of the
+ * workgroup that runs each batch.
  * Replaces evolve_perturbations_batched (src/discoeb/perturbations.py:1228-1337) with
  * evolve_modes_batched (:1001-1120), determine_start_times_per_batch (:954-999) and
  * Rodas5Batched (src/discoeb/ode_integrators_stiff.py:904-1059). */
 int deb_evolve_batched(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units, int batch_size,
                        const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
-                       int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, void *stream);
+                       int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const DebSchedule *sched, void *stream);
 
 /* deb_evolve on a frozen step mesh: unit u takes exactly d_mesh_len[u] steps whose ends are
  * t0 + s (t1 - t0) for the increasing s = d_mesh[u * mesh_stride + i] (the last one 1), with

@@ -135,3 +135,28 @@ def test_cost_order_matches_index_order_for_eight_momentum_bins(deb):
     got = deb.evolve_units(p["_deb_tables"], 256, uc, kt, tau, **kw, schedule={"order": "k_desc"})
     for a, b in zip(got, ref):
         assert _same(a, b)
+
+
+@pytest.mark.parametrize("nq,lmaxnu", [(8, 17), (3, 8), (4, 6)])
+def test_workgroup_width_gives_identical_results(deb, nq, lmaxnu):
+    """One, two or four wavefronts per unit perform the same operations on every entry (rows,
+    matrix elements and Jacobian columns are only distributed differently): bitwise the same
+    states, step counts and step meshes, per-mode and batched; nq = 4 runs the generic
+    (run-time block size) factorisation."""
+    print(f"case: threads per unit 64/128/256, nq={nq} lmax_nu={lmaxnu}, 6 k 1e-5..5", flush=True)
+    p = deb.evolve_background(param=dict(nu_w0wa()), num_thermo=256)
+    tau = p["tau_of_a_spline"].evaluate(np.asarray([1 / 3, 0.5, 1.0])).reshape(1, -1)
+    kt = torch.as_tensor(deb.k_grid({}, 1e-5, 5.0, 6, "log"), device="cuda")
+    uc = torch.zeros(6, dtype=torch.int32, device="cuda")
+    kw = dict(lmaxg=12, lmaxgp=12, lmaxr=12, lmaxnu=lmaxnu, nqmax=nq, rtol=1e-5, atol=1e-10)
+    ref = deb.evolve_units(p["_deb_tables"], 256, uc, kt, tau, **kw, step_log=True, schedule={"threads": 64})
+    assert int(ref[1].abs().max()) == 0
+    refb = deb.evolve_units_batched(p["_deb_tables"], 256, uc, kt, tau, batch_size=3, **kw, step_log=True, threads=64)
+    assert int(refb[1].abs().max()) == 0
+    for nt in (128, 256):
+        got = deb.evolve_units(p["_deb_tables"], 256, uc, kt, tau, **kw, step_log=True, schedule={"order": "k_desc", "threads": nt})
+        for a, b in zip(got, ref):
+            assert _same(a, b), nt
+        gotb = deb.evolve_units_batched(p["_deb_tables"], 256, uc, kt, tau, batch_size=3, **kw, step_log=True, threads=nt)
+        for a, b in zip(gotb, refb):
+            assert _same(a, b), nt
