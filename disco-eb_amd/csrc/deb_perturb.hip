@@ -357,8 +357,9 @@ __device__ __forceinline__ void tail_coef(int ch, int l, const PLay &L, double k
 // ---------------------------------------------------------------------------------------
 // per-unit LDS workspace (shared by the workgroup's threads)
 struct Ws {
-  double *y0, *y1, *u, *f, *J0, *dT, *K, *S, *Jh, *tlo, *tdi, *tup, *tmul, *td, *xd;
-  int *dmap, *rk, *chead, *cstride, *clen, *cpar, *piv, *pi;
+  double *y0, *y1, *u, *f, *J0, *dT, *K, *S, *Jh, *tlo, *tdi, *tup, *tmul, *td, *xd, *chx;
+  double *S2, *snap, *Sinv;  // second matrix buffer, pivot snapshots, the buffer holding the inverse
+  int *dmap, *rk, *chead, *cstride, *clen, *cpar, *piv, *pi, *rmap;
   SplCache *sc;
 };
 
@@ -453,6 +454,8 @@ extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max
 namespace deb {
 namespace {
 
+constexpr int GJ_CHUNK = 9;  // entries per thread loaded together in the Gauss-Jordan updates
+
 // In-place Gauss-Jordan inversion of the MC x MC row-major matrix S (LDS), MC <= 64 known at
 // compile time, with partial pivoting, by a workgroup of NT threads.  Per pivot, wave 0 finds
 // the pivot row (one wave reduction over a 64-bit key: |S[r][p]| with its six lowest mantissa
@@ -461,7 +464,7 @@ namespace {
 // share of the MC^2 entries.  Row interchanges leave the inverse with permuted columns; the
 // composite permutation pi (inverse[:, j] = S[:, pi[j]]) is built across lanes with readlane.
 template <int MC, int NT>
-__device__ __forceinline__ void gauss_jordan_fixed(double *S, int *piv, int *pi, double *prow, double *colp, int tid) {
+__device__ __forceinline__ void gauss_jordan_fixed(double *S, int *piv, int *pi, int *rmap, double *prow, double *colp, int tid) {
   static_assert(MC <= WAVE, "one pivot-column entry per lane");
   constexpr int NE = (MC * MC + NT - 1) / NT;
   const int lane = tid & (WAVE - 1);
@@ -487,14 +490,28 @@ __device__ __forceinline__ void gauss_jordan_fixed(double *S, int *piv, int *pi,
     }
     __syncthreads();
     if (NT > WAVE) ip = prow[p];
-#pragma unroll 9
-    for (int i = 0; i < NE; ++i) {
-      const int e = tid + i * NT;
-      if (e < MC * MC) {
-        const int r = e / MC, c = e - r * MC;
-        const double cr = colp[r], pc = prow[c];
-        double v = (c == p) ? -cr * ip : S[e] - cr * pc;
-        S[e] = (r == p) ? pc : v;
+    // each thread reads a chunk of its entries before writing them (loads are not held up by stores)
+#pragma unroll 1
+    for (int i0 = 0; i0 < NE; i0 += GJ_CHUNK) {
+      double a[GJ_CHUNK], cr[GJ_CHUNK], pc[GJ_CHUNK];
+#pragma unroll
+      for (int i = 0; i < GJ_CHUNK; ++i) {
+        const int e = tid + (i0 + i) * NT;
+        if (i0 + i < NE && e < MC * MC) {
+          const int r = e / MC, c = e - r * MC;
+          a[i] = S[e];
+          cr[i] = colp[r];
+          pc[i] = prow[c];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < GJ_CHUNK; ++i) {
+        const int e = tid + (i0 + i) * NT;
+        if (i0 + i < NE && e < MC * MC) {
+          const int r = e / MC, c = e - r * MC;
+          const double v = (c == p) ? -cr[i] * ip : a[i] - cr[i] * pc[i];
+          S[e] = (r == p) ? pc[i] : v;
+        }
       }
     }
     __syncthreads();
@@ -506,9 +523,141 @@ __device__ __forceinline__ void gauss_jordan_fixed(double *S, int *piv, int *pi,
       const int at_q = __builtin_amdgcn_readlane(cur, q), at_p = __builtin_amdgcn_readlane(cur, p);
       cur = (lane == p) ? at_q : ((lane == q) ? at_p : cur);
     }
-    if (lane < MC) pi[lane] = cur;
+    if (lane < MC) { pi[lane] = cur; rmap[lane] = lane; }
   }
   __syncthreads();
+}
+
+// Gauss-Jordan inversion for two or more wavefronts, with partial pivoting (first maximum among
+// the rows not yet used as pivots) and no row interchanges: step p eliminates column p with the
+// pivot in row piv[p].  The matrix alternates between buffers A and B (step p reads one and
+// writes the other), so while waves 1.. apply step p, wave 0 already prepares step p + 1 from
+// the buffer being read: it forms column p + 1 and the pivot row as step p leaves them, picks
+// the pivot and publishes the scaled pivot row, column p + 1 and 1 / pivot.  One barrier per
+// step.  Each entry is computed by the same expressions as in the interchanging variant.  The
+// result R holds row p of the inverse of the row-permuted matrix in row piv[p]: x = A^-1 b is
+// x[p] = sum_c R[piv[p]][c] b[piv[c]], so rmap[p] = piv[p] and the right-hand side is scattered
+// through pi = piv^-1.  snap: 5 MC doubles (two pivot rows, two columns, 1 / pivot per step).
+// Returns the buffer holding R.
+template <int MC, int NT>
+__device__ __forceinline__ double *gauss_jordan_lookahead(double *A, double *B, int *piv, int *pi, int *rmap, double *snap, int tid) {
+  static_assert(MC <= WAVE && NT > WAVE, "one pivot-column entry per lane; one wave prepares, the others update");
+  constexpr int NU = NT - WAVE;  // threads of waves 1..
+  constexpr int NE = (MC * MC + NU - 1) / NU;
+  const int lane = tid & (WAVE - 1);
+  double *prow0 = snap, *colp0 = snap + 2 * MC, *ipv = snap + 4 * MC;
+  bool used = false;  // wave 0, lane r: row r has been a pivot row
+  // the pivot search over column values col (lane r holds row r), rows not yet used
+  auto choose = [&](double col) {
+    unsigned long long key = 0;
+    if (lane < MC && !used) key = ((unsigned long long)__double_as_longlong(fabs(col)) & ~63ull) | (unsigned long long)(63 - lane);
+    key = __ockl_wfred_max_u64(key);  // all 64 lanes of wave 0 active
+    return 63 - (int)(key & 63ull);
+  };
+  if (tid < WAVE) {  // step 0 from the matrix as built
+    const double col = (lane < MC) ? A[lane * MC] : 0.0;
+    const int bi = choose(col);
+    const double ip = 1.0 / A[bi * MC];
+    if (lane < MC) {
+      prow0[lane] = (lane == 0) ? ip : A[bi * MC + lane] * ip;
+      colp0[lane] = col;
+    }
+    used = (lane == bi);
+    if (lane == 0) { piv[0] = bi; ipv[0] = ip; }
+  }
+  __syncthreads();
+  // rows and columns of this thread's entries (waves 1..), fixed across the steps, when they
+  // fit in one chunk of registers
+  constexpr bool FIXED = NE <= GJ_CHUNK;
+  int er[FIXED ? NE : 1], ec[FIXED ? NE : 1];
+  if constexpr (FIXED) {
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      const int e = (tid - WAVE) + i * NU;
+      er[i] = e / MC;
+      ec[i] = e - er[i] * MC;
+    }
+  }
+  const bool last_ok = (tid - WAVE) + (NE - 1) * NU < MC * MC;  // the last entry exists
+  double *cur = A, *nxt = B;
+  for (int p = 0; p < MC; ++p) {
+    const double *prow = prow0 + (p & 1) * MC, *colp = colp0 + (p & 1) * MC;
+    const int bi = piv[p];
+    const double ip = ipv[p];
+    if (FIXED && tid >= WAVE) {
+      // all loads of this thread's entries first, then its stores (the buffers are not known
+      // to the compiler to be distinct, so interleaving would serialise on LDS latency)
+      double a[FIXED ? NE : 1], cr[FIXED ? NE : 1], pc[FIXED ? NE : 1];
+#pragma unroll
+      for (int i = 0; i < NE; ++i) {
+        if (i < NE - 1 || last_ok) {
+          a[i] = cur[(tid - WAVE) + i * NU];
+          cr[i] = colp[er[i]];
+          pc[i] = prow[ec[i]];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NE; ++i) {
+        if (i < NE - 1 || last_ok) {
+          const double v = (ec[i] == p) ? -cr[i] * ip : a[i] - cr[i] * pc[i];
+          nxt[(tid - WAVE) + i * NU] = (er[i] == bi) ? pc[i] : v;
+        }
+      }
+    } else if (tid >= WAVE) {
+      // loads of a chunk of this thread's entries first, then its stores
+#pragma unroll 1
+      for (int i0 = 0; i0 < NE; i0 += GJ_CHUNK) {
+        double a[GJ_CHUNK], cr[GJ_CHUNK], pc[GJ_CHUNK];
+#pragma unroll
+        for (int i = 0; i < GJ_CHUNK; ++i) {
+          const int e = (tid - WAVE) + (i0 + i) * NU;
+          if (i0 + i < NE && e < MC * MC) {
+            const int r = e / MC, c = e - r * MC;
+            a[i] = cur[e];
+            cr[i] = colp[r];
+            pc[i] = prow[c];
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < GJ_CHUNK; ++i) {
+          const int e = (tid - WAVE) + (i0 + i) * NU;
+          if (i0 + i < NE && e < MC * MC) {
+            const int r = e / MC, c = e - r * MC;
+            const double v = (c == p) ? -cr[i] * ip : a[i] - cr[i] * pc[i];
+            nxt[e] = (r == bi) ? pc[i] : v;
+          }
+        }
+      }
+    } else if (p + 1 < MC) {  // wave 0: step p + 1
+      double *prown = prow0 + ((p + 1) & 1) * MC, *colpn = colp0 + ((p + 1) & 1) * MC;
+      const int q = p + 1;
+      double col = 0.0;  // column q after step p
+      if (lane < MC) {
+        const double cr = colp[lane], pc = prow[q];
+        col = (lane == bi) ? pc : cur[lane * MC + q] - cr * pc;
+      }
+      const int bn = choose(col);
+      const unsigned long long cb = (unsigned long long)__double_as_longlong(col);  // broadcast lane bn's value
+      const unsigned lo = __builtin_amdgcn_readlane((unsigned)cb, bn), hi = __builtin_amdgcn_readlane((unsigned)(cb >> 32), bn);
+      const double ipn = 1.0 / __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+      if (lane < MC) {  // pivot row bn after step p (bn != bi), scaled
+        const double cr = colp[bn], pc = prow[lane];
+        const double v = (lane == p) ? -cr * ip : cur[bn * MC + lane] - cr * pc;
+        prown[lane] = (lane == q) ? ipn : v * ipn;
+        colpn[lane] = col;
+      }
+      used = used || (lane == bn);
+      if (lane == 0) { piv[q] = bn; ipv[q] = ipn; }
+    }
+    __syncthreads();
+    double *t = cur; cur = nxt; nxt = t;
+  }
+  if (tid < MC) {
+    rmap[tid] = piv[tid];
+    pi[piv[tid]] = tid;
+  }
+  __syncthreads();
+  return cur;
 }
 
 // Jacobian pieces at (t0, y0) and the structured factorisation of W = I/(gamma dt) - J
@@ -570,9 +719,13 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
   // moves row p to the pivot row's slot and snapshots the scaled pivot row and column p, then
   // one element-parallel pass updates all m^2 entries.  Row interchanges leave the inverse with
   // permuted columns; the composite permutation pi is applied to the right-hand side in wsolve.
-  if constexpr (MC > 0) {
-    gauss_jordan_fixed<MC, NT>(W.S, W.piv, W.pi, W.u, W.f, tid);
+  if constexpr (MC > 0 && NT > WAVE) {
+    W.Sinv = gauss_jordan_lookahead<MC, NT>(W.S, W.S2, W.piv, W.pi, W.rmap, W.snap, tid);
+  } else if constexpr (MC > 0) {
+    gauss_jordan_fixed<MC, NT>(W.S, W.piv, W.pi, W.rmap, W.u, W.f, tid);
+    W.Sinv = W.S;
   } else {
+    W.Sinv = W.S;
     double *prow = W.u, *colp = W.f;  // free during the factorisation
     const int lane = tid & (WAVE - 1);
     for (int p = 0; p < m; ++p) {
@@ -612,7 +765,7 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
       __syncthreads();
     }
     if (tid == 0) {  // inverse[:, j] = S[:, pi[j]]
-      for (int j = 0; j < m; ++j) W.pi[j] = j;
+      for (int j = 0; j < m; ++j) { W.pi[j] = j; W.rmap[j] = j; }
       for (int p = m - 1; p >= 0; --p) {
         int q = W.piv[p];
         int t = W.pi[p]; W.pi[p] = W.pi[q]; W.pi[q] = t;
@@ -625,45 +778,54 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
 // solve W x = v in place (v in LDS, length n)
 template <int NT>
 __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double *v, int tid) {
-  const int n = L.n, m = L.m, nch = L.nch;
-  double x0 = v[0] / W00;
-  __syncthreads();
-  for (int i = tid; i < n; i += NT) v[i] = (i == 0) ? x0 : v[i] + W.J0[i] * x0;
-  __syncthreads();
-  for (int c = tid; c < nch; c += NT) {  // bottom-up sweep
-    int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
+  const int m = L.m, nch = L.nch;
+  const double x0 = v[0] / W00;  // row 0 first; v[0] itself is overwritten only after a barrier
+  // bottom-up sweep of each chain, one lane per chain, with the -J0 x0 elimination of row 0
+  // applied on the way; the next element's operands are loaded before the current store
+  for (int c = tid; c < nch; c += NT) {
+    const int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
+    int ii = h + (len - 1) * s;
+    double x = v[ii] + W.J0[ii] * x0;
     for (int j = len - 2; j >= 0; --j) {
-      int ii = h + j * s;
-      v[ii] = v[ii] - W.tmul[ii] * v[ii + s];
+      const int in = ii - s;
+      const double vn = v[in], jn = W.J0[in], mn = W.tmul[in];
+      v[ii] = x;
+      x = (vn + jn * x0) - mn * x;
+      ii = in;
     }
+    v[ii] = x;
+    W.chx[c] = x / W.td[h];  // chain head over its pivot, used by every dense row below
   }
   __syncthreads();
+  // Schur right-hand side of the dense block (row 0 and the chain heads eliminated)
   for (int r = tid; r < m; r += NT) {
-    double acc = v[W.dmap[r]];
-    for (int c = 0; c < nch; ++c) {
-      int h = W.chead[c];
-      acc += W.Jh[c * m + r] * (v[h] / W.td[h]);
-    }
+    const int i = W.dmap[r];
+    double acc = v[i] + W.J0[i] * x0;
+    for (int c = 0; c < nch; ++c) acc += W.Jh[c * m + r] * W.chx[c];
     W.xd[W.pi[r]] = acc;
   }
   __syncthreads();
   for (int r = tid; r < m; r += NT) {
     double acc = 0.0;
-    const double *row = W.S + r * m;
+    const double *row = W.Sinv + W.rmap[r] * m;
     for (int j = 0; j < m; ++j) acc += row[j] * W.xd[j];
     v[W.dmap[r]] = acc;
   }
+  if (tid == 0) v[0] = x0;
   __syncthreads();
   for (int c = tid; c < nch; c += NT) {  // top-down sweep
-    int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
-    double xp = v[W.dmap[W.cpar[c]]];
+    const int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
+    const double xp = v[W.dmap[W.cpar[c]]];
     double x = (v[h] + W.tlo[h] * xp) / W.td[h];
-    v[h] = x;
+    int ii = h;
     for (int j = 1; j < len; ++j) {
-      int ii = h + j * s;
-      x = (v[ii] + W.tlo[ii] * x) / W.td[ii];
+      const int in = ii + s;
+      const double vn = v[in], ln = W.tlo[in], dn = W.td[in];
       v[ii] = x;
+      x = (vn + ln * x) / dn;
+      ii = in;
     }
+    v[ii] = x;
   }
   __syncthreads();
 }
@@ -1059,7 +1221,7 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
 
 // Layout of the state vector for the truncations in prob (NQ > 0: compile-time bin count) and
 // the carve-up of the dynamic LDS into the solver workspace, which must match
-// deb_perturb_lds_bytes(): 19 n + m^2 + nch m + m + 3 NQMAX doubles, then m + n + 4 nch + 2 m
+// deb_perturb_lds_bytes(): 19 n + 2 m^2 + nch m + 6 m + nch + 3 NQMAX doubles, then m + n + 4 nch + 3 m
 // ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
 template <int NQ, int NT>
 __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const double *__restrict__ nu_qw, long lds_bytes, PLay &L, Ws &W,
@@ -1075,11 +1237,12 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   double *p = smem;
   W.y0 = p; p += n; W.y1 = p; p += n; W.u = p; p += n; W.f = p; p += n; W.J0 = p; p += n; W.dT = p; p += n;
   W.K = p; p += 8 * n; W.tlo = p; p += n; W.tdi = p; p += n; W.tup = p; p += n; W.tmul = p; p += n; W.td = p; p += n;
-  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m;
+  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m; W.chx = p; p += nch;
+  W.S2 = p; p += (size_t)m * m; W.snap = p; p += 5 * (size_t)m;
   double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
-  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m;
+  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m; W.rmap = ip; ip += m;
   ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;  // 8-byte alignment for the cache
   W.sc = (SplCache *)ip; ip += sizeof(SplCache) / 4;
   if ((long)((char *)ip - (char *)smem) > lds_bytes) return false;
@@ -1318,8 +1481,8 @@ __global__ void tau_out_kernel(const double *__restrict__ tables, long tstride, 
 extern "C" long deb_perturb_lds_bytes(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
   long n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + (long)nqmax * (lmaxnu + 1) + 2;
   long m = 17 + 3 * (long)nqmax, nch = 3 + nqmax;
-  long dbl = n * 19 + m * m + nch * m + m + 3 * deb::NQMAX;
-  long ints = m + n + 4 * nch + 2 * m + 1;
+  long dbl = n * 19 + 2 * m * m + nch * m + 6 * m + nch + 3 * deb::NQMAX;
+  long ints = m + n + 4 * nch + 3 * m + 1;
   return dbl * 8 + ints * 4 + (long)sizeof(deb::SplCache);
 }
 
