@@ -221,7 +221,7 @@ def main():
                    "all_modes_ok": ok, "gradients_per_s": None},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": measured_traffic("perturb_kernel<8> 512 k-modes nq 8 lmax_nu 17 rtol 1e-5"),
+                     "traffic": measured_traffic("perturb_kernel<8, 256> 512 k-modes nq 8 lmax_nu 17 rtol 1e-5"),
                      "kernel": "perturb_kernel", "kernel_ms": kmean, "attempted_steps_per_launch": float(np.mean(steps_attempted)),
                      "flop_per_step": W},
         "cpu_baseline": None,
