@@ -83,15 +83,41 @@ __global__ void nu_bins_kernel(int nq, double *qw) {
 // ---------------------------------------------------------------------------------------
 // background and thermodynamics at (tau, a) for one mode
 // ---------------------------------------------------------------------------------------
+// value of lane src (uniform src) of every wave lane, whatever the current exec mask
+__device__ __forceinline__ double lane_bcast(double x, int src) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src), hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int N> __device__ __forceinline__ Dn<N> lane_bcast(const Dn<N> &x, int src) {
+  Dn<N> r;
+  r.v = lane_bcast(x.v, src);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = lane_bcast(x.d[i], src);
+  return r;
+}
+
 template <class T> struct PBg {
   T a, tau, aH, aHH, cs2, xe, xeprime, opac, tauc, taucprime, pb43, rho_Q, w_Q, ca2_Q;
   T amnu_a;  // a m_nu / T_nu (in units of the momentum grid)
+  T vl;      // velocity factor of bin `lane` (lanes below nq), computed once per evaluation
   bool rsa, ufa, tca;
-  // velocity factor q / eps of momentum bin q, recomputed where used: an array indexed by a
-  // run-time bin would live in scratch memory
+  // velocity factor q / eps of momentum bin q
   __device__ __forceinline__ T vq(const PLay &L, int q) const {
     T aq = amnu_a / L.q[q];
     return 1.0 / dsqrt(1.0 + aq * aq);
+  }
+  // the same, read from lane q: q uniform across the wave
+  __device__ __forceinline__ T vq_u(int q) const { return lane_bcast(vl, q); }
+  // the same for a bin that may differ between lanes (selected among the broadcasts; an array
+  // indexed by a run-time bin would live in scratch memory)
+  __device__ __forceinline__ T vq_l(const PLay &L, int q) const {
+    T r = lane_bcast(vl, 0);
+    for (int j = 1; j < L.nq; ++j) {
+      const T t = lane_bcast(vl, j);
+      if (q == j) r = t;
+    }
+    return r;
   }
 };
 
@@ -206,6 +232,8 @@ __device__ __forceinline__ void pbg_eval(const BgView &b, const PLay &L, double 
   bool uncoupled = (tc / fmax(tk, 1e-30) > b.tca_k) || ((tc / fmax(th, 1e-30) > b.tca_h) && (tc / fmax(tk, 1e-30) > 0.1 * b.tca_k));
   g.tca = !g.rsa && b.use_tca && !uncoupled;
   g.amnu_a = a * b.amnu;
+  const int lane = tid & (WAVE - 1);  // all lanes take part here: the bins are read from lanes 0..nq-1
+  g.vl = g.vq(L, lane < L.nq ? lane : 0);
 }
 
 // metric perturbations and effective (RSA/TCA) fields, perturbations.py:292-351,407-427
@@ -224,7 +252,7 @@ __device__ __forceinline__ void metric(const BgView &b, const PLay &L, double k,
   T drhonu(0.0), dpnu(0.0), fnu(0.0), shearnu(0.0);
   for (int q = 0; q < nq; ++q) {
     double p0 = y(iq0 + q), p1 = y(iq0 + nq + q), p2 = y(iq0 + 2 * nq + q);
-    const T v = g.vq(L, q);
+    const T v = g.vq_u(q);
     drhonu += L.w[q] * p0 / v;
     dpnu += L.w[q] * p0 * v;
     fnu += T(L.w[q] * p1);
@@ -315,7 +343,7 @@ __device__ __forceinline__ T core_row(int i, const BgView &b, const PLay &L, dou
   }
   if (i >= iq0 && i < iq0 + 3 * nq) {  // massive neutrinos l = 0, 1, 2
     int l = (i - iq0) / nq, q = (i - iq0) - l * nq;
-    T kv = k * g.vq(L, q);
+    T kv = k * g.vq_l(L, q);
     double dl = L.dlfdlq[q];
     if (l == 0) return -kv * y(iq0 + nq + q) + m.hprime * dl / 6.0;
     if (l == 1) return kv * (y(iq0 + q) - 2.0 * y(iq0 + 2 * nq + q)) / 3.0;
@@ -338,7 +366,7 @@ __device__ __forceinline__ void tail_coef(int ch, int l, const PLay &L, double k
   if (ch == 0) { lmax = L.lmaxg; off = g.rsa || g.tca; damp = g.opac; }
   else if (ch == 1) { lmax = L.lmaxgp; off = g.rsa || g.tca; damp = g.opac; }
   else if (ch == 2) { lmax = L.lmaxr; off = g.rsa || g.ufa; }
-  else { lmax = L.lmaxnu; off = false; kv = k * g.vq(L, ch - 3); }
+  else { lmax = L.lmaxnu; off = false; kv = k * g.vq_l(L, ch - 3); }
   if (off) { lo = T(0.0); di = T(0.0); up = T(0.0); return; }
   if (l < lmax) {
     double inv = 1.0 / (double)(2 * l + 1);
@@ -546,24 +574,33 @@ __device__ __forceinline__ double *gauss_jordan_lookahead(double *A, double *B, 
   constexpr int NE = (MC * MC + NU - 1) / NU;
   const int lane = tid & (WAVE - 1);
   double *prow0 = snap, *colp0 = snap + 2 * MC, *ipv = snap + 4 * MC;
-  bool used = false;  // wave 0, lane r: row r has been a pivot row
-  // the pivot search over column values col (lane r holds row r), rows not yet used
-  auto choose = [&](double col) {
+  // wave 0 keeps the step it last prepared in registers: lane r holds entry r of the scaled
+  // pivot row (wrow) and of the pivot column (wcol); wbi, wip are the pivot row and 1 / pivot
+  bool used = false;  // lane r: row r has been a pivot row
+  double wrow = 0.0, wcol = 0.0, wip = 0.0;
+  int wbi = 0;
+  // pivot search over column values col (lane r holds row r) among the rows not yet used;
+  // rc = 1 / col, formed alongside so the division overlaps the reduction
+  auto choose = [&](double col, double rc, double &ipn) {
     unsigned long long key = 0;
     if (lane < MC && !used) key = ((unsigned long long)__double_as_longlong(fabs(col)) & ~63ull) | (unsigned long long)(63 - lane);
     key = __ockl_wfred_max_u64(key);  // all 64 lanes of wave 0 active
-    return 63 - (int)(key & 63ull);
+    const int bn = 63 - (int)(key & 63ull);
+    ipn = lane_bcast(rc, bn);
+    return bn;
   };
   if (tid < WAVE) {  // step 0 from the matrix as built
-    const double col = (lane < MC) ? A[lane * MC] : 0.0;
-    const int bi = choose(col);
-    const double ip = 1.0 / A[bi * MC];
+    const double col = (lane < MC) ? A[lane * MC] : 1.0;
+    const int bi = choose(col, 1.0 / col, wip);
     if (lane < MC) {
-      prow0[lane] = (lane == 0) ? ip : A[bi * MC + lane] * ip;
-      colp0[lane] = col;
+      wrow = (lane == 0) ? wip : A[bi * MC + lane] * wip;
+      wcol = col;
+      prow0[lane] = wrow;
+      colp0[lane] = wcol;
     }
     used = (lane == bi);
-    if (lane == 0) { piv[0] = bi; ipv[0] = ip; }
+    wbi = bi;
+    if (lane == 0) { piv[0] = bi; ipv[0] = wip; }
   }
   __syncthreads();
   // rows and columns of this thread's entries (waves 1..), fixed across the steps, when they
@@ -581,10 +618,10 @@ __device__ __forceinline__ double *gauss_jordan_lookahead(double *A, double *B, 
   const bool last_ok = (tid - WAVE) + (NE - 1) * NU < MC * MC;  // the last entry exists
   double *cur = A, *nxt = B;
   for (int p = 0; p < MC; ++p) {
-    const double *prow = prow0 + (p & 1) * MC, *colp = colp0 + (p & 1) * MC;
-    const int bi = piv[p];
-    const double ip = ipv[p];
     if (FIXED && tid >= WAVE) {
+      const double *prow = prow0 + (p & 1) * MC, *colp = colp0 + (p & 1) * MC;
+      const int bi = piv[p];
+      const double ip = ipv[p];
       // all loads of this thread's entries first, then its stores (the buffers are not known
       // to the compiler to be distinct, so interleaving would serialise on LDS latency)
       double a[FIXED ? NE : 1], cr[FIXED ? NE : 1], pc[FIXED ? NE : 1];
@@ -604,6 +641,9 @@ __device__ __forceinline__ double *gauss_jordan_lookahead(double *A, double *B, 
         }
       }
     } else if (tid >= WAVE) {
+      const double *prow = prow0 + (p & 1) * MC, *colp = colp0 + (p & 1) * MC;
+      const int bi = piv[p];
+      const double ip = ipv[p];
       // loads of a chunk of this thread's entries first, then its stores
 #pragma unroll 1
       for (int i0 = 0; i0 < NE; i0 += GJ_CHUNK) {
@@ -628,25 +668,26 @@ __device__ __forceinline__ double *gauss_jordan_lookahead(double *A, double *B, 
           }
         }
       }
-    } else if (p + 1 < MC) {  // wave 0: step p + 1
+    } else if (p + 1 < MC) {  // wave 0: step p + 1, from the registers of step p
       double *prown = prow0 + ((p + 1) & 1) * MC, *colpn = colp0 + ((p + 1) & 1) * MC;
-      const int q = p + 1;
-      double col = 0.0;  // column q after step p
-      if (lane < MC) {
-        const double cr = colp[lane], pc = prow[q];
-        col = (lane == bi) ? pc : cur[lane * MC + q] - cr * pc;
-      }
-      const int bn = choose(col);
-      const unsigned long long cb = (unsigned long long)__double_as_longlong(col);  // broadcast lane bn's value
-      const unsigned lo = __builtin_amdgcn_readlane((unsigned)cb, bn), hi = __builtin_amdgcn_readlane((unsigned)(cb >> 32), bn);
-      const double ipn = 1.0 / __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+      const int q = p + 1, bi = wbi;
+      const double ip = wip;
+      const double pq = lane_bcast(wrow, q);  // scaled pivot row, column q
+      double col = 1.0;  // column q after step p
+      if (lane < MC) col = (lane == bi) ? pq : cur[lane * MC + q] - wcol * pq;
+      double ipn;
+      const int bn = choose(col, 1.0 / col, ipn);
+      const double cbn = lane_bcast(wcol, bn);  // column p entry of the new pivot row
       if (lane < MC) {  // pivot row bn after step p (bn != bi), scaled
-        const double cr = colp[bn], pc = prow[lane];
-        const double v = (lane == p) ? -cr * ip : cur[bn * MC + lane] - cr * pc;
-        prown[lane] = (lane == q) ? ipn : v * ipn;
-        colpn[lane] = col;
+        const double v = (lane == p) ? -cbn * ip : cur[bn * MC + lane] - cbn * wrow;
+        wrow = (lane == q) ? ipn : v * ipn;
+        wcol = col;
+        prown[lane] = wrow;
+        colpn[lane] = wcol;
       }
       used = used || (lane == bn);
+      wbi = bn;
+      wip = ipn;
       if (lane == 0) { piv[q] = bn; ipv[q] = ipn; }
     }
     __syncthreads();
