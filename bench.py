@@ -93,7 +93,7 @@ def measure_gradient(deb, nk, repeats=2):
     theta = deb.param_to_theta(P)
     k = deb.k_grid({}, 1e-4, 1.0, nk, "log")
     cot = np.random.default_rng(0).standard_normal(nk)
-    opts = dict(**LMAX, **SOLVER, aexp_out=(1.0,), schedule={"order": "k_desc"})
+    opts = dict(**LMAX, **SOLVER, aexp_out=(1.0,))
     deb.pk_and_vjp(theta, P, k, cot, **opts)  # warm-up
     torch.cuda.synchronize()
     t = time.perf_counter()

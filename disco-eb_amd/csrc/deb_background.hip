@@ -275,6 +275,11 @@ __device__ double romb_dtauda(const BgView &b, double a0, double a1) {
   return prev[divmax];
 }
 
+// The 8-node momentum grid of the neutrino background, once computed on this device (each
+// device holds its own copy; launches that compute it at once write the same values).
+__device__ double g_bg_nu[16];
+__device__ int g_bg_nu_ready;
+
 __global__ __launch_bounds__(256) void background_kernel(const double *__restrict__ params, int N, double *__restrict__ tables,
                                                          long stride) {
   const int cosmo = blockIdx.x;
@@ -304,7 +309,14 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
     t[DEB_HDR_LAMBDA_DE] = (p[DEB_P_W0] == -1.0 && p[DEB_P_WA] == 0.0) ? 1.0 : 0.0;
     t[DEB_HDR_N] = (double)N;
     t[DEB_HDR_TAUMIN] = 1e-9 / t[DEB_HDR_ADOTRAD];
-    gauss_laguerre_fd(8, s_q, s_w);
+    if (g_bg_nu_ready) {
+      for (int i = 0; i < 8; ++i) { s_q[i] = g_bg_nu[i]; s_w[i] = g_bg_nu[8 + i]; }
+    } else {
+      gauss_laguerre_fd(8, s_q, s_w);
+      for (int i = 0; i < 8; ++i) { g_bg_nu[i] = s_q[i]; g_bg_nu[8 + i] = s_w[i]; }
+      __threadfence();
+      g_bg_nu_ready = 1;
+    }
   }
   __syncthreads();
   // neutrino background on 512 log-spaced a in [0.9 amin, 1.1 amax] (background.py:158-168)

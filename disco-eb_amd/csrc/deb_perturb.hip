@@ -30,6 +30,7 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr int NQMAX = 16;
+constexpr int VQ_LANE0 = 16;  // lanes 16..16+nq-1 hold the momentum bins' velocity factors
 
 // Rodas5 tableau (transformed form), ode_integrators_stiff.py:680-745
 namespace r5 {
@@ -61,7 +62,16 @@ struct PLay {
 // neutrino momentum bins (background.py:14-45): CAMB tables for 3-5 bins, else generalized
 // Gauss-Laguerre (alpha = 1) reweighted to the Fermi-Dirac integrand; qw[0..nq) nodes,
 // qw[NQMAX..NQMAX+nq) weights
+// Nodes and weights already computed on this device, per bin count (each device holds its own
+// copy).  Two launches that compute the same count at once write the same values.
+__device__ double g_nu_cache[NQMAX + 1][2 * NQMAX];
+__device__ int g_nu_ready[NQMAX + 1];
+
 __global__ void nu_bins_kernel(int nq, double *qw) {
+  if (g_nu_ready[nq]) {  // computed by an earlier launch: copy
+    for (int i = threadIdx.x; i < 2 * NQMAX; i += blockDim.x) qw[i] = g_nu_cache[nq][i];
+    return;
+  }
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const double fdc = 5.682196976983475;
   if (nq >= 3 && nq <= 5) {
@@ -75,9 +85,15 @@ __global__ void nu_bins_kernel(int nq, double *qw) {
       qw[i] = qq[i];
       qw[NQMAX + i] = ww[i] / (-0.25 * dl) / fdc;
     }
-    return;
+  } else {
+    gauss_laguerre_fd(nq, qw, qw + NQMAX);
   }
-  gauss_laguerre_fd(nq, qw, qw + NQMAX);
+  for (int i = 0; i < nq; ++i) {
+    g_nu_cache[nq][i] = qw[i];
+    g_nu_cache[nq][NQMAX + i] = qw[NQMAX + i];
+  }
+  __threadfence();
+  g_nu_ready[nq] = 1;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -100,21 +116,21 @@ template <int N> __device__ __forceinline__ Dn<N> lane_bcast(const Dn<N> &x, int
 template <class T> struct PBg {
   T a, tau, aH, aHH, cs2, xe, xeprime, opac, tauc, taucprime, pb43, rho_Q, w_Q, ca2_Q;
   T amnu_a;  // a m_nu / T_nu (in units of the momentum grid)
-  T vl;      // velocity factor of bin `lane` (lanes below nq), computed once per evaluation
+  T vl;      // lane VQ_LANE0 + q: velocity factor of bin q (q < nq), computed once per evaluation
   bool rsa, ufa, tca;
   // velocity factor q / eps of momentum bin q
   __device__ __forceinline__ T vq(const PLay &L, int q) const {
     T aq = amnu_a / L.q[q];
     return 1.0 / dsqrt(1.0 + aq * aq);
   }
-  // the same, read from lane q: q uniform across the wave
-  __device__ __forceinline__ T vq_u(int q) const { return lane_bcast(vl, q); }
+  // the same, read from lane VQ_LANE0 + q: q uniform across the wave
+  __device__ __forceinline__ T vq_u(int q) const { return lane_bcast(vl, VQ_LANE0 + q); }
   // the same for a bin that may differ between lanes (selected among the broadcasts; an array
   // indexed by a run-time bin would live in scratch memory)
   __device__ __forceinline__ T vq_l(const PLay &L, int q) const {
-    T r = lane_bcast(vl, 0);
+    T r = lane_bcast(vl, VQ_LANE0);
     for (int j = 1; j < L.nq; ++j) {
-      const T t = lane_bcast(vl, j);
+      const T t = lane_bcast(vl, VQ_LANE0 + j);
       if (q == j) r = t;
     }
     return r;
@@ -232,8 +248,109 @@ __device__ __forceinline__ void pbg_eval(const BgView &b, const PLay &L, double 
   bool uncoupled = (tc / fmax(tk, 1e-30) > b.tca_k) || ((tc / fmax(th, 1e-30) > b.tca_h) && (tc / fmax(tk, 1e-30) > 0.1 * b.tca_k));
   g.tca = !g.rsa && b.use_tca && !uncoupled;
   g.amnu_a = a * b.amnu;
-  const int lane = tid & (WAVE - 1);  // all lanes take part here: the bins are read from lanes 0..nq-1
-  g.vl = g.vq(L, lane < L.nq ? lane : 0);
+  const int lane = tid & (WAVE - 1);  // all lanes take part here: the bins are read from lanes VQ_LANE0..
+  const int qb = lane - VQ_LANE0;
+  g.vl = g.vq(L, (qb >= 0 && qb < L.nq) ? qb : 0);
+}
+
+// pbg_eval for plain doubles with the independent transcendentals and divisions spread over
+// lanes: one exp serves rho_nu, p_nu (lanes 0, 1) and the dark-energy density (lane 2, as
+// exp(-3 (1 + w0 + wa) ln a + 3 (a - 1) wa)); then five rounds of one division per lane and
+// one square root, each result broadcast with readlane.  Every other quantity is the same
+// expression as in pbg_eval.  All 64 lanes of every wave must be active.
+__device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, double k, double tau, double a, PBg<double> &g,
+                                               SplCache *sc, int tid) {
+  const int lane = tid & (WAVE - 1);
+  const int qb = lane - VQ_LANE0;
+  const bool binlane = qb >= 0 && qb < L.nq;
+  g.a = a;
+  g.tau = tau;
+  const double loga = log(a);
+  SplCoef c;
+  cache_lookup(b, sc, loga, tid, c);
+  const double cs2c = cubic_at(c.x[0], c.x[1], c.yc[0], c.yc[1], c.Sc[0], c.Sc[1], loga);
+  g.xe = cubic_at(c.x[0], c.x[1], c.ye[0], c.ye[1], c.Se[0], c.Se[1], loga);
+  const double dxe = cubic_deriv_at(c.x[0], c.x[1], c.ye[0], c.ye[1], c.Se[0], c.Se[1], loga);
+  const double a2 = a * a;
+  g.w_Q = b.w0 + b.wa * (1.0 - a);
+  g.amnu_a = a * b.amnu;
+  // one exp: lanes 0, 1 the neutrino density and pressure splines, lane 2 the dark energy
+  const bool lp = lane == 1;
+  double x = cubic_at(c.nx[0], c.nx[1], lp ? c.np[0] : c.nr[0], lp ? c.np[1] : c.nr[1], lp ? c.nSp[0] : c.nSr[0],
+                      lp ? c.nSp[1] : c.nSr[1], loga);
+  if (lane == 2) x = -3.0 * (1.0 + b.w0 + b.wa) * loga + 3.0 * (a - 1.0) * b.wa;
+  const double ex = exp(x);
+  const double rhonu = lane_bcast(ex, 0), pnu = lane_bcast(ex, 1);
+  g.rho_Q = lane_bcast(ex, 2);
+  // divisions, round 1
+  double num = 1.0, den = 1.0;
+  switch (lane) {
+    case 0: num = cs2c; den = a; break;
+    case 1: num = b.grhom * b.Omegam; den = a; break;
+    case 2: num = b.grhog + b.grhor * (b.Neff + b.Nmnu * rhonu); den = a2; break;
+    case 3: num = b.grhog; den = b.grhom * b.Omegab * a; break;
+    case 4: num = b.grhog + b.grhor * b.Neff; den = 3.0; break;
+    case 5: num = g.xe * b.akthom; den = a2; break;
+    case 6: num = 1.0; den = fmax(k, 1e-30); break;
+    default: if (binlane) { num = g.amnu_a; den = L.q[qb]; } break;
+  }
+  double r = num / den;
+  g.cs2 = lane_bcast(r, 0);
+  const double t1 = lane_bcast(r, 1), t2 = lane_bcast(r, 2);
+  g.pb43 = 4.0 / 3.0 * lane_bcast(r, 3);
+  const double gin = lane_bcast(r, 4);
+  g.opac = lane_bcast(r, 5);
+  const double tk = lane_bcast(r, 6);
+  const double aq = r;  // bin lanes
+  const double grho = t1 + t2 + b.grhom * b.OmegaDE * g.rho_Q * a2 + b.grhom * b.Omegak;
+  // round 2
+  num = 1.0; den = 1.0;
+  switch (lane) {
+    case 0: num = grho; den = 3.0; break;
+    case 1: num = gin + b.grhor * b.Nmnu * pnu; den = a2; break;
+    case 2: num = 1.0; den = fmax(g.opac, 1e-30); break;
+    default: if (binlane) num = 1.0 + aq * aq; break;
+  }
+  r = num / den;
+  const double gp1 = lane_bcast(r, 1);
+  g.tauc = lane_bcast(r, 2);
+  const double sq = sqrt(r);  // lane 0: aH; bin lanes: sqrt(1 + aq^2)
+  g.aH = lane_bcast(sq, 0);
+  g.xeprime = dxe * g.aH;
+  const double gpres = gp1 + g.w_Q * b.grhom * b.OmegaDE * g.rho_Q * a2;
+  g.aHH = 0.5 * (g.aH * g.aH - gpres);
+  const double wprime = -b.wa * g.aH * a;
+  const double tv = tau, tc = g.tauc;
+  // round 3
+  num = 1.0; den = 1.0;
+  switch (lane) {
+    case 0: num = wprime; den = 3.0; break;
+    case 1: num = g.xeprime; den = fmax(g.xe, 1e-30); break;
+    case 2: num = 1.0; den = fmax(g.aH, 1e-30); break;
+    case 3: num = tc; den = fmax(tv, 1e-30); break;
+    case 4: num = tc; den = fmax(tk, 1e-30); break;
+    default: if (binlane) den = sq; break;
+  }
+  r = num / den;
+  const double w3 = lane_bcast(r, 0);
+  g.taucprime = g.tauc * (2.0 * g.aH - lane_bcast(r, 1));
+  const double th = lane_bcast(r, 2), r_tv = lane_bcast(r, 3), r_tk = lane_bcast(r, 4);
+  g.vl = r;  // bin lanes: 1 / sqrt(1 + aq^2)
+  // round 4
+  num = 1.0; den = 1.0;
+  switch (lane) {
+    case 0: num = w3; den = (1.0 + g.w_Q) + 1e-6; break;
+    case 1: num = tc; den = fmax(th, 1e-30); break;
+    default: break;
+  }
+  r = num / den;
+  const double w4 = lane_bcast(r, 0), r_th = lane_bcast(r, 1);
+  g.ca2_Q = g.w_Q - w4 / g.aH;  // round 5: one division, every lane
+  // approximation switches (approximations.py:50-75)
+  g.rsa = b.use_rsa && (r_tv > b.rsa_c) && (tv * k > b.rsa_k);
+  g.ufa = !g.rsa && b.use_ufa && (tv * k > b.ufa_k);
+  bool uncoupled = (r_tk > b.tca_k) || ((r_th > b.tca_h) && (r_tk > 0.1 * b.tca_k));
+  g.tca = !g.rsa && b.use_tca && !uncoupled;
 }
 
 // metric perturbations and effective (RSA/TCA) fields, perturbations.py:292-351,407-427
@@ -423,7 +540,7 @@ __device__ __forceinline__ void setup_maps(const PLay &L, Ws &W, int tid) {
 template <int NT>
 __device__ __forceinline__ void rhs_eval(const BgView &b, const PLay &L, double k, double tau, const double *u, double *out, Ws &W, int tid) {
   PBg<double> g;
-  pbg_eval<double>(b, L, k, tau, u[0], g, W.sc, tid);
+  pbg_eval_lanes(b, L, k, tau, u[0], g, W.sc, tid);
   auto yv = [&](int i) { return u[i]; };
   Met<double> m;
   metric<double>(b, L, k, g, yv, m);
@@ -708,7 +825,7 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
   const double gi = 1.0 / (dt * r5::gamma);
   rhs_dual<NT>(b, L, k, t0, W.y0, W.J0, W.dT, W, tid);
   PBg<double> g;
-  pbg_eval<double>(b, L, k, t0, W.y0[0], g, W.sc, tid);
+  pbg_eval_lanes(b, L, k, t0, W.y0[0], g, W.sc, tid);
   // dense-block columns 0..m-1 and chain-head columns m..m+nch-1, each split into NR row groups
   constexpr int NR = NT / WAVE;
   for (int it = tid; it < (m + nch) * NR; it += NT) {
@@ -1528,6 +1645,7 @@ extern "C" long deb_perturb_lds_bytes(int lmaxg, int lmaxgp, int lmaxr, int lmax
 }
 
 extern "C" int deb_launch_nu_bins(int nqmax, double *d_qw, hipStream_t stream) {
+  if (nqmax < 1 || nqmax > deb::NQMAX) return -1;
   hipLaunchKernelGGL(deb::nu_bins_kernel, dim3(1), dim3(64), 0, stream, nqmax, d_qw);
   return (int)hipGetLastError();
 }

@@ -65,6 +65,11 @@ def _accepted_mesh(log, nacc, t1):
     return torch.tensor(mesh), torch.tensor(na, dtype=torch.int32)
 
 
+# Launch of the replayed shifted cosmologies (twelve times the k-modes): largest k first, two
+# wavefronts per unit (the fastest width for a few thousand units on one MI355X).
+REPLAY_SCHEDULE = {"order": "k_desc", "threads": 128}
+
+
 def pk_jacobian(theta, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, num_thermo=256, schedule=None, lmaxg=12, lmaxgp=12, lmaxr=12,
                 lmaxnu=17, nqmax=8, **solver):
     """P(k) at aexp_out and its Jacobian with respect to theta.
@@ -81,7 +86,7 @@ def pk_jacobian(theta, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, 
     t0 = background_tables([p0], num_thermo)
     tau0 = tau_out_all(t0, num_thermo, aexp)
     uc0 = torch.zeros(nk, dtype=torch.int32, device=dev)
-    y, st, na, nr, log = evolve_units(t0, num_thermo, uc0, kk, tau0, **lm, **solver, step_log=True)
+    y, st, na, nr, log = evolve_units(t0, num_thermo, uc0, kk, tau0, **lm, **solver, step_log=True, schedule={"order": "k_desc"})
     _raise_on_status(st)
     pk = power_units(t0, num_thermo, uc0, kk, y, idx)
     t1 = np.full(nk, float(tau0.max().cpu()))
@@ -99,7 +104,7 @@ def pk_jacobian(theta, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, 
     tau = tau_out_all(tab, num_thermo, aexp)
     uc, uk, _ = units_for([kk.cpu().numpy()] * len(params), dev)
     reps = len(params)
-    yr, str_, _, _ = evolve_units(tab, num_thermo, uc, uk, tau, **lm, **solver, schedule=schedule,
+    yr, str_, _, _ = evolve_units(tab, num_thermo, uc, uk, tau, **lm, **solver, schedule=REPLAY_SCHEDULE if schedule is None else schedule,
                                   replay=(mesh.repeat(reps, 1), mlen.repeat(reps)))
     _raise_on_status(str_)
     pkr = power_units(tab, num_thermo, uc, uk, yr, idx).reshape(reps, nk, -1)
