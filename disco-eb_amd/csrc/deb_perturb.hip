@@ -1288,11 +1288,13 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   const int cosmo = unit_cosmo[unit];
   const double k = unit_k[unit];
   const int n = L.n;
-  BgView b;
-  b.init(tables + (long)cosmo * tstride);
+  // the cosmology's background view lives in LDS: its ~100 scalar registers' worth of fields
+  // would not fit beside the solver's other uniform values, so each use reads it back
+  __shared__ BgView s_bg;
   __syncthreads();  // the previous unit's last LDS reads are done
-  if (tid == 0) { W.sc->i = -1; W.sc->j = -1; }
+  if (tid == 0) { W.sc->i = -1; W.sc->j = -1; s_bg.init(tables + (long)cosmo * tstride); }
   __syncthreads();
+  const BgView &b = s_bg;
   const int wout = prob.return_full ? n : 20;
   double *yo = y_out + (long)unit * nout * wout;
   // w(a) = w0 + wa (1 - a) reaching -1 for some a in (0, 1] (the phantom divide) makes the
@@ -1467,10 +1469,10 @@ __global__ __launch_bounds__(NT) void perturb_batched_kernel(const double *__res
   }
   const int n = L.n;
   const int cosmo = unit_cosmo[u0];
-  BgView b;
-  b.init(tables + (long)cosmo * tstride);
-  if (tid == 0) { W.sc->i = -1; W.sc->j = -1; }
+  __shared__ BgView s_bg;  // in LDS, as in solve_unit
+  if (tid == 0) { W.sc->i = -1; W.sc->j = -1; s_bg.init(tables + (long)cosmo * tstride); }
   __syncthreads();
+  const BgView &b = s_bg;
   const int wout = prob.return_full ? n : 20;
   double *yo = y_out + (long)u0 * nout * wout;  // unit j's outputs at yo + j nout wout
   int mixed = 0;
