@@ -31,6 +31,8 @@ namespace {
 constexpr int WAVE = 64;
 constexpr int NQMAX = 16;
 constexpr int VQ_LANE0 = 16;  // lanes 16..16+nq-1 hold the momentum bins' velocity factors
+constexpr int NCORE = 17;     // dense-block unknowns outside the massive-neutrino l = 0, 1, 2 rows
+constexpr int NRED = NCORE + 2;  // reduced system: those plus h' and eta'
 
 // Rodas5 tableau (transformed form), ode_integrators_stiff.py:680-745
 namespace r5 {
@@ -420,8 +422,9 @@ __device__ __forceinline__ void metric(const BgView &b, const PLay &L, double k,
 }
 
 // RHS row i for i outside the l >= 3 tails (row 0 included)
-template <class T, class Y>
-__device__ __forceinline__ T core_row(int i, const BgView &b, const PLay &L, double k, const PBg<T> &g, const Met<T> &m, const Y &y) {
+template <class T, class Y, class V>
+__device__ __forceinline__ T core_row(int i, const BgView &b, const PLay &L, double k, const PBg<T> &g, const Met<T> &m, const Y &y,
+                                      const V &vrow) {
   const int n = L.n, nq = L.nq, iq0 = L.iq0, idxgp = L.idxgp, idxr = L.idxr;
   const double k2 = k * k;
   if (i == 0) return g.aH * g.a;
@@ -460,7 +463,7 @@ __device__ __forceinline__ T core_row(int i, const BgView &b, const PLay &L, dou
   }
   if (i >= iq0 && i < iq0 + 3 * nq) {  // massive neutrinos l = 0, 1, 2
     int l = (i - iq0) / nq, q = (i - iq0) - l * nq;
-    T kv = k * g.vq_l(L, q);
+    T kv = k * vrow(q);
     double dl = L.dlfdlq[q];
     if (l == 0) return -kv * y(iq0 + nq + q) + m.hprime * dl / 6.0;
     if (l == 1) return kv * (y(iq0 + q) - 2.0 * y(iq0 + 2 * nq + q)) / 3.0;
@@ -475,20 +478,21 @@ __device__ __forceinline__ T core_row(int i, const BgView &b, const PLay &L, dou
 }
 
 // tail chain coefficients: f_l = lo y_{l-1} + di y_l + up y_{l+1}, l = 3..lmax
-template <class T>
-__device__ __forceinline__ void tail_coef(int ch, int l, const PLay &L, double k, const PBg<T> &g, T &lo, T &di, T &up) {
+// ca, cb: l / (2l + 1) and -(l + 1) / (2l + 1); vrow(q): velocity factor of momentum bin q
+template <class T, class V>
+__device__ __forceinline__ void tail_coef(int ch, int l, const PLay &L, double k, const PBg<T> &g, double ca, double cb, const V &vrow,
+                                          T &lo, T &di, T &up) {
   int lmax;
   bool off;
   T kv(k), damp(0.0);
   if (ch == 0) { lmax = L.lmaxg; off = g.rsa || g.tca; damp = g.opac; }
   else if (ch == 1) { lmax = L.lmaxgp; off = g.rsa || g.tca; damp = g.opac; }
   else if (ch == 2) { lmax = L.lmaxr; off = g.rsa || g.ufa; }
-  else { lmax = L.lmaxnu; off = false; kv = k * g.vq_l(L, ch - 3); }
+  else { lmax = L.lmaxnu; off = false; kv = k * vrow(ch - 3); }
   if (off) { lo = T(0.0); di = T(0.0); up = T(0.0); return; }
   if (l < lmax) {
-    double inv = 1.0 / (double)(2 * l + 1);
-    lo = kv * ((double)l * inv);
-    up = kv * (-(double)(l + 1) * inv);
+    lo = kv * ca;
+    up = kv * cb;
     di = -damp;
   } else {
     lo = kv;
@@ -505,6 +509,12 @@ struct Ws {
   double *y0, *y1, *u, *f, *J0, *dT, *K, *S, *Jh, *tlo, *tdi, *tup, *tmul, *td, *xd, *chx;
   double *S2, *snap, *Sinv;  // second matrix buffer, pivot snapshots, the buffer holding the inverse
   int *dmap, *rk, *chead, *cstride, *clen, *cpar, *piv, *pi, *rmap;
+  int *rs, *rbin;       // per row: chain stride (tail rows), momentum bin (massive-neutrino rows)
+  double *tca, *tcb;    // per tail row l < lmax: l / (2l + 1) and -(l + 1) / (2l + 1)
+  // bordered solve of the dense block (fixed block sizes): metric derivatives G (2 per dense
+  // column), per-bin 3x3 inverses, Q = M^-1 U, the neutrino part of a right-hand side, the
+  // reduced right-hand side and solution, the reduced matrix in two buffers
+  double *G, *Minv, *Q, *rn, *xr, *R, *R2;
   SplCache *sc;
 };
 
@@ -529,10 +539,21 @@ __device__ __forceinline__ void setup_maps(const PLay &L, Ws &W, int tid) {
       W.chead[3 + q] = L.iq0 + 3 * nq + q; W.cstride[3 + q] = nq; W.clen[3 + q] = L.lmaxnu - 2; W.cpar[3 + q] = 15 + 2 * nq + q;
     }
   }
-  for (int i = tid; i < n; i += NT) W.rk[i] = -1;
+  for (int i = tid; i < n; i += NT) {
+    W.rk[i] = -1; W.rs[i] = 0; W.tca[i] = 0.0; W.tcb[i] = 0.0;
+    W.rbin[i] = (i >= L.iq0 && i < L.iq0 + 3 * nq) ? (i - L.iq0) % nq : 0;
+  }
   __syncthreads();
   for (int c = 0; c < L.nch; ++c)
-    for (int j = tid; j < W.clen[c]; j += NT) W.rk[W.chead[c] + j * W.cstride[c]] = (c << 8) | (3 + j);
+    for (int j = tid; j < W.clen[c]; j += NT) {
+      const int i = W.chead[c] + j * W.cstride[c], l = 3 + j;
+      const double inv = 1.0 / (double)(2 * l + 1);
+      W.rk[i] = (c << 8) | l;
+      W.rs[i] = W.cstride[c];
+      W.rbin[i] = c >= 3 ? c - 3 : 0;
+      W.tca[i] = (double)l * inv;
+      W.tcb[i] = -(double)(l + 1) * inv;
+    }
   __syncthreads();
 }
 
@@ -544,20 +565,26 @@ __device__ __forceinline__ void rhs_eval(const BgView &b, const PLay &L, double 
   auto yv = [&](int i) { return u[i]; };
   Met<double> m;
   metric<double>(b, L, k, g, yv, m);
-  for (int i = tid; i < L.n; i += NT) {
-    int rk = W.rk[i];
-    double r;
-    if (rk < 0) {
-      r = core_row<double>(i, b, L, k, g, m, yv);
-    } else {
-      int ch = rk >> 8, l = rk & 255;
-      double lo, di, up;
-      tail_coef<double>(ch, l, L, k, g, lo, di, up);
-      int s = W.cstride[ch];
-      double yn = (up != 0.0) ? u[i + s] : 0.0;
-      r = lo * u[i - s] + di * u[i] + up * yn;
+  const int nit = (L.n + NT - 1) / NT;
+  for (int it = 0; it < nit; ++it) {  // a uniform trip count: every lane takes part in the shuffle
+    const int i = tid + it * NT;
+    const int ic = i < L.n ? i : 0;
+    const double vsh = __shfl(g.vl, VQ_LANE0 + W.rbin[ic]);  // this row's momentum-bin factor
+    auto vrow = [&](int) { return vsh; };
+    if (i < L.n) {
+      const int rk = W.rk[i];
+      double r;
+      if (rk < 0) {
+        r = core_row<double>(i, b, L, k, g, m, yv, vrow);
+      } else {
+        const int ch = rk >> 8, l = rk & 255, s = W.rs[i];
+        double lo, di, up;
+        tail_coef<double>(ch, l, L, k, g, W.tca[i], W.tcb[i], vrow, lo, di, up);
+        const double yn = (up != 0.0) ? u[i + s] : 0.0;
+        r = lo * u[i - s] + di * u[i] + up * yn;
+      }
+      out[i] = r;
     }
-    out[i] = r;
   }
 }
 
@@ -574,21 +601,31 @@ __device__ __forceinline__ void rhs_dual(const BgView &b, const PLay &L, double 
   auto yv = [&](int i) { return y[i]; };
   Met<D2> m;
   metric<D2>(b, L, k, g, yv, m);
-  for (int i = tid; i < L.n; i += NT) {
-    int rk = W.rk[i];
-    D2 r;
-    if (rk < 0) {
-      r = core_row<D2>(i, b, L, k, g, m, yv);
-    } else {
-      int ch = rk >> 8, l = rk & 255;
-      D2 lo, di, up;
-      tail_coef<D2>(ch, l, L, k, g, lo, di, up);
-      int s = W.cstride[ch];
-      double yn = (up.v != 0.0 || up.d[0] != 0.0 || up.d[1] != 0.0) ? y[i + s] : 0.0;
-      r = lo * y[i - s] + di * y[i] + up * yn;
+  const int nit = (L.n + NT - 1) / NT;
+  for (int it = 0; it < nit; ++it) {  // a uniform trip count: every lane takes part in the shuffle
+    const int i = tid + it * NT;
+    const int ic = i < L.n ? i : 0;
+    const int src = VQ_LANE0 + W.rbin[ic];
+    D2 vsh;
+    vsh.v = __shfl(g.vl.v, src);
+    vsh.d[0] = __shfl(g.vl.d[0], src);
+    vsh.d[1] = __shfl(g.vl.d[1], src);
+    auto vrow = [&](int) { return vsh; };
+    if (i < L.n) {
+      const int rk = W.rk[i];
+      D2 r;
+      if (rk < 0) {
+        r = core_row<D2>(i, b, L, k, g, m, yv, vrow);
+      } else {
+        const int ch = rk >> 8, l = rk & 255, s = W.rs[i];
+        D2 lo, di, up;
+        tail_coef<D2>(ch, l, L, k, g, W.tca[i], W.tcb[i], vrow, lo, di, up);
+        const double yn = (up.v != 0.0 || up.d[0] != 0.0 || up.d[1] != 0.0) ? y[i + s] : 0.0;
+        r = lo * y[i - s] + di * y[i] + up * yn;
+      }
+      outa[i] = r.d[0];
+      outt[i] = r.d[1];
     }
-    outa[i] = r.d[0];
-    outt[i] = r.d[1];
   }
 }
 
@@ -818,6 +855,98 @@ __device__ __forceinline__ double *gauss_jordan_lookahead(double *A, double *B, 
   return cur;
 }
 
+// Inverse of a 3x3 matrix by Gauss-Jordan with partial pivoting, in registers.
+__device__ __forceinline__ void inv3(double (&a)[3][6]) {  // a = [M | I] in, [I | M^-1] out
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+#pragma unroll
+    for (int i = p + 1; i < 3; ++i) {
+      if (fabs(a[i][p]) > fabs(a[p][p])) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) { const double t = a[i][j]; a[i][j] = a[p][j]; a[p][j] = t; }
+      }
+    }
+    const double ip = 1.0 / a[p][p];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) a[p][j] *= ip;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (i == p) continue;
+      const double f = a[i][p];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) a[i][j] -= f * a[p][j];
+    }
+  }
+}
+
+// dense-block index of core unknown i (i < NCORE): rows 1..9, polarisation and massless
+// neutrino l = 0..2, then the two dark-energy rows after the 3 nq massive-neutrino rows
+__device__ __forceinline__ int core_index(int i, int nq) { return i < 15 ? i : 3 * nq + i; }
+
+// Bordered reduction of the dense block W (m = 17 + 3 nq unknowns).  The massive-neutrino rows
+// l = 0, 1, 2 of bin q depend on other unknowns only through h' and eta' (linear functionals
+// of the state, G^T x): with U the rows' coefficients of (h', eta'),
+//   W_nn = M - U G_n^T (M block-diagonal, one 3x3 block per bin),  W_nc = -U G_c^T.
+// Writing z = G^T x, the neutrino unknowns are x_n = M^-1 b_n + Q z (Q = M^-1 U), and
+//   [ W_cc          W_cn Q      ] [x_c]   [ b_c - W_cn M^-1 b_n ]
+//   [ -G_c^T   I - G_n^T Q      ] [ z ] = [ G_n^T M^-1 b_n      ]
+// is a 19 x 19 system for every nq.  W (as built), G (from the metric on the unit vectors
+// that give W's columns) are numeric; U is (dl/6, 0, -dl/15) for h' and (0, 0, -2 dl/5) for
+// eta' in the l = 0, 1, 2 rows (the reference's massive-neutrino equations).
+template <int MC, int NT>
+__device__ __forceinline__ void reduce_dense(const PLay &L, Ws &W, int tid) {
+  constexpr int NQ = (MC - NCORE) / 3;
+  constexpr int NN = 3 * NQ;
+  const int m = MC;
+  for (int q = tid; q < NQ; q += NT) {  // M_q = W_qq + U_q G_q^T, inverted
+    const double dl = L.dlfdlq[q];
+    const double uh[3] = {dl / 6.0, 0.0, -dl / 15.0}, ue[3] = {0.0, 0.0, -0.4 * dl};
+    double a[3][6];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      const int row = 15 + l * NQ + q;
+#pragma unroll
+      for (int lp = 0; lp < 3; ++lp) {
+        const int col = 15 + lp * NQ + q;
+        a[l][lp] = W.S[row * m + col] + uh[l] * W.G[2 * col] + ue[l] * W.G[2 * col + 1];
+        a[l][3 + lp] = (l == lp) ? 1.0 : 0.0;
+      }
+    }
+    inv3(a);
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int lp = 0; lp < 3; ++lp) W.Minv[9 * q + 3 * l + lp] = a[l][3 + lp];
+  }
+  __syncthreads();
+  for (int j = tid; j < NN; j += NT) {  // Q = M^-1 U
+    const int l = j / NQ, q = j - l * NQ;
+    const double dl = L.dlfdlq[q];
+    const double *mi = W.Minv + 9 * q + 3 * l;
+    W.Q[2 * j] = mi[0] * (dl / 6.0) + mi[2] * (-dl / 15.0);
+    W.Q[2 * j + 1] = mi[2] * (-0.4 * dl);
+  }
+  __syncthreads();
+  for (int e = tid; e < NRED * NRED; e += NT) {
+    const int i = e / NRED, j = e - i * NRED;
+    double v;
+    if (i < NCORE && j < NCORE) {
+      v = W.S[core_index(i, NQ) * m + core_index(j, NQ)];
+    } else if (i < NCORE) {  // W_cn Q
+      const double *row = W.S + core_index(i, NQ) * m + 15;
+      v = 0.0;
+      for (int jn = 0; jn < NN; ++jn) v += row[jn] * W.Q[2 * jn + (j - NCORE)];
+    } else if (j < NCORE) {
+      v = -W.G[2 * core_index(j, NQ) + (i - NCORE)];
+    } else {  // I - G_n^T Q
+      v = (i == j) ? 1.0 : 0.0;
+      for (int jn = 0; jn < NN; ++jn) v -= W.G[2 * (15 + jn) + (i - NCORE)] * W.Q[2 * jn + (j - NCORE)];
+    }
+    W.R[e] = v;
+  }
+  __syncthreads();
+}
+
 // Jacobian pieces at (t0, y0) and the structured factorisation of W = I/(gamma dt) - J
 template <int MC, int NT>
 __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double k, double t0, double dt, Ws &W, int tid, double &W00) {
@@ -834,10 +963,15 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
     auto ev = [&](int i) { return i == s ? 1.0 : 0.0; };
     Met<double> mt;
     metric<double>(b, L, k, g, ev, mt);
+    auto vsel = [&](int q) { return g.vq_l(L, q); };
     for (int r = rg; r < m; r += NR) {
-      double v = core_row<double>(W.dmap[r], b, L, k, g, mt, ev);
+      double v = core_row<double>(W.dmap[r], b, L, k, g, mt, ev, vsel);
       if (col < m) W.S[r * m + col] = ((r == col) ? gi : 0.0) - v;  // W_DD = I/(gamma h) - J_DD
       else W.Jh[(col - m) * m + r] = v;
+    }
+    if (MC > 0 && col < m && rg == 0) {  // h' and eta' per unit of this unknown
+      W.G[2 * col] = mt.hprime;
+      W.G[2 * col + 1] = mt.etaprime;
     }
   }
   // tail coefficients
@@ -845,7 +979,8 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
     int rk = W.rk[i];
     if (rk >= 0) {
       double lo, di, up;
-      tail_coef<double>(rk >> 8, rk & 255, L, k, g, lo, di, up);
+      auto vsel = [&](int q) { return g.vq_l(L, q); };
+      tail_coef<double>(rk >> 8, rk & 255, L, k, g, W.tca[i], W.tcb[i], vsel, lo, di, up);
       W.tlo[i] = lo; W.tdi[i] = di; W.tup[i] = up;
     }
   }
@@ -877,11 +1012,14 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
   // moves row p to the pivot row's slot and snapshots the scaled pivot row and column p, then
   // one element-parallel pass updates all m^2 entries.  Row interchanges leave the inverse with
   // permuted columns; the composite permutation pi is applied to the right-hand side in wsolve.
-  if constexpr (MC > 0 && NT > WAVE) {
-    W.Sinv = gauss_jordan_lookahead<MC, NT>(W.S, W.S2, W.piv, W.pi, W.rmap, W.snap, tid);
-  } else if constexpr (MC > 0) {
-    gauss_jordan_fixed<MC, NT>(W.S, W.piv, W.pi, W.rmap, W.u, W.f, tid);
-    W.Sinv = W.S;
+  if constexpr (MC > 0) {
+    reduce_dense<MC, NT>(L, W, tid);
+    if constexpr (NT > WAVE) {
+      W.Sinv = gauss_jordan_lookahead<NRED, NT>(W.R, W.R2, W.piv, W.pi, W.rmap, W.snap, tid);
+    } else {
+      gauss_jordan_fixed<NRED, NT>(W.R, W.piv, W.pi, W.rmap, W.u, W.f, tid);
+      W.Sinv = W.R;
+    }
   } else {
     W.Sinv = W.S;
     double *prow = W.u, *colp = W.f;  // free during the factorisation
@@ -934,7 +1072,7 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
 }
 
 // solve W x = v in place (v in LDS, length n)
-template <int NT>
+template <int MC, int NT>
 __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double *v, int tid) {
   const int m = L.m, nch = L.nch;
   const double x0 = v[0] / W00;  // row 0 first; v[0] itself is overwritten only after a barrier
@@ -955,22 +1093,67 @@ __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double 
     W.chx[c] = x / W.td[h];  // chain head over its pivot, used by every dense row below
   }
   __syncthreads();
-  // Schur right-hand side of the dense block (row 0 and the chain heads eliminated)
-  for (int r = tid; r < m; r += NT) {
-    const int i = W.dmap[r];
-    double acc = v[i] + W.J0[i] * x0;
-    for (int c = 0; c < nch; ++c) acc += W.Jh[c * m + r] * W.chx[c];
-    W.xd[W.pi[r]] = acc;
+  if constexpr (MC > 0) {
+    // bordered solve (see reduce_dense): dense right-hand side, neutrino blocks, the reduced
+    // 19 x 19 system, then the neutrino unknowns from h' and eta'
+    constexpr int NQ = (MC - NCORE) / 3;
+    constexpr int NN = 3 * NQ;
+    for (int r = tid; r < MC; r += NT) {
+      const int i = W.dmap[r];
+      double acc = v[i] + W.J0[i] * x0;
+      for (int c = 0; c < nch; ++c) acc += W.Jh[c * MC + r] * W.chx[c];
+      W.xd[r] = acc;
+    }
+    __syncthreads();
+    for (int j = tid; j < NN; j += NT) {  // M^-1 b_n
+      const int l = j / NQ, q = j - l * NQ;
+      const double *mi = W.Minv + 9 * q + 3 * l;
+      W.rn[j] = mi[0] * W.xd[15 + q] + mi[1] * W.xd[15 + NQ + q] + mi[2] * W.xd[15 + 2 * NQ + q];
+    }
+    __syncthreads();
+    for (int i = tid; i < NRED; i += NT) {  // reduced right-hand side, through the pivot permutation
+      double acc;
+      if (i < NCORE) {
+        const int ci = core_index(i, NQ);
+        const double *row = W.S + ci * MC + 15;
+        acc = W.xd[ci];
+        for (int jn = 0; jn < NN; ++jn) acc -= row[jn] * W.rn[jn];
+      } else {
+        acc = 0.0;
+        for (int jn = 0; jn < NN; ++jn) acc += W.G[2 * (15 + jn) + (i - NCORE)] * W.rn[jn];
+      }
+      W.xr[W.pi[i]] = acc;
+    }
+    __syncthreads();
+    for (int i = tid; i < NRED; i += NT) {
+      const double *row = W.Sinv + W.rmap[i] * NRED;
+      double acc = 0.0;
+      for (int j = 0; j < NRED; ++j) acc += row[j] * W.xr[j];
+      if (i < NCORE) v[W.dmap[core_index(i, NQ)]] = acc;
+      else W.xr[NRED + i - NCORE] = acc;  // h', eta'
+    }
+    if (tid == 0) v[0] = x0;
+    __syncthreads();
+    for (int j = tid; j < NN; j += NT) v[W.dmap[15 + j]] = W.rn[j] + W.Q[2 * j] * W.xr[NRED] + W.Q[2 * j + 1] * W.xr[NRED + 1];
+    __syncthreads();
+  } else {
+    // Schur right-hand side of the dense block (row 0 and the chain heads eliminated)
+    for (int r = tid; r < m; r += NT) {
+      const int i = W.dmap[r];
+      double acc = v[i] + W.J0[i] * x0;
+      for (int c = 0; c < nch; ++c) acc += W.Jh[c * m + r] * W.chx[c];
+      W.xd[W.pi[r]] = acc;
+    }
+    __syncthreads();
+    for (int r = tid; r < m; r += NT) {
+      double acc = 0.0;
+      const double *row = W.Sinv + W.rmap[r] * m;
+      for (int j = 0; j < m; ++j) acc += row[j] * W.xd[j];
+      v[W.dmap[r]] = acc;
+    }
+    if (tid == 0) v[0] = x0;
+    __syncthreads();
   }
-  __syncthreads();
-  for (int r = tid; r < m; r += NT) {
-    double acc = 0.0;
-    const double *row = W.Sinv + W.rmap[r] * m;
-    for (int j = 0; j < m; ++j) acc += row[j] * W.xd[j];
-    v[W.dmap[r]] = acc;
-  }
-  if (tid == 0) v[0] = x0;
-  __syncthreads();
   for (int c = tid; c < nch; c += NT) {  // top-down sweep
     const int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
     const double xp = v[W.dmap[W.cpar[c]]];
@@ -1245,7 +1428,7 @@ __device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, do
       ks[i] = r;
     }
     __syncthreads();
-    wsolve<NT>(L, W, W00, ks, tid);
+    wsolve<(NQ > 0 ? 17 + 3 * NQ : 0), NT>(L, W, W00, ks, tid);
   }
   // candidate y1 = u + k8 (perturbations.py:855-859)
   int nanflag = 0;
@@ -1381,7 +1564,7 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
 
 // Layout of the state vector for the truncations in prob (NQ > 0: compile-time bin count) and
 // the carve-up of the dynamic LDS into the solver workspace, which must match
-// deb_perturb_lds_bytes(): 19 n + 2 m^2 + nch m + 6 m + nch + 3 NQMAX doubles, then m + n + 4 nch + 3 m
+// deb_perturb_lds_bytes(): 21 n + 2 m^2 + nch m + 8 m + nch + 21 NQMAX + 24 + 2 NRED^2 doubles, then m + 3 n + 4 nch + 3 m
 // ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
 template <int NQ, int NT>
 __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const double *__restrict__ nu_qw, long lds_bytes, PLay &L, Ws &W,
@@ -1398,11 +1581,13 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   W.y0 = p; p += n; W.y1 = p; p += n; W.u = p; p += n; W.f = p; p += n; W.J0 = p; p += n; W.dT = p; p += n;
   W.K = p; p += 8 * n; W.tlo = p; p += n; W.tdi = p; p += n; W.tup = p; p += n; W.tmul = p; p += n; W.td = p; p += n;
   W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m; W.chx = p; p += nch;
-  W.S2 = p; p += (size_t)m * m; W.snap = p; p += 5 * (size_t)m;
+  W.S2 = p; p += (size_t)m * m; W.snap = p; p += 5 * (size_t)m; W.tca = p; p += n; W.tcb = p; p += n;
+  W.G = p; p += 2 * (size_t)m; W.Minv = p; p += 9 * NQMAX; W.Q = p; p += 6 * NQMAX; W.rn = p; p += 3 * NQMAX; W.xr = p; p += 24;
+  W.R = p; p += NRED * NRED; W.R2 = p; p += NRED * NRED;
   double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
-  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m; W.rmap = ip; ip += m;
+  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m; W.rmap = ip; ip += m; W.rs = ip; ip += n; W.rbin = ip; ip += n;
   ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;  // 8-byte alignment for the cache
   W.sc = (SplCache *)ip; ip += sizeof(SplCache) / 4;
   if ((long)((char *)ip - (char *)smem) > lds_bytes) return false;
@@ -1597,7 +1782,7 @@ __global__ __launch_bounds__(WAVE) void time_deriv_kernel(const double *__restri
   extern __shared__ double smem[];
   Ws W;
   double *p = smem;
-  W.u = p; p += n; W.f = p; p += n;
+  W.u = p; p += n; W.f = p; p += n; W.tca = p; p += n; W.tcb = p; p += n;
   double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
   for (int q = lane; q < L.nq; q += WAVE) {
     sq[q] = nu_qw[q];
@@ -1607,7 +1792,7 @@ __global__ __launch_bounds__(WAVE) void time_deriv_kernel(const double *__restri
   L.q = sq; L.w = sw; L.dlfdlq = sdl;
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
-  W.cpar = ip; ip += nch;
+  W.cpar = ip; ip += nch; W.rs = ip; ip += n; W.rbin = ip; ip += n;
   ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;
   W.sc = (SplCache *)ip;
   if (lane == 0) { W.sc->i = -1; W.sc->j = -1; }
@@ -1641,8 +1826,8 @@ __global__ void tau_out_kernel(const double *__restrict__ tables, long tstride, 
 extern "C" long deb_perturb_lds_bytes(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
   long n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + (long)nqmax * (lmaxnu + 1) + 2;
   long m = 17 + 3 * (long)nqmax, nch = 3 + nqmax;
-  long dbl = n * 19 + 2 * m * m + nch * m + 6 * m + nch + 3 * deb::NQMAX;
-  long ints = m + n + 4 * nch + 3 * m + 1;
+  long dbl = n * 21 + 2 * m * m + nch * m + 8 * m + nch + 21 * deb::NQMAX + 24 + 2 * deb::NRED * deb::NRED;
+  long ints = m + 3 * n + 4 * nch + 3 * m + 1;
   return dbl * 8 + ints * 4 + (long)sizeof(deb::SplCache);
 }
 
@@ -1656,7 +1841,7 @@ extern "C" int deb_launch_time_deriv(const double *d_tables, const double *d_y, 
                                      int n_k, const double *d_nu_qw, const DebProblem *prob, double *d_dy, hipStream_t stream) {
   long n = 7 + (prob->lmaxg + 1) + (prob->lmaxgp + 1) + (prob->lmaxr + 1) + (long)prob->nqmax * (prob->lmaxnu + 1) + 2;
   long m = 17 + 3 * (long)prob->nqmax, nch = 3 + prob->nqmax;
-  long lds = (2 * n + 3 * deb::NQMAX) * 8 + (m + n + 4 * nch + 1) * 4 + (long)sizeof(deb::SplCache);
+  long lds = (4 * n + 3 * deb::NQMAX) * 8 + (m + 3 * n + 4 * nch + 1) * 4 + (long)sizeof(deb::SplCache);
   hipLaunchKernelGGL(deb::time_deriv_kernel, dim3(n_k), dim3(deb::WAVE), (size_t)lds, stream, d_tables, d_y, d_tau, n_tau, d_k, d_nu_qw,
                      *prob, d_dy);
   return (int)hipGetLastError();
