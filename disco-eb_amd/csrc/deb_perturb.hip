@@ -515,6 +515,7 @@ struct Ws {
   // column), per-bin 3x3 inverses, Q = M^-1 U, the neutrino part of a right-hand side, the
   // reduced right-hand side and solution, the reduced matrix in two buffers
   double *G, *Minv, *Q, *rn, *xr, *R, *R2;
+  double *rtd;          // 1 / td of the chain pivots, for the stage solves
   SplCache *sc;
 };
 
@@ -998,6 +999,8 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
     }
   }
   __syncthreads();
+  for (int i = tid; i < n; i += NT)
+    if (W.rk[i] >= 0) W.rtd[i] = 1.0 / W.td[i];
   // Schur correction of the parent diagonal column: S[:, p_c] -= Jh_c * g_c * lo_1c
   for (int r = tid; r < m; r += NT) {
     for (int c = 0; c < nch; ++c) {
@@ -1082,7 +1085,20 @@ __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double 
     const int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
     int ii = h + (len - 1) * s;
     double x = v[ii] + W.J0[ii] * x0;
-    for (int j = len - 2; j >= 0; --j) {
+    int j = len - 2;
+    for (; j >= 3; j -= 4) {  // four elements: all operands loaded, then the serial recurrence
+      const int i0 = ii - s, i1 = ii - 2 * s, i2 = ii - 3 * s, i3 = ii - 4 * s;
+      const double a0 = v[i0] + W.J0[i0] * x0, a1 = v[i1] + W.J0[i1] * x0, a2 = v[i2] + W.J0[i2] * x0, a3 = v[i3] + W.J0[i3] * x0;
+      const double m0 = W.tmul[i0], m1 = W.tmul[i1], m2 = W.tmul[i2], m3 = W.tmul[i3];
+      v[ii] = x;
+      const double x1 = a0 - m0 * x, x2 = a1 - m1 * x1, x3 = a2 - m2 * x2;
+      v[i0] = x1;
+      v[i1] = x2;
+      v[i2] = x3;
+      x = a3 - m3 * x3;
+      ii = i3;
+    }
+    for (; j >= 0; --j) {
       const int in = ii - s;
       const double vn = v[in], jn = W.J0[in], mn = W.tmul[in];
       v[ii] = x;
@@ -1090,7 +1106,7 @@ __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double 
       ii = in;
     }
     v[ii] = x;
-    W.chx[c] = x / W.td[h];  // chain head over its pivot, used by every dense row below
+    W.chx[c] = x * W.rtd[h];  // chain head over its pivot, used by every dense row below
   }
   __syncthreads();
   if constexpr (MC > 0) {
@@ -1154,16 +1170,30 @@ __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double 
     if (tid == 0) v[0] = x0;
     __syncthreads();
   }
-  for (int c = tid; c < nch; c += NT) {  // top-down sweep
+  for (int c = tid; c < nch; c += NT) {  // top-down sweep, four elements at a time as above
     const int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
     const double xp = v[W.dmap[W.cpar[c]]];
-    double x = (v[h] + W.tlo[h] * xp) / W.td[h];
+    double x = (v[h] + W.tlo[h] * xp) * W.rtd[h];
     int ii = h;
-    for (int j = 1; j < len; ++j) {
-      const int in = ii + s;
-      const double vn = v[in], ln = W.tlo[in], dn = W.td[in];
+    int j = 1;
+    for (; j + 3 < len; j += 4) {
+      const int i0 = ii + s, i1 = ii + 2 * s, i2 = ii + 3 * s, i3 = ii + 4 * s;
+      const double v0 = v[i0], v1 = v[i1], v2 = v[i2], v3 = v[i3];
+      const double l0 = W.tlo[i0], l1 = W.tlo[i1], l2 = W.tlo[i2], l3 = W.tlo[i3];
+      const double r0 = W.rtd[i0], r1 = W.rtd[i1], r2 = W.rtd[i2], r3 = W.rtd[i3];
       v[ii] = x;
-      x = (vn + ln * x) / dn;
+      const double x1 = (v0 + l0 * x) * r0, x2 = (v1 + l1 * x1) * r1, x3 = (v2 + l2 * x2) * r2;
+      v[i0] = x1;
+      v[i1] = x2;
+      v[i2] = x3;
+      x = (v3 + l3 * x3) * r3;
+      ii = i3;
+    }
+    for (; j < len; ++j) {
+      const int in = ii + s;
+      const double vn = v[in], ln = W.tlo[in], rn = W.rtd[in];
+      v[ii] = x;
+      x = (vn + ln * x) * rn;
       ii = in;
     }
     v[ii] = x;
@@ -1564,7 +1594,7 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
 
 // Layout of the state vector for the truncations in prob (NQ > 0: compile-time bin count) and
 // the carve-up of the dynamic LDS into the solver workspace, which must match
-// deb_perturb_lds_bytes(): 21 n + 2 m^2 + nch m + 8 m + nch + 21 NQMAX + 24 + 2 NRED^2 doubles, then m + 3 n + 4 nch + 3 m
+// deb_perturb_lds_bytes(): 22 n + 2 m^2 + nch m + 8 m + nch + 21 NQMAX + 24 + 2 NRED^2 doubles, then m + 3 n + 4 nch + 3 m
 // ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
 template <int NQ, int NT>
 __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const double *__restrict__ nu_qw, long lds_bytes, PLay &L, Ws &W,
@@ -1584,6 +1614,7 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   W.S2 = p; p += (size_t)m * m; W.snap = p; p += 5 * (size_t)m; W.tca = p; p += n; W.tcb = p; p += n;
   W.G = p; p += 2 * (size_t)m; W.Minv = p; p += 9 * NQMAX; W.Q = p; p += 6 * NQMAX; W.rn = p; p += 3 * NQMAX; W.xr = p; p += 24;
   W.R = p; p += NRED * NRED; W.R2 = p; p += NRED * NRED;
+  W.rtd = p; p += n;
   double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
@@ -1826,7 +1857,7 @@ __global__ void tau_out_kernel(const double *__restrict__ tables, long tstride, 
 extern "C" long deb_perturb_lds_bytes(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
   long n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + (long)nqmax * (lmaxnu + 1) + 2;
   long m = 17 + 3 * (long)nqmax, nch = 3 + nqmax;
-  long dbl = n * 21 + 2 * m * m + nch * m + 8 * m + nch + 21 * deb::NQMAX + 24 + 2 * deb::NRED * deb::NRED;
+  long dbl = n * 22 + 2 * m * m + nch * m + 8 * m + nch + 21 * deb::NQMAX + 24 + 2 * deb::NRED * deb::NRED;
   long ints = m + 3 * n + 4 * nch + 3 * m + 1;
   return dbl * 8 + ints * 4 + (long)sizeof(deb::SplCache);
 }
