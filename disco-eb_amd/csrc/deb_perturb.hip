@@ -965,7 +965,15 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
     Met<double> mt;
     metric<double>(b, L, k, g, ev, mt);
     auto vsel = [&](int q) { return g.vq_l(L, q); };
+    // the bordered solve reads, of the massive-neutrino rows (dense rows 15 .. 15 + 3 nq - 1),
+    // only each bin's own 3 x 3 block: their other entries are not formed
+    const int nq = L.nq, cq = col - 15;
+    const bool ncol = MC > 0 && col < m && cq >= 0 && cq < 3 * nq;
     for (int r = rg; r < m; r += NR) {
+      const int rq = r - 15;
+      if (MC > 0 && col < m && rq >= 0 && rq < 3 * nq && !(ncol && rq % nq == cq % nq)) continue;
+      // a hierarchy's l = 3 entry (a chain head) appears only in that hierarchy's l = 2 row
+      if (col >= m && r != W.cpar[col - m]) { W.Jh[(col - m) * m + r] = 0.0; continue; }
       double v = core_row<double>(W.dmap[r], b, L, k, g, mt, ev, vsel);
       if (col < m) W.S[r * m + col] = ((r == col) ? gi : 0.0) - v;  // W_DD = I/(gamma h) - J_DD
       else W.Jh[(col - m) * m + r] = v;
