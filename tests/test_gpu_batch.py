@@ -137,12 +137,12 @@ def test_cost_order_matches_index_order_for_eight_momentum_bins(deb):
         assert _same(a, b)
 
 
-@pytest.mark.parametrize("nq,lmaxnu", [(8, 17), (3, 8), (4, 6)])
+@pytest.mark.parametrize("nq,lmaxnu", [(8, 17), (5, 10), (3, 8), (4, 6)])
 def test_workgroup_width_gives_identical_results(deb, nq, lmaxnu):
     """One, two or four wavefronts per unit perform the same operations on every entry (rows,
     matrix elements and Jacobian columns are only distributed differently): bitwise the same
-    states, step counts and step meshes, per-mode and batched; nq = 4 runs the generic
-    (run-time block size) factorisation."""
+    states, step counts and step meshes, per-mode and batched; nq = 3, 5, 8 run the bordered
+    solve, nq = 4 the generic (run-time block size) factorisation."""
     print(f"case: threads per unit 64/128/256, nq={nq} lmax_nu={lmaxnu}, 6 k 1e-5..5", flush=True)
     p = deb.evolve_background(param=dict(nu_w0wa()), num_thermo=256)
     tau = p["tau_of_a_spline"].evaluate(np.asarray([1 / 3, 0.5, 1.0])).reshape(1, -1)

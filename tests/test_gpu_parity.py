@@ -185,6 +185,18 @@ def test_evolve_massive_nu_w0wa_matches_oracle(deb, oracle):
     assert end.max() < TOL_TRANSFER
 
 
+def test_evolve_five_momentum_bins_matches_oracle(deb, oracle):
+    """The five-bin momentum grid (the compile-time 32-unknown dense block), lmax_nu 10, 4 modes."""
+    k = oracle.geomspace(1e-4, 2.0, 4)
+    aout = [0.25, 1.0]
+    lmax = [10, 10, 10, 10, 5]
+    P = nu_w0wa_param()
+    print("case: evolve massive nu nq=5 nk=4 nout=2", flush=True)
+    p, yg, na, nr, log = _gpu_run(deb, P, k, aout, lmax, 1e-5, 1e-10)
+    bg = oracle_on_gpu_tables(deb, oracle, p)
+    _check_replay(bg, k, aout, lmax, 1e-5, 1e-10, yg, log, na, nr, label="nq5")
+
+
 def test_return_full_state_and_time_derivatives(deb, oracle):
     """Projected full state (RSA projection included) and compute_time_derivatives vs the oracle RHS."""
     P = disco_notebook()
