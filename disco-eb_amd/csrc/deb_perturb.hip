@@ -510,6 +510,7 @@ struct Ws {
   double *S2, *snap, *Sinv;  // second matrix buffer, pivot snapshots, the buffer holding the inverse
   int *dmap, *rk, *chead, *cstride, *clen, *cpar, *piv, *pi, *rmap;
   int *rs, *rbin;       // per row: chain stride (tail rows), momentum bin (massive-neutrino rows)
+  int *perm;            // RHS row of each thread slot (slot tid + it NT), -1 for none
   double *tca, *tcb;    // per tail row l < lmax: l / (2l + 1) and -(l + 1) / (2l + 1)
   // bordered solve of the dense block (fixed block sizes): metric derivatives G (2 per dense
   // column), per-bin 3x3 inverses, Q = M^-1 U, the neutrino part of a right-hand side, the
@@ -556,6 +557,31 @@ __device__ __forceinline__ void setup_maps(const PLay &L, Ws &W, int tid) {
       W.tcb[i] = -(double)(l + 1) * inv;
     }
   __syncthreads();
+  // Row order of the RHS evaluation: a wave pays for every row formula any of its lanes takes,
+  // so each class of core rows (row 0 and rows 1..9; polarisation, massless-neutrino and
+  // dark-energy rows l <= 2; massive-neutrino rows l <= 2) gets a wave-sized block of slots
+  // of its own where there are enough, and the tail rows fill the rest.
+  if (tid == 0) {
+    const int nit = (n + NT - 1) / NT, slots = nit * NT, nb = slots / WAVE;
+    for (int t = 0; t < slots; ++t) W.perm[t] = -1;
+    auto cls = [&](int i) {
+      if (W.rk[i] >= 0) return 3;
+      if (i <= 9) return 0;
+      if (i >= L.iq0 && i < L.iq0 + 3 * nq) return 2;
+      return 1;
+    };
+    const int home[3] = {nb - 1, 0, nb > 2 ? 1 : 0};
+    int used[64];  // slots taken per block (nb <= 64 for n <= 4096 rows)
+    for (int bk = 0; bk < nb; ++bk) used[bk] = 0;
+    for (int c = 0; c < 4; ++c)
+      for (int i = 0; i < n; ++i) {
+        if (cls(i) != c) continue;
+        int bk = (c < 3) ? home[c] : 0;
+        while (used[bk] == WAVE) bk = (bk + 1) % nb;
+        W.perm[bk * WAVE + used[bk]++] = i;
+      }
+  }
+  __syncthreads();
 }
 
 // full RHS f(tau, u) into out[] (lanes over rows); u in LDS
@@ -568,11 +594,11 @@ __device__ __forceinline__ void rhs_eval(const BgView &b, const PLay &L, double 
   metric<double>(b, L, k, g, yv, m);
   const int nit = (L.n + NT - 1) / NT;
   for (int it = 0; it < nit; ++it) {  // a uniform trip count: every lane takes part in the shuffle
-    const int i = tid + it * NT;
-    const int ic = i < L.n ? i : 0;
+    const int i = W.perm[tid + it * NT];
+    const int ic = i >= 0 ? i : 0;
     const double vsh = __shfl(g.vl, VQ_LANE0 + W.rbin[ic]);  // this row's momentum-bin factor
     auto vrow = [&](int) { return vsh; };
-    if (i < L.n) {
+    if (i >= 0) {
       const int rk = W.rk[i];
       double r;
       if (rk < 0) {
@@ -604,15 +630,15 @@ __device__ __forceinline__ void rhs_dual(const BgView &b, const PLay &L, double 
   metric<D2>(b, L, k, g, yv, m);
   const int nit = (L.n + NT - 1) / NT;
   for (int it = 0; it < nit; ++it) {  // a uniform trip count: every lane takes part in the shuffle
-    const int i = tid + it * NT;
-    const int ic = i < L.n ? i : 0;
+    const int i = W.perm[tid + it * NT];
+    const int ic = i >= 0 ? i : 0;
     const int src = VQ_LANE0 + W.rbin[ic];
     D2 vsh;
     vsh.v = __shfl(g.vl.v, src);
     vsh.d[0] = __shfl(g.vl.d[0], src);
     vsh.d[1] = __shfl(g.vl.d[1], src);
     auto vrow = [&](int) { return vsh; };
-    if (i < L.n) {
+    if (i >= 0) {
       const int rk = W.rk[i];
       D2 r;
       if (rk < 0) {
@@ -1602,7 +1628,7 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
 
 // Layout of the state vector for the truncations in prob (NQ > 0: compile-time bin count) and
 // the carve-up of the dynamic LDS into the solver workspace, which must match
-// deb_perturb_lds_bytes(): 22 n + 2 m^2 + nch m + 8 m + nch + 21 NQMAX + 24 + 2 NRED^2 doubles, then m + 3 n + 4 nch + 3 m
+// deb_perturb_lds_bytes(): 22 n + 2 m^2 + nch m + 8 m + nch + 21 NQMAX + 24 + 2 NRED^2 doubles, then m + 4 n + 256 + 4 nch + 3 m
 // ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
 template <int NQ, int NT>
 __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const double *__restrict__ nu_qw, long lds_bytes, PLay &L, Ws &W,
@@ -1626,7 +1652,7 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
-  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m; W.rmap = ip; ip += m; W.rs = ip; ip += n; W.rbin = ip; ip += n;
+  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m; W.rmap = ip; ip += m; W.rs = ip; ip += n; W.rbin = ip; ip += n; W.perm = ip; ip += n + 256;
   ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;  // 8-byte alignment for the cache
   W.sc = (SplCache *)ip; ip += sizeof(SplCache) / 4;
   if ((long)((char *)ip - (char *)smem) > lds_bytes) return false;
@@ -1831,7 +1857,7 @@ __global__ __launch_bounds__(WAVE) void time_deriv_kernel(const double *__restri
   L.q = sq; L.w = sw; L.dlfdlq = sdl;
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
-  W.cpar = ip; ip += nch; W.rs = ip; ip += n; W.rbin = ip; ip += n;
+  W.cpar = ip; ip += nch; W.rs = ip; ip += n; W.rbin = ip; ip += n; W.perm = ip; ip += n + 256;
   ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;
   W.sc = (SplCache *)ip;
   if (lane == 0) { W.sc->i = -1; W.sc->j = -1; }
@@ -1866,7 +1892,7 @@ extern "C" long deb_perturb_lds_bytes(int lmaxg, int lmaxgp, int lmaxr, int lmax
   long n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + (long)nqmax * (lmaxnu + 1) + 2;
   long m = 17 + 3 * (long)nqmax, nch = 3 + nqmax;
   long dbl = n * 22 + 2 * m * m + nch * m + 8 * m + nch + 21 * deb::NQMAX + 24 + 2 * deb::NRED * deb::NRED;
-  long ints = m + 3 * n + 4 * nch + 3 * m + 1;
+  long ints = m + 4 * n + 256 + 4 * nch + 3 * m + 1;
   return dbl * 8 + ints * 4 + (long)sizeof(deb::SplCache);
 }
 
@@ -1880,7 +1906,7 @@ extern "C" int deb_launch_time_deriv(const double *d_tables, const double *d_y, 
                                      int n_k, const double *d_nu_qw, const DebProblem *prob, double *d_dy, hipStream_t stream) {
   long n = 7 + (prob->lmaxg + 1) + (prob->lmaxgp + 1) + (prob->lmaxr + 1) + (long)prob->nqmax * (prob->lmaxnu + 1) + 2;
   long m = 17 + 3 * (long)prob->nqmax, nch = 3 + prob->nqmax;
-  long lds = (4 * n + 3 * deb::NQMAX) * 8 + (m + 3 * n + 4 * nch + 1) * 4 + (long)sizeof(deb::SplCache);
+  long lds = (4 * n + 3 * deb::NQMAX) * 8 + (m + 4 * n + 256 + 4 * nch + 1) * 4 + (long)sizeof(deb::SplCache);
   hipLaunchKernelGGL(deb::time_deriv_kernel, dim3(n_k), dim3(deb::WAVE), (size_t)lds, stream, d_tables, d_y, d_tau, n_tau, d_k, d_nu_qw,
                      *prob, d_dy);
   return (int)hipGetLastError();
