@@ -982,10 +982,11 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
   rhs_dual<NT>(b, L, k, t0, W.y0, W.J0, W.dT, W, tid);
   PBg<double> g;
   pbg_eval_lanes(b, L, k, t0, W.y0[0], g, W.sc, tid);
-  // dense-block columns 0..m-1 and chain-head columns m..m+nch-1, each split into NR row groups
+  // dense-block columns 0..m-1 and chain-head columns m..m+nch-1, one per lane; wave w takes
+  // rows w, w + NR, ... of every column, so a wave evaluates one row formula at a time
   constexpr int NR = NT / WAVE;
-  for (int it = tid; it < (m + nch) * NR; it += NT) {
-    const int col = it % (m + nch), rg = it / (m + nch);
+  const int rg = tid / WAVE;
+  for (int col = tid % WAVE; col < m + nch; col += WAVE) {
     int s = (col < m) ? W.dmap[col] : W.chead[col - m];
     auto ev = [&](int i) { return i == s ? 1.0 : 0.0; };
     Met<double> mt;
