@@ -154,10 +154,20 @@ struct SplCoef {
   double x[2], ye[2], Se[2], yc[2], Sc[2], nx[2], nr[2], nSr[2], np[2], nSp[2];
 };
 
+// Orders the LDS accesses of one wavefront: what a lane wrote before it is visible to every
+// lane of the same wavefront after it (LDS operations of one wavefront complete in order).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Interval coefficients for log a = la.  A hit reads the LDS entry; a miss loads from the
 // tables on every thread, thread 0 republishes the entry and a barrier orders that write before
 // any later read.  The hit/miss decision is uniform across the workgroup (same inputs, same
-// entry), so the barriers are reached by all threads.
+// entry), so the barriers are reached by all threads.  WL: called by one wavefront alone while
+// no other wavefront uses the cache; its lane 0 writes and wave_sync orders the accesses.
+template <bool WL = false>
 __device__ __forceinline__ void cache_lookup(const BgView &b, SplCache *c, double la, int tid, SplCoef &k) {
   const int N = b.N;
   bool refreshed = false;
@@ -185,8 +195,8 @@ __device__ __forceinline__ void cache_lookup(const BgView &b, SplCache *c, doubl
     refreshed = true;
   }
   if (refreshed) {
-    __syncthreads();  // every thread has read the old entry before thread 0 overwrites it
-    if (tid == 0) {
+    if constexpr (WL) wave_sync(); else __syncthreads();  // every thread has read the old entry before it is overwritten
+    if ((WL ? (tid & (WAVE - 1)) : tid) == 0) {
       if (!hit) {
         c->i = i; c->x[0] = k.x[0]; c->x[1] = k.x[1]; c->ye[0] = k.ye[0]; c->ye[1] = k.ye[1]; c->Se[0] = k.Se[0];
         c->Se[1] = k.Se[1]; c->yc[0] = k.yc[0]; c->yc[1] = k.yc[1]; c->Sc[0] = k.Sc[0]; c->Sc[1] = k.Sc[1];
@@ -196,7 +206,7 @@ __device__ __forceinline__ void cache_lookup(const BgView &b, SplCache *c, doubl
         c->nSr[1] = k.nSr[1]; c->np[0] = k.np[0]; c->np[1] = k.np[1]; c->nSp[0] = k.nSp[0]; c->nSp[1] = k.nSp[1];
       }
     }
-    __syncthreads();  // the new entry is visible before any later lookup
+    if constexpr (WL) wave_sync(); else __syncthreads();  // the new entry is visible before any later lookup
   }
 }
 
@@ -259,7 +269,9 @@ __device__ __forceinline__ void pbg_eval(const BgView &b, const PLay &L, double 
 // lanes: one exp serves rho_nu, p_nu (lanes 0, 1) and the dark-energy density (lane 2, as
 // exp(-3 (1 + w0 + wa) ln a + 3 (a - 1) wa)); then five rounds of one division per lane and
 // one square root, each result broadcast with readlane.  Every other quantity is the same
-// expression as in pbg_eval.  All 64 lanes of every wave must be active.
+// expression as in pbg_eval.  All 64 lanes of every wave must be active.  WL: one wavefront
+// evaluates alone (see cache_lookup).
+template <bool WL = false>
 __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, double k, double tau, double a, PBg<double> &g,
                                                SplCache *sc, int tid) {
   const int lane = tid & (WAVE - 1);
@@ -269,7 +281,7 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
   g.tau = tau;
   const double loga = log(a);
   SplCoef c;
-  cache_lookup(b, sc, loga, tid, c);
+  cache_lookup<WL>(b, sc, loga, tid, c);
   const double cs2c = cubic_at(c.x[0], c.x[1], c.yc[0], c.yc[1], c.Sc[0], c.Sc[1], loga);
   g.xe = cubic_at(c.x[0], c.x[1], c.ye[0], c.ye[1], c.Se[0], c.Se[1], loga);
   const double dxe = cubic_deriv_at(c.x[0], c.x[1], c.ye[0], c.ye[1], c.Se[0], c.Se[1], loga);
@@ -353,6 +365,37 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
   g.ufa = !g.rsa && b.use_ufa && (tv * k > b.ufa_k);
   bool uncoupled = (r_tk > b.tca_k) || ((r_th > b.tca_h) && (r_tk > 0.1 * b.tca_k));
   g.tca = !g.rsa && b.use_tca && !uncoupled;
+}
+
+// A PBg<double> in LDS, handed from the wavefront that evaluated it to the whole workgroup:
+// the uniform fields, the per-bin velocity factors (lanes VQ_LANE0 + q) and the switches.
+// x0 carries the row-0 unknown of the current stage solve.
+struct PBgShared {
+  double f[15];
+  double vl[NQMAX];
+  double x0;
+  int flags;
+};
+
+__device__ __forceinline__ void pbg_store(const PBg<double> &g, PBgShared *s, const PLay &L, int lane) {
+  if (lane == 0) {
+    s->f[0] = g.a; s->f[1] = g.tau; s->f[2] = g.aH; s->f[3] = g.aHH; s->f[4] = g.cs2; s->f[5] = g.xe; s->f[6] = g.xeprime;
+    s->f[7] = g.opac; s->f[8] = g.tauc; s->f[9] = g.taucprime; s->f[10] = g.pb43; s->f[11] = g.rho_Q; s->f[12] = g.w_Q;
+    s->f[13] = g.ca2_Q; s->f[14] = g.amnu_a;
+    s->flags = (g.rsa ? 1 : 0) | (g.ufa ? 2 : 0) | (g.tca ? 4 : 0);
+  }
+  const int qb = lane - VQ_LANE0;
+  if (qb >= 0 && qb < L.nq) s->vl[qb] = g.vl;
+}
+
+__device__ __forceinline__ void pbg_load(PBg<double> &g, const PBgShared *s, const PLay &L, int lane) {
+  g.a = s->f[0]; g.tau = s->f[1]; g.aH = s->f[2]; g.aHH = s->f[3]; g.cs2 = s->f[4]; g.xe = s->f[5]; g.xeprime = s->f[6];
+  g.opac = s->f[7]; g.tauc = s->f[8]; g.taucprime = s->f[9]; g.pb43 = s->f[10]; g.rho_Q = s->f[11]; g.w_Q = s->f[12];
+  g.ca2_Q = s->f[13]; g.amnu_a = s->f[14];
+  const int fl = s->flags;
+  g.rsa = fl & 1; g.ufa = (fl & 2) != 0; g.tca = (fl & 4) != 0;
+  const int qb = lane - VQ_LANE0;
+  g.vl = (qb >= 0 && qb < L.nq) ? s->vl[qb] : 0.0;  // only the bin lanes are ever read
 }
 
 // metric perturbations and effective (RSA/TCA) fields, perturbations.py:292-351,407-427
@@ -517,6 +560,7 @@ struct Ws {
   // reduced right-hand side and solution, the reduced matrix in two buffers
   double *G, *Minv, *Q, *rn, *xr, *R, *R2;
   double *rtd;          // 1 / td of the chain pivots, for the stage solves
+  PBgShared *pg;        // background of the next stage, evaluated during the current stage solve
   SplCache *sc;
 };
 
@@ -584,11 +628,10 @@ __device__ __forceinline__ void setup_maps(const PLay &L, Ws &W, int tid) {
   __syncthreads();
 }
 
-// full RHS f(tau, u) into out[] (lanes over rows); u in LDS
+// full RHS f(tau, u) into out[] (lanes over rows) with the background g at (tau, u[0]); u in LDS
 template <int NT>
-__device__ __forceinline__ void rhs_eval(const BgView &b, const PLay &L, double k, double tau, const double *u, double *out, Ws &W, int tid) {
-  PBg<double> g;
-  pbg_eval_lanes(b, L, k, tau, u[0], g, W.sc, tid);
+__device__ __forceinline__ void rhs_rows(const BgView &b, const PLay &L, double k, const PBg<double> &g, const double *u, double *out, Ws &W,
+                                         int tid) {
   auto yv = [&](int i) { return u[i]; };
   Met<double> m;
   metric<double>(b, L, k, g, yv, m);
@@ -613,6 +656,13 @@ __device__ __forceinline__ void rhs_eval(const BgView &b, const PLay &L, double 
       out[i] = r;
     }
   }
+}
+
+template <int NT>
+__device__ __forceinline__ void rhs_eval(const BgView &b, const PLay &L, double k, double tau, const double *u, double *out, Ws &W, int tid) {
+  PBg<double> g;
+  pbg_eval_lanes(b, L, k, tau, u[0], g, W.sc, tid);
+  rhs_rows<NT>(b, L, k, g, u, out, W, tid);
 }
 
 // d f / da (into outa) and d f / dtau (into outt) at (tau, y): one pass with two dual directions
@@ -976,11 +1026,12 @@ __device__ __forceinline__ void reduce_dense(const PLay &L, Ws &W, int tid) {
 
 // Jacobian pieces at (t0, y0) and the structured factorisation of W = I/(gamma dt) - J
 template <int MC, int NT>
-__device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double k, double t0, double dt, Ws &W, int tid, double &W00) {
+// g receives the background at (t0, y0[0]), which is also the first stage's.
+__device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double k, double t0, double dt, Ws &W, int tid, double &W00,
+                                          PBg<double> &g) {
   const int n = L.n, m = L.m, nch = L.nch;
   const double gi = 1.0 / (dt * r5::gamma);
   rhs_dual<NT>(b, L, k, t0, W.y0, W.J0, W.dT, W, tid);
-  PBg<double> g;
   pbg_eval_lanes(b, L, k, t0, W.y0[0], g, W.sc, tid);
   // dense-block columns 0..m-1 and chain-head columns m..m+nch-1, one per lane; wave w takes
   // rows w, w + NR, ... of every column, so a wave evaluates one row formula at a time
@@ -1109,14 +1160,16 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
   }
 }
 
-// solve W x = v in place (v in LDS, length n)
-template <int MC, int NT>
-__device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double *v, int tid) {
+// solve W x = v in place (v in LDS, length n), x0 = v[0] / W_00 given: wavefront 0 alone (tid
+// is its lane), its phases ordered by wave_sync; the caller's workgroup barrier publishes x.
+// Every phase has at most 64 items (chains, dense rows, reduced rows) but the dense rows of a
+// 16-bin grid, so the other wavefronts are free for other work meanwhile.
+template <int MC>
+__device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double x0, double *v, int tid) {
   const int m = L.m, nch = L.nch;
-  const double x0 = v[0] / W00;  // row 0 first; v[0] itself is overwritten only after a barrier
   // bottom-up sweep of each chain, one lane per chain, with the -J0 x0 elimination of row 0
   // applied on the way; the next element's operands are loaded before the current store
-  for (int c = tid; c < nch; c += NT) {
+  for (int c = tid; c < nch; c += WAVE) {
     const int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
     int ii = h + (len - 1) * s;
     double x = v[ii] + W.J0[ii] * x0;
@@ -1143,26 +1196,26 @@ __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double 
     v[ii] = x;
     W.chx[c] = x * W.rtd[h];  // chain head over its pivot, used by every dense row below
   }
-  __syncthreads();
+  wave_sync();
   if constexpr (MC > 0) {
     // bordered solve (see reduce_dense): dense right-hand side, neutrino blocks, the reduced
     // 19 x 19 system, then the neutrino unknowns from h' and eta'
     constexpr int NQ = (MC - NCORE) / 3;
     constexpr int NN = 3 * NQ;
-    for (int r = tid; r < MC; r += NT) {
+    for (int r = tid; r < MC; r += WAVE) {
       const int i = W.dmap[r];
       double acc = v[i] + W.J0[i] * x0;
       for (int c = 0; c < nch; ++c) acc += W.Jh[c * MC + r] * W.chx[c];
       W.xd[r] = acc;
     }
-    __syncthreads();
-    for (int j = tid; j < NN; j += NT) {  // M^-1 b_n
+    wave_sync();
+    for (int j = tid; j < NN; j += WAVE) {  // M^-1 b_n
       const int l = j / NQ, q = j - l * NQ;
       const double *mi = W.Minv + 9 * q + 3 * l;
       W.rn[j] = mi[0] * W.xd[15 + q] + mi[1] * W.xd[15 + NQ + q] + mi[2] * W.xd[15 + 2 * NQ + q];
     }
-    __syncthreads();
-    for (int i = tid; i < NRED; i += NT) {  // reduced right-hand side, through the pivot permutation
+    wave_sync();
+    for (int i = tid; i < NRED; i += WAVE) {  // reduced right-hand side, through the pivot permutation
       double acc;
       if (i < NCORE) {
         const int ci = core_index(i, NQ);
@@ -1175,37 +1228,37 @@ __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double 
       }
       W.xr[W.pi[i]] = acc;
     }
-    __syncthreads();
-    for (int i = tid; i < NRED; i += NT) {
+    wave_sync();
+    for (int i = tid; i < NRED; i += WAVE) {
       const double *row = W.Sinv + W.rmap[i] * NRED;
       double acc = 0.0;
       for (int j = 0; j < NRED; ++j) acc += row[j] * W.xr[j];
       if (i < NCORE) v[W.dmap[core_index(i, NQ)]] = acc;
       else W.xr[NRED + i - NCORE] = acc;  // h', eta'
     }
-    if (tid == 0) v[0] = x0;
-    __syncthreads();
-    for (int j = tid; j < NN; j += NT) v[W.dmap[15 + j]] = W.rn[j] + W.Q[2 * j] * W.xr[NRED] + W.Q[2 * j + 1] * W.xr[NRED + 1];
-    __syncthreads();
+    if (tid == 0) v[0] = x0;  // read by no one before the workgroup barrier that follows the solve
+    wave_sync();
+    for (int j = tid; j < NN; j += WAVE) v[W.dmap[15 + j]] = W.rn[j] + W.Q[2 * j] * W.xr[NRED] + W.Q[2 * j + 1] * W.xr[NRED + 1];
+    wave_sync();
   } else {
     // Schur right-hand side of the dense block (row 0 and the chain heads eliminated)
-    for (int r = tid; r < m; r += NT) {
+    for (int r = tid; r < m; r += WAVE) {
       const int i = W.dmap[r];
       double acc = v[i] + W.J0[i] * x0;
       for (int c = 0; c < nch; ++c) acc += W.Jh[c * m + r] * W.chx[c];
       W.xd[W.pi[r]] = acc;
     }
-    __syncthreads();
-    for (int r = tid; r < m; r += NT) {
+    wave_sync();
+    for (int r = tid; r < m; r += WAVE) {
       double acc = 0.0;
       const double *row = W.Sinv + W.rmap[r] * m;
       for (int j = 0; j < m; ++j) acc += row[j] * W.xd[j];
       v[W.dmap[r]] = acc;
     }
-    if (tid == 0) v[0] = x0;
-    __syncthreads();
+    if (tid == 0) v[0] = x0;  // read by no one before the workgroup barrier that follows the solve
+    wave_sync();
   }
-  for (int c = tid; c < nch; c += NT) {  // top-down sweep, four elements at a time as above
+  for (int c = tid; c < nch; c += WAVE) {  // top-down sweep, four elements at a time as above
     const int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
     const double xp = v[W.dmap[W.cpar[c]]];
     double x = (v[h] + W.tlo[h] * xp) * W.rtd[h];
@@ -1233,7 +1286,6 @@ __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double W00, double 
     }
     v[ii] = x;
   }
-  __syncthreads();
 }
 
 // adiabatic initial conditions (perturbations.py:680-781), written by one thread
@@ -1434,38 +1486,54 @@ struct Pid {
 // One Rodas5 step attempt of the mode with wavenumber k from W.y0 at tprev over dt: factorises
 // W = 1 / (gamma dt) - J, runs the eight stages and writes the candidate y1 = u + k8 to W.y1
 // (K + 7 n holds the error estimate k8).  Returns 1 on every thread if y1 has a NaN.
+// Rodas5 stage time (stages 6-8 at t0 + dt) and stage input y0 + sum_j a_sj k_j (u + k_5,
+// u + k_6 for the last two); kk(j) returns k_j of the row.  Both the stage-input pass and the
+// look-ahead of the next stage's row 0 use these, so the two agree to the bit.
+__device__ __forceinline__ double stage_time(int s, double tprev, double dt) {
+  using namespace r5;
+  const double cs = (s == 0) ? 0.0 : (s == 1) ? c2 : (s == 2) ? c3 : (s == 3) ? c4 : c5;
+  return (s < 5) ? tprev + cs * dt : tprev + dt;
+}
+template <class KK>
+__device__ __forceinline__ double stage_in(int s, double y0i, double ui, const KK &kk) {
+  using namespace r5;
+  switch (s) {
+    case 0: return y0i;
+    case 1: return fma(a21, kk(0), y0i);
+    case 2: return fma(a32, kk(1), fma(a31, kk(0), y0i));
+    case 3: return fma(a43, kk(2), fma(a42, kk(1), fma(a41, kk(0), y0i)));
+    case 4: return fma(a54, kk(3), fma(a53, kk(2), fma(a52, kk(1), fma(a51, kk(0), y0i))));
+    case 5: return fma(a65, kk(4), fma(a64, kk(3), fma(a63, kk(2), fma(a62, kk(1), fma(a61, kk(0), y0i)))));
+    case 6: return ui + kk(5);
+    default: return ui + kk(6);
+  }
+}
+
+// One Rodas5 attempt from (tprev, y0) with step dt.  The first stage reuses the background the
+// Jacobian pass evaluated at (tprev, y0[0]).  With more than one wavefront, the stage solve runs
+// on wavefront 0 while wavefront 1 evaluates the next stage's background: it needs only row 0
+// of the stage input, and row 0 of the solve (x0 = v[0] / W_00) is known before the solve starts.
 template <int NQ, int NT>
 __device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, double k, double tprev, double dt, Ws &W, int tid) {
+  constexpr int MC = NQ > 0 ? 17 + 3 * NQ : 0;
   const int n = L.n;
   double *K = W.K;
   double *u = W.u, *f = W.f, *y0 = W.y0, *dT = W.dT;
   double W00;
-  factorise<(NQ > 0 ? 17 + 3 * NQ : 0), NT>(b, L, k, tprev, dt, W, tid, W00);
+  PBg<double> g;
+  factorise<MC, NT>(b, L, k, tprev, dt, W, tid, W00, g);
   using namespace r5;
-  // stages 1-5 (with the df/dtau term), 6-8 at t0 + dt
   for (int s = 0; s < 8; ++s) {
     double *ks = K + s * n;
-    // stage input u
-    for (int i = tid; i < n; i += NT) {
-      double v;
-      switch (s) {
-        case 0: v = y0[i]; break;
-        case 1: v = y0[i] + a21 * K[i]; break;
-        case 2: v = y0[i] + a31 * K[i] + a32 * K[n + i]; break;
-        case 3: v = y0[i] + a41 * K[i] + a42 * K[n + i] + a43 * K[2 * n + i]; break;
-        case 4: v = y0[i] + a51 * K[i] + a52 * K[n + i] + a53 * K[2 * n + i] + a54 * K[3 * n + i]; break;
-        case 5: v = y0[i] + a61 * K[i] + a62 * K[n + i] + a63 * K[2 * n + i] + a64 * K[3 * n + i] + a65 * K[4 * n + i]; break;
-        case 6: v = u[i] + K[5 * n + i]; break;
-        default: v = u[i] + K[6 * n + i]; break;
-      }
-      u[i] = v;
-    }
+    for (int i = tid; i < n; i += NT) u[i] = stage_in(s, y0[i], u[i], [&](int j) { return K[j * n + i]; });
     __syncthreads();
-    // stage time and df/dtau weight, selected without a run-time-indexed array (scratch memory)
-    const double cs = (s == 0) ? 0.0 : (s == 1) ? c2 : (s == 2) ? c3 : (s == 3) ? c4 : c5;
+    // df/dtau weight, selected without a run-time-indexed array (scratch memory)
     const double dts = dt * ((s == 0) ? d1 : (s == 1) ? d2 : (s == 2) ? d3 : (s == 3) ? d4 : d5);
-    double ts = (s < 5) ? tprev + cs * dt : tprev + dt;
-    rhs_eval<NT>(b, L, k, ts, u, f, W, tid);
+    if (s > 0) {
+      if constexpr (NT == WAVE) pbg_eval_lanes(b, L, k, stage_time(s, tprev, dt), u[0], g, W.sc, tid);
+      else pbg_load(g, W.pg, L, tid & (WAVE - 1));
+    }
+    rhs_rows<NT>(b, L, k, g, u, f, W, tid);
     __syncthreads();
     for (int i = tid; i < n; i += NT) {
       double r = f[i];
@@ -1491,9 +1559,21 @@ __device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, do
           break;
       }
       ks[i] = r;
+      if (i == 0) W.pg->x0 = r / W00;
     }
     __syncthreads();
-    wsolve<(NQ > 0 ? 17 + 3 * NQ : 0), NT>(L, W, W00, ks, tid);
+    if (tid < WAVE) {
+      wsolve<MC>(L, W, W.pg->x0, ks, tid);
+    } else if constexpr (NT > WAVE) {
+      if (tid < 2 * WAVE && s < 7) {
+        const double x0 = W.pg->x0;
+        const double un = stage_in(s + 1, y0[0], u[0], [&](int j) { return j == s ? x0 : K[j * n]; });
+        PBg<double> gn;
+        pbg_eval_lanes<true>(b, L, k, stage_time(s + 1, tprev, dt), un, gn, W.sc, tid);
+        pbg_store(gn, W.pg, L, tid - WAVE);
+      }
+    }
+    __syncthreads();
   }
   // candidate y1 = u + k8 (perturbations.py:855-859)
   int nanflag = 0;
@@ -1629,7 +1709,7 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
 
 // Layout of the state vector for the truncations in prob (NQ > 0: compile-time bin count) and
 // the carve-up of the dynamic LDS into the solver workspace, which must match
-// deb_perturb_lds_bytes(): 22 n + 2 m^2 + nch m + 8 m + nch + 21 NQMAX + 24 + 2 NRED^2 doubles, then m + 4 n + 256 + 4 nch + 3 m
+// deb_perturb_lds_bytes(): 22 n + 2 m^2 + nch m + 8 m + nch + 21 NQMAX + 24 + 2 NRED^2 doubles and a PBgShared, then m + 4 n + 256 + 4 nch + 3 m
 // ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
 template <int NQ, int NT>
 __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const double *__restrict__ nu_qw, long lds_bytes, PLay &L, Ws &W,
@@ -1650,6 +1730,7 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   W.G = p; p += 2 * (size_t)m; W.Minv = p; p += 9 * NQMAX; W.Q = p; p += 6 * NQMAX; W.rn = p; p += 3 * NQMAX; W.xr = p; p += 24;
   W.R = p; p += NRED * NRED; W.R2 = p; p += NRED * NRED;
   W.rtd = p; p += n;
+  W.pg = (PBgShared *)p; p += sizeof(PBgShared) / 8;
   double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
@@ -1892,7 +1973,7 @@ __global__ void tau_out_kernel(const double *__restrict__ tables, long tstride, 
 extern "C" long deb_perturb_lds_bytes(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
   long n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + (long)nqmax * (lmaxnu + 1) + 2;
   long m = 17 + 3 * (long)nqmax, nch = 3 + nqmax;
-  long dbl = n * 22 + 2 * m * m + nch * m + 8 * m + nch + 21 * deb::NQMAX + 24 + 2 * deb::NRED * deb::NRED;
+  long dbl = n * 22 + 2 * m * m + nch * m + 8 * m + nch + 21 * deb::NQMAX + 24 + 2 * deb::NRED * deb::NRED + (long)(sizeof(deb::PBgShared) / 8);
   long ints = m + 4 * n + 256 + 4 * nch + 3 * m + 1;
   return dbl * 8 + ints * 4 + (long)sizeof(deb::SplCache);
 }
