@@ -549,16 +549,18 @@ __device__ __forceinline__ void tail_coef(int ch, int l, const PLay &L, double k
 // ---------------------------------------------------------------------------------------
 // per-unit LDS workspace (shared by the workgroup's threads)
 struct Ws {
-  double *y0, *y1, *u, *f, *J0, *dT, *K, *S, *Jh, *tlo, *tdi, *tup, *tmul, *td, *xd, *chx;
+  double *y0, *y1, *u, *f, *J0, *dT, *K, *S, *Jh, *tlo, *tdi, *tup, *tmul, *td, *xd;
   double *S2, *snap, *Sinv;  // second matrix buffer, pivot snapshots, the buffer holding the inverse
   int *dmap, *rk, *chead, *cstride, *clen, *cpar, *piv, *pi, *rmap;
   int *rs, *rbin;       // per row: chain stride (tail rows), momentum bin (massive-neutrino rows)
   int *perm;            // RHS row of each thread slot (slot tid + it NT), -1 for none
+  int *pch;             // per dense row: the chain whose head couples to it (its l = 2 parent), -1 for none
   double *tca, *tcb;    // per tail row l < lmax: l / (2l + 1) and -(l + 1) / (2l + 1)
   // bordered solve of the dense block (fixed block sizes): metric derivatives G (2 per dense
   // column), per-bin 3x3 inverses, Q = M^-1 U, the neutrino part of a right-hand side, the
   // reduced right-hand side and solution, the reduced matrix in two buffers
   double *G, *Minv, *Q, *rn, *xr, *R, *R2;
+  double *P;            // [W_cn; G_n^T] M^-1 (NRED x 3 nq): the reduced right-hand side straight from b
   double *rtd;          // 1 / td of the chain pivots, for the stage solves
   PBgShared *pg;        // background of the next stage, evaluated during the current stage solve
   SplCache *sc;
@@ -969,7 +971,9 @@ __device__ __forceinline__ int core_index(int i, int nq) { return i < 15 ? i : 3
 //   [ -G_c^T   I - G_n^T Q      ] [ z ] = [ G_n^T M^-1 b_n      ]
 // is a 19 x 19 system for every nq.  W (as built), G (from the metric on the unit vectors
 // that give W's columns) are numeric; U is (dl/6, 0, -dl/15) for h' and (0, 0, -2 dl/5) for
-// eta' in the l = 0, 1, 2 rows (the reference's massive-neutrino equations).
+// eta' in the l = 0, 1, 2 rows (the reference's massive-neutrino equations).  P = [W_cn;
+// G_n^T] M^-1 is kept as well, so that a stage solve forms the reduced right-hand side from b
+// in one pass.
 template <int MC, int NT>
 __device__ __forceinline__ void reduce_dense(const PLay &L, Ws &W, int tid) {
   constexpr int NQ = (MC - NCORE) / 3;
@@ -1021,6 +1025,20 @@ __device__ __forceinline__ void reduce_dense(const PLay &L, Ws &W, int tid) {
     }
     W.R[e] = v;
   }
+  for (int e = tid; e < NRED * NN; e += NT) {  // P = [W_cn; G_n^T] M^-1, bin by bin
+    const int i = e / NN, jn = e - i * NN;
+    const int l = jn / NQ, q = jn - l * NQ;
+    const double *mi = W.Minv + 9 * q + l;  // column l of M_q^-1
+    double a0, a1, a2;
+    if (i < NCORE) {
+      const double *row = W.S + core_index(i, NQ) * m + 15 + q;
+      a0 = row[0]; a1 = row[NQ]; a2 = row[2 * NQ];
+    } else {
+      const int t = i - NCORE;
+      a0 = W.G[2 * (15 + q) + t]; a1 = W.G[2 * (15 + NQ + q) + t]; a2 = W.G[2 * (15 + 2 * NQ + q) + t];
+    }
+    W.P[e] = a0 * mi[0] + a1 * mi[3] + a2 * mi[6];
+  }
   __syncthreads();
 }
 
@@ -1051,7 +1069,7 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
       const int rq = r - 15;
       if (MC > 0 && col < m && rq >= 0 && rq < 3 * nq && !(ncol && rq % nq == cq % nq)) continue;
       // a hierarchy's l = 3 entry (a chain head) appears only in that hierarchy's l = 2 row
-      if (col >= m && r != W.cpar[col - m]) { W.Jh[(col - m) * m + r] = 0.0; continue; }
+      if (col >= m && r != W.cpar[col - m]) continue;
       double v = core_row<double>(W.dmap[r], b, L, k, g, mt, ev, vsel);
       if (col < m) W.S[r * m + col] = ((r == col) ? gi : 0.0) - v;  // W_DD = I/(gamma h) - J_DD
       else W.Jh[(col - m) * m + r] = v;
@@ -1087,13 +1105,11 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
   __syncthreads();
   for (int i = tid; i < n; i += NT)
     if (W.rk[i] >= 0) W.rtd[i] = 1.0 / W.td[i];
-  // Schur correction of the parent diagonal column: S[:, p_c] -= Jh_c * g_c * lo_1c
-  for (int r = tid; r < m; r += NT) {
-    for (int c = 0; c < nch; ++c) {
-      int h = W.chead[c];
-      double gcoef = W.tlo[h] / W.td[h];
-      W.S[r * m + W.cpar[c]] -= W.Jh[c * m + r] * gcoef;
-    }
+  // Schur correction of the parent diagonal column, S[:, p_c] -= Jh_c * g_c * lo_1c, whose one
+  // nonzero entry is the parent's diagonal (only that entry of Jh_c is formed)
+  for (int c = tid; c < nch; c += NT) {
+    const int h = W.chead[c], pr = W.cpar[c];
+    W.S[pr * m + pr] -= W.Jh[c * m + pr] * (W.tlo[h] / W.td[h]);
   }
   W00 = gi - W.J0[0];
   __syncthreads();
@@ -1167,8 +1183,10 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
 template <int MC>
 __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double x0, double *v, int tid) {
   const int m = L.m, nch = L.nch;
-  // bottom-up sweep of each chain, one lane per chain, with the -J0 x0 elimination of row 0
-  // applied on the way; the next element's operands are loaded before the current store
+  // Phase 1: bottom-up sweep of each chain, one lane per chain, with the -J0 x0 elimination of
+  // row 0 applied on the way; the next element's operands are loaded before the current store.
+  // A chain head couples to its parent row alone, so the chain's lane also forms that dense
+  // row's right-hand side b (row 0 and the chain heads eliminated); other lanes form the rest.
   for (int c = tid; c < nch; c += WAVE) {
     const int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
     int ii = h + (len - 1) * s;
@@ -1194,73 +1212,71 @@ __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double x0, double *
       ii = in;
     }
     v[ii] = x;
-    W.chx[c] = x * W.rtd[h];  // chain head over its pivot, used by every dense row below
+    const int pr = W.cpar[c], ip = W.dmap[pr];
+    W.xd[MC > 0 ? pr : W.pi[pr]] = (v[ip] + W.J0[ip] * x0) + W.Jh[c * m + pr] * (x * W.rtd[h]);
+  }
+  for (int r = tid; r < m; r += WAVE) {
+    if (W.pch[r] >= 0) continue;
+    const int i = W.dmap[r];
+    W.xd[MC > 0 ? r : W.pi[r]] = v[i] + W.J0[i] * x0;
   }
   wave_sync();
   if constexpr (MC > 0) {
-    // bordered solve (see reduce_dense): dense right-hand side, neutrino blocks, the reduced
-    // 19 x 19 system, then the neutrino unknowns from h' and eta'
+    // bordered solve (see reduce_dense)
     constexpr int NQ = (MC - NCORE) / 3;
     constexpr int NN = 3 * NQ;
-    for (int r = tid; r < MC; r += WAVE) {
-      const int i = W.dmap[r];
-      double acc = v[i] + W.J0[i] * x0;
-      for (int c = 0; c < nch; ++c) acc += W.Jh[c * MC + r] * W.chx[c];
-      W.xd[r] = acc;
-    }
-    wave_sync();
-    for (int j = tid; j < NN; j += WAVE) {  // M^-1 b_n
-      const int l = j / NQ, q = j - l * NQ;
-      const double *mi = W.Minv + 9 * q + 3 * l;
-      W.rn[j] = mi[0] * W.xd[15 + q] + mi[1] * W.xd[15 + NQ + q] + mi[2] * W.xd[15 + 2 * NQ + q];
-    }
-    wave_sync();
-    for (int i = tid; i < NRED; i += WAVE) {  // reduced right-hand side, through the pivot permutation
-      double acc;
-      if (i < NCORE) {
-        const int ci = core_index(i, NQ);
-        const double *row = W.S + ci * MC + 15;
-        acc = W.xd[ci];
-        for (int jn = 0; jn < NN; ++jn) acc -= row[jn] * W.rn[jn];
+    // Phase 2: M^-1 b_n (items 0 .. NN-1) and the reduced right-hand side (the next NRED
+    // items, stored through the pivot permutation): b_c - P_c b_n for the core rows and
+    // P_h b_n for h' and eta', one partial sum per l block of b_n
+    for (int e = tid; e < NN + NRED; e += WAVE) {
+      if (e < NN) {
+        const int l = e / NQ, q = e - l * NQ;
+        const double *mi = W.Minv + 9 * q + 3 * l;
+        W.rn[e] = mi[0] * W.xd[15 + q] + mi[1] * W.xd[15 + NQ + q] + mi[2] * W.xd[15 + 2 * NQ + q];
       } else {
-        acc = 0.0;
-        for (int jn = 0; jn < NN; ++jn) acc += W.G[2 * (15 + jn) + (i - NCORE)] * W.rn[jn];
+        const int i = e - NN;
+        const double *row = W.P + i * NN;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          a0 += row[q] * W.xd[15 + q];
+          a1 += row[NQ + q] * W.xd[15 + NQ + q];
+          a2 += row[2 * NQ + q] * W.xd[15 + 2 * NQ + q];
+        }
+        const double pb = (a0 + a1) + a2;
+        W.xr[W.pi[i]] = (i < NCORE) ? W.xd[core_index(i, NQ)] - pb : pb;
       }
-      W.xr[W.pi[i]] = acc;
     }
     wave_sync();
+    // Phase 3: the reduced solve, core unknowns and z = (h', eta')
     for (int i = tid; i < NRED; i += WAVE) {
       const double *row = W.Sinv + W.rmap[i] * NRED;
       double acc = 0.0;
       for (int j = 0; j < NRED; ++j) acc += row[j] * W.xr[j];
       if (i < NCORE) v[W.dmap[core_index(i, NQ)]] = acc;
-      else W.xr[NRED + i - NCORE] = acc;  // h', eta'
+      else W.xr[NRED + i - NCORE] = acc;
     }
-    if (tid == 0) v[0] = x0;  // read by no one before the workgroup barrier that follows the solve
-    wave_sync();
-    for (int j = tid; j < NN; j += WAVE) v[W.dmap[15 + j]] = W.rn[j] + W.Q[2 * j] * W.xr[NRED] + W.Q[2 * j + 1] * W.xr[NRED + 1];
-    wave_sync();
   } else {
-    // Schur right-hand side of the dense block (row 0 and the chain heads eliminated)
-    for (int r = tid; r < m; r += WAVE) {
-      const int i = W.dmap[r];
-      double acc = v[i] + W.J0[i] * x0;
-      for (int c = 0; c < nch; ++c) acc += W.Jh[c * m + r] * W.chx[c];
-      W.xd[W.pi[r]] = acc;
-    }
-    wave_sync();
+    // Phase 2: the dense solve, x_D = W_DD^-1 b through the pivot permutation
     for (int r = tid; r < m; r += WAVE) {
       double acc = 0.0;
       const double *row = W.Sinv + W.rmap[r] * m;
       for (int j = 0; j < m; ++j) acc += row[j] * W.xd[j];
       v[W.dmap[r]] = acc;
     }
-    if (tid == 0) v[0] = x0;  // read by no one before the workgroup barrier that follows the solve
-    wave_sync();
   }
-  for (int c = tid; c < nch; c += WAVE) {  // top-down sweep, four elements at a time as above
+  if (tid == 0) v[0] = x0;  // read by no one before the workgroup barrier that follows the solve
+  wave_sync();
+  // Last phase: the neutrino unknowns x_n = M^-1 b_n + Q z, and the top-down sweeps of the
+  // chains, four elements at a time as above; a chain whose parent is a neutrino row forms that
+  // unknown itself with the same expression
+  auto xn = [&](int j) { return fma(W.Q[2 * j + 1], W.xr[NRED + 1], fma(W.Q[2 * j], W.xr[NRED], W.rn[j])); };
+  if constexpr (MC > 0)
+    for (int j = tid; j < MC - NCORE; j += WAVE) v[W.dmap[15 + j]] = xn(j);
+  for (int c = tid; c < nch; c += WAVE) {
     const int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
-    const double xp = v[W.dmap[W.cpar[c]]];
+    const int pr = W.cpar[c];
+    const double xp = (MC > 0 && pr >= 15 && pr < MC - 2) ? xn(pr - 15) : v[W.dmap[pr]];
     double x = (v[h] + W.tlo[h] * xp) * W.rtd[h];
     int ii = h;
     int j = 1;
@@ -1709,8 +1725,8 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
 
 // Layout of the state vector for the truncations in prob (NQ > 0: compile-time bin count) and
 // the carve-up of the dynamic LDS into the solver workspace, which must match
-// deb_perturb_lds_bytes(): 22 n + 2 m^2 + nch m + 8 m + nch + 21 NQMAX + 24 + 2 NRED^2 doubles and a PBgShared, then m + 4 n + 256 + 4 nch + 3 m
-// ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
+// deb_perturb_lds_bytes(): 22 n + 2 m^2 + nch m + 8 m + 3 NRED NQMAX + 21 NQMAX + 24 + 2 NRED^2 doubles and a PBgShared, then
+// m + 4 n + 256 + 4 nch + 4 m ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
 template <int NQ, int NT>
 __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const double *__restrict__ nu_qw, long lds_bytes, PLay &L, Ws &W,
                                              int tid) {
@@ -1725,7 +1741,7 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   double *p = smem;
   W.y0 = p; p += n; W.y1 = p; p += n; W.u = p; p += n; W.f = p; p += n; W.J0 = p; p += n; W.dT = p; p += n;
   W.K = p; p += 8 * n; W.tlo = p; p += n; W.tdi = p; p += n; W.tup = p; p += n; W.tmul = p; p += n; W.td = p; p += n;
-  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m; W.chx = p; p += nch;
+  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m; W.P = p; p += NRED * 3 * NQMAX;
   W.S2 = p; p += (size_t)m * m; W.snap = p; p += 5 * (size_t)m; W.tca = p; p += n; W.tcb = p; p += n;
   W.G = p; p += 2 * (size_t)m; W.Minv = p; p += 9 * NQMAX; W.Q = p; p += 6 * NQMAX; W.rn = p; p += 3 * NQMAX; W.xr = p; p += 24;
   W.R = p; p += NRED * NRED; W.R2 = p; p += NRED * NRED;
@@ -1734,7 +1750,7 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
-  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m; W.rmap = ip; ip += m; W.rs = ip; ip += n; W.rbin = ip; ip += n; W.perm = ip; ip += n + 256;
+  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m; W.rmap = ip; ip += m; W.rs = ip; ip += n; W.rbin = ip; ip += n; W.perm = ip; ip += n + 256; W.pch = ip; ip += m;
   ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;  // 8-byte alignment for the cache
   W.sc = (SplCache *)ip; ip += sizeof(SplCache) / 4;
   if ((long)((char *)ip - (char *)smem) > lds_bytes) return false;
@@ -1745,6 +1761,12 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   }
   L.q = sq; L.w = sw; L.dlfdlq = sdl;
   setup_maps<NT>(L, W, tid);
+  for (int r = tid; r < m; r += NT) {  // the chain table was published by a barrier in setup_maps
+    int c = -1;
+    for (int j = 0; j < nch; ++j)
+      if (W.cpar[j] == r) c = j;
+    W.pch[r] = c;
+  }
   return true;
 }
 
@@ -1973,8 +1995,9 @@ __global__ void tau_out_kernel(const double *__restrict__ tables, long tstride, 
 extern "C" long deb_perturb_lds_bytes(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
   long n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + (long)nqmax * (lmaxnu + 1) + 2;
   long m = 17 + 3 * (long)nqmax, nch = 3 + nqmax;
-  long dbl = n * 22 + 2 * m * m + nch * m + 8 * m + nch + 21 * deb::NQMAX + 24 + 2 * deb::NRED * deb::NRED + (long)(sizeof(deb::PBgShared) / 8);
-  long ints = m + 4 * n + 256 + 4 * nch + 3 * m + 1;
+  long dbl = n * 22 + 2 * m * m + nch * m + 8 * m + 3 * deb::NRED * deb::NQMAX + 21 * deb::NQMAX + 24 + 2 * deb::NRED * deb::NRED +
+             (long)(sizeof(deb::PBgShared) / 8);
+  long ints = m + 4 * n + 256 + 4 * nch + 4 * m + 1;
   return dbl * 8 + ints * 4 + (long)sizeof(deb::SplCache);
 }
 
