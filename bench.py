@@ -194,10 +194,13 @@ def main():
         torch.cuda.synchronize()
         if (st != 0).any():
             raise RuntimeError(f"solver failures in warm-up: status {st.cpu().numpy().tolist()}")
+    step_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        ts = time.perf_counter()
         st, na, nr, pk = one_step(True)
         torch.cuda.synchronize()
+        step_ms.append(1e3 * (time.perf_counter() - ts))
         kernel_ms.append(ev0.elapsed_time(ev1))
         steps_attempted.append(int((na + nr).sum().item()))
     torch.cuda.synchronize()
@@ -218,7 +221,7 @@ def main():
                                "(-0.95, 0.1), lmax 12, rtol 1e-5 atol 1e-10, P(k) at z=2,1,0",
                    "k_modes": NK, "nvar": n, "dense_block": m, "num_thermo": 256, "solver": "Rodas5, per-mode adaptive",
                    "launch": {"order": "largest k first", "threads_per_unit": deb._lib.lib().deb_default_threads_per_unit()},
-                   "all_modes_ok": ok, "gradients_per_s": None},
+                   "all_modes_ok": ok, "gradients_per_s": None, "step_ms": [round(v, 2) for v in step_ms]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": measured_traffic("perturb_kernel<8, 256> 512 k-modes nq 8 lmax_nu 17 rtol 1e-5"),

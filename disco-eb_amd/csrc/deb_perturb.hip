@@ -550,22 +550,50 @@ __device__ __forceinline__ void tail_coef(int ch, int l, const PLay &L, double k
 // per-unit LDS workspace (shared by the workgroup's threads)
 struct Ws {
   double *y0, *y1, *u, *f, *J0, *dT, *K, *S, *Jh, *tlo, *tdi, *tup, *tmul, *td, *xd;
-  double *S2, *snap, *Sinv;  // second matrix buffer, pivot snapshots, the buffer holding the inverse
+  double *Sinv;         // the buffer holding the dense (or reduced) inverse
   int *dmap, *rk, *chead, *cstride, *clen, *cpar, *piv, *pi, *rmap;
   int *rs, *rbin;       // per row: chain stride (tail rows), momentum bin (massive-neutrino rows)
   int *perm;            // RHS row of each thread slot (slot tid + it NT), -1 for none
+  int *perm1;           // the same for the NT - 64 threads of waves 1..
   int *pch;             // per dense row: the chain whose head couples to it (its l = 2 parent), -1 for none
   double *tca, *tcb;    // per tail row l < lmax: l / (2l + 1) and -(l + 1) / (2l + 1)
   // bordered solve of the dense block (fixed block sizes): metric derivatives G (2 per dense
   // column), per-bin 3x3 inverses, Q = M^-1 U, the neutrino part of a right-hand side, the
-  // reduced right-hand side and solution, the reduced matrix in two buffers
-  double *G, *Minv, *Q, *rn, *xr, *R, *R2;
+  // reduced right-hand side and solution, the reduced matrix (inverted in place)
+  double *G, *Minv, *Q, *rn, *xr, *R;
   double *P;            // [W_cn; G_n^T] M^-1 (NRED x 3 nq): the reduced right-hand side straight from b
   double *rtd;          // 1 / td of the chain pivots, for the stage solves
   PBgShared *pg;        // background of the next stage, evaluated during the current stage solve
   SplCache *sc;
 };
 
+
+// Row order of the RHS evaluation for nthreads threads (slot t + it nthreads of thread t): a wave
+// pays for every row formula any of its lanes takes, so each class of core rows (row 0 and rows
+// 1..9; polarisation, massless-neutrino and dark-energy rows l <= 2; massive-neutrino rows
+// l <= 2) gets a wave-sized block of slots of its own where there are enough, and the tail
+// rows fill the rest.  perm holds n + 256 entries; run by one thread.
+__device__ __forceinline__ void build_perm(const PLay &L, const Ws &W, int *perm, int nthreads) {
+  const int n = L.n, nq = L.nq;
+  const int nit = (n + nthreads - 1) / nthreads, slots = nit * nthreads, nb = slots / WAVE;
+  for (int t = 0; t < slots; ++t) perm[t] = -1;
+  auto cls = [&](int i) {
+    if (W.rk[i] >= 0) return 3;
+    if (i <= 9) return 0;
+    if (i >= L.iq0 && i < L.iq0 + 3 * nq) return 2;
+    return 1;
+  };
+  const int home[3] = {nb - 1, 0, nb > 2 ? 1 : 0};
+  int used[64];  // slots taken per block (nb <= 64 for n <= 4096 rows)
+  for (int bk = 0; bk < nb; ++bk) used[bk] = 0;
+  for (int c = 0; c < 4; ++c)
+    for (int i = 0; i < n; ++i) {
+      if (cls(i) != c) continue;
+      int bk = (c < 3) ? home[c] : 0;
+      while (used[bk] == WAVE) bk = (bk + 1) % nb;
+      perm[bk * WAVE + used[bk]++] = i;
+    }
+}
 
 // rows of the tail chains: rk[i] = (chain << 8) | l, -1 for rows in the dense block or row 0
 template <int NT>
@@ -603,43 +631,28 @@ __device__ __forceinline__ void setup_maps(const PLay &L, Ws &W, int tid) {
       W.tcb[i] = -(double)(l + 1) * inv;
     }
   __syncthreads();
-  // Row order of the RHS evaluation: a wave pays for every row formula any of its lanes takes,
-  // so each class of core rows (row 0 and rows 1..9; polarisation, massless-neutrino and
-  // dark-energy rows l <= 2; massive-neutrino rows l <= 2) gets a wave-sized block of slots
-  // of its own where there are enough, and the tail rows fill the rest.
+  // row orders for the whole workgroup and, with more than one wavefront, for waves 1.. alone
+  // (they evaluate the first stage's rows while wave 0 inverts the dense block)
   if (tid == 0) {
-    const int nit = (n + NT - 1) / NT, slots = nit * NT, nb = slots / WAVE;
-    for (int t = 0; t < slots; ++t) W.perm[t] = -1;
-    auto cls = [&](int i) {
-      if (W.rk[i] >= 0) return 3;
-      if (i <= 9) return 0;
-      if (i >= L.iq0 && i < L.iq0 + 3 * nq) return 2;
-      return 1;
-    };
-    const int home[3] = {nb - 1, 0, nb > 2 ? 1 : 0};
-    int used[64];  // slots taken per block (nb <= 64 for n <= 4096 rows)
-    for (int bk = 0; bk < nb; ++bk) used[bk] = 0;
-    for (int c = 0; c < 4; ++c)
-      for (int i = 0; i < n; ++i) {
-        if (cls(i) != c) continue;
-        int bk = (c < 3) ? home[c] : 0;
-        while (used[bk] == WAVE) bk = (bk + 1) % nb;
-        W.perm[bk * WAVE + used[bk]++] = i;
-      }
+    build_perm(L, W, W.perm, NT);
+    if (NT > WAVE) build_perm(L, W, W.perm1, NT - WAVE);
   }
   __syncthreads();
 }
 
-// full RHS f(tau, u) into out[] (lanes over rows) with the background g at (tau, u[0]); u in LDS
-template <int NT>
-__device__ __forceinline__ void rhs_rows(const BgView &b, const PLay &L, double k, const PBg<double> &g, const double *u, double *out, Ws &W,
-                                         int tid) {
+
+// full RHS f(tau, u) (lanes over rows) with the background g at (tau, u[0]); u in LDS.  NT
+// threads t = 0 .. NT-1 (whole wavefronts) take the rows in the order of perm (built for NT
+// threads by build_perm); emit(i, f_i) receives each row.
+template <int NT, class E>
+__device__ __forceinline__ void rhs_rows(const BgView &b, const PLay &L, double k, const PBg<double> &g, const double *u, const int *perm,
+                                         Ws &W, int tid, const E &emit) {
   auto yv = [&](int i) { return u[i]; };
   Met<double> m;
   metric<double>(b, L, k, g, yv, m);
   const int nit = (L.n + NT - 1) / NT;
   for (int it = 0; it < nit; ++it) {  // a uniform trip count: every lane takes part in the shuffle
-    const int i = W.perm[tid + it * NT];
+    const int i = perm[tid + it * NT];
     const int ic = i >= 0 ? i : 0;
     const double vsh = __shfl(g.vl, VQ_LANE0 + W.rbin[ic]);  // this row's momentum-bin factor
     auto vrow = [&](int) { return vsh; };
@@ -655,7 +668,7 @@ __device__ __forceinline__ void rhs_rows(const BgView &b, const PLay &L, double 
         const double yn = (up != 0.0) ? u[i + s] : 0.0;
         r = lo * u[i - s] + di * u[i] + up * yn;
       }
-      out[i] = r;
+      emit(i, r);
     }
   }
 }
@@ -664,7 +677,7 @@ template <int NT>
 __device__ __forceinline__ void rhs_eval(const BgView &b, const PLay &L, double k, double tau, const double *u, double *out, Ws &W, int tid) {
   PBg<double> g;
   pbg_eval_lanes(b, L, k, tau, u[0], g, W.sc, tid);
-  rhs_rows<NT>(b, L, k, g, u, out, W, tid);
+  rhs_rows<NT>(b, L, k, g, u, W.perm, W, tid, [&](int i, double r) { out[i] = r; });
 }
 
 // d f / da (into outa) and d f / dtau (into outt) at (tau, y): one pass with two dual directions
@@ -708,230 +721,65 @@ __device__ __forceinline__ void rhs_dual(const BgView &b, const PLay &L, double 
   }
 }
 
-}  // namespace
-}  // namespace deb
-// Wave-wide unsigned 64-bit maximum from the ROCm device library (DPP reduction).
-extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(unsigned long long);
-namespace deb {
-namespace {
-
-constexpr int GJ_CHUNK = 9;  // entries per thread loaded together in the Gauss-Jordan updates
-
-// In-place Gauss-Jordan inversion of the MC x MC row-major matrix S (LDS), MC <= 64 known at
-// compile time, with partial pivoting, by a workgroup of NT threads.  Per pivot, wave 0 finds
-// the pivot row (one wave reduction over a 64-bit key: |S[r][p]| with its six lowest mantissa
-// bits replaced by 63 - r, so ties and near-ties go to the first row), moves row p to the pivot
-// row's slot and snapshots the scaled pivot row and column p; then every thread updates its
-// share of the MC^2 entries.  Row interchanges leave the inverse with permuted columns; the
-// composite permutation pi (inverse[:, j] = S[:, pi[j]]) is built across lanes with readlane.
-template <int MC, int NT>
-__device__ __forceinline__ void gauss_jordan_fixed(double *S, int *piv, int *pi, int *rmap, double *prow, double *colp, int tid) {
-  static_assert(MC <= WAVE, "one pivot-column entry per lane");
-  constexpr int NE = (MC * MC + NT - 1) / NT;
-  const int lane = tid & (WAVE - 1);
-  for (int p = 0; p < MC; ++p) {
-    double ip = 0.0;
-    if (NT == WAVE || tid < WAVE) {  // wave 0
-      unsigned long long key = 0;
-      if (lane >= p && lane < MC) {
-        const double v = fabs(S[lane * MC + p]);
-        key = ((unsigned long long)__double_as_longlong(v) & ~63ull) | (unsigned long long)(63 - lane);
-      }
-      key = __ockl_wfred_max_u64(key);  // all 64 lanes of the wave active here
-      const int bi = 63 - (int)(key & 63ull);
-      if (lane == 0) piv[p] = bi;
-      ip = 1.0 / S[bi * MC + p];
-      if (lane < MC) {
-        const int j = lane;
-        const double vp = S[p * MC + j], vb = S[bi * MC + j];
-        prow[j] = (j == p) ? ip : vb * ip;
-        S[bi * MC + j] = vp;                                       // old row p moves to the pivot slot
-        colp[j] = (j == bi) ? S[p * MC + p] : S[j * MC + p];       // column p, after the interchange
-      }
-    }
-    __syncthreads();
-    if (NT > WAVE) ip = prow[p];
-    // each thread reads a chunk of its entries before writing them (loads are not held up by stores)
-#pragma unroll 1
-    for (int i0 = 0; i0 < NE; i0 += GJ_CHUNK) {
-      double a[GJ_CHUNK], cr[GJ_CHUNK], pc[GJ_CHUNK];
-#pragma unroll
-      for (int i = 0; i < GJ_CHUNK; ++i) {
-        const int e = tid + (i0 + i) * NT;
-        if (i0 + i < NE && e < MC * MC) {
-          const int r = e / MC, c = e - r * MC;
-          a[i] = S[e];
-          cr[i] = colp[r];
-          pc[i] = prow[c];
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < GJ_CHUNK; ++i) {
-        const int e = tid + (i0 + i) * NT;
-        if (i0 + i < NE && e < MC * MC) {
-          const int r = e / MC, c = e - r * MC;
-          const double v = (c == p) ? -cr[i] * ip : a[i] - cr[i] * pc[i];
-          S[e] = (r == p) ? pc[i] : v;
-        }
-      }
-    }
-    __syncthreads();
-  }
-  if (NT == WAVE || tid < WAVE) {  // wave 0
-    int cur = lane;  // pi[lane]
-    for (int p = MC - 1; p >= 0; --p) {
-      const int q = __builtin_amdgcn_readfirstlane(piv[p]);
-      const int at_q = __builtin_amdgcn_readlane(cur, q), at_p = __builtin_amdgcn_readlane(cur, p);
-      cur = (lane == p) ? at_q : ((lane == q) ? at_p : cur);
-    }
-    if (lane < MC) { pi[lane] = cur; rmap[lane] = lane; }
-  }
-  __syncthreads();
+// Wave-wide maximum of a 64-bit unsigned value, returned to every lane: DPP row shifts by 1, 2,
+// 4 and 8 leave each row's maximum in its lane 15, row broadcasts of lanes 15 and 31 carry it
+// to lane 63, which readlane returns.  Lanes a DPP move does not write keep their own value.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ unsigned long long umax_dpp(unsigned long long v) {
+  const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
+  const unsigned tlo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, 0xf, false);
+  const unsigned thi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, 0xf, false);
+  const unsigned long long t = ((unsigned long long)thi << 32) | tlo;
+  return t > v ? t : v;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+  v = umax_dpp<0x111, 0xf>(v);  // row_shr:1
+  v = umax_dpp<0x112, 0xf>(v);  // row_shr:2
+  v = umax_dpp<0x114, 0xf>(v);  // row_shr:4
+  v = umax_dpp<0x118, 0xf>(v);  // row_shr:8
+  v = umax_dpp<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v = umax_dpp<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, WAVE - 1), hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), WAVE - 1);
+  return ((unsigned long long)hi << 32) | lo;
 }
 
-// Gauss-Jordan inversion for two or more wavefronts, with partial pivoting (first maximum among
-// the rows not yet used as pivots) and no row interchanges: step p eliminates column p with the
-// pivot in row piv[p].  The matrix alternates between buffers A and B (step p reads one and
-// writes the other), so while waves 1.. apply step p, wave 0 already prepares step p + 1 from
-// the buffer being read: it forms column p + 1 and the pivot row as step p leaves them, picks
-// the pivot and publishes the scaled pivot row, column p + 1 and 1 / pivot.  One barrier per
-// step.  Each entry is computed by the same expressions as in the interchanging variant.  The
-// result R holds row p of the inverse of the row-permuted matrix in row piv[p]: x = A^-1 b is
-// x[p] = sum_c R[piv[p]][c] b[piv[c]], so rmap[p] = piv[p] and the right-hand side is scattered
-// through pi = piv^-1.  snap: 5 MC doubles (two pivot rows, two columns, 1 / pivot per step).
-// Returns the buffer holding R.
-template <int MC, int NT>
-__device__ __forceinline__ double *gauss_jordan_lookahead(double *A, double *B, int *piv, int *pi, int *rmap, double *snap, int tid) {
-  static_assert(MC <= WAVE && NT > WAVE, "one pivot-column entry per lane; one wave prepares, the others update");
-  constexpr int NU = NT - WAVE;  // threads of waves 1..
-  constexpr int NE = (MC * MC + NU - 1) / NU;
-  const int lane = tid & (WAVE - 1);
-  double *prow0 = snap, *colp0 = snap + 2 * MC, *ipv = snap + 4 * MC;
-  // wave 0 keeps the step it last prepared in registers: lane r holds entry r of the scaled
-  // pivot row (wrow) and of the pivot column (wcol); wbi, wip are the pivot row and 1 / pivot
-  bool used = false;  // lane r: row r has been a pivot row
-  double wrow = 0.0, wcol = 0.0, wip = 0.0;
-  int wbi = 0;
-  // pivot search over column values col (lane r holds row r) among the rows not yet used;
-  // rc = 1 / col, formed alongside so the division overlaps the reduction
-  auto choose = [&](double col, double rc, double &ipn) {
-    unsigned long long key = 0;
-    if (lane < MC && !used) key = ((unsigned long long)__double_as_longlong(fabs(col)) & ~63ull) | (unsigned long long)(63 - lane);
-    key = __ockl_wfred_max_u64(key);  // all 64 lanes of wave 0 active
-    const int bn = 63 - (int)(key & 63ull);
-    ipn = lane_bcast(rc, bn);
-    return bn;
-  };
-  if (tid < WAVE) {  // step 0 from the matrix as built
-    const double col = (lane < MC) ? A[lane * MC] : 1.0;
-    const int bi = choose(col, 1.0 / col, wip);
-    if (lane < MC) {
-      wrow = (lane == 0) ? wip : A[bi * MC + lane] * wip;
-      wcol = col;
-      prow0[lane] = wrow;
-      colp0[lane] = wcol;
+// Gauss-Jordan inversion of the N x N row-major matrix A (LDS, N <= 64 known at compile time)
+// by one wavefront, lane r holding row r in registers: no barriers and no LDS traffic between
+// pivots.  Pivot p is the not-yet-used row with the largest |entry| in column p: one wave maximum
+// of a 64-bit key, the magnitude's bit pattern with its six lowest bits replaced by 63 - row, so
+// that ties and near-ties go to the first row; rows are never moved.  Each step replaces column p by the multipliers, in place: after the last step
+// row rmap[p] (the row that pivoted on column p) gives unknown p,
+//   x[p] = sum_c Ainv[rmap[p]][c] b[rmap[c]],
+// so a right-hand side is scattered through pi = rmap^-1.  Ainv may alias A (each lane reads
+// its own row before writing it).
+template <int N>
+__device__ __forceinline__ void gauss_jordan_wave(const double *A, double *Ainv, int *rmap, int *pi, int lane) {
+  static_assert(N <= WAVE, "one row per lane");
+  double a[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) a[j] = (lane < N) ? A[lane * N + j] : 0.0;
+  bool used = lane >= N;
+#pragma unroll
+  for (int p = 0; p < N; ++p) {
+    const unsigned long long mag = (unsigned long long)__double_as_longlong(a[p]) & 0x7fffffffffffffc0ull;
+    const unsigned long long key = used ? 0ull : (mag | (unsigned long long)(WAVE - 1 - lane));
+    const int bi = WAVE - 1 - (int)(wave_max_u64(key) & (WAVE - 1));
+    const double ip = 1.0 / lane_bcast(a[p], bi);
+    const bool me = lane == bi;
+    const double f = a[p];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      if (j == p) continue;
+      const double pj = lane_bcast(a[j], bi) * ip;
+      a[j] = me ? pj : a[j] - f * pj;
     }
-    used = (lane == bi);
-    wbi = bi;
-    if (lane == 0) { piv[0] = bi; ipv[0] = wip; }
+    a[p] = me ? ip : -f * ip;
+    used = used || me;
+    if (lane == 0) { rmap[p] = bi; pi[bi] = p; }
   }
-  __syncthreads();
-  // rows and columns of this thread's entries (waves 1..), fixed across the steps, when they
-  // fit in one chunk of registers
-  constexpr bool FIXED = NE <= GJ_CHUNK;
-  int er[FIXED ? NE : 1], ec[FIXED ? NE : 1];
-  if constexpr (FIXED) {
+  if (lane < N)
 #pragma unroll
-    for (int i = 0; i < NE; ++i) {
-      const int e = (tid - WAVE) + i * NU;
-      er[i] = e / MC;
-      ec[i] = e - er[i] * MC;
-    }
-  }
-  const bool last_ok = (tid - WAVE) + (NE - 1) * NU < MC * MC;  // the last entry exists
-  double *cur = A, *nxt = B;
-  for (int p = 0; p < MC; ++p) {
-    if (FIXED && tid >= WAVE) {
-      const double *prow = prow0 + (p & 1) * MC, *colp = colp0 + (p & 1) * MC;
-      const int bi = piv[p];
-      const double ip = ipv[p];
-      // all loads of this thread's entries first, then its stores (the buffers are not known
-      // to the compiler to be distinct, so interleaving would serialise on LDS latency)
-      double a[FIXED ? NE : 1], cr[FIXED ? NE : 1], pc[FIXED ? NE : 1];
-#pragma unroll
-      for (int i = 0; i < NE; ++i) {
-        if (i < NE - 1 || last_ok) {
-          a[i] = cur[(tid - WAVE) + i * NU];
-          cr[i] = colp[er[i]];
-          pc[i] = prow[ec[i]];
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < NE; ++i) {
-        if (i < NE - 1 || last_ok) {
-          const double v = (ec[i] == p) ? -cr[i] * ip : a[i] - cr[i] * pc[i];
-          nxt[(tid - WAVE) + i * NU] = (er[i] == bi) ? pc[i] : v;
-        }
-      }
-    } else if (tid >= WAVE) {
-      const double *prow = prow0 + (p & 1) * MC, *colp = colp0 + (p & 1) * MC;
-      const int bi = piv[p];
-      const double ip = ipv[p];
-      // loads of a chunk of this thread's entries first, then its stores
-#pragma unroll 1
-      for (int i0 = 0; i0 < NE; i0 += GJ_CHUNK) {
-        double a[GJ_CHUNK], cr[GJ_CHUNK], pc[GJ_CHUNK];
-#pragma unroll
-        for (int i = 0; i < GJ_CHUNK; ++i) {
-          const int e = (tid - WAVE) + (i0 + i) * NU;
-          if (i0 + i < NE && e < MC * MC) {
-            const int r = e / MC, c = e - r * MC;
-            a[i] = cur[e];
-            cr[i] = colp[r];
-            pc[i] = prow[c];
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < GJ_CHUNK; ++i) {
-          const int e = (tid - WAVE) + (i0 + i) * NU;
-          if (i0 + i < NE && e < MC * MC) {
-            const int r = e / MC, c = e - r * MC;
-            const double v = (c == p) ? -cr[i] * ip : a[i] - cr[i] * pc[i];
-            nxt[e] = (r == bi) ? pc[i] : v;
-          }
-        }
-      }
-    } else if (p + 1 < MC) {  // wave 0: step p + 1, from the registers of step p
-      double *prown = prow0 + ((p + 1) & 1) * MC, *colpn = colp0 + ((p + 1) & 1) * MC;
-      const int q = p + 1, bi = wbi;
-      const double ip = wip;
-      const double pq = lane_bcast(wrow, q);  // scaled pivot row, column q
-      double col = 1.0;  // column q after step p
-      if (lane < MC) col = (lane == bi) ? pq : cur[lane * MC + q] - wcol * pq;
-      double ipn;
-      const int bn = choose(col, 1.0 / col, ipn);
-      const double cbn = lane_bcast(wcol, bn);  // column p entry of the new pivot row
-      if (lane < MC) {  // pivot row bn after step p (bn != bi), scaled
-        const double v = (lane == p) ? -cbn * ip : cur[bn * MC + lane] - cbn * wrow;
-        wrow = (lane == q) ? ipn : v * ipn;
-        wcol = col;
-        prown[lane] = wrow;
-        colpn[lane] = wcol;
-      }
-      used = used || (lane == bn);
-      wbi = bn;
-      wip = ipn;
-      if (lane == 0) { piv[q] = bn; ipv[q] = ipn; }
-    }
-    __syncthreads();
-    double *t = cur; cur = nxt; nxt = t;
-  }
-  if (tid < MC) {
-    rmap[tid] = piv[tid];
-    pi[piv[tid]] = tid;
-  }
-  __syncthreads();
-  return cur;
+    for (int j = 0; j < N; ++j) Ainv[lane * N + j] = a[j];
 }
 
 // Inverse of a 3x3 matrix by Gauss-Jordan with partial pivoting, in registers.
@@ -1043,10 +891,12 @@ __device__ __forceinline__ void reduce_dense(const PLay &L, Ws &W, int tid) {
 }
 
 // Jacobian pieces at (t0, y0) and the structured factorisation of W = I/(gamma dt) - J
-template <int MC, int NT>
-// g receives the background at (t0, y0[0]), which is also the first stage's.
+// g receives the background at (t0, y0[0]), which is also the first stage's.  With the bordered
+// solve and more than one wavefront, waves 1.. run first(tid - 64) while wave 0 inverts the
+// reduced matrix (g and W00 are set by then).
+template <int MC, int NT, class F>
 __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double k, double t0, double dt, Ws &W, int tid, double &W00,
-                                          PBg<double> &g) {
+                                          PBg<double> &g, const F &first) {
   const int n = L.n, m = L.m, nch = L.nch;
   const double gi = 1.0 / (dt * r5::gamma);
   rhs_dual<NT>(b, L, k, t0, W.y0, W.J0, W.dT, W, tid);
@@ -1113,18 +963,18 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
   }
   W00 = gi - W.J0[0];
   __syncthreads();
-  // In-place Gauss-Jordan inversion with partial pivoting (first maximum).  Per pivot: wave 0
-  // moves row p to the pivot row's slot and snapshots the scaled pivot row and column p, then
-  // one element-parallel pass updates all m^2 entries.  Row interchanges leave the inverse with
-  // permuted columns; the composite permutation pi is applied to the right-hand side in wsolve.
+  // Dense block: with a compile-time bin count, the bordered reduction and a one-wavefront
+  // Gauss-Jordan of the 19 x 19 reduced matrix.  Otherwise in-place Gauss-Jordan of the m x m
+  // block with partial pivoting (first maximum): per pivot, wave 0 moves row p to the pivot
+  // row's slot and snapshots the scaled pivot row and column p, then one element-parallel pass
+  // updates all m^2 entries.  Row interchanges leave the inverse with permuted columns; the
+  // composite permutation pi is applied to the right-hand side in wsolve.
   if constexpr (MC > 0) {
     reduce_dense<MC, NT>(L, W, tid);
-    if constexpr (NT > WAVE) {
-      W.Sinv = gauss_jordan_lookahead<NRED, NT>(W.R, W.R2, W.piv, W.pi, W.rmap, W.snap, tid);
-    } else {
-      gauss_jordan_fixed<NRED, NT>(W.R, W.piv, W.pi, W.rmap, W.u, W.f, tid);
-      W.Sinv = W.R;
-    }
+    if (tid < WAVE) gauss_jordan_wave<NRED>(W.R, W.R, W.rmap, W.pi, tid);
+    else if constexpr (NT > WAVE) first(tid - WAVE);
+    W.Sinv = W.R;
+    __syncthreads();
   } else {
     W.Sinv = W.S;
     double *prow = W.u, *colp = W.f;  // free during the factorisation
@@ -1532,52 +1382,66 @@ __device__ __forceinline__ double stage_in(int s, double y0i, double ui, const K
 template <int NQ, int NT>
 __device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, double k, double tprev, double dt, Ws &W, int tid) {
   constexpr int MC = NQ > 0 ? 17 + 3 * NQ : 0;
+  // the first stage's right-hand side is formed beside the inversion of the reduced matrix
+  constexpr bool FIRST_BESIDE_GJ = MC > 0 && NT > WAVE;
   const int n = L.n;
   double *K = W.K;
-  double *u = W.u, *f = W.f, *y0 = W.y0, *dT = W.dT;
+  double *u = W.u, *y0 = W.y0, *dT = W.dT;
   double W00;
   PBg<double> g;
-  factorise<MC, NT>(b, L, k, tprev, dt, W, tid, W00, g);
   using namespace r5;
+  // right-hand side of stage s, row i, from f_i: dts df/dtau (stages 1-5) and sum_j (C_sj / dt) k_j
+  auto assemble = [&](int s, int i, double r, double dts) {
+    switch (s) {
+      case 0: r = r + dts * dT[i]; break;
+      case 1: r = r + dts * dT[i] + (C21 / dt) * K[i]; break;
+      case 2: r = r + dts * dT[i] + ((C31 / dt) * K[i] + (C32 / dt) * K[n + i]); break;
+      case 3: r = r + dts * dT[i] + ((C41 / dt) * K[i] + (C42 / dt) * K[n + i] + (C43 / dt) * K[2 * n + i]); break;
+      case 4:
+        r = r + dts * dT[i] + ((C51 / dt) * K[i] + (C52 / dt) * K[n + i] + (C53 / dt) * K[2 * n + i] + (C54 / dt) * K[3 * n + i]);
+        break;
+      case 5:
+        r = r + ((C61 / dt) * K[i] + (C62 / dt) * K[n + i] + (C63 / dt) * K[2 * n + i] + (C64 / dt) * K[3 * n + i] +
+                 (C65 / dt) * K[4 * n + i]);
+        break;
+      case 6:
+        r = r + ((C71 / dt) * K[i] + (C72 / dt) * K[n + i] + (C73 / dt) * K[2 * n + i] + (C74 / dt) * K[3 * n + i] +
+                 (C75 / dt) * K[4 * n + i] + (C76 / dt) * K[5 * n + i]);
+        break;
+      default:
+        r = r + ((C81 / dt) * K[i] + (C82 / dt) * K[n + i] + (C83 / dt) * K[2 * n + i] + (C84 / dt) * K[3 * n + i] +
+                 (C85 / dt) * K[4 * n + i] + (C86 / dt) * K[5 * n + i] + (C87 / dt) * K[6 * n + i]);
+        break;
+    }
+    return r;
+  };
+  // each row's stage right-hand side goes to k_s as the row is evaluated; row 0 also gives x0
+  auto emit_stage = [&](int s, double dts) {
+    double *ks = K + s * n;
+    return [&, s, dts, ks](int i, double r) {
+      const double v = assemble(s, i, r, dts);
+      ks[i] = v;
+      if (i == 0) W.pg->x0 = v / W00;
+    };
+  };
+  auto first = [&](int t) {
+    rhs_rows<(NT > WAVE ? NT - WAVE : WAVE)>(b, L, k, g, y0, W.perm1, W, t, emit_stage(0, dt * d1));
+  };
+  factorise<MC, NT>(b, L, k, tprev, dt, W, tid, W00, g, first);
   for (int s = 0; s < 8; ++s) {
     double *ks = K + s * n;
-    for (int i = tid; i < n; i += NT) u[i] = stage_in(s, y0[i], u[i], [&](int j) { return K[j * n + i]; });
-    __syncthreads();
     // df/dtau weight, selected without a run-time-indexed array (scratch memory)
     const double dts = dt * ((s == 0) ? d1 : (s == 1) ? d2 : (s == 2) ? d3 : (s == 3) ? d4 : d5);
-    if (s > 0) {
-      if constexpr (NT == WAVE) pbg_eval_lanes(b, L, k, stage_time(s, tprev, dt), u[0], g, W.sc, tid);
-      else pbg_load(g, W.pg, L, tid & (WAVE - 1));
-    }
-    rhs_rows<NT>(b, L, k, g, u, f, W, tid);
-    __syncthreads();
-    for (int i = tid; i < n; i += NT) {
-      double r = f[i];
-      switch (s) {
-        case 0: r = r + dts * dT[i]; break;
-        case 1: r = r + dts * dT[i] + (C21 / dt) * K[i]; break;
-        case 2: r = r + dts * dT[i] + ((C31 / dt) * K[i] + (C32 / dt) * K[n + i]); break;
-        case 3: r = r + dts * dT[i] + ((C41 / dt) * K[i] + (C42 / dt) * K[n + i] + (C43 / dt) * K[2 * n + i]); break;
-        case 4:
-          r = r + dts * dT[i] + ((C51 / dt) * K[i] + (C52 / dt) * K[n + i] + (C53 / dt) * K[2 * n + i] + (C54 / dt) * K[3 * n + i]);
-          break;
-        case 5:
-          r = r + ((C61 / dt) * K[i] + (C62 / dt) * K[n + i] + (C63 / dt) * K[2 * n + i] + (C64 / dt) * K[3 * n + i] +
-                   (C65 / dt) * K[4 * n + i]);
-          break;
-        case 6:
-          r = r + ((C71 / dt) * K[i] + (C72 / dt) * K[n + i] + (C73 / dt) * K[2 * n + i] + (C74 / dt) * K[3 * n + i] +
-                   (C75 / dt) * K[4 * n + i] + (C76 / dt) * K[5 * n + i]);
-          break;
-        default:
-          r = r + ((C81 / dt) * K[i] + (C82 / dt) * K[n + i] + (C83 / dt) * K[2 * n + i] + (C84 / dt) * K[3 * n + i] +
-                   (C85 / dt) * K[4 * n + i] + (C86 / dt) * K[5 * n + i] + (C87 / dt) * K[6 * n + i]);
-          break;
+    if (!(FIRST_BESIDE_GJ && s == 0)) {
+      for (int i = tid; i < n; i += NT) u[i] = stage_in(s, y0[i], u[i], [&](int j) { return K[j * n + i]; });
+      __syncthreads();
+      if (s > 0) {
+        if constexpr (NT == WAVE) pbg_eval_lanes(b, L, k, stage_time(s, tprev, dt), u[0], g, W.sc, tid);
+        else pbg_load(g, W.pg, L, tid & (WAVE - 1));
       }
-      ks[i] = r;
-      if (i == 0) W.pg->x0 = r / W00;
+      rhs_rows<NT>(b, L, k, g, u, W.perm, W, tid, emit_stage(s, dts));
+      __syncthreads();
     }
-    __syncthreads();
     if (tid < WAVE) {
       wsolve<MC>(L, W, W.pg->x0, ks, tid);
     } else if constexpr (NT > WAVE) {
@@ -1696,6 +1560,7 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
         __syncthreads();
         ++isave;
       }
+      __syncthreads();  // every wave has read y0 for the error norm before it is replaced
       for (int i = tid; i < n; i += NT) y0[i] = W.y1[i];
       ++na;
     } else {
@@ -1725,8 +1590,8 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
 
 // Layout of the state vector for the truncations in prob (NQ > 0: compile-time bin count) and
 // the carve-up of the dynamic LDS into the solver workspace, which must match
-// deb_perturb_lds_bytes(): 22 n + 2 m^2 + nch m + 8 m + 3 NRED NQMAX + 21 NQMAX + 24 + 2 NRED^2 doubles and a PBgShared, then
-// m + 4 n + 256 + 4 nch + 4 m ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
+// deb_perturb_lds_bytes(): 22 n + m^2 + nch m + 3 m + 3 NRED NQMAX + 21 NQMAX + 24 + NRED^2 doubles and a PBgShared, then
+// m + 5 n + 512 + 4 nch + 4 m ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
 template <int NQ, int NT>
 __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const double *__restrict__ nu_qw, long lds_bytes, PLay &L, Ws &W,
                                              int tid) {
@@ -1742,15 +1607,15 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   W.y0 = p; p += n; W.y1 = p; p += n; W.u = p; p += n; W.f = p; p += n; W.J0 = p; p += n; W.dT = p; p += n;
   W.K = p; p += 8 * n; W.tlo = p; p += n; W.tdi = p; p += n; W.tup = p; p += n; W.tmul = p; p += n; W.td = p; p += n;
   W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m; W.P = p; p += NRED * 3 * NQMAX;
-  W.S2 = p; p += (size_t)m * m; W.snap = p; p += 5 * (size_t)m; W.tca = p; p += n; W.tcb = p; p += n;
+  W.tca = p; p += n; W.tcb = p; p += n;
   W.G = p; p += 2 * (size_t)m; W.Minv = p; p += 9 * NQMAX; W.Q = p; p += 6 * NQMAX; W.rn = p; p += 3 * NQMAX; W.xr = p; p += 24;
-  W.R = p; p += NRED * NRED; W.R2 = p; p += NRED * NRED;
+  W.R = p; p += NRED * NRED;
   W.rtd = p; p += n;
   W.pg = (PBgShared *)p; p += sizeof(PBgShared) / 8;
   double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
-  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m; W.rmap = ip; ip += m; W.rs = ip; ip += n; W.rbin = ip; ip += n; W.perm = ip; ip += n + 256; W.pch = ip; ip += m;
+  W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m; W.rmap = ip; ip += m; W.rs = ip; ip += n; W.rbin = ip; ip += n; W.perm = ip; ip += n + 256; W.perm1 = ip; ip += n + 256; W.pch = ip; ip += m;
   ip += (2 - ((long)((char *)ip - (char *)smem) / 4) % 2) % 2;  // 8-byte alignment for the cache
   W.sc = (SplCache *)ip; ip += sizeof(SplCache) / 4;
   if ((long)((char *)ip - (char *)smem) > lds_bytes) return false;
@@ -1995,9 +1860,9 @@ __global__ void tau_out_kernel(const double *__restrict__ tables, long tstride, 
 extern "C" long deb_perturb_lds_bytes(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
   long n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + (long)nqmax * (lmaxnu + 1) + 2;
   long m = 17 + 3 * (long)nqmax, nch = 3 + nqmax;
-  long dbl = n * 22 + 2 * m * m + nch * m + 8 * m + 3 * deb::NRED * deb::NQMAX + 21 * deb::NQMAX + 24 + 2 * deb::NRED * deb::NRED +
+  long dbl = n * 22 + m * m + nch * m + 3 * m + 3 * deb::NRED * deb::NQMAX + 21 * deb::NQMAX + 24 + deb::NRED * deb::NRED +
              (long)(sizeof(deb::PBgShared) / 8);
-  long ints = m + 4 * n + 256 + 4 * nch + 4 * m + 1;
+  long ints = m + 5 * n + 512 + 4 * nch + 4 * m + 1;
   return dbl * 8 + ints * 4 + (long)sizeof(deb::SplCache);
 }
 
