@@ -134,24 +134,35 @@ __device__ void ionization(const BgView &b, const RfConst &K, const T &a, const 
   f[2] = f2 * dzda;
 }
 
+// value of x on lane src (uniform src) of the calling wavefront
+__device__ __forceinline__ double lane_read(double x, int src) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src), hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // One GRKT4 step (ode_integrators_stiff.py:166-281, GRK4A coefficients); every stage uses the
-// vector field at t0.
-__device__ void grkt4_step(const BgView &b, const RfConst &K, double t0, double dt, const double *y0, double *y1, double *yerr) {
+// vector field at t0.  Called by a whole wavefront with the same arguments on every lane: lane
+// j < 3 carries the dual direction of y_j, so one dual evaluation gives the Jacobian (column j
+// read from lane j) and, in its value part, the first stage's vector field.
+__device__ void grkt4_step(const BgView &b, const RfConst &K, double t0, double dt, const double *y0, double *y1, double *yerr, int lane) {
   const double gamma = 0.395, g21 = -0.767672395484, g31 = -0.851675323742, g32 = 0.522967289188;
   const double g41 = 0.288463109545, g42 = 0.880214273381e-1, g43 = -0.337389840627;
   const double a21 = 0.438, a31 = 0.796920457938, a32 = 0.730795420615e-1;
   const double ch1 = 0.346325833758, ch2 = 0.285693175712, ch3 = 0.367980990530;
   const double c1 = 0.199293275701, c2 = 0.482645235674, c3 = 0.680614886256e-1, c4 = 0.25;
-  double J[3][3];
+  double J[3][3], r[3];
   {
-    Dn<3> yd[3], fd[3];
+    Dual yd[3], fd[3];
     for (int i = 0; i < 3; ++i) {
-      yd[i] = Dn<3>(y0[i]);
-      yd[i].d[i] = 1.0;
+      yd[i] = Dual(y0[i]);
+      yd[i].d[0] = (lane == i) ? 1.0 : 0.0;
     }
-    ionization<Dn<3>>(b, K, Dn<3>(t0), yd, fd);
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) J[i][j] = dt * fd[i].d[j];
+    ionization<Dual>(b, K, Dual(t0), yd, fd);
+    for (int i = 0; i < 3; ++i) {
+      r[i] = fd[i].v * dt;
+      for (int j = 0; j < 3; ++j) J[i][j] = dt * lane_read(fd[i].d[0], j);
+    }
   }
   double A00 = 1.0 - gamma * J[0][0], A01 = -gamma * J[0][1], A02 = -gamma * J[0][2];
   double A10 = -gamma * J[1][0], A11 = 1.0 - gamma * J[1][1], A12 = -gamma * J[1][2];
@@ -170,8 +181,7 @@ __device__ void grkt4_step(const BgView &b, const RfConst &K, double t0, double 
     ionization<double>(b, K, t0, y, f);
     f[0] *= dt; f[1] *= dt; f[2] *= dt;
   };
-  double r[3], k1[3], k2[3], k3[3], k4[3], yy[3], f[3], Jk1[3], Jk2[3], Jk3[3];
-  vf(y0, r);
+  double k1[3], k2[3], k3[3], k4[3], yy[3], f[3], Jk1[3], Jk2[3], Jk3[3];
   solve(r, k1);
   for (int i = 0; i < 3; ++i) yy[i] = y0[i] + a21 * k1[i];
   vf(yy, f);
@@ -194,7 +204,8 @@ __device__ void grkt4_step(const BgView &b, const RfConst &K, double t0, double 
 
 // Adaptive GRKT4 over [astart, aend] with the default I-controller (safety 0.9, factor clip
 // [0.2 (1 on accept), 10], error order 3), rtol 1e-3, atol 1e-6, at most 128 steps, RMS norm.
-__device__ bool solve_ionization(const BgView &b, const RfConst &K, double astart, double aend, const double *ys, double *ye) {
+// Run by a whole wavefront (see grkt4_step).
+__device__ bool solve_ionization(const BgView &b, const RfConst &K, double astart, double aend, const double *ys, double *ye, int lane) {
   const double rtol = 1e-3, atol = 1e-6;
   double y[3] = {ys[0], ys[1], ys[2]};
   double tprev = astart, t1 = aend;
@@ -202,7 +213,7 @@ __device__ bool solve_ionization(const BgView &b, const RfConst &K, double astar
   int steps = 0;
   while (tprev < t1 && steps < 128) {
     double y1[3], err[3];
-    grkt4_step(b, K, tprev, tnext - tprev, y, y1, err);
+    grkt4_step(b, K, tprev, tnext - tprev, y, y1, err, lane);
     bool nan = isnan(y1[0]) || isnan(y1[1]) || isnan(y1[2]);
     double s = 0.0;
     for (int i = 0; i < 3; ++i) {
@@ -381,10 +392,11 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
     }
   }
   __syncthreads();
-  if (tid == 0) {
-    // tau_max by Romberg over [amin, amax] (background.py:181-187)
-    t[DEB_HDR_TAUMAX] = b.taumin + romb_dtauda(b, 1e-9, 1.01);
-    // RECFAST interval loop (thermodynamics_recfast.py:392-452)
+  // tau_max by Romberg over [amin, amax] (background.py:181-187), beside the RECFAST loop
+  if (tid == 64) t[DEB_HDR_TAUMAX] = b.taumin + romb_dtauda(b, 1e-9, 1.01);
+  if (tid < 64) {
+    // RECFAST interval loop (thermodynamics_recfast.py:392-452) on wavefront 0 (every lane runs
+    // the same sequence; lanes 0-2 carry the Jacobian directions), lane 0 writes
     const double *aexp = arr(DEB_A_AEXP);
     double *xH = arr(DEB_A_XEHI), *xHe = arr(DEB_A_XEHEI), *Tm = arr(DEB_A_TM);
     double *d0 = arr(DEB_A_SCR0), *d1 = arr(DEB_A_SCR1), *d2 = arr(DEB_A_SCR2);
@@ -414,7 +426,7 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
         out[0] = 1.0; out[1] = (xHe0 - 1.0) / fHe; out[2] = Tcmb * zz; out[3] = 0.0; out[4] = dxHeIdz * dzda; out[5] = -Tcmb * zz;
       } else {
         double ye[3], f[3];
-        if (!solve_ionization(b, K, astart, aend, yp, ye)) ++fails;
+        if (!solve_ionization(b, K, astart, aend, yp, ye, tid)) ++fails;
         ionization<double>(b, K, aend, ye, f);
         out[0] = ye[0]; out[1] = ye[1]; out[2] = ye[2]; out[3] = f[0]; out[4] = f[1]; out[5] = f[2];
         if (i > 0 && yp[0] > 0.99) {  // hydrogen Saha (f_case3)
@@ -429,10 +441,10 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
           out[0] = xH0; out[3] = dxHdz * dzda;
         }
       }
-      xH[i] = out[0]; xHe[i] = out[1]; Tm[i] = out[2]; d0[i] = out[3]; d1[i] = out[4]; d2[i] = out[5];
+      if (tid == 0) { xH[i] = out[0]; xHe[i] = out[1]; Tm[i] = out[2]; d0[i] = out[3]; d1[i] = out[4]; d2[i] = out[5]; }
       for (int j = 0; j < 6; ++j) yp[j] = out[j];
     }
-    t[DEB_HDR_RECFAST_FAIL] = (double)fails;
+    if (tid == 0) t[DEB_HDR_RECFAST_FAIL] = (double)fails;
   }
   __syncthreads();
   // evaluate_thermo per knot (thermodynamics_recfast.py:456-492)

@@ -65,10 +65,10 @@ def _accepted_mesh(log, nacc, t1):
     return torch.tensor(mesh), torch.tensor(na, dtype=torch.int32)
 
 
-# Launch of the replayed shifted cosmologies (twelve times the k-modes): largest k first, four
-# wavefronts per unit (for 12 x 256 units on one MI355X: 436 ms per gradient at 256 threads,
-# 486 ms at 128, 701 ms at 64).
-REPLAY_SCHEDULE = {"order": "k_desc", "threads": 256}
+# Launch of the replayed shifted cosmologies (twelve times the k-modes): largest k first, two
+# wavefronts per unit, so that two units share a compute unit (for 12 x 256 units on one
+# MI355X: 274 ms per gradient at 128 threads, 415 ms at 256, 376 ms at 64).
+REPLAY_SCHEDULE = {"order": "k_desc", "threads": 128}
 
 
 def pk_jacobian(theta, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, num_thermo=256, schedule=None, lmaxg=12, lmaxgp=12, lmaxr=12,
