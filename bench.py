@@ -175,7 +175,6 @@ def main():
     aout = torch.tensor(AOUT, dtype=torch.float64, device=dev)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     kernel_ms = []
-    steps_attempted = []
 
     def one_step(record):
         tables = deb.background_tables([P], 256)
@@ -195,6 +194,8 @@ def main():
         if (st != 0).any():
             raise RuntimeError(f"solver failures in warm-up: status {st.cpu().numpy().tolist()}")
     step_ms = []
+    counts = []
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
@@ -202,9 +203,11 @@ def main():
         torch.cuda.synchronize()
         step_ms.append(1e3 * (time.perf_counter() - ts))
         kernel_ms.append(ev0.elapsed_time(ev1))
-        steps_attempted.append(int((na + nr).sum().item()))
+        counts.append((na, nr))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # step counts after the timed region (a first reduction would load its kernel inside it)
+    steps_attempted = [int((na + nr).sum().item()) for na, nr in counts]
     ok = bool((st == 0).all())
     ms_per_step = 1e3 * wall / args.steps
     value = NK * args.steps / wall
