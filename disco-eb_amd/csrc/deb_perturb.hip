@@ -577,19 +577,24 @@ __device__ __forceinline__ void build_perm(const PLay &L, const Ws &W, int *perm
   const int n = L.n, nq = L.nq;
   const int nit = (n + nthreads - 1) / nthreads, slots = nit * nthreads, nb = slots / WAVE;
   for (int t = 0; t < slots; ++t) perm[t] = -1;
+  // classes: 0 rows 0..9; 1 and 4 the other core rows (the first three polarisation rows and
+  // the first massless-neutrino row, then the rest), split over two blocks; 2 massive-neutrino
+  // rows l <= 2; 3 tail rows
   auto cls = [&](int i) {
     if (W.rk[i] >= 0) return 3;
     if (i <= 9) return 0;
     if (i >= L.iq0 && i < L.iq0 + 3 * nq) return 2;
-    return 1;
+    return (i <= L.idxr) ? 1 : 4;
   };
-  const int home[3] = {nb - 1, 0, nb > 2 ? 1 : 0};
+  const int home[5] = {nb - 1, 0, nb > 2 ? 1 : 0, 0, nb > 3 ? 2 : 0};
   int used[64];  // slots taken per block (nb <= 64 for n <= 4096 rows)
   for (int bk = 0; bk < nb; ++bk) used[bk] = 0;
-  for (int c = 0; c < 4; ++c)
+  const int order[5] = {0, 1, 4, 2, 3};
+  for (int oc = 0; oc < 5; ++oc)
     for (int i = 0; i < n; ++i) {
+      const int c = order[oc];
       if (cls(i) != c) continue;
-      int bk = (c < 3) ? home[c] : 0;
+      int bk = home[c];
       while (used[bk] == WAVE) bk = (bk + 1) % nb;
       perm[bk * WAVE + used[bk]++] = i;
     }
@@ -1026,25 +1031,26 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
   }
 }
 
-// solve W x = v in place (v in LDS, length n), x0 = v[0] / W_00 given: wavefront 0 alone (tid
+// solve W x = v in place (v in LDS, length n), x0 = v[0] / W_00 given and the other rows of v
+// already holding b_i + J_i0 x0 (row 0 eliminated): wavefront 0 alone (tid
 // is its lane), its phases ordered by wave_sync; the caller's workgroup barrier publishes x.
 // Every phase has at most 64 items (chains, dense rows, reduced rows) but the dense rows of a
 // 16-bin grid, so the other wavefronts are free for other work meanwhile.
 template <int MC>
 __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double x0, double *v, int tid) {
   const int m = L.m, nch = L.nch;
-  // Phase 1: bottom-up sweep of each chain, one lane per chain, with the -J0 x0 elimination of
-  // row 0 applied on the way; the next element's operands are loaded before the current store.
+  // Phase 1: bottom-up sweep of each chain, one lane per chain; the next element's operands are
+  // loaded before the current store.
   // A chain head couples to its parent row alone, so the chain's lane also forms that dense
   // row's right-hand side b (row 0 and the chain heads eliminated); other lanes form the rest.
   for (int c = tid; c < nch; c += WAVE) {
     const int h = W.chead[c], s = W.cstride[c], len = W.clen[c];
     int ii = h + (len - 1) * s;
-    double x = v[ii] + W.J0[ii] * x0;
+    double x = v[ii];
     int j = len - 2;
     for (; j >= 3; j -= 4) {  // four elements: all operands loaded, then the serial recurrence
       const int i0 = ii - s, i1 = ii - 2 * s, i2 = ii - 3 * s, i3 = ii - 4 * s;
-      const double a0 = v[i0] + W.J0[i0] * x0, a1 = v[i1] + W.J0[i1] * x0, a2 = v[i2] + W.J0[i2] * x0, a3 = v[i3] + W.J0[i3] * x0;
+      const double a0 = v[i0], a1 = v[i1], a2 = v[i2], a3 = v[i3];
       const double m0 = W.tmul[i0], m1 = W.tmul[i1], m2 = W.tmul[i2], m3 = W.tmul[i3];
       v[ii] = x;
       const double x1 = a0 - m0 * x, x2 = a1 - m1 * x1, x3 = a2 - m2 * x2;
@@ -1056,19 +1062,18 @@ __device__ __forceinline__ void wsolve(const PLay &L, Ws &W, double x0, double *
     }
     for (; j >= 0; --j) {
       const int in = ii - s;
-      const double vn = v[in], jn = W.J0[in], mn = W.tmul[in];
+      const double vn = v[in], mn = W.tmul[in];
       v[ii] = x;
-      x = (vn + jn * x0) - mn * x;
+      x = vn - mn * x;
       ii = in;
     }
     v[ii] = x;
     const int pr = W.cpar[c], ip = W.dmap[pr];
-    W.xd[MC > 0 ? pr : W.pi[pr]] = (v[ip] + W.J0[ip] * x0) + W.Jh[c * m + pr] * (x * W.rtd[h]);
+    W.xd[MC > 0 ? pr : W.pi[pr]] = v[ip] + W.Jh[c * m + pr] * (x * W.rtd[h]);
   }
   for (int r = tid; r < m; r += WAVE) {
     if (W.pch[r] >= 0) continue;
-    const int i = W.dmap[r];
-    W.xd[MC > 0 ? r : W.pi[r]] = v[i] + W.J0[i] * x0;
+    W.xd[MC > 0 ? r : W.pi[r]] = v[W.dmap[r]];
   }
   wave_sync();
   if constexpr (MC > 0) {
@@ -1415,13 +1420,20 @@ __device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, do
     }
     return r;
   };
-  // each row's stage right-hand side goes to k_s as the row is evaluated; row 0 also gives x0
+  // Each row's stage right-hand side goes to k_s as the row is evaluated, with row 0 already
+  // eliminated: row 0 of W is W_00 alone, so x0 = b_0 / W_00 and b_i + J_i0 x0 is the right-hand
+  // side of the rest.  Row 0's f is g.aH g.a (core_row), so every thread forms x0 itself.
   auto emit_stage = [&](int s, double dts) {
     double *ks = K + s * n;
-    return [&, s, dts, ks](int i, double r) {
+    const double x0 = assemble(s, 0, g.aH * g.a, dts) / W00;
+    return [&, s, dts, ks, x0](int i, double r) {
       const double v = assemble(s, i, r, dts);
-      ks[i] = v;
-      if (i == 0) W.pg->x0 = v / W00;
+      if (i == 0) {
+        ks[0] = v;
+        W.pg->x0 = x0;
+      } else {
+        ks[i] = v + W.J0[i] * x0;
+      }
     };
   };
   auto first = [&](int t) {
