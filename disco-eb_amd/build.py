@@ -24,8 +24,11 @@ def _hipcc():
 
 
 def _flags():
+    # -ffp-contract=on: a multiply-add is fused only within one source expression, so every
+    # instantiation of a kernel template (threads per unit) rounds the same expressions the same
+    # way; the default lets the backend fuse across statements depending on each one's dataflow.
     return ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-munsafe-fp-atomics", "-Wno-unused-result", "-I", CSRC]
+            "-ffp-contract=on", "-munsafe-fp-atomics", "-Wno-unused-result", "-I", CSRC]
 
 
 def build(verbose=False):

@@ -118,7 +118,9 @@ template <int N> __device__ __forceinline__ Dn<N> lane_bcast(const Dn<N> &x, int
 template <class T> struct PBg {
   T a, tau, aH, aHH, cs2, xe, xeprime, opac, tauc, taucprime, pb43, rho_Q, w_Q, ca2_Q;
   T amnu_a;  // a m_nu / T_nu (in units of the momentum grid)
+  T ia, ia2, iaH, ipb, ipbm;  // 1 / a, 1 / a^2, 1 / max(aH, 1e-30), 1 / (1 + pb43), 1 / max(pb43, 1e-30)
   T vl;      // lane VQ_LANE0 + q: velocity factor of bin q (q < nq), computed once per evaluation
+  T ivl;     // the same lanes: its reciprocal, eps / q = sqrt(1 + (a m / q)^2)
   bool rsa, ufa, tca;
   // velocity factor q / eps of momentum bin q
   __device__ __forceinline__ T vq(const PLay &L, int q) const {
@@ -127,6 +129,7 @@ template <class T> struct PBg {
   }
   // the same, read from lane VQ_LANE0 + q: q uniform across the wave
   __device__ __forceinline__ T vq_u(int q) const { return lane_bcast(vl, VQ_LANE0 + q); }
+  __device__ __forceinline__ T ivq_u(int q) const { return lane_bcast(ivl, VQ_LANE0 + q); }
   // the same for a bin that may differ between lanes (selected among the broadcasts; an array
   // indexed by a run-time bin would live in scratch memory)
   __device__ __forceinline__ T vq_l(const PLay &L, int q) const {
@@ -260,9 +263,16 @@ __device__ __forceinline__ void pbg_eval(const BgView &b, const PLay &L, double 
   bool uncoupled = (tc / fmax(tk, 1e-30) > b.tca_k) || ((tc / fmax(th, 1e-30) > b.tca_h) && (tc / fmax(tk, 1e-30) > 0.1 * b.tca_k));
   g.tca = !g.rsa && b.use_tca && !uncoupled;
   g.amnu_a = a * b.amnu;
+  g.ia = 1.0 / a;
+  g.ia2 = 1.0 / a2;
+  g.iaH = 1.0 / dmax(g.aH, 1e-30);
+  g.ipb = 1.0 / (1.0 + g.pb43);
+  g.ipbm = 1.0 / dmax(g.pb43, 1e-30);
   const int lane = tid & (WAVE - 1);  // all lanes take part here: the bins are read from lanes VQ_LANE0..
   const int qb = lane - VQ_LANE0;
-  g.vl = g.vq(L, (qb >= 0 && qb < L.nq) ? qb : 0);
+  const T aq = g.amnu_a / L.q[(qb >= 0 && qb < L.nq) ? qb : 0];
+  g.ivl = dsqrt(1.0 + aq * aq);
+  g.vl = 1.0 / g.ivl;
 }
 
 // pbg_eval for plain doubles with the independent transcendentals and divisions spread over
@@ -306,6 +316,8 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
     case 4: num = b.grhog + b.grhor * b.Neff; den = 3.0; break;
     case 5: num = g.xe * b.akthom; den = a2; break;
     case 6: num = 1.0; den = fmax(k, 1e-30); break;
+    case 7: num = 1.0; den = a; break;
+    case 8: num = 1.0; den = a2; break;
     default: if (binlane) { num = g.amnu_a; den = L.q[qb]; } break;
   }
   double r = num / den;
@@ -315,6 +327,8 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
   const double gin = lane_bcast(r, 4);
   g.opac = lane_bcast(r, 5);
   const double tk = lane_bcast(r, 6);
+  g.ia = lane_bcast(r, 7);
+  g.ia2 = lane_bcast(r, 8);
   const double aq = r;  // bin lanes
   const double grho = t1 + t2 + b.grhom * b.OmegaDE * g.rho_Q * a2 + b.grhom * b.Omegak;
   // round 2
@@ -323,12 +337,17 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
     case 0: num = grho; den = 3.0; break;
     case 1: num = gin + b.grhor * b.Nmnu * pnu; den = a2; break;
     case 2: num = 1.0; den = fmax(g.opac, 1e-30); break;
+    case 3: num = 1.0; den = 1.0 + g.pb43; break;
+    case 4: num = 1.0; den = fmax(g.pb43, 1e-30); break;
     default: if (binlane) num = 1.0 + aq * aq; break;
   }
   r = num / den;
   const double gp1 = lane_bcast(r, 1);
   g.tauc = lane_bcast(r, 2);
+  g.ipb = lane_bcast(r, 3);
+  g.ipbm = lane_bcast(r, 4);
   const double sq = sqrt(r);  // lane 0: aH; bin lanes: sqrt(1 + aq^2)
+  g.ivl = sq;
   g.aH = lane_bcast(sq, 0);
   g.xeprime = dxe * g.aH;
   const double gpres = gp1 + g.w_Q * b.grhom * b.OmegaDE * g.rho_Q * a2;
@@ -349,6 +368,7 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
   const double w3 = lane_bcast(r, 0);
   g.taucprime = g.tauc * (2.0 * g.aH - lane_bcast(r, 1));
   const double th = lane_bcast(r, 2), r_tv = lane_bcast(r, 3), r_tk = lane_bcast(r, 4);
+  g.iaH = th;
   g.vl = r;  // bin lanes: 1 / sqrt(1 + aq^2)
   // round 4
   num = 1.0; den = 1.0;
@@ -371,8 +391,8 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
 // the uniform fields, the per-bin velocity factors (lanes VQ_LANE0 + q) and the switches.
 // x0 carries the row-0 unknown of the current stage solve.
 struct PBgShared {
-  double f[15];
-  double vl[NQMAX];
+  double f[20];
+  double vl[NQMAX], ivl[NQMAX];
   double x0;
   int flags;
 };
@@ -382,20 +402,24 @@ __device__ __forceinline__ void pbg_store(const PBg<double> &g, PBgShared *s, co
     s->f[0] = g.a; s->f[1] = g.tau; s->f[2] = g.aH; s->f[3] = g.aHH; s->f[4] = g.cs2; s->f[5] = g.xe; s->f[6] = g.xeprime;
     s->f[7] = g.opac; s->f[8] = g.tauc; s->f[9] = g.taucprime; s->f[10] = g.pb43; s->f[11] = g.rho_Q; s->f[12] = g.w_Q;
     s->f[13] = g.ca2_Q; s->f[14] = g.amnu_a;
+    s->f[15] = g.ia; s->f[16] = g.ia2; s->f[17] = g.iaH; s->f[18] = g.ipb; s->f[19] = g.ipbm;
     s->flags = (g.rsa ? 1 : 0) | (g.ufa ? 2 : 0) | (g.tca ? 4 : 0);
   }
   const int qb = lane - VQ_LANE0;
-  if (qb >= 0 && qb < L.nq) s->vl[qb] = g.vl;
+  if (qb >= 0 && qb < L.nq) { s->vl[qb] = g.vl; s->ivl[qb] = g.ivl; }
 }
 
 __device__ __forceinline__ void pbg_load(PBg<double> &g, const PBgShared *s, const PLay &L, int lane) {
   g.a = s->f[0]; g.tau = s->f[1]; g.aH = s->f[2]; g.aHH = s->f[3]; g.cs2 = s->f[4]; g.xe = s->f[5]; g.xeprime = s->f[6];
   g.opac = s->f[7]; g.tauc = s->f[8]; g.taucprime = s->f[9]; g.pb43 = s->f[10]; g.rho_Q = s->f[11]; g.w_Q = s->f[12];
   g.ca2_Q = s->f[13]; g.amnu_a = s->f[14];
+  g.ia = s->f[15]; g.ia2 = s->f[16]; g.iaH = s->f[17]; g.ipb = s->f[18]; g.ipbm = s->f[19];
   const int fl = s->flags;
   g.rsa = fl & 1; g.ufa = (fl & 2) != 0; g.tca = (fl & 4) != 0;
   const int qb = lane - VQ_LANE0;
-  g.vl = (qb >= 0 && qb < L.nq) ? s->vl[qb] : 0.0;  // only the bin lanes are ever read
+  const bool bin = qb >= 0 && qb < L.nq;  // only the bin lanes are ever read
+  g.vl = bin ? s->vl[qb] : 0.0;
+  g.ivl = bin ? s->ivl[qb] : 0.0;
 }
 
 // metric perturbations and effective (RSA/TCA) fields, perturbations.py:292-351,407-427
@@ -411,25 +435,26 @@ __device__ __forceinline__ void metric(const BgView &b, const PLay &L, double k,
   const double deltag = y(7), thetag = y(8), shearg = y(9) / 2.0;
   const double deltar = y(L.idxr), thetar = y(L.idxr + 1), shearr = y(L.idxr + 2) / 2.0;
   const double deltaq = y(n - 2), thetaq = y(n - 1);
+  const double ik2 = 1.0 / (k * k);
   T drhonu(0.0), dpnu(0.0), fnu(0.0), shearnu(0.0);
   for (int q = 0; q < nq; ++q) {
     double p0 = y(iq0 + q), p1 = y(iq0 + nq + q), p2 = y(iq0 + 2 * nq + q);
-    const T v = g.vq_u(q);
-    drhonu += L.w[q] * p0 / v;
+    const T v = g.vq_u(q), iv = g.ivq_u(q);
+    drhonu += L.w[q] * p0 * iv;
     dpnu += L.w[q] * p0 * v;
     fnu += T(L.w[q] * p1);
     shearnu += L.w[q] * p2 * v;
   }
-  dpnu = dpnu / 3.0;
-  shearnu = shearnu * 2.0 / 3.0;
+  dpnu = dpnu * (1.0 / 3.0);
+  shearnu = shearnu * (2.0 / 3.0);
   const T a = g.a, a2 = g.a * g.a;
   const double cs2_Q = b.cs2DE;
   T rpptq = (1.0 + g.w_Q) * g.rho_Q * b.grhom * b.OmegaDE * thetaq * a2;
   T de_rho = b.grhom * b.OmegaDE * deltaq * g.rho_Q * a2;
-  T dgrho = b.grhom * (Omegac * deltac + b.Omegab * deltab) / a + (b.grhog * deltag + b.grhor * (b.Neff * deltar + b.Nmnu * drhonu)) / a2 +
+  T dgrho = b.grhom * (Omegac * deltac + b.Omegab * deltab) * g.ia + (b.grhog * deltag + b.grhor * (b.Neff * deltar + b.Nmnu * drhonu)) * g.ia2 +
             de_rho;
-  T dgrho_wo = b.grhom * (Omegac * deltac + b.Omegab * deltab) / a + b.grhor * b.Nmnu * drhonu / a2 + de_rho;
-  T hp_rsa = (2.0 * k * k * eta + dgrho_wo) / dmax(g.aH, 1e-30);
+  T dgrho_wo = b.grhom * (Omegac * deltac + b.Omegab * deltab) * g.ia + b.grhor * b.Nmnu * drhonu * g.ia2 + de_rho;
+  T hp_rsa = (2.0 * k * k * eta + dgrho_wo) * g.iaH;
   if (g.rsa) {  // compute_fields_rsa (approximations.py:28-47)
     const double sk2 = fmax(k * k, 1e-30);
     T stc = dmax(g.tauc, 1e-30);
@@ -443,24 +468,24 @@ __device__ __forceinline__ void metric(const BgView &b, const PLay &L, double k,
     m.dg = T(deltag); m.tg = T(thetag); m.sg = T(shearg);
     m.dr = T(deltar); m.tr = T(thetar); m.sr = T(shearr);
   }
-  m.dgpres = (b.grhog * m.dg + b.grhor * b.Neff * m.dr) / a2 / 3.0 + b.grhor * b.Nmnu * dpnu / a2 +
-             (cs2_Q * de_rho + (cs2_Q - g.ca2_Q) * (3.0 * g.aH * rpptq / (k * k)));
-  T dgtheta = b.grhom * (Omegac * thetac + b.Omegab * thetab) / a + 4.0 / 3.0 * (b.grhog * m.tg + b.Neff * b.grhor * m.tr) / a2 +
-              b.Nmnu * b.grhor * k * fnu / a2 + rpptq;
+  m.dgpres = (b.grhog * m.dg + b.grhor * b.Neff * m.dr) * g.ia2 * (1.0 / 3.0) + b.grhor * b.Nmnu * dpnu * g.ia2 +
+             (cs2_Q * de_rho + (cs2_Q - g.ca2_Q) * (3.0 * g.aH * rpptq * ik2));
+  T dgtheta = b.grhom * (Omegac * thetac + b.Omegab * thetab) * g.ia + 4.0 / 3.0 * (b.grhog * m.tg + b.Neff * b.grhor * m.tr) * g.ia2 +
+              b.Nmnu * b.grhor * k * fnu * g.ia2 + rpptq;
   m.dgrho = dgrho;
-  m.hprime = g.rsa ? hp_rsa : (2.0 * k * k * eta + dgrho) / g.aH;
-  m.etaprime = 0.5 * dgtheta / (k * k);
-  m.alpha = (m.hprime + 6.0 * m.etaprime) / 2.0 / (k * k);
+  m.hprime = g.rsa ? hp_rsa : (2.0 * k * k * eta + dgrho) * g.iaH;
+  m.etaprime = 0.5 * dgtheta * ik2;
+  m.alpha = (m.hprime + 6.0 * m.etaprime) * 0.5 * ik2;
   if (g.tca) {  // _photon_tca (perturbations.py:407-435)
     const double k2 = k * k;
     m.dbp_tca = -thetab - 0.5 * m.hprime;
-    T tbp = (-g.aH * thetab + k2 * g.cs2 * deltab + k2 * g.pb43 * (0.25 * deltag - shearg)) / (1.0 + g.pb43);
+    T tbp = (-g.aH * thetab + k2 * g.cs2 * deltab + k2 * g.pb43 * (0.25 * deltag - shearg)) * g.ipb;
     m.dgp_tca = 4.0 / 3.0 * (-thetag - 0.5 * m.hprime);
-    T slip = 2.0 * g.pb43 / (1.0 + g.pb43) * g.aH * (thetab - thetag) +
-             g.tauc * (-g.aHH * thetab - 0.5 * g.aH * k2 * deltag + k2 * (g.cs2 * m.dbp_tca - 0.25 * m.dgp_tca)) / (1.0 + g.pb43);
-    tbp = tbp + g.pb43 / (1.0 + g.pb43) * slip;
+    T slip = 2.0 * g.pb43 * g.ipb * g.aH * (thetab - thetag) +
+             g.tauc * (-g.aHH * thetab - 0.5 * g.aH * k2 * deltag + k2 * (g.cs2 * m.dbp_tca - 0.25 * m.dgp_tca)) * g.ipb;
+    tbp = tbp + g.pb43 * g.ipb * slip;
     m.tbp_tca = tbp;
-    m.tgp_tca = (-tbp - g.aH * thetab + k2 * g.cs2 * deltab) / dmax(g.pb43, 1e-30) + k2 * (0.25 * deltag - shearg);
+    m.tgp_tca = (-tbp - g.aH * thetab + k2 * g.cs2 * deltab) * g.ipbm + k2 * (0.25 * deltag - shearg);
   }
 }
 
@@ -493,8 +518,8 @@ __device__ __forceinline__ T core_row(int i, const BgView &b, const PLay &L, dou
     double polter = y(9) + y(idxgp) + y(idxgp + 2);
     int l = i - idxgp;
     if (l == 0) return -k * y(idxgp + 1) - g.opac * y(idxgp) + g.opac * polter / 2.0;
-    if (l == 1) return k / 3.0 * (y(idxgp) - 2.0 * y(idxgp + 2)) - g.opac * y(idxgp + 1);
-    return k / 5.0 * (2.0 * y(idxgp + 1) - 3.0 * y(idxgp + 3)) - g.opac * y(idxgp + 2) + g.opac * polter / 10.0;
+    if (l == 1) return k * (1.0 / 3.0) * (y(idxgp) - 2.0 * y(idxgp + 2)) - g.opac * y(idxgp + 1);
+    return k * 0.2 * (2.0 * y(idxgp + 1) - 3.0 * y(idxgp + 3)) - g.opac * y(idxgp + 2) + g.opac * polter * 0.1;
   }
   if (i >= idxr && i <= idxr + 2) {  // massless neutrinos l = 0, 1, 2
     if (g.rsa) return T(0.0);
@@ -508,9 +533,9 @@ __device__ __forceinline__ T core_row(int i, const BgView &b, const PLay &L, dou
     int l = (i - iq0) / nq, q = (i - iq0) - l * nq;
     T kv = k * vrow(q);
     double dl = L.dlfdlq[q];
-    if (l == 0) return -kv * y(iq0 + nq + q) + m.hprime * dl / 6.0;
-    if (l == 1) return kv * (y(iq0 + q) - 2.0 * y(iq0 + 2 * nq + q)) / 3.0;
-    return kv * (2.0 * y(iq0 + nq + q) - 3.0 * y(iq0 + 3 * nq + q)) / 5.0 - (m.hprime / 15.0 + 2.0 / 5.0 * m.etaprime) * dl;
+    if (l == 0) return -kv * y(iq0 + nq + q) + m.hprime * dl * (1.0 / 6.0);
+    if (l == 1) return kv * (y(iq0 + q) - 2.0 * y(iq0 + 2 * nq + q)) * (1.0 / 3.0);
+    return kv * (2.0 * y(iq0 + nq + q) - 3.0 * y(iq0 + 3 * nq + q)) * 0.2 - (m.hprime * (1.0 / 15.0) + 0.4 * m.etaprime) * dl;
   }
   if (b.lambda_de) return T(0.0);
   const double cs2_Q = b.cs2DE;
