@@ -589,6 +589,7 @@ struct Ws {
   double *P;            // [W_cn; G_n^T] M^-1 (NRED x 3 nq): the reduced right-hand side straight from b
   double *rtd;          // 1 / td of the chain pivots, for the stage solves
   PBgShared *pg;        // background of the next stage, evaluated during the current stage solve
+  PBgShared *pgc, *pgn; // this step's background (Jacobian point) and the next step's, if accepted
   SplCache *sc;
 };
 
@@ -921,16 +922,23 @@ __device__ __forceinline__ void reduce_dense(const PLay &L, Ws &W, int tid) {
 }
 
 // Jacobian pieces at (t0, y0) and the structured factorisation of W = I/(gamma dt) - J
-// g receives the background at (t0, y0[0]), which is also the first stage's.  With the bordered
-// solve and more than one wavefront, waves 1.. run first(tid - 64) while wave 0 inverts the
-// reduced matrix (g and W00 are set by then).
+// g receives the background at (t0, y0[0]), which is also the first stage's: evaluated here
+// (gsrc 0) or loaded from W.pgn (1) or W.pgc (2), where earlier attempts left it; with more than
+// one wavefront it is also kept in W.pgc.  With the bordered solve and more than one wavefront,
+// waves 1.. run first(tid - 64) while wave 0 inverts the reduced matrix (g and W00 are set by
+// then).
 template <int MC, int NT, class F>
 __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double k, double t0, double dt, Ws &W, int tid, double &W00,
-                                          PBg<double> &g, const F &first) {
+                                          PBg<double> &g, int gsrc, const F &first) {
   const int n = L.n, m = L.m, nch = L.nch;
   const double gi = 1.0 / (dt * r5::gamma);
   rhs_dual<NT>(b, L, k, t0, W.y0, W.J0, W.dT, W, tid);
-  pbg_eval_lanes(b, L, k, t0, W.y0[0], g, W.sc, tid);
+  if (gsrc == 0) {
+    pbg_eval_lanes(b, L, k, t0, W.y0[0], g, W.sc, tid);
+  } else {
+    pbg_load(g, gsrc == 1 ? W.pgn : W.pgc, L, tid & (WAVE - 1));
+  }
+  if (NT > WAVE && gsrc != 2 && tid < WAVE) pbg_store(g, W.pgc, L, tid);
   // dense-block columns 0..m-1 and chain-head columns m..m+nch-1, one per lane; wave w takes
   // rows w, w + NR, ... of every column, so a wave evaluates one row formula at a time
   constexpr int NR = NT / WAVE;
@@ -1409,8 +1417,12 @@ __device__ __forceinline__ double stage_in(int s, double y0i, double ui, const K
 // Jacobian pass evaluated at (tprev, y0[0]).  With more than one wavefront, the stage solve runs
 // on wavefront 0 while wavefront 1 evaluates the next stage's background: it needs only row 0
 // of the stage input, and row 0 of the solve (x0 = v[0] / W_00) is known before the solve starts.
+// gsrc: where the Jacobian point's background comes from (see factorise).  tspec >= 0: with
+// more than one wavefront, wavefront 1 also evaluates the background at (tspec, y1[0]) into
+// W.pgn during the last stage solve: the next step's Jacobian point if this step is accepted.
 template <int NQ, int NT>
-__device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, double k, double tprev, double dt, Ws &W, int tid) {
+__device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, double k, double tprev, double dt, Ws &W, int tid,
+                                              int gsrc = 0, double tspec = -1.0) {
   constexpr int MC = NQ > 0 ? 17 + 3 * NQ : 0;
   // the first stage's right-hand side is formed beside the inversion of the reduced matrix
   constexpr bool FIRST_BESIDE_GJ = MC > 0 && NT > WAVE;
@@ -1464,7 +1476,7 @@ __device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, do
   auto first = [&](int t) {
     rhs_rows<(NT > WAVE ? NT - WAVE : WAVE)>(b, L, k, g, y0, W.perm1, W, t, emit_stage(0, dt * d1));
   };
-  factorise<MC, NT>(b, L, k, tprev, dt, W, tid, W00, g, first);
+  factorise<MC, NT>(b, L, k, tprev, dt, W, tid, W00, g, gsrc, first);
   for (int s = 0; s < 8; ++s) {
     double *ks = K + s * n;
     // df/dtau weight, selected without a run-time-indexed array (scratch memory)
@@ -1488,6 +1500,10 @@ __device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, do
         PBg<double> gn;
         pbg_eval_lanes<true>(b, L, k, stage_time(s + 1, tprev, dt), un, gn, W.sc, tid);
         pbg_store(gn, W.pg, L, tid - WAVE);
+      } else if (tid < 2 * WAVE && tspec >= 0.0) {  // y1[0] = u[0] + k_8[0] (as below)
+        PBg<double> gn;
+        pbg_eval_lanes<true>(b, L, k, tspec, u[0] + W.pg->x0, gn, W.sc, tid);
+        pbg_store(gn, W.pgn, L, tid - WAVE);
       }
     }
     __syncthreads();
@@ -1572,12 +1588,13 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   pid.prev_inv = 1.0;
   pid.prev_prev_inv = 1.0;
   int isave = 0, steps = 0, na = 0, nr = 0, st = 0;
+  int gsrc = 0;  // the next Jacobian point's background: evaluate, W.pgn (accepted), W.pgc (rejected)
   double *K = W.K;
   double *u = W.u, *y0 = W.y0;
   while (tprev < t1) {
     if (steps >= cfg.max_steps || (rs != nullptr && ridx >= rlen)) { st = 1; break; }
     const double dt = tnext - tprev;
-    const int nanflag = rodas5_attempt<NQ, NT>(b, L, k, tprev, dt, W, tid);
+    const int nanflag = rodas5_attempt<NQ, NT>(b, L, k, tprev, dt, W, tid, gsrc, fmin(tnext, t1));
     const double en = sqrt(filtered_sq(k, W.y0, W.y1, K + 7 * n, nanflag, cfg) / 6.0);
     if (step_log != nullptr && tid == 0) {  // (t_prev, t_next, error norm) of every attempted step
       double *lg = step_log + ((long)unit * cfg.max_steps + steps) * 3;
@@ -1606,6 +1623,7 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
     __syncthreads();
     double nt0 = keep ? tnext : tprev;
     tprev = fmin(nt0, t1);
+    gsrc = (NT > WAVE) ? (keep ? 1 : 2) : 0;
     double tn = nt0 + dt * factor;
     tnext = (tn > t1 - 1e-10) ? t1 : tn;
     if (rs != nullptr) {
@@ -1627,7 +1645,7 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
 
 // Layout of the state vector for the truncations in prob (NQ > 0: compile-time bin count) and
 // the carve-up of the dynamic LDS into the solver workspace, which must match
-// deb_perturb_lds_bytes(): 22 n + m^2 + nch m + 3 m + 3 NRED NQMAX + 21 NQMAX + 24 + NRED^2 doubles and a PBgShared, then
+// deb_perturb_lds_bytes(): 22 n + m^2 + nch m + 3 m + 3 NRED NQMAX + 21 NQMAX + 24 + NRED^2 doubles and three PBgShared, then
 // m + 5 n + 512 + 4 nch + 4 m ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
 template <int NQ, int NT>
 __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const double *__restrict__ nu_qw, long lds_bytes, PLay &L, Ws &W,
@@ -1649,6 +1667,8 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   W.R = p; p += NRED * NRED;
   W.rtd = p; p += n;
   W.pg = (PBgShared *)p; p += sizeof(PBgShared) / 8;
+  W.pgc = (PBgShared *)p; p += sizeof(PBgShared) / 8;
+  W.pgn = (PBgShared *)p; p += sizeof(PBgShared) / 8;
   double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
@@ -1898,7 +1918,7 @@ extern "C" long deb_perturb_lds_bytes(int lmaxg, int lmaxgp, int lmaxr, int lmax
   long n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + (long)nqmax * (lmaxnu + 1) + 2;
   long m = 17 + 3 * (long)nqmax, nch = 3 + nqmax;
   long dbl = n * 22 + m * m + nch * m + 3 * m + 3 * deb::NRED * deb::NQMAX + 21 * deb::NQMAX + 24 + deb::NRED * deb::NRED +
-             (long)(sizeof(deb::PBgShared) / 8);
+             3 * (long)(sizeof(deb::PBgShared) / 8);
   long ints = m + 5 * n + 512 + 4 * nch + 4 * m + 1;
   return dbl * 8 + ints * 4 + (long)sizeof(deb::SplCache);
 }
