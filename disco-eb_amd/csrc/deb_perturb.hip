@@ -942,7 +942,7 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
   // dense-block columns 0..m-1 and chain-head columns m..m+nch-1, one per lane; wave w takes
   // rows w, w + NR, ... of every column, so a wave evaluates one row formula at a time
   constexpr int NR = NT / WAVE;
-  const int rg = tid / WAVE;
+  const int rg = __builtin_amdgcn_readfirstlane(tid / WAVE);  // uniform: the row formulas branch on scalars
   for (int col = tid % WAVE; col < m + nch; col += WAVE) {
     int s = (col < m) ? W.dmap[col] : W.chead[col - m];
     auto ev = [&](int i) { return i == s ? 1.0 : 0.0; };
@@ -958,7 +958,7 @@ __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double
       if (MC > 0 && col < m && rq >= 0 && rq < 3 * nq && !(ncol && rq % nq == cq % nq)) continue;
       // a hierarchy's l = 3 entry (a chain head) appears only in that hierarchy's l = 2 row
       if (col >= m && r != W.cpar[col - m]) continue;
-      double v = core_row<double>(W.dmap[r], b, L, k, g, mt, ev, vsel);
+      double v = core_row<double>(__builtin_amdgcn_readfirstlane(W.dmap[r]), b, L, k, g, mt, ev, vsel);
       if (col < m) W.S[r * m + col] = ((r == col) ? gi : 0.0) - v;  // W_DD = I/(gamma h) - J_DD
       else W.Jh[(col - m) * m + r] = v;
     }
