@@ -1,5 +1,6 @@
-// deb_perturb.hip -- synchronous-gauge perturbations of one k-mode per wavefront, integrated
-// with the linearly implicit Rodas5 method under a PI step-size controller.
+// deb_perturb.hip -- synchronous-gauge perturbations of one k-mode per workgroup (one, two or
+// four wavefronts), integrated with the linearly implicit Rodas5 method under a PI step-size
+// controller.
 //
 // Follows the reference's evolve_one_mode / evolve_modes_batched (src/discoeb/perturbations.py:
 // 882-1120), model_synchronous (:88-525), approximations.py, Rodas5Transformed
@@ -14,9 +15,12 @@
 //     touches the rest of the system only through its l = 2 variable; a bottom-up sweep
 //     eliminates each chain into the dense block;
 //   * the dense block D (metric, CDM, baryons, l <= 2 moments, dark energy: 17 + 3 nq
-//     unknowns) receives one diagonal correction per chain (Schur complement) and is
-//     inverted by Gauss-Jordan with partial pivoting in LDS; each of the 8 stage solves is
-//     then a mat-vec plus two chain sweeps.
+//     unknowns) receives one diagonal correction per chain (Schur complement); with a
+//     compile-time bin count its massive-neutrino rows are eliminated through h' and eta'
+//     (bordered reduction, reduce_dense) and the remaining 19 x 19 system is inverted by one
+//     wavefront (Gauss-Jordan with partial pivoting, rows in registers).  Each of the 8 stage
+//     solves is then two chain sweeps and small mat-vecs on wavefront 0, while wavefront 1
+//     evaluates the next stage's background quantities.
 // The Jacobian column of a and df/dtau come from dual-number evaluations of the RHS; the
 // dense block's columns from lane-parallel evaluation of the (linear) RHS rows on unit
 // vectors; chain entries are the hierarchy coefficients themselves.
