@@ -310,7 +310,7 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
   const double ex = exp(x);
   const double rhonu = lane_bcast(ex, 0), pnu = lane_bcast(ex, 1);
   g.rho_Q = lane_bcast(ex, 2);
-  // divisions, round 1
+  // divisions, lane-parallel pass 1
   double num = 1.0, den = 1.0;
   switch (lane) {
     case 0: num = cs2c; den = a; break;
@@ -335,7 +335,7 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
   g.ia2 = lane_bcast(r, 8);
   const double aq = r;  // bin lanes
   const double grho = t1 + t2 + b.grhom * b.OmegaDE * g.rho_Q * a2 + b.grhom * b.Omegak;
-  // round 2
+  // division pass 2
   num = 1.0; den = 1.0;
   switch (lane) {
     case 0: num = grho; den = 3.0; break;
@@ -358,7 +358,7 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
   g.aHH = 0.5 * (g.aH * g.aH - gpres);
   const double wprime = -b.wa * g.aH * a;
   const double tv = tau, tc = g.tauc;
-  // round 3
+  // division pass 3
   num = 1.0; den = 1.0;
   switch (lane) {
     case 0: num = wprime; den = 3.0; break;
@@ -374,7 +374,7 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
   const double th = lane_bcast(r, 2), r_tv = lane_bcast(r, 3), r_tk = lane_bcast(r, 4);
   g.iaH = th;
   g.vl = r;  // bin lanes: 1 / sqrt(1 + aq^2)
-  // round 4
+  // division pass 4
   num = 1.0; den = 1.0;
   switch (lane) {
     case 0: num = w3; den = (1.0 + g.w_Q) + 1e-6; break;
@@ -383,7 +383,7 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
   }
   r = num / den;
   const double w4 = lane_bcast(r, 0), r_th = lane_bcast(r, 1);
-  g.ca2_Q = g.w_Q - w4 / g.aH;  // round 5: one division, every lane
+  g.ca2_Q = g.w_Q - w4 / g.aH;  // division pass 5: one division, every lane
   // approximation switches (approximations.py:50-75)
   g.rsa = b.use_rsa && (r_tv > b.rsa_c) && (tv * k > b.rsa_k);
   g.ufa = !g.rsa && b.use_ufa && (tv * k > b.ufa_k);
