@@ -146,6 +146,28 @@ def test_batch_of_one_is_the_per_mode_solve(deb):
     assert diff < 1e-12
 
 
+def test_batched_workgroup_width_gives_identical_results(deb):
+    """The shared-step kernel at 64, 128 and 256 threads per batch: the extra wavefronts only
+    evaluate rows and the next stage's background beside wavefront 0, so the step mesh, the step
+    counts and every output bit agree (each instantiation rounds every expression alike)."""
+    P = nu_w0wa_param()
+    p = deb.evolve_background(param=dict(P), num_thermo=256)
+    tau = p["tau_of_a_spline"].evaluate(np.asarray([1 / 3, 0.5, 1.0])).reshape(1, -1)
+    k = torch.as_tensor(np.geomspace(1e-5, 5.0, 6), device="cuda")
+    uc = torch.zeros(6, dtype=torch.int32, device="cuda")
+    kw = dict(lmaxg=12, lmaxgp=12, lmaxr=12, lmaxnu=17, nqmax=8, rtol=1e-5, atol=1e-10)
+    runs = {}
+    for nt in (64, 128, 256):
+        y, st, na, nr, log = deb.evolve_units_batched(p["_deb_tables"], 256, uc, k, tau, batch_size=3, step_log=True,
+                                                      threads=nt, **kw)
+        torch.cuda.synchronize()
+        assert (st.cpu().numpy() == 0).all(), nt
+        runs[nt] = [t.cpu().numpy() for t in (y, na, nr, log)]
+    for nt in (128, 256):
+        for a, b in zip(runs[nt], runs[64]):
+            assert np.array_equal(a, b, equal_nan=a.dtype.kind == "f"), nt
+
+
 def test_batched_reference_surface_and_errors(deb):
     P = disco_notebook()
     p = deb.evolve_background(param=dict(P))
