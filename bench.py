@@ -11,6 +11,8 @@ CPU oracle timed on a bounded sample of the same modes.  After the timed steps (
 them) it also measures, unless switched off:
   * gradients: P(k, z = 0) and its 9-parameter VJP for 256 k-modes log-spaced in 1e-4..1/Mpc
     (same cosmology and solver settings), evaluations/s;
+  * batched gradients: the P(k, z = 0) Jacobian of 16 cosmologies around the workload's in two
+    perturbation launches, against one call per cosmology (results bitwise equal);
   * line of sight: Theta_l(k) for 3000 k x 2500 conformal times, l = 0..2500, 16384-point
     FFTLog (synthetic smooth sources), and its FP64 matrix-core contraction on its own.
 """
@@ -106,6 +108,34 @@ def measure_gradient(deb, nk, repeats=2):
             "vjp": [float(v) for v in g.cpu().numpy()]}
 
 
+def measure_gradient_many(deb, n_cosmo, nk):
+    """P(k, z=0) and its 9-parameter Jacobian for n_cosmo cosmologies (theta within +-5% of the
+    workload's), batched (pk_jacobian_many: two perturbation launches for all of them) and one
+    pk_jacobian call per cosmology; the two must agree bitwise."""
+    import torch
+    P = workload_param()
+    th0 = deb.param_to_theta(P)
+    rng = np.random.default_rng(1)
+    scale = np.where(np.isin(np.arange(9), (3, 4, 8)), 0.0, 0.05)  # n_s, A_s, tau_reion held
+    thetas = th0 * (1.0 + scale * (2.0 * rng.random((n_cosmo, 9)) - 1.0))
+    k = deb.k_grid({}, 1e-4, 1.0, nk, "log")
+    opts = dict(**LMAX, **SOLVER, aexp_out=(1.0,))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    pkm, Jm = deb.pk_jacobian_many(thetas, P, k, **opts)
+    torch.cuda.synchronize()
+    t_many = time.perf_counter() - t
+    t = time.perf_counter()
+    same = True
+    for c in range(n_cosmo):
+        pk, J = deb.pk_jacobian(thetas[c], P, k, **opts)
+        same = same and bool(torch.equal(pk, pkm[c])) and bool(torch.equal(J, Jm[c]))
+    torch.cuda.synchronize()
+    t_one = time.perf_counter() - t
+    return {"cosmologies": n_cosmo, "k_modes": nk, "batched_s": t_many, "one_call_per_cosmology_s": t_one,
+            "gradients_per_s": n_cosmo / t_many, "speedup_vs_one_call_per_cosmology": t_one / t_many, "bitwise_identical": same}
+
+
 def measure_los(deb, nk=3000, nt=2500, ellmax=2500, nfft=16384):
     """Line-of-sight projection at the CMB shape, and the contraction kernel alone."""
     import torch
@@ -154,6 +184,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=256, help="k-modes timed on the CPU oracle (0: skip)")
     ap.add_argument("--grad-modes", type=int, default=256, help="k-modes of the gradient measurement (0: skip)")
+    ap.add_argument("--grad-cosmologies", type=int, default=16,
+                    help="cosmologies of the batched gradient measurement, at --grad-modes k-modes (0: skip)")
     ap.add_argument("--los", type=int, default=1, help="measure the line-of-sight projection (0: skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a known-size kernel for scaling HBM counters)")
@@ -241,6 +273,8 @@ def main():
     if args.grad_modes > 0:
         out["gradient"] = measure_gradient(deb, args.grad_modes)
         out["config"]["gradients_per_s"] = out["gradient"]["gradients_per_s"]
+        if args.grad_cosmologies > 0:
+            out["gradient_many"] = measure_gradient_many(deb, args.grad_cosmologies, args.grad_modes)
     if args.los:
         out["line_of_sight"] = measure_los(deb)
     if args.cpu_sample > 0 and rank == 0:

@@ -48,21 +48,24 @@ def param_to_theta(param, tau_reion=0.054):
 
 
 def _accepted_mesh(log, nacc, t1):
-    """Accepted step ends of every unit as fractions s = (t - t0) / (t1 - t0), the last exactly 1."""
-    lg = log.cpu().numpy()
-    na = nacc.cpu().numpy()
-    n_units = lg.shape[0]
-    L = int(na.max())
-    mesh = np.ones((n_units, L))
-    for u in range(n_units):
-        rows = lg[u]
-        ok = np.isfinite(rows[:, 2]) & (rows[:, 2] < 1.0)
-        tn = rows[ok, 1]
-        t0 = rows[0, 0]
-        s = (tn - t0) / (t1[u] - t0)
-        s[-1] = 1.0
-        mesh[u, :s.size] = s
-    return torch.tensor(mesh), torch.tensor(na, dtype=torch.int32)
+    """Accepted step ends of every unit as fractions s = (t - t0) / (t1 - t0), the last exactly 1.
+
+    log: (n_units, max_steps, 3) step log (t_prev, t_next, error norm; NaN past the last
+    attempt); nacc: accepted steps per unit; t1: (n_units,) end of each unit's range.  Returns
+    (mesh (n_units, max nacc) with 1 past each unit's end, mesh_len) device tensors."""
+    t1 = torch.as_tensor(t1, dtype=torch.float64, device=log.device)
+    en = log[:, :, 2]
+    ok = torch.isfinite(en) & (en < 1.0)
+    t0 = log[:, 0, 0]
+    s = (log[:, :, 1] - t0[:, None]) / (t1 - t0)[:, None]
+    pos = torch.cumsum(ok.to(torch.int64), dim=1) - 1
+    L = int(nacc.max())
+    mesh = torch.ones((log.shape[0], L), dtype=torch.float64, device=log.device)
+    rows = torch.arange(log.shape[0], device=log.device)[:, None].expand_as(ok)
+    mesh[rows[ok], pos[ok]] = s[ok]
+    mlen = nacc.to(torch.int32)
+    mesh[torch.arange(log.shape[0], device=log.device), mlen.to(torch.int64) - 1] = 1.0
+    return mesh, mlen
 
 
 # Launch of the replayed shifted cosmologies (twelve times the k-modes): largest k first, two
@@ -71,50 +74,76 @@ def _accepted_mesh(log, nacc, t1):
 REPLAY_SCHEDULE = {"order": "k_desc", "threads": 128}
 
 
-def pk_jacobian(theta, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, num_thermo=256, schedule=None, lmaxg=12, lmaxgp=12, lmaxr=12,
-                lmaxnu=17, nqmax=8, **solver):
-    """P(k) at aexp_out and its Jacobian with respect to theta.
+def pk_jacobian_many(thetas, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, num_thermo=256, schedule=None, lmaxg=12,
+                     lmaxgp=12, lmaxr=12, lmaxnu=17, nqmax=8, **solver):
+    """P(k) at aexp_out and its Jacobian with respect to theta for C cosmologies at once.
 
-    Returns (pk (n_k, nout), J (n_k, nout, 9)) device tensors."""
-    theta = np.asarray(theta, dtype=np.float64)
-    p0 = theta_to_param(theta, base)
+    thetas: (C, 9); every cosmology uses the same k-modes.  Two perturbation launches serve all
+    of them: the adaptive base solves of the C x n_k units (with their step logs), then the
+    12 C x n_k replays of the shifted cosmologies, each on its base unit's accepted mesh.  The
+    result of cosmology c is bitwise that of pk_jacobian(thetas[c], ...): units are independent.
+    Returns (pk (C, n_k, nout), J (C, n_k, nout, 9)) device tensors."""
+    thetas = np.atleast_2d(np.asarray(thetas, dtype=np.float64))
+    if thetas.ndim != 2 or thetas.shape[1] != 9:
+        raise ValueError("thetas must be (C, 9)")
+    C = thetas.shape[0]
+    p0s = [theta_to_param(th, base) for th in thetas]
     dev = _lib.device()
     lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
-    kk = torch.as_tensor(np.asarray(kmodes, dtype=np.float64), device=dev)
+    knp = np.asarray(kmodes, dtype=np.float64)
+    kk = torch.as_tensor(knp, device=dev)
     nk = kk.numel()
     aexp = np.atleast_1d(np.asarray(aexp_out, dtype=np.float64))
-    # base solve with its step log
-    t0 = background_tables([p0], num_thermo)
+    # base solves with their step logs
+    t0 = background_tables(p0s, num_thermo)
     tau0 = tau_out_all(t0, num_thermo, aexp)
-    uc0 = torch.zeros(nk, dtype=torch.int32, device=dev)
-    y, st, na, nr, log = evolve_units(t0, num_thermo, uc0, kk, tau0, **lm, **solver, step_log=True, schedule={"order": "k_desc"})
+    uc0, uk0, _ = units_for([knp] * C, dev)
+    y, st, na, nr, log = evolve_units(t0, num_thermo, uc0, uk0, tau0, **lm, **solver, step_log=True, schedule={"order": "k_desc"})
     _raise_on_status(st)
-    pk = power_units(t0, num_thermo, uc0, kk, y, idx)
-    t1 = np.full(nk, float(tau0.max().cpu()))
+    pk = power_units(t0, num_thermo, uc0, uk0, y, idx).reshape(C, nk, -1)
+    t1 = tau0.amax(dim=1)[uc0.to(torch.int64)]
     mesh, mlen = _accepted_mesh(log, na, t1)
-    # twelve shifted cosmologies, replayed on the base mesh in one launch
-    steps, params = [], []
-    for j in ODE_PARAMS:
-        d = rel_step * max(abs(theta[j]), 1e-2)
-        steps.append(d)
-        for sgn in (1.0, -1.0):
-            th = theta.copy()
-            th[j] += sgn * d
-            params.append(theta_to_param(th, base))
+    del log
+    # twelve shifted cosmologies per base cosmology, replayed on its mesh, all in one launch
+    steps = np.zeros((C, len(ODE_PARAMS)))
+    params = []
+    for c in range(C):
+        for i, j in enumerate(ODE_PARAMS):
+            d = rel_step * max(abs(thetas[c, j]), 1e-2)
+            steps[c, i] = d
+            for sgn in (1.0, -1.0):
+                th = thetas[c].copy()
+                th[j] += sgn * d
+                params.append(theta_to_param(th, base))
+    reps = 2 * len(ODE_PARAMS)
     tab = background_tables(params, num_thermo)
     tau = tau_out_all(tab, num_thermo, aexp)
-    uc, uk, _ = units_for([kk.cpu().numpy()] * len(params), dev)
-    reps = len(params)
+    uc, uk, _ = units_for([knp] * len(params), dev)
+    L = mesh.shape[1]
+    rmesh = mesh.reshape(C, 1, nk, L).expand(C, reps, nk, L).reshape(C * reps * nk, L)
+    rlen = mlen.reshape(C, 1, nk).expand(C, reps, nk).reshape(-1)
     yr, str_, _, _ = evolve_units(tab, num_thermo, uc, uk, tau, **lm, **solver, schedule=REPLAY_SCHEDULE if schedule is None else schedule,
-                                  replay=(mesh.repeat(reps, 1), mlen.repeat(reps)))
+                                  replay=(rmesh, rlen))
     _raise_on_status(str_)
-    pkr = power_units(tab, num_thermo, uc, uk, yr, idx).reshape(reps, nk, -1)
-    J = torch.zeros((nk, pk.shape[1], 9), dtype=torch.float64, device=dev)
-    for i, j in enumerate(ODE_PARAMS):
-        J[:, :, j] = (pkr[2 * i] - pkr[2 * i + 1]) / (2.0 * steps[i])
-    J[:, :, 3] = pk * torch.log(kk / p0.get("k_p", 0.05))[:, None]
-    J[:, :, 4] = pk
+    pkr = power_units(tab, num_thermo, uc, uk, yr, idx).reshape(C, reps, nk, -1)
+    J = torch.zeros((C, nk, pk.shape[2], 9), dtype=torch.float64, device=dev)
+    for c in range(C):
+        for i, j in enumerate(ODE_PARAMS):
+            J[c, :, :, j] = (pkr[c, 2 * i] - pkr[c, 2 * i + 1]) / (2.0 * steps[c, i])
+        J[c, :, :, 3] = pk[c] * torch.log(kk / p0s[c].get("k_p", 0.05))[:, None]
+        J[c, :, :, 4] = pk[c]
     return pk, J
+
+
+def pk_jacobian(theta, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, num_thermo=256, schedule=None, lmaxg=12, lmaxgp=12, lmaxr=12,
+                lmaxnu=17, nqmax=8, **solver):
+    """P(k) at aexp_out and its Jacobian with respect to theta (one cosmology).
+
+    Returns (pk (n_k, nout), J (n_k, nout, 9)) device tensors."""
+    pk, J = pk_jacobian_many(np.asarray(theta, dtype=np.float64)[None, :], base, kmodes, aexp_out=aexp_out, idx=idx, rel_step=rel_step,
+                             num_thermo=num_thermo, schedule=schedule, lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu,
+                             nqmax=nqmax, **solver)
+    return pk[0], J[0]
 
 
 class PkFunction(torch.autograd.Function):
@@ -140,4 +169,4 @@ def pk_and_vjp(theta, base, kmodes, cotangent, **opts):
     return pk, torch.einsum("kon,ko->n", J, c)
 
 
-__all__ = ["PARAM9", "theta_to_param", "param_to_theta", "pk_jacobian", "PkFunction", "pk_and_vjp"]
+__all__ = ["PARAM9", "theta_to_param", "param_to_theta", "pk_jacobian", "pk_jacobian_many", "PkFunction", "pk_and_vjp"]

@@ -73,7 +73,7 @@ def test_gradient_matches_oracle_frozen_mesh(deb, oracle, jac):
     uc = torch.zeros(k.size, dtype=torch.int32, device="cuda")
     y, st, na, nr, log = deb.evolve_units(tab, 256, uc, kt, tau, **LM, **SOLVER, step_log=True)
     mesh, mlen = deb.grad._accepted_mesh(log, na, np.full(k.size, float(tau.max().cpu())))
-    mesh, mlen = mesh.numpy(), mlen.numpy()
+    mesh, mlen = mesh.cpu().numpy(), mlen.cpu().numpy()
     kw = dict(LM, **SOLVER)
     for j in deb.grad.ODE_PARAMS:
         d = deb.grad.REL_STEP * max(abs(th[j]), 1e-2)
@@ -118,3 +118,20 @@ def test_autograd_function_returns_the_vjp(deb, jac):
     (out[:, 0].cpu() * torch.tensor(g)).sum().backward()
     np.testing.assert_allclose(theta.grad.numpy(), J[:, 0, :].T @ g, rtol=1e-12, atol=0)
     np.testing.assert_array_equal(out.detach().cpu().numpy(), pk)
+
+
+def test_many_cosmologies_match_one_at_a_time(deb):
+    """pk_jacobian_many (two perturbation launches for all cosmologies) gives, cosmology by
+    cosmology, bitwise the P(k) and Jacobian of separate pk_jacobian calls."""
+    print("case: 3 cosmologies x 4 k, batched gradient vs one call per cosmology", flush=True)
+    P = base_param()
+    th0 = deb.param_to_theta(P)
+    thetas = np.stack([th0, th0 * (1 + 0.02 * np.array([1, -1, 1, 0, 0, 1, 1, -1, 0])),
+                       th0 * (1 - 0.03 * np.array([1, 1, -1, 0, 0, -1, 1, 1, 0]))])
+    k = deb.k_grid({}, 1e-4, 0.5, 4, "log")
+    pkm, Jm = deb.pk_jacobian_many(thetas, P, k, aexp_out=(0.5, 1.0), **LM, **SOLVER)
+    assert pkm.shape == (3, 4, 2) and Jm.shape == (3, 4, 2, 9)
+    for c in range(3):
+        pk, J = deb.pk_jacobian(thetas[c], P, k, aexp_out=(0.5, 1.0), **LM, **SOLVER)
+        assert torch.equal(pkm[c], pk) and torch.equal(Jm[c], J), c
+    assert not torch.equal(Jm[0], Jm[1])
