@@ -74,6 +74,15 @@ def _accepted_mesh(log, nacc, t1):
 REPLAY_SCHEDULE = {"order": "k_desc", "threads": 128}
 
 
+def _base_schedule(n_units):
+    """Launch of the adaptive base solves: largest k first; up to 512 units (one launch's worth of
+    slots on one MI355X) at the library's default width, where the slowest unit's latency
+    decides, and beyond that at 128 threads, where units in flight decide (two per compute unit;
+    12.9k k-solves/s against 8.1k at 256 threads for 15,518 units).  Results do not depend on
+    the width."""
+    return {"order": "k_desc"} if n_units <= 512 else {"order": "k_desc", "threads": 128}
+
+
 def pk_jacobian_many(thetas, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, num_thermo=256, schedule=None, lmaxg=12,
                      lmaxgp=12, lmaxr=12, lmaxnu=17, nqmax=8, **solver):
     """P(k) at aexp_out and its Jacobian with respect to theta for C cosmologies at once.
@@ -98,7 +107,7 @@ def pk_jacobian_many(thetas, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_
     t0 = background_tables(p0s, num_thermo)
     tau0 = tau_out_all(t0, num_thermo, aexp)
     uc0, uk0, _ = units_for([knp] * C, dev)
-    y, st, na, nr, log = evolve_units(t0, num_thermo, uc0, uk0, tau0, **lm, **solver, step_log=True, schedule={"order": "k_desc"})
+    y, st, na, nr, log = evolve_units(t0, num_thermo, uc0, uk0, tau0, **lm, **solver, step_log=True, schedule=_base_schedule(C * nk))
     _raise_on_status(st)
     pk = power_units(t0, num_thermo, uc0, uk0, y, idx).reshape(C, nk, -1)
     t1 = tau0.amax(dim=1)[uc0.to(torch.int64)]
