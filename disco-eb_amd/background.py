@@ -44,16 +44,27 @@ class DeviceSpline:
         return self._call(x, 3)
 
 
-def background_tables(params, num_thermo=256):
+def recfast_failures(tables):
+    """Per-cosmology count of RECFAST interval solves that exceeded their step limit (header slot)."""
+    return tables[:, _lib.HDR["recfast_failures"]]
+
+
+def background_tables(params, num_thermo=256, check=True):
     """Background tables of several cosmologies at once: one workgroup per cosmology.
 
-    params: list of param dicts.  Returns a (n_cosmo, deb_tables_size) float64 device tensor."""
+    params: list of param dicts.  Returns a (n_cosmo, deb_tables_size) float64 device tensor.
+    With check (and outside graph capture) a RECFAST interval solve that exceeded its 128
+    steps in any cosmology raises RuntimeError, as attach_background does for one."""
     L = _lib.lib()
     dev = _lib.device()
     pv = torch.tensor([_lib.param_vector(p) for p in params], dtype=torch.float64, device=dev)
     size = L.deb_tables_size(int(num_thermo))
     tables = torch.empty((len(params), size), dtype=torch.float64, device=dev)
     _lib.check(L.deb_background(_lib.ptr(pv), len(params), int(num_thermo), _lib.ptr(tables), _lib.stream()))
+    if check and not torch.cuda.is_current_stream_capturing():
+        bad = torch.nonzero(recfast_failures(tables) != 0).flatten().cpu().tolist()
+        if bad:
+            raise RuntimeError(f"RECFAST interval solve exceeded 128 steps (cosmologies {bad})")
     return tables
 
 

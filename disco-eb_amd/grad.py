@@ -83,15 +83,46 @@ def _base_schedule(n_units):
     return {"order": "k_desc"} if n_units <= 512 else {"order": "k_desc", "threads": 128}
 
 
+# Per-cosmology status of the gradient functions: 0 ok; 4 the cosmology's w(a) crosses -1 (no
+# P(k), no derivatives: NaN); 6 a finite-difference partner of one column crosses -1 while the
+# cosmology does not (that column is NaN).
+GRAD_OK, GRAD_PHANTOM, GRAD_COLUMN_UNDEFINED = 0, 4, 6
+
+
+def _is_lambda(theta):
+    return theta[6] == -1.0 and theta[7] == 0.0
+
+
+def _shift_pair(theta, j, d):
+    """The two cosmologies whose central difference gives column j.  At an exact cosmological
+    constant (w0 = -1, wa = 0) every wa shift leaves w(1) = -1 (the fluid equations' 1 / (1 + w)
+    singularity at the last output), so the wa column is taken at w0 = -1 + 2 d, where both
+    partners keep w(a) > -1 on (0, 1] (a joint shift, error O(d) in the mixed derivative)."""
+    hi, lo = theta.copy(), theta.copy()
+    hi[j] += d
+    lo[j] -= d
+    if j == 7 and _is_lambda(theta):
+        hi[6] = lo[6] = -1.0 + 2.0 * d
+    return hi, lo
+
+
+def _cosmo_status(unit_status, C, nk):
+    """Per-cosmology flag from the per-unit status of C x nk units (4: phantom divide)."""
+    return (unit_status.reshape(C, nk) == 4).any(dim=1)
+
+
 def pk_jacobian_many(thetas, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, num_thermo=256, schedule=None, lmaxg=12,
-                     lmaxgp=12, lmaxr=12, lmaxnu=17, nqmax=8, **solver):
+                     lmaxgp=12, lmaxr=12, lmaxnu=17, nqmax=8, return_status=False, **solver):
     """P(k) at aexp_out and its Jacobian with respect to theta for C cosmologies at once.
 
     thetas: (C, 9); every cosmology uses the same k-modes.  Two perturbation launches serve all
     of them: the adaptive base solves of the C x n_k units (with their step logs), then the
     12 C x n_k replays of the shifted cosmologies, each on its base unit's accepted mesh.  The
     result of cosmology c is bitwise that of pk_jacobian(thetas[c], ...): units are independent.
-    Returns (pk (C, n_k, nout), J (C, n_k, nout, 9)) device tensors."""
+    A cosmology whose w(a) crosses -1 does not stop the batch: its P(k) and Jacobian are NaN
+    (status 4); a column whose shifted partner crosses -1 is NaN (status 6).
+    Returns (pk (C, n_k, nout), J (C, n_k, nout, 9)) device tensors, and the per-cosmology
+    status (C,) int32 numpy array when return_status is set."""
     thetas = np.atleast_2d(np.asarray(thetas, dtype=np.float64))
     if thetas.ndim != 2 or thetas.shape[1] != 9:
         raise ValueError("thetas must be (C, 9)")
@@ -108,10 +139,14 @@ def pk_jacobian_many(thetas, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_
     tau0 = tau_out_all(t0, num_thermo, aexp)
     uc0, uk0, _ = units_for([knp] * C, dev)
     y, st, na, nr, log = evolve_units(t0, num_thermo, uc0, uk0, tau0, **lm, **solver, step_log=True, schedule=_base_schedule(C * nk))
-    _raise_on_status(st)
+    _raise_on_status(st, allow_phantom=True)
+    bad = _cosmo_status(st, C, nk).cpu().numpy()
     pk = power_units(t0, num_thermo, uc0, uk0, y, idx).reshape(C, nk, -1)
     t1 = tau0.amax(dim=1)[uc0.to(torch.int64)]
     mesh, mlen = _accepted_mesh(log, na, t1)
+    # phantom units never stepped: give them a one-step mesh (their replays are skipped as well)
+    mlen = torch.where(mlen < 1, torch.ones_like(mlen), mlen)
+    mesh[:, 0] = torch.where(na < 1, torch.ones_like(mesh[:, 0]), mesh[:, 0])
     del log
     # twelve shifted cosmologies per base cosmology, replayed on its mesh, all in one launch
     steps = np.zeros((C, len(ODE_PARAMS)))
@@ -120,9 +155,9 @@ def pk_jacobian_many(thetas, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_
         for i, j in enumerate(ODE_PARAMS):
             d = rel_step * max(abs(thetas[c, j]), 1e-2)
             steps[c, i] = d
-            for sgn in (1.0, -1.0):
-                th = thetas[c].copy()
-                th[j] += sgn * d
+            # a cosmology on the phantom divide replays nothing: its partners are itself (skipped too)
+            pair = (thetas[c], thetas[c]) if bad[c] else _shift_pair(thetas[c], j, d)
+            for th in pair:
                 params.append(theta_to_param(th, base))
     reps = 2 * len(ODE_PARAMS)
     tab = background_tables(params, num_thermo)
@@ -133,14 +168,26 @@ def pk_jacobian_many(thetas, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_
     rlen = mlen.reshape(C, 1, nk).expand(C, reps, nk).reshape(-1)
     yr, str_, _, _ = evolve_units(tab, num_thermo, uc, uk, tau, **lm, **solver, schedule=REPLAY_SCHEDULE if schedule is None else schedule,
                                   replay=(rmesh, rlen))
-    _raise_on_status(str_)
+    _raise_on_status(str_, allow_phantom=True)
+    colbad = _cosmo_status(str_, C * reps, nk).reshape(C, len(ODE_PARAMS), 2).any(dim=2).cpu().numpy()
     pkr = power_units(tab, num_thermo, uc, uk, yr, idx).reshape(C, reps, nk, -1)
     J = torch.zeros((C, nk, pk.shape[2], 9), dtype=torch.float64, device=dev)
+    status = np.zeros(C, dtype=np.int32)
+    nan = float("nan")
     for c in range(C):
         for i, j in enumerate(ODE_PARAMS):
             J[c, :, :, j] = (pkr[c, 2 * i] - pkr[c, 2 * i + 1]) / (2.0 * steps[c, i])
+            if colbad[c, i]:
+                J[c, :, :, j] = nan
+                status[c] = GRAD_COLUMN_UNDEFINED
         J[c, :, :, 3] = pk[c] * torch.log(kk / p0s[c].get("k_p", 0.05))[:, None]
         J[c, :, :, 4] = pk[c]
+        if bad[c]:
+            pk[c] = nan
+            J[c] = nan
+            status[c] = GRAD_PHANTOM
+    if return_status:
+        return pk, J, status
     return pk, J
 
 
@@ -149,10 +196,20 @@ def pk_jacobian(theta, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, 
     """P(k) at aexp_out and its Jacobian with respect to theta (one cosmology).
 
     Returns (pk (n_k, nout), J (n_k, nout, 9)) device tensors."""
-    pk, J = pk_jacobian_many(np.asarray(theta, dtype=np.float64)[None, :], base, kmodes, aexp_out=aexp_out, idx=idx, rel_step=rel_step,
-                             num_thermo=num_thermo, schedule=schedule, lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu,
-                             nqmax=nqmax, **solver)
+    pk, J, st = pk_jacobian_many(np.asarray(theta, dtype=np.float64)[None, :], base, kmodes, aexp_out=aexp_out, idx=idx,
+                                 rel_step=rel_step, num_thermo=num_thermo, schedule=schedule, lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr,
+                                 lmaxnu=lmaxnu, nqmax=nqmax, return_status=True, **solver)
+    _raise_on_grad_status(st[0])
     return pk[0], J[0]
+
+
+def _raise_on_grad_status(st):
+    if st == GRAD_PHANTOM:
+        raise ValueError("w(a) = w_DE_0 + w_DE_a (1 - a) crosses -1 in (0, 1]: the dark-energy fluid equations are singular "
+                         "at the phantom divide, so P(k) and its gradient are undefined")
+    if st == GRAD_COLUMN_UNDEFINED:
+        raise ValueError("a finite-difference partner of a dark-energy column crosses w = -1 while the cosmology does not: "
+                         "that derivative is undefined at this step size (move the cosmology off the divide)")
 
 
 class PkFunction(torch.autograd.Function):

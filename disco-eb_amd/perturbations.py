@@ -103,6 +103,7 @@ def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, l
         mlen = replay[1].to(device=dev, dtype=torch.int32).contiguous()
         if mesh.ndim != 2 or mesh.shape[0] != n_units or mlen.numel() != n_units:
             raise ValueError("replay mesh must be (n_units, L) with one length per unit")
+        _check_mesh(mesh, mlen)
         _lib.check(L.deb_evolve_replay(_lib.ptr(tables), int(num_thermo), _lib.ptr(uc), _lib.ptr(uk), n_units, _lib.ptr(to), nout,
                                        ctypes.byref(prob), ctypes.byref(cfg), _lib.ptr(mesh), mesh.shape[1], _lib.ptr(mlen), _lib.ptr(y),
                                        _lib.ptr(status), _lib.ptr(nacc), _lib.ptr(nrej), _lib.ptr(work), sref, _lib.stream()))
@@ -174,6 +175,26 @@ def _schedule(schedule, uk):
     if order is None:
         return (_lib.DebSchedule(None, threads) if threads else None), None
     return _lib.DebSchedule(order.data_ptr(), threads), order
+
+
+def _check_mesh(mesh, mlen):
+    """Host check of a replay mesh (skipped under graph capture): the kernel reads mesh[u, i] for
+    every i < mlen[u], so each length must lie in 1..L; each row must increase from above 0 and
+    end at 1."""
+    if torch.cuda.is_current_stream_capturing():
+        return
+    L = mesh.shape[1]
+    lens = mlen.to(torch.int64)
+    if lens.numel() and (int(lens.min()) < 1 or int(lens.max()) > L):
+        raise ValueError(f"replay mesh lengths must lie in 1..{L}")
+    cols = torch.arange(L, device=mesh.device)[None, :]
+    inside = cols < lens[:, None]
+    prev = torch.cat([torch.zeros_like(mesh[:, :1]), mesh[:, :-1]], dim=1)
+    if bool((inside & ~(mesh > prev)).any()):
+        raise ValueError("replay mesh rows must increase from above 0")
+    last = mesh.gather(1, (lens - 1)[:, None]).flatten()
+    if bool((last != 1.0).any()):
+        raise ValueError("replay mesh rows must end at 1 (the last output time)")
 
 
 def _raise_on_status(status, allow_phantom=False):

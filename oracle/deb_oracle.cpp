@@ -1511,6 +1511,370 @@ double start_tau(const Bg &bg, double k, const double *tau_out, int nout) {
   return 0.99 * std::min(tmin, starting_time(bg, k));
 }
 
+// ----------------------------------------------------------------------------------------
+// Discrete adjoint of the frozen-mesh replay (checker of the HIP adjoint kernel).
+//
+// The map: the mode starts at t0 = 0.99 min(min tau_out, starting_time) with the adiabatic
+// initial state, takes Rodas5 steps ending at t0 + s_i (t1 - t0) (t1 = max tau_out, s_last = 1),
+// and its outputs at tau_out are linear interpolants between step ends, RSA-projected and
+// converted to the 20 output fields.  The adjoint returns d<gbar, outputs>/d theta_j for six
+// parameters through the exact reverse sweep of each Rodas5 step (ode_integrators_stiff.py:
+// 776-892): transposed solves with W = I / (gamma h) - J(t0, y0), vector-Jacobian products of
+// the right-hand side at every stage point, the dependence of J and df/dtau on (t0, y0, theta),
+// the step sizes' dependence on the mesh ends.  The pieces are taken independently of the
+// HIP kernel: dense LU, complex-step Jacobians, and central differences at the level of one
+// right-hand side for derivatives with respect to theta (shifted backgrounds, step d_j), a0 and
+// tau (relative step 1e-6).  Initial state, start and end times, and output times are
+// differentiated by central differences over the same shifted backgrounds.
+// ----------------------------------------------------------------------------------------
+void lu_solve_T(int n, const double *A, const int *piv, double *b) {  // solves W^T x = b with P W = L U
+  for (int i = 0; i < n; ++i) {  // U^T z = b
+    double s = b[i];
+    for (int j = 0; j < i; ++j) s -= A[j * n + i] * b[j];
+    b[i] = s / A[i * n + i];
+  }
+  for (int i = n - 1; i >= 0; --i) {  // L^T w = z
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= A[j * n + i] * b[j];
+    b[i] = s;
+  }
+  for (int kk = n - 1; kk >= 0; --kk)
+    if (piv[kk] != kk) std::swap(b[kk], b[piv[kk]]);
+}
+
+struct AdjCtx {
+  const Bg *bg;
+  const Bg *const *bgs;  // 12 shifted backgrounds: parameter j at 2 j (+d_j) and 2 j + 1 (-d_j)
+  const double *dth;     // d_j
+  const Layout *L;
+  double k;
+};
+
+// sum_i w_i f_i(tau, y) with the background bg
+double wdot_rhs(const Bg &bg, const Layout &L, double tau, const double *y, double k, const double *w) {
+  std::vector<double> f(L.n);
+  rhs_real(bg, L, tau, y, k, f.data());
+  double s = 0.0;
+  for (int i = 0; i < L.n; ++i) s += w[i] * f[i];
+  return s;
+}
+
+// psi(bg, t, y) = sum_s rb_s^T J(t, y) k_s + dTb^T dT(t, y), J and dT from the complex-step Jacobian
+double psi_J(const Bg &bg, const Layout &L, double t, const double *y, double k, const std::vector<std::vector<double>> &rb,
+             const double (*kst)[1], const std::vector<double> *ks, const double *dTb, Workspace &ws) {
+  (void)kst;
+  const int n = L.n;
+  jacobian(bg, L, t, y, k, ws);
+  double s = 0.0;
+  for (int st = 0; st < 8; ++st)
+    for (int i = 0; i < n; ++i) {
+      double acc = 0.0;
+      const double *Ji = ws.J.data() + (size_t)i * n;
+      for (int j = 0; j < n; ++j) acc += Ji[j] * ks[st][j];
+      s += rb[st][i] * acc;
+    }
+  for (int i = 0; i < n; ++i) s += dTb[i] * ws.dT[i];
+  return s;
+}
+
+// Reverse sweep of one Rodas5 step (t0, h, y0) -> y1 for the cotangent y1b.  Returns y0b, the
+// cotangents of t0 and h, and the direct parameter cotangents gth[6].
+void step_vjp(const AdjCtx &C, double t0, double h, const double *y0, const double *y1b, double *y0b, double &t0b, double &hb,
+              double *gth) {
+  using namespace r5;
+  const Bg &bg = *C.bg;
+  const Layout &L = *C.L;
+  const int n = L.n;
+  const double k = C.k;
+  Workspace ws;
+  ws.init(n);
+  // forward recomputation: stages, stage inputs, stage times
+  std::vector<double> y1(n), err(n);
+  rodas5_step(bg, L, k, t0, h, y0, y1.data(), err.data(), ws);
+  std::vector<std::vector<double>> K(8, std::vector<double>(n)), U(8, std::vector<double>(n));
+  for (int st = 0; st < 8; ++st) K[st] = ws.k[st];
+  const double A[6][5] = {{0, 0, 0, 0, 0}, {a21, 0, 0, 0, 0}, {a31, a32, 0, 0, 0}, {a41, a42, a43, 0, 0},
+                          {a51, a52, a53, a54, 0}, {a61, a62, a63, a64, a65}};
+  const double Cm[8][7] = {{0}, {C21}, {C31, C32}, {C41, C42, C43}, {C51, C52, C53, C54}, {C61, C62, C63, C64, C65},
+                           {C71, C72, C73, C74, C75, C76}, {C81, C82, C83, C84, C85, C86, C87}};
+  const double cst[8] = {0.0, c2, c3, c4, c5, 1.0, 1.0, 1.0};
+  const double dst[8] = {d1, d2, d3, d4, d5, 0.0, 0.0, 0.0};
+  for (int st = 0; st < 8; ++st)
+    for (int i = 0; i < n; ++i) {
+      if (st <= 5) {
+        double u = y0[i];
+        for (int j = 0; j < st; ++j) u += A[st][j] * K[j][i];
+        U[st][i] = u;
+      } else {
+        U[st][i] = U[st - 1][i] + K[st - 1][i];
+      }
+    }
+  // W = I / (gamma h) - J(t0, y0), factorised; dT(t0, y0)
+  jacobian(bg, L, t0, y0, k, ws);
+  std::vector<double> dT0 = ws.dT;
+  std::vector<double> LU(n * n);
+  std::vector<int> piv(n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) LU[i * n + j] = (i == j ? 1.0 / (h * gamma) : 0.0) - ws.J[i * n + j];
+  lu_factor(n, LU.data(), piv.data());
+
+  std::vector<std::vector<double>> kb(8, std::vector<double>(n, 0.0)), rb(8, std::vector<double>(n, 0.0));
+  std::vector<double> ucar(y1b, y1b + n), usb(n), dTb(n, 0.0);
+  for (int i = 0; i < n; ++i) y0b[i] = 0.0;
+  kb[7].assign(y1b, y1b + n);
+  t0b = 0.0;
+  hb = 0.0;
+  for (int j = 0; j < 6; ++j) gth[j] = 0.0;
+  Workspace wss;
+  wss.init(n);
+  for (int st = 7; st >= 0; --st) {
+    std::vector<double> &r = rb[st];
+    r = kb[st];
+    lu_solve_T(n, LU.data(), piv.data(), r.data());
+    const double ts = t0 + cst[st] * h;
+    // stage VJP: usb = J(ts, u_s)^T r, ts-bar = r . dT(ts, u_s), theta-bar by differences
+    jacobian(bg, L, ts, U[st].data(), k, wss);
+    for (int q = 0; q < n; ++q) {
+      double acc = 0.0;
+      for (int p = 0; p < n; ++p) acc += wss.J[(size_t)p * n + q] * r[p];
+      usb[q] = acc;
+    }
+    double tsb = 0.0;
+    for (int p = 0; p < n; ++p) tsb += r[p] * wss.dT[p];
+    for (int j = 0; j < 6; ++j)
+      gth[j] += (wdot_rhs(*C.bgs[2 * j], L, ts, U[st].data(), k, r.data()) - wdot_rhs(*C.bgs[2 * j + 1], L, ts, U[st].data(), k, r.data())) /
+                (2.0 * C.dth[j]);
+    t0b += tsb;
+    hb += cst[st] * tsb;
+    // right-hand side r_s = f + h d_s dT + sum_j (C_sj / h) k_j
+    if (st <= 4) {
+      double rdT = 0.0;
+      for (int p = 0; p < n; ++p) { dTb[p] += h * dst[st] * r[p]; rdT += r[p] * dT0[p]; }
+      hb += dst[st] * rdT;
+    }
+    for (int j = 0; j < st; ++j) {
+      double rk = 0.0;
+      for (int p = 0; p < n; ++p) { kb[j][p] += Cm[st][j] / h * r[p]; rk += r[p] * K[j][p]; }
+      hb -= Cm[st][j] / (h * h) * rk;
+    }
+    // k_s = W^-1 r_s with W = I / (gamma h) - J
+    double rks = 0.0;
+    for (int p = 0; p < n; ++p) rks += r[p] * K[st][p];
+    hb += rks / (gamma * h * h);
+    // stage inputs
+    if (st >= 5) {
+      for (int p = 0; p < n; ++p) ucar[p] += usb[p];
+      if (st >= 6) {
+        for (int p = 0; p < n; ++p) kb[st - 1][p] += ucar[p];
+      } else {
+        for (int p = 0; p < n; ++p) {
+          y0b[p] += ucar[p];
+          for (int j = 0; j < 5; ++j) kb[j][p] += A[5][j] * ucar[p];
+        }
+      }
+    } else {
+      for (int p = 0; p < n; ++p) {
+        y0b[p] += usb[p];
+        for (int j = 0; j < st; ++j) kb[j][p] += A[st][j] * usb[p];
+      }
+    }
+  }
+  // J(t0, y0) and dT(t0, y0) depend on a0, t0, theta and (through the a-column and dT) on y0[1:]
+  std::vector<double> rho(n, 0.0);
+  for (int st = 0; st < 8; ++st)
+    for (int p = 0; p < n; ++p) rho[p] += K[st][0] * rb[st][p];
+  Workspace wj;
+  wj.init(n);
+  std::vector<double> yp(y0, y0 + n);
+  const double da = 1e-6 * std::fabs(y0[0]), dtt = 1e-6 * t0;
+  // a0 and y0[1:]: d/da [A(a)^T rho] (with dTb: d/dtau [A(tau)^T dTb]) for columns >= 1
+  std::vector<double> Jp(n * n), Jm(n * n), dTp(n), dTm(n);
+  yp[0] = y0[0] + da;
+  jacobian(bg, L, t0, yp.data(), k, wj);
+  Jp = wj.J;
+  yp[0] = y0[0] - da;
+  jacobian(bg, L, t0, yp.data(), k, wj);
+  Jm = wj.J;
+  yp[0] = y0[0];
+  for (int q = 1; q < n; ++q) {
+    double acc = 0.0;
+    for (int p = 0; p < n; ++p) acc += rho[p] * (Jp[(size_t)p * n + q] - Jm[(size_t)p * n + q]) / (2.0 * da);
+    y0b[q] += acc;
+  }
+  jacobian(bg, L, t0 + dtt, y0, k, wj);
+  Jp = wj.J;
+  jacobian(bg, L, t0 - dtt, y0, k, wj);
+  Jm = wj.J;
+  for (int q = 1; q < n; ++q) {
+    double acc = 0.0;
+    for (int p = 0; p < n; ++p) acc += dTb[p] * (Jp[(size_t)p * n + q] - Jm[(size_t)p * n + q]) / (2.0 * dtt);
+    y0b[q] += acc;
+  }
+  // scalar directions of psi = sum_s rb_s^T J k_s + dTb^T dT
+  yp[0] = y0[0] + da;
+  double pp = psi_J(bg, L, t0, yp.data(), k, rb, nullptr, K.data(), dTb.data(), wj);
+  yp[0] = y0[0] - da;
+  double pm = psi_J(bg, L, t0, yp.data(), k, rb, nullptr, K.data(), dTb.data(), wj);
+  yp[0] = y0[0];
+  y0b[0] += (pp - pm) / (2.0 * da);
+  pp = psi_J(bg, L, t0 + dtt, y0, k, rb, nullptr, K.data(), dTb.data(), wj);
+  pm = psi_J(bg, L, t0 - dtt, y0, k, rb, nullptr, K.data(), dTb.data(), wj);
+  t0b += (pp - pm) / (2.0 * dtt);
+  for (int j = 0; j < 6; ++j) {
+    pp = psi_J(*C.bgs[2 * j], L, t0, y0, k, rb, nullptr, K.data(), dTb.data(), wj);
+    pm = psi_J(*C.bgs[2 * j + 1], L, t0, y0, k, rb, nullptr, K.data(), dTb.data(), wj);
+    gth[j] += (pp - pm) / (2.0 * C.dth[j]);
+  }
+}
+
+// outputs (20 fields) of an interpolated state at tau: RSA projection and conversion
+void output_fields(const Bg &bg, const Layout &L, double tau, double k, const double *y, double *out) {
+  std::vector<double> t(y, y + L.n);
+  rsa_projection(bg, L, tau, k, t.data());
+  convert_output(bg, L, k, t.data(), out);
+}
+double output_dot(const Bg &bg, const Layout &L, double tau, double k, const double *y, const double *gb) {
+  double o[20];
+  output_fields(bg, L, tau, k, y, o);
+  double s = 0.0;
+  for (int f = 0; f < 20; ++f) s += gb[f] * o[f];
+  return s;
+}
+
+// Adjoint of the whole frozen-mesh replay.  mesh: s_0 < ... < s_{m-1} = 1.  gbar: nout x 20.
+// parts: 4 x 6 (direct ODE, outputs at fixed state, initial state, mesh and output times).
+int adjoint_replay(const AdjCtx &C, const double *tau_out, const double *tau_out_sh, int nout, const double *mesh, int mlen,
+                   const double *gbar, double *grad, double *parts, double *lam0, double *tb2) {
+  const Bg &bg = *C.bg;
+  const Layout &L = *C.L;
+  const int n = L.n;
+  const double k = C.k;
+  double t1 = tau_out[0];
+  for (int i = 1; i < nout; ++i) t1 = std::max(t1, tau_out[i]);
+  const double t0 = start_tau(bg, k, tau_out, nout);
+  const int N = mlen;
+  std::vector<double> T(N + 1), sv(N + 1);
+  T[0] = t0;
+  sv[0] = 0.0;
+  for (int i = 0; i < N; ++i) {
+    sv[i + 1] = mesh[i] >= 1.0 ? 1.0 : mesh[i];
+    T[i + 1] = mesh[i] >= 1.0 ? t1 : t0 + mesh[i] * (t1 - t0);
+  }
+  std::vector<std::vector<double>> Y(N + 1, std::vector<double>(n));
+  initial_conditions(bg, L, t0, k, Y[0].data());
+  Workspace ws;
+  ws.init(n);
+  std::vector<double> err(n);
+  for (int i = 0; i < N; ++i) rodas5_step(bg, L, k, T[i], T[i + 1] - T[i], Y[i].data(), Y[i + 1].data(), err.data(), ws);
+  // output j falls in the first step whose end reaches it
+  std::vector<int> ostep(nout, -1);
+  for (int j = 0; j < nout; ++j)
+    for (int i = 0; i < N; ++i)
+      if (tau_out[j] <= T[i + 1]) { ostep[j] = i; break; }
+  for (int j = 0; j < nout; ++j)
+    if (ostep[j] < 0) return 1;
+  std::vector<double> lam(n, 0.0), tb(N + 1, 0.0), taub(nout, 0.0), pend(n), yo(n), yb0(n), yq(n);
+  double g_ode[6] = {0}, g_out[6] = {0}, g_ic[6] = {0}, g_mesh[6] = {0};
+  for (int i = N - 1; i >= 0; --i) {
+    const double hh = T[i + 1] - T[i];
+    std::fill(pend.begin(), pend.end(), 0.0);
+    for (int j = 0; j < nout; ++j) {
+      if (ostep[j] != i) continue;
+      const double th = (tau_out[j] - T[i]) / hh;
+      for (int q = 0; q < n; ++q) yo[q] = Y[i][q] + th * (Y[i + 1][q] - Y[i][q]);
+      const double *gb = gbar + (size_t)j * 20;
+      // d<gb, F(y)>/dy: linear in y[1:] at fixed a, by differences along unit vectors
+      std::vector<double> yob(n, 0.0);
+      for (int q = 1; q < n; ++q) {
+        yq = yo;
+        const double e = std::max(std::fabs(yo[q]), 1e-30);
+        yq[q] = yo[q] + e;
+        double fp = output_dot(bg, L, tau_out[j], k, yq.data(), gb);
+        yq[q] = yo[q] - e;
+        double fm = output_dot(bg, L, tau_out[j], k, yq.data(), gb);
+        yob[q] = (fp - fm) / (2.0 * e);
+      }
+      {
+        yq = yo;
+        const double e = 1e-6 * std::fabs(yo[0]);
+        yq[0] = yo[0] + e;
+        double fp = output_dot(bg, L, tau_out[j], k, yq.data(), gb);
+        yq[0] = yo[0] - e;
+        double fm = output_dot(bg, L, tau_out[j], k, yq.data(), gb);
+        yob[0] = (fp - fm) / (2.0 * e);
+      }
+      for (int p = 0; p < 6; ++p)
+        g_out[p] += (output_dot(*C.bgs[2 * p], L, tau_out[j], k, yo.data(), gb) - output_dot(*C.bgs[2 * p + 1], L, tau_out[j], k, yo.data(), gb)) /
+                    (2.0 * C.dth[p]);
+      double thb = 0.0;
+      for (int q = 0; q < n; ++q) {
+        lam[q] += th * yob[q];
+        pend[q] += (1.0 - th) * yob[q];
+        thb += yob[q] * (Y[i + 1][q] - Y[i][q]);
+      }
+      tb[i] += -thb * (1.0 - th) / hh;
+      tb[i + 1] += -thb * th / hh;
+      taub[j] += thb / hh;
+    }
+    double t0b, hb, gth[6];
+    step_vjp(C, T[i], hh, Y[i].data(), lam.data(), yb0.data(), t0b, hb, gth);
+    tb[i] += t0b - hb;
+    tb[i + 1] += hb;
+    for (int p = 0; p < 6; ++p) g_ode[p] += gth[p];
+    for (int q = 0; q < n; ++q) lam[q] = yb0[q] + pend[q];
+  }
+  double T0b = 0.0, T1b = 0.0;
+  for (int i = 0; i <= N; ++i) { T0b += (1.0 - sv[i]) * tb[i]; T1b += sv[i] * tb[i]; }
+  std::vector<double> yp(n), ym(n);
+  for (int p = 0; p < 6; ++p) {
+    const Bg &bp = *C.bgs[2 * p], &bm = *C.bgs[2 * p + 1];
+    const double *top = tau_out_sh + (size_t)(2 * p) * nout, *tom = tau_out_sh + (size_t)(2 * p + 1) * nout;
+    const double t0p = start_tau(bp, k, top, nout), t0m = start_tau(bm, k, tom, nout);
+    double t1p = top[0], t1m = tom[0];
+    for (int j = 1; j < nout; ++j) { t1p = std::max(t1p, top[j]); t1m = std::max(t1m, tom[j]); }
+    initial_conditions(bp, L, t0p, k, yp.data());
+    initial_conditions(bm, L, t0m, k, ym.data());
+    const double d2 = 2.0 * C.dth[p];
+    for (int q = 0; q < n; ++q) g_ic[p] += lam[q] * (yp[q] - ym[q]) / d2;
+    g_mesh[p] += T0b * (t0p - t0m) / d2 + T1b * (t1p - t1m) / d2;
+    for (int j = 0; j < nout; ++j) g_mesh[p] += taub[j] * (top[j] - tom[j]) / d2;
+  }
+  for (int p = 0; p < 6; ++p) {
+    grad[p] = g_ode[p] + g_out[p] + g_ic[p] + g_mesh[p];
+    if (parts) { parts[p] = g_ode[p]; parts[6 + p] = g_out[p]; parts[12 + p] = g_ic[p]; parts[18 + p] = g_mesh[p]; }
+  }
+  if (lam0) std::memcpy(lam0, lam.data(), sizeof(double) * n);
+  if (tb2) { tb2[0] = T0b; tb2[1] = T1b; }
+  return 0;
+}
+
+// Outputs (20 fields) of the frozen-mesh replay on a normalized mesh (as the HIP replay takes it).
+int replay_norm(const Bg &bg, const Layout &L, double k, const double *tau_out, int nout, const double *mesh, int mlen, double *yout) {
+  const int n = L.n;
+  double t1 = tau_out[0];
+  for (int i = 1; i < nout; ++i) t1 = std::max(t1, tau_out[i]);
+  const double t0 = start_tau(bg, k, tau_out, nout);
+  Workspace ws;
+  ws.init(n);
+  std::vector<double> y(n), y1(n), err(n), tmp(n);
+  initial_conditions(bg, L, t0, k, y.data());
+  int isave = 0;
+  double tprev = t0;
+  for (int i = 0; i < mlen; ++i) {
+    const double tnext = mesh[i] >= 1.0 ? t1 : t0 + mesh[i] * (t1 - t0);
+    rodas5_step(bg, L, k, tprev, tnext - tprev, y.data(), y1.data(), err.data(), ws);
+    while (isave < nout && tau_out[isave] <= tnext) {
+      double th = (tau_out[isave] - tprev) / (tnext - tprev);
+      for (int q = 0; q < n; ++q) tmp[q] = y[q] + th * (y1[q] - y[q]);
+      output_fields(bg, L, tau_out[isave], k, tmp.data(), yout + (size_t)isave * 20);
+      ++isave;
+    }
+    y.swap(y1);
+    tprev = tnext;
+  }
+  return isave == nout ? 0 : 1;
+}
+
 const Spline *find_spline(const Bg &bg, const std::string &name) {
   if (name == "tau_of_a") return &bg.tau_of_a;
   if (name == "a_of_tau") return &bg.a_of_tau;
@@ -1723,5 +2087,39 @@ int orc_evolve_replay(void *h, double k, const double *tau_out, int nout, const 
 }
 
 double orc_start_tau(void *h, double k, const double *tau_out, int nout) { return start_tau(*(Bg *)h, k, tau_out, nout); }
+
+// Frozen-mesh replay on a normalized mesh, and its discrete adjoint (see adjoint_replay).
+// bgs: 12 shifted backgrounds (parameter j: +d_j at 2 j, -d_j at 2 j + 1); tau_out_sh: their
+// output times (12 x nout).  One thread per mode over nk modes sharing tau_out.
+int orc_replay_norm(void *h, const int *lmax, const double *kk, int nk, const double *tau_out, int nout, const double *mesh,
+                    const int *mlen, int mstride, int nthreads, double *yout) {
+  const Bg &bg = *(Bg *)h;
+  Layout L;
+  L.init(lmax[0], lmax[1], lmax[2], lmax[3], lmax[4]);
+  int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : bad)
+#endif
+  for (int u = 0; u < nk; ++u)
+    bad += replay_norm(bg, L, kk[u], tau_out, nout, mesh + (size_t)u * mstride, mlen[u], yout + (size_t)u * nout * 20);
+  return bad;
+}
+int orc_adjoint(void *h, void *const *bgs, const double *dth, const int *lmax, const double *kk, int nk, const double *tau_out,
+                const double *tau_out_sh, int nout, const double *mesh, const int *mlen, int mstride, const double *gbar, int nthreads,
+                double *grad, double *parts, double *lam0, double *tb) {
+  Layout L;
+  L.init(lmax[0], lmax[1], lmax[2], lmax[3], lmax[4]);
+  int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : bad)
+#endif
+  for (int u = 0; u < nk; ++u) {
+    AdjCtx C{(const Bg *)h, (const Bg *const *)bgs, dth, &L, kk[u]};
+    bad += adjoint_replay(C, tau_out, tau_out_sh, nout, mesh + (size_t)u * mstride, mlen[u], gbar + (size_t)u * nout * 20,
+                          grad + (size_t)u * 6, parts ? parts + (size_t)u * 24 : nullptr, lam0 ? lam0 + (size_t)u * L.n : nullptr,
+                          tb ? tb + (size_t)u * 2 : nullptr);
+  }
+  return bad;
+}
 
 }  // extern "C"

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libdeb_oracle.so")
+# DEB_ORACLE_LIB names another build of the same source (bench.py's -O3 -march=native CPU baseline)
+_LIB_PATH = os.environ.get("DEB_ORACLE_LIB") or os.path.join(_HERE, "_build", "libdeb_oracle.so")
 
 PARAM_KEYS = [
     "Omegam", "Omegab", "H0", "Tcmb", "YHe", "Neff", "Nmnu", "mnu", "w_DE_0", "w_DE_a", "cs2_DE", "Omegak",
