@@ -1318,7 +1318,7 @@ struct Pid {
 // evolve_one_mode (perturbations.py:882-949): per-k adaptive Rodas5 with SaveAt(ts) linear
 // interpolation.  yout receives nout x nvar projected states (or nout x 20 converted fields).
 int evolve_one(const Bg &bg, const Layout &L, double kmode, const double *tau_out, int nout, const SolverCfg &s, bool full,
-               double *yout, int *nacc, int *nrej) {
+               double *yout, int *nacc, int *nrej, double *acc_t = nullptr) {
   const int n = L.n;
   double tmin = tau_out[0];
   for (int i = 1; i < nout; ++i) tmin = std::min(tmin, tau_out[i]);
@@ -1354,6 +1354,7 @@ int evolve_one(const Bg &bg, const Layout &L, double kmode, const double *tau_ou
         ++isave;
       }
       y.swap(y1);
+      if (acc_t) acc_t[na] = tnext;
       ++na;
     } else {
       ++nr;
@@ -2087,6 +2088,17 @@ int orc_evolve_replay(void *h, double k, const double *tau_out, int nout, const 
 }
 
 double orc_start_tau(void *h, double k, const double *tau_out, int nout) { return start_tau(*(Bg *)h, k, tau_out, nout); }
+
+// Adaptive solve of one mode that also returns its accepted step ends (acc_t, max_steps slots).
+int orc_evolve_mesh(void *h, double k, const double *tau_out, int nout, const int *lmax, const double *solver, int max_steps,
+                    double *y_out, double *acc_t, int *n_acc) {
+  const Bg &bg = *(Bg *)h;
+  Layout L;
+  L.init(lmax[0], lmax[1], lmax[2], lmax[3], lmax[4]);
+  SolverCfg s{solver[0], solver[1], solver[2], solver[3], solver[4], solver[5], solver[6], max_steps};
+  int nr = 0;
+  return evolve_one(bg, L, k, tau_out, nout, s, false, y_out, n_acc, &nr, acc_t);
+}
 
 // Frozen-mesh replay on a normalized mesh, and its discrete adjoint (see adjoint_replay).
 // bgs: 12 shifted backgrounds (parameter j: +d_j at 2 j, -d_j at 2 j + 1); tau_out_sh: their

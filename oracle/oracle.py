@@ -69,6 +69,11 @@ def _load():
                                       ctypes.c_int, _dp, _dp]
     lib.orc_start_tau.restype = ctypes.c_double
     lib.orc_start_tau.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, ctypes.c_int]
+    lib.orc_evolve_mesh.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, _dp, _ip]
+    lib.orc_replay_norm.argtypes = [ctypes.c_void_p, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _ip, ctypes.c_int,
+                                    ctypes.c_int, _dp]
+    lib.orc_adjoint.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), _dp, _ip, _dp, ctypes.c_int, _dp, _dp,
+                                ctypes.c_int, _dp, _ip, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
     _lib = lib
     return lib
 
@@ -253,6 +258,79 @@ def _replay(self, k, aexp_out, mesh, keep, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu
 
 
 Background.replay = _replay
+
+
+def _mesh(self, k, tau_out, lmax, rtol, atol, max_steps=2048, pcoeff=0.25, icoeff=0.80, dcoeff=0.0, factormax=20.0,
+          factormin=0.3):
+    """Adaptive solve of one mode at the given output times; returns (outputs (nout, 20), accepted
+    step ends as fractions s = (t - t0) / (t1 - t0) of the mode's range, the last exactly 1)."""
+    lib = _load()
+    tau_out, tp = _d(tau_out)
+    lm, lp = _i(lmax)
+    sv, sp = _d([rtol, atol, pcoeff, icoeff, dcoeff, factormax, factormin])
+    y = np.empty((tau_out.size, 20))
+    acc = np.empty(max_steps)
+    na = np.zeros(1, dtype=np.int32)
+    st = lib.orc_evolve_mesh(self._h, float(k), tp, tau_out.size, lp, sp, int(max_steps), y.ctypes.data_as(_dp),
+                             acc.ctypes.data_as(_dp), na.ctypes.data_as(_ip))
+    if st != 0:
+        raise RuntimeError(f"oracle solve failed with status {st}")
+    t0 = lib.orc_start_tau(self._h, float(k), tp, tau_out.size)
+    t1 = float(tau_out.max())
+    s = (acc[:na[0]] - t0) / (t1 - t0)
+    s[-1] = 1.0
+    return y, s
+
+
+def _replay_norm(self, kk, tau_out, mesh, mlen, lmax, nthreads=None):
+    """Frozen-mesh replay of modes kk on normalized meshes (rows of mesh, lengths mlen): (nk, nout, 20)."""
+    lib = _load()
+    kk, kp = _d(np.atleast_1d(kk))
+    tau_out, tp = _d(tau_out)
+    mesh, mp = _d(np.atleast_2d(mesh))
+    mlen, mlp = _i(np.atleast_1d(mlen))
+    lm, lp = _i(lmax)
+    y = np.full((kk.size, tau_out.size, 20), np.nan)
+    bad = lib.orc_replay_norm(self._h, lp, kp, kk.size, tp, tau_out.size, mp, mlp, mesh.shape[1],
+                              int(nthreads or os.cpu_count() or 1), y.ctypes.data_as(_dp))
+    if bad:
+        raise RuntimeError("replay did not reach every output time")
+    return y
+
+
+def _adjoint(self, shifted, dth, kk, tau_out, tau_out_sh, mesh, mlen, gbar, lmax, nthreads=None):
+    """Discrete adjoint of the frozen-mesh replay (oracle/deb_oracle.cpp adjoint_replay).
+
+    shifted: 12 Backgrounds (parameter j at +d_j: 2 j, at -d_j: 2 j + 1); gbar: (nk, nout, 20)
+    output cotangents.  Returns (grad (nk, 6), parts (nk, 4, 6): direct ODE, outputs at fixed
+    state, initial state, mesh and output times; lam0 (nk, n); (T0bar, T1bar) (nk, 2))."""
+    lib = _load()
+    kk, kp = _d(np.atleast_1d(kk))
+    nk = kk.size
+    tau_out, tp = _d(tau_out)
+    tsh, tshp = _d(tau_out_sh)
+    mesh, mp = _d(np.atleast_2d(mesh))
+    mlen, mlp = _i(np.atleast_1d(mlen))
+    lm, lp = _i(lmax)
+    dth, dp = _d(dth)
+    gbar, gp = _d(gbar)
+    hs = (ctypes.c_void_p * 12)(*[b._h for b in shifted])
+    n = lib.orc_layout(*[int(v) for v in lmax])
+    grad = np.empty((nk, 6))
+    parts = np.empty((nk, 4, 6))
+    lam0 = np.empty((nk, n))
+    tb = np.empty((nk, 2))
+    bad = lib.orc_adjoint(self._h, hs, dp, lp, kp, nk, tp, tshp, tau_out.size, mp, mlp, mesh.shape[1], gp,
+                          int(nthreads or os.cpu_count() or 1), grad.ctypes.data_as(_dp), parts.ctypes.data_as(_dp),
+                          lam0.ctypes.data_as(_dp), tb.ctypes.data_as(_dp))
+    if bad:
+        raise RuntimeError("adjoint: an output time lies outside the mesh")
+    return grad, parts, lam0, tb
+
+
+Background.mesh = _mesh
+Background.replay_norm = _replay_norm
+Background.adjoint = _adjoint
 
 
 def nvar(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax):
