@@ -30,6 +30,10 @@ extern "C" int deb_launch_perturb(const double *, int, const int *, const double
                                   const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, const int *, int,
                                   const double *, long, const int *, hipStream_t);
 extern "C" long deb_perturb_lds_bytes(int, int, int, int, int);
+extern "C" long deb_adjoint_lds_bytes(const DebProblem *, int);
+extern "C" int deb_launch_adjoint(const double *, int, const double *, double *, const double *, int, const int *, const double *, int,
+                                  const double *, const double *, int, const double *, const DebProblem *, const double *, long, const int *,
+                                  const double *, double *, long, double *, double *, double *, int *, const int *, hipStream_t);
 extern "C" int deb_launch_spline_eval(const double *, int, int, int, const double *, int, double *, hipStream_t);
 extern "C" int deb_launch_power(const double *, int, const double *, long, int, int, double, double, double, double *, hipStream_t);
 extern "C" int deb_launch_time_deriv(const double *, const double *, const double *, int, const double *, int, const double *,
@@ -144,6 +148,41 @@ int deb_evolve_replay(const double *d_tables, int num_thermo, const int *d_unit_
   if (mesh_stride < 1) return fail(-1, "mesh_stride must be >= 1");
   return evolve_common(d_tables, num_thermo, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout, prob, cfg, d_y_out, d_status, d_n_accept,
                        d_n_reject, d_work, nullptr, sched, d_mesh, mesh_stride, d_mesh_len, stream);
+}
+
+long deb_adjoint_workspace_doubles(int n_units, int n_cosmo, int num_thermo, int max_mesh_len, const DebProblem *prob) {
+  if (!prob || n_units < 0 || n_cosmo < 0 || max_mesh_len < 1) return -1;
+  const long n = deb_nvar(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
+  return 32 + 6L * n_cosmo * deb_table_doubles(num_thermo) + (long)n_units * (max_mesh_len + 1) * n;
+}
+
+int deb_evolve_adjoint(const double *d_tables, int num_thermo, int n_cosmo, const double *d_shifted_tables, const double *d_dtheta,
+                       const int *d_unit_cosmo, const double *d_unit_k, int n_units, const double *d_tau_out, const double *d_tau_out_shifted,
+                       int nout, const DebProblem *prob, const double *d_mesh, long mesh_stride, int max_mesh_len, const int *d_mesh_len,
+                       const double *d_gbar, double *d_grad, int *d_status, double *d_work, double *d_lam0, double *d_tbar,
+                       const DebSchedule *sched, void *stream) {
+  int rc = check_problem(prob);
+  if (rc) return rc;
+  if (n_units < 0 || n_cosmo < 1) return fail(-1, "n_units must be >= 0 and n_cosmo >= 1");
+  if (n_units == 0) return 0;
+  if (nout < 1 || nout > 64) return fail(-1, "the adjoint takes 1..64 output times");
+  if (max_mesh_len < 1 || mesh_stride < max_mesh_len) return fail(-1, "max_mesh_len must lie in 1..mesh_stride");
+  if (!d_tables || !d_shifted_tables || !d_dtheta || !d_unit_cosmo || !d_unit_k || !d_tau_out || !d_tau_out_shifted || !d_mesh ||
+      !d_mesh_len || !d_gbar || !d_grad || !d_status || !d_work)
+    return fail(-1, "NULL device pointer");
+  if (sched && sched->threads_per_unit != 0 && sched->threads_per_unit != 256) return fail(-1, "the adjoint runs 256 threads per unit");
+  if (deb_adjoint_lds_bytes(prob, 256) > 160 * 1024) return fail(-1, "state too large for LDS");
+  rc = deb_launch_nu_bins(prob->nqmax, d_work, (hipStream_t)stream);
+  if (rc) return hip_rc(rc, "deb_evolve_adjoint (momentum bins)");
+  const long n = deb_nvar(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
+  double *dtab = d_work + 32;
+  double *ckpt = dtab + 6L * n_cosmo * deb_table_doubles(num_thermo);
+  rc = deb_launch_adjoint(d_tables, num_thermo, d_shifted_tables, dtab, d_dtheta, n_cosmo, d_unit_cosmo, d_unit_k, n_units, d_tau_out,
+                          d_tau_out_shifted, nout, d_work, prob, d_mesh, mesh_stride, d_mesh_len, d_gbar, ckpt, (long)(max_mesh_len + 1) * n,
+                          d_grad, d_lam0, d_tbar, d_status, sched ? sched->d_order : nullptr, (hipStream_t)stream);
+  if (rc == -2) return fail(-1, "state too large for LDS");
+  if (rc == -5) return fail(-1, "the adjoint takes 1..64 output times");
+  return hip_rc(rc, "deb_evolve_adjoint");
 }
 
 long deb_evolve_batched_workspace_doubles(int n_units, const DebProblem *prob) {

@@ -289,12 +289,8 @@ __global__ void tau_out_kernel(const double *__restrict__ tables, long tstride, 
 }  // namespace deb
 
 extern "C" long deb_perturb_lds_bytes(int lmaxg, int lmaxgp, int lmaxr, int lmaxnu, int nqmax) {
-  long n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + (long)nqmax * (lmaxnu + 1) + 2;
-  long m = 17 + 3 * (long)nqmax, nch = 3 + nqmax;
-  long dbl = n * 22 + m * m + nch * m + 3 * m + 3 * deb::NRED * deb::NQMAX + 21 * deb::NQMAX + 24 + deb::NRED * deb::NRED +
-             3 * (long)(sizeof(deb::PBgShared) / 8);
-  long ints = m + 5 * n + 512 + 4 * nch + 4 * m + 1;
-  return dbl * 8 + ints * 4 + (long)sizeof(deb::SplCache);
+  const DebProblem p{lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax, 0};
+  return deb::deb_perturb_lds_bytes_dev(p);
 }
 
 extern "C" int deb_launch_nu_bins(int nqmax, double *d_qw, hipStream_t stream) {

@@ -1615,6 +1615,17 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   }
 }
 
+// LDS bytes of the forward solver's carve-up (setup_solver): 22 n + m^2 + nch m + 3 m + 3 NRED NQMAX
+// + 21 NQMAX + 24 + NRED^2 doubles and three PBgShared, then m + 5 n + 512 + 4 nch + 4 m + 1 ints and
+// the spline cache.
+__host__ __device__ inline long deb_perturb_lds_bytes_dev(const DebProblem &p) {
+  const long n = 7 + (p.lmaxg + 1) + (p.lmaxgp + 1) + (p.lmaxr + 1) + (long)p.nqmax * (p.lmaxnu + 1) + 2;
+  const long m = 17 + 3 * (long)p.nqmax, nch = 3 + p.nqmax;
+  const long dbl = n * 22 + m * m + nch * m + 3 * m + 3 * NRED * NQMAX + 21 * NQMAX + 24 + NRED * NRED + 3 * (long)(sizeof(PBgShared) / 8);
+  const long ints = m + 5 * n + 512 + 4 * nch + 4 * m + 1;
+  return dbl * 8 + ints * 4 + (long)sizeof(SplCache);
+}
+
 // Layout of the state vector for the truncations in prob (NQ > 0: compile-time bin count) and
 // the carve-up of the dynamic LDS into the solver workspace, which must match
 // deb_perturb_lds_bytes(): 22 n + m^2 + nch m + 3 m + 3 NRED NQMAX + 21 NQMAX + 24 + NRED^2 doubles and three PBgShared, then

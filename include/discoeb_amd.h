@@ -106,6 +106,31 @@ int deb_evolve_replay(const double *d_tables, int num_thermo, const int *d_unit_
                       const int *d_mesh_len, double *d_y_out, int *d_status, int *d_n_accept, int *d_n_reject, double *d_work,
                       const DebSchedule *sched, void *stream);
 
+/* Discrete adjoint of deb_evolve_replay: per unit, the gradient of <gbar, outputs> with respect to
+ * the six parameters that enter the background and the perturbations, theta_p for p = 0..5 =
+ * (omega_c, omega_b, h, m_nu, w0, wa).  Reverse Rodas5 sweep over checkpointed states (the
+ * transposed structured solve of every stage, vector-Jacobian products of the right-hand side);
+ * the initial state, start and end times and output times by central differences.
+ * d_shifted_tables: n_cosmo x 12 table blocks, cosmology c's block 2 p + 0 at theta_p + dtheta[c, p]
+ *   and 2 p + 1 at theta_p - dtheta[c, p] (the other parameters at cosmology c's values);
+ * d_dtheta: n_cosmo x 6 steps; d_tau_out_shifted: n_cosmo x 12 x nout output times of those;
+ * d_mesh, mesh_stride, d_mesh_len: as deb_evolve_replay (mesh_len <= max_mesh_len);
+ * d_gbar: n_units x nout x 20 cotangents of the output fields (nout <= 64);
+ * d_grad: n_units x 6, written (NaN and status 4 for a unit whose w(a) crosses -1);
+ * d_work: deb_adjoint_workspace_doubles(...) doubles (momentum bins, derivative tables,
+ *   checkpoints); d_lam0 (n_units x nvar) and d_tbar (n_units x 2), optional (NULL): the
+ *   cotangents of the initial state and of the start and end times.  256 threads per unit.
+ * Replaces the forward-mode derivative jax.jacfwd takes through diffeqsolve (DirectAdjoint,
+ * src/discoeb/perturbations.py:904-925) and the Rodas5 step (src/discoeb/ode_integrators_stiff.py:
+ * 776-892), with the step mesh held fixed; reverse mode gives all six parameter derivatives of
+ * one mode in one sweep. */
+long deb_adjoint_workspace_doubles(int n_units, int n_cosmo, int num_thermo, int max_mesh_len, const DebProblem *prob);
+int deb_evolve_adjoint(const double *d_tables, int num_thermo, int n_cosmo, const double *d_shifted_tables, const double *d_dtheta,
+                       const int *d_unit_cosmo, const double *d_unit_k, int n_units, const double *d_tau_out, const double *d_tau_out_shifted,
+                       int nout, const DebProblem *prob, const double *d_mesh, long mesh_stride, int max_mesh_len, const int *d_mesh_len,
+                       const double *d_gbar, double *d_grad, int *d_status, double *d_work, double *d_lam0, double *d_tbar,
+                       const DebSchedule *sched, void *stream);
+
 /* Conformal output times tau_out[c, j] = tau(a = d_aexp_out[j]) of every cosmology c, from its
  * tau_of_a spline; d_tau_out: n_cosmo x nout doubles.
  * Replaces tau_out = param['tau_of_a_spline'].evaluate(aexp_out) (perturbations.py:1200-1206). */
