@@ -65,12 +65,12 @@ def _accepted_mesh(log, nacc, t1):
     t0 = log[:, 0, 0]
     s = (log[:, :, 1] - t0[:, None]) / (t1 - t0)[:, None]
     pos = torch.cumsum(ok.to(torch.int64), dim=1) - 1
-    L = int(nacc.max())
+    L = max(1, int(nacc.max()))  # at least one column: units that never stepped get a one-step mesh
     mesh = torch.ones((log.shape[0], L), dtype=torch.float64, device=log.device)
     rows = torch.arange(log.shape[0], device=log.device)[:, None].expand_as(ok)
     mesh[rows[ok], pos[ok]] = s[ok]
     mlen = nacc.to(torch.int32)
-    mesh[torch.arange(log.shape[0], device=log.device), mlen.to(torch.int64) - 1] = 1.0
+    mesh[torch.arange(log.shape[0], device=log.device), (mlen.to(torch.int64) - 1).clamp(min=0)] = 1.0
     return mesh, mlen
 
 

@@ -1,9 +1,10 @@
 """P(k) gradients with respect to the nine sampling parameters (HIP path).
 
 The gradient is that of the frozen-mesh map (the base solve's accepted step ends held as
-fractions of each mode's range).  Checked against the oracle's replay of the same normalized
-mesh on shifted cosmologies (central differences, same steps), against the closed forms for
-A_s, n_s and tau_reion, for step-size convergence, and through torch.autograd.
+fractions of each mode's range), by the discrete adjoint (the default method).  Checked against
+the oracle's replay of the same normalized mesh on shifted cosmologies (central differences,
+same steps), against the closed forms for A_s, n_s and tau_reion, for the step of the tables'
+differences, and through torch.autograd.
 """
 import numpy as np
 import pytest
@@ -116,7 +117,9 @@ def test_autograd_function_returns_the_vjp(deb, jac):
     theta = torch.tensor(th, dtype=torch.float64, requires_grad=True)
     out = deb.PkFunction.apply(theta, k, base_param(), dict(LM, **SOLVER))
     (out[:, 0].cpu() * torch.tensor(g)).sum().backward()
-    np.testing.assert_allclose(theta.grad.numpy(), J[:, 0, :].T @ g, rtol=1e-12, atol=0)
+    # backward runs the adjoint with this cotangent; J's rows came from unit cotangents (the
+    # kernel is linear in the cotangent, so the two agree to rounding)
+    np.testing.assert_allclose(theta.grad.numpy(), J[:, 0, :].T @ g, rtol=1e-10, atol=0)
     np.testing.assert_array_equal(out.detach().cpu().numpy(), pk)
 
 
