@@ -148,29 +148,36 @@ def cpu_baseline(kmodes, sample, sample_1core):
 
 
 def measure_gradient(deb, nk, repeats=2):
-    """P(k, z=0) plus its 9-parameter VJP (seeded standard-normal cotangent) per second."""
+    """P(k, z=0) plus its 9-parameter VJP (seeded standard-normal cotangent) per second, by the
+    discrete adjoint (one base launch, one adjoint launch) and, beside it, by differences."""
     import torch
     P = workload_param()
     theta = deb.param_to_theta(P)
     k = deb.k_grid({}, 1e-4, 1.0, nk, "log")
     cot = np.random.default_rng(0).standard_normal(nk)
     opts = dict(**LMAX, **SOLVER, aexp_out=(1.0,))
-    deb.pk_and_vjp(theta, P, k, cot, **opts)  # warm-up
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for _ in range(repeats):
-        pk, g = deb.pk_and_vjp(theta, P, k, cot, **opts)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t) / repeats
-    return {"gradients_per_s": 1.0 / dt, "ms_per_gradient": 1e3 * dt, "k_modes": nk, "params": list(deb.PARAM9),
-            "method": "frozen-mesh central differences, 12 shifted cosmologies replayed in one launch; n_s, A_s closed form",
-            "vjp": [float(v) for v in g.cpu().numpy()]}
+    out = {"k_modes": nk, "params": list(deb.PARAM9)}
+    for method in ("adjoint", "fd"):
+        deb.pk_and_vjp(theta, P, k, cot, method=method, **opts)  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(repeats):
+            pk, g = deb.pk_and_vjp(theta, P, k, cot, method=method, **opts)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / repeats
+        out[method] = {"gradients_per_s": 1.0 / dt, "ms_per_gradient": 1e3 * dt, "vjp": [float(v) for v in g.cpu().numpy()]}
+    out["gradients_per_s"] = out["adjoint"]["gradients_per_s"]
+    out["method"] = ("discrete adjoint: reverse Rodas5 sweep per mode over checkpointed states on the frozen accepted-step mesh; "
+                     "background dependence through tables differenced over 12 shifted cosmologies; n_s, A_s closed form")
+    return out
 
 
-def measure_gradient_many(deb, n_cosmo, nk):
+def measure_gradient_many(deb, n_cosmo, nk, n_plain=2):
     """P(k, z=0) and its 9-parameter Jacobian for n_cosmo cosmologies (theta within +-5% of the
-    workload's), batched (pk_jacobian_many: two perturbation launches for all of them) and one
-    pk_jacobian call per cosmology; the two must agree bitwise."""
+    workload's): batched by the adjoint (one base launch + one adjoint launch for all), batched by
+    differences (one base launch + one replay launch of the 12 n_cosmo shifted cosmologies), and
+    the plain form -- one cosmology at a time, one launch per shifted cosmology -- on n_plain of
+    them.  Rates are gradients (full Jacobians) per second."""
     import torch
     P = workload_param()
     th0 = deb.param_to_theta(P)
@@ -179,20 +186,44 @@ def measure_gradient_many(deb, n_cosmo, nk):
     thetas = th0 * (1.0 + scale * (2.0 * rng.random((n_cosmo, 9)) - 1.0))
     k = deb.k_grid({}, 1e-4, 1.0, nk, "log")
     opts = dict(**LMAX, **SOLVER, aexp_out=(1.0,))
-    torch.cuda.synchronize()
+    res = {"cosmologies": n_cosmo, "k_modes": nk}
+    Js = {}
+    for method in ("adjoint", "fd"):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        pkm, Jm = deb.pk_jacobian_many(thetas, P, k, method=method, **opts)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        Js[method] = Jm
+        res[method] = {"s": dt, "gradients_per_s": n_cosmo / dt}
+    # plain form: per cosmology, the base solve and each of the 12 shifted replays in its own launch
     t = time.perf_counter()
-    pkm, Jm = deb.pk_jacobian_many(thetas, P, k, **opts)
+    for c in range(n_plain):
+        _plain_jacobian(deb, thetas[c], P, k, opts)
     torch.cuda.synchronize()
-    t_many = time.perf_counter() - t
-    t = time.perf_counter()
-    same = True
-    for c in range(n_cosmo):
-        pk, J = deb.pk_jacobian(thetas[c], P, k, **opts)
-        same = same and bool(torch.equal(pk, pkm[c])) and bool(torch.equal(J, Jm[c]))
-    torch.cuda.synchronize()
-    t_one = time.perf_counter() - t
-    return {"cosmologies": n_cosmo, "k_modes": nk, "batched_s": t_many, "one_call_per_cosmology_s": t_one,
-            "gradients_per_s": n_cosmo / t_many, "speedup_vs_one_call_per_cosmology": t_one / t_many, "bitwise_identical": same}
+    dt = (time.perf_counter() - t) / n_plain
+    res["plain"] = {"s_per_gradient": dt, "gradients_per_s": 1.0 / dt, "sample": n_plain,
+                    "form": "one cosmology per call, base solve + 12 shifted-cosmology replays, one launch each"}
+    res["speedup_adjoint_vs_plain"] = res["adjoint"]["gradients_per_s"] / res["plain"]["gradients_per_s"]
+    cols = [0, 1, 2, 5, 6, 7]
+    res["adjoint_vs_fd_median_rel"] = float((Js["adjoint"][..., cols] - Js["fd"][..., cols]).abs().median() /
+                                            Js["fd"][..., cols].abs().max())
+    return res
+
+
+def _plain_jacobian(deb, theta, P, k, opts):
+    """The unbatched difference gradient: one launch per shifted cosmology."""
+    import torch
+    aexp = np.atleast_1d(opts["aexp_out"])
+    lm = {x: opts[x] for x in ("lmaxg", "lmaxgp", "lmaxr", "lmaxnu", "nqmax")}
+    sv = {x: opts[x] for x in ("rtol", "atol")}
+    S = deb.grad._setup(np.asarray(theta)[None, :], P, k, aexp, lm, sv, 256, deb.grad.REL_STEP)
+    kt = S["kk"]
+    uc = torch.zeros(kt.numel(), dtype=torch.int32, device=kt.device)
+    for r in range(12):
+        tab = S["tab"][r:r + 1]
+        tau = S["tau"][r:r + 1]
+        deb.evolve_units(tab, 256, uc, kt, tau, **lm, **sv, replay=(S["mesh"], S["mlen"]), schedule={"order": "k_desc"})
 
 
 def measure_los(deb, nk=3000, nt=2500, ellmax=2500, nfft=16384):
@@ -333,6 +364,7 @@ def main():
     if args.grad_modes > 0:
         out["gradient"] = measure_gradient(deb, args.grad_modes)
         out["config"]["gradients_per_s"] = out["gradient"]["gradients_per_s"]
+        out["config"]["gradient_method"] = "adjoint"
         if args.grad_cosmologies > 0:
             out["gradient_many"] = measure_gradient_many(deb, args.grad_cosmologies, args.grad_modes)
     if args.los:

@@ -110,7 +110,7 @@ def lib():
     L.deb_cmb_sources_workspace_doubles.restype = ctypes.c_long
     L.deb_cmb_sources_workspace_doubles.argtypes = [ctypes.c_int] * 3
     L.deb_cmb_sources.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
-                                  ctypes.POINTER(DebProblem), _vp, _vp, _vp]
+                                  ctypes.POINTER(DebProblem), ctypes.c_int, _vp, _vp, _vp]
     L.deb_los_workspace_doubles.restype = ctypes.c_long
     L.deb_los_workspace_doubles.argtypes = [ctypes.c_int] * 5
     L.deb_los_theta_ell.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,

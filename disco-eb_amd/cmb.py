@@ -57,10 +57,14 @@ def compute_visibility_functions(tau, param, aexp=None):
     return {nm: out[i] for i, nm in enumerate(names)}
 
 
-def compute_source_function_from_states(yout, yprime, kmodes, aexp_out, tau, param, lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax):
+def compute_source_function_from_states(yout, yprime, kmodes, aexp_out, tau, param, lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax, isw="spline"):
     """S1 (ISW), S2 (Sachs-Wolfe), S3 (Doppler), S4 (polarisation), S and the metric alpha,
     alpha' for states yout and their time derivatives yprime (n_k, n_tau, nvar) -- the chain of
-    compute_Cell_spectrum_from_cosmo_params steps 6-10 (cmb.py:1050-1073)."""
+    compute_Cell_spectrum_from_cosmo_params steps 6-10 (cmb.py:1050-1073).  isw: "spline" (the
+    reference's d/dtau(eta + alpha') by a spline in tau, cmb.py:437-449) or "algebraic"
+    (eta' + alpha'', the form of its docstring, cmb.py:406-408)."""
+    if isw not in ("spline", "algebraic"):
+        raise ValueError("isw must be 'spline' or 'algebraic'")
     t, N, c = _tables(param)
     dev = t.device
     y, yp = _dev(yout, dev), _dev(yprime, dev)
@@ -74,7 +78,7 @@ def compute_source_function_from_states(yout, yprime, kmodes, aexp_out, tau, par
     work = torch.empty(L.deb_cmb_sources_workspace_doubles(N, nk, nt), dtype=torch.float64, device=dev)
     prob = _lib.DebProblem(int(lmaxg), int(lmaxgp), int(lmaxr), int(lmaxnu), int(nqmax), 1)
     _lib.check(L.deb_cmb_sources(_lib.ptr(t), N, c, _lib.ptr(y), _lib.ptr(yp), _lib.ptr(kk), nk, _lib.ptr(aa), _lib.ptr(tt), nt,
-                                 ctypes.byref(prob), _lib.ptr(out), _lib.ptr(work), _lib.stream()))
+                                 ctypes.byref(prob), 1 if isw == "algebraic" else 0, _lib.ptr(out), _lib.ptr(work), _lib.stream()))
     return {"S1": out[0], "S2": out[1], "S3": out[2], "S4": out[3], "S": out[4], "alpha": out[5], "alphaprime": out[6]}
 
 

@@ -11,7 +11,7 @@ extern "C" int deb_launch_background(const double *, int, int, double *, hipStre
 extern "C" int deb_launch_nu_bins(int, double *, hipStream_t);
 extern "C" long deb_cmb_sources_work_doubles(int, int, int);
 extern "C" int deb_launch_cmb_sources(const double *, int, const double *, const double *, const double *, int, const double *, const double *,
-                                      int, const DebProblem *, double *, double *, hipStream_t);
+                                      int, const DebProblem *, int, double *, double *, hipStream_t);
 extern "C" int deb_launch_visibility(const double *, int, const double *, const double *, int, double *, double *, hipStream_t);
 extern "C" long deb_los_work_doubles(int, int, int, int, int);
 extern "C" int deb_launch_los(const double *, int, const double *, int, const double *, double, int, int, int, double *, double *, double *,
@@ -249,14 +249,17 @@ int deb_time_derivatives(const double *d_tables, int num_thermo, int cosmo, cons
 long deb_cmb_sources_workspace_doubles(int num_thermo, int n_k, int n_tau) { return deb_cmb_sources_work_doubles(num_thermo, n_k, n_tau); }
 
 int deb_cmb_sources(const double *d_tables, int num_thermo, int cosmo, const double *d_y, const double *d_yprime, const double *d_k, int n_k,
-                    const double *d_aexp, const double *d_tau, int n_tau, const DebProblem *prob, double *d_out, double *d_work, void *stream) {
+                    const double *d_aexp, const double *d_tau, int n_tau, const DebProblem *prob, int isw_mode, double *d_out, double *d_work,
+                    void *stream) {
   int rc = check_problem(prob);
   if (rc) return rc;
+  if (isw_mode != 0 && isw_mode != 1) return fail(-1, "isw_mode must be 0 (spline derivative) or 1 (algebraic)");
   if (n_k < 1 || n_tau < 3) return fail(-1, "need n_k >= 1 and n_tau >= 3");
   if (cosmo < 0) return fail(-1, "cosmology index must be >= 0");
   if (!d_tables || !d_y || !d_yprime || !d_k || !d_aexp || !d_tau || !d_out || !d_work) return fail(-1, "NULL device pointer");
   const double *t = d_tables + (long)cosmo * deb_table_doubles(num_thermo);
-  return hip_rc(deb_launch_cmb_sources(t, num_thermo, d_y, d_yprime, d_k, n_k, d_aexp, d_tau, n_tau, prob, d_out, d_work, (hipStream_t)stream),
+  return hip_rc(deb_launch_cmb_sources(t, num_thermo, d_y, d_yprime, d_k, n_k, d_aexp, d_tau, n_tau, prob, isw_mode, d_out, d_work,
+                                       (hipStream_t)stream),
                 "deb_cmb_sources");
 }
 

@@ -155,7 +155,7 @@ __global__ void vis_eval_kernel(const double *__restrict__ t, const double *__re
 // pot: n_tau x n_k (transposed) potential eta + alpha' for the ISW spline
 __global__ void source_point_kernel(const double *__restrict__ t, const double *__restrict__ y, const double *__restrict__ yp,
                                     const double *__restrict__ kk, int n_k, int n_tau, const double *__restrict__ aexp,
-                                    const double *__restrict__ V, const double *__restrict__ qw, DebProblem prob,
+                                    const double *__restrict__ V, const double *__restrict__ qw, DebProblem prob, int isw,
                                     double *__restrict__ out, double *__restrict__ pot) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)n_k * n_tau) return;
@@ -211,14 +211,11 @@ __global__ void source_point_kernel(const double *__restrict__ t, const double *
   const double etaprime = 0.5 * dgtheta / k2;
   const double alpha = (hprime + 6.0 * etaprime) / 2.0 / k2;
   const double alphap = -3.0 * dgshear / (2.0 * k2) + eta - 2.0 * aH * alpha;
-  (void)etaprime;
   // polarisation terms (cmb.py:349-395)
   const double couplp = 8.0 * (thetagp + k2 * alphap) / 15.0 - k * 0.6 * (P[idxg + 3] + P[idxgp + 1] + P[idxgp + 3]);
   const double pol = Y[idxg + 2] + Y[idxgp] + Y[idxgp + 2];
   const double polp = P[idxg + 2] + P[idxgp] + P[idxgp + 2];
   const double polpp = couplp - 0.3 * (opacp * pol + opac * polp);
-  (void)aHp;
-  (void)dgshearp;
   // S2, S3, S4 (cmb.py:455-554)
   const double S2 = g * (0.25 * deltag + 2.0 * alphap + pol / 16.0 + (thetabp + 3.0 / 16.0 * polpp) / k2);
   const double S3 = g1 * (alpha + (thetab + 3.0 / 8.0 * polp) / k2);
@@ -229,7 +226,14 @@ __global__ void source_point_kernel(const double *__restrict__ t, const double *
   out[3 * plane + idx] = S4;
   out[5 * plane + idx] = alpha;
   out[6 * plane + idx] = alphap;
-  pot[(long)j * n_k + ik] = eta + alphap;
+  if (isw == 1) {  // S1 = e^-tau_c (eta' + alpha''), the algebraic form (cmb.py:406-408)
+    const double alphapp = -3.0 * dgshearp / (2.0 * k2) + etaprime - 2.0 * (aHp * alpha + aH * alphap);
+    const double S1 = exp(-V[5L * n_tau + j]) * (etaprime + alphapp);
+    out[0 * plane + idx] = S1;
+    out[4 * plane + idx] = S1 + S2 + S3 + S4;
+  } else {
+    pot[(long)j * n_k + ik] = eta + alphap;
+  }
 }
 
 // S1 = exp(-tau_c) d(eta + alpha')/dtau from the natural spline of the potential in tau
@@ -552,7 +556,7 @@ extern "C" long deb_cmb_sources_work_doubles(int num_thermo, int n_k, int n_tau)
 }
 
 extern "C" int deb_launch_cmb_sources(const double *d_t, int num_thermo, const double *d_y, const double *d_yp, const double *d_k, int n_k,
-                                      const double *d_aexp, const double *d_tau, int n_tau, const DebProblem *prob, double *d_out,
+                                      const double *d_aexp, const double *d_tau, int n_tau, const DebProblem *prob, int isw, double *d_out,
                                       double *d_work, hipStream_t stream) {
   double *vw = d_work, *V = vw + 12L * num_thermo, *pot = V + (long)deb::NV * n_tau, *Spot = pot + (long)n_k * n_tau;
   double *cp = Spot + (long)n_k * n_tau, *qw = cp + n_tau;
@@ -562,9 +566,11 @@ extern "C" int deb_launch_cmb_sources(const double *d_t, int num_thermo, const d
   hipLaunchKernelGGL(deb::vis_eval_kernel, dim3((n_tau + 127) / 128), dim3(128), 0, stream, d_t, vw, d_tau, d_aexp, n_tau, V);
   long pts = (long)n_k * n_tau;
   hipLaunchKernelGGL(deb::source_point_kernel, dim3((unsigned)((pts + 127) / 128)), dim3(128), 0, stream, d_t, d_y, d_yp, d_k, n_k, n_tau,
-                     d_aexp, V, qw, *prob, d_out, pot);
-  hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(1), 0, stream, d_tau, n_tau, cp);
-  hipLaunchKernelGGL(deb::source_isw_kernel, dim3((n_k + 63) / 64), dim3(64), 0, stream, d_tau, n_tau, n_k, cp, pot, Spot, V, d_out);
+                     d_aexp, V, qw, *prob, isw, d_out, pot);
+  if (isw == 0) {
+    hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(1), 0, stream, d_tau, n_tau, cp);
+    hipLaunchKernelGGL(deb::source_isw_kernel, dim3((n_k + 63) / 64), dim3(64), 0, stream, d_tau, n_tau, n_k, cp, pot, Spot, V, d_out);
+  }
   return (int)hipGetLastError();
 }
 

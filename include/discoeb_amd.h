@@ -175,11 +175,14 @@ long deb_cmb_sources_workspace_doubles(int num_thermo, int n_k, int n_tau);
  * (both n_k x n_tau x nvar, as deb_evolve with return_full and deb_time_derivatives write
  * them) at scale factors d_aexp / conformal times d_tau (n_tau, increasing).  d_out: 7 x n_k x
  * n_tau doubles, planes S1 (ISW), S2 (Sachs-Wolfe), S3 (Doppler), S4 (polarisation), S, alpha,
- * alpha'.  Replaces extract_perturbations .. compute_source_function
- * (src/discoeb/cmb.py:17-633) as chained in compute_Cell_spectrum_from_cosmo_params
- * (:1050-1073). */
+ * alpha'.  isw_mode 0: S1 = e^-tau_c d/dtau(eta + alpha') from a natural spline in tau (the
+ * reference's compute_source_term_isw, src/discoeb/cmb.py:437-449); 1: S1 = e^-tau_c (eta' +
+ * alpha''), the algebraic form of its docstring (:406-408).  Replaces extract_perturbations ..
+ * compute_source_function (src/discoeb/cmb.py:17-633) as chained in
+ * compute_Cell_spectrum_from_cosmo_params (:1050-1073). */
 int deb_cmb_sources(const double *d_tables, int num_thermo, int cosmo, const double *d_y, const double *d_yprime, const double *d_k, int n_k,
-                    const double *d_aexp, const double *d_tau, int n_tau, const DebProblem *prob, double *d_out, double *d_work, void *stream);
+                    const double *d_aexp, const double *d_tau, int n_tau, const DebProblem *prob, int isw_mode, double *d_out, double *d_work,
+                    void *stream);
 
 /* Doubles of workspace deb_los_theta_ell needs. */
 long deb_los_workspace_doubles(int n_k, int n_tau, int ellmax, int n_fftlog, int n_k_dense);

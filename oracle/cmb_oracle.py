@@ -211,13 +211,18 @@ def polarization_terms(pt, metric, vis, yout, yprime, kmodes):
     return {"polarization_term": P, "polarization_termprime": Pp, "polarization_termpprime": Ppp}
 
 
-def source_function(pt, metric, vis, pol, kmodes, tau):
-    """S1 (ISW: e^-tau_c d/dtau(eta + alpha') from a natural spline in tau), S2 (Sachs-Wolfe),
-    S3 (Doppler), S4 (polarisation) and their sum (cmb.py:398-633)."""
+def source_function(pt, metric, vis, pol, kmodes, tau, isw="spline"):
+    """S1 (ISW), S2 (Sachs-Wolfe), S3 (Doppler), S4 (polarisation) and their sum
+    (cmb.py:398-633).  isw="spline": S1 = e^-tau_c d/dtau(eta + alpha') from a natural spline in
+    tau (the reference's current compute_source_term_isw, cmb.py:437-449); "algebraic": S1 =
+    e^-tau_c (eta' + alpha''), the form its docstring states (cmb.py:406-408)."""
     k = np.asarray(kmodes, dtype=np.float64)[:, None]
     tau = np.asarray(tau, dtype=np.float64)
-    pot = pt["eta"] + metric["alphaprime"]
-    s1 = spline(tau, pot, tau, kind=1)
+    if isw == "algebraic":
+        s1 = metric["etaprime"] + metric["alphapprime"]
+    else:
+        pot = pt["eta"] + metric["alphaprime"]
+        s1 = spline(tau, pot, tau, kind=1)
     S1 = np.exp(-vis["optical_depth"]) * s1
     S2 = vis["gvis"] * (0.25 * pt["deltag"] + 2 * metric["alphaprime"] + pol["polarization_term"] / 16 +
                         (pt["thetabprime"] + 3 / 16 * pol["polarization_termpprime"]) / k ** 2)
@@ -226,7 +231,7 @@ def source_function(pt, metric, vis, pol, kmodes, tau):
     return {"S1": S1, "S2": S2, "S3": S3, "S4": S4, "S": S1 + S2 + S3 + S4}
 
 
-def sources_from_states(yout, yprime, kmodes, aexp, tau, bgd, lmaxg, lmaxgp, lmaxr, nqmax):
+def sources_from_states(yout, yprime, kmodes, aexp, tau, bgd, lmaxg, lmaxgp, lmaxr, nqmax, isw="spline"):
     """The whole chain of compute_Cell_spectrum_from_cosmo_params steps 6-10
     (cmb.py:1050-1073) for states yout / yprime (n_k, n_tau, nvar).  bgd: dict with the
     background scalars (grhom, grhog, grhor, Omegam, Omegab, Neff, Nmnu, OmegaDE, Omegak,
@@ -241,7 +246,7 @@ def sources_from_states(yout, yprime, kmodes, aexp, tau, bgd, lmaxg, lmaxgp, lma
     nu = neutrino_perturbations(yout, yprime, a, bgd["amnu"], bgd["q"], bgd["w"], pt["iq0"], aH)
     metric = metric_perturbations(pt, nu, bq, bgd, kmodes, a)
     pol = polarization_terms(pt, metric, vis, yout, yprime, kmodes)
-    out = source_function(pt, metric, vis, pol, kmodes, tau)
+    out = source_function(pt, metric, vis, pol, kmodes, tau, isw=isw)
     out.update(metric=metric, vis=vis, pol=pol, nu=nu)
     return out
 

@@ -69,11 +69,18 @@ def main():
     out = {}
     for name in ["compute_theta_ell", "compute_Cell", "compute_source_term_sachs_wolfe", "compute_source_term_doppler",
                  "compute_source_term_polarization", "compute_metric_perturbations", "compute_polarization_terms",
-                 "compute_neutrino_perturbations"]:
+                 "compute_neutrino_perturbations", "extract_perturbations"]:
         df = pd.read_csv(f"{REF}/test_cmb/test_benchmark_{name}.csv")
         for col in df.columns[1:]:
             out[col] = df[col].values
     np.savez_compressed(f"{HERE}/cmb_chain.npz", **out)
+
+    # CAMB / CLASS D_ell^TT for DISCO-Notebook (tests/test_cmb.py:1410-1472, 1764-1859) and the
+    # reference's own per-ell relative differences to them
+    for code in ("camb", "class"):
+        df = pd.read_csv(f"{REF}/resources/{code}_benchmarks/DISCO-Notebook.csv")
+        rel = pd.read_csv(f"{REF}/test_cmb/test_cmb_vs_{code}_DISCO_Notebook_.csv")["relative_error"].values
+        np.savez_compressed(f"{HERE}/{code}_dell.npz", ref_relative_error=rel, **{c: df[c].values for c in df.columns})
     print("written:", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
 
 

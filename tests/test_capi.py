@@ -47,6 +47,14 @@ def test_argument_errors_without_gpu(pkg):
     assert rc < 0 and b"nqmax" in L.deb_last_error()
     assert L.deb_background(None, 1, 4, None, None) < 0
     assert L.deb_tau_out(None, 256, 1, None, 3, None, None) < 0
+    # the adjoint: at most 64 output times, 256 threads per unit, a mesh stride that holds the mesh
+    prob = pkg._lib.DebProblem(12, 12, 12, 17, 8, 0)
+    args = lambda nout, stride, maxlen: (None, 256, 1, None, None, None, None, 4, None, None, nout, ctypes.byref(prob), None, stride,
+                                         maxlen, None, None, None, None, None, None, None, None, None)
+    assert L.deb_evolve_adjoint(*args(65, 10, 10)) < 0 and b"64" in L.deb_last_error()
+    assert L.deb_evolve_adjoint(*args(1, 5, 10)) < 0 and b"mesh_stride" in L.deb_last_error()
+    assert L.deb_evolve_adjoint(*args(1, 10, 10)) < 0 and b"NULL" in L.deb_last_error()
+    assert L.deb_adjoint_workspace_doubles(4, 1, 256, 10, ctypes.byref(prob)) == 32 + 6 * L.deb_tables_size(256) + 4 * 11 * 192
 
 
 def test_k_grid_matches_reference_rules(pkg, oracle):
