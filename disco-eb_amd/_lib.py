@@ -47,7 +47,7 @@ SPLINES = ["tau_of_a", "a_of_tau", "xe_of_tau", "cs2a_of_tau", "tempba_of_tau", 
 EXPORTS = ["deb_version", "deb_last_error", "deb_tables_size", "deb_table_offset", "deb_background", "deb_spline_eval",
            "deb_default_threads_per_unit", "deb_evolve_workspace_doubles", "deb_evolve", "deb_evolve_replay", "deb_adjoint_workspace_doubles",
            "deb_evolve_adjoint", "deb_evolve_batched_workspace_doubles",
-           "deb_evolve_batched", "deb_tau_out", "deb_nvar", "deb_power", "deb_power_units",
+           "deb_evolve_batched", "deb_noalloc_scratch_doubles", "deb_evolve_batched_noalloc", "deb_tau_out", "deb_nvar", "deb_power", "deb_power_units",
            "deb_time_derivatives", "deb_visibility", "deb_cmb_sources_workspace_doubles", "deb_cmb_sources",
            "deb_los_workspace_doubles", "deb_los_theta_ell", "deb_contract", "deb_cell", "deb_dell"]
 
@@ -105,6 +105,11 @@ def lib():
     L.deb_evolve_batched.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
                                      ctypes.POINTER(DebProblem), ctypes.POINTER(DebSolverCfg), _vp, _vp, _vp, _vp, _vp, _vp,
                                      ctypes.POINTER(DebSchedule), _vp]
+    L.deb_noalloc_scratch_doubles.restype = ctypes.c_long
+    L.deb_noalloc_scratch_doubles.argtypes = []
+    L.deb_evolve_batched_noalloc.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
+                                             ctypes.POINTER(DebProblem), ctypes.POINTER(DebSolverCfg), _vp, _vp, _vp, _vp, _vp,
+                                             ctypes.POINTER(DebSchedule), _vp]
     L.deb_tau_out.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
     L.deb_visibility.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]
     L.deb_cmb_sources_workspace_doubles.restype = ctypes.c_long

@@ -63,6 +63,11 @@ int check_problem(const DebProblem *p) {
 }
 }  // namespace
 
+namespace deb {
+// Error reporting for entry points defined in other source files (deb_noalloc.hip).
+int capi_fail(int code, const char *msg) { return fail(code, msg); }
+}  // namespace deb
+
 extern "C" {
 
 const char *deb_version(void) { return "discoeb_amd 0.1 (gfx950)"; }

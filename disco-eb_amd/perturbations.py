@@ -119,11 +119,15 @@ def evolve_units(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, lmaxg=11, l
 
 def evolve_units_batched(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, batch_size=16, lmaxg=11, lmaxgp=11, lmaxr=11,
                          lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4, pcoeff=0.25, icoeff=0.80, dcoeff=0.0, factormax=20.0,
-                         factormin=0.3, max_steps=2048, return_full=False, step_log=False, threads=0):
+                         factormin=0.3, max_steps=2048, return_full=False, step_log=False, threads=0, static_scratch=False):
     """Low-level launch of the batched shared-step solve (deb_evolve_batched): the units
     g * batch_size .. (g + 1) * batch_size - 1, which must share a cosmology, step together.
-    Arguments as evolve_units; threads: 64, 128 or 256 threads per batch (0: library default).  Returns (y, status, n_accept, n_reject) device tensors, plus the
-    per-batch step mesh (n_units / batch_size, max_steps, 3) when step_log is set."""
+    Arguments as evolve_units; threads: 64, 128 or 256 threads per batch (0: library default).
+    static_scratch: call deb_evolve_batched_noalloc, whose workspace is the library's static
+    per-device array (no workspace tensor; at most deb_noalloc_scratch_doubles() doubles, i.e.
+    n_units x 2 nvar + 32; calls that overlap on one device race).  Returns (y, status,
+    n_accept, n_reject) device tensors, plus the per-batch step mesh (n_units / batch_size,
+    max_steps, 3) when step_log is set."""
     L = _lib.lib()
     dev = tables.device
     n_units = int(unit_k.numel())
@@ -136,17 +140,23 @@ def evolve_units_batched(tables, num_thermo, unit_cosmo, unit_k, tau_out, *, bat
     prob = _lib.DebProblem(int(lmaxg), int(lmaxgp), int(lmaxr), int(lmaxnu), int(nqmax), int(bool(return_full)))
     cfg = _lib.DebSolverCfg(float(rtol), float(atol), float(pcoeff), float(icoeff), float(dcoeff), float(factormax),
                             float(factormin), int(max_steps))
-    work = torch.empty(L.deb_evolve_batched_workspace_doubles(n_units, ctypes.byref(prob)), dtype=torch.float64, device=dev)
+    work = (None if static_scratch else
+            torch.empty(L.deb_evolve_batched_workspace_doubles(n_units, ctypes.byref(prob)), dtype=torch.float64, device=dev))
     uc = unit_cosmo.to(device=dev, dtype=torch.int32).contiguous()
     uk = unit_k.to(device=dev, dtype=torch.float64).contiguous()
     to = tau_out.to(device=dev, dtype=torch.float64).contiguous()
     log = (torch.full((n_units // batch_size, int(max_steps), 3), float("nan"), dtype=torch.float64, device=dev)
            if step_log else None)
     sched, _ = _schedule({"threads": threads}, uk)
-    _lib.check(L.deb_evolve_batched(_lib.ptr(tables), int(num_thermo), _lib.ptr(uc), _lib.ptr(uk), n_units, int(batch_size), _lib.ptr(to),
-                                    nout, ctypes.byref(prob), ctypes.byref(cfg), _lib.ptr(y), _lib.ptr(status), _lib.ptr(nacc),
-                                    _lib.ptr(nrej), _lib.ptr(work), _lib.ptr(log) if log is not None else None,
-                                    ctypes.byref(sched) if sched is not None else None, _lib.stream()))
+    lg, sr = _lib.ptr(log) if log is not None else None, ctypes.byref(sched) if sched is not None else None
+    if static_scratch:
+        _lib.check(L.deb_evolve_batched_noalloc(_lib.ptr(tables), int(num_thermo), _lib.ptr(uc), _lib.ptr(uk), n_units, int(batch_size),
+                                                _lib.ptr(to), nout, ctypes.byref(prob), ctypes.byref(cfg), _lib.ptr(y), _lib.ptr(status),
+                                                _lib.ptr(nacc), _lib.ptr(nrej), lg, sr, _lib.stream()))
+    else:
+        _lib.check(L.deb_evolve_batched(_lib.ptr(tables), int(num_thermo), _lib.ptr(uc), _lib.ptr(uk), n_units, int(batch_size),
+                                        _lib.ptr(to), nout, ctypes.byref(prob), ctypes.byref(cfg), _lib.ptr(y), _lib.ptr(status),
+                                        _lib.ptr(nacc), _lib.ptr(nrej), _lib.ptr(work), lg, sr, _lib.stream()))
     if step_log:
         return y, status, nacc, nrej, log
     return y, status, nacc, nrej

@@ -95,6 +95,24 @@ int deb_evolve_batched(const double *d_tables, int num_thermo, const int *d_unit
                        const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg, double *d_y_out, int *d_status,
                        int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const DebSchedule *sched, void *stream);
 
+/* Doubles of the static device scratch deb_evolve_batched_noalloc uses on each device (8 MiB). */
+#define DEB_NOALLOC_SCRATCH_DOUBLES (1L << 20)
+long deb_noalloc_scratch_doubles(void);
+
+/* deb_evolve_batched with no workspace argument and no device allocation: its workspace (the
+ * momentum bins and two state vectors per unit, deb_evolve_batched_workspace_doubles(n_units,
+ * prob) doubles) is a file-scope device array of DEB_NOALLOC_SCRATCH_DOUBLES doubles, one per
+ * device, held until the process ends; larger problems are rejected (error -1).  Arguments,
+ * outputs and status as deb_evolve_batched, on the device current at the call.  Calls that
+ * overlap on one device (two streams or two host threads) share the array and race; calls
+ * ordered on one stream do not.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+int deb_evolve_batched_noalloc(const double *d_tables, int num_thermo, const int *d_unit_cosmo, const double *d_unit_k, int n_units,
+                               int batch_size, const double *d_tau_out, int nout, const DebProblem *prob, const DebSolverCfg *cfg,
+                               double *d_y_out, int *d_status, int *d_n_accept, int *d_n_reject, double *d_step_log,
+                               const DebSchedule *sched, void *stream);
+
 /* deb_evolve on a frozen step mesh: unit u takes exactly d_mesh_len[u] steps whose ends are
  * t0 + s (t1 - t0) for the increasing s = d_mesh[u * mesh_stride + i] (the last one 1), with
  * no error control or rejection; t0 and t1 are the unit's own start and final output times.
