@@ -171,3 +171,29 @@ def test_adiabatic_sound_speed_of_the_fluid(oracle):
         assert abs(ca2_num - ca2_exact) < 1e-8
         assert abs(ca2_rhs - ca2_ref) < 1e-9 * max(1.0, abs(ca2_ref))
         assert abs(ca2_ref - ca2_exact) > 1e-7 * abs(ca2_exact - wq)  # the +1e-6 is there, and resolved
+
+
+@pytest.mark.parametrize("w0,wa", [(-0.95, 0.1), (-1.1, -0.1)])
+def test_superhorizon_fluid_attractor(oracle, w0, wa):
+    """Outside the horizon in the radiation era the c_s^2 = 1 fluid follows the attractor
+    delta_Q / delta_c = (1 + w)(4 - 3 c_s^2) / (4 - 6 w + 3 c_s^2) (BL10 eqs. 3.5-3.7 with
+    delta_c = -(k tau)^2 C / 4), w = w0 + wa (1 - a) ~ w0 + wa at a << 1.  The reference's initial
+    delta_Q (perturbations.py:732) has the opposite sign to BL10 eq. 3.7 and CLASS, so the solution
+    first relaxes onto the attractor; from a = 3e-7 the ratio minus its linear-in-a matter
+    correction, extrapolated to a -> 0, must sit on it.  A dropped or mis-signed (c_s^2 - c_a^2),
+    (1 + w) or h' term of the fluid equations moves the attractor by O(1)."""
+    P = dict(disco_notebook(), w_DE_0=w0, w_DE_a=wa, Nmnu=0.0, mnu=0.0, Neff=3.046)
+    bg = oracle.Background(P, 1024)
+    aexp = np.concatenate([[1e-9], np.geomspace(3e-7, 3e-6, 6)])
+    y, tau, st, _, _ = bg.evolve(np.array([1e-4]), aexp, lmaxg=12, lmaxgp=12, lmaxr=12, lmaxnu=8, nqmax=3, rtol=1e-9, atol=1e-14,
+                                 return_full=True)
+    assert (st == 0).all() and (1e-4 * tau < 1e-2).all()
+    n = y.shape[-1]
+    w = w0 + wa * (1 - aexp)
+    pred = (1 + w) / (4 - 6 * w + 3)
+    r = y[0, :, n - 2] / y[0, :, 3]
+    assert r[0] / pred[0] < 0  # the reference's initial sign
+    slope, at0 = np.polyfit(aexp[1:], r[1:] / pred[1:], 1)
+    print(f"  w0 {w0} wa {wa}: attractor at a -> 0 {at0 - 1:+.1e}, matter correction {slope:.0f} a")
+    assert abs(at0 - 1) < 2e-3
+    assert np.abs(r[1:] / pred[1:] - 1).max() < 2e-2
