@@ -267,6 +267,48 @@ def measure_los(deb, nk=3000, nt=2500, ellmax=2500, nfft=16384):
                          "traffic": measured_traffic(f"contract_kernel {nk} x {ellmax + 1} x {K}")}}
 
 
+def measure_cmb_chain(deb, nk=3000, ellmax=2500, nfft=16384, k_chunk=500):
+    """The CMB workload end to end at its own sizes: flat LCDM with massless neutrinos, photon
+    hierarchies to l = 24 (lmaxr 12), 3000 k on geomspace(1e-5, 0.5), 2500 output times on the
+    recombination-weighted split, rtol = atol = 1e-4 (the C_l driver's tolerances), perturbations
+    -> time derivatives -> sources streamed over k in chunks of k_chunk modes (full states of one
+    chunk at a time), line of sight for l <= 2500 at n_fftlog 16384, C_l^TT.  Per-mode adaptive
+    steps, and beside them the reference's form: modes in batches on shared steps (8 per batch,
+    since 3000 is not a multiple of the reference's 16)."""
+    import torch
+    dev = torch.device("cuda", 0)
+    P = dict(workload_param(), Neff=3.046, Nmnu=0.0, mnu=0.0, w_DE_0=-1.0, w_DE_a=0.0)
+    aexp = deb.cmb_aexp_out(250, 500, 1000, 750)
+    lm = dict(lmaxg=24, lmaxgp=24, lmaxr=12, lmaxnu=8, nqmax=3)
+    k = torch.as_tensor(np.geomspace(1e-5, 0.5, nk), dtype=torch.float64, device=dev)
+    res = {"shape": {"k_modes": nk, "n_tau": int(aexp.size), "ellmax": ellmax, "n_fftlog": nfft, "nvar": 99, "k_chunk": k_chunk,
+                     "rtol": 1e-4, "atol": 1e-4}}
+    deb.evolve_background(param=dict(P), num_thermo=1024)  # warm-up: first use of the background kernel at this size
+    for solver in ("per_mode", "batched"):
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        t = [time.perf_counter()]
+        p = deb.evolve_background(param=dict(P), num_thermo=1024)
+        torch.cuda.synchronize()
+        t.append(time.perf_counter())
+        extra = dict(batch_size=8, k_chunk=504) if solver == "batched" else dict(k_chunk=k_chunk)
+        src = deb.compute_sources_streamed(p, k, aexp, solver=solver, **extra, **lm)
+        torch.cuda.synchronize()
+        t.append(time.perf_counter())
+        th, kk = deb.compute_theta_ell(ellmax, k, src["tau"], src["S"], float(p["tau0"]), n_fftlog=nfft)
+        cl = deb.compute_Cell(th, kk, p["n_s"], p["k_p"])
+        torch.cuda.synchronize()
+        t.append(time.perf_counter())
+        total = t[-1] - t[0]
+        res[solver] = {"s": total, "background_s": t[1] - t[0], "perturbations_sources_s": t[2] - t[1], "los_cell_s": t[3] - t[2],
+                       "k_solves_per_s": nk / (t[2] - t[1]), "accepted_steps_mean": float(src["n_accept"].double().mean()),
+                       "peak_device_bytes": int(torch.cuda.max_memory_allocated()), "cell_finite": bool(torch.isfinite(cl[2:]).all()),
+                       **extra}
+        del src, th, cl
+    res["full_states_bytes"] = nk * int(aexp.size) * 99 * 8
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +320,7 @@ def main():
     ap.add_argument("--grad-cosmologies", type=int, default=16,
                     help="cosmologies of the batched gradient measurement, at --grad-modes k-modes (0: skip)")
     ap.add_argument("--los", type=int, default=1, help="measure the line-of-sight projection (0: skip)")
+    ap.add_argument("--cmb-chain", type=int, default=1, help="measure the CMB chain at 3000 k x 2500 tau, l <= 2500 (0: skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a known-size kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -369,6 +412,8 @@ def main():
             out["gradient_many"] = measure_gradient_many(deb, args.grad_cosmologies, args.grad_modes)
     if args.los:
         out["line_of_sight"] = measure_los(deb)
+    if args.cmb_chain:
+        out["cmb_chain"] = measure_cmb_chain(deb)
     if args.cpu_sample > 0 and rank == 0:
         out["cpu_baseline"] = cpu_baseline(kmodes, args.cpu_sample, args.cpu_sample_1core)
     print(json.dumps(out))

@@ -135,24 +135,91 @@ def cmb_aexp_out(n_early=32, n_pre_recomb=64, n_recomb=128, n_post=96):
                            _geomspace(a2, a3, n_recomb, endpoint=False), _geomspace(a3, 1.0, n_post)])
 
 
+def compute_sources_streamed(param, kmodes, aexp_out, *, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4,
+                             max_steps=2048, solver="per_mode", batch_size=16, k_chunk=512, isw="spline"):
+    """The source function S(k, tau) of the chain perturbations -> time derivatives -> sources
+    (cmb.py:1030-1073) computed k_chunk modes at a time, so that the full states and their
+    derivatives (n_k x n_tau x nvar each) never exist at once: only one chunk's, plus the
+    (n_k, n_tau) source planes.  Every stage is independent across k (the ISW spline runs along
+    tau), so the result equals the unchunked chain's.  solver: "per_mode" (adaptive steps per
+    mode, largest k first) or "batched" (batch_size modes on shared steps, as the reference's
+    driver; k_chunk then a multiple of batch_size).  param: from evolve_background.  Returns
+    {"S": (n_k, n_tau), "S1".."S4", "tau": tau_out, "n_accept": (n_k,), "n_reject": (n_k,)}."""
+    from .perturbations import evolve_units, evolve_units_batched, DEFAULT_SCHEDULE, _raise_on_status
+    if solver not in ("per_mode", "batched"):
+        raise ValueError("solver must be 'per_mode' or 'batched'")
+    tables = param["_deb_tables"]
+    dev = tables.device
+    kk = torch.as_tensor(kmodes, dtype=torch.float64, device=dev).reshape(-1)
+    aexp = torch.as_tensor(np.asarray(aexp_out, dtype=np.float64) if not torch.is_tensor(aexp_out) else aexp_out, dtype=torch.float64,
+                           device=dev).reshape(-1)
+    n_k, n_tau = kk.numel(), aexp.numel()
+    k_chunk = int(k_chunk) if k_chunk else n_k
+    if solver == "batched" and (k_chunk % batch_size or n_k % batch_size):
+        raise ValueError("with solver='batched', num_k and k_chunk must be multiples of batch_size")
+    tau_out = param["tau_of_a_spline"].evaluate(aexp)
+    cosmo = int(param.get("_deb_cosmo", 0))
+    n_cosmo = tables.shape[0] if tables.ndim == 2 else 1
+    tau_all = torch.zeros((n_cosmo, n_tau), dtype=torch.float64, device=dev)
+    tau_all[cosmo] = tau_out
+    lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
+    out = {name: torch.empty((n_k, n_tau), dtype=torch.float64, device=dev) for name in ("S", "S1", "S2", "S3", "S4")}
+    nacc = torch.empty(n_k, dtype=torch.int32, device=dev)
+    nrej = torch.empty(n_k, dtype=torch.int32, device=dev)
+    N = param["_deb_num_thermo"]
+    for k0 in range(0, n_k, k_chunk):
+        k1 = min(n_k, k0 + k_chunk)
+        kc = kk[k0:k1].contiguous()
+        uc = torch.full((k1 - k0,), cosmo, dtype=torch.int32, device=dev)
+        kw = dict(rtol=rtol, atol=atol, max_steps=max_steps, return_full=True, **lm)
+        if solver == "per_mode":
+            y, st, na, nr = evolve_units(tables, N, uc, kc, tau_all, schedule=DEFAULT_SCHEDULE, **kw)
+        else:
+            y, st, na, nr = evolve_units_batched(tables, N, uc, kc, tau_all, batch_size=batch_size, **kw)
+        _raise_on_status(st)
+        yp = compute_time_derivatives(y, tau_out, kc, param, lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax)
+        src = compute_source_function_from_states(y, yp, kc, aexp, tau_out, param, isw=isw, **lm)
+        for name in out:
+            out[name][k0:k1] = src[name]
+        nacc[k0:k1], nrej[k0:k1] = na, nr
+        del y, yp, src
+    out.update(tau=tau_out, n_accept=nacc, n_reject=nrej)
+    return out
+
+
 def compute_Cell_spectrum_from_cosmo_params(param_dict, ellmax=2500, nmodes=128, kmin=1e-4, kmax=1.0, n_k_dense=8192, n_fftlog=16384,
-                                            lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3):
+                                            lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, *, k_sampling_method="camb",
+                                            solver="batched", k_chunk=None, aexp_out=None, rtol=1e-4, atol=1e-4):
     """TT C_l from cosmological parameters (cmb.py:930-1090): background (num_thermo 1024),
     perturbations on the camb k-sampling at rtol = atol = 1e-4 with full states, integrated in
     batches of 16 modes on one shared step sequence as the reference does (nmodes a multiple of
-    16), time derivatives, source function, line of sight, C_l.  Returns (Cell, param)."""
+    16), time derivatives, source function, line of sight, C_l.  Returns (Cell, param).
+
+    Keyword-only extensions (the defaults are the reference's behaviour): k_sampling_method;
+    solver "per_mode" for independent adaptive steps per mode; k_chunk to stream the chain over
+    k (compute_sources_streamed) so that full states are held for k_chunk modes at a time;
+    aexp_out (default cmb_aexp_out()); rtol, atol."""
     param = evolve_background(param=dict(param_dict), thermo_module="RECFAST", num_thermo=1024)
-    aexp_out = cmb_aexp_out()
+    aexp_out = cmb_aexp_out() if aexp_out is None else np.asarray(aexp_out, dtype=np.float64)
     lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
-    yout, kmodes, param = evolve_perturbations_batched(param=param, aexp_out=aexp_out, kmin=kmin, kmax=kmax, num_k=nmodes, rtol=1e-4,
-                                                       atol=1e-4, return_full=True, k_sampling_method="camb", **lm)
-    tau = param["tau_out"]
-    yprime = compute_time_derivatives(yout, tau, kmodes, param, lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax)
-    src = compute_source_function_from_states(yout, yprime, kmodes, aexp_out, tau, param, **lm)
     tau0 = float(param["tau_of_a_spline"].evaluate(np.array([1.0])).cpu()[0])
-    theta, kk = compute_theta_ell(ellmax, kmodes, tau, src["S"], tau0, n_fftlog=n_fftlog, n_k_dense=n_k_dense)
+    if solver == "batched" and k_chunk is None:
+        yout, kmodes, param = evolve_perturbations_batched(param=param, aexp_out=aexp_out, kmin=kmin, kmax=kmax, num_k=nmodes, rtol=rtol,
+                                                           atol=atol, return_full=True, k_sampling_method=k_sampling_method, **lm)
+        tau = param["tau_out"]
+        yprime = compute_time_derivatives(yout, tau, kmodes, param, lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax)
+        S = compute_source_function_from_states(yout, yprime, kmodes, aexp_out, tau, param, **lm)["S"]
+    else:
+        from .perturbations import k_grid
+        kmodes = torch.as_tensor(k_grid(param, kmin, kmax, nmodes, k_sampling_method), dtype=torch.float64,
+                                 device=param["_deb_tables"].device)
+        src = compute_sources_streamed(param, kmodes, aexp_out, rtol=rtol, atol=atol, solver=solver, k_chunk=k_chunk, **lm)
+        tau, S = src["tau"], src["S"]
+        param.update(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax, nout=int(tau.numel()), tau_out=tau,
+                     _deb_steps=(src["n_accept"], src["n_reject"]))
+    theta, kk = compute_theta_ell(ellmax, kmodes, tau, S, tau0, n_fftlog=n_fftlog, n_k_dense=n_k_dense)
     return compute_Cell(theta, kk, param["n_s"], param.get("k_p", 0.05)), param
 
 
 __all__ = ["extract_perturbations", "compute_visibility_functions", "compute_source_function_from_states", "compute_theta_ell",
-           "compute_Cell", "compute_Dell", "cmb_aexp_out", "compute_Cell_spectrum_from_cosmo_params"]
+           "compute_Cell", "compute_Dell", "cmb_aexp_out", "compute_sources_streamed", "compute_Cell_spectrum_from_cosmo_params"]
