@@ -218,7 +218,7 @@ def _plain_jacobian(deb, theta, P, k, opts):
     lm = {x: opts[x] for x in ("lmaxg", "lmaxgp", "lmaxr", "lmaxnu", "nqmax")}
     sv = {x: opts[x] for x in ("rtol", "atol")}
     S = deb.grad._setup(np.asarray(theta)[None, :], P, k, aexp, lm, sv, 256, deb.grad.REL_STEP)
-    kt = S["kk"]
+    kt = S["uk0"]
     uc = torch.zeros(kt.numel(), dtype=torch.int32, device=kt.device)
     for r in range(12):
         tab = S["tab"][r:r + 1]
@@ -309,6 +309,80 @@ def measure_cmb_chain(deb, nk=3000, ellmax=2500, nfft=16384, k_chunk=500):
     return res
 
 
+def fisher_workload(n_cosmo=256):
+    """The many-cosmology workload: n_cosmo parameter vectors theta = (omega_c, omega_b, h, n_s,
+    ln 10^10 A_s, m_nu, w0, wa, tau_reion) from a Latin hypercube (scipy.stats.qmc, d = 9,
+    seed 2) spanning +-10% of the single-cosmology workload's values, and per cosmology a log
+    k-grid over 1e-4..2/Mpc of n_k = default_rng(3).integers(128, 769) modes."""
+    from scipy.stats import qmc
+    th0 = np.array([0.120, 0.0224, 0.674, 0.965, np.log(21.0), 0.06, -0.95, 0.1, 0.054])
+    u = qmc.LatinHypercube(d=9, seed=2).random(n_cosmo)
+    thetas = th0 + (2.0 * u - 1.0) * 0.1 * np.abs(th0)
+    nks = np.random.default_rng(3).integers(128, 769, size=n_cosmo)
+    return thetas, [np.geomspace(1e-4, 2.0, int(n)) for n in nks]
+
+
+def measure_fisher(deb, n_cosmo=256, n_latency=16, n_plain=3):
+    """The many-cosmology workload end to end: P(k, z = 0) and its 9-column Jacobian for every
+    cosmology (adaptive base solves of all units in one launch, the discrete adjoint of all units
+    in k-ordered launches sized to their checkpoints), with the forward alone as a captured graph
+    beside it; per-request latency (one cosmology per call) p50/p99 over a sample; the plain form
+    (one cosmology per call, base solve and twelve shifted-cosmology replay launches) on a
+    sample, scaled to the batch."""
+    import torch
+    P = workload_param()
+    thetas, grids = fisher_workload(n_cosmo)
+    units = int(sum(g.size for g in grids))
+    res = {"cosmologies": n_cosmo, "units": units, "n_k_range": [int(min(g.size for g in grids)), int(max(g.size for g in grids))],
+           "seeds": {"latin_hypercube": 2, "n_k": 3}, "truncation": "lmax 12, nq 8, lmax_nu 17", "rtol": 1e-5, "atol": 1e-10}
+    # forward only, captured once and replayed
+    params = [deb.theta_to_param(th, P) for th in thetas]
+    g = deb.PkGraph(params, grids, (1.0,), schedule={"order": "k_desc", "threads": 128}, **LMAX, **SOLVER)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    pk, st = g.replay()
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t
+    phantom = int(sum(bool((x == 4).any()) for x in g.split(st)))
+    res["forward_graph"] = {"s": fwd, "k_solves_per_s": units / fwd, "phantom_cosmologies": phantom}
+    del g
+    # forward + Jacobian of every cosmology
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    pks, Js, status = deb.pk_jacobian_many(thetas, P, grids, aexp_out=(1.0,), return_status=True, method="adjoint", **LMAX, **SOLVER)
+    torch.cuda.synchronize()
+    tot = time.perf_counter() - t
+    valid = [c for c in range(n_cosmo) if status[c] == 0]
+    res["jacobian"] = {"s": tot, "gradients_per_s": n_cosmo / tot, "k_modes_per_s": units / tot,
+                       "status_counts": {str(k): int(v) for k, v in zip(*np.unique(status, return_counts=True))},
+                       "phantom_cosmologies": int((status == 4).sum()), "finite": bool(all(torch.isfinite(Js[c]).all() for c in valid))}
+    # per-request latency: one cosmology per call
+    lat = []
+    for c in valid[:n_latency]:
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        deb.pk_jacobian(thetas[c], P, grids[c], aexp_out=(1.0,), **LMAX, **SOLVER)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t)
+    res["latency_one_per_call"] = {"sample": len(lat), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+                                   "n_k": [int(grids[c].size) for c in valid[:n_latency]]}
+    # the plain form on a sample, scaled by modes
+    opts = dict(aexp_out=(1.0,), **LMAX, **SOLVER)
+    tp, kp = 0.0, 0
+    for c in valid[:n_plain]:
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _plain_jacobian(deb, thetas[c], P, grids[c], opts)
+        torch.cuda.synchronize()
+        tp += time.perf_counter() - t
+        kp += grids[c].size
+    est = tp / kp * sum(grids[c].size for c in valid)
+    res["plain"] = {"sample": n_plain, "s_sample": tp, "s_estimated_for_batch": est,
+                    "form": "one cosmology per call, base solve + 12 shifted-cosmology replays, one launch each (difference Jacobian)"}
+    res["speedup_vs_plain"] = est / tot
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,6 +395,8 @@ def main():
                     help="cosmologies of the batched gradient measurement, at --grad-modes k-modes (0: skip)")
     ap.add_argument("--los", type=int, default=1, help="measure the line-of-sight projection (0: skip)")
     ap.add_argument("--cmb-chain", type=int, default=1, help="measure the CMB chain at 3000 k x 2500 tau, l <= 2500 (0: skip)")
+    ap.add_argument("--fisher", type=int, default=0,
+                    help="also measure the 256-cosmology forward + Jacobian workload (about two minutes; 0: skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a known-size kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -414,6 +490,8 @@ def main():
         out["line_of_sight"] = measure_los(deb)
     if args.cmb_chain:
         out["cmb_chain"] = measure_cmb_chain(deb)
+    if args.fisher:
+        out["fisher"] = measure_fisher(deb)
     if args.cpu_sample > 0 and rank == 0:
         out["cpu_baseline"] = cpu_baseline(kmodes, args.cpu_sample, args.cpu_sample_1core)
     print(json.dumps(out))

@@ -588,6 +588,11 @@ __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict_
   const double t0 = 0.99 * fmin(tmin, starting_time(b, k));
   const double *rs = mesh + (long)unit * mstride;
   const int N = mesh_len[unit];
+  if (N < 1 || N > mstride || (long)(N + 1) * n > cstride) {  // the mesh row or the checkpoint slot would overflow
+    if (tid < NPAR) gu[tid] = __builtin_nan("");
+    if (tid == 0) status[unit] = 7;
+    return;
+  }
   auto mesh_s = [&](int i) { return i == 0 ? 0.0 : (rs[i - 1] >= 1.0 ? 1.0 : rs[i - 1]); };  // point i of 0..N
   auto mesh_t = [&](int i) { return i == 0 ? t0 : (rs[i - 1] >= 1.0 ? t1 : t0 + rs[i - 1] * (t1 - t0)); };
   double *ck = ckpt + (long)unit * cstride;

@@ -112,11 +112,6 @@ def _shift_pair(theta, j, d):
     return hi, lo
 
 
-def _cosmo_status(unit_status, C, nk):
-    """Per-cosmology flag from the per-unit status of C x nk units (4: phantom divide)."""
-    return (unit_status.reshape(C, nk) == 4).any(dim=1)
-
-
 def evolve_units_adjoint(tables, num_thermo, shifted, dtheta, unit_cosmo, unit_k, tau_out, tau_out_shifted, mesh, mesh_len, gbar, *,
                          lmaxg=12, lmaxgp=12, lmaxr=12, lmaxnu=17, nqmax=8, max_ckpt_bytes=8 << 30, return_parts=False):
     """Low-level launch of the discrete adjoint (deb_evolve_adjoint) over n_units units.
@@ -124,7 +119,8 @@ def evolve_units_adjoint(tables, num_thermo, shifted, dtheta, unit_cosmo, unit_k
     tables (C, size) and shifted (C * 12, size) background tables, shifted block 2 p (+) and 2 p + 1
     (-) of cosmology c at theta_p +- dtheta[c, p]; tau_out (C, nout), tau_out_shifted (C * 12, nout);
     mesh / mesh_len as evolve_units(replay=...); gbar (n_units, nout, 20) output-field cotangents.
-    Units are launched largest k first, in chunks whose checkpoints fit max_ckpt_bytes.
+    Units are launched largest k first, in consecutive chunks of that order whose checkpoints
+    ((chunk's longest mesh + 1) x nvar doubles per unit) fit max_ckpt_bytes.
     Returns (grad (n_units, 6), status (n_units,)), plus (lam0 (n_units, nvar), tbar (n_units, 2))
     when return_parts is set."""
     L = _lib.lib()
@@ -137,7 +133,6 @@ def evolve_units_adjoint(tables, num_thermo, shifted, dtheta, unit_cosmo, unit_k
     mesh = mesh.to(device=dev, dtype=torch.float64).contiguous()
     mlen = mesh_len.to(device=dev, dtype=torch.int32).contiguous()
     _check_mesh(mesh, mlen)
-    Lm = int(mlen.max()) if U else 1
     gbar = gbar.to(device=dev, dtype=torch.float64).contiguous()
     if gbar.shape != (U, nout, 20):
         raise ValueError("gbar must be (n_units, nout, 20)")
@@ -150,41 +145,66 @@ def evolve_units_adjoint(tables, num_thermo, shifted, dtheta, unit_cosmo, unit_k
     status = torch.empty(U, dtype=torch.int32, device=dev)
     lam0 = torch.empty((U, n), dtype=torch.float64, device=dev) if return_parts else None
     tbar = torch.empty((U, 2), dtype=torch.float64, device=dev) if return_parts else None
-    chunk = max(1, min(U, int(max_ckpt_bytes // ((Lm + 1) * n * 8))))
     to = tau_out.to(device=dev, dtype=torch.float64).contiguous()
     tsh = tau_out_shifted.to(device=dev, dtype=torch.float64).contiguous()
-    for s0 in range(0, U, chunk):
-        s1 = min(U, s0 + chunk)
+    order = torch.argsort(uk, descending=True, stable=True)
+    ml = mlen[order].cpu().numpy().astype(np.int64) if U else np.zeros(0, dtype=np.int64)
+    per = n * 8
+    s0 = 0
+    while s0 < U:  # greedy chunks over the k-descending order
+        m, s1 = int(ml[s0]), s0 + 1
+        while s1 < U and (s1 - s0 + 1) * (max(m, int(ml[s1])) + 1) * per <= max_ckpt_bytes:
+            m = max(m, int(ml[s1]))
+            s1 += 1
+        idx = order[s0:s1]
         cnt = s1 - s0
-        work = torch.empty(L.deb_adjoint_workspace_doubles(cnt, C, int(num_thermo), Lm, ctypes.byref(prob)), dtype=torch.float64, device=dev)
-        order = torch.argsort(uk[s0:s1], descending=True, stable=True).to(torch.int32)
-        sched = _lib.DebSchedule(order.data_ptr(), 256)
-        _lib.check(L.deb_evolve_adjoint(_lib.ptr(tables), int(num_thermo), C, _lib.ptr(shifted), _lib.ptr(dth), _lib.ptr(uc[s0:s1]),
-                                        _lib.ptr(uk[s0:s1]), cnt, _lib.ptr(to), _lib.ptr(tsh), nout, ctypes.byref(prob),
-                                        _lib.ptr(mesh[s0:s1]), mesh.shape[1], Lm, _lib.ptr(mlen[s0:s1]), _lib.ptr(gbar[s0:s1]),
-                                        _lib.ptr(grad[s0:s1]), _lib.ptr(status[s0:s1]), _lib.ptr(work),
-                                        _lib.ptr(lam0[s0:s1]) if lam0 is not None else None,
-                                        _lib.ptr(tbar[s0:s1]) if tbar is not None else None, ctypes.byref(sched), _lib.stream()))
-        del order, work  # stream-ordered reuse only after the launch has run
+        cuc, cuk = uc[idx].contiguous(), uk[idx].contiguous()
+        cmesh, cmlen, cgbar = mesh[idx, :m].contiguous(), mlen[idx].contiguous(), gbar[idx].contiguous()
+        cgrad = torch.empty((cnt, 6), dtype=torch.float64, device=dev)
+        cst = torch.empty(cnt, dtype=torch.int32, device=dev)
+        clam = torch.empty((cnt, n), dtype=torch.float64, device=dev) if return_parts else None
+        ctb = torch.empty((cnt, 2), dtype=torch.float64, device=dev) if return_parts else None
+        work = torch.empty(L.deb_adjoint_workspace_doubles(cnt, C, int(num_thermo), m, ctypes.byref(prob)), dtype=torch.float64, device=dev)
+        sched = _lib.DebSchedule(None, 256)
+        _lib.check(L.deb_evolve_adjoint(_lib.ptr(tables), int(num_thermo), C, _lib.ptr(shifted), _lib.ptr(dth), _lib.ptr(cuc), _lib.ptr(cuk),
+                                        cnt, _lib.ptr(to), _lib.ptr(tsh), nout, ctypes.byref(prob), _lib.ptr(cmesh), m, m,
+                                        _lib.ptr(cmlen), _lib.ptr(cgbar), _lib.ptr(cgrad), _lib.ptr(cst), _lib.ptr(work),
+                                        _lib.ptr(clam) if clam is not None else None, _lib.ptr(ctb) if ctb is not None else None,
+                                        ctypes.byref(sched), _lib.stream()))
+        grad[idx], status[idx] = cgrad, cst
+        if return_parts:
+            lam0[idx], tbar[idx] = clam, ctb
+        del work, cmesh  # stream-ordered reuse only after the launch has run
+        s0 = s1
     if return_parts:
         return grad, status, lam0, tbar
     return grad, status
 
 
+def _kgrids(kmodes, C):
+    """Per-cosmology k-grids: a list of C one-dimensional grids (ragged), or one grid shared by all.
+    Returns (list of C float64 arrays, ragged flag)."""
+    if isinstance(kmodes, (list, tuple)) and len(kmodes) == C and C > 0 and np.ndim(kmodes[0]) == 1:
+        return [np.asarray(k, dtype=np.float64).reshape(-1) for k in kmodes], True
+    knp = np.asarray(kmodes, dtype=np.float64).reshape(-1)
+    return [knp] * C, False
+
+
 def _setup(thetas, base, kmodes, aexp, lm, solver, num_thermo, rel_step):
-    """Base solves (with step logs) of C cosmologies on one k-grid, their accepted meshes, and the
-    twelve shifted cosmologies per cosmology (tables and output times)."""
+    """Base solves (with step logs) of C cosmologies, each on its own k-grid or on one shared grid,
+    their accepted meshes, and the twelve shifted cosmologies per cosmology (tables and output
+    times).  Units are cosmology-major: cosmology c owns units offs[c]..offs[c + 1] - 1."""
     C = thetas.shape[0]
     p0s = [theta_to_param(th, base) for th in thetas]
     dev = _lib.device()
-    knp = np.asarray(kmodes, dtype=np.float64)
-    nk = knp.size
+    kgrids, ragged = _kgrids(kmodes, C)
     t0 = background_tables(p0s, num_thermo)
     tau0 = tau_out_all(t0, num_thermo, aexp)
-    uc0, uk0, _ = units_for([knp] * C, dev)
-    y, st, na, nr, log = evolve_units(t0, num_thermo, uc0, uk0, tau0, **lm, **solver, step_log=True, schedule=_base_schedule(C * nk))
+    uc0, uk0, offs = units_for(kgrids, dev)
+    U = int(uk0.numel())
+    y, st, na, nr, log = evolve_units(t0, num_thermo, uc0, uk0, tau0, **lm, **solver, step_log=True, schedule=_base_schedule(U))
     _raise_on_status(st, allow_phantom=True)
-    bad = _cosmo_status(st, C, nk).cpu().numpy()
+    bad = (torch.zeros(C, dtype=torch.float64, device=dev).index_add_(0, uc0.to(torch.int64), (st == 4).to(torch.float64)) > 0).cpu().numpy()
     t1 = tau0.amax(dim=1)[uc0.to(torch.int64)]
     mesh, mlen = _accepted_mesh(log, na, t1)
     # phantom units never stepped: give them a one-step mesh (nothing is replayed for them)
@@ -203,8 +223,8 @@ def _setup(thetas, base, kmodes, aexp, lm, solver, num_thermo, rel_step):
                 params.append(theta_to_param(th, base))
     tab = background_tables(params, num_thermo)
     tau = tau_out_all(tab, num_thermo, aexp)
-    return dict(p0s=p0s, t0=t0, tau0=tau0, uc0=uc0, uk0=uk0, y=y, bad=bad, mesh=mesh, mlen=mlen, steps=steps, tab=tab, tau=tau, nk=nk,
-                kk=torch.as_tensor(knp, device=dev))
+    return dict(p0s=p0s, t0=t0, tau0=tau0, uc0=uc0, uk0=uk0, y=y, bad=bad, mesh=mesh, mlen=mlen, steps=steps, tab=tab, tau=tau,
+                offs=offs, kgrids=kgrids, ragged=ragged)
 
 
 def _power_gbar(y, pk, idx, cot):
@@ -214,21 +234,32 @@ def _power_gbar(y, pk, idx, cot):
     return g
 
 
+def _per_cosmology(S, x):
+    """Unit-major tensor x (n_units, ...) as (C, n_k, ...) on a shared grid, else a list of C tensors."""
+    C = len(S["p0s"])
+    if not S["ragged"]:
+        return x.reshape(C, -1, *x.shape[1:])
+    o = S["offs"]
+    return [x[o[c]:o[c + 1]] for c in range(C)]
+
+
 def pk_jacobian_many(thetas, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, num_thermo=256, schedule=None, lmaxg=12,
                      lmaxgp=12, lmaxr=12, lmaxnu=17, nqmax=8, return_status=False, method="adjoint", **solver):
     """P(k) at aexp_out and its Jacobian with respect to theta for C cosmologies at once.
 
-    thetas: (C, 9); every cosmology uses the same k-modes.  The adaptive base solves of all
-    C x n_k units run in one launch (with their step logs).  method="adjoint": one adjoint launch
-    per output time gives every unit's six ODE-parameter derivatives (the rows of the Jacobian);
-    method="fd": one replay launch of the 12 C x n_k shifted units.  The result of cosmology c
-    does not depend on the others (units are independent).  A cosmology whose w(a) crosses -1
-    does not stop the batch: its P(k) and Jacobian are NaN (status 4); with method="fd" a column
-    whose shifted partner crosses -1 is NaN (status 6).  At an exact cosmological constant the w0
-    and wa columns are taken by differences (method "fd"): the fluid's perturbations vanish there,
+    thetas: (C, 9).  kmodes: one k-grid shared by every cosmology, or a list of C grids (each
+    cosmology its own, of any length).  The adaptive base solves of all units run in one launch
+    (with their step logs).  method="adjoint": one adjoint launch per output time gives every
+    unit's six ODE-parameter derivatives (the rows of the Jacobian); method="fd": one replay
+    launch of the twelve shifted copies of every unit.  The result of cosmology c does not depend
+    on the others (units are independent).  A cosmology whose w(a) crosses -1 does not stop the
+    batch: its P(k) and Jacobian are NaN (status 4); with method="fd" a column whose shifted
+    partner crosses -1 is NaN (status 6).  At an exact cosmological constant the w0 and wa
+    columns are taken by differences (method "fd"): the fluid's perturbations vanish there,
     while their response to w is first order.
-    Returns (pk (C, n_k, nout), J (C, n_k, nout, 9)) device tensors, and the per-cosmology
-    status (C,) int32 numpy array when return_status is set."""
+    Returns (pk (C, n_k, nout), J (C, n_k, nout, 9)) device tensors on a shared grid, lists of C
+    tensors (n_k_c, nout) and (n_k_c, nout, 9) on ragged grids, and the per-cosmology status (C,)
+    int32 numpy array when return_status is set."""
     if method not in ("adjoint", "fd"):
         raise ValueError("method must be 'adjoint' or 'fd'")
     thetas = np.atleast_2d(np.asarray(thetas, dtype=np.float64))
@@ -238,54 +269,68 @@ def pk_jacobian_many(thetas, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_
     lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
     aexp = np.atleast_1d(np.asarray(aexp_out, dtype=np.float64))
     S = _setup(thetas, base, kmodes, aexp, lm, solver, num_thermo, rel_step)
-    nk, kk, bad = S["nk"], S["kk"], S["bad"]
-    dev = kk.device
-    pku = power_units(S["t0"], num_thermo, S["uc0"], S["uk0"], S["y"], idx)
-    pk = pku.reshape(C, nk, -1)
-    nout = pk.shape[2]
-    J = torch.zeros((C, nk, nout, 9), dtype=torch.float64, device=dev)
+    bad, offs, uc0, uk0 = S["bad"], S["offs"], S["uc0"], S["uk0"]
+    dev = uk0.device
+    U = int(uk0.numel())
+    pku = power_units(S["t0"], num_thermo, uc0, uk0, S["y"], idx)
+    nout = pku.shape[1]
+    J = torch.zeros((U, nout, 9), dtype=torch.float64, device=dev)
     status = np.zeros(C, dtype=np.int32)
     nan = float("nan")
     reps = 2 * len(ODE_PARAMS)
+    nks = np.diff(offs)
     if method == "fd":
-        uc, uk, _ = units_for([kk.cpu().numpy()] * (C * reps), dev)
-        Lm = S["mesh"].shape[1]
-        rmesh = S["mesh"].reshape(C, 1, nk, Lm).expand(C, reps, nk, Lm).reshape(C * reps * nk, Lm)
-        rlen = S["mlen"].reshape(C, 1, nk).expand(C, reps, nk).reshape(-1)
-        yr, str_, _, _ = evolve_units(S["tab"], num_thermo, uc, uk, S["tau"], **lm, **solver,
-                                      schedule=REPLAY_SCHEDULE if schedule is None else schedule, replay=(rmesh, rlen))
+        # replay units: cosmology-major, then shifted copy r, then the cosmology's k-modes
+        base_idx = np.concatenate([np.tile(np.arange(offs[c], offs[c + 1]), reps) for c in range(C)]).astype(np.int64)
+        rep_uc = np.concatenate([np.repeat(c * reps + np.arange(reps), nks[c]) for c in range(C)]).astype(np.int32)
+        start = np.concatenate([[0], np.cumsum(reps * nks)])[:-1]
+        pos = np.arange(U) - np.repeat(offs[:-1], nks)
+        ucn = np.repeat(np.arange(C), nks)
+        bi = torch.as_tensor(base_idx, device=dev)
+        uc = torch.as_tensor(rep_uc, device=dev)
+        yr, str_, _, _ = evolve_units(S["tab"], num_thermo, uc, uk0[bi].contiguous(), S["tau"], **lm, **solver,
+                                      schedule=REPLAY_SCHEDULE if schedule is None else schedule,
+                                      replay=(S["mesh"][bi].contiguous(), S["mlen"][bi].contiguous()))
         _raise_on_status(str_, allow_phantom=True)
-        colbad = _cosmo_status(str_, C * reps, nk).reshape(C, len(ODE_PARAMS), 2).any(dim=2).cpu().numpy()
-        pkr = power_units(S["tab"], num_thermo, uc, uk, yr, idx).reshape(C, reps, nk, -1)
+        rbad = (torch.zeros(C * reps, dtype=torch.float64, device=dev).index_add_(0, uc.to(torch.int64), (str_ == 4).to(torch.float64))
+                > 0).reshape(C, len(ODE_PARAMS), 2).any(dim=2).cpu().numpy()
+        pkr = power_units(S["tab"], num_thermo, uc, uk0[bi].contiguous(), yr, idx)
+        stp = torch.as_tensor(S["steps"], dtype=torch.float64, device=dev)[torch.as_tensor(ucn, device=dev)]
+        for i, j in enumerate(ODE_PARAMS):
+            plus = torch.as_tensor(start[ucn] + (2 * i) * nks[ucn] + pos, device=dev)
+            minus = torch.as_tensor(start[ucn] + (2 * i + 1) * nks[ucn] + pos, device=dev)
+            J[:, :, j] = (pkr[plus] - pkr[minus]) / (2.0 * stp[:, i:i + 1])
         for c in range(C):
             for i, j in enumerate(ODE_PARAMS):
-                J[c, :, :, j] = (pkr[c, 2 * i] - pkr[c, 2 * i + 1]) / (2.0 * S["steps"][c, i])
-                if colbad[c, i]:
-                    J[c, :, :, j] = nan
+                if rbad[c, i]:
+                    J[offs[c]:offs[c + 1], :, j] = nan
                     status[c] = GRAD_COLUMN_UNDEFINED
     else:
         dth = torch.as_tensor(S["steps"], dtype=torch.float64, device=dev)
         for jo in range(nout):
-            cot = torch.zeros((C * nk, nout), dtype=torch.float64, device=dev)
+            cot = torch.zeros((U, nout), dtype=torch.float64, device=dev)
             cot[:, jo] = 1.0
             gbar = _power_gbar(S["y"], pku, idx, cot)
-            g, st = evolve_units_adjoint(S["t0"], num_thermo, S["tab"], dth, S["uc0"], S["uk0"], S["tau0"], S["tau"], S["mesh"], S["mlen"],
-                                         gbar, **lm)
-            J[:, :, jo, list(ODE_PARAMS)] = g.reshape(C, nk, len(ODE_PARAMS))
+            g, st = evolve_units_adjoint(S["t0"], num_thermo, S["tab"], dth, uc0, uk0, S["tau0"], S["tau"], S["mesh"], S["mlen"], gbar,
+                                         **lm)
+            J[:, jo, list(ODE_PARAMS)] = g
         lam = [c for c in range(C) if _is_lambda(thetas[c]) and not bad[c]]
         if lam:  # w0, wa at a cosmological constant: differences of the frozen-mesh map
-            _, Jf, stf = pk_jacobian_many(thetas[lam], base, kmodes, aexp_out=aexp_out, idx=idx, rel_step=rel_step, num_thermo=num_thermo,
+            sub = [S["kgrids"][c] for c in lam] if S["ragged"] else S["kgrids"][0]
+            _, Jf, stf = pk_jacobian_many(thetas[lam], base, sub, aexp_out=aexp_out, idx=idx, rel_step=rel_step, num_thermo=num_thermo,
                                           schedule=schedule, return_status=True, method="fd", **lm, **solver)
             for i, c in enumerate(lam):
-                J[c, :, :, 6:8] = Jf[i, :, :, 6:8]
+                J[offs[c]:offs[c + 1], :, 6:8] = Jf[i][:, :, 6:8]
                 status[c] = max(status[c], int(stf[i]))
+    kp = torch.tensor([p.get("k_p", 0.05) for p in S["p0s"]], dtype=torch.float64, device=dev)[uc0.to(torch.int64)]
+    J[:, :, 3] = pku * torch.log(uk0 / kp)[:, None]
+    J[:, :, 4] = pku
     for c in range(C):
-        J[c, :, :, 3] = pk[c] * torch.log(kk / S["p0s"][c].get("k_p", 0.05))[:, None]
-        J[c, :, :, 4] = pk[c]
         if bad[c]:
-            pk[c] = nan
-            J[c] = nan
+            pku[offs[c]:offs[c + 1]] = nan
+            J[offs[c]:offs[c + 1]] = nan
             status[c] = GRAD_PHANTOM
+    pk, J = _per_cosmology(S, pku), _per_cosmology(S, J)
     if return_status:
         return pk, J, status
     return pk, J
@@ -294,40 +339,50 @@ def pk_jacobian_many(thetas, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_
 def pk_and_vjp_many(thetas, base, kmodes, cotangents, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, num_thermo=256, lmaxg=12, lmaxgp=12,
                     lmaxr=12, lmaxnu=17, nqmax=8, **solver):
     """P(k) and the 9-parameter VJP of sum_k,z cotangent * P for C cosmologies: one base launch and
-    one adjoint launch for all of them.  cotangents: (C, n_k, nout).  Returns (pk (C, n_k, nout),
-    vjp (C, 9), status (C,)).  At an exact cosmological constant the w0 and wa entries come from
-    the difference Jacobian (see pk_jacobian_many)."""
+    one adjoint launch for all of them.  kmodes: one shared grid, with cotangents (C, n_k, nout);
+    or a list of C grids, with a list of C cotangent arrays (n_k_c, nout).  Returns (pk, vjp (C, 9),
+    status (C,)), pk shaped as the cotangents.  At an exact cosmological constant the w0 and wa
+    entries come from the difference Jacobian (see pk_jacobian_many)."""
     thetas = np.atleast_2d(np.asarray(thetas, dtype=np.float64))
     C = thetas.shape[0]
     lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
     aexp = np.atleast_1d(np.asarray(aexp_out, dtype=np.float64))
     S = _setup(thetas, base, kmodes, aexp, lm, solver, num_thermo, rel_step)
-    nk, kk, bad = S["nk"], S["kk"], S["bad"]
-    dev = kk.device
-    pku = power_units(S["t0"], num_thermo, S["uc0"], S["uk0"], S["y"], idx)
-    pk = pku.reshape(C, nk, -1)
-    cot = torch.as_tensor(np.asarray(cotangents, dtype=np.float64), device=dev).reshape(C * nk, -1)
+    bad, offs, uc0, uk0 = S["bad"], S["offs"], S["uc0"], S["uk0"]
+    dev = uk0.device
+    U = int(uk0.numel())
+    pku = power_units(S["t0"], num_thermo, uc0, uk0, S["y"], idx)
+    nout = pku.shape[1]
+    if S["ragged"]:
+        cot = torch.cat([torch.as_tensor(np.asarray(c, dtype=np.float64), device=dev).reshape(-1, nout) for c in cotangents])
+    else:
+        cot = torch.as_tensor(np.asarray(cotangents, dtype=np.float64), device=dev).reshape(-1, nout)
+    if cot.shape[0] != U:
+        raise ValueError("cotangents must match the k-grids")
     gbar = _power_gbar(S["y"], pku, idx, cot)
     dth = torch.as_tensor(S["steps"], dtype=torch.float64, device=dev)
-    g, st = evolve_units_adjoint(S["t0"], num_thermo, S["tab"], dth, S["uc0"], S["uk0"], S["tau0"], S["tau"], S["mesh"], S["mlen"], gbar,
-                                 **lm)
+    g, st = evolve_units_adjoint(S["t0"], num_thermo, S["tab"], dth, uc0, uk0, S["tau0"], S["tau"], S["mesh"], S["mlen"], gbar, **lm)
+    ucl = uc0.to(torch.int64)
     v = torch.zeros((C, 9), dtype=torch.float64, device=dev)
-    v[:, list(ODE_PARAMS)] = g.reshape(C, nk, len(ODE_PARAMS)).sum(dim=1)
-    cr = cot.reshape(C, nk, -1)
-    logk = torch.log(kk)[None, :, None] - torch.log(torch.tensor([p.get("k_p", 0.05) for p in S["p0s"]], dtype=torch.float64,
-                                                                  device=dev))[:, None, None]
-    v[:, 3] = (cr * pk * logk).sum(dim=(1, 2))
-    v[:, 4] = (cr * pk).sum(dim=(1, 2))
+    v[:, list(ODE_PARAMS)] = torch.zeros((C, len(ODE_PARAMS)), dtype=torch.float64, device=dev).index_add_(0, ucl, g)
+    kp = torch.tensor([p.get("k_p", 0.05) for p in S["p0s"]], dtype=torch.float64, device=dev)[ucl]
+    cp = (cot * pku).sum(dim=1)
+    v[:, 3] = torch.zeros(C, dtype=torch.float64, device=dev).index_add_(0, ucl, cp * torch.log(uk0 / kp))
+    v[:, 4] = torch.zeros(C, dtype=torch.float64, device=dev).index_add_(0, ucl, cp)
     status = np.where(bad, GRAD_PHANTOM, GRAD_OK).astype(np.int32)
     lam = [c for c in range(C) if _is_lambda(thetas[c]) and not bad[c]]
     if lam:
-        _, Jf, stf = pk_jacobian_many(thetas[lam], base, kmodes, aexp_out=aexp_out, idx=idx, rel_step=rel_step, num_thermo=num_thermo,
+        sub = [S["kgrids"][c] for c in lam] if S["ragged"] else S["kgrids"][0]
+        _, Jf, stf = pk_jacobian_many(thetas[lam], base, sub, aexp_out=aexp_out, idx=idx, rel_step=rel_step, num_thermo=num_thermo,
                                       return_status=True, method="fd", **lm, **solver)
         for i, c in enumerate(lam):
-            v[c, 6:8] = torch.einsum("kon,ko->n", Jf[i, :, :, 6:8], cr[c])
+            v[c, 6:8] = torch.einsum("kon,ko->n", Jf[i][:, :, 6:8], cot[offs[c]:offs[c + 1]])
             status[c] = max(status[c], int(stf[i]))
     v[torch.as_tensor(bad, device=dev)] = float("nan")
-    return pk, v, status
+    for c in range(C):
+        if bad[c]:
+            pku[offs[c]:offs[c + 1]] = float("nan")
+    return _per_cosmology(S, pku), v, status
 
 
 def pk_jacobian(theta, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_STEP, num_thermo=256, schedule=None, lmaxg=12, lmaxgp=12, lmaxr=12,
@@ -390,7 +445,7 @@ class PkFunction(torch.autograd.Function):
                                      gbar, **ctx.lm)
         v = torch.zeros(9, dtype=torch.float64, device=dev)
         v[list(ODE_PARAMS)] = gr.sum(dim=0)
-        logk = torch.log(S["kk"] / S["p0s"][0].get("k_p", 0.05))[:, None]
+        logk = torch.log(S["uk0"] / S["p0s"][0].get("k_p", 0.05))[:, None]
         v[3] = (cot * pku * logk).sum()
         v[4] = (cot * pku).sum()
         if _is_lambda(ctx.th[0]):

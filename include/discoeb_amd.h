@@ -134,7 +134,8 @@ int deb_evolve_replay(const double *d_tables, int num_thermo, const int *d_unit_
  * d_dtheta: n_cosmo x 6 steps; d_tau_out_shifted: n_cosmo x 12 x nout output times of those;
  * d_mesh, mesh_stride, d_mesh_len: as deb_evolve_replay (mesh_len <= max_mesh_len);
  * d_gbar: n_units x nout x 20 cotangents of the output fields (nout <= 64);
- * d_grad: n_units x 6, written (NaN and status 4 for a unit whose w(a) crosses -1);
+ * d_grad: n_units x 6, written (NaN and status 4 for a unit whose w(a) crosses -1, status 7 for a
+ *   unit whose mesh_len is outside 1..max_mesh_len);
  * d_work: deb_adjoint_workspace_doubles(...) doubles (momentum bins, derivative tables,
  *   checkpoints); d_lam0 (n_units x nvar) and d_tbar (n_units x 2), optional (NULL): the
  *   cotangents of the initial state and of the start and end times.  256 threads per unit.

@@ -138,3 +138,34 @@ def test_many_cosmologies_match_one_at_a_time(deb):
         pk, J = deb.pk_jacobian(thetas[c], P, k, aexp_out=(0.5, 1.0), **LM, **SOLVER)
         assert torch.equal(pkm[c], pk) and torch.equal(Jm[c], J), c
     assert not torch.equal(Jm[0], Jm[1])
+
+
+@pytest.mark.parametrize("method", ["adjoint", "fd"])
+def test_ragged_grids_match_one_at_a_time(deb, method):
+    """Each cosmology on its own k-grid (lengths 5, 3, 6, 4), one of them across the phantom divide
+    and one an exact cosmological constant: the batched Jacobian of every valid cosmology equals a
+    separate pk_jacobian call bit for bit; the phantom one is NaN with status 4.  pk_and_vjp_many
+    on the same grids equals per-cosmology pk_and_vjp to rounding (sums in another order)."""
+    print(f"case: ragged grids 5/3/6/4 k, 4 cosmologies (one phantom, one Lambda), method {method}", flush=True)
+    P = base_param()
+    th0 = deb.param_to_theta(P)
+    th_ph = th0.copy()
+    th_ph[6], th_ph[7] = -1.02, 0.1  # w crosses -1
+    th_l = th0.copy()
+    th_l[6], th_l[7] = -1.0, 0.0
+    thetas = np.stack([th0, th_ph, th0 * (1 + 0.02 * np.array([1, -1, 1, 0, 0, 1, 1, -1, 0])), th_l])
+    grids = [deb.k_grid({}, 1e-4, 0.5, n, "log") for n in (5, 3, 6)] + [deb.k_grid({}, 2e-4, 0.3, 4, "log")]
+    pkm, Jm, st = deb.pk_jacobian_many(thetas, P, grids, aexp_out=(0.5, 1.0), return_status=True, method=method, **LM, **SOLVER)
+    assert isinstance(pkm, list) and [x.shape[0] for x in pkm] == [5, 3, 6, 4]
+    assert st.tolist()[1] == 4 and torch.isnan(Jm[1]).all() and torch.isnan(pkm[1]).all()
+    for c in (0, 2, 3):
+        pk, J = deb.pk_jacobian(thetas[c], P, grids[c], aexp_out=(0.5, 1.0), method=method, **LM, **SOLVER)
+        assert st[c] == 0 and torch.equal(pkm[c], pk) and torch.equal(Jm[c], J), c
+    if method == "adjoint":
+        rng = np.random.default_rng(5)
+        cots = [rng.standard_normal((g.size, 1)) for g in grids]
+        pkv, v, sv = deb.pk_and_vjp_many(thetas, P, grids, cots, **LM, **SOLVER)
+        assert sv[1] == 4 and torch.isnan(v[1]).all()
+        for c in (0, 2, 3):
+            _, vc = deb.pk_and_vjp(thetas[c], P, grids[c], cots[c][:, 0], **LM, **SOLVER)
+            np.testing.assert_allclose(v[c].cpu().numpy(), vc.cpu().numpy(), rtol=1e-12, atol=0)
