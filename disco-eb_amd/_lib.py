@@ -49,7 +49,9 @@ EXPORTS = ["deb_version", "deb_last_error", "deb_tables_size", "deb_table_offset
            "deb_evolve_adjoint", "deb_evolve_batched_workspace_doubles",
            "deb_evolve_batched", "deb_noalloc_scratch_doubles", "deb_evolve_batched_noalloc", "deb_tau_out", "deb_nvar", "deb_power", "deb_power_units",
            "deb_time_derivatives", "deb_visibility", "deb_cmb_sources_workspace_doubles", "deb_cmb_sources",
-           "deb_los_workspace_doubles", "deb_los_theta_ell", "deb_contract", "deb_cell", "deb_dell"]
+           "deb_los_workspace_doubles", "deb_los_theta_ell", "deb_contract", "deb_cell", "deb_dell", "deb_savgol_coefficients",
+           "deb_savgol_filter", "deb_power_smoothed_workspace_doubles", "deb_power_smoothed", "deb_power_kaiser", "deb_power_multipoles",
+           "deb_xi_workspace_doubles", "deb_xi_from_power"]
 
 
 class DebProblem(ctypes.Structure):
@@ -111,6 +113,17 @@ def lib():
                                              ctypes.POINTER(DebProblem), ctypes.POINTER(DebSolverCfg), _vp, _vp, _vp, _vp, _vp,
                                              ctypes.POINTER(DebSchedule), _vp]
     L.deb_tau_out.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
+    _d = ctypes.c_double
+    L.deb_savgol_coefficients.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp]
+    L.deb_savgol_filter.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
+    L.deb_power_smoothed_workspace_doubles.restype = ctypes.c_long
+    L.deb_power_smoothed_workspace_doubles.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.deb_power_smoothed.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _d, _d, ctypes.c_int, _vp, _vp, _vp]
+    L.deb_power_kaiser.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _d, _d, _d, _d, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
+    L.deb_power_multipoles.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _d, _d, _d, _d, _vp, _vp, _vp, _vp]
+    L.deb_xi_workspace_doubles.restype = ctypes.c_long
+    L.deb_xi_workspace_doubles.argtypes = [ctypes.c_int]
+    L.deb_xi_from_power.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp]
     L.deb_visibility.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]
     L.deb_cmb_sources_workspace_doubles.restype = ctypes.c_long
     L.deb_cmb_sources_workspace_doubles.argtypes = [ctypes.c_int] * 3

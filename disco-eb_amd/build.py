@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdiscoeb_amd.so")
-SOURCES = ["deb_perturb.hip", "deb_adjoint.hip", "deb_background.hip", "deb_misc.hip", "deb_cmb.hip", "deb_noalloc.hip", "deb_capi.cpp"]
+SOURCES = ["deb_perturb.hip", "deb_adjoint.hip", "deb_background.hip", "deb_misc.hip", "deb_cmb.hip", "deb_lss.hip", "deb_noalloc.hip", "deb_capi.cpp"]
 ARCH = os.environ.get("DEB_OFFLOAD_ARCH", "gfx950")
 
 

@@ -36,6 +36,15 @@ extern "C" int deb_launch_adjoint(const double *, int, const double *, double *,
                                   const double *, double *, long, double *, double *, double *, int *, const int *, hipStream_t);
 extern "C" int deb_launch_spline_eval(const double *, int, int, int, const double *, int, double *, hipStream_t);
 extern "C" int deb_launch_power(const double *, int, const double *, long, int, int, double, double, double, double *, hipStream_t);
+extern "C" int deb_launch_savgol_coefficients(int, int, double *, hipStream_t);
+extern "C" int deb_launch_savgol_apply(const double *, int, const double *, int, double *, hipStream_t);
+extern "C" int deb_launch_power_smoothed(const double *, const double *, int, int, int, double, double, double, int, double *, double *,
+                                         hipStream_t);
+extern "C" int deb_launch_kaiser(const double *, const double *, int, int, const double *, const double *, double, double, double, double, int,
+                                 int, double *, double *, hipStream_t);
+extern "C" int deb_launch_multipoles(const double *, const double *, int, int, double, double, double, double, double *, double *, double *,
+                                     hipStream_t);
+extern "C" int deb_launch_xi(const double *, const double *, int, int, double *, double *, double *, hipStream_t);
 extern "C" int deb_launch_time_deriv(const double *, const double *, const double *, int, const double *, int, const double *,
                                      const DebProblem *, double *, hipStream_t);
 
@@ -310,6 +319,60 @@ int deb_dell(const double *d_cell, int ellmax, double A_s, double Tcmb, double *
   if (ellmax < 2) return fail(-1, "ellmax must be >= 2");
   if (!d_cell || !d_ell || !d_dell) return fail(-1, "NULL device pointer");
   return hip_rc(deb_launch_dell(d_cell, ellmax, A_s, Tcmb, d_ell, d_dell, (hipStream_t)stream), "deb_dell");
+}
+
+// ------------------------------------------------------------------ large-scale structure
+
+int deb_savgol_coefficients(int window_length, int polyorder, double *d_coef, void *stream) {
+  if (polyorder < 0 || polyorder > 8) return fail(-1, "polyorder must lie in 0..8");
+  if (window_length < polyorder + 1) return fail(-1, "window_length must exceed polyorder");
+  if (!d_coef) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_savgol_coefficients(window_length, polyorder, d_coef, (hipStream_t)stream), "deb_savgol_coefficients");
+}
+
+int deb_savgol_filter(const double *d_y, int n, const double *d_coef, int window_length, double *d_out, void *stream) {
+  if (n < 1 || window_length < 1 || window_length > n) return fail(-1, "need 1 <= window_length <= n");
+  if (!d_y || !d_coef || !d_out) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_savgol_apply(d_y, n, d_coef, window_length, d_out, (hipStream_t)stream), "deb_savgol_filter");
+}
+
+long deb_power_smoothed_workspace_doubles(int n, int window_length) { return (long)n + window_length; }
+
+int deb_power_smoothed(const double *d_k, const double *d_y, int n, int ystride, int idx, double A_s, double n_s, double k_p,
+                       int window_length, double *d_out, double *d_work, void *stream) {
+  if (window_length < 5 || window_length % 2 == 0 || window_length > n)
+    return fail(-1, "window_length must be odd and lie in 5..n (the smoothing scale spans too few or too many k-modes)");
+  if (ystride < 1 || idx < 0 || idx >= ystride) return fail(-1, "invalid stride or field index");
+  if (!d_k || !d_y || !d_out || !d_work) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_power_smoothed(d_k, d_y, n, ystride, idx, A_s, n_s, k_p, window_length, d_out, d_work, (hipStream_t)stream),
+                "deb_power_smoothed");
+}
+
+int deb_power_kaiser(const double *d_k, const double *d_y, int n, int ystride, const double *d_pdelta, const double *d_ptheta, double bias,
+                     double A_s, double n_s, double k_p, int nmu, int mu_sampling, double *d_pkmu, double *d_mu, void *stream) {
+  if (n < 1 || nmu < 1) return fail(-1, "need n >= 1 and nmu >= 1");
+  if ((d_pdelta == nullptr) != (d_ptheta == nullptr)) return fail(-1, "give both smoothed spectra or neither");
+  if (!d_pdelta && (!d_y || ystride < 6)) return fail(-1, "the unsmoothed form reads fields 4 and 5 of d_y");
+  if (!d_k || !d_pkmu || !d_mu) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_kaiser(d_k, d_y, n, ystride, d_pdelta, d_ptheta, bias, A_s, n_s, k_p, nmu, mu_sampling ? 1 : 0, d_pkmu, d_mu,
+                                  (hipStream_t)stream),
+                "deb_power_kaiser");
+}
+
+int deb_power_multipoles(const double *d_k, const double *d_y, int n, int ystride, double b, double A_s, double n_s, double k_p, double *d_p0,
+                         double *d_p2, double *d_p4, void *stream) {
+  if (n < 1 || ystride < 6) return fail(-1, "need n >= 1 and fields 4 and 5 in d_y");
+  if (!d_k || !d_y || !d_p0 || !d_p2 || !d_p4) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_multipoles(d_k, d_y, n, ystride, b, A_s, n_s, k_p, d_p0, d_p2, d_p4, (hipStream_t)stream), "deb_power_multipoles");
+}
+
+long deb_xi_workspace_doubles(int n) { return 2L * (n / 2 + 1); }
+
+int deb_xi_from_power(const double *d_k, const double *d_pk, int n, int ell, double *d_xi, double *d_r, double *d_work, void *stream) {
+  if (n < 4 || n % 2) return fail(-1, "n must be even and >= 4");
+  if (ell < 0) return fail(-1, "ell must be >= 0");
+  if (!d_k || !d_pk || !d_xi || !d_r || !d_work) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_xi(d_k, d_pk, n, ell, d_xi, d_r, d_work, (hipStream_t)stream), "deb_xi_from_power");
 }
 
 }  // extern "C"

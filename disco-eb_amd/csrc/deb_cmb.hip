@@ -23,11 +23,13 @@
 //   * cell / dell: trapezoid over ln k, and the D_l scaling.
 #include <hip/hip_runtime.h>
 
+#include "deb_complex.hpp"
 #include "deb_cosmo.hpp"
 #include "deb_perturb.h"
 
 namespace deb {
 namespace {
+using namespace cplx;
 
 constexpr double kPi = 3.141592653589793;
 
@@ -322,7 +324,6 @@ __global__ void twiddle_kernel(int N, double2 *__restrict__ tw) {
   tw[t] = make_double2(c, s);
 }
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
 // FFTLog coefficients of one k-row per workgroup (cmb.py:753-803).  The row's spline in log chi
 // (knots lco) is built by one lane into Schi; samples f_m = S(chi_m) chi_m^1.1 on the grid
@@ -394,42 +395,6 @@ __global__ __launch_bounds__(256) void fftlog_kernel(const double *__restrict__ 
     Ar[2 * nn + 1] = -wn * X.y;
   }
   for (long c = (long)N + 2 + tid; c < Kp; c += blockDim.x) Ar[c] = 0.0;
-}
-
-// complex helpers for the log-gamma
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 clog(double2 a) { return make_double2(log(hypot(a.x, a.y)), atan2(a.y, a.x)); }
-
-// log Gamma(z): Lanczos approximation, g = 7, 9 terms, reflection for Re z <= 1/2 (util.py:13-45)
-__device__ double2 lngamma_complex(double2 z) {
-  const double c[9] = {0.99999999999980993227684700473478, 676.520368121885098567009190444019,
-                       -1259.13921672240287047156078755283, 771.3234287776530788486528258894,
-                       -176.61502916214059906584551354, 12.507343278686904814458936853,
-                       -0.13857109526572011689554707, 9.984369578019570859563e-6, 1.50563273514931155834e-7};
-  const bool refl = z.x <= 0.5;
-  double2 zz = refl ? make_double2(1.0 - z.x, -z.y) : z;
-  zz.x -= 1.0;
-  double2 Ag = make_double2(c[0], 0.0);
-  for (int j = 1; j < 9; ++j) {
-    const double2 t = make_double2(zz.x + j, zz.y);
-    const double n2 = t.x * t.x + t.y * t.y;
-    Ag.x += c[j] / n2 * t.x;
-    Ag.y += c[j] / n2 * (-t.y);
-  }
-  const double2 zh = make_double2(zz.x + 0.5, zz.y), z75 = make_double2(zz.x + 7.5, zz.y);
-  double2 r = cmul(zh, clog(z75));
-  r = csub(r, z75);
-  r.x += 0.9189385332046727418;
-  r = cadd(r, clog(Ag));
-  if (!refl) return r;
-  // ln pi - ln sin(pi z) - lnGamma(1 - z); sin(pi (x + i y)) = sin(pi x) cosh(pi y) + i cos(pi x) sinh(pi y)
-  double sx, cx;
-  sincospi(z.x, &sx, &cx);
-  const double2 sn = make_double2(sx * cosh(kPi * z.y), cx * sinh(kPi * z.y));
-  double2 out = make_double2(1.14472988584940017414342735135, 0.0);
-  out = csub(out, clog(sn));
-  return csub(out, r);
 }
 
 // B[l, 2n] = Re M_l(p_n), B[l, 2n + 1] = Im M_l(p_n), p_n = -1.1 + i eta_n (cmb.py:636-677)

@@ -228,6 +228,51 @@ int deb_cell(const double *d_theta, const double *d_k, int n_k, int n_ell, doubl
  * Replaces compute_Dell (src/discoeb/cmb.py:876-927). */
 int deb_dell(const double *d_cell, int ellmax, double A_s, double Tcmb, double *d_ell, double *d_dell, void *stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Large-scale-structure observables on the solver's output.  d_y: n x ystride doubles (one output
+ * time of evolve, the 20 output fields or the full state's first columns: delta_m at 4,
+ * theta_m / aH at 5); A_s, n_s, k_p the primordial parameters.
+ * ------------------------------------------------------------------------------------------- */
+
+/* Savitzky-Golay taps (window_length >= polyorder + 1 >= 1, polyorder <= 8): d_coef, window_length
+ * doubles.  Replaces the coefficient part of savgol_filter (src/discoeb/util.py:291-328). */
+int deb_savgol_coefficients(int window_length, int polyorder, double *d_coef, void *stream);
+
+/* d_out[i] = sum_m d_coef[m] d_y[i + (window_length - 1) / 2 - m] (zero beyond the ends), the
+ * 'same'-length convolution of savgol_filter (src/discoeb/util.py:327); window_length <= n. */
+int deb_savgol_filter(const double *d_y, int n, const double *d_coef, int window_length, double *d_out, void *stream);
+
+/* Doubles of workspace deb_power_smoothed needs. */
+long deb_power_smoothed_workspace_doubles(int n, int window_length);
+
+/* P(k) of field idx smoothed in log P by a cubic Savitzky-Golay filter of window_length (odd,
+ * 5..n); the first window_length / 2 and the last (window_length + 1) / 2 points keep the
+ * unsmoothed P.  Replaces get_power_smoothed (src/discoeb/perturbations.py:1473-1505), whose
+ * window length the caller computes from dlogk and the k spacing. */
+int deb_power_smoothed(const double *d_k, const double *d_y, int n, int ystride, int idx, double A_s, double n_s, double k_p,
+                       int window_length, double *d_out, double *d_work, void *stream);
+
+/* P(k, mu) = (bias delta_m - mu^2 theta_m)^2 on n x nmu points, mu uniform on [-1, 1]
+ * (mu_sampling 1) or cos of uniform angles on [0, pi] (0); d_mu (nmu) written.  With d_pdelta
+ * and d_ptheta (smoothed spectra of fields 4 and 5) delta_m = sqrt(pdelta), theta_m =
+ * -sqrt(ptheta); NULL: the scaled transfer functions.  Replaces power_Kaiser
+ * (src/discoeb/perturbations.py:1507-1543). */
+int deb_power_kaiser(const double *d_k, const double *d_y, int n, int ystride, const double *d_pdelta, const double *d_ptheta, double bias,
+                     double A_s, double n_s, double k_p, int nmu, int mu_sampling, double *d_pkmu, double *d_mu, void *stream);
+
+/* Monopole, quadrupole and hexadecapole of the Kaiser spectrum with linear bias b.  Replaces
+ * power_multipoles (src/discoeb/perturbations.py:1547-1569). */
+int deb_power_multipoles(const double *d_k, const double *d_y, int n, int ystride, double b, double A_s, double n_s, double k_p, double *d_p0,
+                         double *d_p2, double *d_p4, void *stream);
+
+/* Doubles of workspace deb_xi_from_power needs. */
+long deb_xi_workspace_doubles(int n);
+
+/* The correlation-function multipole xi_l(r) of P(k) on the log-uniform grid d_k (n even, >= 4) by
+ * FFTLog: d_xi and d_r (n each, increasing r = 2 pi / k).  Replaces get_xi_from_P
+ * (src/discoeb/perturbations.py:1409-1443). */
+int deb_xi_from_power(const double *d_k, const double *d_pk, int n, int ell, double *d_xi, double *d_r, double *d_work, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

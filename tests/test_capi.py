@@ -55,6 +55,12 @@ def test_argument_errors_without_gpu(pkg):
     assert L.deb_evolve_adjoint(*args(1, 5, 10)) < 0 and b"mesh_stride" in L.deb_last_error()
     assert L.deb_evolve_adjoint(*args(1, 10, 10)) < 0 and b"NULL" in L.deb_last_error()
     assert L.deb_adjoint_workspace_doubles(4, 1, 256, 10, ctypes.byref(prob)) == 32 + 6 * L.deb_tables_size(256) + 4 * 11 * 192
+    # large-scale structure: rejected before any launch
+    assert L.deb_savgol_coefficients(3, 3, None, None) < 0 and b"exceed" in L.deb_last_error()
+    assert L.deb_savgol_coefficients(11, 9, None, None) < 0
+    assert L.deb_power_smoothed(None, None, 100, 20, 4, 2e-9, 0.96, 0.05, 4, None, None, None) < 0 and b"odd" in L.deb_last_error()
+    assert L.deb_xi_from_power(None, None, 511, 0, None, None, None, None) < 0 and b"even" in L.deb_last_error()
+    assert L.deb_power_kaiser(None, None, 10, 20, None, None, 1.0, 2e-9, 0.96, 0.05, 0, 1, None, None, None) < 0
 
 
 def test_k_grid_matches_reference_rules(pkg, oracle):
