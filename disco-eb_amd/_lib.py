@@ -29,7 +29,7 @@ PARAM_DEFAULTS = {
 HDR = {  # header slots written by the background kernel
     "grhom": 24, "grhog": 25, "grhor": 26, "adotrad": 27, "amnu": 28, "Omegamnu": 29, "OmegaDE": 30, "taumin": 31,
     "taumax": 32, "fHe": 33, "tau0": 34, "tau_maxvis": 35, "optical_depth_today": 36, "akthom": 37,
-    "recfast_failures": 40,
+    "recfast_failures": 40, "amin": 44, "amax": 45,
 }
 ARRAY_SLOTS = [
     "nu_loga", "nu_logrho", "nu_logrho_S", "nu_logp", "nu_logp_S", "nu_logpp", "nu_logpp_S",
@@ -44,7 +44,8 @@ SPLINES = ["tau_of_a", "a_of_tau", "xe_of_tau", "cs2a_of_tau", "tempba_of_tau", 
            "logrhonu_of_loga", "logpnu_of_loga", "logppseudonu_of_loga", "opac"]
 
 # symbols declared in include/discoeb_amd.h
-EXPORTS = ["deb_version", "deb_last_error", "deb_tables_size", "deb_table_offset", "deb_background", "deb_spline_eval",
+EXPORTS = ["deb_version", "deb_last_error", "deb_tables_size", "deb_table_offset", "deb_background", "deb_background_thermo",
+           "deb_spline_eval",
            "deb_default_threads_per_unit", "deb_evolve_workspace_doubles", "deb_evolve", "deb_evolve_replay", "deb_adjoint_workspace_doubles",
            "deb_evolve_adjoint", "deb_evolve_batched_workspace_doubles",
            "deb_evolve_batched", "deb_noalloc_scratch_doubles", "deb_evolve_batched_noalloc", "deb_tau_out", "deb_nvar", "deb_power", "deb_power_units",
@@ -91,6 +92,7 @@ def lib():
     L.deb_evolve_workspace_doubles.restype = ctypes.c_long
     L.deb_nvar.argtypes = [ctypes.c_int] * 5
     L.deb_background.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
+    L.deb_background_thermo.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
     L.deb_spline_eval.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
     L.deb_evolve.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.POINTER(DebProblem),
                              ctypes.POINTER(DebSolverCfg), _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(DebSchedule), _vp]

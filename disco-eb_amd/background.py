@@ -49,18 +49,52 @@ def recfast_failures(tables):
     return tables[:, _lib.HDR["recfast_failures"]]
 
 
-def background_tables(params, num_thermo=256, check=True):
+THERMO_MODULES = {"RECFAST": 0, "MB95": 1, "CLASS": 2}
+CLASS_COLUMNS = ("scale factor a", "conf. time [Mpc]", "x_e", "Tb [K]", "c_b^2")
+
+
+def _class_table(class_thermo, dev):
+    """The CLASS thermodynamics columns (CLASS order, decreasing a) as the 5 x n block the kernel
+    reads: a ascending, conformal time, x_e, T_b, c_b^2 (background.py:271-285 reverses them)."""
+    cols = []
+    for name in CLASS_COLUMNS:
+        if name not in class_thermo:
+            raise ValueError(f"class_thermo lacks the column {name!r}")
+        cols.append(torch.flip(torch.as_tensor(np.ascontiguousarray(class_thermo[name], dtype=np.float64)), dims=[0]))
+    n = cols[0].numel()
+    if any(c.numel() != n for c in cols) or n < 16:
+        raise ValueError("class_thermo columns must have one common length >= 16")
+    if not bool((cols[0][1:] > cols[0][:-1]).all()) or not bool((cols[1][1:] > cols[1][:-1]).all()):
+        raise ValueError("class_thermo's scale factor and conformal time must decrease along the table (CLASS order)")
+    return torch.stack(cols).to(dev).contiguous(), n
+
+
+def background_tables(params, num_thermo=256, check=True, thermo_module="RECFAST", class_thermo=None):
     """Background tables of several cosmologies at once: one workgroup per cosmology.
 
-    params: list of param dicts.  Returns a (n_cosmo, deb_tables_size) float64 device tensor.
-    With check (and outside graph capture) a RECFAST interval solve that exceeded its 128
-    steps in any cosmology raises RuntimeError, as attach_background does for one."""
+    params: list of param dicts.  thermo_module: "RECFAST", "MB95" or "CLASS" (then class_thermo:
+    one dict of CLASS thermodynamics columns, or a list with one per cosmology, each of the same
+    length, which replaces num_thermo).  Returns a (n_cosmo, deb_tables_size) float64 device
+    tensor.  With check (and outside graph capture) a RECFAST interval solve that exceeded its
+    128 steps in any cosmology raises RuntimeError, as attach_background does for one."""
+    if thermo_module not in THERMO_MODULES:
+        raise ValueError(f"thermo_module must be one of {sorted(THERMO_MODULES)}")
     L = _lib.lib()
     dev = _lib.device()
     pv = torch.tensor([_lib.param_vector(p) for p in params], dtype=torch.float64, device=dev)
+    tin = None
+    if thermo_module == "CLASS":
+        if class_thermo is None:
+            raise ValueError("thermo_module='CLASS' needs class_thermo")
+        blocks = [_class_table(c, dev) for c in (class_thermo if isinstance(class_thermo, (list, tuple)) else [class_thermo] * len(params))]
+        if len(blocks) != len(params) or len({n for _, n in blocks}) != 1:
+            raise ValueError("one class_thermo table per cosmology, all of one length")
+        num_thermo = blocks[0][1]
+        tin = torch.stack([b for b, _ in blocks]).contiguous()
     size = L.deb_tables_size(int(num_thermo))
     tables = torch.empty((len(params), size), dtype=torch.float64, device=dev)
-    _lib.check(L.deb_background(_lib.ptr(pv), len(params), int(num_thermo), _lib.ptr(tables), _lib.stream()))
+    _lib.check(L.deb_background_thermo(_lib.ptr(pv), len(params), int(num_thermo), THERMO_MODULES[thermo_module],
+                                       _lib.ptr(tin) if tin is not None else None, _lib.ptr(tables), _lib.stream()))
     if check and not torch.cuda.is_current_stream_capturing():
         bad = torch.nonzero(recfast_failures(tables) != 0).flatten().cpu().tolist()
         if bad:
@@ -75,7 +109,6 @@ def attach_background(param, tables, num_thermo, cosmo=0):
     hdr = t[:64].detach().cpu().numpy()
     for k, slot in _lib.HDR.items():
         param[k] = float(hdr[slot])
-    param["amin"], param["amax"] = 1e-9, 1.01
 
     def arr(name, n=N):
         off = _lib.table_offset(name, N)
@@ -99,15 +132,18 @@ def attach_background(param, tables, num_thermo, cosmo=0):
 
 def evolve_background(*, param, thermo_module="RECFAST", num_thermo: int = 256, rtol: float = 1e-5, atol: float = 1e-7,
                       order: int = 5, class_thermo=None):
-    """Background evolution and RECFAST thermal history on the GPU.
+    """Background evolution and thermal history on the GPU.
 
-    Same signature and returned keys as the reference (background.py:191-345).  Only the
-    'RECFAST' thermal history is provided; rtol/atol/order are accepted and, as in the
-    reference, not used by the RECFAST path (thermodynamics_recfast.py:461)."""
-    if thermo_module != "RECFAST":
-        raise NotImplementedError(f"thermo_module={thermo_module!r}: only 'RECFAST' is provided")
-    tables = background_tables([param], num_thermo)
-    return attach_background(param, tables, num_thermo, 0)
+    Same signature and returned keys as the reference (background.py:191-345).
+    thermo_module: 'RECFAST' (default), 'MB95' (num_thermo points log-uniform in conformal time,
+    thermodynamics_mb95.py) or 'CLASS' (the thermal history of class_thermo, a dict of CLASS
+    thermodynamics columns 'scale factor a', 'conf. time [Mpc]', 'x_e', 'Tb [K]', 'c_b^2' in CLASS
+    order; its length replaces num_thermo and its a-range sets amin, amax).  rtol/atol/order are
+    accepted and, as in the reference, not used (thermodynamics_recfast.py:461).  Keys the
+    reference sets only for RECFAST (mu, xeprime_recf) hold zeros for the other modules."""
+    tables = background_tables([param], num_thermo, thermo_module=thermo_module, class_thermo=class_thermo)
+    n = len(class_thermo[CLASS_COLUMNS[0]]) if thermo_module == "CLASS" else num_thermo
+    return attach_background(param, tables, n, 0)
 
 
 def compute_background_quantities(aexp, param):

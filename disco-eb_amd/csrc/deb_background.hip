@@ -291,8 +291,88 @@ __device__ double romb_dtauda(const BgView &b, double a0, double a1) {
 __device__ double g_bg_nu[16];
 __device__ int g_bg_nu_ready;
 
+// thermal history of the MB95 module (thermodynamics_mb95.py:6-90): semi-implicit hydrogen step
+__device__ double mb95_ionize(double tempb, double a, double adot, double dtau, double xe, double YHe, double H0, double Omegab) {
+  const double iswitch = 0.5, tion = 1.5789e5, beta0 = 43.082, dec2g = 8.468e14;
+  const double alpha0 = 2.3866e-6 * (1.0 - YHe) * Omegab * H0 * H0;
+  const double crec = 8.0138e-26 * (1.0 - YHe) * Omegab * H0 * H0;
+  const double phi2 = fmax(0.448 * log(tion / tempb), 0.0);
+  const double alpha = alpha0 / sqrt(tempb) * phi2 / (a * a * a);
+  const double beta = tempb * phi2 * exp(beta0 - tion / tempb);
+  double cpeebles = 1.0;
+  if (!(tempb <= 200.0)) {
+    const double cp1 = crec * dec2g * (1.0 - xe) / (a * adot);
+    const double cp2 = crec * tempb * phi2 * exp(beta0 - 0.25 * tion / tempb) * (1.0 - xe) / (a * adot);
+    cpeebles = (1.0 + cp1) / (1.0 + cp1 + cp2);
+  }
+  const double aa = a * dtau * alpha * cpeebles, bb = a * dtau * beta * cpeebles;
+  const double b1 = 1.0 + iswitch * bb;
+  const double bbxe = bb + xe - (1.0 - iswitch) * (bb * xe + aa * xe * xe);
+  const double rat = iswitch * aa * bbxe / (b1 * b1);
+  return rat < 5e-5 ? bbxe / b1 * (1.0 - rat) : b1 / (2.0 * iswitch * aa) * (sqrt(4.0 * rat + 1.0) - 1.0);
+}
+
+// helium Saha equations, six fixed-point iterations (thermodynamics_mb95.py:47-89)
+__device__ void mb95_ionHe(double tempb, double a, double x0, double &x1, double &x2, double YHe, double H0, double Omegab) {
+  const double tion1 = 2.855e5, tion2 = 6.313e5;
+  const double b0 = 2.150e24 / ((1.0 - YHe) * Omegab * H0 * H0);
+  const double bp = b0 * a * a * a * tempb * sqrt(tempb);
+  const double r1 = 4.0 * bp * exp(-tion1 / tempb), r2 = bp * exp(-tion2 / tempb);
+  const double c = 0.25 * YHe / (1.0 - YHe);
+  for (int it = 0; it < 6; ++it) {
+    const double xe = x0 + c * (x1 + 2.0 * x2);
+    const double den = r1 * r2 + xe * r1 + xe * xe;
+    const double x2new = r1 * r2 / den;
+    x1 = xe * r1 / den;
+    x2 = x2new;
+  }
+}
+
+// the MB95 scan (thermodynamics_mb95.py:92-212) on N points log-uniform in tau over [taumin, taumax]
+__device__ void mb95_scan(const BgView &b, int N, double taumax, double *aexp, double *tauv, double *xeo, double *xHo, double *xHe1o,
+                          double *xHe2o, double *tbo, double *cs2o) {
+  const double Tcmb = b.Tcmb, YHe = b.YHe, H0 = b.H0, Omegab = b.Omegab;
+  const double thomc0 = 5.0577e-8 * Tcmb * Tcmb * Tcmb * Tcmb;
+  const double barssc_raw = 9.1820e-14, fx = 0.25 * YHe / (1.0 - YHe);
+  const double taumin = b.taumin, dlntau = log(taumax / taumin) / (N - 1);
+  double adot = b.adotrad, tau = taumin, a = adot * tau, tb = Tcmb / a;
+  double xHII = 1.0, xHeII = 0.0, xHeIII = 1.0;
+  double xe = xHII + fx * (xHeII + 2.0 * xHeIII);
+  double cs2 = 4.0 / 3.0 * barssc_raw * (1.0 - 0.75 * YHe + (1.0 - YHe) * xe) * tb;
+  for (int i = 0;; ++i) {
+    aexp[i] = a; tauv[i] = tau; xeo[i] = xe; xHo[i] = xHII; xHe1o[i] = xHeII; xHe2o[i] = xHeIII; tbo[i] = tb; cs2o[i] = cs2;
+    if (i == N - 1) break;
+    const double new_tau = taumin * exp((i + 1) * dlntau);
+    const double dtau = new_tau - tau;
+    double new_a = a + adot * dtau;
+    const double new_adot = b.aprimeoa(new_a) * new_a;
+    new_a = a + 2.0 * dtau / (1.0 / adot + 1.0 / new_adot);
+    const double tg0 = Tcmb / a, ahalf = 0.5 * (a + new_a), adothalf = 0.5 * (adot + new_adot);
+    const double fe = (1.0 - YHe) * xe / (1.0 - 0.75 * YHe + (1.0 - YHe) * xe);
+    const double thomc = thomc0 * fe / adothalf / (ahalf * ahalf * ahalf);
+    double a2t;
+    if (fe < 1e-3) {
+      a2t = (a - new_a) * Tcmb + a * a * (tb - tg0) + (0.5 * (a - new_a) * (a - new_a) * Tcmb + a * a * (a - new_a) * (tb - tg0)) * thomc;
+    } else {
+      const double etc = exp(-thomc * (new_a - a));
+      a2t = a * a * (tb - tg0) * etc - Tcmb / thomc * (1.0 - etc);
+    }
+    const double new_tb = Tcmb / new_a + a2t / (new_a * new_a);
+    const double tbhalf = 0.5 * (tb + new_tb);
+    const double new_xHII = mb95_ionize(tbhalf, ahalf, adothalf, dtau, xHII, YHe, H0, Omegab);
+    mb95_ionHe(tb, new_a, new_xHII, xHeII, xHeIII, YHe, H0, Omegab);
+    const double new_xe = new_xHII + fx * (xHeII + 2.0 * xHeIII);
+    const double dtbdla = -2.0 * new_tb - thomc * a2t / new_a;
+    const double barssc = barssc_raw * (1.0 - 0.75 * YHe + (1.0 - YHe) * new_xe);
+    cs2 = barssc * new_tb * (1.0 - dtbdla / new_tb / 3.0);
+    a = new_a; adot = new_adot; tau = new_tau; tb = new_tb; xHII = new_xHII; xe = new_xe;
+  }
+}
+
+// thermo: 0 RECFAST, 1 MB95, 2 the thermal history given in thermo_in (per cosmology 5 x N
+// doubles: a ascending, tau, x_e, T_b, c_b^2 -- the CLASS input of background.py:271-285)
 __global__ __launch_bounds__(256) void background_kernel(const double *__restrict__ params, int N, double *__restrict__ tables,
-                                                         long stride) {
+                                                         long stride, int thermo, const double *__restrict__ thermo_in) {
   const int cosmo = blockIdx.x;
   const int tid = threadIdx.x, nth = blockDim.x;
   double *t = tables + (long)cosmo * stride;
@@ -301,6 +381,9 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
   __shared__ RfConst K;
   double *A = t;  // alias for readability
   auto arr = [&](int slot) { return t + deb_array_offset(slot, N); };
+  const double *tin = thermo == 2 ? thermo_in + (long)cosmo * 5 * N : nullptr;
+  // the a-range of the background (background.py:230-236): the given table's, else [1e-9, 1.01]
+  const double amin = tin ? tin[0] : 1e-9, amax = tin ? tin[N - 1] : 1.01;
 
   if (tid == 0) {
     K.init();
@@ -319,7 +402,10 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
     t[DEB_HDR_AKTHOM] = 2.3038921003709498e-9 * (1.0 - p[DEB_P_YHE]) * p[DEB_P_OMEGAB] * H0 * H0;
     t[DEB_HDR_LAMBDA_DE] = (p[DEB_P_W0] == -1.0 && p[DEB_P_WA] == 0.0) ? 1.0 : 0.0;
     t[DEB_HDR_N] = (double)N;
-    t[DEB_HDR_TAUMIN] = 1e-9 / t[DEB_HDR_ADOTRAD];
+    t[DEB_HDR_TAUMIN] = amin / t[DEB_HDR_ADOTRAD];
+    t[DEB_HDR_AMIN] = amin;
+    t[DEB_HDR_AMAX] = amax;
+    t[DEB_HDR_THERMO] = (double)thermo;
     if (g_bg_nu_ready) {
       for (int i = 0; i < 8; ++i) { s_q[i] = g_bg_nu[i]; s_w[i] = g_bg_nu[8 + i]; }
     } else {
@@ -335,7 +421,7 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
     const double amnu = t[DEB_HDR_AMNU];
     double *la = arr(DEB_A_NU_LOGA), *lr = arr(DEB_A_NU_LOGRHO), *lp = arr(DEB_A_NU_LOGP), *lpp = arr(DEB_A_NU_LOGPP);
     for (int i = tid; i < DEB_NNU; i += nth) {
-      double a = geom_point(1e-9 * 0.9, 1.01 * 1.1, i, DEB_NNU, true);
+      double a = geom_point(amin * 0.9, amax * 1.1, i, DEB_NNU, true);
       double r = 0.0, ps = 0.0, pps = 0.0;
       for (int j = 0; j < 8; ++j) {
         double aq = a * amnu / s_q[j];
@@ -374,7 +460,7 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
   b.init(t);
   // adaptive a-grid with N+1 points (thermodynamics_recfast.py:325-361); a[0] -> header,
   // a[1..N] -> aexp
-  {
+  if (thermo == 0) {
     const int Np = N + 1;
     const int n1 = max(8, (int)(Np * 0.05)), n2 = max(8, (int)(Np * 0.10)), n3 = (int)(Np * 0.50);
     const int n4 = Np - n1 - n2 - n3;
@@ -393,8 +479,37 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
   }
   __syncthreads();
   // tau_max by Romberg over [amin, amax] (background.py:181-187), beside the RECFAST loop
-  if (tid == 64) t[DEB_HDR_TAUMAX] = b.taumin + romb_dtauda(b, 1e-9, 1.01);
-  if (tid < 64) {
+  if (tid == 64) t[DEB_HDR_TAUMAX] = b.taumin + romb_dtauda(b, amin, amax);
+  if (thermo != 0) {
+    __syncthreads();  // tau_max first: the MB95 grid ends there
+    if (tid == 0) {
+      double *xe = arr(DEB_A_XE), *xH = arr(DEB_A_XEHI), *xHe1 = arr(DEB_A_XEHEI), *xHe2 = arr(DEB_A_XEHEII), *Tm = arr(DEB_A_TM);
+      double *cs2 = arr(DEB_A_CS2), *aexp = arr(DEB_A_AEXP), *tau = arr(DEB_A_TAU);
+      if (thermo == 1) {
+        mb95_scan(b, N, t[DEB_HDR_TAUMAX], aexp, tau, xe, xH, xHe1, xHe2, Tm, cs2);
+      } else {
+        for (int i = 0; i < N; ++i) {
+          aexp[i] = tin[i]; tau[i] = tin[N + i]; xe[i] = tin[2 * N + i]; Tm[i] = tin[3 * N + i]; cs2[i] = tin[4 * N + i];
+          xH[i] = 0.0; xHe1[i] = 0.0; xHe2[i] = 0.0;
+        }
+      }
+      t[DEB_HDR_RECFAST_FAIL] = 0.0;
+      t[DEB_HDR_A_START] = aexp[0];
+    }
+    __syncthreads();
+    double *aexp = arr(DEB_A_AEXP), *mu = arr(DEB_A_MU), *cs2a = arr(DEB_A_CS2A), *tempba = arr(DEB_A_TEMPBA);
+    double *xpr = arr(DEB_A_XEPRIME_RECF), *loga = arr(DEB_A_LOGA);
+    const double *cs2 = arr(DEB_A_CS2), *Tm = arr(DEB_A_TM);
+    for (int i = tid; i < N; i += nth) {
+      const double a = aexp[i];
+      cs2a[i] = a * cs2[i];
+      tempba[i] = a * Tm[i];
+      loga[i] = log(a);
+      mu[i] = 0.0;  // not produced by these modules
+      xpr[i] = 0.0;
+    }
+  }
+  if (thermo == 0 && tid < 64) {
     // RECFAST interval loop (thermodynamics_recfast.py:392-452) on wavefront 0 (every lane runs
     // the same sequence; lanes 0-2 carry the Jacobian directions), lane 0 writes
     const double *aexp = arr(DEB_A_AEXP);
@@ -448,7 +563,7 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
   }
   __syncthreads();
   // evaluate_thermo per knot (thermodynamics_recfast.py:456-492)
-  {
+  if (thermo == 0) {
     const double *aexp = arr(DEB_A_AEXP), *xH = arr(DEB_A_XEHI), *xHe = arr(DEB_A_XEHEI), *Tm = arr(DEB_A_TM);
     const double *d0 = arr(DEB_A_SCR0), *d1 = arr(DEB_A_SCR1), *d2 = arr(DEB_A_SCR2);
     double *xe = arr(DEB_A_XE), *xeHeII = arr(DEB_A_XEHEII), *mu = arr(DEB_A_MU), *cs2 = arr(DEB_A_CS2), *cs2a = arr(DEB_A_CS2A);
@@ -474,7 +589,7 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
     }
   }
   __syncthreads();
-  if (tid == 0) {  // tau = taumin + concatenate([0], cumsum(increments))
+  if (thermo == 0 && tid == 0) {  // tau = taumin + concatenate([0], cumsum(increments))
     double *tau = arr(DEB_A_TAU);
     const double *inc = arr(DEB_A_SCR3);
     double cum = 0.0;
@@ -564,8 +679,10 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
 }  // namespace deb
 
 // host launcher (called by the C ABI)
-extern "C" int deb_launch_background(const double *d_params, int n_cosmo, int num_thermo, double *d_tables, hipStream_t stream) {
+extern "C" int deb_launch_background(const double *d_params, int n_cosmo, int num_thermo, int thermo, const double *d_thermo_in,
+                                     double *d_tables, hipStream_t stream) {
   long stride = deb_table_doubles(num_thermo);
-  hipLaunchKernelGGL(deb::background_kernel, dim3(n_cosmo), dim3(256), 0, stream, d_params, num_thermo, d_tables, stride);
+  hipLaunchKernelGGL(deb::background_kernel, dim3(n_cosmo), dim3(256), 0, stream, d_params, num_thermo, d_tables, stride, thermo,
+                     d_thermo_in);
   return (int)hipGetLastError();
 }

@@ -7,7 +7,7 @@
 
 #include "../../include/discoeb_amd.h"
 
-extern "C" int deb_launch_background(const double *, int, int, double *, hipStream_t);
+extern "C" int deb_launch_background(const double *, int, int, int, const double *, double *, hipStream_t);
 extern "C" int deb_launch_nu_bins(int, double *, hipStream_t);
 extern "C" long deb_cmb_sources_work_doubles(int, int, int);
 extern "C" int deb_launch_cmb_sources(const double *, int, const double *, const double *, const double *, int, const double *, const double *,
@@ -96,7 +96,18 @@ int deb_background(const double *d_params, int n_cosmo, int num_thermo, double *
   if (num_thermo < 16) return fail(-1, "num_thermo must be >= 16");
   if (n_cosmo == 0) return 0;
   if (!d_params || !d_tables) return fail(-1, "NULL device pointer");
-  return hip_rc(deb_launch_background(d_params, n_cosmo, num_thermo, d_tables, (hipStream_t)stream), "deb_background");
+  return hip_rc(deb_launch_background(d_params, n_cosmo, num_thermo, 0, nullptr, d_tables, (hipStream_t)stream), "deb_background");
+}
+
+int deb_background_thermo(const double *d_params, int n_cosmo, int num_thermo, int thermo_module, const double *d_thermo_in,
+                          double *d_tables, void *stream) {
+  if (thermo_module < 0 || thermo_module > 2) return fail(-1, "thermo_module must be 0 (RECFAST), 1 (MB95) or 2 (given table)");
+  if (n_cosmo < 0) return fail(-1, "n_cosmo must be >= 0");
+  if (num_thermo < 16) return fail(-1, "num_thermo must be >= 16");
+  if (n_cosmo == 0) return 0;
+  if (!d_params || !d_tables || (thermo_module == 2 && !d_thermo_in)) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_background(d_params, n_cosmo, num_thermo, thermo_module, d_thermo_in, d_tables, (hipStream_t)stream),
+                "deb_background_thermo");
 }
 
 int deb_spline_eval(const double *d_tables, int num_thermo, int cosmo, int spline_id, int kind, const double *d_x, int nx,

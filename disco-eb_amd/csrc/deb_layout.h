@@ -32,7 +32,8 @@ enum {
 enum {
   DEB_HDR_GRHOM = 24, DEB_HDR_GRHOG, DEB_HDR_GRHOR, DEB_HDR_ADOTRAD, DEB_HDR_AMNU, DEB_HDR_OMEGAMNU, DEB_HDR_OMEGADE,
   DEB_HDR_TAUMIN, DEB_HDR_TAUMAX, DEB_HDR_FHE, DEB_HDR_TAU0, DEB_HDR_TAU_MAXVIS, DEB_HDR_OPTDEPTH_TODAY, DEB_HDR_AKTHOM,
-  DEB_HDR_LAMBDA_DE, DEB_HDR_N, DEB_HDR_RECFAST_FAIL, DEB_HDR_NU_X0, DEB_HDR_NU_DXINV, DEB_HDR_A_START,
+  DEB_HDR_LAMBDA_DE, DEB_HDR_N, DEB_HDR_RECFAST_FAIL, DEB_HDR_NU_X0, DEB_HDR_NU_DXINV, DEB_HDR_A_START, DEB_HDR_AMIN,
+  DEB_HDR_AMAX, DEB_HDR_THERMO,
   DEB_HDR_SIZE = 64
 };
 

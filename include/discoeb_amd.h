@@ -36,6 +36,16 @@ long deb_table_offset(int slot, int num_thermo);
  * (src/discoeb/background.py:191-345, thermodynamics_recfast.py:456-492). */
 int deb_background(const double *d_params, int n_cosmo, int num_thermo, double *d_tables, void *stream);
 
+/* deb_background with the thermal-history module chosen: thermo_module 0 RECFAST (as
+ * deb_background), 1 MB95 (num_thermo points log-uniform in conformal time; the reference's
+ * compute_thermo, src/discoeb/thermodynamics_mb95.py:92-212), 2 a given thermal history:
+ * d_thermo_in holds per cosmology 5 x num_thermo doubles -- scale factor (ascending), conformal
+ * time [Mpc], x_e, T_b [K], c_b^2 -- from which the splines, opacity and visibility are built and
+ * whose a-range sets the background's (the CLASS input of background.py:230-236, 271-285).
+ * Replaces evolve_background's thermo_module / class_thermo options (src/discoeb/background.py:191-345). */
+int deb_background_thermo(const double *d_params, int n_cosmo, int num_thermo, int thermo_module, const double *d_thermo_in,
+                          double *d_tables, void *stream);
+
 /* Spline queries on the background tables of one cosmology: kind 0 value, 1 first derivative,
  * 2 second derivative, 3 integral from the first knot (opacity spline only).
  * Replaces spline_interpolation.evaluate/derivative/derivative2/integral
