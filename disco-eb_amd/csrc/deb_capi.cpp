@@ -2,6 +2,7 @@
 // kernel launches on the caller's stream.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstdio>
 #include <string>
 
@@ -315,6 +316,7 @@ int deb_los_theta_ell(const double *d_k, int n_k, const double *d_tau, int n_tau
 
 int deb_contract(const double *d_A, const double *d_B, int m, int n, long k, double *d_C, void *stream) {
   if (m < 0 || n < 0 || k < 0 || (k % 16)) return fail(-1, "m, n >= 0 and k a non-negative multiple of 16");
+  if (((uintptr_t)d_A | (uintptr_t)d_B) & 15) return fail(-1, "d_A and d_B must be 16-byte aligned");
   if ((long)m * n == 0) return 0;
   if (!d_A || !d_B || !d_C) return fail(-1, "NULL device pointer");
   return hip_rc(deb_launch_contract(d_A, d_B, m, n, k, d_C, (hipStream_t)stream), "deb_contract");

@@ -258,8 +258,46 @@ __global__ void source_isw_kernel(const double *__restrict__ tau, int n_tau, int
   }
 }
 
-__global__ void spline_cp_kernel(const double *__restrict__ x, int n, double *__restrict__ cp) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) spline_cp(x, n, cp);
+// spline_cp on one block of 256 threads: cp_i = h1_i / (b_i - h0_i cp_{i-1}) is the Moebius map of
+// [[0, h1_i], [-h0_i, b_i]] applied to (cp_{i-1}, 1); each thread composes its chunk's maps, a
+// block-wide inclusive scan of the (max-normalised) 2 x 2 products gives every chunk its incoming
+// value, and the chunks are replayed.
+__global__ __launch_bounds__(256) void spline_cp_kernel(const double *__restrict__ x, int n, double *__restrict__ cp) {
+  __shared__ double s00[256], s01[256], s10[256], s11[256];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int m = n - 2;
+  if (m <= 0) return;
+  const int C = (m + nth - 1) / nth, c0 = tid * C, c1 = min(m, c0 + C);
+  double a = 1.0, b = 0.0, c = 0.0, d = 1.0;
+  for (int i = c0; i < c1; ++i) {
+    const double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1], bb = 2.0 * (h0 + h1);
+    const double na = h1 * c, nb = h1 * d, nc = bb * c - h0 * a, nd = bb * d - h0 * b;
+    const double sc = 1.0 / fmax(fmax(fabs(na), fabs(nb)), fmax(fabs(nc), fabs(nd)));
+    a = na * sc; b = nb * sc; c = nc * sc; d = nd * sc;
+  }
+  s00[tid] = a; s01[tid] = b; s10[tid] = c; s11[tid] = d;
+  __syncthreads();
+  for (int off = 1; off < nth; off <<= 1) {  // P_t = M_t P_{t - off}
+    double pa = a, pb = b, pc = c, pd = d;
+    if (tid >= off) {
+      const double qa = s00[tid - off], qb = s01[tid - off], qc = s10[tid - off], qd = s11[tid - off];
+      pa = a * qa + b * qc; pb = a * qb + b * qd; pc = c * qa + d * qc; pd = c * qb + d * qd;
+      const double sc = 1.0 / fmax(fmax(fabs(pa), fabs(pb)), fmax(fabs(pc), fabs(pd)));
+      pa *= sc; pb *= sc; pc *= sc; pd *= sc;
+    }
+    __syncthreads();
+    s00[tid] = a = pa; s01[tid] = b = pb; s10[tid] = c = pc; s11[tid] = d = pd;
+    __syncthreads();
+  }
+  double p = 0.0, q = 1.0;  // (cp_{-1}, 1)
+  if (tid > 0) { p = s01[tid - 1]; q = s11[tid - 1]; }
+  for (int i = c0; i < c1; ++i) {
+    const double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1], bb = 2.0 * (h0 + h1);
+    const double np_ = h1 * q, nq = bb * q - h0 * p;
+    const double sc = 1.0 / fmax(fabs(np_), fabs(nq));
+    p = np_ * sc; q = nq * sc;
+    cp[i] = (i < m - 1) ? p / q : 0.0;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -325,35 +363,149 @@ __global__ void twiddle_kernel(int N, double2 *__restrict__ tw) {
 }
 
 
-// FFTLog coefficients of one k-row per workgroup (cmb.py:753-803).  The row's spline in log chi
-// (knots lco) is built by one lane into Schi; samples f_m = S(chi_m) chi_m^1.1 on the grid
-// log chi_m = lmin + m dl are packed as z_j = f_2j + i f_2j+1 at bit-reversed LDS slots; a
-// radix-2 FFT of M = N/2 points gives Z, the real-FFT split gives X_n = rfft(f)_n for n <= N/2,
+// Row-independent tables of the FFTLog stage, built once per launch (one block):
+//   knot tables of the log-chi spline solve: rden_i = 1 / den_i (den_i the forward-elimination
+//   pivot of spline_build), bco_i = -h0_i / den_i (cp comes from spline_cp);
+//   per sample m of the log-chi grid: its knot interval I[m] (the index rule of spl_index) and
+//   the four spline-basis weights of spl_value times chi_m^1.1 (W[4 m .. 4 m + 3]);
+//   per output frequency n <= N/2: e^{-i eta_n lmin} / N (ph[n]).
+__global__ __launch_bounds__(256) void fftlog_setup_kernel(const double *__restrict__ lco, int n_tau, const double *__restrict__ cp,
+                                                           const double *__restrict__ tau, double tau0, int N, double *__restrict__ rden,
+                                                           double *__restrict__ bco, int *__restrict__ I, double *__restrict__ W,
+                                                           double2 *__restrict__ ph) {
+  const int tid = threadIdx.x;
+  const int m = n_tau - 2;
+  for (int i = tid; i < m; i += blockDim.x) {  // cp is known: the pivots are independent
+    const double h0 = lco[i + 1] - lco[i], h1 = lco[i + 2] - lco[i + 1];
+    const double b = 2.0 * (h0 + h1);
+    const double den = (i == 0) ? b : b - h0 * cp[i - 1];
+    rden[i] = 1.0 / den;
+    bco[i] = (i == 0) ? 0.0 : -h0 / den;
+  }
+  double lmin, dl;
+  fft_grid(tau, n_tau, tau0, N, lmin, dl);
+  for (int q = tid; q < N; q += blockDim.x) {
+    const double lx = lmin + (double)q * dl;
+    const int i = spl_index(lco, n_tau, lx);
+    const double h = lco[i + 1] - lco[i];
+    const double t = (lx - lco[i]) / h;
+    const double A = 1.0 - t, B = t, e = exp(1.1 * lx), c = h * h / 6.0;
+    I[q] = i;
+    W[4 * q] = A * e;
+    W[4 * q + 1] = B * e;
+    W[4 * q + 2] = (A * A * A - A) * c * e;
+    W[4 * q + 3] = (B * B * B - B) * c * e;
+  }
+  for (int n = tid; n <= N / 2; n += blockDim.x) {
+    const double eta = 2.0 * kPi * (double)n / ((double)N * dl);
+    double s1, c1;
+    sincos(-eta * lmin, &s1, &c1);
+    ph[n] = make_double2(c1 / (double)N, s1 / (double)N);
+  }
+}
+
+// inclusive scan over the block of affine maps x -> a + b x, composed in thread order
+// (entry t after entry t - 1); sa, sb: blockDim.x doubles each in LDS
+__device__ void affine_scan(double *sa, double *sb, int tid, int nth) {
+  for (int off = 1; off < nth; off <<= 1) {
+    double a = sa[tid], b = sb[tid];
+    if (tid >= off) {
+      const double pa = sa[tid - off], pb = sb[tid - off];
+      a = a + b * pa;
+      b = b * pb;
+    }
+    __syncthreads();
+    sa[tid] = a;
+    sb[tid] = b;
+    __syncthreads();
+  }
+}
+
+// FFTLog coefficients of one k-row per workgroup (cmb.py:753-803).  The row's natural spline in
+// log chi (knots lco, the row reversed so chi increases) is solved by two block-wide scans of the
+// affine recurrences of spline_build (forward elimination dp_i = d_i rden_i + bco_i dp_{i-1}, back
+// substitution v_i = dp_i - cp_i v_{i+1}) into the second derivatives Sd (LDS when they fit, else
+// the row's slot of Schi); samples f_m = S(chi_m) chi_m^1.1 on log chi_m = lmin + m dl are four
+// multiply-adds with the setup's weights, packed as z_j = f_2j + i f_2j+1 at bit-reversed LDS
+// slots; a radix-2 FFT of M = N/2 points gives Z, the real-FFT split X_n = rfft(f)_n for n <= N/2,
 // and ck_n = X_n e^{-i eta_n lmin} / N * k^{-(p_n + 1)} is written as A[row, 2n] = w_n Re ck,
 // A[row, 2n + 1] = -w_n Im ck (w = 1 for n = 0, N/2, else 2), zero beyond 2 (N/2 + 1).
 template <int LOGM>
 __global__ __launch_bounds__(256) void fftlog_kernel(const double *__restrict__ S, int n_tau, const double *__restrict__ lco,
-                                                     const double *__restrict__ cpchi, double *__restrict__ Schi,
-                                                     const double *__restrict__ kk, const double *__restrict__ tau, double tau0,
-                                                     const double2 *__restrict__ tw, double *__restrict__ A, long Kp) {
+                                                     const double *__restrict__ cp, const double *__restrict__ rden,
+                                                     const double *__restrict__ bco, const int *__restrict__ Iq,
+                                                     const double *__restrict__ Wq, const double2 *__restrict__ ph, int sd_in_lds,
+                                                     double *__restrict__ Schi, const double *__restrict__ kk, const double *__restrict__ tau,
+                                                     double tau0, const double2 *__restrict__ tw, double *__restrict__ A, long Kp) {
   constexpr int M = 1 << LOGM, N = 2 * M;
   extern __shared__ double2 z[];
+  double *sa = reinterpret_cast<double *>(z + M), *sb = sa + 256;
+  const int row = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
+  double *Sd = sd_in_lds ? sb + 256 : Schi + (long)row * n_tau;
   double lmin, dl;
   fft_grid(tau, n_tau, tau0, N, lmin, dl);
-  const int row = blockIdx.x, tid = threadIdx.x;
-  const double *Sr = S + (long)row * n_tau;
-  double *Sc = Schi + (long)row * n_tau;
-  if (tid == 0) spline_build(lco, Sr + (n_tau - 1), -1, n_tau, cpchi, Sc, 1);  // the row reversed (increasing chi)
+  const double *Yr = S + (long)row * n_tau + (n_tau - 1);  // Yr[-i] = value at knot i
+  const int m = n_tau - 2;
+  const int C = (m + nth - 1) / nth;
+  const int c0 = tid * C, c1 = min(m, c0 + C);
+  // forward elimination: a_i = d_i rden_i kept in Sd[i + 1], then the chunk's composite map
+  {
+    double ca = 0.0, cb = 1.0;
+    for (int i = c0; i < c1; ++i) {
+      const double h0 = lco[i + 1] - lco[i], h1 = lco[i + 2] - lco[i + 1];
+      const double y0 = Yr[-i], y1 = Yr[-(i + 1)], y2 = Yr[-(i + 2)];
+      const double d = 6.0 * ((y2 - y1) / h1 - (y1 - y0) / h0);
+      const double a = d * rden[i], b = bco[i];
+      Sd[i + 1] = a;
+      ca = a + b * ca;
+      cb = b * cb;
+    }
+    sa[tid] = ca;
+    sb[tid] = cb;
+  }
   __syncthreads();
-  const double *Yr = Sr + (n_tau - 1);  // Yr[-i] = value at knot i
-  for (int j = tid; j < M; j += blockDim.x) {
+  affine_scan(sa, sb, tid, nth);
+  {
+    double x = (tid == 0) ? 0.0 : sa[tid - 1];  // dp before the chunk (dp_{-1} = 0)
+    for (int i = c0; i < c1; ++i) {
+      x = Sd[i + 1] + bco[i] * x;
+      Sd[i + 1] = x;
+    }
+  }
+  __syncthreads();
+  // back substitution in reverse order: chunk of thread t' = nth - 1 - t taken from the end
+  {
+    const int tr = nth - 1 - tid;
+    const int r0 = tr * C, r1 = min(m, r0 + C);  // indices m-1-r for r in [r0, r1)
+    double ca = 0.0, cb = 1.0;
+    for (int r = r0; r < r1; ++r) {
+      const int i = m - 1 - r;
+      const double a = Sd[i + 1], b = -cp[i];
+      ca = a + b * ca;
+      cb = b * cb;
+    }
+    __syncthreads();
+    sa[tr] = ca;
+    sb[tr] = cb;
+    __syncthreads();
+    affine_scan(sa, sb, tid, nth);
+    double x = (tr == 0) ? 0.0 : sa[tr - 1];  // v_m = 0
+    for (int r = r0; r < r1; ++r) {
+      const int i = m - 1 - r;
+      x = Sd[i + 1] - cp[i] * x;
+      Sd[i + 1] = x;
+    }
+    if (tid == 0) { Sd[0] = 0.0; Sd[n_tau - 1] = 0.0; }
+  }
+  __syncthreads();
+  for (int j = tid; j < M; j += nth) {
     double f[2];
+#pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int m = 2 * j + h;
-      const double lx = lmin + (double)m * dl;
-      const int i = spl_index(lco, n_tau, lx);
-      const double sv = spl_value(lco, Yr, -1, Sc, 1, i, lx);
-      f[h] = sv * exp(1.1 * lx);  // chi^(-bias), bias = -1.1
+      const int q = 2 * j + h;
+      const int i = Iq[q];
+      const double *w = Wq + 4 * (long)q;
+      f[h] = w[0] * Yr[-i] + w[1] * Yr[-(i + 1)] + w[2] * Sd[i] + w[3] * Sd[i + 1];
     }
     const int r = (int)(__brev((unsigned)j) >> (32 - LOGM));
     z[r] = make_double2(f[0], f[1]);
@@ -362,7 +514,7 @@ __global__ __launch_bounds__(256) void fftlog_kernel(const double *__restrict__ 
   for (int s = 1; s <= LOGM; ++s) {  // in-place decimation in time
     const int half = 1 << (s - 1);
     const int stride = N >> s;  // twiddle index step: e^{-2 pi i j / 2^s} = tw[j N / 2^s]
-    for (int bfly = tid; bfly < M / 2; bfly += blockDim.x) {
+    for (int bfly = tid; bfly < M / 2; bfly += nth) {
       const int j = bfly & (half - 1);
       const int i0 = ((bfly >> (s - 1)) << s) | j, i1 = i0 + half;
       const double2 w = tw[j * stride];
@@ -375,26 +527,25 @@ __global__ __launch_bounds__(256) void fftlog_kernel(const double *__restrict__ 
   const double lk = log(kk[row]);
   const double kpow = exp(-(-1.1 + 1.0) * lk);
   double *Ar = A + (long)row * Kp;
-  for (int nn = tid; nn <= M; nn += blockDim.x) {
+  for (int nn = tid; nn <= M; nn += nth) {
     const double2 Zn = z[nn & (M - 1)];
     const double2 Zm = z[(M - nn) & (M - 1)];
     const double2 Zc = make_double2(Zm.x, -Zm.y);
     const double2 E = make_double2(0.5 * (Zn.x + Zc.x), 0.5 * (Zn.y + Zc.y));
     const double2 D = make_double2(0.5 * (Zn.x - Zc.x), 0.5 * (Zn.y - Zc.y));
-    const double2 W = (nn < M) ? tw[nn] : make_double2(-1.0, 0.0);
-    const double2 O = cmul(W, make_double2(D.y, -D.x));  // e^{-2 pi i n / N} (-i) D
+    const double2 Wt = (nn < M) ? tw[nn] : make_double2(-1.0, 0.0);
+    const double2 O = cmul(Wt, make_double2(D.y, -D.x));  // e^{-2 pi i n / N} (-i) D
     double2 X = make_double2(E.x + O.x, E.y + O.y);
     const double eta = 2.0 * kPi * (double)nn / ((double)N * dl);
-    double s1, c1, s2, c2;
-    sincos(-eta * lmin, &s1, &c1);
+    double s2, c2;
     sincos(-eta * lk, &s2, &c2);
-    X = cmul(X, make_double2(c1 / (double)N, s1 / (double)N));
+    X = cmul(X, ph[nn]);
     X = cmul(X, make_double2(kpow * c2, kpow * s2));
     const double wn = (nn == 0 || nn == M) ? 1.0 : 2.0;
     Ar[2 * nn] = wn * X.x;
     Ar[2 * nn + 1] = -wn * X.y;
   }
-  for (long c = (long)N + 2 + tid; c < Kp; c += blockDim.x) Ar[c] = 0.0;
+  for (long c = (long)N + 2 + tid; c < Kp; c += nth) Ar[c] = 0.0;
 }
 
 // B[l, 2n] = Re M_l(p_n), B[l, 2n + 1] = Im M_l(p_n), p_n = -1.1 + i eta_n (cmb.py:636-677)
@@ -427,60 +578,120 @@ __global__ void bessel_coef_kernel(int n_ell, int N, const double *__restrict__ 
 
 // ---------------------------------------------------------------------------------------------
 // Theta = A B^T with FP64 MFMA.  A: M x K, B: N x K (both K-contiguous, K a multiple of 16),
-// C: M x N row-major.  Workgroup: 256 lanes (4 waves in 2 x 2), tile 128 x 128, each wave a
-// 64 x 64 block of 4 x 4 v_mfma_f64_16x16x4f64 accumulators; K staged through LDS 16 at a time.
+// C: M x N row-major.  Workgroup: 512 lanes (8 waves in 4 x 2), tile 128 x 128, each wave a
+// 32 x 64 block of 2 x 4 v_mfma_f64_16x16x4f64 accumulators (small enough for two waves per SIMD).
 // Operand layout of the 16x16x4 f64 MFMA: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16];
 // the accumulator's element v of lane l is C[4 v + l / 16][l % 16] (checked on the device).
+// K is staged through LDS 32 at a time; the next 32 are loaded from HBM into registers while the
+// current ones are multiplied.  Inside a K-tile, MFMA step s of lane (l % 16, l / 16) takes
+// k = 8 (l / 16) + s for both operands (a permutation of the tile's k, the same on both sides, so
+// the sum is unchanged), which makes each lane's two steps s, s + 1 one 16-byte LDS read.
+// Block b runs on XCD b % 8 (the dispatcher's round robin): the tiles are numbered so that the
+// blocks of one XCD cover one compact rectangle of tiles (gm tile rows by about half the tile
+// columns), whose operand panels its L2 then shares.
 // ---------------------------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
-constexpr int TM = 128, TN = 128, TK = 16, TKP = TK + 1;  // +1: LDS rows padded against bank conflicts
+constexpr int TM = 128, TN = 128, TK = 32, TKP = TK + 2;  // +2: 16-byte rows offset against LDS bank conflicts
+constexpr int NXCD = 8;
 
-__global__ __launch_bounds__(256) void contract_kernel(const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C,
-                                                       int Mr, int Nc, long K) {
-  __shared__ double As[TM * TKP], Bs[TN * TKP];
+constexpr int CT = 512;  // threads per contraction workgroup
+__global__ __launch_bounds__(CT) void contract_kernel(const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C,
+                                                       int Mr, int Nc, long K, int tiles_m, int tiles_n, int per_xcd, int gm) {
+  __shared__ __align__(16) double As[TM * TKP], Bs[TN * TKP];
+  // virtual tile number: XCD-major, then grouped column-major in bands of gm tile rows
+  const int b = blockIdx.x;
+  const long v = (long)(b % NXCD) * per_xcd + b / NXCD;
+  if (v >= (long)tiles_m * tiles_n) return;
+  const long band = v / ((long)gm * tiles_n), within = v % ((long)gm * tiles_n);
+  const int rows_in_band = (int)min((long)gm, (long)tiles_m - band * gm);
+  const int tm = (int)(band * gm + within % rows_in_band), tn = (int)(within / rows_in_band);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-  const long m0 = (long)blockIdx.y * TM, n0 = (long)blockIdx.x * TN;
-  v4d acc[4][4];
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+  const long m0 = (long)tm * TM, n0 = (long)tn * TN;
+  v4d acc[2][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
   const int lr = lane & 15, lk = lane >> 4;
+  // global -> register staging: thread covers double pairs e = tid + CT t, row e / 16, columns 2 (e % 16), + 1
+  constexpr int NL = TM * TK / 2 / CT;
+  double2 ga[NL], gb[NL];
+  auto load = [&](long k0) {
+#pragma unroll
+    for (int t = 0; t < NL; ++t) {
+      const int e = tid + t * CT;
+      const int r = e >> 4, c = 2 * (e & 15);
+      const long ra = m0 + r, rb = n0 + r;
+      const bool kin = k0 + c < K;  // K is even: a pair is wholly inside or outside
+      ga[t] = (ra < Mr && kin) ? *reinterpret_cast<const double2 *>(A + ra * K + k0 + c) : make_double2(0.0, 0.0);
+      gb[t] = (rb < Nc && kin) ? *reinterpret_cast<const double2 *>(B + rb * K + k0 + c) : make_double2(0.0, 0.0);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int t = 0; t < NL; ++t) {
+      const int e = tid + t * CT;
+      const int r = e >> 4, c = 2 * (e & 15);
+      *reinterpret_cast<double2 *>(As + r * TKP + c) = ga[t];
+      *reinterpret_cast<double2 *>(Bs + r * TKP + c) = gb[t];
+    }
+  };
+  load(0);
+  store();
+  __syncthreads();
   for (long k0 = 0; k0 < K; k0 += TK) {
-    // stage: 128 rows x 16 columns of each operand, 8 doubles per thread
+    const bool more = k0 + TK < K;
+    if (more) load(k0 + TK);
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int e = tid + t * 256;
-      const int r = e >> 4, c = e & 15;
-      const long ga = m0 + r, gb = n0 + r;
-      As[r * TKP + c] = (ga < Mr) ? A[ga * K + k0 + c] : 0.0;
-      Bs[r * TKP + c] = (gb < Nc) ? B[gb * K + k0 + c] : 0.0;
+    for (int s = 0; s < TK / 4; s += 2) {
+      double2 a[2], bb[4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const double2 *>(As + (wm + i * 16 + lr) * TKP + lk * 8 + s);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bb[j] = *reinterpret_cast<const double2 *>(Bs + (wn + j * 16 + lr) * TKP + lk * 8 + s);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, bb[j].x, acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, bb[j].y, acc[i][j], 0, 0, 0);
     }
     __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < TK; kk += 4) {
-      double a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[(wm + i * 16 + lr) * TKP + kk + lk];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[(wn + j * 16 + lr) * TKP + kk + lk];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (more) {
+      store();
+      __syncthreads();
     }
-    __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const long r = m0 + wm + i * 16 + 4 * v + lk, c = n0 + wn + j * 16 + lr;
-        if (r < Mr && c < Nc) C[r * Nc + c] = acc[i][j][v];
+      for (int v4 = 0; v4 < 4; ++v4) {
+        const long r = m0 + wm + i * 16 + 4 * v4 + lk, c = n0 + wn + j * 16 + lr;
+        if (r < Mr && c < Nc) C[r * Nc + c] = acc[i][j][v4];
       }
+}
+
+// grid of contract_kernel: NXCD x per_xcd blocks (the surplus exits), bands of gm tile rows
+struct ContractGrid {
+  int tiles_m, tiles_n, per_xcd, gm, blocks;
+};
+static ContractGrid contract_grid(int m, int n) {
+  ContractGrid g;
+  g.tiles_m = (m + TM - 1) / TM;
+  g.tiles_n = (n + TN - 1) / TN;
+  const int tiles = g.tiles_m * g.tiles_n;
+  g.per_xcd = (tiles + NXCD - 1) / NXCD;
+  const int half_cols = (g.tiles_n + 1) / 2;
+  g.gm = (g.per_xcd + half_cols - 1) / half_cols;
+  if (g.gm < 1) g.gm = 1;
+  if (g.gm > g.tiles_m) g.gm = g.tiles_m;
+  g.blocks = NXCD * g.per_xcd;
+  return g;
 }
 
 // C_l = sum_i (ln k_{i+1} - ln k_i) (f_i + f_{i+1}) / 2, f = (k / k_p)^(n_s - 1) Theta_l(k)^2
@@ -533,7 +744,7 @@ extern "C" int deb_launch_cmb_sources(const double *d_t, int num_thermo, const d
   hipLaunchKernelGGL(deb::source_point_kernel, dim3((unsigned)((pts + 127) / 128)), dim3(128), 0, stream, d_t, d_y, d_yp, d_k, n_k, n_tau,
                      d_aexp, V, qw, *prob, isw, d_out, pot);
   if (isw == 0) {
-    hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(1), 0, stream, d_tau, n_tau, cp);
+    hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(256), 0, stream, d_tau, n_tau, cp);
     hipLaunchKernelGGL(deb::source_isw_kernel, dim3((n_k + 63) / 64), dim3(64), 0, stream, d_tau, n_tau, n_k, cp, pot, Spot, V, d_out);
   }
   return (int)hipGetLastError();
@@ -546,11 +757,12 @@ extern "C" int deb_launch_visibility(const double *d_t, int num_thermo, const do
   return (int)hipGetLastError();
 }
 
-static long los_kp(int n_fftlog) { return (((long)n_fftlog + 2 + 15) / 16) * 16; }
+static long los_kp(int n_fftlog) { return (((long)n_fftlog + 2 + 31) / 32) * 32; }
 
 extern "C" long deb_los_work_doubles(int n_k, int n_tau, int n_ell, int n_fftlog, int n_k_dense) {
   const long nk = n_k_dense > 0 ? n_k_dense : n_k, Kp = los_kp(n_fftlog);
   long w = (long)n_fftlog + 3L * n_k + 2L * n_tau;  // twiddles, log k / cp_k, lco / cp_chi
+  w += 2L * n_tau + 6L * n_fftlog + 64;               // the FFTLog setup tables
   if (n_k_dense > 0) w += (long)n_k * n_tau + 2L * n_k_dense + 2L * n_k_dense * n_tau;
   w += nk * n_tau + nk * Kp + (long)n_ell * Kp;
   return w + 64;
@@ -570,6 +782,12 @@ extern "C" int deb_launch_los(const double *d_k, int n_k, const double *d_tau, i
   double *cpk = w; w += 2L * n_k;
   double *lco = w; w += n_tau;
   double *cpchi = w; w += n_tau;
+  double *rden = w; w += n_tau;
+  double *bco = w; w += n_tau;
+  double *Wq = w; w += 4L * n_fftlog;
+  w += (w - d_work) & 1;  // 16-byte alignment of the phase table
+  double2 *ph = (double2 *)w; w += 2L * (n_fftlog / 2 + 1);
+  int *Iq = (int *)w; w += (n_fftlog + 1) / 2 + 1;
   const double *S = d_S;
   const double *kuse = d_k;
   int nk = n_k;
@@ -582,7 +800,7 @@ extern "C" int deb_launch_los(const double *d_k, int n_k, const double *d_tau, i
     double *Sd = w; w += 2L * n_k_dense * n_tau;  // densified S (the second half is slack)
     int mx = n_k > n_k_dense ? n_k : n_k_dense;
     hipLaunchKernelGGL(deb::log_k_kernel, dim3((mx + 255) / 256), dim3(256), 0, stream, d_k, n_k, n_k_dense, lk, lkd, kd);
-    hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(1), 0, stream, lk, n_k, cpk);
+    hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(256), 0, stream, lk, n_k, cpk);
     hipLaunchKernelGGL(deb::densify_build_kernel, dim3((n_tau + 127) / 128), dim3(128), 0, stream, lk, n_k, n_tau, cpk, d_S, Sk);
     long pts = (long)n_k_dense * n_tau;
     hipLaunchKernelGGL(deb::densify_eval_kernel, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, stream, lk, n_k, n_tau, d_S, Sk, lkd,
@@ -597,17 +815,25 @@ extern "C" int deb_launch_los(const double *d_k, int n_k, const double *d_tau, i
     if (e != hipSuccess) return (int)e;
   }
   double *Schi = w; w += (long)nk * n_tau;
+  w += (w - d_work) & 1;  // 16-byte alignment of the contraction operands (Kp is even)
   double *A = w; w += (long)nk * Kp;
   double *B = w; w += (long)n_ell * Kp;
   hipLaunchKernelGGL(deb::chi_knots_kernel, dim3((n_tau + 255) / 256), dim3(256), 0, stream, d_tau, n_tau, tau0, lco);
-  hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(1), 0, stream, lco, n_tau, cpchi);
-  const size_t lds = sizeof(double2) * (size_t)(n_fftlog / 2);
+  hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(256), 0, stream, lco, n_tau, cpchi);
+  hipLaunchKernelGGL(deb::fftlog_setup_kernel, dim3(1), dim3(256), 0, stream, lco, n_tau, cpchi, d_tau, tau0, n_fftlog, rden, bco, Iq, Wq,
+                     ph);
+  // LDS: the FFT buffer, the scan's 2 x 256 doubles, and the row's spline second derivatives if
+  // they fit in the 160 KiB of a compute unit (else they stay in Schi)
+  const size_t lds_base = sizeof(double2) * (size_t)(n_fftlog / 2) + 2 * 256 * sizeof(double);
+  const int sd_lds = (lds_base + sizeof(double) * (size_t)n_tau <= 160 * 1024) ? 1 : 0;
+  const size_t lds = lds_base + (sd_lds ? sizeof(double) * (size_t)n_tau : 0);
   auto launch_fft = [&](auto kern) -> int {
     if (lds > 64 * 1024) {
       hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL(kern, dim3(nk), dim3(256), lds, stream, S, n_tau, lco, cpchi, Schi, kuse, d_tau, tau0, tw, A, Kp);
+    hipLaunchKernelGGL(kern, dim3(nk), dim3(256), lds, stream, S, n_tau, lco, cpchi, rden, bco, Iq, Wq, ph, sd_lds, Schi, kuse, d_tau, tau0,
+                       tw, A, Kp);
     return (int)hipGetLastError();
   };
   int rc = 0;
@@ -625,14 +851,17 @@ extern "C" int deb_launch_los(const double *d_k, int n_k, const double *d_tau, i
   long cells = (long)n_ell * (Kp / 2);
   hipLaunchKernelGGL(deb::bessel_coef_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, stream, n_ell, n_fftlog, d_tau, n_tau,
                      tau0, B, Kp);
-  dim3 grid((n_ell + deb::TN - 1) / deb::TN, (nk + deb::TM - 1) / deb::TM);
-  hipLaunchKernelGGL(deb::contract_kernel, grid, dim3(256), 0, stream, A, B, d_theta, nk, n_ell, Kp);
+  const deb::ContractGrid g = deb::contract_grid(nk, n_ell);
+  hipLaunchKernelGGL(deb::contract_kernel, dim3(g.blocks), dim3(deb::CT), 0, stream, A, B, d_theta, nk, n_ell, Kp, g.tiles_m, g.tiles_n, g.per_xcd,
+                     g.gm);
   return (int)hipGetLastError();
 }
 
 extern "C" int deb_launch_contract(const double *d_A, const double *d_B, int m, int n, long k, double *d_C, hipStream_t stream) {
-  dim3 grid((n + deb::TN - 1) / deb::TN, (m + deb::TM - 1) / deb::TM);
-  hipLaunchKernelGGL(deb::contract_kernel, grid, dim3(256), 0, stream, d_A, d_B, d_C, m, n, k);
+  if (m == 0 || n == 0) return 0;
+  const deb::ContractGrid g = deb::contract_grid(m, n);
+  hipLaunchKernelGGL(deb::contract_kernel, dim3(g.blocks), dim3(deb::CT), 0, stream, d_A, d_B, d_C, m, n, k, g.tiles_m, g.tiles_n, g.per_xcd,
+                     g.gm);
   return (int)hipGetLastError();
 }
 

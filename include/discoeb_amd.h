@@ -225,7 +225,7 @@ long deb_los_workspace_doubles(int n_k, int n_tau, int ellmax, int n_fftlog, int
 int deb_los_theta_ell(const double *d_k, int n_k, const double *d_tau, int n_tau, const double *d_S, double tau0, int ellmax, int n_fftlog,
                       int n_k_dense, double *d_theta, double *d_k_out, double *d_work, void *stream);
 
-/* C = A B^T in FP64 on the matrix cores: A m x k, B n x k (row-major, k a multiple of 16), C
+/* C = A B^T in FP64 on the matrix cores: A m x k, B n x k (row-major, k a multiple of 16, both 16-byte aligned), C
  * m x n.  The contraction of compute_theta_ell (src/discoeb/cmb.py:805-833, the einsums over
  * the FFTLog frequencies), exposed on its own. */
 int deb_contract(const double *d_A, const double *d_B, int m, int n, long k, double *d_C, void *stream);
