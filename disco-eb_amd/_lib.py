@@ -50,7 +50,8 @@ EXPORTS = ["deb_version", "deb_last_error", "deb_tables_size", "deb_table_offset
            "deb_evolve_adjoint", "deb_evolve_batched_workspace_doubles",
            "deb_evolve_batched", "deb_noalloc_scratch_doubles", "deb_evolve_batched_noalloc", "deb_tau_out", "deb_nvar", "deb_power", "deb_power_units",
            "deb_time_derivatives", "deb_visibility", "deb_cmb_sources_workspace_doubles", "deb_cmb_sources",
-           "deb_los_workspace_doubles", "deb_los_theta_ell", "deb_contract", "deb_cell", "deb_dell", "deb_savgol_coefficients",
+           "deb_cmb_neutrinos", "deb_cmb_metric", "deb_cmb_polarization", "deb_cmb_source_term", "deb_cmb_isw_workspace_doubles",
+           "deb_cmb_source_isw", "deb_los_workspace_doubles", "deb_los_theta_ell", "deb_contract", "deb_cell", "deb_dell", "deb_savgol_coefficients",
            "deb_savgol_filter", "deb_power_smoothed_workspace_doubles", "deb_power_smoothed", "deb_power_kaiser", "deb_power_multipoles",
            "deb_xi_workspace_doubles", "deb_xi_from_power"]
 
@@ -64,6 +65,14 @@ class DebSolverCfg(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("pcoeff", ctypes.c_double),
                 ("icoeff", ctypes.c_double), ("dcoeff", ctypes.c_double), ("factormax", ctypes.c_double),
                 ("factormin", ctypes.c_double), ("max_steps", ctypes.c_int)]
+
+
+METRIC_INPUTS = ("eta", "deltac", "thetac", "deltab", "thetab", "deltag", "thetag", "shearg", "sheargp", "deltar", "thetar", "shearr",
+                 "shearrp", "deltaq", "thetaq", "drhonu", "fnu", "shearnu", "rho_nu_prime", "shear_nu_prime", "aH", "rhonu", "rho_Q", "w_Q")
+
+
+class DebMetricInputs(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in METRIC_INPUTS]
 
 
 class DebSchedule(ctypes.Structure):
@@ -116,6 +125,14 @@ def lib():
                                              ctypes.POINTER(DebSchedule), _vp]
     L.deb_tau_out.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
     _d = ctypes.c_double
+    _i = ctypes.c_int
+    L.deb_cmb_neutrinos.argtypes = [_vp, _i, _i, _vp, _vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]
+    L.deb_cmb_metric.argtypes = [_vp, _i, _i, ctypes.POINTER(DebMetricInputs), _vp, _i, _vp, _i, _vp, _vp]
+    L.deb_cmb_polarization.argtypes = [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.deb_cmb_source_term.argtypes = [_i, _vp, _i, _i] + [_vp] * 11
+    L.deb_cmb_isw_workspace_doubles.restype = ctypes.c_long
+    L.deb_cmb_isw_workspace_doubles.argtypes = [_i, _i]
+    L.deb_cmb_source_isw.argtypes = [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]
     L.deb_savgol_coefficients.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp]
     L.deb_savgol_filter.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
     L.deb_power_smoothed_workspace_doubles.restype = ctypes.c_long

@@ -82,6 +82,147 @@ def compute_source_function_from_states(yout, yprime, kmodes, aexp_out, tau, par
     return {"S1": out[0], "S2": out[1], "S3": out[2], "S4": out[3], "S": out[4], "alpha": out[5], "alphaprime": out[6]}
 
 
+# ---------------------------------------------------------------------------------------------
+# the stages of the source function one at a time (cmb.py:170-633)
+# ---------------------------------------------------------------------------------------------
+def _plane(x, dev):
+    return torch.as_tensor(x, dtype=torch.float64).to(dev).contiguous()
+
+
+def _kt(kmodes, dev):
+    return torch.as_tensor(kmodes, dtype=torch.float64).to(dev).reshape(-1).contiguous()
+
+
+def compute_neutrino_perturbations(yout, yprime, aexp_out, param, nqmax, iq0):
+    """Massive-neutrino density, pressure, flux and shear perturbations and the derivatives of the
+    density and shear (cmb.py:170-240): momentum integrals of the states yout and yprime
+    (n_k, n_tau, nvar) whose bins start at iq0.  Returns the reference's dict (planes (n_k, n_tau),
+    aprimeoa (n_tau,))."""
+    t, N, c = _tables(param)
+    dev = t.device
+    y, yp = _dev(yout, dev), _dev(yprime, dev)
+    if y.ndim != 3 or y.shape != yp.shape:
+        raise ValueError("yout and yprime must be (n_k, n_tau, nvar) of one shape")
+    nk, nt, nv = y.shape
+    aa = _dev(aexp_out, dev).reshape(-1)
+    if aa.numel() != nt:
+        raise ValueError("aexp_out must have n_tau entries")
+    out = torch.empty((6, nk, nt), dtype=torch.float64, device=dev)
+    aH = torch.empty(nt, dtype=torch.float64, device=dev)
+    work = torch.empty(32, dtype=torch.float64, device=dev)
+    _lib.check(_lib.lib().deb_cmb_neutrinos(_lib.ptr(t), N, c, _lib.ptr(y), _lib.ptr(yp), nk, _lib.ptr(aa), nt, nv, int(iq0), int(nqmax),
+                                            _lib.ptr(out), _lib.ptr(aH), _lib.ptr(work), _lib.stream()))
+    names = ["drhonu", "dpnu", "fnu", "shearnu", "rho_nu_prime", "shear_nu_prime"]
+    res = {nm: out[i] for i, nm in enumerate(names)}
+    res["aprimeoa"] = aH
+    return res
+
+
+def compute_metric_perturbations(perturbations, neutrinos, background_quantities, param, kmodes, aexp_out):
+    """hprime, etaprime, alpha, alpha', alpha'' and the derivative of the conformal Hubble rate
+    from the Einstein equations (cmb.py:243-345), on the dicts of extract_perturbations,
+    compute_neutrino_perturbations and compute_background_quantities."""
+    t, N, c = _tables(param)
+    dev = t.device
+    kk = _kt(kmodes, dev)
+    aa = _dev(aexp_out, dev).reshape(-1).contiguous()
+    pt, nu, bq = perturbations, neutrinos, background_quantities
+    src = {"eta": pt["eta"], "deltac": pt["deltac"], "thetac": pt["thetac"], "deltab": pt["deltab"], "thetab": pt["thetab"],
+           "deltag": pt["deltag"], "thetag": pt["thetag"], "shearg": pt["shearg"], "sheargp": pt["sheargprime"], "deltar": pt["deltar"],
+           "thetar": pt["thetar"], "shearr": pt["shearr"], "shearrp": pt["shearrprime"], "deltaq": pt["deltaq"], "thetaq": pt["thetaq"],
+           "drhonu": nu["drhonu"], "fnu": nu["fnu"], "shearnu": nu["shearnu"], "rho_nu_prime": nu["rho_nu_prime"],
+           "shear_nu_prime": nu["shear_nu_prime"], "aH": nu["aprimeoa"], "rhonu": bq["rhonu"], "rho_Q": bq["rho_Q"], "w_Q": bq["w_Q"]}
+    nk, nt = kk.numel(), aa.numel()
+    planes = {}
+    for name in _lib.METRIC_INPUTS:
+        v = _plane(src[name], dev)
+        want = (nt,) if name in ("aH", "rhonu", "rho_Q", "w_Q") else (nk, nt)
+        if tuple(v.reshape(want).shape) != want:
+            raise ValueError(f"{name} must have shape {want}")
+        planes[name] = v.reshape(want)
+    ins = _lib.DebMetricInputs(*[_lib.ptr(planes[name]) for name in _lib.METRIC_INPUTS])
+    out = torch.empty((6, nk, nt), dtype=torch.float64, device=dev)
+    _lib.check(_lib.lib().deb_cmb_metric(_lib.ptr(t), N, c, ctypes.byref(ins), _lib.ptr(kk), nk, _lib.ptr(aa), nt, _lib.ptr(out),
+                                         _lib.stream()))
+    return {"hprime": out[0], "etaprime": out[1], "alpha": out[2], "alphaprime": out[3], "alphapprime": out[4], "aprimeoa": planes["aH"],
+            "aprimeoaprime": out[5]}
+
+
+def compute_polarization_terms(perturbations, metric, visibility_functions, yout, yprime, kmodes, lmaxg, lmaxgp):
+    """F_2 + G_0 + G_2 and its first and second derivatives (cmb.py:349-395)."""
+    dev = _lib.device()
+    ys, yps = _dev(yout, dev), _dev(yprime, dev)
+    nk, nt, nv = ys.shape
+    kk = _kt(kmodes, dev)
+    tg = _plane(perturbations["thetagprime"], dev).reshape(nk, nt)
+    ap = _plane(metric["alphaprime"], dev).reshape(nk, nt)
+    op = _plane(visibility_functions["opac"], dev).reshape(nt)
+    opp = _plane(visibility_functions["opacprime"], dev).reshape(nt)
+    out = torch.empty((3, nk, nt), dtype=torch.float64, device=dev)
+    _lib.check(_lib.lib().deb_cmb_polarization(_lib.ptr(ys), _lib.ptr(yps), nk, nt, nv, int(lmaxg), int(lmaxgp), _lib.ptr(kk), _lib.ptr(tg),
+                                               _lib.ptr(ap), _lib.ptr(op), _lib.ptr(opp), _lib.ptr(out), _lib.stream()))
+    return {"polarization_term": out[0], "polarization_termprime": out[1], "polarization_termpprime": out[2]}
+
+
+def _source_term(term, kmodes, vis, planes):
+    dev = _lib.device()
+    kk = _kt(kmodes, dev)
+    pol = _plane(planes["pol"], dev)
+    nk, nt = pol.shape
+    v = _plane(vis, dev).reshape(nt)
+    args = [_plane(planes[n], dev).reshape(nk, nt) if planes.get(n) is not None else None
+            for n in ("deltag", "alpha", "alphaprime", "thetab", "thetabprime", "pol", "polp", "polpp")]
+    out = torch.empty((nk, nt), dtype=torch.float64, device=dev)
+    _lib.check(_lib.lib().deb_cmb_source_term(int(term), _lib.ptr(kk), nk, nt, _lib.ptr(v), *[_lib.ptr(a) if a is not None else None for a in args],
+                                              _lib.ptr(out), _lib.stream()))
+    return out
+
+
+def compute_source_term_isw(metric, visibility_functions, perturbations, tau):
+    """S1 = e^-tau_c d(eta + alpha')/dtau by a natural spline in tau (cmb.py:401-452)."""
+    dev = _lib.device()
+    tt = _dev(tau, dev).reshape(-1).contiguous()
+    eta = _plane(perturbations["eta"], dev)
+    nk, nt = eta.shape
+    ap = _plane(metric["alphaprime"], dev).reshape(nk, nt)
+    od = _plane(visibility_functions["optical_depth"], dev).reshape(nt)
+    L = _lib.lib()
+    out = torch.empty((nk, nt), dtype=torch.float64, device=dev)
+    work = torch.empty(L.deb_cmb_isw_workspace_doubles(nk, nt), dtype=torch.float64, device=dev)
+    _lib.check(L.deb_cmb_source_isw(_lib.ptr(tt), nt, nk, _lib.ptr(eta), _lib.ptr(ap), _lib.ptr(od), _lib.ptr(out), _lib.ptr(work),
+                                    _lib.stream()))
+    return out
+
+
+def compute_source_term_sachs_wolfe(perturbations, metric, visibility_functions, polarization_terms, kmodes):
+    """S2 = g (delta_g / 4 + 2 alpha' + Pi / 16 + (theta_b' + 3 Pi'' / 16) / k^2) (cmb.py:455-496)."""
+    return _source_term(2, kmodes, visibility_functions["gvis"],
+                        {"deltag": perturbations["deltag"], "alphaprime": metric["alphaprime"], "thetabprime": perturbations["thetabprime"],
+                         "pol": polarization_terms["polarization_term"], "polpp": polarization_terms["polarization_termpprime"]})
+
+
+def compute_source_term_doppler(perturbations, metric, visibility_functions, polarization_terms, kmodes):
+    """S3 = g' (alpha + (theta_b + 3 Pi' / 8) / k^2) (cmb.py:499-534)."""
+    return _source_term(3, kmodes, visibility_functions["gvisprime"],
+                        {"alpha": metric["alpha"], "thetab": perturbations["thetab"], "pol": polarization_terms["polarization_term"],
+                         "polp": polarization_terms["polarization_termprime"]})
+
+
+def compute_source_term_polarization(visibility_functions, polarization_terms, kmodes):
+    """S4 = g'' 3 Pi / (16 k^2) (cmb.py:537-563)."""
+    return _source_term(4, kmodes, visibility_functions["gvispprime"], {"pol": polarization_terms["polarization_term"]})
+
+
+def compute_source_function(perturbations, metric, visibility_functions, yout, yprime, kmodes, lmaxg, lmaxgp, tau):
+    """S1..S4, their sum and the polarisation terms (cmb.py:566-633), stage by stage."""
+    pol = compute_polarization_terms(perturbations, metric, visibility_functions, yout, yprime, kmodes, lmaxg, lmaxgp)
+    S1 = compute_source_term_isw(metric, visibility_functions, perturbations, tau)
+    S2 = compute_source_term_sachs_wolfe(perturbations, metric, visibility_functions, pol, kmodes)
+    S3 = compute_source_term_doppler(perturbations, metric, visibility_functions, pol, kmodes)
+    S4 = compute_source_term_polarization(visibility_functions, pol, kmodes)
+    return {"S1": S1, "S2": S2, "S3": S3, "S4": S4, "S": S1 + S2 + S3 + S4, "polarization_terms": pol}
+
+
 def compute_theta_ell(ellmax, kmodes, tau, S, tau0, n_fftlog=16384, n_k_dense=0):
     """Theta_l(k) by FFTLog line-of-sight integration (cmb.py:680-835).  Returns (theta_ell
     (n_k, ellmax + 1), kmodes used) as device tensors."""
@@ -221,5 +362,7 @@ def compute_Cell_spectrum_from_cosmo_params(param_dict, ellmax=2500, nmodes=128,
     return compute_Cell(theta, kk, param["n_s"], param.get("k_p", 0.05)), param
 
 
-__all__ = ["extract_perturbations", "compute_visibility_functions", "compute_source_function_from_states", "compute_theta_ell",
+__all__ = ["extract_perturbations", "compute_visibility_functions", "compute_neutrino_perturbations", "compute_metric_perturbations",
+           "compute_polarization_terms", "compute_source_term_isw", "compute_source_term_sachs_wolfe", "compute_source_term_doppler",
+           "compute_source_term_polarization", "compute_source_function", "compute_source_function_from_states", "compute_theta_ell",
            "compute_Cell", "compute_Dell", "cmb_aexp_out", "compute_sources_streamed", "compute_Cell_spectrum_from_cosmo_params"]

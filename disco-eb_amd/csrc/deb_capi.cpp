@@ -37,6 +37,14 @@ extern "C" int deb_launch_adjoint(const double *, int, const double *, double *,
                                   const double *, double *, long, double *, double *, double *, int *, const int *, hipStream_t);
 extern "C" int deb_launch_spline_eval(const double *, int, int, int, const double *, int, double *, hipStream_t);
 extern "C" int deb_launch_power(const double *, int, const double *, long, int, int, double, double, double, double *, hipStream_t);
+extern "C" int deb_launch_cmb_neutrinos(const double *, const double *, const double *, int, const double *, int, int, int, int, double *,
+                                        double *, double *, hipStream_t);
+extern "C" int deb_launch_cmb_metric(const double *, const DebMetricInputs *, const double *, int, const double *, int, double *, hipStream_t);
+extern "C" int deb_launch_cmb_polarization(const double *, const double *, int, int, int, int, int, const double *, const double *,
+                                           const double *, const double *, const double *, double *, hipStream_t);
+extern "C" int deb_launch_cmb_source_term(int, const double *, int, int, const double *, const double *, const double *, const double *,
+                                          const double *, const double *, const double *, const double *, const double *, double *, hipStream_t);
+extern "C" int deb_launch_cmb_isw(const double *, int, int, const double *, const double *, const double *, double *, double *, hipStream_t);
 extern "C" int deb_launch_savgol_coefficients(int, int, double *, hipStream_t);
 extern "C" int deb_launch_savgol_apply(const double *, int, const double *, int, double *, hipStream_t);
 extern "C" int deb_launch_power_smoothed(const double *, const double *, int, int, int, double, double, double, int, double *, double *,
@@ -332,6 +340,70 @@ int deb_dell(const double *d_cell, int ellmax, double A_s, double Tcmb, double *
   if (ellmax < 2) return fail(-1, "ellmax must be >= 2");
   if (!d_cell || !d_ell || !d_dell) return fail(-1, "NULL device pointer");
   return hip_rc(deb_launch_dell(d_cell, ellmax, A_s, Tcmb, d_ell, d_dell, (hipStream_t)stream), "deb_dell");
+}
+
+// ------------------------------------------------------------------ CMB source stages
+
+int deb_cmb_neutrinos(const double *d_tables, int num_thermo, int cosmo, const double *d_y, const double *d_yprime, int n_k,
+                      const double *d_aexp, int n_tau, int nvar, int iq0, int nqmax, double *d_out, double *d_aprimeoa, double *d_work,
+                      void *stream) {
+  if (nqmax < 1 || nqmax > 16) return fail(-1, "nqmax must be in [1, 16]");
+  if (iq0 < 0 || iq0 + 3 * nqmax > nvar) return fail(-1, "iq0 + 3 nqmax must not exceed nvar");
+  if (n_k < 1 || n_tau < 1) return fail(-1, "need n_k >= 1 and n_tau >= 1");
+  if (!d_tables || !d_y || !d_yprime || !d_aexp || !d_out || !d_aprimeoa || !d_work) return fail(-1, "NULL device pointer");
+  const double *t = d_tables + (long)cosmo * deb_table_doubles(num_thermo);
+  return hip_rc(deb_launch_cmb_neutrinos(t, d_y, d_yprime, n_k, d_aexp, n_tau, nvar, iq0, nqmax, d_out, d_aprimeoa, d_work, (hipStream_t)stream),
+                "deb_cmb_neutrinos");
+}
+
+int deb_cmb_metric(const double *d_tables, int num_thermo, int cosmo, const DebMetricInputs *in, const double *d_k, int n_k,
+                   const double *d_aexp, int n_tau, double *d_out, void *stream) {
+  if (!in) return fail(-1, "metric inputs are NULL");
+  if (n_k < 1 || n_tau < 1) return fail(-1, "need n_k >= 1 and n_tau >= 1");
+  const double *const ptrs[] = {in->eta,    in->deltac,  in->thetac,  in->deltab,  in->thetab,       in->deltag,        in->thetag,
+                                in->shearg, in->sheargp, in->deltar,  in->thetar,  in->shearr,       in->shearrp,       in->deltaq,
+                                in->thetaq, in->drhonu,  in->fnu,     in->shearnu, in->rho_nu_prime, in->shear_nu_prime, in->aH,
+                                in->rhonu,  in->rho_Q,   in->w_Q,     d_tables,    d_k,              d_aexp,            d_out};
+  for (const double *q : ptrs)
+    if (!q) return fail(-1, "NULL device pointer");
+  const double *t = d_tables + (long)cosmo * deb_table_doubles(num_thermo);
+  return hip_rc(deb_launch_cmb_metric(t, in, d_k, n_k, d_aexp, n_tau, d_out, (hipStream_t)stream), "deb_cmb_metric");
+}
+
+int deb_cmb_polarization(const double *d_y, const double *d_yprime, int n_k, int n_tau, int nvar, int lmaxg, int lmaxgp, const double *d_k,
+                         const double *d_thetagprime, const double *d_alphaprime, const double *d_opac, const double *d_opacprime,
+                         double *d_out, void *stream) {
+  if (lmaxg < 3 || lmaxgp < 3 || 10 + lmaxg + lmaxgp > nvar) return fail(-1, "need lmaxg, lmaxgp >= 3 within nvar");
+  if (n_k < 1 || n_tau < 1) return fail(-1, "need n_k >= 1 and n_tau >= 1");
+  if (!d_y || !d_yprime || !d_k || !d_thetagprime || !d_alphaprime || !d_opac || !d_opacprime || !d_out)
+    return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_cmb_polarization(d_y, d_yprime, n_k, n_tau, nvar, 7, 8 + lmaxg, d_k, d_thetagprime, d_alphaprime, d_opac, d_opacprime,
+                                            d_out, (hipStream_t)stream),
+                "deb_cmb_polarization");
+}
+
+int deb_cmb_source_term(int term, const double *d_k, int n_k, int n_tau, const double *d_vis, const double *d_deltag, const double *d_alpha,
+                        const double *d_alphaprime, const double *d_thetab, const double *d_thetabprime, const double *d_pol,
+                        const double *d_polprime, const double *d_polpprime, double *d_out, void *stream) {
+  if (term < 2 || term > 4) return fail(-1, "term must be 2 (Sachs-Wolfe), 3 (Doppler) or 4 (polarisation)");
+  if (n_k < 1 || n_tau < 1) return fail(-1, "need n_k >= 1 and n_tau >= 1");
+  bool ok = d_k && d_vis && d_pol && d_out;
+  if (term == 2) ok = ok && d_deltag && d_alphaprime && d_thetabprime && d_polpprime;
+  if (term == 3) ok = ok && d_alpha && d_thetab && d_polprime;
+  if (!ok) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_cmb_source_term(term, d_k, n_k, n_tau, d_vis, d_deltag, d_alpha, d_alphaprime, d_thetab, d_thetabprime, d_pol,
+                                           d_polprime, d_polpprime, d_out, (hipStream_t)stream),
+                "deb_cmb_source_term");
+}
+
+long deb_cmb_isw_workspace_doubles(int n_k, int n_tau) { return 2L * n_k * n_tau + n_tau + 64; }
+
+int deb_cmb_source_isw(const double *d_tau, int n_tau, int n_k, const double *d_eta, const double *d_alphaprime, const double *d_optical_depth,
+                       double *d_S1, double *d_work, void *stream) {
+  if (n_k < 1 || n_tau < 3) return fail(-1, "need n_k >= 1 and n_tau >= 3");
+  if (!d_tau || !d_eta || !d_alphaprime || !d_optical_depth || !d_S1 || !d_work) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_cmb_isw(d_tau, n_tau, n_k, d_eta, d_alphaprime, d_optical_depth, d_S1, d_work, (hipStream_t)stream),
+                "deb_cmb_source_isw");
 }
 
 // ------------------------------------------------------------------ large-scale structure

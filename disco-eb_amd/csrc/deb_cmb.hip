@@ -151,8 +151,95 @@ __global__ void vis_eval_kernel(const double *__restrict__ t, const double *__re
 }
 
 // ---------------------------------------------------------------------------------------------
-// source terms
+// source terms: the point functions are shared by the fused kernel (every stage of one point in
+// one thread) and the stage kernels (the reference's compute_* functions one at a time)
 // ---------------------------------------------------------------------------------------------
+// massive neutrinos (perturbations.py:29-85, cmb.py:170-240): momentum integrals of the state and
+// its derivative at one point; Y, P the state and its time derivative, qw the bins
+struct NuPt {
+  double drhonu, dpnu, fnu, shearnu, rnp, snp;
+};
+__device__ inline NuPt nu_point(const double *Y, const double *P, int iq0, int nq, const double *qw, double a, double amnu, double aH) {
+  NuPt r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int q = 0; q < nq; ++q) {
+    const double qq = qw[q], wq = qw[16 + q];
+    const double aq = a * amnu / qq;
+    const double v = 1.0 / sqrt(1.0 + aq * aq);
+    const double vp = -0.5 * aq * aH * v * v * v;
+    const double p0 = Y[iq0 + q], p1 = Y[iq0 + nq + q], p2 = Y[iq0 + 2 * nq + q];
+    const double p0p = P[iq0 + q], p2p = P[iq0 + 2 * nq + q];
+    r.drhonu += wq * p0 / v;
+    r.dpnu += wq * p0 * v;
+    r.fnu += wq * p1;
+    r.shearnu += wq * p2 * v;
+    r.rnp += wq * (p0p / v - p0 / (v * v) * vp);
+    r.snp += wq * (p2p * v + p2 * vp);
+  }
+  r.dpnu = r.dpnu / 3.0;
+  r.shearnu = r.shearnu * 2.0 / 3.0;
+  r.snp = r.snp * 2.0 / 3.0;
+  return r;
+}
+
+// the perturbations the Einstein equations read at one point (cmb.py:243-345)
+struct PertPt {
+  double eta, deltac, thetac, deltab, thetab, deltag, thetag, shearg, sheargp, deltar, thetar, shearr, shearrp, deltaq, thetaq;
+};
+struct MetricPt {
+  double hprime, etaprime, alpha, alphap, alphapp, aHp;
+};
+__device__ inline MetricPt metric_point(const BgView &b, double k, double a, double aH, double rhonu, double rho_Q, double w_Q, const PertPt &x,
+                                        double drhonu, double fnu, double shearnu, double rnp, double snp) {
+  const double k2 = k * k, a2 = a * a;
+  const double Omegac = b.Omegam - b.Omegab;
+  const double rpptq = (1.0 + w_Q) * rho_Q * b.grhom * b.OmegaDE * x.thetaq * a2;
+  const double grhoprime =
+      (-b.grhom * b.Omegam / a - 2.0 * (b.grhog + b.grhor * (b.Neff + b.Nmnu * rhonu)) / a2 +
+       b.grhom * b.OmegaDE * (-3.0 * (1.0 + b.w0 + (1.0 - a) * b.wa) * rho_Q * a2 + 2.0 * rho_Q * a2)) *
+          aH +
+      b.grhor * b.Nmnu * rnp / a2;
+  MetricPt r;
+  r.aHp = grhoprime / (6.0 * aH);
+  const double dgrho = b.grhom * (Omegac * x.deltac + b.Omegab * x.deltab) / a +
+                       (b.grhog * x.deltag + b.grhor * (b.Neff * x.deltar + b.Nmnu * drhonu)) / a2 +
+                       b.grhom * b.OmegaDE * x.deltaq * rho_Q * a2;
+  const double dgtheta = b.grhom * (Omegac * x.thetac + b.Omegab * x.thetab) / a +
+                         4.0 / 3.0 * (b.grhog * x.thetag + b.Neff * b.grhor * x.thetar) / a2 + b.Nmnu * b.grhor * k * fnu / a2 + rpptq;
+  const double dgshear = 4.0 / 3.0 * (b.grhog * x.shearg + b.Neff * b.grhor * x.shearr) / a2 + b.Nmnu * b.grhor * shearnu / a2;
+  const double dgshearp =
+      4.0 / 3.0 * (b.grhog * x.sheargp + b.Neff * b.grhor * x.shearrp) / a2 + b.Nmnu * b.grhor * snp / a2 - 2.0 * aH * dgshear;
+  r.hprime = (2.0 * k2 * x.eta + dgrho) / aH;
+  r.etaprime = 0.5 * dgtheta / k2;
+  r.alpha = (r.hprime + 6.0 * r.etaprime) / 2.0 / k2;
+  r.alphap = -3.0 * dgshear / (2.0 * k2) + x.eta - 2.0 * aH * r.alpha;
+  r.alphapp = -3.0 * dgshearp / (2.0 * k2) + r.etaprime - 2.0 * (r.aHp * r.alpha + aH * r.alphap);
+  return r;
+}
+
+// polarisation terms (cmb.py:349-395): y2 = (F_2, G_0, G_2), p2 their derivatives, p3 = (F_3', G_1', G_3')
+struct PolPt {
+  double pol, polp, polpp;
+};
+__device__ inline PolPt pol_point(double k, double thetagp, double alphap, double opac, double opacp, const double *y2, const double *p2,
+                                  const double *p3) {
+  const double k2 = k * k;
+  const double couplp = 8.0 * (thetagp + k2 * alphap) / 15.0 - k * 0.6 * (p3[0] + p3[1] + p3[2]);
+  PolPt r;
+  r.pol = y2[0] + y2[1] + y2[2];
+  r.polp = p2[0] + p2[1] + p2[2];
+  r.polpp = couplp - 0.3 * (opacp * r.pol + opac * r.polp);
+  return r;
+}
+
+// S2, S3, S4 (cmb.py:455-563)
+__device__ inline double s2_point(double k, double g, double deltag, double alphap, double pol, double thetabp, double polpp) {
+  return g * (0.25 * deltag + 2.0 * alphap + pol / 16.0 + (thetabp + 3.0 / 16.0 * polpp) / (k * k));
+}
+__device__ inline double s3_point(double k, double g1, double alpha, double thetab, double polp) {
+  return g1 * (alpha + (thetab + 3.0 / 8.0 * polp) / (k * k));
+}
+__device__ inline double s4_point(double k, double g2, double pol) { return g2 * 3.0 / 16.0 * pol / (k * k); }
+
 // outputs (each n_k x n_tau, row-major by k): 0 S1, 1 S2, 2 S3, 3 S4, 4 S, 5 alpha, 6 alpha';
 // pot: n_tau x n_k (transposed) potential eta + alpha' for the ISW spline
 __global__ void source_point_kernel(const double *__restrict__ t, const double *__restrict__ y, const double *__restrict__ yp,
@@ -168,74 +255,119 @@ __global__ void source_point_kernel(const double *__restrict__ t, const double *
   const int n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + nq * (prob.lmaxnu + 1) + 2;
   const int idxg = 7, idxgp = 8 + lmaxg, idxr = 9 + lmaxg + lmaxgp, iq0 = 10 + lmaxg + lmaxgp + lmaxr;
   const double *Y = y + idx * n, *P = yp + idx * n;
-  const double k = kk[ik], k2 = k * k, a = aexp[j], a2 = a * a;
+  const double k = kk[ik], a = aexp[j];
   const double opac = V[0L * n_tau + j], opacp = V[1L * n_tau + j], g = V[2L * n_tau + j], g1 = V[3L * n_tau + j],
                g2 = V[4L * n_tau + j];
   const double aH = V[6L * n_tau + j], rhonu = V[7L * n_tau + j], rho_Q = V[8L * n_tau + j], w_Q = V[9L * n_tau + j];
-  // massive neutrinos (perturbations.py:29-85)
-  double drhonu = 0.0, fnu = 0.0, shearnu = 0.0, rnp = 0.0, snp = 0.0;
-  for (int q = 0; q < nq; ++q) {
-    const double qq = qw[q], wq = qw[16 + q];
-    const double aq = a * b.amnu / qq;
-    const double v = 1.0 / sqrt(1.0 + aq * aq);
-    const double vp = -0.5 * aq * aH * v * v * v;
-    const double p0 = Y[iq0 + q], p1 = Y[iq0 + nq + q], p2 = Y[iq0 + 2 * nq + q];
-    const double p0p = P[iq0 + q], p2p = P[iq0 + 2 * nq + q];
-    drhonu += wq * p0 / v;
-    fnu += wq * p1;
-    shearnu += wq * p2 * v;
-    rnp += wq * (p0p / v - p0 / (v * v) * vp);
-    snp += wq * (p2p * v + p2 * vp);
-  }
-  shearnu = shearnu * 2.0 / 3.0;
-  snp = snp * 2.0 / 3.0;
-  // metric (cmb.py:243-345)
-  const double Omegac = b.Omegam - b.Omegab;
-  const double eta = Y[2], deltac = Y[3], thetac = Y[4], deltab = Y[5], thetab = Y[6], thetabp = P[6];
-  const double deltag = Y[idxg], thetag = Y[idxg + 1], shearg = Y[idxg + 2] / 2.0, thetagp = P[idxg + 1], sheargp = P[idxg + 2] / 2.0;
-  const double deltar = Y[idxr], thetar = Y[idxr + 1], shearr = Y[idxr + 2] / 2.0, shearrp = P[idxr + 2] / 2.0;
-  const double deltaq = Y[n - 2], thetaq = Y[n - 1];
-  const double rpptq = (1.0 + w_Q) * rho_Q * b.grhom * b.OmegaDE * thetaq * a2;
-  const double grhoprime =
-      (-b.grhom * b.Omegam / a - 2.0 * (b.grhog + b.grhor * (b.Neff + b.Nmnu * rhonu)) / a2 +
-       b.grhom * b.OmegaDE * (-3.0 * (1.0 + b.w0 + (1.0 - a) * b.wa) * rho_Q * a2 + 2.0 * rho_Q * a2)) *
-          aH +
-      b.grhor * b.Nmnu * rnp / a2;
-  const double aHp = grhoprime / (6.0 * aH);
-  const double dgrho = b.grhom * (Omegac * deltac + b.Omegab * deltab) / a +
-                       (b.grhog * deltag + b.grhor * (b.Neff * deltar + b.Nmnu * drhonu)) / a2 + b.grhom * b.OmegaDE * deltaq * rho_Q * a2;
-  const double dgtheta = b.grhom * (Omegac * thetac + b.Omegab * thetab) / a +
-                         4.0 / 3.0 * (b.grhog * thetag + b.Neff * b.grhor * thetar) / a2 + b.Nmnu * b.grhor * k * fnu / a2 + rpptq;
-  const double dgshear = 4.0 / 3.0 * (b.grhog * shearg + b.Neff * b.grhor * shearr) / a2 + b.Nmnu * b.grhor * shearnu / a2;
-  const double dgshearp =
-      4.0 / 3.0 * (b.grhog * sheargp + b.Neff * b.grhor * shearrp) / a2 + b.Nmnu * b.grhor * snp / a2 - 2.0 * aH * dgshear;
-  const double hprime = (2.0 * k2 * eta + dgrho) / aH;
-  const double etaprime = 0.5 * dgtheta / k2;
-  const double alpha = (hprime + 6.0 * etaprime) / 2.0 / k2;
-  const double alphap = -3.0 * dgshear / (2.0 * k2) + eta - 2.0 * aH * alpha;
-  // polarisation terms (cmb.py:349-395)
-  const double couplp = 8.0 * (thetagp + k2 * alphap) / 15.0 - k * 0.6 * (P[idxg + 3] + P[idxgp + 1] + P[idxgp + 3]);
-  const double pol = Y[idxg + 2] + Y[idxgp] + Y[idxgp + 2];
-  const double polp = P[idxg + 2] + P[idxgp] + P[idxgp + 2];
-  const double polpp = couplp - 0.3 * (opacp * pol + opac * polp);
-  // S2, S3, S4 (cmb.py:455-554)
-  const double S2 = g * (0.25 * deltag + 2.0 * alphap + pol / 16.0 + (thetabp + 3.0 / 16.0 * polpp) / k2);
-  const double S3 = g1 * (alpha + (thetab + 3.0 / 8.0 * polp) / k2);
-  const double S4 = g2 * 3.0 / 16.0 * pol / k2;
+  const NuPt nu = nu_point(Y, P, iq0, nq, qw, a, b.amnu, aH);
+  const PertPt x{Y[2],           Y[3],           Y[4],           Y[5],     Y[6],     Y[idxg],  Y[idxg + 1], Y[idxg + 2] / 2.0,
+                 P[idxg + 2] / 2.0, Y[idxr], Y[idxr + 1], Y[idxr + 2] / 2.0, P[idxr + 2] / 2.0, Y[n - 2], Y[n - 1]};
+  const MetricPt mt = metric_point(b, k, a, aH, rhonu, rho_Q, w_Q, x, nu.drhonu, nu.fnu, nu.shearnu, nu.rnp, nu.snp);
+  const double y2[3] = {Y[idxg + 2], Y[idxgp], Y[idxgp + 2]}, p2[3] = {P[idxg + 2], P[idxgp], P[idxgp + 2]};
+  const double p3[3] = {P[idxg + 3], P[idxgp + 1], P[idxgp + 3]};
+  const PolPt pl = pol_point(k, P[idxg + 1], mt.alphap, opac, opacp, y2, p2, p3);
+  const double S2 = s2_point(k, g, x.deltag, mt.alphap, pl.pol, P[6], pl.polpp);
+  const double S3 = s3_point(k, g1, mt.alpha, x.thetab, pl.polp);
+  const double S4 = s4_point(k, g2, pl.pol);
   const long plane = (long)n_k * n_tau;
   out[1 * plane + idx] = S2;
   out[2 * plane + idx] = S3;
   out[3 * plane + idx] = S4;
-  out[5 * plane + idx] = alpha;
-  out[6 * plane + idx] = alphap;
+  out[5 * plane + idx] = mt.alpha;
+  out[6 * plane + idx] = mt.alphap;
   if (isw == 1) {  // S1 = e^-tau_c (eta' + alpha''), the algebraic form (cmb.py:406-408)
-    const double alphapp = -3.0 * dgshearp / (2.0 * k2) + etaprime - 2.0 * (aHp * alpha + aH * alphap);
-    const double S1 = exp(-V[5L * n_tau + j]) * (etaprime + alphapp);
+    const double S1 = exp(-V[5L * n_tau + j]) * (mt.etaprime + mt.alphapp);
     out[0 * plane + idx] = S1;
     out[4 * plane + idx] = S1 + S2 + S3 + S4;
   } else {
-    pot[(long)j * n_k + ik] = eta + alphap;
+    pot[(long)j * n_k + ik] = x.eta + mt.alphap;
   }
+}
+
+// compute_neutrino_perturbations (cmb.py:170-240) on its own: planes 0..5 (n_k x n_tau) drhonu,
+// dpnu, fnu, shearnu, rho_nu_prime, shear_nu_prime; aH (n_tau) the conformal Hubble rate
+__global__ void nu_stage_kernel(const double *__restrict__ t, const double *__restrict__ y, const double *__restrict__ yp, int n_k, int n_tau,
+                                const double *__restrict__ aexp, const double *__restrict__ qw, int n, int iq0, int nq, double *__restrict__ out,
+                                double *__restrict__ aHo) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n_k * n_tau) return;
+  const int j = (int)(idx % n_tau);
+  BgView b;
+  b.init(t);
+  const double a = aexp[j], aH = b.aprimeoa(a);
+  const NuPt nu = nu_point(y + idx * n, yp + idx * n, iq0, nq, qw, a, b.amnu, aH);
+  const long plane = (long)n_k * n_tau;
+  out[0 * plane + idx] = nu.drhonu;
+  out[1 * plane + idx] = nu.dpnu;
+  out[2 * plane + idx] = nu.fnu;
+  out[3 * plane + idx] = nu.shearnu;
+  out[4 * plane + idx] = nu.rnp;
+  out[5 * plane + idx] = nu.snp;
+  if (idx < n_tau) aHo[idx] = b.aprimeoa(aexp[idx]);
+}
+
+// compute_metric_perturbations (cmb.py:243-345) from the dictionaries' planes: planes 0..5 of out
+// hprime, etaprime, alpha, alphaprime, alphapprime, aprimeoaprime
+__global__ void metric_stage_kernel(const double *__restrict__ t, DebMetricInputs in, const double *__restrict__ kk, int n_k,
+                                    const double *__restrict__ aexp, int n_tau, double *__restrict__ out) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n_k * n_tau) return;
+  const int j = (int)(idx % n_tau), ik = (int)(idx / n_tau);
+  BgView b;
+  b.init(t);
+  const PertPt x{in.eta[idx],    in.deltac[idx], in.thetac[idx],  in.deltab[idx], in.thetab[idx], in.deltag[idx],
+                 in.thetag[idx], in.shearg[idx], in.sheargp[idx], in.deltar[idx], in.thetar[idx], in.shearr[idx],
+                 in.shearrp[idx], in.deltaq[idx], in.thetaq[idx]};
+  const MetricPt mt = metric_point(b, kk[ik], aexp[j], in.aH[j], in.rhonu[j], in.rho_Q[j], in.w_Q[j], x, in.drhonu[idx], in.fnu[idx],
+                                   in.shearnu[idx], in.rho_nu_prime[idx], in.shear_nu_prime[idx]);
+  const long plane = (long)n_k * n_tau;
+  out[0 * plane + idx] = mt.hprime;
+  out[1 * plane + idx] = mt.etaprime;
+  out[2 * plane + idx] = mt.alpha;
+  out[3 * plane + idx] = mt.alphap;
+  out[4 * plane + idx] = mt.alphapp;
+  out[5 * plane + idx] = mt.aHp;
+}
+
+// compute_polarization_terms (cmb.py:349-395): planes 0..2 polarization_term and its first and
+// second derivatives, from the state, its derivative, theta_gamma', alpha' and the opacity
+__global__ void pol_stage_kernel(const double *__restrict__ y, const double *__restrict__ yp, int n_k, int n_tau, int n, int idxg, int idxgp,
+                                 const double *__restrict__ kk, const double *__restrict__ thetagp, const double *__restrict__ alphap,
+                                 const double *__restrict__ opac, const double *__restrict__ opacp, double *__restrict__ out) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n_k * n_tau) return;
+  const int j = (int)(idx % n_tau), ik = (int)(idx / n_tau);
+  const double *Y = y + idx * n, *P = yp + idx * n;
+  const double y2[3] = {Y[idxg + 2], Y[idxgp], Y[idxgp + 2]}, p2[3] = {P[idxg + 2], P[idxgp], P[idxgp + 2]};
+  const double p3[3] = {P[idxg + 3], P[idxgp + 1], P[idxgp + 3]};
+  const PolPt pl = pol_point(kk[ik], thetagp[idx], alphap[idx], opac[j], opacp[j], y2, p2, p3);
+  const long plane = (long)n_k * n_tau;
+  out[0 * plane + idx] = pl.pol;
+  out[1 * plane + idx] = pl.polp;
+  out[2 * plane + idx] = pl.polpp;
+}
+
+// compute_source_term_sachs_wolfe / _doppler / _polarization (cmb.py:455-563): term 2, 3 or 4
+__global__ void source_term_kernel(int term, const double *__restrict__ kk, int n_k, int n_tau, const double *__restrict__ gv,
+                                   const double *__restrict__ deltag, const double *__restrict__ alpha, const double *__restrict__ alphap,
+                                   const double *__restrict__ thetab, const double *__restrict__ thetabp, const double *__restrict__ pol,
+                                   const double *__restrict__ polp, const double *__restrict__ polpp, double *__restrict__ out) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n_k * n_tau) return;
+  const int j = (int)(idx % n_tau), ik = (int)(idx / n_tau);
+  const double k = kk[ik];
+  if (term == 2) out[idx] = s2_point(k, gv[j], deltag[idx], alphap[idx], pol[idx], thetabp[idx], polpp[idx]);
+  else if (term == 3) out[idx] = s3_point(k, gv[j], alpha[idx], thetab[idx], polp[idx]);
+  else out[idx] = s4_point(k, gv[j], pol[idx]);
+}
+
+// the ISW potential eta + alpha' transposed to n_tau x n_k for source_isw_kernel
+__global__ void isw_potential_kernel(const double *__restrict__ eta, const double *__restrict__ alphap, int n_k, int n_tau,
+                                     double *__restrict__ pot) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n_k * n_tau) return;
+  const int j = (int)(idx % n_tau), ik = (int)(idx / n_tau);
+  pot[(long)j * n_k + ik] = eta[idx] + alphap[idx];
 }
 
 // S1 = exp(-tau_c) d(eta + alpha')/dtau from the natural spline of the potential in tau
@@ -255,6 +387,21 @@ __global__ void source_isw_kernel(const double *__restrict__ tau, int n_tau, int
     const double S1 = exp(-V[5L * n_tau + j]) * d;
     out[0 * plane + o] = S1;
     out[4 * plane + o] = S1 + out[1 * plane + o] + out[2 * plane + o] + out[3 * plane + o];
+  }
+}
+
+// compute_source_term_isw (cmb.py:401-452) on its own: S1 = exp(-tau_c) d(eta + alpha')/dtau, the
+// potential pot (n_tau x n_k) from isw_potential_kernel, optical depth od (n_tau)
+__global__ void isw_stage_kernel(const double *__restrict__ tau, int n_tau, int n_k, const double *__restrict__ cp,
+                                 const double *__restrict__ pot, double *__restrict__ Spot, const double *__restrict__ od,
+                                 double *__restrict__ S1) {
+  const int ik = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ik >= n_k) return;
+  spline_build(tau, pot + ik, n_k, n_tau, cp, Spot + ik, n_k);
+  for (int j = 0; j < n_tau; ++j) {
+    const int i = spl_index(tau, n_tau, tau[j]);
+    const double d = spl_slope(tau, pot + ik, n_k, Spot + ik, n_k, i, tau[j]);
+    S1[(long)ik * n_tau + j] = exp(-od[j]) * d;
   }
 }
 
@@ -747,6 +894,53 @@ extern "C" int deb_launch_cmb_sources(const double *d_t, int num_thermo, const d
     hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(256), 0, stream, d_tau, n_tau, cp);
     hipLaunchKernelGGL(deb::source_isw_kernel, dim3((n_k + 63) / 64), dim3(64), 0, stream, d_tau, n_tau, n_k, cp, pot, Spot, V, d_out);
   }
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_cmb_neutrinos(const double *d_t, const double *d_y, const double *d_yp, int n_k, const double *d_aexp, int n_tau,
+                                        int nvar, int iq0, int nqmax, double *d_out, double *d_aH, double *d_qw, hipStream_t stream) {
+  int rc = deb_launch_nu_bins(nqmax, d_qw, stream);
+  if (rc) return rc;
+  const long pts = (long)n_k * n_tau;
+  hipLaunchKernelGGL(deb::nu_stage_kernel, dim3((unsigned)((pts + 127) / 128)), dim3(128), 0, stream, d_t, d_y, d_yp, n_k, n_tau, d_aexp, d_qw,
+                     nvar, iq0, nqmax, d_out, d_aH);
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_cmb_metric(const double *d_t, const DebMetricInputs *in, const double *d_k, int n_k, const double *d_aexp, int n_tau,
+                                     double *d_out, hipStream_t stream) {
+  const long pts = (long)n_k * n_tau;
+  hipLaunchKernelGGL(deb::metric_stage_kernel, dim3((unsigned)((pts + 127) / 128)), dim3(128), 0, stream, d_t, *in, d_k, n_k, d_aexp, n_tau,
+                     d_out);
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_cmb_polarization(const double *d_y, const double *d_yp, int n_k, int n_tau, int nvar, int idxg, int idxgp,
+                                           const double *d_k, const double *d_thetagp, const double *d_alphap, const double *d_opac,
+                                           const double *d_opacp, double *d_out, hipStream_t stream) {
+  const long pts = (long)n_k * n_tau;
+  hipLaunchKernelGGL(deb::pol_stage_kernel, dim3((unsigned)((pts + 127) / 128)), dim3(128), 0, stream, d_y, d_yp, n_k, n_tau, nvar, idxg, idxgp,
+                     d_k,
+                     d_thetagp, d_alphap, d_opac, d_opacp, d_out);
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_cmb_source_term(int term, const double *d_k, int n_k, int n_tau, const double *d_g, const double *d_deltag,
+                                          const double *d_alpha, const double *d_alphap, const double *d_thetab, const double *d_thetabp,
+                                          const double *d_pol, const double *d_polp, const double *d_polpp, double *d_out, hipStream_t stream) {
+  const long pts = (long)n_k * n_tau;
+  hipLaunchKernelGGL(deb::source_term_kernel, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, stream, term, d_k, n_k, n_tau, d_g, d_deltag,
+                     d_alpha, d_alphap, d_thetab, d_thetabp, d_pol, d_polp, d_polpp, d_out);
+  return (int)hipGetLastError();
+}
+
+extern "C" int deb_launch_cmb_isw(const double *d_tau, int n_tau, int n_k, const double *d_eta, const double *d_alphap, const double *d_od,
+                                  double *d_S1, double *d_work, hipStream_t stream) {
+  double *pot = d_work, *Spot = pot + (long)n_k * n_tau, *cp = Spot + (long)n_k * n_tau;
+  const long pts = (long)n_k * n_tau;
+  hipLaunchKernelGGL(deb::isw_potential_kernel, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, stream, d_eta, d_alphap, n_k, n_tau, pot);
+  hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(256), 0, stream, d_tau, n_tau, cp);
+  hipLaunchKernelGGL(deb::isw_stage_kernel, dim3((n_k + 63) / 64), dim3(64), 0, stream, d_tau, n_tau, n_k, cp, pot, Spot, d_od, d_S1);
   return (int)hipGetLastError();
 }
 

@@ -27,6 +27,16 @@ typedef struct DebSchedule {
   const int *d_order;
   int threads_per_unit;
 } DebSchedule;
+
+/* Inputs of the Einstein-equation stage of the CMB sources (compute_metric_perturbations): planes
+ * n_k x n_tau (row-major by k) of the perturbations and neutrino moments, and n_tau values of the
+ * background at the output times. */
+typedef struct DebMetricInputs {
+  const double *eta, *deltac, *thetac, *deltab, *thetab, *deltag, *thetag, *shearg, *sheargp, *deltar, *thetar, *shearr, *shearrp,
+      *deltaq, *thetaq;
+  const double *drhonu, *fnu, *shearnu, *rho_nu_prime, *shear_nu_prime;
+  const double *aH, *rhonu, *rho_Q, *w_Q;
+} DebMetricInputs;
 #ifdef __cplusplus
 }
 #endif

@@ -213,6 +213,44 @@ int deb_cmb_sources(const double *d_tables, int num_thermo, int cosmo, const dou
                     const double *d_aexp, const double *d_tau, int n_tau, const DebProblem *prob, int isw_mode, double *d_out, double *d_work,
                     void *stream);
 
+/* The stages of the source function one at a time (the reference's per-stage functions).  Planes are
+ * n_k x n_tau doubles, row-major by k; per-time arrays n_tau doubles.
+ *
+ * Massive-neutrino moments of the states (nvar entries each; momentum bins from iq0, nqmax of
+ * them per multipole): d_out planes 0..5 = drhonu, dpnu, fnu, shearnu, rho_nu_prime,
+ * shear_nu_prime; d_aprimeoa the conformal Hubble rate at d_aexp; d_work 32 doubles.
+ * Replaces compute_neutrino_perturbations (src/discoeb/cmb.py:170-240). */
+int deb_cmb_neutrinos(const double *d_tables, int num_thermo, int cosmo, const double *d_y, const double *d_yprime, int n_k,
+                      const double *d_aexp, int n_tau, int nvar, int iq0, int nqmax, double *d_out, double *d_aprimeoa, double *d_work,
+                      void *stream);
+
+/* Einstein equations: d_out planes 0..5 = hprime, etaprime, alpha, alphaprime, alphapprime,
+ * aprimeoaprime.  Replaces compute_metric_perturbations (src/discoeb/cmb.py:243-345). */
+int deb_cmb_metric(const double *d_tables, int num_thermo, int cosmo, const DebMetricInputs *in, const double *d_k, int n_k,
+                   const double *d_aexp, int n_tau, double *d_out, void *stream);
+
+/* Polarisation combination F_2 + G_0 + G_2 and its first and second derivatives: d_out planes 0..2.
+ * Replaces compute_polarization_terms (src/discoeb/cmb.py:349-395). */
+int deb_cmb_polarization(const double *d_y, const double *d_yprime, int n_k, int n_tau, int nvar, int lmaxg, int lmaxgp, const double *d_k,
+                         const double *d_thetagprime, const double *d_alphaprime, const double *d_opac, const double *d_opacprime,
+                         double *d_out, void *stream);
+
+/* One source term: term 2 Sachs-Wolfe (d_vis = g; reads deltag, alphaprime, thetabprime, pol,
+ * polpprime), 3 Doppler (d_vis = g'; alpha, thetab, polprime), 4 polarisation (d_vis = g''; pol);
+ * pointers a term does not read may be NULL.  Replaces compute_source_term_sachs_wolfe, _doppler
+ * and _polarization (src/discoeb/cmb.py:455-563). */
+int deb_cmb_source_term(int term, const double *d_k, int n_k, int n_tau, const double *d_vis, const double *d_deltag, const double *d_alpha,
+                        const double *d_alphaprime, const double *d_thetab, const double *d_thetabprime, const double *d_pol,
+                        const double *d_polprime, const double *d_polpprime, double *d_out, void *stream);
+
+/* Doubles of workspace deb_cmb_source_isw needs. */
+long deb_cmb_isw_workspace_doubles(int n_k, int n_tau);
+
+/* S1 = e^-tau_c d(eta + alpha')/dtau by a natural spline of the potential in tau (n_tau >= 3).
+ * Replaces compute_source_term_isw (src/discoeb/cmb.py:401-452). */
+int deb_cmb_source_isw(const double *d_tau, int n_tau, int n_k, const double *d_eta, const double *d_alphaprime, const double *d_optical_depth,
+                       double *d_S1, double *d_work, void *stream);
+
 /* Doubles of workspace deb_los_theta_ell needs. */
 long deb_los_workspace_doubles(int n_k, int n_tau, int ellmax, int n_fftlog, int n_k_dense);
 
