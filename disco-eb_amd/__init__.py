@@ -14,7 +14,8 @@ from .grad import (PARAM9, PkFunction, evolve_units_adjoint, param_to_theta, pk_
 from .lss import get_power_smoothed, get_xi_from_P, power_Kaiser, power_multipoles, savgol_filter
 from .cmb import (cmb_aexp_out, compute_Cell, compute_Cell_spectrum_from_cosmo_params, compute_Dell, compute_metric_perturbations,
                   compute_neutrino_perturbations, compute_polarization_terms, compute_source_function, compute_source_function_from_states,
-                  compute_source_term_doppler, compute_source_term_isw, compute_source_term_polarization, compute_source_term_sachs_wolfe,
+                  compute_source_term_doppler, compute_source_term_E, compute_source_term_isw, compute_source_term_polarization,
+                  compute_source_term_sachs_wolfe, compute_Cell_spectra_from_cosmo_params,
                   compute_sources_streamed, compute_theta_ell, compute_visibility_functions, extract_perturbations)
 
 __all__ = ["evolve_background", "background_tables", "attach_background", "compute_background_quantities", "DeviceSpline",
@@ -22,7 +23,8 @@ __all__ = ["evolve_background", "background_tables", "attach_background", "compu
            "power_units", "tau_out_all", "autotune_schedule", "PkGraph", "extract_perturbations", "compute_visibility_functions",
            "compute_source_function_from_states", "compute_sources_streamed", "compute_theta_ell", "compute_neutrino_perturbations",
            "compute_metric_perturbations", "compute_polarization_terms", "compute_source_term_isw", "compute_source_term_sachs_wolfe",
-           "compute_source_term_doppler", "compute_source_term_polarization", "compute_source_function", "compute_Cell", "compute_Dell", "cmb_aexp_out",
+           "compute_source_term_doppler", "compute_source_term_polarization", "compute_source_function", "compute_source_term_E",
+           "compute_Cell_spectra_from_cosmo_params", "compute_Cell", "compute_Dell", "cmb_aexp_out",
            "compute_Cell_spectrum_from_cosmo_params", "PARAM9", "theta_to_param", "param_to_theta", "pk_jacobian", "pk_jacobian_many", "PkFunction",
            "pk_and_vjp", "pk_and_vjp_many", "evolve_units_adjoint", "get_power_smoothed", "get_xi_from_P", "power_Kaiser",
            "power_multipoles", "savgol_filter"]

@@ -51,7 +51,7 @@ EXPORTS = ["deb_version", "deb_last_error", "deb_tables_size", "deb_table_offset
            "deb_evolve_batched", "deb_noalloc_scratch_doubles", "deb_evolve_batched_noalloc", "deb_tau_out", "deb_nvar", "deb_power", "deb_power_units",
            "deb_time_derivatives", "deb_visibility", "deb_cmb_sources_workspace_doubles", "deb_cmb_sources",
            "deb_cmb_neutrinos", "deb_cmb_metric", "deb_cmb_polarization", "deb_cmb_source_term", "deb_cmb_isw_workspace_doubles",
-           "deb_cmb_source_isw", "deb_los_workspace_doubles", "deb_los_theta_ell", "deb_contract", "deb_cell", "deb_dell", "deb_savgol_coefficients",
+           "deb_cmb_source_isw", "deb_los_workspace_doubles", "deb_los_theta_ell", "deb_los_theta_ell_field", "deb_cell_cross", "deb_contract", "deb_cell", "deb_dell", "deb_savgol_coefficients",
            "deb_savgol_filter", "deb_power_smoothed_workspace_doubles", "deb_power_smoothed", "deb_power_kaiser", "deb_power_multipoles",
            "deb_xi_workspace_doubles", "deb_xi_from_power"]
 
@@ -129,7 +129,7 @@ def lib():
     L.deb_cmb_neutrinos.argtypes = [_vp, _i, _i, _vp, _vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]
     L.deb_cmb_metric.argtypes = [_vp, _i, _i, ctypes.POINTER(DebMetricInputs), _vp, _i, _vp, _i, _vp, _vp]
     L.deb_cmb_polarization.argtypes = [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
-    L.deb_cmb_source_term.argtypes = [_i, _vp, _i, _i] + [_vp] * 11
+    L.deb_cmb_source_term.argtypes = [_i, _vp, _i, _i] + [_vp] * 10 + [_d, _vp, _vp]
     L.deb_cmb_isw_workspace_doubles.restype = ctypes.c_long
     L.deb_cmb_isw_workspace_doubles.argtypes = [_i, _i]
     L.deb_cmb_source_isw.argtypes = [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]
@@ -152,6 +152,9 @@ def lib():
     L.deb_los_workspace_doubles.argtypes = [ctypes.c_int] * 5
     L.deb_los_theta_ell.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, _vp, _vp, _vp, _vp]
+    L.deb_los_theta_ell_field.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp]
+    L.deb_cell_cross.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp]
     L.deb_contract.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_long, _vp, _vp]
     L.deb_cell.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp]
     L.deb_dell.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp]

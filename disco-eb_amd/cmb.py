@@ -59,7 +59,8 @@ def compute_visibility_functions(tau, param, aexp=None):
 
 def compute_source_function_from_states(yout, yprime, kmodes, aexp_out, tau, param, lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax, isw="spline"):
     """S1 (ISW), S2 (Sachs-Wolfe), S3 (Doppler), S4 (polarisation), S and the metric alpha,
-    alpha' for states yout and their time derivatives yprime (n_k, n_tau, nvar) -- the chain of
+    alpha' for states yout and their time derivatives yprime (n_k, n_tau, nvar), and the E-mode
+    source SE (no counterpart in the reference; see compute_source_term_E) -- the chain of
     compute_Cell_spectrum_from_cosmo_params steps 6-10 (cmb.py:1050-1073).  isw: "spline" (the
     reference's d/dtau(eta + alpha') by a spline in tau, cmb.py:437-449) or "algebraic"
     (eta' + alpha'', the form of its docstring, cmb.py:406-408)."""
@@ -74,12 +75,12 @@ def compute_source_function_from_states(yout, yprime, kmodes, aexp_out, tau, par
     if tuple(y.shape) != (nk, nt, nv) or y.shape != yp.shape:
         raise ValueError(f"yout and yprime must have shape (n_k, n_tau, nvar) = ({nk}, {nt}, {nv})")
     L = _lib.lib()
-    out = torch.empty((7, nk, nt), dtype=torch.float64, device=dev)
+    out = torch.empty((8, nk, nt), dtype=torch.float64, device=dev)
     work = torch.empty(L.deb_cmb_sources_workspace_doubles(N, nk, nt), dtype=torch.float64, device=dev)
     prob = _lib.DebProblem(int(lmaxg), int(lmaxgp), int(lmaxr), int(lmaxnu), int(nqmax), 1)
     _lib.check(L.deb_cmb_sources(_lib.ptr(t), N, c, _lib.ptr(y), _lib.ptr(yp), _lib.ptr(kk), nk, _lib.ptr(aa), _lib.ptr(tt), nt,
                                  ctypes.byref(prob), 1 if isw == "algebraic" else 0, _lib.ptr(out), _lib.ptr(work), _lib.stream()))
-    return {"S1": out[0], "S2": out[1], "S3": out[2], "S4": out[3], "S": out[4], "alpha": out[5], "alphaprime": out[6]}
+    return {"S1": out[0], "S2": out[1], "S3": out[2], "S4": out[3], "S": out[4], "alpha": out[5], "alphaprime": out[6], "SE": out[7]}
 
 
 # ---------------------------------------------------------------------------------------------
@@ -174,7 +175,7 @@ def _source_term(term, kmodes, vis, planes):
             for n in ("deltag", "alpha", "alphaprime", "thetab", "thetabprime", "pol", "polp", "polpp")]
     out = torch.empty((nk, nt), dtype=torch.float64, device=dev)
     _lib.check(_lib.lib().deb_cmb_source_term(int(term), _lib.ptr(kk), nk, nt, _lib.ptr(v), *[_lib.ptr(a) if a is not None else None for a in args],
-                                              _lib.ptr(out), _lib.stream()))
+                                              None, 0.0, _lib.ptr(out), _lib.stream()))
     return out
 
 
@@ -213,6 +214,23 @@ def compute_source_term_polarization(visibility_functions, polarization_terms, k
     return _source_term(4, kmodes, visibility_functions["gvispprime"], {"pol": polarization_terms["polarization_term"]})
 
 
+def compute_source_term_E(visibility_functions, polarization_terms, kmodes, tau, tau0):
+    """E-mode polarisation source S_E = 3 g Pi / (16 (k chi)^2), chi = tau0 - tau, zero within 1 Mpc
+    of today.  No counterpart in the reference (which computes TT only): the line-of-sight E-mode
+    source of Zaldarriaga & Seljak (1997) in the reference's normalisation of Pi (= F_2 + G_0 + G_2,
+    four times the brightness multipoles), to be projected with compute_theta_ell(field="E")."""
+    dev = _lib.device()
+    kk = _kt(kmodes, dev)
+    pol = _plane(polarization_terms["polarization_term"], dev)
+    nk, nt = pol.shape
+    g = _plane(visibility_functions["gvis"], dev).reshape(nt)
+    tt = _dev(tau, dev).reshape(-1).contiguous()
+    out = torch.empty((nk, nt), dtype=torch.float64, device=dev)
+    _lib.check(_lib.lib().deb_cmb_source_term(5, _lib.ptr(kk), nk, nt, _lib.ptr(g), None, None, None, None, None, _lib.ptr(pol), None, None,
+                                              _lib.ptr(tt), float(tau0), _lib.ptr(out), _lib.stream()))
+    return out
+
+
 def compute_source_function(perturbations, metric, visibility_functions, yout, yprime, kmodes, lmaxg, lmaxgp, tau):
     """S1..S4, their sum and the polarisation terms (cmb.py:566-633), stage by stage."""
     pol = compute_polarization_terms(perturbations, metric, visibility_functions, yout, yprime, kmodes, lmaxg, lmaxgp)
@@ -223,9 +241,13 @@ def compute_source_function(perturbations, metric, visibility_functions, yout, y
     return {"S1": S1, "S2": S2, "S3": S3, "S4": S4, "S": S1 + S2 + S3 + S4, "polarization_terms": pol}
 
 
-def compute_theta_ell(ellmax, kmodes, tau, S, tau0, n_fftlog=16384, n_k_dense=0):
+def compute_theta_ell(ellmax, kmodes, tau, S, tau0, n_fftlog=16384, n_k_dense=0, field="T"):
     """Theta_l(k) by FFTLog line-of-sight integration (cmb.py:680-835).  Returns (theta_ell
-    (n_k, ellmax + 1), kmodes used) as device tensors."""
+    (n_k, ellmax + 1), kmodes used) as device tensors.  field="E" (an extension: the reference
+    projects temperature only) projects an E-mode source with the spin-2 factor
+    sqrt((l + 2)! / (l - 2)!)."""
+    if field not in ("T", "E"):
+        raise ValueError("field must be 'T' or 'E'")
     S = torch.as_tensor(S, dtype=torch.float64)
     dev = S.device if S.is_cuda else _lib.device()
     S = S.to(dev).contiguous()
@@ -238,20 +260,29 @@ def compute_theta_ell(ellmax, kmodes, tau, S, tau0, n_fftlog=16384, n_k_dense=0)
     theta = torch.empty((nku, int(ellmax) + 1), dtype=torch.float64, device=dev)
     kout = torch.empty(nku, dtype=torch.float64, device=dev)
     work = torch.empty(L.deb_los_workspace_doubles(nk, nt, int(ellmax), int(n_fftlog), int(n_k_dense)), dtype=torch.float64, device=dev)
-    _lib.check(L.deb_los_theta_ell(_lib.ptr(kk), nk, _lib.ptr(tt), nt, _lib.ptr(S), float(tau0), int(ellmax), int(n_fftlog),
-                                   int(n_k_dense), _lib.ptr(theta), _lib.ptr(kout), _lib.ptr(work), _lib.stream()))
+    _lib.check(L.deb_los_theta_ell_field(_lib.ptr(kk), nk, _lib.ptr(tt), nt, _lib.ptr(S), float(tau0), int(ellmax), int(n_fftlog),
+                                         int(n_k_dense), 1 if field == "E" else 0, _lib.ptr(theta), _lib.ptr(kout), _lib.ptr(work),
+                                         _lib.stream()))
     return theta, kout
 
 
-def compute_Cell(theta_ell, kmodes, n_s, k_p):
-    """C_l = trapezoid over ln k of (k/k_p)^(n_s-1) Theta_l^2 (cmb.py:838-873)."""
+def compute_Cell(theta_ell, kmodes, n_s, k_p, theta_ell2=None):
+    """C_l = trapezoid over ln k of (k/k_p)^(n_s-1) Theta_l^2 (cmb.py:838-873); with theta_ell2 (an
+    extension) the cross spectrum of Theta_l Theta2_l, e.g. C_l^TE."""
     th = torch.as_tensor(theta_ell, dtype=torch.float64)
     dev = th.device if th.is_cuda else _lib.device()
     th = th.to(dev).contiguous()
     kk = _dev(kmodes, dev).reshape(-1)
     cl = torch.empty(th.shape[1], dtype=torch.float64, device=dev)
-    _lib.check(_lib.lib().deb_cell(_lib.ptr(th), _lib.ptr(kk), kk.numel(), th.shape[1], float(n_s), float(k_p), _lib.ptr(cl),
-                                   _lib.stream()))
+    if theta_ell2 is None:
+        _lib.check(_lib.lib().deb_cell(_lib.ptr(th), _lib.ptr(kk), kk.numel(), th.shape[1], float(n_s), float(k_p), _lib.ptr(cl),
+                                       _lib.stream()))
+    else:
+        th2 = torch.as_tensor(theta_ell2, dtype=torch.float64).to(dev).contiguous()
+        if th2.shape != th.shape:
+            raise ValueError("theta_ell and theta_ell2 must have one shape")
+        _lib.check(_lib.lib().deb_cell_cross(_lib.ptr(th), _lib.ptr(th2), _lib.ptr(kk), kk.numel(), th.shape[1], float(n_s), float(k_p),
+                                             _lib.ptr(cl), _lib.stream()))
     return cl
 
 
@@ -285,7 +316,8 @@ def compute_sources_streamed(param, kmodes, aexp_out, *, lmaxg=11, lmaxgp=11, lm
     tau), so the result equals the unchunked chain's.  solver: "per_mode" (adaptive steps per
     mode, largest k first) or "batched" (batch_size modes on shared steps, as the reference's
     driver; k_chunk then a multiple of batch_size).  param: from evolve_background.  Returns
-    {"S": (n_k, n_tau), "S1".."S4", "tau": tau_out, "n_accept": (n_k,), "n_reject": (n_k,)}."""
+    {"S": (n_k, n_tau), "S1".."S4", "SE" (the E-mode source), "tau": tau_out, "n_accept": (n_k,),
+    "n_reject": (n_k,)}."""
     from .perturbations import evolve_units, evolve_units_batched, DEFAULT_SCHEDULE, _raise_on_status
     if solver not in ("per_mode", "batched"):
         raise ValueError("solver must be 'per_mode' or 'batched'")
@@ -304,7 +336,7 @@ def compute_sources_streamed(param, kmodes, aexp_out, *, lmaxg=11, lmaxgp=11, lm
     tau_all = torch.zeros((n_cosmo, n_tau), dtype=torch.float64, device=dev)
     tau_all[cosmo] = tau_out
     lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
-    out = {name: torch.empty((n_k, n_tau), dtype=torch.float64, device=dev) for name in ("S", "S1", "S2", "S3", "S4")}
+    out = {name: torch.empty((n_k, n_tau), dtype=torch.float64, device=dev) for name in ("S", "S1", "S2", "S3", "S4", "SE")}
     nacc = torch.empty(n_k, dtype=torch.int32, device=dev)
     nrej = torch.empty(n_k, dtype=torch.int32, device=dev)
     N = param["_deb_num_thermo"]
@@ -362,7 +394,39 @@ def compute_Cell_spectrum_from_cosmo_params(param_dict, ellmax=2500, nmodes=128,
     return compute_Cell(theta, kk, param["n_s"], param.get("k_p", 0.05)), param
 
 
+def compute_Cell_spectra_from_cosmo_params(param_dict, spectra=("TT", "TE", "EE"), ellmax=2500, nmodes=128, kmin=1e-4, kmax=1.0,
+                                            n_k_dense=8192, n_fftlog=16384, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, *,
+                                            k_sampling_method="camb", solver="batched", k_chunk=None, aexp_out=None, rtol=1e-4, atol=1e-4):
+    """C_l^TT, C_l^TE and C_l^EE from cosmological parameters: the chain of
+    compute_Cell_spectrum_from_cosmo_params (cmb.py:930-1090) with the E-mode source projected
+    beside the temperature source (an extension: the reference computes TT only; no reionization,
+    as in the reference's thermal history).  Returns ({spectrum: C_l}, param)."""
+    for x in spectra:
+        if x not in ("TT", "TE", "EE"):
+            raise ValueError("spectra are drawn from 'TT', 'TE', 'EE'")
+    from .perturbations import k_grid
+    param = evolve_background(param=dict(param_dict), thermo_module="RECFAST", num_thermo=1024)
+    aexp_out = cmb_aexp_out() if aexp_out is None else np.asarray(aexp_out, dtype=np.float64)
+    lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
+    tau0 = float(param["tau_of_a_spline"].evaluate(np.array([1.0])).cpu()[0])
+    kmodes = torch.as_tensor(k_grid(param, kmin, kmax, nmodes, k_sampling_method), dtype=torch.float64, device=param["_deb_tables"].device)
+    src = compute_sources_streamed(param, kmodes, aexp_out, rtol=rtol, atol=atol, solver=solver, k_chunk=k_chunk, **lm)
+    param.update(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax, nout=int(src["tau"].numel()), tau_out=src["tau"])
+    th = {}
+    if "TT" in spectra or "TE" in spectra:
+        th["T"], kk = compute_theta_ell(ellmax, kmodes, src["tau"], src["S"], tau0, n_fftlog=n_fftlog, n_k_dense=n_k_dense)
+    if "EE" in spectra or "TE" in spectra:
+        th["E"], kk = compute_theta_ell(ellmax, kmodes, src["tau"], src["SE"], tau0, n_fftlog=n_fftlog, n_k_dense=n_k_dense, field="E")
+    n_s, k_p = param["n_s"], param.get("k_p", 0.05)
+    out = {}
+    for x in spectra:
+        a, b = th[x[0]], th[x[1]]
+        out[x] = compute_Cell(a, kk, n_s, k_p) if x[0] == x[1] else compute_Cell(a, kk, n_s, k_p, theta_ell2=b)
+    return out, param
+
+
 __all__ = ["extract_perturbations", "compute_visibility_functions", "compute_neutrino_perturbations", "compute_metric_perturbations",
            "compute_polarization_terms", "compute_source_term_isw", "compute_source_term_sachs_wolfe", "compute_source_term_doppler",
            "compute_source_term_polarization", "compute_source_function", "compute_source_function_from_states", "compute_theta_ell",
-           "compute_Cell", "compute_Dell", "cmb_aexp_out", "compute_sources_streamed", "compute_Cell_spectrum_from_cosmo_params"]
+           "compute_Cell", "compute_Dell", "cmb_aexp_out", "compute_sources_streamed", "compute_Cell_spectrum_from_cosmo_params",
+           "compute_source_term_E", "compute_Cell_spectra_from_cosmo_params"]

@@ -15,10 +15,10 @@ extern "C" int deb_launch_cmb_sources(const double *, int, const double *, const
                                       int, const DebProblem *, int, double *, double *, hipStream_t);
 extern "C" int deb_launch_visibility(const double *, int, const double *, const double *, int, double *, double *, hipStream_t);
 extern "C" long deb_los_work_doubles(int, int, int, int, int);
-extern "C" int deb_launch_los(const double *, int, const double *, int, const double *, double, int, int, int, double *, double *, double *,
+extern "C" int deb_launch_los(const double *, int, const double *, int, const double *, double, int, int, int, int, double *, double *, double *,
                               hipStream_t);
 extern "C" int deb_launch_contract(const double *, const double *, int, int, long, double *, hipStream_t);
-extern "C" int deb_launch_cell(const double *, const double *, int, int, double, double, double *, hipStream_t);
+extern "C" int deb_launch_cell(const double *, const double *, const double *, int, int, double, double, double *, hipStream_t);
 extern "C" int deb_launch_dell(const double *, int, double, double, double *, double *, hipStream_t);
 
 extern "C" int deb_launch_tau_out(const double *, int, int, const double *, int, double *, hipStream_t);
@@ -43,7 +43,8 @@ extern "C" int deb_launch_cmb_metric(const double *, const DebMetricInputs *, co
 extern "C" int deb_launch_cmb_polarization(const double *, const double *, int, int, int, int, int, const double *, const double *,
                                            const double *, const double *, const double *, double *, hipStream_t);
 extern "C" int deb_launch_cmb_source_term(int, const double *, int, int, const double *, const double *, const double *, const double *,
-                                          const double *, const double *, const double *, const double *, const double *, double *, hipStream_t);
+                                          const double *, const double *, const double *, const double *, const double *, const double *, double,
+                                          double *, hipStream_t);
 extern "C" int deb_launch_cmb_isw(const double *, int, int, const double *, const double *, const double *, double *, double *, hipStream_t);
 extern "C" int deb_launch_savgol_coefficients(int, int, double *, hipStream_t);
 extern "C" int deb_launch_savgol_apply(const double *, int, const double *, int, double *, hipStream_t);
@@ -312,12 +313,18 @@ long deb_los_workspace_doubles(int n_k, int n_tau, int ellmax, int n_fftlog, int
 
 int deb_los_theta_ell(const double *d_k, int n_k, const double *d_tau, int n_tau, const double *d_S, double tau0, int ellmax, int n_fftlog,
                       int n_k_dense, double *d_theta, double *d_k_out, double *d_work, void *stream) {
+  return deb_los_theta_ell_field(d_k, n_k, d_tau, n_tau, d_S, tau0, ellmax, n_fftlog, n_k_dense, 0, d_theta, d_k_out, d_work, stream);
+}
+
+int deb_los_theta_ell_field(const double *d_k, int n_k, const double *d_tau, int n_tau, const double *d_S, double tau0, int ellmax, int n_fftlog,
+                            int n_k_dense, int field, double *d_theta, double *d_k_out, double *d_work, void *stream) {
+  if (field != 0 && field != 1) return fail(-1, "field must be 0 (temperature) or 1 (E-mode)");
   if (ellmax < 0) return fail(-1, "ellmax must be >= 0");
   if (n_fftlog < 128 || n_fftlog > 16384 || (n_fftlog & (n_fftlog - 1))) return fail(-1, "n_fftlog must be a power of two in [128, 16384]");
   if (n_k < 2 || n_tau < 3) return fail(-1, "need n_k >= 2 and n_tau >= 3");
   if (n_k_dense < 0 || n_k_dense == 1) return fail(-1, "n_k_dense must be 0 or >= 2");
   if (!d_k || !d_tau || !d_S || !d_theta || !d_k_out || !d_work) return fail(-1, "NULL device pointer");
-  int rc = deb_launch_los(d_k, n_k, d_tau, n_tau, d_S, tau0, ellmax, n_fftlog, n_k_dense, d_theta, d_k_out, d_work, (hipStream_t)stream);
+  int rc = deb_launch_los(d_k, n_k, d_tau, n_tau, d_S, tau0, ellmax, n_fftlog, n_k_dense, field, d_theta, d_k_out, d_work, (hipStream_t)stream);
   if (rc == -2) return fail(-1, "n_fftlog must be a power of two in [128, 16384]");
   return hip_rc(rc, "deb_los_theta_ell");
 }
@@ -333,7 +340,14 @@ int deb_contract(const double *d_A, const double *d_B, int m, int n, long k, dou
 int deb_cell(const double *d_theta, const double *d_k, int n_k, int n_ell, double n_s, double k_p, double *d_cell, void *stream) {
   if (n_k < 2 || n_ell < 1) return fail(-1, "need n_k >= 2 and n_ell >= 1");
   if (!d_theta || !d_k || !d_cell) return fail(-1, "NULL device pointer");
-  return hip_rc(deb_launch_cell(d_theta, d_k, n_k, n_ell, n_s, k_p, d_cell, (hipStream_t)stream), "deb_cell");
+  return hip_rc(deb_launch_cell(d_theta, nullptr, d_k, n_k, n_ell, n_s, k_p, d_cell, (hipStream_t)stream), "deb_cell");
+}
+
+int deb_cell_cross(const double *d_theta1, const double *d_theta2, const double *d_k, int n_k, int n_ell, double n_s, double k_p, double *d_cell,
+                   void *stream) {
+  if (n_k < 2 || n_ell < 1) return fail(-1, "need n_k >= 2 and n_ell >= 1");
+  if (!d_theta1 || !d_theta2 || !d_k || !d_cell) return fail(-1, "NULL device pointer");
+  return hip_rc(deb_launch_cell(d_theta1, d_theta2, d_k, n_k, n_ell, n_s, k_p, d_cell, (hipStream_t)stream), "deb_cell_cross");
 }
 
 int deb_dell(const double *d_cell, int ellmax, double A_s, double Tcmb, double *d_ell, double *d_dell, void *stream) {
@@ -384,15 +398,16 @@ int deb_cmb_polarization(const double *d_y, const double *d_yprime, int n_k, int
 
 int deb_cmb_source_term(int term, const double *d_k, int n_k, int n_tau, const double *d_vis, const double *d_deltag, const double *d_alpha,
                         const double *d_alphaprime, const double *d_thetab, const double *d_thetabprime, const double *d_pol,
-                        const double *d_polprime, const double *d_polpprime, double *d_out, void *stream) {
-  if (term < 2 || term > 4) return fail(-1, "term must be 2 (Sachs-Wolfe), 3 (Doppler) or 4 (polarisation)");
+                        const double *d_polprime, const double *d_polpprime, const double *d_tau, double tau0, double *d_out, void *stream) {
+  if (term < 2 || term > 5) return fail(-1, "term must be 2 (Sachs-Wolfe), 3 (Doppler), 4 (polarisation) or 5 (E-mode)");
   if (n_k < 1 || n_tau < 1) return fail(-1, "need n_k >= 1 and n_tau >= 1");
   bool ok = d_k && d_vis && d_pol && d_out;
   if (term == 2) ok = ok && d_deltag && d_alphaprime && d_thetabprime && d_polpprime;
   if (term == 3) ok = ok && d_alpha && d_thetab && d_polprime;
+  if (term == 5) ok = ok && d_tau;
   if (!ok) return fail(-1, "NULL device pointer");
   return hip_rc(deb_launch_cmb_source_term(term, d_k, n_k, n_tau, d_vis, d_deltag, d_alpha, d_alphaprime, d_thetab, d_thetabprime, d_pol,
-                                           d_polprime, d_polpprime, d_out, (hipStream_t)stream),
+                                           d_polprime, d_polpprime, d_tau, tau0, d_out, (hipStream_t)stream),
                 "deb_cmb_source_term");
 }
 

@@ -239,13 +239,19 @@ __device__ inline double s3_point(double k, double g1, double alpha, double thet
   return g1 * (alpha + (thetab + 3.0 / 8.0 * polp) / (k * k));
 }
 __device__ inline double s4_point(double k, double g2, double pol) { return g2 * 3.0 / 16.0 * pol / (k * k); }
+// E-mode source (no counterpart in the reference; Zaldarriaga & Seljak 1997 in this normalisation,
+// Pi = F_2 + G_0 + G_2 = 4 x the brightness multipoles): S_E = 3 g Pi / (16 (k chi)^2), chi = tau0 - tau,
+// zero within 1 Mpc of today (where 1 / chi^2 diverges and g is ~1e-10)
+__device__ inline double se_point(double k, double g, double pol, double chi) {
+  return chi >= 1.0 ? g * 3.0 / 16.0 * pol / (k * k * chi * chi) : 0.0;
+}
 
-// outputs (each n_k x n_tau, row-major by k): 0 S1, 1 S2, 2 S3, 3 S4, 4 S, 5 alpha, 6 alpha';
+// outputs (each n_k x n_tau, row-major by k): 0 S1, 1 S2, 2 S3, 3 S4, 4 S, 5 alpha, 6 alpha', 7 S_E;
 // pot: n_tau x n_k (transposed) potential eta + alpha' for the ISW spline
 __global__ void source_point_kernel(const double *__restrict__ t, const double *__restrict__ y, const double *__restrict__ yp,
                                     const double *__restrict__ kk, int n_k, int n_tau, const double *__restrict__ aexp,
                                     const double *__restrict__ V, const double *__restrict__ qw, DebProblem prob, int isw,
-                                    double *__restrict__ out, double *__restrict__ pot) {
+                                    const double *__restrict__ tau, double *__restrict__ out, double *__restrict__ pot) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)n_k * n_tau) return;
   const int j = (int)(idx % n_tau), ik = (int)(idx / n_tau);
@@ -275,6 +281,7 @@ __global__ void source_point_kernel(const double *__restrict__ t, const double *
   out[3 * plane + idx] = S4;
   out[5 * plane + idx] = mt.alpha;
   out[6 * plane + idx] = mt.alphap;
+  out[7 * plane + idx] = se_point(k, g, pl.pol, t[DEB_HDR_TAU0] - tau[j]);
   if (isw == 1) {  // S1 = e^-tau_c (eta' + alpha''), the algebraic form (cmb.py:406-408)
     const double S1 = exp(-V[5L * n_tau + j]) * (mt.etaprime + mt.alphapp);
     out[0 * plane + idx] = S1;
@@ -347,18 +354,21 @@ __global__ void pol_stage_kernel(const double *__restrict__ y, const double *__r
   out[2 * plane + idx] = pl.polpp;
 }
 
-// compute_source_term_sachs_wolfe / _doppler / _polarization (cmb.py:455-563): term 2, 3 or 4
+// compute_source_term_sachs_wolfe / _doppler / _polarization (cmb.py:455-563): term 2, 3 or 4;
+// term 5 the E-mode source
 __global__ void source_term_kernel(int term, const double *__restrict__ kk, int n_k, int n_tau, const double *__restrict__ gv,
                                    const double *__restrict__ deltag, const double *__restrict__ alpha, const double *__restrict__ alphap,
                                    const double *__restrict__ thetab, const double *__restrict__ thetabp, const double *__restrict__ pol,
-                                   const double *__restrict__ polp, const double *__restrict__ polpp, double *__restrict__ out) {
+                                   const double *__restrict__ polp, const double *__restrict__ polpp, const double *__restrict__ tau, double tau0,
+                                   double *__restrict__ out) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)n_k * n_tau) return;
   const int j = (int)(idx % n_tau), ik = (int)(idx / n_tau);
   const double k = kk[ik];
   if (term == 2) out[idx] = s2_point(k, gv[j], deltag[idx], alphap[idx], pol[idx], thetabp[idx], polpp[idx]);
   else if (term == 3) out[idx] = s3_point(k, gv[j], alpha[idx], thetab[idx], polp[idx]);
-  else out[idx] = s4_point(k, gv[j], pol[idx]);
+  else if (term == 4) out[idx] = s4_point(k, gv[j], pol[idx]);
+  else out[idx] = se_point(k, gv[j], pol[idx], tau0 - tau[j]);
 }
 
 // the ISW potential eta + alpha' transposed to n_tau x n_k for source_isw_kernel
@@ -697,7 +707,7 @@ __global__ __launch_bounds__(256) void fftlog_kernel(const double *__restrict__ 
 
 // B[l, 2n] = Re M_l(p_n), B[l, 2n + 1] = Im M_l(p_n), p_n = -1.1 + i eta_n (cmb.py:636-677)
 __global__ void bessel_coef_kernel(int n_ell, int N, const double *__restrict__ tau, int n_tau, double tau0, double *__restrict__ B,
-                                   long Kp) {
+                                   long Kp, int field) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double lmin, dl;
   fft_grid(tau, n_tau, tau0, N, lmin, dl);
@@ -718,6 +728,10 @@ __global__ void bessel_coef_kernel(int n_ell, int N, const double *__restrict__ 
     sincos(lg.y, &s, &c);
     M = make_double2(e * c, e * s);
     if (e == 0.0) M = make_double2(0.0, 0.0);  // the reflection branch overflowed: Gamma -> 0
+    if (field == 1) {  // E-mode: Theta_E,l = sqrt((l + 2)! / (l - 2)!) int S_E j_l (no counterpart in the reference)
+      const double fl = l >= 2 ? sqrt((double)(l + 2) * (l + 1) * l * (l - 1)) : 0.0;
+      M = make_double2(M.x * fl, M.y * fl);
+    }
   }
   B[(long)l * Kp + 2 * nn] = M.x;
   B[(long)l * Kp + 2 * nn + 1] = M.y;
@@ -842,14 +856,14 @@ static ContractGrid contract_grid(int m, int n) {
 }
 
 // C_l = sum_i (ln k_{i+1} - ln k_i) (f_i + f_{i+1}) / 2, f = (k / k_p)^(n_s - 1) Theta_l(k)^2
-__global__ void cell_kernel(const double *__restrict__ th, const double *__restrict__ kk, int n_k, int n_ell, double n_s, double k_p,
-                            double *__restrict__ cl) {
+__global__ void cell_kernel(const double *__restrict__ th, const double *__restrict__ th2, const double *__restrict__ kk, int n_k, int n_ell,
+                            double n_s, double k_p, double *__restrict__ cl) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n_ell) return;
   double acc = 0.0, fprev = 0.0, lkprev = 0.0;
   for (int i = 0; i < n_k; ++i) {
-    const double k = kk[i], t = th[(long)i * n_ell + l];
-    const double f = pow(k / k_p, n_s - 1.0) * t * t, lk = log(k);
+    const double k = kk[i], t = th[(long)i * n_ell + l], t2 = th2[(long)i * n_ell + l];
+    const double f = pow(k / k_p, n_s - 1.0) * t * t2, lk = log(k);
     if (i > 0) acc += (lk - lkprev) * (f + fprev) / 2.0;
     fprev = f;
     lkprev = lk;
@@ -889,7 +903,7 @@ extern "C" int deb_launch_cmb_sources(const double *d_t, int num_thermo, const d
   hipLaunchKernelGGL(deb::vis_eval_kernel, dim3((n_tau + 127) / 128), dim3(128), 0, stream, d_t, vw, d_tau, d_aexp, n_tau, V);
   long pts = (long)n_k * n_tau;
   hipLaunchKernelGGL(deb::source_point_kernel, dim3((unsigned)((pts + 127) / 128)), dim3(128), 0, stream, d_t, d_y, d_yp, d_k, n_k, n_tau,
-                     d_aexp, V, qw, *prob, isw, d_out, pot);
+                     d_aexp, V, qw, *prob, isw, d_tau, d_out, pot);
   if (isw == 0) {
     hipLaunchKernelGGL(deb::spline_cp_kernel, dim3(1), dim3(256), 0, stream, d_tau, n_tau, cp);
     hipLaunchKernelGGL(deb::source_isw_kernel, dim3((n_k + 63) / 64), dim3(64), 0, stream, d_tau, n_tau, n_k, cp, pot, Spot, V, d_out);
@@ -927,10 +941,11 @@ extern "C" int deb_launch_cmb_polarization(const double *d_y, const double *d_yp
 
 extern "C" int deb_launch_cmb_source_term(int term, const double *d_k, int n_k, int n_tau, const double *d_g, const double *d_deltag,
                                           const double *d_alpha, const double *d_alphap, const double *d_thetab, const double *d_thetabp,
-                                          const double *d_pol, const double *d_polp, const double *d_polpp, double *d_out, hipStream_t stream) {
+                                          const double *d_pol, const double *d_polp, const double *d_polpp, const double *d_tau, double tau0,
+                                          double *d_out, hipStream_t stream) {
   const long pts = (long)n_k * n_tau;
   hipLaunchKernelGGL(deb::source_term_kernel, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, stream, term, d_k, n_k, n_tau, d_g, d_deltag,
-                     d_alpha, d_alphap, d_thetab, d_thetabp, d_pol, d_polp, d_polpp, d_out);
+                     d_alpha, d_alphap, d_thetab, d_thetabp, d_pol, d_polp, d_polpp, d_tau, tau0, d_out);
   return (int)hipGetLastError();
 }
 
@@ -963,7 +978,7 @@ extern "C" long deb_los_work_doubles(int n_k, int n_tau, int n_ell, int n_fftlog
 }
 
 extern "C" int deb_launch_los(const double *d_k, int n_k, const double *d_tau, int n_tau, const double *d_S, double tau0, int ellmax,
-                              int n_fftlog, int n_k_dense, double *d_theta, double *d_kout, double *d_work, hipStream_t stream) {
+                              int n_fftlog, int n_k_dense, int field, double *d_theta, double *d_kout, double *d_work, hipStream_t stream) {
   const int n_ell = ellmax + 1;
   const long Kp = los_kp(n_fftlog);
   int logm = 0;
@@ -1044,7 +1059,7 @@ extern "C" int deb_launch_los(const double *d_k, int n_k, const double *d_tau, i
   if (rc) return rc;
   long cells = (long)n_ell * (Kp / 2);
   hipLaunchKernelGGL(deb::bessel_coef_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, stream, n_ell, n_fftlog, d_tau, n_tau,
-                     tau0, B, Kp);
+                     tau0, B, Kp, field);
   const deb::ContractGrid g = deb::contract_grid(nk, n_ell);
   hipLaunchKernelGGL(deb::contract_kernel, dim3(g.blocks), dim3(deb::CT), 0, stream, A, B, d_theta, nk, n_ell, Kp, g.tiles_m, g.tiles_n, g.per_xcd,
                      g.gm);
@@ -1059,9 +1074,10 @@ extern "C" int deb_launch_contract(const double *d_A, const double *d_B, int m, 
   return (int)hipGetLastError();
 }
 
-extern "C" int deb_launch_cell(const double *d_theta, const double *d_k, int n_k, int n_ell, double n_s, double k_p, double *d_cl,
-                               hipStream_t stream) {
-  hipLaunchKernelGGL(deb::cell_kernel, dim3((n_ell + 127) / 128), dim3(128), 0, stream, d_theta, d_k, n_k, n_ell, n_s, k_p, d_cl);
+extern "C" int deb_launch_cell(const double *d_theta, const double *d_theta2, const double *d_k, int n_k, int n_ell, double n_s, double k_p,
+                               double *d_cl, hipStream_t stream) {
+  hipLaunchKernelGGL(deb::cell_kernel, dim3((n_ell + 127) / 128), dim3(128), 0, stream, d_theta, d_theta2 ? d_theta2 : d_theta, d_k, n_k,
+                     n_ell, n_s, k_p, d_cl);
   return (int)hipGetLastError();
 }
 

@@ -202,9 +202,10 @@ long deb_cmb_sources_workspace_doubles(int num_thermo, int n_k, int n_tau);
 
 /* CMB temperature source function from full states d_y and their time derivatives d_yprime
  * (both n_k x n_tau x nvar, as deb_evolve with return_full and deb_time_derivatives write
- * them) at scale factors d_aexp / conformal times d_tau (n_tau, increasing).  d_out: 7 x n_k x
+ * them) at scale factors d_aexp / conformal times d_tau (n_tau, increasing).  d_out: 8 x n_k x
  * n_tau doubles, planes S1 (ISW), S2 (Sachs-Wolfe), S3 (Doppler), S4 (polarisation), S, alpha,
- * alpha'.  isw_mode 0: S1 = e^-tau_c d/dtau(eta + alpha') from a natural spline in tau (the
+ * alpha', S_E (the E-mode source 3 g Pi / (16 (k chi)^2), chi = tau0 - tau, zero for chi < 1 Mpc;
+ * no counterpart in the reference).  isw_mode 0: S1 = e^-tau_c d/dtau(eta + alpha') from a natural spline in tau (the
  * reference's compute_source_term_isw, src/discoeb/cmb.py:437-449); 1: S1 = e^-tau_c (eta' +
  * alpha''), the algebraic form of its docstring (:406-408).  Replaces extract_perturbations ..
  * compute_source_function (src/discoeb/cmb.py:17-633) as chained in
@@ -236,12 +237,13 @@ int deb_cmb_polarization(const double *d_y, const double *d_yprime, int n_k, int
                          double *d_out, void *stream);
 
 /* One source term: term 2 Sachs-Wolfe (d_vis = g; reads deltag, alphaprime, thetabprime, pol,
- * polpprime), 3 Doppler (d_vis = g'; alpha, thetab, polprime), 4 polarisation (d_vis = g''; pol);
- * pointers a term does not read may be NULL.  Replaces compute_source_term_sachs_wolfe, _doppler
- * and _polarization (src/discoeb/cmb.py:455-563). */
+ * polpprime), 3 Doppler (d_vis = g'; alpha, thetab, polprime), 4 polarisation (d_vis = g''; pol),
+ * 5 the E-mode source (d_vis = g; pol, d_tau and tau0; no counterpart in the reference); pointers
+ * a term does not read may be NULL.  Replaces compute_source_term_sachs_wolfe, _doppler and
+ * _polarization (src/discoeb/cmb.py:455-563). */
 int deb_cmb_source_term(int term, const double *d_k, int n_k, int n_tau, const double *d_vis, const double *d_deltag, const double *d_alpha,
                         const double *d_alphaprime, const double *d_thetab, const double *d_thetabprime, const double *d_pol,
-                        const double *d_polprime, const double *d_polpprime, double *d_out, void *stream);
+                        const double *d_polprime, const double *d_polpprime, const double *d_tau, double tau0, double *d_out, void *stream);
 
 /* Doubles of workspace deb_cmb_source_isw needs. */
 long deb_cmb_isw_workspace_doubles(int n_k, int n_tau);
@@ -263,6 +265,12 @@ long deb_los_workspace_doubles(int n_k, int n_tau, int ellmax, int n_fftlog, int
 int deb_los_theta_ell(const double *d_k, int n_k, const double *d_tau, int n_tau, const double *d_S, double tau0, int ellmax, int n_fftlog,
                       int n_k_dense, double *d_theta, double *d_k_out, double *d_work, void *stream);
 
+/* deb_los_theta_ell for field 0 (temperature, as deb_los_theta_ell) or 1 (E-mode polarisation:
+ * Theta_E,l = sqrt((l + 2)! / (l - 2)!) int S_E j_l(k chi) dtau, zero for l < 2; no counterpart in
+ * the reference, which computes TT only). */
+int deb_los_theta_ell_field(const double *d_k, int n_k, const double *d_tau, int n_tau, const double *d_S, double tau0, int ellmax, int n_fftlog,
+                            int n_k_dense, int field, double *d_theta, double *d_k_out, double *d_work, void *stream);
+
 /* C = A B^T in FP64 on the matrix cores: A m x k, B n x k (row-major, k a multiple of 16, both 16-byte aligned), C
  * m x n.  The contraction of compute_theta_ell (src/discoeb/cmb.py:805-833, the einsums over
  * the FFTLog frequencies), exposed on its own. */
@@ -271,6 +279,11 @@ int deb_contract(const double *d_A, const double *d_B, int m, int n, long k, dou
 /* C_l = trapezoid over ln k of (k/k_p)^(n_s-1) Theta_l(k)^2: d_theta n_k x n_ell, d_cell n_ell.
  * Replaces compute_Cell (src/discoeb/cmb.py:838-873). */
 int deb_cell(const double *d_theta, const double *d_k, int n_k, int n_ell, double n_s, double k_p, double *d_cell, void *stream);
+
+/* Cross spectrum: trapezoid over ln k of (k/k_p)^(n_s-1) Theta1_l Theta2_l (C_l^TE from the
+ * temperature and E-mode Theta; no counterpart in the reference). */
+int deb_cell_cross(const double *d_theta1, const double *d_theta2, const double *d_k, int n_k, int n_ell, double n_s, double k_p, double *d_cell,
+                   void *stream);
 
 /* D_l = l (l + 1) C_l A_s 2 T_cmb^2 1e12 for l = 2..ellmax: d_ell, d_dell (ellmax - 1 each).
  * Replaces compute_Dell (src/discoeb/cmb.py:876-927). */

@@ -328,8 +328,24 @@ def fftlog_coefficients(kmodes, tau, S, tau0, n_fftlog):
     return ck, p
 
 
-def compute_theta_ell(ellmax, kmodes, tau, S, tau0, n_fftlog=16384, n_k_dense=0):
-    """Theta_l(k) = int S(k, tau) j_l(k (tau0 - tau)) d tau by FFTLog (cmb.py:680-835)."""
+def source_E(gvis, pol, kmodes, tau, tau0):
+    """E-mode source 3 g Pi / (16 (k chi)^2), chi = tau0 - tau, zero for chi < 1 Mpc (no counterpart
+    in the reference; Zaldarriaga & Seljak 1997 with Pi = F_2 + G_0 + G_2)."""
+    chi = tau0 - np.asarray(tau, dtype=np.float64)
+    k = np.asarray(kmodes, dtype=np.float64)[:, None]
+    safe = np.maximum(chi, 1.0)[None, :]
+    return np.where(chi[None, :] >= 1.0, 3.0 / 16.0 * np.asarray(gvis)[None, :] * pol / (k ** 2 * safe ** 2), 0.0)
+
+
+def spin2_factor(ellmax):
+    """sqrt((l + 2)! / (l - 2)!) for l = 0..ellmax, zero below l = 2."""
+    ell = np.arange(ellmax + 1, dtype=np.float64)
+    return np.sqrt(np.where(ell >= 2, (ell + 2) * (ell + 1) * ell * (ell - 1), 0.0))
+
+
+def compute_theta_ell(ellmax, kmodes, tau, S, tau0, n_fftlog=16384, n_k_dense=0, field="T"):
+    """Theta_l(k) = int S(k, tau) j_l(k (tau0 - tau)) d tau by FFTLog (cmb.py:680-835); field "E"
+    multiplies by spin2_factor (the E-mode projection, no counterpart in the reference)."""
     kmodes = np.asarray(kmodes, dtype=np.float64)
     if n_k_dense > 0:
         kmodes, S = densify_k(kmodes, S, n_k_dense)
@@ -341,13 +357,17 @@ def compute_theta_ell(ellmax, kmodes, tau, S, tau0, n_fftlog=16384, n_k_dense=0)
         th = th + np.real(np.outer(ck[:, -1], M[:, -1]))
     else:
         th = th + 2.0 * (ck[:, 1:].real @ M[:, 1:].real.T - ck[:, 1:].imag @ M[:, 1:].imag.T)
+    if field == "E":
+        th = th * spin2_factor(ellmax)[None, :]
     return th, kmodes
 
 
-def compute_Cell(theta_ell, kmodes, n_s, k_p):
-    """C_l = trapezoid over ln k of (k/k_p)^(n_s-1) Theta_l(k)^2 (cmb.py:838-873)."""
+def compute_Cell(theta_ell, kmodes, n_s, k_p, theta_ell2=None):
+    """C_l = trapezoid over ln k of (k/k_p)^(n_s-1) Theta_l(k)^2 (cmb.py:838-873); with theta_ell2
+    the cross spectrum."""
     kmodes = np.asarray(kmodes, dtype=np.float64)
-    f = (kmodes[:, None] / k_p) ** (n_s - 1) * np.asarray(theta_ell) ** 2
+    t2 = np.asarray(theta_ell if theta_ell2 is None else theta_ell2)
+    f = (kmodes[:, None] / k_p) ** (n_s - 1) * np.asarray(theta_ell) * t2
     lk = np.log(kmodes)
     return np.sum(np.diff(lk)[:, None] * (f[1:] + f[:-1]) / 2.0, axis=0)
 
