@@ -271,10 +271,12 @@ def measure_cmb_chain(deb, nk=3000, ellmax=2500, nfft=16384, k_chunk=500):
     """The CMB workload end to end at its own sizes: flat LCDM with massless neutrinos, photon
     hierarchies to l = 24 (lmaxr 12), 3000 k on geomspace(1e-5, 0.5), 2500 output times on the
     recombination-weighted split, rtol = atol = 1e-4 (the C_l driver's tolerances), perturbations
-    -> time derivatives -> sources streamed over k in chunks of k_chunk modes (full states of one
-    chunk at a time), line of sight for l <= 2500 at n_fftlog 16384, C_l^TT.  Per-mode adaptive
-    steps, and beside them the reference's form: modes in batches on shared steps (8 per batch,
-    since 3000 is not a multiple of the reference's 16)."""
+    -> time derivatives -> sources, line of sight for l <= 2500 at n_fftlog 16384, C_l^TT.  The
+    sources are formed in the solver's output epilogue ("fused": only the source planes are
+    stored) or by the staged chain streamed over k in chunks of k_chunk modes (full states and
+    derivatives of one chunk at a time); per-mode adaptive steps, and the reference's form:
+    modes in batches on shared steps (8 per batch, since 3000 is not a multiple of the
+    reference's 16).  Every form gives the same planes bit for bit on the same solver."""
     import torch
     dev = torch.device("cuda", 0)
     P = dict(workload_param(), Neff=3.046, Nmnu=0.0, mnu=0.0, w_DE_0=-1.0, w_DE_a=0.0)
@@ -284,15 +286,19 @@ def measure_cmb_chain(deb, nk=3000, ellmax=2500, nfft=16384, k_chunk=500):
     res = {"shape": {"k_modes": nk, "n_tau": int(aexp.size), "ellmax": ellmax, "n_fftlog": nfft, "nvar": 99, "k_chunk": k_chunk,
                      "rtol": 1e-4, "atol": 1e-4}}
     deb.evolve_background(param=dict(P), num_thermo=1024)  # warm-up: first use of the background kernel at this size
-    for solver in ("per_mode", "batched"):
+    for solver, form in (("per_mode", "fused"), ("per_mode", "streamed"), ("batched", "fused")):
         torch.cuda.synchronize()
         torch.cuda.reset_peak_memory_stats()
         t = [time.perf_counter()]
         p = deb.evolve_background(param=dict(P), num_thermo=1024)
         torch.cuda.synchronize()
         t.append(time.perf_counter())
-        extra = dict(batch_size=8, k_chunk=504) if solver == "batched" else dict(k_chunk=k_chunk)
-        src = deb.compute_sources_streamed(p, k, aexp, solver=solver, **extra, **lm)
+        if form == "fused":
+            extra = dict(batch_size=8) if solver == "batched" else {}
+            src = deb.compute_sources_fused(p, k, aexp, solver=solver, **extra, **lm)
+        else:
+            extra = dict(batch_size=8, k_chunk=504) if solver == "batched" else dict(k_chunk=k_chunk)
+            src = deb.compute_sources_streamed(p, k, aexp, solver=solver, **extra, **lm)
         torch.cuda.synchronize()
         t.append(time.perf_counter())
         th, kk = deb.compute_theta_ell(ellmax, k, src["tau"], src["S"], float(p["tau0"]), n_fftlog=nfft)
@@ -300,7 +306,7 @@ def measure_cmb_chain(deb, nk=3000, ellmax=2500, nfft=16384, k_chunk=500):
         torch.cuda.synchronize()
         t.append(time.perf_counter())
         total = t[-1] - t[0]
-        res[solver] = {"s": total, "background_s": t[1] - t[0], "perturbations_sources_s": t[2] - t[1], "los_cell_s": t[3] - t[2],
+        res[f"{solver}_{form}"] = {"s": total, "background_s": t[1] - t[0], "perturbations_sources_s": t[2] - t[1], "los_cell_s": t[3] - t[2],
                        "k_solves_per_s": nk / (t[2] - t[1]), "accepted_steps_mean": float(src["n_accept"].double().mean()),
                        "peak_device_bytes": int(torch.cuda.max_memory_allocated()), "cell_finite": bool(torch.isfinite(cl[2:]).all()),
                        **extra}

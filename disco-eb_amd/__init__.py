@@ -16,12 +16,13 @@ from .cmb import (cmb_aexp_out, compute_Cell, compute_Cell_spectrum_from_cosmo_p
                   compute_neutrino_perturbations, compute_polarization_terms, compute_source_function, compute_source_function_from_states,
                   compute_source_term_doppler, compute_source_term_E, compute_source_term_isw, compute_source_term_polarization,
                   compute_source_term_sachs_wolfe, compute_Cell_spectra_from_cosmo_params,
-                  compute_sources_streamed, compute_theta_ell, compute_visibility_functions, extract_perturbations)
+                  compute_sources_fused, compute_sources_streamed, compute_theta_ell, compute_visibility_functions,
+                  extract_perturbations)
 
 __all__ = ["evolve_background", "background_tables", "attach_background", "compute_background_quantities", "DeviceSpline",
            "evolve_perturbations", "evolve_perturbations_batched", "evolve_units", "evolve_units_batched", "get_power", "compute_time_derivatives", "k_grid", "nvar", "evolve_many",
            "power_units", "tau_out_all", "autotune_schedule", "PkGraph", "extract_perturbations", "compute_visibility_functions",
-           "compute_source_function_from_states", "compute_sources_streamed", "compute_theta_ell", "compute_neutrino_perturbations",
+           "compute_source_function_from_states", "compute_sources_fused", "compute_sources_streamed", "compute_theta_ell", "compute_neutrino_perturbations",
            "compute_metric_perturbations", "compute_polarization_terms", "compute_source_term_isw", "compute_source_term_sachs_wolfe",
            "compute_source_term_doppler", "compute_source_term_polarization", "compute_source_function", "compute_source_term_E",
            "compute_Cell_spectra_from_cosmo_params", "compute_Cell", "compute_Dell", "cmb_aexp_out",

@@ -50,6 +50,7 @@ EXPORTS = ["deb_version", "deb_last_error", "deb_tables_size", "deb_table_offset
            "deb_evolve_adjoint", "deb_evolve_batched_workspace_doubles",
            "deb_evolve_batched", "deb_noalloc_scratch_doubles", "deb_evolve_batched_noalloc", "deb_tau_out", "deb_nvar", "deb_power", "deb_power_units",
            "deb_time_derivatives", "deb_visibility", "deb_cmb_sources_workspace_doubles", "deb_cmb_sources",
+           "deb_evolve_sources_workspace_doubles", "deb_evolve_sources",
            "deb_cmb_neutrinos", "deb_cmb_metric", "deb_cmb_polarization", "deb_cmb_source_term", "deb_cmb_isw_workspace_doubles",
            "deb_cmb_source_isw", "deb_los_workspace_doubles", "deb_los_theta_ell", "deb_los_theta_ell_field", "deb_cell_cross", "deb_contract", "deb_cell", "deb_dell", "deb_savgol_coefficients",
            "deb_savgol_filter", "deb_power_smoothed_workspace_doubles", "deb_power_smoothed", "deb_power_kaiser", "deb_power_multipoles",
@@ -148,6 +149,11 @@ def lib():
     L.deb_cmb_sources_workspace_doubles.argtypes = [ctypes.c_int] * 3
     L.deb_cmb_sources.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                   ctypes.POINTER(DebProblem), ctypes.c_int, _vp, _vp, _vp]
+    L.deb_evolve_sources_workspace_doubles.restype = ctypes.c_long
+    L.deb_evolve_sources_workspace_doubles.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(DebProblem)]
+    L.deb_evolve_sources.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                     ctypes.POINTER(DebProblem), ctypes.POINTER(DebSolverCfg), ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                                     ctypes.POINTER(DebSchedule), _vp]
     L.deb_los_workspace_doubles.restype = ctypes.c_long
     L.deb_los_workspace_doubles.argtypes = [ctypes.c_int] * 5
     L.deb_los_theta_ell.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,

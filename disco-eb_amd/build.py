@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdiscoeb_amd.so")
-SOURCES = ["deb_perturb.hip", "deb_adjoint.hip", "deb_background.hip", "deb_misc.hip", "deb_cmb.hip", "deb_lss.hip", "deb_noalloc.hip", "deb_capi.cpp"]
+SOURCES = ["deb_perturb.hip", "deb_adjoint.hip", "deb_background.hip", "deb_misc.hip", "deb_cmb.hip", "deb_lss.hip", "deb_noalloc.hip", "deb_sources.hip", "deb_capi.cpp"]
 ARCH = os.environ.get("DEB_OFFLOAD_ARCH", "gfx950")
 
 
@@ -45,7 +45,7 @@ def build(verbose=False):
             print(r.stderr[-3000:], file=sys.stderr)
         return out
 
-    with ThreadPoolExecutor(max_workers=4) as ex:
+    with ThreadPoolExecutor(max_workers=5) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]

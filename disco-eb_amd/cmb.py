@@ -307,6 +307,55 @@ def cmb_aexp_out(n_early=32, n_pre_recomb=64, n_recomb=128, n_post=96):
                            _geomspace(a2, a3, n_recomb, endpoint=False), _geomspace(a3, 1.0, n_post)])
 
 
+def compute_sources_fused(param, kmodes, aexp_out, *, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4,
+                          max_steps=2048, solver="per_mode", batch_size=16, isw="spline", threads=0):
+    """The source function S(k, tau) of the chain perturbations -> time derivatives -> sources
+    (cmb.py:1030-1073) in one launch (deb_evolve_sources): each mode's solve forms the source
+    terms at its output times from the projected state and its right-hand side, so only the
+    (n_k, n_tau) planes are stored, never the states (n_k x n_tau x nvar) or their derivatives.
+    Equal to compute_sources_streamed's unfused chain on the same solver.  solver: "per_mode"
+    (largest k first) or "batched" (batch_size modes on shared steps; num_k a multiple of it);
+    threads: threads per mode or batch (0: 64 per mode, 128 per batch -- with thousands of light
+    modes, each spending most of its time on the 2500 output evaluations, more modes in flight
+    beat wider workgroups: 0.15 s per mode at 64 threads against 0.35 s at 256 for the 3000-mode
+    workload).  param: from evolve_background.
+    Returns {"S", "S1".."S4", "SE", "alpha", "alphaprime": (n_k, n_tau), "tau": tau_out,
+    "n_accept", "n_reject": (n_k,)}."""
+    from .perturbations import DEFAULT_SCHEDULE, _raise_on_status, _schedule
+    if solver not in ("per_mode", "batched"):
+        raise ValueError("solver must be 'per_mode' or 'batched'")
+    if isw not in ("spline", "algebraic"):
+        raise ValueError("isw must be 'spline' or 'algebraic'")
+    t, N, c = _tables(param)
+    dev = t.device
+    kk = torch.as_tensor(kmodes, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+    aexp = torch.as_tensor(np.asarray(aexp_out, dtype=np.float64) if not torch.is_tensor(aexp_out) else aexp_out, dtype=torch.float64,
+                           device=dev).reshape(-1).contiguous()
+    n_k, n_tau = kk.numel(), aexp.numel()
+    bs = 0 if solver == "per_mode" else int(batch_size)
+    if bs and n_k % bs:
+        raise ValueError("with solver='batched', num_k must be a multiple of batch_size")
+    tau_out = param["tau_of_a_spline"].evaluate(aexp).contiguous()
+    L = _lib.lib()
+    prob = _lib.DebProblem(int(lmaxg), int(lmaxgp), int(lmaxr), int(lmaxnu), int(nqmax), 1)
+    cfg = _lib.DebSolverCfg(float(rtol), float(atol), 0.25, 0.80, 0.0, 20.0, 0.3, int(max_steps))
+    out = torch.empty((8, n_k, n_tau), dtype=torch.float64, device=dev)
+    status = torch.empty(n_k, dtype=torch.int32, device=dev)
+    nacc = torch.empty(n_k, dtype=torch.int32, device=dev)
+    nrej = torch.empty(n_k, dtype=torch.int32, device=dev)
+    work = torch.empty(L.deb_evolve_sources_workspace_doubles(N, n_k, n_tau, bs, ctypes.byref(prob)), dtype=torch.float64, device=dev)
+    threads = int(threads) or (64 if bs == 0 else 128)
+    sched_dict = dict(DEFAULT_SCHEDULE, threads=threads) if bs == 0 else {"threads": threads}
+    sched, order = _schedule(sched_dict, kk)
+    _lib.check(L.deb_evolve_sources(_lib.ptr(t), N, c, _lib.ptr(kk), n_k, bs, _lib.ptr(aexp), _lib.ptr(tau_out), n_tau, ctypes.byref(prob),
+                                    ctypes.byref(cfg), 1 if isw == "algebraic" else 0, _lib.ptr(out), _lib.ptr(status), _lib.ptr(nacc),
+                                    _lib.ptr(nrej), _lib.ptr(work), ctypes.byref(sched) if sched is not None else None, _lib.stream()))
+    del order, work  # stream-ordered: reused by the allocator only after the launches have run
+    _raise_on_status(status)
+    return {"S1": out[0], "S2": out[1], "S3": out[2], "S4": out[3], "S": out[4], "alpha": out[5], "alphaprime": out[6], "SE": out[7],
+            "tau": tau_out, "n_accept": nacc, "n_reject": nrej}
+
+
 def compute_sources_streamed(param, kmodes, aexp_out, *, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4,
                              max_steps=2048, solver="per_mode", batch_size=16, k_chunk=512, isw="spline"):
     """The source function S(k, tau) of the chain perturbations -> time derivatives -> sources
@@ -362,7 +411,7 @@ def compute_sources_streamed(param, kmodes, aexp_out, *, lmaxg=11, lmaxgp=11, lm
 
 def compute_Cell_spectrum_from_cosmo_params(param_dict, ellmax=2500, nmodes=128, kmin=1e-4, kmax=1.0, n_k_dense=8192, n_fftlog=16384,
                                             lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, *, k_sampling_method="camb",
-                                            solver="batched", k_chunk=None, aexp_out=None, rtol=1e-4, atol=1e-4):
+                                            solver="batched", k_chunk=None, aexp_out=None, rtol=1e-4, atol=1e-4, fused=True):
     """TT C_l from cosmological parameters (cmb.py:930-1090): background (num_thermo 1024),
     perturbations on the camb k-sampling at rtol = atol = 1e-4 with full states, integrated in
     batches of 16 modes on one shared step sequence as the reference does (nmodes a multiple of
@@ -371,12 +420,22 @@ def compute_Cell_spectrum_from_cosmo_params(param_dict, ellmax=2500, nmodes=128,
     Keyword-only extensions (the defaults are the reference's behaviour): k_sampling_method;
     solver "per_mode" for independent adaptive steps per mode; k_chunk to stream the chain over
     k (compute_sources_streamed) so that full states are held for k_chunk modes at a time;
-    aexp_out (default cmb_aexp_out()); rtol, atol."""
+    fused (default) forms the sources in the solver's output epilogue (compute_sources_fused: no
+    states stored, the same planes bit for bit; fused=False runs the staged chain, streamed when
+    k_chunk is given); aexp_out (default cmb_aexp_out()); rtol, atol."""
     param = evolve_background(param=dict(param_dict), thermo_module="RECFAST", num_thermo=1024)
     aexp_out = cmb_aexp_out() if aexp_out is None else np.asarray(aexp_out, dtype=np.float64)
     lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
     tau0 = float(param["tau_of_a_spline"].evaluate(np.array([1.0])).cpu()[0])
-    if solver == "batched" and k_chunk is None:
+    if fused:
+        from .perturbations import k_grid
+        kmodes = torch.as_tensor(k_grid(param, kmin, kmax, nmodes, k_sampling_method), dtype=torch.float64,
+                                 device=param["_deb_tables"].device)
+        src = compute_sources_fused(param, kmodes, aexp_out, rtol=rtol, atol=atol, solver=solver, **lm)
+        tau, S = src["tau"], src["S"]
+        param.update(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax, nout=int(tau.numel()), tau_out=tau,
+                     _deb_steps=(src["n_accept"], src["n_reject"]))
+    elif solver == "batched" and k_chunk is None:
         yout, kmodes, param = evolve_perturbations_batched(param=param, aexp_out=aexp_out, kmin=kmin, kmax=kmax, num_k=nmodes, rtol=rtol,
                                                            atol=atol, return_full=True, k_sampling_method=k_sampling_method, **lm)
         tau = param["tau_out"]
@@ -396,7 +455,8 @@ def compute_Cell_spectrum_from_cosmo_params(param_dict, ellmax=2500, nmodes=128,
 
 def compute_Cell_spectra_from_cosmo_params(param_dict, spectra=("TT", "TE", "EE"), ellmax=2500, nmodes=128, kmin=1e-4, kmax=1.0,
                                             n_k_dense=8192, n_fftlog=16384, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, *,
-                                            k_sampling_method="camb", solver="batched", k_chunk=None, aexp_out=None, rtol=1e-4, atol=1e-4):
+                                            k_sampling_method="camb", solver="batched", k_chunk=None, aexp_out=None, rtol=1e-4, atol=1e-4,
+                                            fused=True):
     """C_l^TT, C_l^TE and C_l^EE from cosmological parameters: the chain of
     compute_Cell_spectrum_from_cosmo_params (cmb.py:930-1090) with the E-mode source projected
     beside the temperature source (an extension: the reference computes TT only; no reionization,
@@ -410,7 +470,10 @@ def compute_Cell_spectra_from_cosmo_params(param_dict, spectra=("TT", "TE", "EE"
     lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
     tau0 = float(param["tau_of_a_spline"].evaluate(np.array([1.0])).cpu()[0])
     kmodes = torch.as_tensor(k_grid(param, kmin, kmax, nmodes, k_sampling_method), dtype=torch.float64, device=param["_deb_tables"].device)
-    src = compute_sources_streamed(param, kmodes, aexp_out, rtol=rtol, atol=atol, solver=solver, k_chunk=k_chunk, **lm)
+    if fused:
+        src = compute_sources_fused(param, kmodes, aexp_out, rtol=rtol, atol=atol, solver=solver, **lm)
+    else:
+        src = compute_sources_streamed(param, kmodes, aexp_out, rtol=rtol, atol=atol, solver=solver, k_chunk=k_chunk, **lm)
     param.update(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax, nout=int(src["tau"].numel()), tau_out=src["tau"])
     th = {}
     if "TT" in spectra or "TE" in spectra:
@@ -429,4 +492,4 @@ __all__ = ["extract_perturbations", "compute_visibility_functions", "compute_neu
            "compute_polarization_terms", "compute_source_term_isw", "compute_source_term_sachs_wolfe", "compute_source_term_doppler",
            "compute_source_term_polarization", "compute_source_function", "compute_source_function_from_states", "compute_theta_ell",
            "compute_Cell", "compute_Dell", "cmb_aexp_out", "compute_sources_streamed", "compute_Cell_spectrum_from_cosmo_params",
-           "compute_source_term_E", "compute_Cell_spectra_from_cosmo_params"]
+           "compute_source_term_E", "compute_Cell_spectra_from_cosmo_params", "compute_sources_fused"]

@@ -31,6 +31,9 @@ extern "C" int deb_launch_perturb(const double *, int, const int *, const double
                                   const DebProblem *, const DebSolverCfg *, double *, double *, int *, int *, int *, const int *, int,
                                   const double *, long, const int *, hipStream_t);
 extern "C" long deb_perturb_lds_bytes(int, int, int, int, int);
+extern "C" long deb_evolve_sources_work_doubles(int, int, int, int, int);
+extern "C" int deb_launch_evolve_sources(const double *, int, const double *, int, int, const double *, const double *, int, const DebProblem *,
+                                         const DebSolverCfg *, int, double *, int *, int *, int *, double *, const int *, int, hipStream_t);
 extern "C" long deb_adjoint_lds_bytes(const DebProblem *, int);
 extern "C" int deb_launch_adjoint(const double *, int, const double *, double *, const double *, int, const int *, const double *, int,
                                   const double *, const double *, int, const double *, const DebProblem *, const double *, long, const int *,
@@ -296,6 +299,33 @@ int deb_cmb_sources(const double *d_tables, int num_thermo, int cosmo, const dou
   return hip_rc(deb_launch_cmb_sources(t, num_thermo, d_y, d_yprime, d_k, n_k, d_aexp, d_tau, n_tau, prob, isw_mode, d_out, d_work,
                                        (hipStream_t)stream),
                 "deb_cmb_sources");
+}
+
+long deb_evolve_sources_workspace_doubles(int num_thermo, int n_k, int n_tau, int batch_size, const DebProblem *prob) {
+  if (!prob || n_k < 0 || n_tau < 0 || batch_size < 0) return -1;
+  return deb_evolve_sources_work_doubles(num_thermo, n_k, n_tau, batch_size,
+                                         deb_nvar(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax));
+}
+
+int deb_evolve_sources(const double *d_tables, int num_thermo, int cosmo, const double *d_k, int n_k, int batch_size, const double *d_aexp,
+                       const double *d_tau, int n_tau, const DebProblem *prob, const DebSolverCfg *cfg, int isw_mode, double *d_out, int *d_status,
+                       int *d_n_accept, int *d_n_reject, double *d_work, const DebSchedule *sched, void *stream) {
+  if (isw_mode != 0 && isw_mode != 1) return fail(-1, "isw_mode must be 0 (spline derivative) or 1 (algebraic)");
+  if (n_tau < 3) return fail(-1, "need n_tau >= 3");
+  if (cosmo < 0) return fail(-1, "cosmology index must be >= 0");
+  if (batch_size < 0) return fail(-1, "batch_size must be >= 0");
+  if (batch_size > 0 && n_k % batch_size != 0) return fail(-1, "with batched steps n_k must be a multiple of batch_size");
+  if (batch_size > 0 && sched && sched->d_order) return fail(-1, "batched steps take no launch order");
+  int rc = check_solve(d_tables, &cosmo, d_k, n_k, d_tau, n_tau, prob, cfg, d_out, d_status, d_n_accept, d_n_reject, d_work);
+  if (rc || n_k == 0) return rc;
+  if (!d_aexp) return fail(-1, "NULL device pointer");
+  const int nt = sched ? sched->threads_per_unit : 0;
+  if (nt != 0 && nt != 64 && nt != 128 && nt != 256) return fail(-1, "threads_per_unit must be 0, 64, 128 or 256");
+  const double *t = d_tables + (long)cosmo * deb_table_doubles(num_thermo);
+  rc = deb_launch_evolve_sources(t, num_thermo, d_k, n_k, batch_size, d_aexp, d_tau, n_tau, prob, cfg, isw_mode, d_out, d_status, d_n_accept,
+                                 d_n_reject, d_work, sched ? sched->d_order : nullptr, nt, (hipStream_t)stream);
+  if (rc == -2) return fail(-1, "state too large for LDS");
+  return hip_rc(rc, "deb_evolve_sources");
 }
 
 int deb_visibility(const double *d_tables, int num_thermo, int cosmo, const double *d_tau, const double *d_aexp, int n_tau, double *d_out,

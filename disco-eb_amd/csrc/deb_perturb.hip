@@ -91,15 +91,7 @@ __global__ __launch_bounds__(NT) void perturb_kernel(const double *__restrict__ 
 }
 
 
-// Batched shared-step evolution (evolve_modes_batched, perturbations.py:1002-1120, with
-// Rodas5Batched): workgroup g integrates the `batch` consecutive units g * batch .. (g + 1) *
-// batch - 1, which must belong to one cosmology, on ONE step sequence: a common start time
-// (the smallest of the units' own), one step size, and one accept decision from the RMS of the
-// filtered errors over all batch x 6 entries.  Each attempt walks the units in turn through the
-// per-mode Rodas5 stages in LDS; their states live in `state` (2 x nvar doubles per unit, the
-// current state and the candidate, swapped on accept), which stays in L2 between visits.
-// step_log, when given, receives (t_prev, t_next, error norm) of every attempt of batch g at
-// step_log[(g * max_steps + step) * 3].  Status 5: the batch's units have different cosmologies.
+// Batched shared-step evolution: one workgroup per batch (solve_batch).
 template <int NQ, int NT>
 __global__ __launch_bounds__(NT) void perturb_batched_kernel(const double *__restrict__ tables, long tstride,
                                                                const int *__restrict__ unit_cosmo, const double *__restrict__ unit_k,
@@ -108,123 +100,8 @@ __global__ __launch_bounds__(NT) void perturb_batched_kernel(const double *__res
                                                                long lds_bytes, double *__restrict__ step_log, double *__restrict__ y_out,
                                                                int *__restrict__ status, int *__restrict__ n_acc, int *__restrict__ n_rej,
                                                                double *__restrict__ state) {
-  const int g = blockIdx.x, tid = threadIdx.x;
-  const int u0 = g * batch;
-  if (u0 + batch > n_units) return;
-  PLay L;
-  Ws W;
-  if (!setup_solver<NQ, NT>(prob, nu_qw, lds_bytes, L, W, tid)) {
-    for (int j = tid; j < batch; j += NT) { status[u0 + j] = 3; n_acc[u0 + j] = 0; n_rej[u0 + j] = 0; }
-    return;
-  }
-  const int n = L.n;
-  const int cosmo = unit_cosmo[u0];
-  __shared__ BgView s_bg;  // in LDS, as in solve_unit
-  if (tid == 0) { W.sc->i = -1; W.sc->j = -1; s_bg.init(tables + (long)cosmo * tstride); }
-  __syncthreads();
-  const BgView &b = s_bg;
-  const int wout = prob.return_full ? n : 20;
-  double *yo = y_out + (long)u0 * nout * wout;  // unit j's outputs at yo + j nout wout
-  int mixed = 0;
-  for (int j = tid; j < batch; j += NT) mixed |= (unit_cosmo[u0 + j] != cosmo);
-  mixed = __syncthreads_or(mixed);
-  const bool phantom = !b.lambda_de && (1.0 + b.w0) * (1.0 + b.w0 + b.wa) <= 0.0;
-  if (mixed || phantom) {  // not integrated: NaN outputs (w crossing -1 as in the per-mode kernel)
-    for (long i = tid; i < (long)batch * nout * wout; i += NT) yo[i] = __builtin_nan("");
-    for (int j = tid; j < batch; j += NT) { status[u0 + j] = mixed ? 5 : 4; n_acc[u0 + j] = 0; n_rej[u0 + j] = 0; }
-    return;
-  }
-
-  const double *tout = tau_out + (long)cosmo * nout;
-  double tmin = tout[0], t1 = tout[0];
-  for (int j = 1; j < nout; ++j) { tmin = fmin(tmin, tout[j]); t1 = fmax(t1, tout[j]); }
-  // determine_start_times_per_batch (perturbations.py:954-999): the earliest start of the batch
-  double t0 = 1e300;
-  for (int j = 0; j < batch; ++j) t0 = fmin(t0, 0.99 * fmin(tmin, starting_time(b, unit_k[u0 + j])));
-  double *st0 = state + (long)u0 * 2 * n;  // unit j: st0 + 2 n j + {cur, 1 - cur} n
-  for (int j = 0; j < batch; ++j) {
-    if (tid == 0) initial_conditions(b, L, t0, unit_k[u0 + j], W.y0);
-    __syncthreads();
-    for (int i = tid; i < n; i += NT) st0[2L * n * j + i] = W.y0[i];
-    __syncthreads();
-  }
-
-  double tprev = t0;
-  double tnext = fmin(t0 + fmin(t0 / 4.0, 0.5 * (t1 - t0)), t1);
-  Pid pid;
-  pid.prev_inv = 1.0;
-  pid.prev_prev_inv = 1.0;
-  int isave = 0, steps = 0, na = 0, nr = 0, st = 0, cur = 0;
-  while (tprev < t1) {
-    if (steps >= cfg.max_steps) { st = 1; break; }
-    const double dt = tnext - tprev;
-    double acc = 0.0, acc_nan = 0.0;  // scaled with max(|y0|, |y1|), and with |y0| alone
-    int nanany = 0;
-    for (int j = 0; j < batch; ++j) {
-      const double k = unit_k[u0 + j];
-      const double *ys = st0 + 2L * n * j + (long)cur * n;
-      double *yd = st0 + 2L * n * j + (long)(1 - cur) * n;
-      for (int i = tid; i < n; i += NT) W.y0[i] = ys[i];
-      __syncthreads();
-      const int nanflag = rodas5_attempt<NQ, NT>(b, L, k, tprev, dt, W, tid);
-      for (int i = tid; i < n; i += NT) yd[i] = W.y1[i];
-      acc += filtered_sq(k, W.y0, W.y1, W.K + 7 * n, 0, cfg);
-      acc_nan += filtered_sq(k, W.y0, W.y1, W.K + 7 * n, 1, cfg);
-      nanany |= nanflag;
-      __syncthreads();
-    }
-    const double en = sqrt((nanany ? acc_nan : acc) / (6.0 * batch));
-    if (step_log != nullptr && tid == 0) {
-      double *lg = step_log + ((long)g * cfg.max_steps + steps) * 3;
-      lg[0] = tprev;
-      lg[1] = tnext;
-      lg[2] = en;
-    }
-    if (!isfinite(en)) { st = 2; break; }
-    const bool keep = en < 1.0;
-    const double factor = pid.adapt(en, keep, cfg);
-    if (keep) {
-      if (isave < nout && tout[isave] <= tnext) {  // outputs in (tprev, tnext]: interpolate every unit
-        int isv = isave;
-        for (int j = 0; j < batch; ++j) {
-          const double k = unit_k[u0 + j];
-          const double *ya = st0 + 2L * n * j + (long)cur * n;
-          const double *yb = st0 + 2L * n * j + (long)(1 - cur) * n;
-          for (int i = tid; i < n; i += NT) { W.y0[i] = ya[i]; W.y1[i] = yb[i]; }
-          __syncthreads();
-          isv = isave;
-          while (isv < nout && tout[isv] <= tnext) {
-            double th = (tout[isv] - tprev) / (tnext - tprev);
-            for (int i = tid; i < n; i += NT) W.u[i] = W.y0[i] + th * (W.y1[i] - W.y0[i]);
-            __syncthreads();
-            emit_output<NT>(b, L, k, tout[isv], W.u, yo + ((long)j * nout + isv) * wout, prob.return_full, tid);
-            __syncthreads();
-            ++isv;
-          }
-        }
-        isave = isv;
-      }
-      cur = 1 - cur;
-      ++na;
-    } else {
-      ++nr;
-    }
-    double nt0 = keep ? tnext : tprev;
-    tprev = fmin(nt0, t1);
-    double tn = nt0 + dt * factor;
-    tnext = (tn > t1 - 1e-10) ? t1 : tn;
-    ++steps;
-  }
-  if (st != 0) {
-    for (int j = 0; j < batch; ++j)
-      for (int r = isave; r < nout; ++r)
-        for (int i = tid; i < wout; i += NT) yo[((long)j * nout + r) * wout + i] = __builtin_nan("");
-  }
-  for (int j = tid; j < batch; j += NT) {
-    status[u0 + j] = st;
-    n_acc[u0 + j] = na;
-    n_rej[u0 + j] = nr;
-  }
+  solve_batch<NQ, NT>(tables, tstride, unit_cosmo, unit_k, n_units, batch, tau_out, nout, nu_qw, prob, cfg, lds_bytes, step_log, y_out, status,
+                      n_acc, n_rej, state);
 }
 
 

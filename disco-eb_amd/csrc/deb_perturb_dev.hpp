@@ -32,6 +32,7 @@
 
 #include "deb_cosmo.hpp"
 #include "deb_perturb.h"
+#include "deb_source_pt.hpp"
 
 namespace deb {
 namespace {
@@ -1273,8 +1274,9 @@ __device__ __forceinline__ void emit_output(const BgView &b, const PLay &L, doub
       __syncthreads();
     }
   }
-  if (full) {
-    for (int j = tid; j < n; j += NT) dst[j] = y[j];
+  if (full) {  // dst NULL: the projection in place only
+    if (dst != nullptr)
+      for (int j = tid; j < n; j += NT) dst[j] = y[j];
     return;
   }
   // 20 output fields
@@ -1337,6 +1339,58 @@ __device__ __forceinline__ void emit_output(const BgView &b, const PLay &L, doub
       default: v = thetaq / aH; break;
     }
     dst[tid] = v;
+  }
+}
+
+// Source epilogue: the chain perturbations -> time derivatives -> source function
+// (compute_time_derivatives, perturbations.py:1339-1406, and cmb.py:17-633 as chained in
+// cmb.py:1030-1073) applied to each output as it is emitted, so full states are never stored.
+// V: the visibility table of the unit's cosmology at the output times (NV rows of nout);
+// planes: 8 planes of n_units x nout, as deb_cmb_sources writes them (S1, S2, S3, S4, S, alpha,
+// alpha', S_E); pot: nout x n_units, the ISW potential eta + alpha' (isw 0: S1 and S come later
+// from its spline in tau); isw 1 writes S1 = e^-tau_c (eta' + alpha'') and S.
+struct SrcOut {
+  const double *V, *aexp, *qw;
+  double *planes, *pot;
+  int isw, n_units;
+};
+
+// The RSA projection of the interpolated state u (LDS, in place), its right-hand side (into W.f,
+// free between steps) and the source point on thread 0, written for (unit, output j).
+template <int NT>
+__device__ __forceinline__ void emit_sources(const BgView &b, const PLay &L, double k, double tau, double *u, int unit, int j, int nout,
+                                             const SrcOut &so, Ws &W, int tid) {
+  emit_output<NT>(b, L, k, tau, u, nullptr, 1, tid);
+  __syncthreads();
+  rhs_eval<NT>(b, L, k, tau, u, W.f, W, tid);
+  __syncthreads();
+  if (tid == 0) {
+    const SrcPt s = source_point(b, u, W.f, L.n, L.idxg, L.idxgp, L.idxr, L.iq0, L.nq, so.qw, k, so.aexp[j], so.V + j, nout,
+                                 b.t[DEB_HDR_TAU0] - tau);
+    const long plane = (long)so.n_units * nout, o = (long)unit * nout + j;
+    so.planes[1 * plane + o] = s.S2;
+    so.planes[2 * plane + o] = s.S3;
+    so.planes[3 * plane + o] = s.S4;
+    so.planes[5 * plane + o] = s.alpha;
+    so.planes[6 * plane + o] = s.alphap;
+    so.planes[7 * plane + o] = s.SE;
+    if (so.isw == 1) {
+      so.planes[0 * plane + o] = s.S1;
+      so.planes[4 * plane + o] = s.S1 + s.S2 + s.S3 + s.S4;
+    } else {
+      so.pot[(long)j * so.n_units + unit] = s.pot;
+    }
+  }
+}
+
+// NaN source outputs of unit for the output times j0 .. nout - 1 (a unit not integrated, or
+// stopped early)
+template <int NT>
+__device__ __forceinline__ void nan_sources(int unit, int j0, int nout, const SrcOut &so, int tid) {
+  const long plane = (long)so.n_units * nout;
+  for (int j = j0 + tid; j < nout; j += NT) {
+    for (int p = 0; p < 8; ++p) so.planes[p * plane + (long)unit * nout + j] = __builtin_nan("");
+    if (so.isw != 1) so.pot[(long)j * so.n_units + unit] = __builtin_nan("");
   }
 }
 
@@ -1511,13 +1565,15 @@ __device__ __forceinline__ double filtered_sq(double k, const double *y0, const 
 
 // One (cosmology, k) unit from its initial conditions to the last output time.  The LDS
 // workspace W and the layout L (with its momentum-bin tables) are set up by the caller.
-template <int NQ, int NT>
+// SRC: the source epilogue (so) replaces the output conversion; y_out is not written.
+template <int NQ, int NT, bool SRC = false>
 __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
                                            const double *__restrict__ unit_k, const double *__restrict__ tau_out, int nout,
                                            const DebProblem &prob, const DebSolverCfg &cfg, double *__restrict__ step_log,
                                            double *__restrict__ y_out, int *__restrict__ status, int *__restrict__ n_acc,
                                            int *__restrict__ n_rej, const double *__restrict__ replay, long replay_stride,
-                                           const int *__restrict__ replay_len, const PLay &L, Ws &W, int tid) {
+                                           const int *__restrict__ replay_len, const PLay &L, Ws &W, int tid,
+                                           const SrcOut *so = nullptr) {
   const int cosmo = unit_cosmo[unit];
   const double k = unit_k[unit];
   const int n = L.n;
@@ -1529,12 +1585,14 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   __syncthreads();
   const BgView &b = s_bg;
   const int wout = prob.return_full ? n : 20;
-  double *yo = y_out + (long)unit * nout * wout;
+  double *yo = SRC ? nullptr : y_out + (long)unit * nout * wout;
   // w(a) = w0 + wa (1 - a) reaching -1 for some a in (0, 1] (the phantom divide) makes the
   // fluid equations singular (their 1 / (1 + w) terms): such units are not integrated
   // (status 4, NaN outputs).  w0 = -1, wa = 0 is the cosmological constant, handled exactly.
   if (!b.lambda_de && (1.0 + b.w0) * (1.0 + b.w0 + b.wa) <= 0.0) {
-    for (int i = tid; i < nout * wout; i += NT) yo[i] = __builtin_nan("");
+    if constexpr (SRC) nan_sources<NT>(unit, 0, nout, *so, tid);
+    else
+      for (int i = tid; i < nout * wout; i += NT) yo[i] = __builtin_nan("");
     if (tid == 0) { status[unit] = 4; n_acc[unit] = 0; n_rej[unit] = 0; }
     return;
   }
@@ -1582,7 +1640,8 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
         double th = (tout[isave] - tprev) / (tnext - tprev);
         for (int i = tid; i < n; i += NT) u[i] = y0[i] + th * (W.y1[i] - y0[i]);
         __syncthreads();
-        emit_output<NT>(b, L, k, tout[isave], u, yo + (long)isave * wout, prob.return_full, tid);
+        if constexpr (SRC) emit_sources<NT>(b, L, k, tout[isave], u, unit, isave, nout, *so, W, tid);
+        else emit_output<NT>(b, L, k, tout[isave], u, yo + (long)isave * wout, prob.return_full, tid);
         __syncthreads();
         ++isave;
       }
@@ -1605,8 +1664,10 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
     ++steps;
   }
   if (st != 0) {
-    for (int j = isave; j < nout; ++j)
-      for (int i = tid; i < wout; i += NT) yo[(long)j * wout + i] = __builtin_nan("");
+    if constexpr (SRC) nan_sources<NT>(unit, isave, nout, *so, tid);
+    else
+      for (int j = isave; j < nout; ++j)
+        for (int i = tid; i < wout; i += NT) yo[(long)j * wout + i] = __builtin_nan("");
   }
   if (tid == 0) {
     status[unit] = st;
@@ -1674,5 +1735,149 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   }
   return true;
 }
+// Batched shared-step evolution (evolve_modes_batched, perturbations.py:1002-1120, with
+// Rodas5Batched): workgroup g integrates the `batch` consecutive units g * batch .. (g + 1) *
+// batch - 1, which must belong to one cosmology, on ONE step sequence: a common start time
+// (the smallest of the units' own), one step size, and one accept decision from the RMS of the
+// filtered errors over all batch x 6 entries.  Each attempt walks the units in turn through the
+// per-mode Rodas5 stages in LDS; their states live in `state` (2 x nvar doubles per unit, the
+// current state and the candidate, swapped on accept), which stays in L2 between visits.
+// step_log, when given, receives (t_prev, t_next, error norm) of every attempt of batch g at
+// step_log[(g * max_steps + step) * 3].  Status 5: the batch's units have different cosmologies.
+// SRC: the source epilogue (so) replaces the output conversion; y_out is not written.
+template <int NQ, int NT, bool SRC = false>
+__device__ __forceinline__ void solve_batch(const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
+                                            const double *__restrict__ unit_k, int n_units, int batch, const double *__restrict__ tau_out,
+                                            int nout, const double *__restrict__ nu_qw, const DebProblem &prob, const DebSolverCfg &cfg,
+                                            long lds_bytes, double *__restrict__ step_log, double *__restrict__ y_out,
+                                            int *__restrict__ status, int *__restrict__ n_acc, int *__restrict__ n_rej,
+                                            double *__restrict__ state, const SrcOut *so = nullptr) {
+  const int g = blockIdx.x, tid = threadIdx.x;
+  const int u0 = g * batch;
+  if (u0 + batch > n_units) return;
+  PLay L;
+  Ws W;
+  if (!setup_solver<NQ, NT>(prob, nu_qw, lds_bytes, L, W, tid)) {
+    for (int j = tid; j < batch; j += NT) { status[u0 + j] = 3; n_acc[u0 + j] = 0; n_rej[u0 + j] = 0; }
+    return;
+  }
+  const int n = L.n;
+  const int cosmo = unit_cosmo[u0];
+  __shared__ BgView s_bg;  // in LDS, as in solve_unit
+  if (tid == 0) { W.sc->i = -1; W.sc->j = -1; s_bg.init(tables + (long)cosmo * tstride); }
+  __syncthreads();
+  const BgView &b = s_bg;
+  const int wout = prob.return_full ? n : 20;
+  double *yo = SRC ? nullptr : y_out + (long)u0 * nout * wout;  // unit j's outputs at yo + j nout wout
+  int mixed = 0;
+  for (int j = tid; j < batch; j += NT) mixed |= (unit_cosmo[u0 + j] != cosmo);
+  mixed = __syncthreads_or(mixed);
+  const bool phantom = !b.lambda_de && (1.0 + b.w0) * (1.0 + b.w0 + b.wa) <= 0.0;
+  if (mixed || phantom) {  // not integrated: NaN outputs (w crossing -1 as in the per-mode kernel)
+    if constexpr (SRC) {
+      for (int j = 0; j < batch; ++j) nan_sources<NT>(u0 + j, 0, nout, *so, tid);
+    } else {
+      for (long i = tid; i < (long)batch * nout * wout; i += NT) yo[i] = __builtin_nan("");
+    }
+    for (int j = tid; j < batch; j += NT) { status[u0 + j] = mixed ? 5 : 4; n_acc[u0 + j] = 0; n_rej[u0 + j] = 0; }
+    return;
+  }
+
+  const double *tout = tau_out + (long)cosmo * nout;
+  double tmin = tout[0], t1 = tout[0];
+  for (int j = 1; j < nout; ++j) { tmin = fmin(tmin, tout[j]); t1 = fmax(t1, tout[j]); }
+  // determine_start_times_per_batch (perturbations.py:954-999): the earliest start of the batch
+  double t0 = 1e300;
+  for (int j = 0; j < batch; ++j) t0 = fmin(t0, 0.99 * fmin(tmin, starting_time(b, unit_k[u0 + j])));
+  double *st0 = state + (long)u0 * 2 * n;  // unit j: st0 + 2 n j + {cur, 1 - cur} n
+  for (int j = 0; j < batch; ++j) {
+    if (tid == 0) initial_conditions(b, L, t0, unit_k[u0 + j], W.y0);
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) st0[2L * n * j + i] = W.y0[i];
+    __syncthreads();
+  }
+
+  double tprev = t0;
+  double tnext = fmin(t0 + fmin(t0 / 4.0, 0.5 * (t1 - t0)), t1);
+  Pid pid;
+  pid.prev_inv = 1.0;
+  pid.prev_prev_inv = 1.0;
+  int isave = 0, steps = 0, na = 0, nr = 0, st = 0, cur = 0;
+  while (tprev < t1) {
+    if (steps >= cfg.max_steps) { st = 1; break; }
+    const double dt = tnext - tprev;
+    double acc = 0.0, acc_nan = 0.0;  // scaled with max(|y0|, |y1|), and with |y0| alone
+    int nanany = 0;
+    for (int j = 0; j < batch; ++j) {
+      const double k = unit_k[u0 + j];
+      const double *ys = st0 + 2L * n * j + (long)cur * n;
+      double *yd = st0 + 2L * n * j + (long)(1 - cur) * n;
+      for (int i = tid; i < n; i += NT) W.y0[i] = ys[i];
+      __syncthreads();
+      const int nanflag = rodas5_attempt<NQ, NT>(b, L, k, tprev, dt, W, tid);
+      for (int i = tid; i < n; i += NT) yd[i] = W.y1[i];
+      acc += filtered_sq(k, W.y0, W.y1, W.K + 7 * n, 0, cfg);
+      acc_nan += filtered_sq(k, W.y0, W.y1, W.K + 7 * n, 1, cfg);
+      nanany |= nanflag;
+      __syncthreads();
+    }
+    const double en = sqrt((nanany ? acc_nan : acc) / (6.0 * batch));
+    if (step_log != nullptr && tid == 0) {
+      double *lg = step_log + ((long)g * cfg.max_steps + steps) * 3;
+      lg[0] = tprev;
+      lg[1] = tnext;
+      lg[2] = en;
+    }
+    if (!isfinite(en)) { st = 2; break; }
+    const bool keep = en < 1.0;
+    const double factor = pid.adapt(en, keep, cfg);
+    if (keep) {
+      if (isave < nout && tout[isave] <= tnext) {  // outputs in (tprev, tnext]: interpolate every unit
+        int isv = isave;
+        for (int j = 0; j < batch; ++j) {
+          const double k = unit_k[u0 + j];
+          const double *ya = st0 + 2L * n * j + (long)cur * n;
+          const double *yb = st0 + 2L * n * j + (long)(1 - cur) * n;
+          for (int i = tid; i < n; i += NT) { W.y0[i] = ya[i]; W.y1[i] = yb[i]; }
+          __syncthreads();
+          isv = isave;
+          while (isv < nout && tout[isv] <= tnext) {
+            double th = (tout[isv] - tprev) / (tnext - tprev);
+            for (int i = tid; i < n; i += NT) W.u[i] = W.y0[i] + th * (W.y1[i] - W.y0[i]);
+            __syncthreads();
+            if constexpr (SRC) emit_sources<NT>(b, L, k, tout[isv], W.u, u0 + j, isv, nout, *so, W, tid);
+            else emit_output<NT>(b, L, k, tout[isv], W.u, yo + ((long)j * nout + isv) * wout, prob.return_full, tid);
+            __syncthreads();
+            ++isv;
+          }
+        }
+        isave = isv;
+      }
+      cur = 1 - cur;
+      ++na;
+    } else {
+      ++nr;
+    }
+    double nt0 = keep ? tnext : tprev;
+    tprev = fmin(nt0, t1);
+    double tn = nt0 + dt * factor;
+    tnext = (tn > t1 - 1e-10) ? t1 : tn;
+    ++steps;
+  }
+  if (st != 0) {
+    for (int j = 0; j < batch; ++j) {
+      if constexpr (SRC) nan_sources<NT>(u0 + j, isave, nout, *so, tid);
+      else
+        for (int r = isave; r < nout; ++r)
+          for (int i = tid; i < wout; i += NT) yo[((long)j * nout + r) * wout + i] = __builtin_nan("");
+    }
+  }
+  for (int j = tid; j < batch; j += NT) {
+    status[u0 + j] = st;
+    n_acc[u0 + j] = na;
+    n_rej[u0 + j] = nr;
+  }
+}
+
 }  // namespace
 }  // namespace deb

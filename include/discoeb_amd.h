@@ -214,6 +214,26 @@ int deb_cmb_sources(const double *d_tables, int num_thermo, int cosmo, const dou
                     const double *d_aexp, const double *d_tau, int n_tau, const DebProblem *prob, int isw_mode, double *d_out, double *d_work,
                     void *stream);
 
+/* Doubles of workspace deb_evolve_sources needs (batch_size 0: per-mode steps). */
+long deb_evolve_sources_workspace_doubles(int num_thermo, int n_k, int n_tau, int batch_size, const DebProblem *prob);
+
+/* The chain perturbations -> time derivatives -> source function in one pass, for n_k modes d_k of
+ * cosmology `cosmo` at n_tau output times d_tau (its conformal times of the scale factors d_aexp,
+ * increasing): each mode's solve, at every output time, projects the interpolated state (RSA),
+ * evaluates the right-hand side there and forms the source terms from the two, so the full states
+ * and their derivatives (n_k x n_tau x nvar each) are never stored.  d_out: 8 x n_k x n_tau
+ * doubles, the planes of deb_cmb_sources, equal to what deb_evolve (return_full) +
+ * deb_time_derivatives + deb_cmb_sources write.  batch_size 0: adaptive steps per mode (as
+ * deb_evolve; sched gives the launch order and width); > 0: batches of batch_size modes on shared
+ * steps (as deb_evolve_batched; n_k a multiple of batch_size, no launch order).  d_status,
+ * d_n_accept, d_n_reject per mode as deb_evolve.  prob->return_full is ignored.
+ * Replaces the chain of compute_Cell_spectrum_from_cosmo_params (src/discoeb/cmb.py:1030-1073):
+ * evolve_perturbations_batched(return_full=True) (perturbations.py:1229-1337),
+ * compute_time_derivatives (perturbations.py:1339-1406), compute_source_function (cmb.py:17-633). */
+int deb_evolve_sources(const double *d_tables, int num_thermo, int cosmo, const double *d_k, int n_k, int batch_size, const double *d_aexp,
+                       const double *d_tau, int n_tau, const DebProblem *prob, const DebSolverCfg *cfg, int isw_mode, double *d_out, int *d_status,
+                       int *d_n_accept, int *d_n_reject, double *d_work, const DebSchedule *sched, void *stream);
+
 /* The stages of the source function one at a time (the reference's per-stage functions).  Planes are
  * n_k x n_tau doubles, row-major by k; per-time arrays n_tau doubles.
  *
