@@ -9,7 +9,7 @@ from .background import DeviceSpline, attach_background, background_tables, comp
 from .perturbations import (compute_time_derivatives, evolve_perturbations, evolve_perturbations_batched, evolve_units, evolve_units_batched,
                             get_power, k_grid, nvar)
 from .batch import PkGraph, autotune_schedule, evolve_many, power_units, tau_out_all
-from .grad import (PARAM9, PkFunction, evolve_units_adjoint, param_to_theta, pk_and_vjp, pk_and_vjp_many, pk_jacobian, pk_jacobian_many,
+from .grad import (PARAM9, PkFunction, cell_vjp, evolve_units_adjoint, param_to_theta, pk_and_vjp, pk_and_vjp_many, pk_jacobian, pk_jacobian_many,
                    theta_to_param)
 from .lss import get_power_smoothed, get_xi_from_P, power_Kaiser, power_multipoles, savgol_filter
 from .cmb import (cmb_aexp_out, compute_Cell, compute_Cell_spectrum_from_cosmo_params, compute_Dell, compute_metric_perturbations,
@@ -27,5 +27,5 @@ __all__ = ["evolve_background", "background_tables", "attach_background", "compu
            "compute_source_term_doppler", "compute_source_term_polarization", "compute_source_function", "compute_source_term_E",
            "compute_Cell_spectra_from_cosmo_params", "compute_Cell", "compute_Dell", "cmb_aexp_out",
            "compute_Cell_spectrum_from_cosmo_params", "PARAM9", "theta_to_param", "param_to_theta", "pk_jacobian", "pk_jacobian_many", "PkFunction",
-           "pk_and_vjp", "pk_and_vjp_many", "evolve_units_adjoint", "get_power_smoothed", "get_xi_from_P", "power_Kaiser",
+           "pk_and_vjp", "pk_and_vjp_many", "evolve_units_adjoint", "cell_vjp", "get_power_smoothed", "get_xi_from_P", "power_Kaiser",
            "power_multipoles", "savgol_filter"]

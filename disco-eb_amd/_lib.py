@@ -153,7 +153,7 @@ def lib():
     L.deb_evolve_sources_workspace_doubles.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(DebProblem)]
     L.deb_evolve_sources.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                      ctypes.POINTER(DebProblem), ctypes.POINTER(DebSolverCfg), ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
-                                     ctypes.POINTER(DebSchedule), _vp]
+                                     _vp, _vp, ctypes.c_long, _vp, ctypes.POINTER(DebSchedule), _vp]
     L.deb_los_workspace_doubles.restype = ctypes.c_long
     L.deb_los_workspace_doubles.argtypes = [ctypes.c_int] * 5
     L.deb_los_theta_ell.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,

@@ -308,7 +308,7 @@ def cmb_aexp_out(n_early=32, n_pre_recomb=64, n_recomb=128, n_post=96):
 
 
 def compute_sources_fused(param, kmodes, aexp_out, *, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4,
-                          max_steps=2048, solver="per_mode", batch_size=16, isw="spline", threads=0):
+                          max_steps=2048, solver="per_mode", batch_size=16, isw="spline", threads=0, step_log=False, replay=None):
     """The source function S(k, tau) of the chain perturbations -> time derivatives -> sources
     (cmb.py:1030-1073) in one launch (deb_evolve_sources): each mode's solve forms the source
     terms at its output times from the projected state and its right-hand side, so only the
@@ -318,7 +318,9 @@ def compute_sources_fused(param, kmodes, aexp_out, *, lmaxg=11, lmaxgp=11, lmaxr
     threads: threads per mode or batch (0: 64 per mode, 128 per batch -- with thousands of light
     modes, each spending most of its time on the 2500 output evaluations, more modes in flight
     beat wider workgroups: 0.15 s per mode at 64 threads against 0.35 s at 256 for the 3000-mode
-    workload).  param: from evolve_background.
+    workload).  param: from evolve_background.  step_log: also return the attempted steps
+    ("step_log": (n_k or n_k / batch_size, max_steps, 3), as evolve_units logs them); replay:
+    (mesh, mesh_len) to take each mode's frozen step mesh (per-mode solver only; see evolve_units).
     Returns {"S", "S1".."S4", "SE", "alpha", "alphaprime": (n_k, n_tau), "tau": tau_out,
     "n_accept", "n_reject": (n_k,)}."""
     from .perturbations import DEFAULT_SCHEDULE, _raise_on_status, _schedule
@@ -347,13 +349,33 @@ def compute_sources_fused(param, kmodes, aexp_out, *, lmaxg=11, lmaxgp=11, lmaxr
     threads = int(threads) or (64 if bs == 0 else 128)
     sched_dict = dict(DEFAULT_SCHEDULE, threads=threads) if bs == 0 else {"threads": threads}
     sched, order = _schedule(sched_dict, kk)
+    log, mesh, mlen, mstride = None, None, None, 0
+    if step_log and replay is not None:
+        raise ValueError("step_log and replay are exclusive")
+    if step_log:
+        log = torch.full((n_k if bs == 0 else n_k // bs, int(max_steps), 3), float("nan"), dtype=torch.float64, device=dev)
+    if replay is not None:
+        from .perturbations import _check_mesh
+        if bs:
+            raise ValueError("a replay mesh takes the per-mode solver")
+        mesh = replay[0].to(device=dev, dtype=torch.float64).contiguous()
+        mlen = replay[1].to(device=dev, dtype=torch.int32).contiguous()
+        if mesh.ndim != 2 or mesh.shape[0] != n_k or mlen.numel() != n_k:
+            raise ValueError("replay mesh must be (num_k, L) with one length per mode")
+        _check_mesh(mesh, mlen)
+        mstride = mesh.shape[1]
     _lib.check(L.deb_evolve_sources(_lib.ptr(t), N, c, _lib.ptr(kk), n_k, bs, _lib.ptr(aexp), _lib.ptr(tau_out), n_tau, ctypes.byref(prob),
                                     ctypes.byref(cfg), 1 if isw == "algebraic" else 0, _lib.ptr(out), _lib.ptr(status), _lib.ptr(nacc),
-                                    _lib.ptr(nrej), _lib.ptr(work), ctypes.byref(sched) if sched is not None else None, _lib.stream()))
+                                    _lib.ptr(nrej), _lib.ptr(work), _lib.ptr(log) if log is not None else None,
+                                    _lib.ptr(mesh) if mesh is not None else None, mstride, _lib.ptr(mlen) if mlen is not None else None,
+                                    ctypes.byref(sched) if sched is not None else None, _lib.stream()))
     del order, work  # stream-ordered: reused by the allocator only after the launches have run
     _raise_on_status(status)
-    return {"S1": out[0], "S2": out[1], "S3": out[2], "S4": out[3], "S": out[4], "alpha": out[5], "alphaprime": out[6], "SE": out[7],
-            "tau": tau_out, "n_accept": nacc, "n_reject": nrej}
+    res = {"S1": out[0], "S2": out[1], "S3": out[2], "S4": out[3], "S": out[4], "alpha": out[5], "alphaprime": out[6], "SE": out[7],
+           "tau": tau_out, "n_accept": nacc, "n_reject": nrej}
+    if log is not None:
+        res["step_log"] = log
+    return res
 
 
 def compute_sources_streamed(param, kmodes, aexp_out, *, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4,

@@ -33,7 +33,8 @@ extern "C" int deb_launch_perturb(const double *, int, const int *, const double
 extern "C" long deb_perturb_lds_bytes(int, int, int, int, int);
 extern "C" long deb_evolve_sources_work_doubles(int, int, int, int, int);
 extern "C" int deb_launch_evolve_sources(const double *, int, const double *, int, int, const double *, const double *, int, const DebProblem *,
-                                         const DebSolverCfg *, int, double *, int *, int *, int *, double *, const int *, int, hipStream_t);
+                                         const DebSolverCfg *, int, double *, int *, int *, int *, double *, const int *, int, double *,
+                                         const double *, long, const int *, hipStream_t);
 extern "C" long deb_adjoint_lds_bytes(const DebProblem *, int);
 extern "C" int deb_launch_adjoint(const double *, int, const double *, double *, const double *, int, const int *, const double *, int,
                                   const double *, const double *, int, const double *, const DebProblem *, const double *, long, const int *,
@@ -309,8 +310,12 @@ long deb_evolve_sources_workspace_doubles(int num_thermo, int n_k, int n_tau, in
 
 int deb_evolve_sources(const double *d_tables, int num_thermo, int cosmo, const double *d_k, int n_k, int batch_size, const double *d_aexp,
                        const double *d_tau, int n_tau, const DebProblem *prob, const DebSolverCfg *cfg, int isw_mode, double *d_out, int *d_status,
-                       int *d_n_accept, int *d_n_reject, double *d_work, const DebSchedule *sched, void *stream) {
+                       int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const double *d_mesh, long mesh_stride,
+                       const int *d_mesh_len, const DebSchedule *sched, void *stream) {
   if (isw_mode != 0 && isw_mode != 1) return fail(-1, "isw_mode must be 0 (spline derivative) or 1 (algebraic)");
+  if (d_mesh && batch_size > 0) return fail(-1, "a replay mesh takes per-mode steps (batch_size 0)");
+  if (d_mesh && (!d_mesh_len || mesh_stride < 1)) return fail(-1, "a replay mesh needs its lengths and a stride >= 1");
+  if (d_mesh && d_step_log) return fail(-1, "step log and replay mesh are exclusive");
   if (n_tau < 3) return fail(-1, "need n_tau >= 3");
   if (cosmo < 0) return fail(-1, "cosmology index must be >= 0");
   if (batch_size < 0) return fail(-1, "batch_size must be >= 0");
@@ -323,7 +328,8 @@ int deb_evolve_sources(const double *d_tables, int num_thermo, int cosmo, const 
   if (nt != 0 && nt != 64 && nt != 128 && nt != 256) return fail(-1, "threads_per_unit must be 0, 64, 128 or 256");
   const double *t = d_tables + (long)cosmo * deb_table_doubles(num_thermo);
   rc = deb_launch_evolve_sources(t, num_thermo, d_k, n_k, batch_size, d_aexp, d_tau, n_tau, prob, cfg, isw_mode, d_out, d_status, d_n_accept,
-                                 d_n_reject, d_work, sched ? sched->d_order : nullptr, nt, (hipStream_t)stream);
+                                 d_n_reject, d_work, sched ? sched->d_order : nullptr, nt, d_step_log, d_mesh, mesh_stride, d_mesh_len,
+                                 (hipStream_t)stream);
   if (rc == -2) return fail(-1, "state too large for LDS");
   return hip_rc(rc, "deb_evolve_sources");
 }

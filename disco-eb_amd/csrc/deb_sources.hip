@@ -16,7 +16,9 @@ __global__ __launch_bounds__(NT) void perturb_sources_kernel(const double *__res
                                                                const double *__restrict__ unit_k, int n_units, const double *__restrict__ tau_out,
                                                                int nout, const double *__restrict__ nu_qw, DebProblem prob, DebSolverCfg cfg,
                                                                long lds_bytes, SrcOut so, int *__restrict__ status, int *__restrict__ n_acc,
-                                                               int *__restrict__ n_rej, const int *__restrict__ order) {
+                                                               int *__restrict__ n_rej, const int *__restrict__ order, double *__restrict__ step_log,
+                                                               const double *__restrict__ replay, long replay_stride,
+                                                               const int *__restrict__ replay_len) {
   if ((int)blockIdx.x >= n_units) return;
   const int tid = threadIdx.x;
   PLay L;
@@ -26,8 +28,8 @@ __global__ __launch_bounds__(NT) void perturb_sources_kernel(const double *__res
     return;
   }
   const int unit = order ? order[blockIdx.x] : (int)blockIdx.x;
-  solve_unit<NQ, NT, true>(unit, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, nullptr, nullptr, status, n_acc, n_rej, nullptr,
-                           0, nullptr, L, W, tid, &so);
+  solve_unit<NQ, NT, true>(unit, tables, tstride, unit_cosmo, unit_k, tau_out, nout, prob, cfg, step_log, nullptr, status, n_acc, n_rej, replay,
+                           replay_stride, replay_len, L, W, tid, &so);
 }
 
 template <int NQ, int NT>
@@ -37,8 +39,8 @@ __global__ __launch_bounds__(NT) void perturb_batched_sources_kernel(const doubl
                                                                        const double *__restrict__ nu_qw, DebProblem prob, DebSolverCfg cfg,
                                                                        long lds_bytes, SrcOut so, int *__restrict__ status,
                                                                        int *__restrict__ n_acc, int *__restrict__ n_rej,
-                                                                       double *__restrict__ state) {
-  solve_batch<NQ, NT, true>(tables, tstride, unit_cosmo, unit_k, n_units, batch, tau_out, nout, nu_qw, prob, cfg, lds_bytes, nullptr, nullptr,
+                                                                       double *__restrict__ state, double *__restrict__ step_log) {
+  solve_batch<NQ, NT, true>(tables, tstride, unit_cosmo, unit_k, n_units, batch, tau_out, nout, nu_qw, prob, cfg, lds_bytes, step_log, nullptr,
                             status, n_acc, n_rej, state, &so);
 }
 
@@ -95,15 +97,18 @@ extern "C" long deb_evolve_sources_work_doubles(int num_thermo, int n_k, int n_t
 }
 
 // d_t: the tables of the one cosmology every unit belongs to; d_tau / d_aexp: its n_tau output
-// times and scale factors.  batch 0: per-mode adaptive steps (d_order, threads_per_unit as
-// deb_launch_perturb); batch > 0: batched shared steps (n_k a multiple of batch).
+// times and scale factors.  batch 0: per-mode adaptive steps (d_order, threads_per_unit, the step
+// log and the replay mesh as deb_launch_perturb); batch > 0: batched shared steps (n_k a multiple
+// of batch; a step log per batch, no replay).
 extern "C" int deb_launch_evolve_sources(const double *d_t, int num_thermo, const double *d_k, int n_k, int batch, const double *d_aexp,
                                          const double *d_tau, int n_tau, const DebProblem *prob, const DebSolverCfg *cfg, int isw,
                                          double *d_out, int *d_status, int *d_nacc, int *d_nrej, double *d_work, const int *d_order,
-                                         int threads_per_unit, hipStream_t stream) {
+                                         int threads_per_unit, double *d_step_log, const double *d_replay, long replay_stride,
+                                         const int *d_replay_len, hipStream_t stream) {
   const long lds = deb::deb_perturb_lds_bytes_dev(*prob);
   if (lds > 160 * 1024) return -2;
   if (batch < 0 || (batch > 0 && n_k % batch != 0)) return -3;
+  if (batch > 0 && d_replay != nullptr) return -5;
   const int nt = threads_per_unit > 0 ? threads_per_unit : DEB_DEFAULT_THREADS_PER_UNIT;
   if (nt != 64 && nt != 128 && nt != 256) return -4;
   double *qw = d_work, *vw = qw + 32, *V = vw + 12L * num_thermo, *pot = V + (long)deb::NV * n_tau;
@@ -133,7 +138,7 @@ extern "C" int deb_launch_evolve_sources(const double *d_t, int num_thermo, cons
       if (e != hipSuccess) return (int)e;
     }
     hipLaunchKernelGGL(kern, dim3(n_k), dim3(nt), (size_t)lds, stream, d_t, stride, unit_cosmo, d_k, n_k, d_tau, n_tau, qw, pr, *cfg, lds, so,
-                       d_status, d_nacc, d_nrej, d_order);
+                       d_status, d_nacc, d_nrej, d_order, d_step_log, d_replay, replay_stride, d_replay_len);
   } else {
     deb::SrcBatchedFn kern = deb::pick_src_batched(prob->nqmax, nt);
     if (lds > 64 * 1024) {
@@ -141,7 +146,7 @@ extern "C" int deb_launch_evolve_sources(const double *d_t, int num_thermo, cons
       if (e != hipSuccess) return (int)e;
     }
     hipLaunchKernelGGL(kern, dim3(n_k / batch), dim3(nt), (size_t)lds, stream, d_t, stride, unit_cosmo, d_k, n_k, batch, d_tau, n_tau, qw, pr,
-                       *cfg, lds, so, d_status, d_nacc, d_nrej, state);
+                       *cfg, lds, so, d_status, d_nacc, d_nrej, state, d_step_log);
   }
   rc = (int)hipGetLastError();
   if (rc) return rc;

@@ -471,5 +471,70 @@ def pk_and_vjp(theta, base, kmodes, cotangent, method="adjoint", **opts):
     return pk[0], v[0]
 
 
+def _cell_weights(weights, ellmax, dev):
+    """Weights per C_l index 0..ellmax: given for l = 2..ellmax (ellmax - 1 values) or for every l."""
+    w = torch.as_tensor(np.asarray(weights, dtype=np.float64), device=dev).reshape(-1)
+    if w.numel() == ellmax - 1:
+        w = torch.cat([torch.zeros(2, dtype=torch.float64, device=dev), w])
+    if w.numel() != ellmax + 1:
+        raise ValueError(f"weights must hold ellmax - 1 = {ellmax - 1} values (l = 2..ellmax) or ellmax + 1 (every l)")
+    return w
+
+
+def cell_vjp(theta, base, kmodes, weights, *, ellmax=2500, aexp_out=None, n_fftlog=16384, rel_step=REL_STEP, num_thermo=1024, lmaxg=24,
+             lmaxgp=24, lmaxr=12, lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4, isw="spline", return_cell=False):
+    """L = sum_l weights_l C_l^TT and its gradient with respect to theta (9,).
+
+    The chain is compute_Cell_spectrum_from_cosmo_params' (cmb.py:930-1090): background
+    (num_thermo knots), per-mode adaptive perturbations at kmodes with the source function formed
+    in the solver's output epilogue (compute_sources_fused), line of sight to ellmax at n_fftlog,
+    C_l = trapezoid over ln k of (k / k_p)^(n_s - 1) Theta_l^2.  The derivative is that of the
+    frozen-mesh map, as for P(k): the base solve's accepted steps are replayed for the twelve
+    shifted cosmologies of the six parameters that enter the background and the perturbations
+    (omega_c, omega_b, h, m_nu, w0, wa), each taken by a central difference of the whole chain
+    (background, sources on the frozen mesh, line of sight, C_l); n_s by the exact cross
+    spectrum of Theta_l and ln(k / k_p) Theta_l; ln(1e10 A_s) and tau_reion do not enter C_l as
+    the reference defines it (A_s enters D_l).  weights: for l = 2..ellmax or for every l.
+    Returns (L, grad (9,)) as a float and a device tensor, plus C_l when return_cell."""
+    from .background import evolve_background
+    from .cmb import cmb_aexp_out, compute_Cell, compute_sources_fused, compute_theta_ell
+    theta = np.asarray(theta, dtype=np.float64).reshape(-1)
+    aexp = cmb_aexp_out() if aexp_out is None else np.asarray(aexp_out, dtype=np.float64)
+    lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
+    dev = _lib.device()
+    kk = torch.as_tensor(np.asarray(kmodes, dtype=np.float64), device=dev).reshape(-1).contiguous()
+    w = _cell_weights(weights, int(ellmax), dev)
+
+    def chain(th, replay=None):
+        p = evolve_background(param=theta_to_param(th, base), num_thermo=num_thermo)
+        src = compute_sources_fused(p, kk, aexp, rtol=rtol, atol=atol, isw=isw, step_log=replay is None, replay=replay, **lm)
+        tau0 = float(p["tau_of_a_spline"].evaluate(np.array([1.0])).cpu()[0])
+        th_l, kout = compute_theta_ell(ellmax, kk, src["tau"], src["S"], tau0, n_fftlog=n_fftlog)
+        cl = compute_Cell(th_l, kout, p["n_s"], p.get("k_p", 0.05))
+        return p, src, th_l, kout, cl
+
+    p0, src0, th0, k0, cl0 = chain(theta)
+    L0 = float((w * cl0).sum())
+    t1 = src0["tau"].max().expand(kk.numel())
+    mesh, mlen = _accepted_mesh(src0["step_log"], src0["n_accept"], t1)
+    del src0
+    g = torch.zeros(9, dtype=torch.float64, device=dev)
+    for j in ODE_PARAMS:
+        d = rel_step * max(abs(theta[j]), 1e-2)
+        Ls = []
+        for th in _shift_pair(theta, j, d):
+            _, _, _, _, cl = chain(th, replay=(mesh, mlen))
+            Ls.append((w * cl).sum())
+        g[j] = (Ls[0] - Ls[1]) / (2.0 * d)
+    # n_s: d C_l / d n_s = trapezoid over ln k of ln(k / k_p) (k / k_p)^(n_s - 1) Theta_l^2, the cross
+    # spectrum of Theta_l with ln(k / k_p) Theta_l
+    kp = p0.get("k_p", 0.05)
+    dcl = compute_Cell(th0, k0, p0["n_s"], kp, theta_ell2=th0 * torch.log(k0 / kp)[:, None])
+    g[3] = (w * dcl).sum()
+    if return_cell:
+        return L0, g, cl0
+    return L0, g
+
+
 __all__ = ["PARAM9", "theta_to_param", "param_to_theta", "pk_jacobian", "pk_jacobian_many", "pk_and_vjp_many", "PkFunction", "pk_and_vjp",
-           "evolve_units_adjoint"]
+           "evolve_units_adjoint", "cell_vjp"]

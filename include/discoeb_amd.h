@@ -226,13 +226,17 @@ long deb_evolve_sources_workspace_doubles(int num_thermo, int n_k, int n_tau, in
  * deb_time_derivatives + deb_cmb_sources write.  batch_size 0: adaptive steps per mode (as
  * deb_evolve; sched gives the launch order and width); > 0: batches of batch_size modes on shared
  * steps (as deb_evolve_batched; n_k a multiple of batch_size, no launch order).  d_status,
- * d_n_accept, d_n_reject per mode as deb_evolve.  prob->return_full is ignored.
+ * d_n_accept, d_n_reject per mode as deb_evolve.  prob->return_full is ignored.  d_step_log: NULL or
+ * the attempted steps as deb_evolve (per mode) / deb_evolve_batched (per batch) log them.  d_mesh,
+ * mesh_stride, d_mesh_len: NULL, or a frozen step mesh per mode as deb_evolve_replay takes it (per-mode
+ * steps only; exclusive with the step log): the map that frozen-mesh derivatives differentiate.
  * Replaces the chain of compute_Cell_spectrum_from_cosmo_params (src/discoeb/cmb.py:1030-1073):
  * evolve_perturbations_batched(return_full=True) (perturbations.py:1229-1337),
  * compute_time_derivatives (perturbations.py:1339-1406), compute_source_function (cmb.py:17-633). */
 int deb_evolve_sources(const double *d_tables, int num_thermo, int cosmo, const double *d_k, int n_k, int batch_size, const double *d_aexp,
                        const double *d_tau, int n_tau, const DebProblem *prob, const DebSolverCfg *cfg, int isw_mode, double *d_out, int *d_status,
-                       int *d_n_accept, int *d_n_reject, double *d_work, const DebSchedule *sched, void *stream);
+                       int *d_n_accept, int *d_n_reject, double *d_work, double *d_step_log, const double *d_mesh, long mesh_stride,
+                       const int *d_mesh_len, const DebSchedule *sched, void *stream);
 
 /* The stages of the source function one at a time (the reference's per-stage functions).  Planes are
  * n_k x n_tau doubles, row-major by k; per-time arrays n_tau doubles.
