@@ -18,6 +18,7 @@ PARAM_KEYS = [
     "use_tca", "tca_tau_c_over_tau_h_trigger", "tca_tau_c_over_tau_k_trigger",
     "use_rsa", "rsa_tau_c_over_tau_trigger", "rsa_tau_over_tau_k_trigger",
     "use_ur_fluid", "ur_fluid_tau_over_tau_k_trigger",
+    "tau_reion",
 ]
 # defaults of the approximation switches (src/discoeb/approximations.py:9-20) and optional keys
 PARAM_DEFAULTS = {
@@ -25,11 +26,13 @@ PARAM_DEFAULTS = {
     "use_tca": False, "tca_tau_c_over_tau_h_trigger": 0.015, "tca_tau_c_over_tau_k_trigger": 0.010,
     "use_rsa": True, "rsa_tau_c_over_tau_trigger": 10.0, "rsa_tau_over_tau_k_trigger": 80.0,
     "use_ur_fluid": True, "ur_fluid_tau_over_tau_k_trigger": 120.0,
+    # reionization optical depth (an extension; 0: none, as the reference's thermal history)
+    "tau_reion": 0.0,
 }
 HDR = {  # header slots written by the background kernel
     "grhom": 24, "grhog": 25, "grhor": 26, "adotrad": 27, "amnu": 28, "Omegamnu": 29, "OmegaDE": 30, "taumin": 31,
     "taumax": 32, "fHe": 33, "tau0": 34, "tau_maxvis": 35, "optical_depth_today": 36, "akthom": 37,
-    "recfast_failures": 40, "amin": 44, "amax": 45,
+    "recfast_failures": 40, "amin": 44, "amax": 45, "z_reion": 47, "tau_reion_achieved": 48,
 }
 ARRAY_SLOTS = [
     "nu_loga", "nu_logrho", "nu_logrho_S", "nu_logp", "nu_logp_S", "nu_logpp", "nu_logpp_S",

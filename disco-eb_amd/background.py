@@ -140,7 +140,11 @@ def evolve_background(*, param, thermo_module="RECFAST", num_thermo: int = 256, 
     thermodynamics columns 'scale factor a', 'conf. time [Mpc]', 'x_e', 'Tb [K]', 'c_b^2' in CLASS
     order; its length replaces num_thermo and its a-range sets amin, amax).  rtol/atol/order are
     accepted and, as in the reference, not used (thermodynamics_recfast.py:461).  Keys the
-    reference sets only for RECFAST (mu, xeprime_recf) hold zeros for the other modules."""
+    reference sets only for RECFAST (mu, xeprime_recf) hold zeros for the other modules.
+    An extension: param["tau_reion"] > 0 adds reionization to the RECFAST history (the tanh
+    model of Lewis 2008 as CAMB uses it, hydrogen and both helium ionisations; the reference has
+    none), its redshift solved so that the optical depth it adds is tau_reion; param then also
+    holds z_reion and tau_reion_achieved.  The other thermal-history modules ignore it."""
     tables = background_tables([param], num_thermo, thermo_module=thermo_module, class_thermo=class_thermo)
     n = len(class_thermo[CLASS_COLUMNS[0]]) if thermo_module == "CLASS" else num_thermo
     return attach_background(param, tables, n, 0)

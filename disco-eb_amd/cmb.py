@@ -299,12 +299,29 @@ def compute_Dell(Cell, A_s, Tcmb, ellmax=None):
     return ell.to(torch.int64), dl
 
 
-def cmb_aexp_out(n_early=32, n_pre_recomb=64, n_recomb=128, n_post=96):
+def cmb_aexp_out(n_early=32, n_pre_recomb=64, n_recomb=128, n_post=96, n_reion=0):
     """Output scale factors of the CMB driver: log-spaced pieces split at z = 3000, 1400, 600
-    (cmb.py:999-1014)."""
+    (cmb.py:999-1014).  n_reion > 0 (an extension) merges n_reion more points, log-spaced in a
+    over z in [0, 30], so that the source function resolves the reionization visibility: the
+    reference's 96 points below z = 600 put about one sample in its width, and the line of sight
+    then depends on where the samples fall (derivatives with respect to tau_reion wander by tens
+    of percent)."""
     a1, a2, a3 = 1.0 / 3001.0, 1.0 / 1401.0, 1.0 / 601.0
-    return np.concatenate([_geomspace(1e-4, a1, n_early, endpoint=False), _geomspace(a1, a2, n_pre_recomb, endpoint=False),
-                           _geomspace(a2, a3, n_recomb, endpoint=False), _geomspace(a3, 1.0, n_post)])
+    a = np.concatenate([_geomspace(1e-4, a1, n_early, endpoint=False), _geomspace(a1, a2, n_pre_recomb, endpoint=False),
+                        _geomspace(a2, a3, n_recomb, endpoint=False), _geomspace(a3, 1.0, n_post)])
+    if n_reion > 0:
+        a = np.unique(np.concatenate([a, _geomspace(1.0 / 31.0, 1.0, int(n_reion))]))
+    return a
+
+
+# late-time output points the drivers add when reionization is on (see cmb_aexp_out)
+REION_OUTPUTS = 200
+
+
+def _driver_aexp(param_dict, aexp_out):
+    if aexp_out is not None:
+        return np.asarray(aexp_out, dtype=np.float64)
+    return cmb_aexp_out(n_reion=REION_OUTPUTS if float(param_dict.get("tau_reion", 0.0)) > 0.0 else 0)
 
 
 def compute_sources_fused(param, kmodes, aexp_out, *, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-4,
@@ -444,9 +461,10 @@ def compute_Cell_spectrum_from_cosmo_params(param_dict, ellmax=2500, nmodes=128,
     k (compute_sources_streamed) so that full states are held for k_chunk modes at a time;
     fused (default) forms the sources in the solver's output epilogue (compute_sources_fused: no
     states stored, the same planes bit for bit; fused=False runs the staged chain, streamed when
-    k_chunk is given); aexp_out (default cmb_aexp_out()); rtol, atol."""
+    k_chunk is given); aexp_out (default cmb_aexp_out(), with REION_OUTPUTS late points merged
+    when param_dict switches on reionization); rtol, atol."""
     param = evolve_background(param=dict(param_dict), thermo_module="RECFAST", num_thermo=1024)
-    aexp_out = cmb_aexp_out() if aexp_out is None else np.asarray(aexp_out, dtype=np.float64)
+    aexp_out = _driver_aexp(param_dict, aexp_out)
     lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
     tau0 = float(param["tau_of_a_spline"].evaluate(np.array([1.0])).cpu()[0])
     if fused:
@@ -481,14 +499,15 @@ def compute_Cell_spectra_from_cosmo_params(param_dict, spectra=("TT", "TE", "EE"
                                             fused=True):
     """C_l^TT, C_l^TE and C_l^EE from cosmological parameters: the chain of
     compute_Cell_spectrum_from_cosmo_params (cmb.py:930-1090) with the E-mode source projected
-    beside the temperature source (an extension: the reference computes TT only; no reionization,
-    as in the reference's thermal history).  Returns ({spectrum: C_l}, param)."""
+    beside the temperature source (an extension: the reference computes TT only; reionization
+    only when param_dict carries tau_reion > 0, see evolve_background).  Returns ({spectrum: C_l},
+    param)."""
     for x in spectra:
         if x not in ("TT", "TE", "EE"):
             raise ValueError("spectra are drawn from 'TT', 'TE', 'EE'")
     from .perturbations import k_grid
     param = evolve_background(param=dict(param_dict), thermo_module="RECFAST", num_thermo=1024)
-    aexp_out = cmb_aexp_out() if aexp_out is None else np.asarray(aexp_out, dtype=np.float64)
+    aexp_out = _driver_aexp(param_dict, aexp_out)
     lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
     tau0 = float(param["tau_of_a_spline"].evaluate(np.array([1.0])).cpu()[0])
     kmodes = torch.as_tensor(k_grid(param, kmin, kmax, nmodes, k_sampling_method), dtype=torch.float64, device=param["_deb_tables"].device)

@@ -369,6 +369,59 @@ __device__ void mb95_scan(const BgView &b, int N, double taumax, double *aexp, d
   }
 }
 
+// Reionization (no counterpart in the reference, whose thermal histories have none): the tanh
+// model of Lewis (2008, Phys. Rev. D 78, 023002), as CAMB uses it -- hydrogen and the first
+// helium ionisation
+//   x_e = x_rec + (f_re - x_rec) F,  F = (1 + tanh((y_re - y) / dy)) / 2,  y = (1 + z)^(3/2),
+//   y_re = (1 + z_re)^(3/2),  dy = (3/2) (1 + z_re)^(1/2) dz,  dz = 0.5,  f_re = 1 + f_He,
+// and the second helium ionisation, f_He (1 + tanh((3.5 - z) / 0.5)) / 2.  z_re is chosen so that
+// the optical depth these add to the recombination history, akthom int (x_e - x_rec) / a^2 dtau
+// from today to z = 150, equals the requested tau_reion.
+struct Reion {
+  double yre, dy, fre, fHe;
+  __device__ void set(double zre, double fhe) {
+    yre = pow(1.0 + zre, 1.5);
+    dy = 1.5 * sqrt(1.0 + zre) * 0.5;
+    fre = 1.0 + fhe;
+    fHe = fhe;
+  }
+  // x_e - x_rec at scale factor a, and (into d) its a-derivative at fixed x_rec; FH the hydrogen fraction F
+  __device__ double dxe(double a, double xrec, double &d, double &FH) const {
+    const double y = pow(a, -1.5), z = 1.0 / a - 1.0;
+    const double th = tanh((yre - y) / dy), te = tanh((3.5 - z) / 0.5);
+    FH = 0.5 * (1.0 + th);
+    const double Fe = 0.5 * (1.0 + te);
+    d = (fre - xrec) * 0.5 * (1.0 - th * th) * 1.5 * pow(a, -2.5) / dy + fHe * 0.5 * (1.0 - te * te) / (0.5 * a * a);
+    return (fre - xrec) * FH + fHe * Fe;
+  }
+};
+constexpr int REION_NQ = 4096;  // Simpson intervals in ln a over z in [0, 150]
+
+// the optical depth added by reionization at z_re, by Simpson's rule in ln a (block-wide sum;
+// every thread returns it); xrec from the natural spline S of the recombination x_e in ln a
+__device__ double reion_depth(const BgView &b, const double *loga, const double *xe, const double *S, int N, double zre, double *red,
+                              int tid, int nth) {
+  Reion r;
+  r.set(zre, b.fHe);
+  const double l0 = -log(151.0), h = -l0 / REION_NQ;
+  double acc = 0.0;
+  for (int i = tid; i <= REION_NQ; i += nth) {
+    const double la = l0 + h * i, a = exp(la);
+    const int j = spl_index(loga, N, la);
+    const double xrec = spl_eval_at(loga, xe, S, j, la);
+    double d, FH;
+    const double w = (i == 0 || i == REION_NQ) ? 1.0 : ((i & 1) ? 4.0 : 2.0);
+    acc += w * r.dxe(a, xrec, d, FH) * b.akthom / a * b.dtauda(a);  // (x_e - x_rec) akthom / a^2 dtau/da da, da = a dln a
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  __syncthreads();
+  if ((tid & 63) == 0) red[tid >> 6] = acc;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < nth / 64; ++w) s += red[w];
+  return s * h / 3.0;
+}
+
 // thermo: 0 RECFAST, 1 MB95, 2 the thermal history given in thermo_in (per cosmology 5 x N
 // doubles: a ascending, tau, x_e, T_b, c_b^2 -- the CLASS input of background.py:271-285)
 __global__ __launch_bounds__(256) void background_kernel(const double *__restrict__ params, int N, double *__restrict__ tables,
@@ -595,6 +648,55 @@ __global__ __launch_bounds__(256) void background_kernel(const double *__restric
     double cum = 0.0;
     tau[0] = b.taumin;
     for (int i = 1; i < N; ++i) { cum += inc[i]; tau[i] = b.taumin + cum; }
+  }
+  __syncthreads();
+  // reionization on the RECFAST history (tau_reion > 0): z_re by bisection on the added optical
+  // depth, then x_e, its conformal-time derivative, mu and the sound speed at every knot
+  if (thermo == 0 && p[DEB_P_TAU_REION] > 0.0) {
+    const double tre = p[DEB_P_TAU_REION];
+    const double *loga = arr(DEB_A_LOGA), *aexp = arr(DEB_A_AEXP), *Tm = arr(DEB_A_TM);
+    double *xe = arr(DEB_A_XE), *xpr = arr(DEB_A_XEPRIME_RECF), *mu = arr(DEB_A_MU), *cs2 = arr(DEB_A_CS2), *cs2a = arr(DEB_A_CS2A);
+    double *S = arr(DEB_A_SCR4);
+    __shared__ double s_red[4], s_lo, s_hi;
+    if (tid == 0) spl_build(loga, xe, N, S, arr(DEB_A_SCR5));
+    __syncthreads();
+    const double dlo = reion_depth(b, loga, xe, S, N, 0.0, s_red, tid, nth);
+    const double dhi = reion_depth(b, loga, xe, S, N, 100.0, s_red, tid, nth);
+    if (tid == 0) { s_lo = 0.0; s_hi = 100.0; }
+    __syncthreads();
+    if (tre > dlo && tre < dhi) {
+      for (int it = 0; it < 60; ++it) {  // the added depth increases with z_re
+        const double zm = 0.5 * (s_lo + s_hi);
+        const double dm = reion_depth(b, loga, xe, S, N, zm, s_red, tid, nth);
+        __syncthreads();
+        if (tid == 0) {
+          if (dm < tre) s_lo = zm; else s_hi = zm;
+        }
+        __syncthreads();
+      }
+    } else if (tid == 0) {  // outside what z_re in [0, 100] gives: the nearest end
+      s_lo = s_hi = (tre <= dlo) ? 0.0 : 100.0;
+    }
+    __syncthreads();
+    const double zre = 0.5 * (s_lo + s_hi);
+    const double dfin = reion_depth(b, loga, xe, S, N, zre, s_red, tid, nth);
+    if (tid == 0) { t[DEB_HDR_Z_REION] = zre; t[DEB_HDR_TAU_REION_OUT] = dfin; }
+    Reion r;
+    r.set(zre, b.fHe);
+    for (int i = tid; i < N; i += nth) {
+      const double a = aexp[i], xrec = xe[i];
+      double d, FH;
+      const double dx = r.dxe(a, xrec, d, FH);
+      const double xn = xrec + dx;
+      // d/dtau [x_rec + (f_re - x_rec) F_H + ...] = x_rec' (1 - F_H) + (d/da at fixed x_rec) / (dtau/da)
+      xpr[i] = xpr[i] * (1.0 - FH) + d / b.dtauda(a);
+      const double mun = 1.0 / (1.0 + (1.0 / K.mHe_mH - 1.0) * b.YHe + (1.0 - b.YHe) * xn);
+      cs2[i] *= mun / mu[i];  // c_s^2 is proportional to 1 / mu at a given T_m
+      cs2a[i] = a * cs2[i];
+      mu[i] = mun;
+      xe[i] = xn;
+      (void)Tm;
+    }
   }
   __syncthreads();
   // splines of background.py:248-256

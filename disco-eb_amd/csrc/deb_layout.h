@@ -25,6 +25,7 @@ enum {
   DEB_P_OMEGAM, DEB_P_OMEGAB, DEB_P_H0, DEB_P_TCMB, DEB_P_YHE, DEB_P_NEFF, DEB_P_NMNU, DEB_P_MNU, DEB_P_W0, DEB_P_WA,
   DEB_P_CS2DE, DEB_P_OMEGAK, DEB_P_AS, DEB_P_NS, DEB_P_KP,
   DEB_P_USE_TCA, DEB_P_TCA_H, DEB_P_TCA_K, DEB_P_USE_RSA, DEB_P_RSA_C, DEB_P_RSA_K, DEB_P_USE_UFA, DEB_P_UFA_K,
+  DEB_P_TAU_REION, /* optical depth of reionization; <= 0: none (the reference's thermal history has none) */
   DEB_NPARAM
 };
 
@@ -33,7 +34,7 @@ enum {
   DEB_HDR_GRHOM = 24, DEB_HDR_GRHOG, DEB_HDR_GRHOR, DEB_HDR_ADOTRAD, DEB_HDR_AMNU, DEB_HDR_OMEGAMNU, DEB_HDR_OMEGADE,
   DEB_HDR_TAUMIN, DEB_HDR_TAUMAX, DEB_HDR_FHE, DEB_HDR_TAU0, DEB_HDR_TAU_MAXVIS, DEB_HDR_OPTDEPTH_TODAY, DEB_HDR_AKTHOM,
   DEB_HDR_LAMBDA_DE, DEB_HDR_N, DEB_HDR_RECFAST_FAIL, DEB_HDR_NU_X0, DEB_HDR_NU_DXINV, DEB_HDR_A_START, DEB_HDR_AMIN,
-  DEB_HDR_AMAX, DEB_HDR_THERMO,
+  DEB_HDR_AMAX, DEB_HDR_THERMO, DEB_HDR_Z_REION, DEB_HDR_TAU_REION_OUT,
   DEB_HDR_SIZE = 64
 };
 

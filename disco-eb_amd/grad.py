@@ -13,8 +13,10 @@ its integration range).  The six parameters that enter the background and pertur
     (one background launch), as do the initial state and the integration range;
   * method="fd": by central differences of the frozen-mesh map, all twelve shifted cosmologies
     replayed in one launch.
-n_s and A_s enter P(k) only through the primordial spectrum (analytic columns), and tau_reion
-does not enter the reference's model (zero column).
+n_s and A_s enter P(k) only through the primordial spectrum (analytic columns).  tau_reion does
+not enter the reference's model: its column is zero unless the base parameters switch on
+reionization (an extension: base["tau_reion"] > 0, see evolve_background), in which case theta's
+tau_reion sets the reionization optical depth and its column is a frozen-mesh central difference.
 """
 import ctypes
 import math
@@ -36,13 +38,52 @@ ODE_PARAMS = (0, 1, 2, 5, 6, 7)
 REL_STEP = 3e-5
 
 
+def _reionized(base):
+    return float(base.get("tau_reion", 0.0)) > 0.0
+
+
 def theta_to_param(theta, base):
-    """Model parameter dict for theta (9 values), other keys from base."""
-    oc, ob, h, ns, lnA, mnu, w0, wa, _tr = [float(v) for v in theta]
+    """Model parameter dict for theta (9 values), other keys from base.  theta's tau_reion becomes
+    the reionization optical depth only when base has reionization switched on (tau_reion > 0)."""
+    oc, ob, h, ns, lnA, mnu, w0, wa, tr = [float(v) for v in theta]
     p = dict(base)
     p.update(Omegam=(oc + ob) / h ** 2, Omegab=ob / h ** 2, H0=100.0 * h, n_s=ns, A_s=math.exp(lnA) / 1e10, mnu=mnu, w_DE_0=w0,
              w_DE_a=wa)
+    if _reionized(base):
+        p["tau_reion"] = tr
     return p
+
+
+def _reion_column(S, thetas, base, aexp, lm, solver, num_thermo, rel_step, idx):
+    """dP/d tau_reion of every unit by a central difference of the frozen-mesh map: each cosmology's
+    two tau_reion-shifted copies replayed on its base mesh.  (U, nout) tensor."""
+    C = thetas.shape[0]
+    offs, uc0, uk0 = S["offs"], S["uc0"], S["uk0"]
+    dev = uk0.device
+    nks = np.diff(offs)
+    steps = np.array([rel_step * max(abs(thetas[c, 8]), 1e-2) for c in range(C)])
+    params = []
+    for c in range(C):
+        for sgn in (1.0, -1.0):
+            th = thetas[c].copy()
+            th[8] += sgn * steps[c]
+            params.append(theta_to_param(th, base))
+    tab = background_tables(params, num_thermo)
+    tau = tau_out_all(tab, num_thermo, aexp)
+    U = int(uk0.numel())
+    bi = torch.as_tensor(np.concatenate([np.tile(np.arange(offs[c], offs[c + 1]), 2) for c in range(C)]).astype(np.int64), device=dev)
+    uc = torch.as_tensor(np.concatenate([np.repeat(2 * c + np.arange(2), nks[c]) for c in range(C)]).astype(np.int32), device=dev)
+    yr, st, _, _ = evolve_units(tab, num_thermo, uc, uk0[bi].contiguous(), tau, **lm, **solver, schedule=REPLAY_SCHEDULE,
+                                replay=(S["mesh"][bi].contiguous(), S["mlen"][bi].contiguous()))
+    _raise_on_status(st, allow_phantom=True)
+    pkr = power_units(tab, num_thermo, uc, uk0[bi].contiguous(), yr, idx)
+    start = np.concatenate([[0], np.cumsum(2 * nks)])[:-1]
+    ucn = np.repeat(np.arange(C), nks)
+    pos = np.arange(U) - np.repeat(offs[:-1], nks)
+    plus = torch.as_tensor(start[ucn] + pos, device=dev)
+    minus = torch.as_tensor(start[ucn] + nks[ucn] + pos, device=dev)
+    d = torch.as_tensor(steps[ucn], dtype=torch.float64, device=dev)
+    return (pkr[plus] - pkr[minus]) / (2.0 * d[:, None])
 
 
 def param_to_theta(param, tau_reion=0.054):
@@ -325,6 +366,8 @@ def pk_jacobian_many(thetas, base, kmodes, aexp_out=(1.0,), idx=4, rel_step=REL_
     kp = torch.tensor([p.get("k_p", 0.05) for p in S["p0s"]], dtype=torch.float64, device=dev)[uc0.to(torch.int64)]
     J[:, :, 3] = pku * torch.log(uk0 / kp)[:, None]
     J[:, :, 4] = pku
+    if _reionized(base):
+        J[:, :, 8] = _reion_column(S, thetas, base, aexp, lm, solver, num_thermo, rel_step, idx)
     for c in range(C):
         if bad[c]:
             pku[offs[c]:offs[c + 1]] = nan
@@ -369,6 +412,9 @@ def pk_and_vjp_many(thetas, base, kmodes, cotangents, aexp_out=(1.0,), idx=4, re
     cp = (cot * pku).sum(dim=1)
     v[:, 3] = torch.zeros(C, dtype=torch.float64, device=dev).index_add_(0, ucl, cp * torch.log(uk0 / kp))
     v[:, 4] = torch.zeros(C, dtype=torch.float64, device=dev).index_add_(0, ucl, cp)
+    if _reionized(base):
+        dr = (cot * _reion_column(S, thetas, base, aexp, lm, solver, num_thermo, rel_step, idx)).sum(dim=1)
+        v[:, 8] = torch.zeros(C, dtype=torch.float64, device=dev).index_add_(0, ucl, dr)
     status = np.where(bad, GRAD_PHANTOM, GRAD_OK).astype(np.int32)
     lam = [c for c in range(C) if _is_lambda(thetas[c]) and not bad[c]]
     if lam:
@@ -448,6 +494,10 @@ class PkFunction(torch.autograd.Function):
         logk = torch.log(S["uk0"] / S["p0s"][0].get("k_p", 0.05))[:, None]
         v[3] = (cot * pku * logk).sum()
         v[4] = (cot * pku).sum()
+        f = ctx.fd
+        if _reionized(f["base"]):
+            v[8] = (cot * _reion_column(S, ctx.th, f["base"], f["aexp_out"], ctx.lm, f["solver"], ctx.num_thermo, f["rel_step"],
+                                        ctx.idx)).sum()
         if _is_lambda(ctx.th[0]):
             f = ctx.fd
             _, Jf = pk_jacobian(ctx.th[0], f["base"], f["kmodes"], aexp_out=f["aexp_out"], idx=f["idx"], rel_step=f["rel_step"],
@@ -493,13 +543,16 @@ def cell_vjp(theta, base, kmodes, weights, *, ellmax=2500, aexp_out=None, n_fftl
     shifted cosmologies of the six parameters that enter the background and the perturbations
     (omega_c, omega_b, h, m_nu, w0, wa), each taken by a central difference of the whole chain
     (background, sources on the frozen mesh, line of sight, C_l); n_s by the exact cross
-    spectrum of Theta_l and ln(k / k_p) Theta_l; ln(1e10 A_s) and tau_reion do not enter C_l as
-    the reference defines it (A_s enters D_l).  weights: for l = 2..ellmax or for every l.
+    spectrum of Theta_l and ln(k / k_p) Theta_l; ln(1e10 A_s) does not enter C_l as the reference
+    defines it (A_s enters D_l); tau_reion enters only with reionization switched on in base
+    (base["tau_reion"] > 0; then by a central difference like the six).  weights: for l =
+    2..ellmax or for every l.
     Returns (L, grad (9,)) as a float and a device tensor, plus C_l when return_cell."""
     from .background import evolve_background
-    from .cmb import cmb_aexp_out, compute_Cell, compute_sources_fused, compute_theta_ell
+    from .cmb import compute_Cell, compute_sources_fused, compute_theta_ell
     theta = np.asarray(theta, dtype=np.float64).reshape(-1)
-    aexp = cmb_aexp_out() if aexp_out is None else np.asarray(aexp_out, dtype=np.float64)
+    from .cmb import _driver_aexp
+    aexp = _driver_aexp(base, aexp_out)
     lm = dict(lmaxg=lmaxg, lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax)
     dev = _lib.device()
     kk = torch.as_tensor(np.asarray(kmodes, dtype=np.float64), device=dev).reshape(-1).contiguous()
@@ -519,7 +572,7 @@ def cell_vjp(theta, base, kmodes, weights, *, ellmax=2500, aexp_out=None, n_fftl
     mesh, mlen = _accepted_mesh(src0["step_log"], src0["n_accept"], t1)
     del src0
     g = torch.zeros(9, dtype=torch.float64, device=dev)
-    for j in ODE_PARAMS:
+    for j in ODE_PARAMS + ((8,) if _reionized(base) else ()):
         d = rel_step * max(abs(theta[j]), 1e-2)
         Ls = []
         for th in _shift_pair(theta, j, d):
