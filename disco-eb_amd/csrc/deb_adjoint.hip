@@ -237,6 +237,104 @@ __host__ __device__ constexpr long adj_extra_doubles(long n, int nt) {
 }
 
 // ---------------------------------------------------------------------------------------
+// per-step record of the forward replay: everything the reverse sweep reads of a step's
+// stages and factorisation -- K (8 n), J0, dT, the chains' rtd, tlo, tmul (5 n), the chain
+// heads' parent entries of Jh (nch) and the dense part: with a compile-time bin count the
+// reduced inverse (NRED^2), Q and the neutrino columns of G (2 x 2 NN), the core rows' neutrino
+// columns of the Schur block (NCORE NN), the 3 x 3 bin inverses (9 NQ) and the pivot maps
+// (2 NRED); otherwise the m x m inverse and its pivot map (m^2 + m).  Stored during the forward
+// replay and read back in the reverse sweep instead of recomputing the step (the same values).
+// ---------------------------------------------------------------------------------------
+__host__ __device__ constexpr long adj_rec_doubles(long n, long m, long nq, long nch, bool reduced) {
+  const long dense = reduced ? NRED * NRED + 4 * 3 * nq + NCORE * 3 * nq + 9 * nq + 2 * NRED : m * m + m;
+  return (13 * n + nch + dense + 3) / 4 * 4;
+}
+// the bin counts with a compile-time instantiation (the bordered reduction)
+__host__ __device__ constexpr bool adj_reduced(int nq) { return nq == 3 || nq == 5 || nq == 8; }
+
+template <int NQ, int NT> __device__ __forceinline__ void save_record(const PLay &L, const Ws &W, double *__restrict__ rec, int tid) {
+  const int n = L.n, m = L.m, nch = L.nch;
+  for (int p = tid; p < 8 * n; p += NT) rec[p] = W.K[p];
+  double *r = rec + 8 * n;
+  for (int p = tid; p < n; p += NT) {
+    r[p] = W.J0[p];
+    r[n + p] = W.dT[p];
+    r[2 * n + p] = W.rtd[p];
+    r[3 * n + p] = W.tlo[p];
+    r[4 * n + p] = W.tmul[p];
+  }
+  r += 5 * n;
+  for (int c = tid; c < nch; c += NT) r[c] = W.Jh[c * m + W.cpar[c]];
+  r += nch;
+  if constexpr (NQ > 0) {
+    constexpr int NN = 3 * NQ;
+    for (int p = tid; p < NRED * NRED; p += NT) r[p] = W.Sinv[p];
+    r += NRED * NRED;
+    for (int p = tid; p < 2 * NN; p += NT) {
+      r[p] = W.Q[p];
+      r[2 * NN + p] = W.G[30 + p];  // G[2 (15 + jn) + t]
+    }
+    r += 4 * NN;
+    for (int p = tid; p < NCORE * NN; p += NT) {
+      const int i = p / NN, jn = p - i * NN;
+      r[p] = W.S[core_index(i, NQ) * m + 15 + jn];
+    }
+    r += NCORE * NN;
+    for (int p = tid; p < 9 * NQ; p += NT) r[p] = W.Minv[p];
+    r += 9 * NQ;
+    for (int p = tid; p < NRED; p += NT) {
+      r[p] = (double)W.rmap[p];
+      r[NRED + p] = (double)W.pi[p];
+    }
+  } else {
+    for (int p = tid; p < m * m; p += NT) r[p] = W.Sinv[p];
+    for (int p = tid; p < m; p += NT) r[m * m + p] = (double)W.pi[p];
+  }
+}
+
+template <int NQ, int NT> __device__ __forceinline__ void load_record(const PLay &L, Ws &W, const double *__restrict__ rec, int tid) {
+  const int n = L.n, m = L.m, nch = L.nch;
+  for (int p = tid; p < 8 * n; p += NT) W.K[p] = rec[p];
+  const double *r = rec + 8 * n;
+  for (int p = tid; p < n; p += NT) {
+    W.J0[p] = r[p];
+    W.dT[p] = r[n + p];
+    W.rtd[p] = r[2 * n + p];
+    W.tlo[p] = r[3 * n + p];
+    W.tmul[p] = r[4 * n + p];
+  }
+  r += 5 * n;
+  for (int c = tid; c < nch; c += NT) W.Jh[c * m + W.cpar[c]] = r[c];
+  r += nch;
+  if constexpr (NQ > 0) {
+    constexpr int NN = 3 * NQ;
+    W.Sinv = W.R;
+    for (int p = tid; p < NRED * NRED; p += NT) W.Sinv[p] = r[p];
+    r += NRED * NRED;
+    for (int p = tid; p < 2 * NN; p += NT) {
+      W.Q[p] = r[p];
+      W.G[30 + p] = r[2 * NN + p];
+    }
+    r += 4 * NN;
+    for (int p = tid; p < NCORE * NN; p += NT) {
+      const int i = p / NN, jn = p - i * NN;
+      W.S[core_index(i, NQ) * m + 15 + jn] = r[p];
+    }
+    r += NCORE * NN;
+    for (int p = tid; p < 9 * NQ; p += NT) W.Minv[p] = r[p];
+    r += 9 * NQ;
+    for (int p = tid; p < NRED; p += NT) {
+      W.rmap[p] = (int)r[p];
+      W.pi[p] = (int)r[NRED + p];
+    }
+  } else {
+    W.Sinv = W.S;
+    for (int p = tid; p < m * m; p += NT) W.Sinv[p] = r[p];
+    for (int p = tid; p < m; p += NT) W.pi[p] = (int)r[m * m + p];
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // transposed structured solve W^T x = b (v in LDS, in place), wavefront 0 alone (tid = lane);
 // the factorisation of the forward step (factorise / reduce_dense / gauss_jordan_wave) is read:
 // W = I/(gamma h) - J with row 0 [W00, 0...], column 0 -J0, the tail chains T = U L (bottom-up
@@ -588,7 +686,8 @@ __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict_
   const double t0 = 0.99 * fmin(tmin, starting_time(b, k));
   const double *rs = mesh + (long)unit * mstride;
   const int N = mesh_len[unit];
-  if (N < 1 || N > mstride || (long)(N + 1) * n > cstride) {  // the mesh row or the checkpoint slot would overflow
+  const long rsz = adj_rec_doubles(n, L.m, L.nq, L.nch, NQ > 0);
+  if (N < 1 || N > mstride || (long)(N + 1) * n + (long)N * rsz > cstride) {  // the mesh row or the checkpoint slot would overflow
     if (tid < NPAR) gu[tid] = __builtin_nan("");
     if (tid == 0) status[unit] = 7;
     return;
@@ -596,7 +695,8 @@ __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict_
   auto mesh_s = [&](int i) { return i == 0 ? 0.0 : (rs[i - 1] >= 1.0 ? 1.0 : rs[i - 1]); };  // point i of 0..N
   auto mesh_t = [&](int i) { return i == 0 ? t0 : (rs[i - 1] >= 1.0 ? t1 : t0 + rs[i - 1] * (t1 - t0)); };
   double *ck = ckpt + (long)unit * cstride;
-  // 1. forward replay with checkpoints
+  double *rec0 = ck + (long)(N + 1) * n;  // step i's record at rec0 + i rsz
+  // 1. forward replay with checkpoints: the state at every step start and the step's record
   if (tid == 0) initial_conditions(b, L, t0, k, W.y0);
   __syncthreads();
   int gsrc = 0;
@@ -604,6 +704,7 @@ __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict_
     for (int p = tid; p < n; p += NT) ck[(long)i * n + p] = W.y0[p];
     const double tp = mesh_t(i), tn = mesh_t(i + 1);
     rodas5_attempt<NQ, NT>(b, L, k, tp, tn - tp, W, tid, gsrc, tn);
+    save_record<NQ, NT>(L, W, rec0 + (long)i * rsz, tid);
     for (int p = tid; p < n; p += NT) W.y0[p] = W.y1[p];
     __syncthreads();
     gsrc = (NT > WAVE) ? 1 : 0;
@@ -681,8 +782,10 @@ __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict_
       __syncthreads();
       --jout;
     }
-    // recompute the step from its checkpoint (factorisation, stages k_s)
-    rodas5_attempt<NQ, NT>(b, L, k, T0, h, W, tid, 0);
+    // the step's stages and factorisation from its record
+    __syncthreads();
+    load_record<NQ, NT>(L, W, rec0 + (long)i * rsz, tid);
+    __syncthreads();
     const double W00 = 1.0 / (h * gamma) - W.J0[0];
     double *K = W.K;
     for (int p = tid; p < n; p += NT) {
@@ -980,6 +1083,11 @@ __global__ void dtab_kernel(const double *__restrict__ shtab, const double *__re
 
 }  // namespace
 }  // namespace deb
+
+extern "C" long deb_adjoint_record_doubles(const DebProblem *prob) {
+  const long n = 7 + (prob->lmaxg + 1) + (prob->lmaxgp + 1) + (prob->lmaxr + 1) + (long)prob->nqmax * (prob->lmaxnu + 1) + 2;
+  return deb::adj_rec_doubles(n, 17 + 3L * prob->nqmax, prob->nqmax, 3 + prob->nqmax, deb::adj_reduced(prob->nqmax));
+}
 
 extern "C" long deb_adjoint_lds_bytes(const DebProblem *prob, int nt) {
   const long n = 7 + (prob->lmaxg + 1) + (prob->lmaxgp + 1) + (prob->lmaxr + 1) + (long)prob->nqmax * (prob->lmaxnu + 1) + 2;

@@ -36,6 +36,7 @@ extern "C" int deb_launch_evolve_sources(const double *, int, const double *, in
                                          const DebSolverCfg *, int, double *, int *, int *, int *, double *, const int *, int, double *,
                                          const double *, long, const int *, hipStream_t);
 extern "C" long deb_adjoint_lds_bytes(const DebProblem *, int);
+extern "C" long deb_adjoint_record_doubles(const DebProblem *);
 extern "C" int deb_launch_adjoint(const double *, int, const double *, double *, const double *, int, const int *, const double *, int,
                                   const double *, const double *, int, const double *, const DebProblem *, const double *, long, const int *,
                                   const double *, double *, long, double *, double *, double *, int *, const int *, hipStream_t);
@@ -192,7 +193,7 @@ int deb_evolve_replay(const double *d_tables, int num_thermo, const int *d_unit_
 long deb_adjoint_workspace_doubles(int n_units, int n_cosmo, int num_thermo, int max_mesh_len, const DebProblem *prob) {
   if (!prob || n_units < 0 || n_cosmo < 0 || max_mesh_len < 1) return -1;
   const long n = deb_nvar(prob->lmaxg, prob->lmaxgp, prob->lmaxr, prob->lmaxnu, prob->nqmax);
-  return 32 + 6L * n_cosmo * deb_table_doubles(num_thermo) + (long)n_units * (max_mesh_len + 1) * n;
+  return 32 + 6L * n_cosmo * deb_table_doubles(num_thermo) + (long)n_units * ((max_mesh_len + 1) * n + (long)max_mesh_len * deb_adjoint_record_doubles(prob));
 }
 
 int deb_evolve_adjoint(const double *d_tables, int num_thermo, int n_cosmo, const double *d_shifted_tables, const double *d_dtheta,
@@ -217,7 +218,8 @@ int deb_evolve_adjoint(const double *d_tables, int num_thermo, int n_cosmo, cons
   double *dtab = d_work + 32;
   double *ckpt = dtab + 6L * n_cosmo * deb_table_doubles(num_thermo);
   rc = deb_launch_adjoint(d_tables, num_thermo, d_shifted_tables, dtab, d_dtheta, n_cosmo, d_unit_cosmo, d_unit_k, n_units, d_tau_out,
-                          d_tau_out_shifted, nout, d_work, prob, d_mesh, mesh_stride, d_mesh_len, d_gbar, ckpt, (long)(max_mesh_len + 1) * n,
+                          d_tau_out_shifted, nout, d_work, prob, d_mesh, mesh_stride, d_mesh_len, d_gbar, ckpt,
+                          (long)(max_mesh_len + 1) * n + (long)max_mesh_len * deb_adjoint_record_doubles(prob),
                           d_grad, d_lam0, d_tbar, d_status, sched ? sched->d_order : nullptr, (hipStream_t)stream);
   if (rc == -2) return fail(-1, "state too large for LDS");
   if (rc == -5) return fail(-1, "the adjoint takes 1..64 output times");

@@ -154,14 +154,15 @@ def _shift_pair(theta, j, d):
 
 
 def evolve_units_adjoint(tables, num_thermo, shifted, dtheta, unit_cosmo, unit_k, tau_out, tau_out_shifted, mesh, mesh_len, gbar, *,
-                         lmaxg=12, lmaxgp=12, lmaxr=12, lmaxnu=17, nqmax=8, max_ckpt_bytes=8 << 30, return_parts=False):
+                         lmaxg=12, lmaxgp=12, lmaxr=12, lmaxnu=17, nqmax=8, max_ckpt_bytes=32 << 30, return_parts=False):
     """Low-level launch of the discrete adjoint (deb_evolve_adjoint) over n_units units.
 
     tables (C, size) and shifted (C * 12, size) background tables, shifted block 2 p (+) and 2 p + 1
     (-) of cosmology c at theta_p +- dtheta[c, p]; tau_out (C, nout), tau_out_shifted (C * 12, nout);
     mesh / mesh_len as evolve_units(replay=...); gbar (n_units, nout, 20) output-field cotangents.
-    Units are launched largest k first, in consecutive chunks of that order whose checkpoints
-    ((chunk's longest mesh + 1) x nvar doubles per unit) fit max_ckpt_bytes.
+    Units are launched largest k first, in consecutive chunks of that order whose checkpoints (per
+    unit the states at the chunk's longest mesh's step starts and one record of stages and
+    factorisation per step, deb_adjoint_workspace_doubles) fit max_ckpt_bytes.
     Returns (grad (n_units, 6), status (n_units,)), plus (lam0 (n_units, nvar), tbar (n_units, 2))
     when return_parts is set."""
     L = _lib.lib()
@@ -190,11 +191,12 @@ def evolve_units_adjoint(tables, num_thermo, shifted, dtheta, unit_cosmo, unit_k
     tsh = tau_out_shifted.to(device=dev, dtype=torch.float64).contiguous()
     order = torch.argsort(uk, descending=True, stable=True)
     ml = mlen[order].cpu().numpy().astype(np.int64) if U else np.zeros(0, dtype=np.int64)
-    per = n * 8
+    base_ws = L.deb_adjoint_workspace_doubles(0, C, int(num_thermo), 1, ctypes.byref(prob))
+    per_unit = lambda m: (L.deb_adjoint_workspace_doubles(1, C, int(num_thermo), m, ctypes.byref(prob)) - base_ws) * 8
     s0 = 0
     while s0 < U:  # greedy chunks over the k-descending order
         m, s1 = int(ml[s0]), s0 + 1
-        while s1 < U and (s1 - s0 + 1) * (max(m, int(ml[s1])) + 1) * per <= max_ckpt_bytes:
+        while s1 < U and (s1 - s0 + 1) * per_unit(max(m, int(ml[s1]))) <= max_ckpt_bytes:
             m = max(m, int(ml[s1]))
             s1 += 1
         idx = order[s0:s1]

@@ -147,7 +147,8 @@ int deb_evolve_replay(const double *d_tables, int num_thermo, const int *d_unit_
  * d_grad: n_units x 6, written (NaN and status 4 for a unit whose w(a) crosses -1, status 7 for a
  *   unit whose mesh_len is outside 1..max_mesh_len);
  * d_work: deb_adjoint_workspace_doubles(...) doubles (momentum bins, derivative tables,
- *   checkpoints); d_lam0 (n_units x nvar) and d_tbar (n_units x 2), optional (NULL): the
+ *   checkpoints: per unit the state at every step start and a record of every step's stages and
+ *   factorisation, so the reverse sweep does not recompute the steps); d_lam0 (n_units x nvar) and d_tbar (n_units x 2), optional (NULL): the
  *   cotangents of the initial state and of the start and end times.  256 threads per unit.
  * Replaces the forward-mode derivative jax.jacfwd takes through diffeqsolve (DirectAdjoint,
  * src/discoeb/perturbations.py:904-925) and the Rodas5 step (src/discoeb/ode_integrators_stiff.py:

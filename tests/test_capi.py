@@ -54,7 +54,10 @@ def test_argument_errors_without_gpu(pkg):
     assert L.deb_evolve_adjoint(*args(65, 10, 10)) < 0 and b"64" in L.deb_last_error()
     assert L.deb_evolve_adjoint(*args(1, 5, 10)) < 0 and b"mesh_stride" in L.deb_last_error()
     assert L.deb_evolve_adjoint(*args(1, 10, 10)) < 0 and b"NULL" in L.deb_last_error()
-    assert L.deb_adjoint_workspace_doubles(4, 1, 256, 10, ctypes.byref(prob)) == 32 + 6 * L.deb_tables_size(256) + 4 * 11 * 192
+    # per unit: 11 checkpointed states and 10 step records (13 n + nch + the reduced dense part, rounded to 4)
+    rec = 13 * 192 + 11 + (19 * 19 + 4 * 24 + 17 * 24 + 9 * 8 + 2 * 19)
+    rec = (rec + 3) // 4 * 4
+    assert L.deb_adjoint_workspace_doubles(4, 1, 256, 10, ctypes.byref(prob)) == 32 + 6 * L.deb_tables_size(256) + 4 * (11 * 192 + 10 * rec)
     # large-scale structure: rejected before any launch
     assert L.deb_savgol_coefficients(3, 3, None, None) < 0 and b"exceed" in L.deb_last_error()
     assert L.deb_savgol_coefficients(11, 9, None, None) < 0
