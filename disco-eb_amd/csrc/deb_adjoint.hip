@@ -229,11 +229,12 @@ struct AdjWs {
   double *sc;    // scalars: see SC_*
   double *taub;  // MAXOUT: cotangents of the output times
   double *gb;    // 20: output-field cotangent of the current output
+  int *perm64;   // n + 256: the right-hand side rows in class-grouped order for one wavefront (rows_dot)
 };
 enum { SC_T0B, SC_HB, SC_TSB, SC_TEB, SC_T0BAR, SC_T1BAR, SC_G0, SC_COUNT = SC_G0 + NPAR };
 
 __host__ __device__ constexpr long adj_extra_doubles(long n, int nt) {
-  return 18 * n + (nt / WAVE) * 128 + 64 + (NQMAX + 3) + (3 * NQMAX + 17) + 2 * NRED + SC_COUNT + MAXOUT + 20 + 8;
+  return 18 * n + (nt / WAVE) * 128 + 64 + (NQMAX + 3) + (3 * NQMAX + 17) + 2 * NRED + SC_COUNT + MAXOUT + 20 + 8 + (n + 256 + 1) / 2;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -443,6 +444,188 @@ __device__ __forceinline__ void wsolveT(const PLay &L, Ws &W, AdjWs &A, double W
 }
 
 // ---------------------------------------------------------------------------------------
+// Transposed linear right-hand side, dense rows: sum_i c_i A_iq over the dense-block rows i for
+// one column q (a dense-block entry or a chain head), where A = d f / d y at the background g
+// (f is linear in y_1..).  Transposed by hand from metric() and core_row(): the rows' cotangents
+// are carried back to the metric's building blocks (h', eta', alpha, the density, pressure and
+// velocity sums, the photon and neutrino fields with their radiation-streaming and
+// tight-coupling forms), and each column collects its direct terms and its share of those.  The
+// same values as evaluating the rows on the unit vector e_q, up to rounding.
+// ---------------------------------------------------------------------------------------
+// vj, ivj: the velocity factor of column q's momentum bin and its reciprocal (read from the bin
+// lanes by the caller, in control flow every lane of the wave takes; any value for other columns)
+template <class T, class BV>
+__device__ __forceinline__ T dense_colT(const BV &b, const PLay &L, double k, const PBg<T> &g, const double *c, int q, const T &vj,
+                                        const T &ivj) {
+  const int n = L.n, nq = L.nq, iq0 = L.iq0, idxgp = L.idxgp, idxr = L.idxr;
+  const double k2 = k * k, ik2 = 1.0 / (k * k), sk2 = fmax(k * k, 1e-30);
+  const auto Omegac = b.Omegam - b.Omegab;
+  const double cs2_Q = b.cs2DE;
+  const bool rsa = g.rsa, tca = g.tca, ufa = g.ufa;
+  const bool gam = !rsa && !tca;  // photon l <= 2 and polarisation rows in their full form
+  // 1. rows -> the metric's fields
+  const double c1 = c[1], c2 = c[2], c3 = c[3], c5 = c[5], c6 = c[6], c7 = c[7], c8 = c[8], c9 = c[9];
+  T hp(0.0), ep(0.0), al(0.0), dgrho(0.0), dgpres(0.0), dg(0.0), tg(0.0), sg(0.0), dr(0.0), tr(0.0), sr(0.0);
+  T tgp(0.0), tbp(0.0), dgp(0.0), dbp(0.0), tbp0(0.0), slip(0.0);
+  dgrho = dgrho - g.a * c1;
+  dgpres = dgpres - 3.0 * g.a * c1;
+  ep = ep + c2;
+  hp = hp - 0.5 * c3 - 0.5 * c5;
+  if (tca) tbp = tbp + c6;
+  else tg = tg + g.pb43 * g.opac * c6;
+  if (!rsa) {
+    if (tca) {
+      dgp = dgp + c7;
+      tgp = tgp + c8;
+    } else {
+      tg = tg - 4.0 / 3.0 * c7 + 8.0 / 15.0 * c9 - g.opac * c8;
+      hp = hp - 2.0 / 3.0 * c7;
+      dg = dg + 0.25 * k2 * c8;
+      sg = sg - k2 * c8;
+      al = al + 8.0 / 15.0 * k2 * c9;
+    }
+    const double cr0 = c[idxr], cr1 = c[idxr + 1], cr2 = c[idxr + 2];
+    tr = tr - 4.0 / 3.0 * cr0;
+    hp = hp - 2.0 / 3.0 * cr0;
+    dr = dr + 0.25 * k2 * cr1;
+    sr = sr - k2 * cr1;
+    if (ufa) {
+      sr = sr - 3.0 / dmax(g.tau, 1e-30) * cr2;
+      tr = tr + 4.0 / 15.0 * cr2;
+      hp = hp + 2.0 / 15.0 * cr2;
+    } else {
+      tr = tr + 8.0 / 15.0 * cr2;
+      al = al + 8.0 / 15.0 * k2 * cr2;
+    }
+  }
+  for (int j = 0; j < nq; ++j) {
+    const double dl = L.dlfdlq[j], c0q = c[iq0 + j], c2q = c[iq0 + 2 * nq + j];
+    hp = hp + dl * (1.0 / 6.0) * c0q - dl * (1.0 / 15.0) * c2q;
+    ep = ep - 0.4 * dl * c2q;
+  }
+  if (!b.lambda_de) hp = hp - 0.5 * (1.0 + g.w_Q) * c[n - 2];
+  if (tca) {  // m.tgp_tca = (-tbp - aH tb + k2 cs2 db) ipbm + ...; tbp = tbp0 + pb43 ipb slip; slip (.. dbp, dgp ..)
+    tbp = tbp - g.ipbm * tgp;
+    tbp0 = tbp;
+    slip = g.pb43 * g.ipb * tbp;
+    dbp = dbp + g.tauc * k2 * g.cs2 * g.ipb * slip;
+    dgp = dgp - 0.25 * g.tauc * k2 * g.ipb * slip;
+    hp = hp - 2.0 / 3.0 * dgp - 0.5 * dbp;
+  }
+  // 2. alpha = (h' + 6 eta') / (2 k^2); eta' = dgtheta / (2 k^2)
+  hp = hp + 0.5 * ik2 * al;
+  ep = ep + 3.0 * ik2 * al;
+  const T dgth = 0.5 * ik2 * ep;
+  tg = tg + 4.0 / 3.0 * b.grhog * g.ia2 * dgth;
+  tr = tr + 4.0 / 3.0 * b.Neff * b.grhor * g.ia2 * dgth;
+  // 3. pressure: dg, dr, and the neutrino, dark-energy density and velocity parts
+  dg = dg + b.grhog * g.ia2 * (1.0 / 3.0) * dgpres;
+  dr = dr + b.grhor * b.Neff * g.ia2 * (1.0 / 3.0) * dgpres;
+  // 4. h': from the full density sum, or under RSA from its form without photons and neutrinos
+  T hprsa(0.0), c_eta(0.0);
+  if (rsa) {
+    hprsa = hp;
+    const T stc = dmax(g.tauc, 1e-30);
+    const T f3 = 3.0 / (sk2 * stc);
+    hprsa = hprsa + (4.0 / sk2 * g.aH + 4.0 / (sk2 * stc) * 0.5) * dg + (-0.5 + f3 * (-g.taucprime / stc) * 0.5) * tg + 4.0 / sk2 * g.aH * dr -
+            0.5 * tr;
+    c_eta = c_eta - 4.0 * dg + f3 * sk2 * tg - 4.0 * dr;
+  } else {
+    dgrho = dgrho + g.iaH * hp;
+    c_eta = c_eta + 2.0 * k2 * g.iaH * hp;
+  }
+  const T wo = g.iaH * hprsa;
+  c_eta = c_eta + 2.0 * k2 * g.iaH * hprsa;
+  const T dens = dgrho + wo;  // the matter and neutrino density terms enter both sums alike
+  // 5. the column
+  if (q == 2) return c_eta;
+  if (q == 3) return b.grhom * Omegac * g.ia * dens;
+  if (q == 4) return -c3 - g.aH * c[4] + b.grhom * Omegac * g.ia * dgth;
+  if (q == 5) {
+    T v = b.grhom * b.Omegab * g.ia * dens;
+    if (!tca) v = v + k2 * g.cs2 * c6;
+    if (rsa) v = v + 3.0 / (sk2 * dmax(g.tauc, 1e-30)) * g.cs2 * sk2 * tg;
+    if (tca) v = v + k2 * g.cs2 * g.ipbm * tgp + k2 * g.cs2 * g.ipb * tbp0;
+    return v;
+  }
+  if (q == 6) {
+    T v = -c5 + b.grhom * b.Omegab * g.ia * dgth;
+    if (!tca) v = v + (-g.aH - g.pb43 * g.opac) * c6;
+    if (gam) v = v + g.opac * c8;
+    if (rsa) {
+      const T stc = dmax(g.tauc, 1e-30);
+      v = v + 4.0 / (sk2 * stc) * dg + 3.0 / (sk2 * stc) * (-g.taucprime / stc - g.aH) * tg;
+    }
+    if (tca) v = v - g.aH * g.ipbm * tgp - g.aH * g.ipb * tbp0 + (2.0 * g.pb43 * g.ipb * g.aH - g.tauc * g.aHH * g.ipb) * slip - dbp;
+    return v;
+  }
+  if (q == 7) {
+    T v = b.grhog * g.ia2 * dgrho;
+    if (!rsa) v = v + dg;
+    if (tca) v = v + 0.25 * k2 * tgp + 0.25 * k2 * g.pb43 * g.ipb * tbp0 - 0.5 * g.tauc * g.aH * k2 * g.ipb * slip;
+    return v;
+  }
+  if (q == 8) {
+    T v(0.0);
+    if (!rsa) v = v + tg;
+    if (tca) v = v - 2.0 * g.pb43 * g.ipb * g.aH * slip - 4.0 / 3.0 * dgp;
+    return v;
+  }
+  const double cg0 = c[idxgp], cg1 = c[idxgp + 1], cg2 = c[idxgp + 2];
+  if (q == 9) {
+    T v(0.0);
+    if (gam) v = v + (-g.opac + 0.1 * g.opac) * c9 + g.opac * 0.5 * cg0 + 0.1 * g.opac * cg2;
+    if (!rsa) v = v + 0.5 * sg;
+    if (tca) v = v - 0.5 * k2 * tgp - 0.5 * k2 * g.pb43 * g.ipb * tbp0;
+    return v;
+  }
+  if (q == 10) return gam ? T(-3.0 / 5.0 * k * c9) : T(0.0);
+  if (q >= idxgp && q <= idxgp + 3) {
+    if (!gam) return T(0.0);
+    const int l = q - idxgp;
+    if (l == 0) return (-g.opac + g.opac * 0.5) * cg0 + k * (1.0 / 3.0) * cg1 + 0.1 * g.opac * cg2 + 0.1 * g.opac * c9;
+    if (l == 1) return T(-k * cg0 + k * 0.2 * 2.0 * cg2) - g.opac * cg1;
+    if (l == 2) return 0.1 * g.opac * c9 + g.opac * 0.5 * cg0 - 2.0 * k * (1.0 / 3.0) * cg1 + (-g.opac + 0.1 * g.opac) * cg2;
+    return T(-k * 0.2 * 3.0 * cg2);
+  }
+  if (q >= idxr && q <= idxr + 3) {
+    const int l = q - idxr;
+    if (l == 0) {
+      T v = b.grhor * b.Neff * g.ia2 * dgrho;
+      if (!rsa) v = v + dr;
+      return v;
+    }
+    if (rsa) return T(0.0);
+    if (l == 1) return tr;
+    if (l == 2) return 0.5 * sr;
+    return ufa ? T(0.0) : T(-0.6 * k * c[idxr + 2]);
+  }
+  if (q >= iq0 && q < iq0 + 4 * nq) {
+    const int l = (q - iq0) / nq, j = (q - iq0) - l * nq;
+    const T v = vj, kv = k * v;
+    if (l == 0)
+      return kv * (1.0 / 3.0) * c[iq0 + nq + j] + L.w[j] * ivj * (b.grhor * b.Nmnu * g.ia2 * dens) +
+             L.w[j] * v * (1.0 / 3.0) * (b.grhor * b.Nmnu * g.ia2 * dgpres);
+    if (l == 1) return -kv * c[iq0 + j] + kv * 0.2 * 2.0 * c[iq0 + 2 * nq + j] + L.w[j] * (b.Nmnu * b.grhor * k * g.ia2 * dgth);
+    if (l == 2) return -kv * (2.0 / 3.0) * c[iq0 + nq + j];
+    return -kv * 0.2 * 3.0 * c[iq0 + 2 * nq + j];
+  }
+  const T a2 = g.a * g.a;
+  if (q == n - 2) {
+    T v = b.grhom * b.OmegaDE * g.rho_Q * a2 * (dens + cs2_Q * dgpres);
+    if (!b.lambda_de) v = v - 3.0 * (cs2_Q - g.w_Q) * g.aH * c[n - 2] + cs2_Q / (1.0 + g.w_Q) * k2 * c[n - 1];
+    return v;
+  }
+  if (q == n - 1) {
+    T v = (1.0 + g.w_Q) * g.rho_Q * b.grhom * b.OmegaDE * a2 * (dgth + (cs2_Q - g.ca2_Q) * (3.0 * g.aH * ik2) * dgpres);
+    if (!b.lambda_de)
+      v = v + (-(1.0 + g.w_Q) - 9.0 * (1.0 + g.w_Q) * (cs2_Q - g.ca2_Q) * g.aH * g.aH / k2) * c[n - 2] - (1.0 - 3.0 * cs2_Q) * g.aH * c[n - 1];
+    return v;
+  }
+  return T(0.0);
+}
+
+// ---------------------------------------------------------------------------------------
 // out[q] += sum_r rb[r] A_rq (q >= 1) for the linear part of the right-hand side at the
 // background g: dense columns by unit-vector evaluation of the dense rows (row group per
 // wavefront), tail-chain entries from the chain coefficients.  All threads.
@@ -459,27 +642,32 @@ __device__ __forceinline__ void colT(const BV &b, const PLay &L, double k, const
     if constexpr (IsDouble<T>::value) return rb[p] * val(v);
     else return rb[p] * v.d[0] + rb2[p] * v.d[1];
   };
-  for (int col = lane; col < m + nch; col += WAVE) {
-    const int s = (col < m) ? W.dmap[col] : W.chead[col - m];
-    auto ev = [&](int i) { return i == s ? 1.0 : 0.0; };
-    Met<T> mt;
-    metric<T>(b, L, k, g, ev, mt);
-    auto vsel = [&](int q) { return g.vq_l(L, q); };
-    double acc = 0.0;
-    for (int r = rg; r < m; r += NR) {
-      if (col >= m && r != W.cpar[col - m]) continue;
-      const int row = __builtin_amdgcn_readfirstlane(W.dmap[r]);
-      const T v = core_row<T>(row, b, L, k, g, mt, ev, vsel);
-      acc += wsum(v, row);
+  // dense-block and chain-head columns by the transposed rows (dense_colT), one column per
+  // thread, the columns spread over the wavefronts; with a dual background the two directions
+  // take their own cotangents (rb for a, rb2 for tau)
+  for (int c0 = 0; c0 < m + nch; c0 += NT) {  // a uniform trip count: the bin factors are read with every lane active
+    const int c = c0 + lane * NR + rg;
+    const int q = (c < m + nch) ? ((c < m) ? W.dmap[c] : W.chead[c - m]) : 0;
+    const int j = (q >= L.iq0 && q < L.iq0 + 4 * L.nq) ? (q - L.iq0) % L.nq : 0;
+    T vj = lane_bcast(g.vl, VQ_LANE0), ivj = lane_bcast(g.ivl, VQ_LANE0);
+    for (int jj = 1; jj < L.nq; ++jj) {
+      const T t = lane_bcast(g.vl, VQ_LANE0 + jj), ti = lane_bcast(g.ivl, VQ_LANE0 + jj);
+      if (j == jj) { vj = t; ivj = ti; }
     }
-    A.part[rg * 128 + col] = acc;
+    if (c < m + nch) {
+      double sum;
+      if constexpr (IsDouble<T>::value) {
+        sum = dense_colT<T>(b, L, k, g, rb, q, vj, ivj);
+      } else {
+        sum = dense_colT<T>(b, L, k, g, rb, q, vj, ivj).d[0] + dense_colT<T>(b, L, k, g, rb2, q, vj, ivj).d[1];
+      }
+      A.part[c] = sum;
+    }
   }
   __syncthreads();
-  for (int col = tid; col < m + nch; col += NT) {
-    double s = 0.0;
-    for (int q = 0; q < NR; ++q) s += A.part[q * 128 + col];
-    const int idx = (col < m) ? W.dmap[col] : W.chead[col - m];
-    out[idx] += s;
+  for (int c = tid; c < m + nch; c += NT) {
+    const int idx = (c < m) ? W.dmap[c] : W.chead[c - m];
+    out[idx] += A.part[c];
   }
   __syncthreads();
   // tail chains: f_i = lo_i y_{i-s} + di_i y_i + up_i y_{i+s}
@@ -546,22 +734,22 @@ __device__ __forceinline__ void dual_point(const DirCtx<ND> &c, double tau, doub
 
 template <int ND, class YF>
 __device__ __forceinline__ void rows_dot(const BgView &b, const DirCtx<ND> &c, const PLay &L, double k, const PBgX<Dn<ND>> &g, const YF &yv,
-                                         const double *r, int row_lo, Ws &W, int lane, double (&acc)[ND]) {
+                                         const double *r, int row_lo, Ws &W, const int *perm64, int lane, double (&acc)[ND]) {
   typedef Dn<ND> D;
   Met<D> m;
   metric<D>(c.bd, L, k, g, yv, m);
   const int n = L.n;
   const int nit = (n + WAVE - 1) / WAVE;
   for (int it = 0; it < nit; ++it) {  // uniform trip count: every lane takes part in the shuffles
-    const int i = lane + it * WAVE;
-    const int ic = i < n ? i : 0;
+    const int i = perm64[lane + it * WAVE];  // rows grouped by formula class, so that a wave takes few branches
+    const int ic = i >= 0 ? i : 0;
     const int src = VQ_LANE0 + W.rbin[ic];
     D vsh;
     vsh.v = __shfl(g.vl.v, src);
 #pragma unroll
     for (int l = 0; l < ND; ++l) vsh.d[l] = __shfl(g.vl.d[l], src);
     auto vrow = [&](int) { return vsh; };
-    if (i < n && i >= row_lo) {
+    if (i >= 0 && i >= row_lo) {
       const int rk = W.rk[i];
       D f;
       if (rk < 0) {
@@ -716,8 +904,11 @@ __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict_
   if (tid < SC_COUNT) A.sc[tid] = 0.0;
   if (tid < MAXOUT) A.taub[tid] = 0.0;
   const double *dtab_c = dtab + (long)cosmo * NPAR * tstride;
-  DirCtx<ND> dc;
-  dir_ctx<ND>(b, dtab_c, tstride, wave, dc);
+  // each wave's directions (derivative-table pointers, dual density parameters) in LDS: read at
+  // each use, so that they do not occupy registers across the sweep
+  __shared__ DirCtx<ND> s_dc[NT / WAVE];
+  if (lane == 0) dir_ctx<ND>(b, dtab_c, tstride, wave, s_dc[wave]);
+  const DirCtx<ND> &dc = s_dc[wave];
   __syncthreads();
   int jout = nout - 1;  // outputs are handled from the last
   for (int i = N - 1; i >= 0; --i) {
@@ -830,7 +1021,7 @@ __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict_
 #pragma unroll
         for (int l = 0; l < ND; ++l) acc[l] = 0.0;
         auto yv = [&](int ii) { return A.ust[ii]; };
-        rows_dot<ND>(b, dc, L, k, gd, yv, rb, 0, W, lane, acc);
+        rows_dot<ND>(b, dc, L, k, gd, yv, rb, 0, W, A.perm64, lane, acc);
 #pragma unroll
         for (int l = 0; l < ND; ++l) acc[l] = wave_sum(acc[l]);
         if (lane == 0)
@@ -914,7 +1105,7 @@ __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict_
       pbg_eval_ad<ND>(b, dc.bd, dc.dt, L, k, td, ad, gd, lane);
       for (int s = 0; s < 8; ++s) {
         auto yv = [&](int ii) { return K[s * n + ii]; };
-        rows_dot<ND>(b, dc, L, k, gd, yv, A.kb + s * n, 1, W, lane, acc);
+        rows_dot<ND>(b, dc, L, k, gd, yv, A.kb + s * n, 1, W, A.perm64, lane, acc);
       }
       // (c) d/da of rho^T df/d(dir) at (t0, (a, y0[1:])), central difference in a
       const double da = 1e-5 * a0;
@@ -924,10 +1115,10 @@ __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict_
       auto yv0 = [&](int ii) { return W.y0[ii]; };
       dual_point<ND>(dc, T0, a0 + da, td, ad);
       pbg_eval_ad<ND>(b, dc.bd, dc.dt, L, k, td, ad, gd, lane);
-      rows_dot<ND>(b, dc, L, k, gd, yv0, A.rho, 0, W, lane, accp);
+      rows_dot<ND>(b, dc, L, k, gd, yv0, A.rho, 0, W, A.perm64, lane, accp);
       dual_point<ND>(dc, T0, a0 - da, td, ad);
       pbg_eval_ad<ND>(b, dc.bd, dc.dt, L, k, td, ad, gd, lane);
-      rows_dot<ND>(b, dc, L, k, gd, yv0, A.rho, 0, W, lane, accm);
+      rows_dot<ND>(b, dc, L, k, gd, yv0, A.rho, 0, W, A.perm64, lane, accm);
 #pragma unroll
       for (int l = 0; l < ND; ++l) acc[l] += (accp[l] - accm[l]) / (2.0 * da);
 #pragma unroll
@@ -1060,10 +1251,13 @@ __global__ __launch_bounds__(NT) void adjoint_kernel(const double *__restrict__ 
   A.dTb = p; p += n; A.rho = p; p += n; A.pend = p; p += n; A.yn = p; p += n;
   A.part = p; p += (NT / WAVE) * 128; A.red = p; p += 64; A.cb = p; p += NQMAX + 3; A.xd2 = p; p += 3 * NQMAX + 17; A.z19 = p; p += 2 * NRED;
   A.sc = p; p += SC_COUNT; A.taub = p; p += MAXOUT; A.gb = p; p += 20;
+  A.perm64 = (int *)p; p += (n + 256 + 1) / 2;
   if ((long)((char *)p - (char *)smem) > lds_bytes) {
     for (int u = blockIdx.x * NT + tid; u < n_units; u += gridDim.x * NT) status[u] = 3;
     return;
   }
+  if (tid == 0) build_perm(L, W, A.perm64, WAVE);
+  __syncthreads();
   const int unit = order ? order[blockIdx.x] : (int)blockIdx.x;
   adjoint_unit<NQ, NT>(unit, tables, tstride, shtab, dtab, dth, unit_cosmo, unit_k, tau_out, tau_out_sh, nout, mesh, mstride, mesh_len, gbar,
                        ckpt, cstride, grad, lam0, tbar, status, L, W, A, tid);
