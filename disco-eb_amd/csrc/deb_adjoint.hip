@@ -768,6 +768,61 @@ __device__ __forceinline__ void rows_dot(const BgView &b, const DirCtx<ND> &c, c
   }
 }
 
+// rows_dot over the dense-block rows alone (one slot per dense row, in block order)
+template <int ND, class YF>
+__device__ __forceinline__ void rows_dot_dense(const DirCtx<ND> &c, const PLay &L, double k, const PBgX<Dn<ND>> &g, const YF &yv,
+                                               const double *r, Ws &W, int lane, double (&acc)[ND]) {
+  typedef Dn<ND> D;
+  Met<D> m;
+  metric<D>(c.bd, L, k, g, yv, m);
+  const int nit = (L.m + WAVE - 1) / WAVE;
+  for (int it = 0; it < nit; ++it) {  // uniform trip count: every lane takes part in the shuffles
+    const int slot = lane + it * WAVE;
+    const int i = slot < L.m ? W.dmap[slot] : -1;
+    const int src = VQ_LANE0 + W.rbin[i >= 0 ? i : 0];
+    D vsh;
+    vsh.v = __shfl(g.vl.v, src);
+#pragma unroll
+    for (int l = 0; l < ND; ++l) vsh.d[l] = __shfl(g.vl.d[l], src);
+    auto vrow = [&](int) { return vsh; };
+    if (i >= 0) {
+      const D f = core_row<D>(i, c.bd, L, k, g, m, yv, vrow);
+      const double ri = r[i];
+#pragma unroll
+      for (int l = 0; l < ND; ++l) acc[l] += ri * f.d[l];
+    }
+  }
+}
+
+// The tail rows' part of sum_s r_s^T dA k_s: a tail row's coefficients (lo, di, up) do not depend on
+// the stage, so its share is lo' pm_i + di' p0_i + up' pp_i with the stage sums pm_i = sum_s r_s,i
+// k_s,i-st, p0_i = sum_s r_s,i k_s,i, pp_i = sum_s r_s,i k_s,i+st formed beforehand
+template <int ND>
+__device__ __forceinline__ void rows_dot_tails(const PLay &L, double k, const PBgX<Dn<ND>> &g, const double *pm, const double *p0,
+                                               const double *pp, Ws &W, int lane, double (&acc)[ND]) {
+  typedef Dn<ND> D;
+  const int n = L.n;
+  const int nit = (n + WAVE - 1) / WAVE;
+  for (int it = 0; it < nit; ++it) {  // uniform trip count: every lane takes part in the shuffles
+    const int i = lane + it * WAVE;
+    const int ic = i < n ? i : 0;
+    const int src = VQ_LANE0 + W.rbin[ic];
+    D vsh;
+    vsh.v = __shfl(g.vl.v, src);
+#pragma unroll
+    for (int l = 0; l < ND; ++l) vsh.d[l] = __shfl(g.vl.d[l], src);
+    auto vrow = [&](int) { return vsh; };
+    const int rk = i < n ? W.rk[i] : -1;
+    if (rk >= 0) {
+      const int ch = rk >> 8, l = rk & 255;
+      D lo, di, up;
+      tail_coef<D>(ch, l, L, k, g, W.tca[i], W.tcb[i], vrow, lo, di, up);
+#pragma unroll
+      for (int d = 0; d < ND; ++d) acc[d] += lo.d[d] * pm[i] + di.d[d] * p0[i] + up.d[d] * pp[i];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // The output map: 20 fields of an interpolated state, with the RSA projection
 // (approximations.py:84-176, perturbations.py:528-677), contracted with gb.
@@ -1086,11 +1141,25 @@ __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict_
       }
       __syncthreads();
     }
-    // J(t0, y0) and df/dtau(t0, y0): rho = sum_s k_s[0] r_s
+    // J(t0, y0) and df/dtau(t0, y0): rho = sum_s k_s[0] r_s; and the tail rows' stage sums
     for (int p = tid; p < n; p += NT) {
       double r = 0.0;
       for (int s = 0; s < 8; ++s) r += K[s * n] * A.kb[s * n + p];
       A.rho[p] = r;
+      double pm = 0.0, p0 = 0.0, pp = 0.0;
+      const int rk = W.rk[p];
+      if (rk >= 0) {
+        const int st = W.rs[p];
+        for (int s = 0; s < 8; ++s) {
+          const double rs = A.kb[s * n + p];
+          pm += rs * K[s * n + p - st];
+          p0 += rs * K[s * n + p];
+          if (p + st < n) pp += rs * K[s * n + p + st];
+        }
+      }
+      A.ust[p] = pm;
+      A.usb[p] = p0;
+      A.yn[p] = pp;
     }
     __syncthreads();
     const double a0 = W.y0[0];
@@ -1099,14 +1168,16 @@ __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict_
 #pragma unroll
       for (int l = 0; l < ND; ++l) acc[l] = 0.0;
       Dn<ND> td, ad;
-      // (b) sum_s r_s^T d(A(t0, a0)) k_s, rows >= 1
+      // (b) sum_s r_s^T d(A(t0, a0)) k_s, rows >= 1: dense rows stage by stage, tail rows through
+      // their stage sums (A.ust, A.usb, A.yn are free here)
       dual_point<ND>(dc, T0, a0, td, ad);
       PBgX<Dn<ND>> gd;
       pbg_eval_ad<ND>(b, dc.bd, dc.dt, L, k, td, ad, gd, lane);
       for (int s = 0; s < 8; ++s) {
         auto yv = [&](int ii) { return K[s * n + ii]; };
-        rows_dot<ND>(b, dc, L, k, gd, yv, A.kb + s * n, 1, W, A.perm64, lane, acc);
+        rows_dot_dense<ND>(dc, L, k, gd, yv, A.kb + s * n, W, lane, acc);
       }
+      rows_dot_tails<ND>(L, k, gd, A.ust, A.usb, A.yn, W, lane, acc);
       // (c) d/da of rho^T df/d(dir) at (t0, (a, y0[1:])), central difference in a
       const double da = 1e-5 * a0;
       double accp[ND], accm[ND];
