@@ -169,8 +169,9 @@ __global__ void source_point_kernel(const double *__restrict__ t, const double *
   const int lmaxg = prob.lmaxg, lmaxgp = prob.lmaxgp, lmaxr = prob.lmaxr, nq = prob.nqmax;
   const int n = 7 + (lmaxg + 1) + (lmaxgp + 1) + (lmaxr + 1) + nq * (prob.lmaxnu + 1) + 2;
   const int idxg = 7, idxgp = 8 + lmaxg, idxr = 9 + lmaxg + lmaxgp, iq0 = 10 + lmaxg + lmaxgp + lmaxr;
-  const SrcPt s = source_point(b, y + idx * n, yp + idx * n, n, idxg, idxgp, idxr, iq0, nq, qw, kk[ik], aexp[j], V + j, n_tau,
-                               t[DEB_HDR_TAU0] - tau[j]);
+  const double *ys = y + idx * n, *yps = yp + idx * n;
+  const SrcPt s = source_point(b, [&](int i) { return ys[i]; }, [&](int i) { return yps[i]; }, n, idxg, idxgp, idxr, iq0, nq, qw, kk[ik],
+                               aexp[j], V + j, n_tau, t[DEB_HDR_TAU0] - tau[j]);
   const long plane = (long)n_k * n_tau;
   out[1 * plane + idx] = s.S2;
   out[2 * plane + idx] = s.S3;
@@ -197,7 +198,8 @@ __global__ void nu_stage_kernel(const double *__restrict__ t, const double *__re
   BgView b;
   b.init(t);
   const double a = aexp[j], aH = b.aprimeoa(a);
-  const NuPt nu = nu_point(y + idx * n, yp + idx * n, iq0, nq, qw, a, b.amnu, aH);
+  const double *ys = y + idx * n, *yps = yp + idx * n;
+  const NuPt nu = nu_point([&](int i) { return ys[i]; }, [&](int i) { return yps[i]; }, iq0, nq, qw, a, b.amnu, aH);
   const long plane = (long)n_k * n_tau;
   out[0 * plane + idx] = nu.drhonu;
   out[1 * plane + idx] = nu.dpnu;

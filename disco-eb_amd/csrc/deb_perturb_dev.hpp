@@ -359,6 +359,93 @@ __device__ __forceinline__ void pbg_eval_lanes(const BgView &b, const PLay &L, d
   g.tca = !g.rsa && b.use_tca && !uncoupled;
 }
 
+// A background that one lane evaluates for its own (tau, a), independently of the other lanes
+// (the output-parallel source epilogue): the bins' velocity factors are the lane's own.
+template <int NQ> struct PBgOwn : PBg<double> {
+  static constexpr int NB = NQ > 0 ? NQ : NQMAX;
+  double vo[NB], ivo[NB];
+  __device__ __forceinline__ double vq_u(int q) const { return vo[q]; }
+  __device__ __forceinline__ double ivq_u(int q) const { return ivo[q]; }
+};
+
+// pbg_eval_lanes for one lane alone: the same expressions, each evaluated by this lane (no lane
+// broadcasts, no shared spline-interval cache: the intervals come from spl_index and
+// spl_index_uniform, the indices a cache hit would hold), so the fields agree with
+// pbg_eval_lanes to the bit.  Lanes may differ in (tau, a) and may be inactive.
+template <int NQ>
+__device__ __forceinline__ void pbg_eval_own(const BgView &b, const PLay &L, double k, double tau, double a, PBgOwn<NQ> &g) {
+  g.a = a;
+  g.tau = tau;
+  const double loga = log(a);
+  const int i = spl_index(b.loga, b.N, loga);
+  const double x0 = b.loga[i], x1 = b.loga[i + 1];
+  const double cs2c = cubic_at(x0, x1, b.cs2a[i], b.cs2a[i + 1], b.S_cs2a_of_loga[i], b.S_cs2a_of_loga[i + 1], loga);
+  g.xe = cubic_at(x0, x1, b.xe[i], b.xe[i + 1], b.S_xe_of_loga[i], b.S_xe_of_loga[i + 1], loga);
+  const double dxe = cubic_deriv_at(x0, x1, b.xe[i], b.xe[i + 1], b.S_xe_of_loga[i], b.S_xe_of_loga[i + 1], loga);
+  const int j = spl_index_uniform(b.nu_x0, b.nu_dxinv, DEB_NNU, loga);
+  const double a2 = a * a;
+  g.w_Q = b.w0 + b.wa * (1.0 - a);
+  g.amnu_a = a * b.amnu;
+  const double rhonu = exp(cubic_at(b.nu_loga[j], b.nu_loga[j + 1], b.nu_lr[j], b.nu_lr[j + 1], b.nu_lrS[j], b.nu_lrS[j + 1], loga));
+  const double pnu = exp(cubic_at(b.nu_loga[j], b.nu_loga[j + 1], b.nu_lp[j], b.nu_lp[j + 1], b.nu_lpS[j], b.nu_lpS[j + 1], loga));
+  const double xq = -3.0 * (1.0 + b.w0 + b.wa) * loga + 3.0 * (a - 1.0) * b.wa;
+  g.rho_Q = exp(xq);
+  // division pass 1
+  double num = cs2c, den = a;
+  g.cs2 = num / den;
+  num = b.grhom * b.Omegam;
+  const double t1 = num / den;
+  num = b.grhog + b.grhor * (b.Neff + b.Nmnu * rhonu);
+  const double t2 = num / a2;
+  num = b.grhog;
+  den = b.grhom * b.Omegab * a;
+  g.pb43 = 4.0 / 3.0 * (num / den);
+  num = b.grhog + b.grhor * b.Neff;
+  const double gin = num / 3.0;
+  num = g.xe * b.akthom;
+  g.opac = num / a2;
+  const double tk = 1.0 / fmax(k, 1e-30);
+  g.ia = 1.0 / a;
+  g.ia2 = 1.0 / a2;
+  const double grho = t1 + t2 + b.grhom * b.OmegaDE * g.rho_Q * a2 + b.grhom * b.Omegak;
+  // division pass 2
+  num = gin + b.grhor * b.Nmnu * pnu;
+  const double gp1 = num / a2;
+  g.tauc = 1.0 / fmax(g.opac, 1e-30);
+  g.ipb = 1.0 / (1.0 + g.pb43);
+  g.ipbm = 1.0 / fmax(g.pb43, 1e-30);
+  g.aH = sqrt(grho / 3.0);
+#pragma unroll
+  for (int q = 0; q < PBgOwn<NQ>::NB; ++q) {
+    if (q >= L.nq) break;
+    const double aq = g.amnu_a / L.q[q];
+    num = 1.0 + aq * aq;
+    g.ivo[q] = sqrt(num / 1.0);
+    g.vo[q] = 1.0 / g.ivo[q];
+  }
+  g.vl = 0.0;
+  g.ivl = 0.0;
+  g.xeprime = dxe * g.aH;
+  const double gpres = gp1 + g.w_Q * b.grhom * b.OmegaDE * g.rho_Q * a2;
+  g.aHH = 0.5 * (g.aH * g.aH - gpres);
+  const double wprime = -b.wa * g.aH * a;
+  const double tv = tau, tc = g.tauc;
+  // division pass 3
+  const double w3 = wprime / 3.0;
+  g.taucprime = g.tauc * (2.0 * g.aH - g.xeprime / fmax(g.xe, 1e-30));
+  const double th = 1.0 / fmax(g.aH, 1e-30);
+  const double r_tv = tc / fmax(tv, 1e-30), r_tk = tc / fmax(tk, 1e-30);
+  g.iaH = th;
+  // division pass 4
+  const double w4 = w3 / ((1.0 + g.w_Q) + 1e-6), r_th = tc / fmax(th, 1e-30);
+  g.ca2_Q = g.w_Q - w4 / g.aH;
+  // approximation switches (approximations.py:50-75)
+  g.rsa = b.use_rsa && (r_tv > b.rsa_c) && (tv * k > b.rsa_k);
+  g.ufa = !g.rsa && b.use_ufa && (tv * k > b.ufa_k);
+  bool uncoupled = (r_tk > b.tca_k) || ((r_th > b.tca_h) && (r_tk > 0.1 * b.tca_k));
+  g.tca = !g.rsa && b.use_tca && !uncoupled;
+}
+
 // A PBg<double> in LDS, handed from the wavefront that evaluated it to the whole workgroup:
 // the uniform fields, the per-bin velocity factors (lanes VQ_LANE0 + q) and the switches.
 // x0 carries the row-0 unknown of the current stage solve.
@@ -399,9 +486,10 @@ template <class T> struct Met {
   T hprime, etaprime, alpha, dgrho, dgpres, dg, tg, sg, dr, tr, sr, dbp_tca, tbp_tca, dgp_tca, tgp_tca;
 };
 
-// BV: BgView, or a view whose density parameters carry derivatives (the adjoint's)
-template <class T, class Y, class BV>
-__device__ __forceinline__ void metric(const BV &b, const PLay &L, double k, const PBg<T> &g, const Y &y, Met<T> &m) {
+// BV: BgView, or a view whose density parameters carry derivatives (the adjoint's); G: PBg<T>
+// or a type derived from it (its vq_u / ivq_u give the bins' velocity factors)
+template <class T, class Y, class BV, class G>
+__device__ __forceinline__ void metric(const BV &b, const PLay &L, double k, const G &g, const Y &y, Met<T> &m) {
   const int n = L.n, nq = L.nq, iq0 = L.iq0;
   const auto Omegac = b.Omegam - b.Omegab;
   const double eta = y(2), deltac = y(3), thetac = y(4), deltab = y(5), thetab = y(6);
@@ -1232,6 +1320,55 @@ __device__ __forceinline__ double starting_time(const BgView &b, double k) {
   return exp(fmin(lt[1], lt[0]));
 }
 
+// The RSA projection's switch and fields at (tau, y) (approximations.py:84-176): when `on`,
+// the photon and massless-neutrino l = 0, 1 moments become (dg, tg) and (dr, tr) and the rest of
+// both hierarchies and the polarisation hierarchy zero (rsa_value).  y(i): the state's row i.
+struct RsaProj {
+  bool on;
+  double dg, tg, dr, tr;
+};
+template <class Y>
+__device__ __forceinline__ RsaProj rsa_project(const BgView &b, const PLay &L, double k, double tau, const Y &y) {
+  RsaProj r{false, 0.0, 0.0, 0.0, 0.0};
+  if (!b.use_rsa) return r;
+  const int n = L.n, nq = L.nq, iq0 = L.iq0;
+  const double a = y(0), loga = log(a);
+  double aH = b.aprimeoa(a);
+  int i = spl_index(b.loga, b.N, loga);
+  double cs2 = spl_eval_at(b.loga, b.cs2a, b.S_cs2a_of_loga, i, loga) / a;
+  double xe = spl_eval_at(b.loga, b.xe, b.S_xe_of_loga, i, loga);
+  double xep = spl_deriv_at(b.loga, b.xe, b.S_xe_of_loga, i, loga) * aH;
+  double opac = xe * b.akthom / (a * a);
+  double tc = 1.0 / fmax(opac, 1e-30);
+  double tcp = tc * (2.0 * aH - xep / fmax(xe, 1e-30));
+  r.on = (tc / fmax(tau, 1e-30) > b.rsa_c) && (tau * k > b.rsa_k);
+  if (r.on) {
+    double drhonu = 0.0;
+    for (int q = 0; q < nq; ++q) {
+      double aq = a * b.amnu / L.q[q];
+      drhonu += L.w[q] * y(iq0 + q) * sqrt(1.0 + aq * aq);
+    }
+    double rho_Q = pow(a, -3.0 * (1.0 + b.w0 + b.wa)) * exp(3.0 * (a - 1.0) * b.wa);
+    double eta = y(2), deltab = y(5), thetab = y(6);
+    double dgw = b.grhom * ((b.Omegam - b.Omegab) * y(3) + b.Omegab * deltab) / a + b.grhor * b.Nmnu * drhonu / (a * a) +
+                 b.grhom * b.OmegaDE * y(n - 2) * rho_Q * a * a;
+    double hp = (2.0 * k * k * eta + dgw) / fmax(aH, 1e-30);
+    double sk2 = fmax(k * k, 1e-30), stc = fmax(tc, 1e-30);
+    r.dg = 4.0 / sk2 * (aH * hp - sk2 * eta) + 4.0 / (sk2 * stc) * (thetab + 0.5 * hp);
+    r.tg = -0.5 * hp + 3.0 / (sk2 * stc) * (-tcp / stc * (thetab + 0.5 * hp) + (-aH * thetab + cs2 * sk2 * deltab + sk2 * eta));
+    r.dr = 4.0 / sk2 * (aH * hp - sk2 * eta);
+    r.tr = -0.5 * hp;
+  }
+  return r;
+}
+// row j of the projected state, v its value before the projection (p.on)
+__device__ __forceinline__ double rsa_value(const PLay &L, const RsaProj &p, int j, double v) {
+  if (j >= L.idxg && j <= L.idxg + L.lmaxg) return (j == L.idxg) ? p.dg : (j == L.idxg + 1 ? p.tg : 0.0);
+  if (j >= L.idxgp && j <= L.idxgp + L.lmaxgp) return 0.0;
+  if (j >= L.idxr && j <= L.idxr + L.lmaxr) return (j == L.idxr) ? p.dr : (j == L.idxr + 1 ? p.tr : 0.0);
+  return v;
+}
+
 // RSA output projection (approximations.py:84-176) and output conversion
 // (perturbations.py:528-677) of the interpolated state y (LDS), written to dst
 template <int NT>
@@ -1239,38 +1376,10 @@ __device__ __forceinline__ void emit_output(const BgView &b, const PLay &L, doub
   const int n = L.n, nq = L.nq, iq0 = L.iq0;
   const double a = y[0], loga = log(a);
   if (b.use_rsa) {
-    double aH = b.aprimeoa(a);
-    int i = spl_index(b.loga, b.N, loga);
-    double cs2 = spl_eval_at(b.loga, b.cs2a, b.S_cs2a_of_loga, i, loga) / a;
-    double xe = spl_eval_at(b.loga, b.xe, b.S_xe_of_loga, i, loga);
-    double xep = spl_deriv_at(b.loga, b.xe, b.S_xe_of_loga, i, loga) * aH;
-    double opac = xe * b.akthom / (a * a);
-    double tc = 1.0 / fmax(opac, 1e-30);
-    double tcp = tc * (2.0 * aH - xep / fmax(xe, 1e-30));
-    bool do_rsa = (tc / fmax(tau, 1e-30) > b.rsa_c) && (tau * k > b.rsa_k);
-    if (do_rsa) {
-      double drhonu = 0.0;
-      for (int q = 0; q < nq; ++q) {
-        double aq = a * b.amnu / L.q[q];
-        drhonu += L.w[q] * y[iq0 + q] * sqrt(1.0 + aq * aq);
-      }
-      double rho_Q = pow(a, -3.0 * (1.0 + b.w0 + b.wa)) * exp(3.0 * (a - 1.0) * b.wa);
-      double eta = y[2], deltab = y[5], thetab = y[6];
-      double dgw = b.grhom * ((b.Omegam - b.Omegab) * y[3] + b.Omegab * deltab) / a + b.grhor * b.Nmnu * drhonu / (a * a) +
-                   b.grhom * b.OmegaDE * y[n - 2] * rho_Q * a * a;
-      double hp = (2.0 * k * k * eta + dgw) / fmax(aH, 1e-30);
-      double sk2 = fmax(k * k, 1e-30), stc = fmax(tc, 1e-30);
-      double dg = 4.0 / sk2 * (aH * hp - sk2 * eta) + 4.0 / (sk2 * stc) * (thetab + 0.5 * hp);
-      double tg = -0.5 * hp + 3.0 / (sk2 * stc) * (-tcp / stc * (thetab + 0.5 * hp) + (-aH * thetab + cs2 * sk2 * deltab + sk2 * eta));
-      double dr = 4.0 / sk2 * (aH * hp - sk2 * eta), tr = -0.5 * hp;
+    const RsaProj pr = rsa_project(b, L, k, tau, [&](int i) { return y[i]; });
+    if (pr.on) {  // uniform: every thread has the same inputs
       __syncthreads();
-      for (int j = tid; j < n; j += NT) {
-        double v = y[j];
-        if (j >= L.idxg && j <= L.idxg + L.lmaxg) v = (j == L.idxg) ? dg : (j == L.idxg + 1 ? tg : 0.0);
-        else if (j >= L.idxgp && j <= L.idxgp + L.lmaxgp) v = 0.0;
-        else if (j >= L.idxr && j <= L.idxr + L.lmaxr) v = (j == L.idxr) ? dr : (j == L.idxr + 1 ? tr : 0.0);
-        y[j] = v;
-      }
+      for (int j = tid; j < n; j += NT) y[j] = rsa_value(L, pr, j, y[j]);
       __syncthreads();
     }
   }
@@ -1355,17 +1464,38 @@ struct SrcOut {
   int isw, n_units;
 };
 
-// The RSA projection of the interpolated state u (LDS, in place), its right-hand side (into W.f,
-// free between steps) and the source point on thread 0, written for (unit, output j).
-template <int NT>
-__device__ __forceinline__ void emit_sources(const BgView &b, const PLay &L, double k, double tau, double *u, int unit, int j, int nout,
-                                             const SrcOut &so, Ws &W, int tid) {
-  emit_output<NT>(b, L, k, tau, u, nullptr, 1, tid);
-  __syncthreads();
-  rhs_eval<NT>(b, L, k, tau, u, W.f, W, tid);
-  __syncthreads();
-  if (tid == 0) {
-    const SrcPt s = source_point(b, u, W.f, L.n, L.idxg, L.idxgp, L.idxr, L.iq0, L.nq, so.qw, k, so.aexp[j], so.V + j, nout,
+// The outputs j0 .. j1 - 1 of one accepted step (tprev, tnext] of a unit, one output per thread
+// (NT at a time): each thread interpolates the step's end states y0, y1 (LDS) to its output time,
+// applies the RSA projection, evaluates the background there on its own (pbg_eval_own) and forms
+// the source point, reading the right-hand side rows that it needs (rhs_rows' row formulas, on
+// the projected state) as it needs them; nothing is shared between the threads, so the outputs
+// of a step cost one source point's latency instead of one each.  The arithmetic of every
+// quantity is that of the staged chain (emit_output's projection, rhs_eval, source_point).
+template <int NQ, int NT>
+__device__ __forceinline__ void emit_sources(const BgView &b, const PLay &L, double k, double tprev, double tnext, const double *y0,
+                                             const double *y1, const double *__restrict__ tout, int j0, int j1, int unit, int nout,
+                                             const SrcOut &so, const Ws &W, int tid) {
+  for (int j = j0 + tid; j < j1; j += NT) {
+    const double tau = tout[j];
+    const double th = (tau - tprev) / (tnext - tprev);
+    auto yi = [&](int i) { return y0[i] + th * (y1[i] - y0[i]); };
+    const RsaProj pr = rsa_project(b, L, k, tau, yi);
+    auto yv = [&](int i) { return pr.on ? rsa_value(L, pr, i, yi(i)) : yi(i); };
+    PBgOwn<NQ> g;
+    pbg_eval_own<NQ>(b, L, k, tau, yv(0), g);
+    Met<double> m;
+    metric<double>(b, L, k, g, yv, m);
+    auto vrow = [&](int q) { return g.vq_u(q); };
+    auto row = [&](int i) -> double {  // right-hand side row i (rhs_rows)
+      const int rk = W.rk[i];
+      if (rk < 0) return core_row<double>(i, b, L, k, g, m, yv, vrow);
+      const int ch = rk >> 8, l = rk & 255, s = W.rs[i];
+      double lo, di, up;
+      tail_coef<double>(ch, l, L, k, g, W.tca[i], W.tcb[i], vrow, lo, di, up);
+      const double yn = (up != 0.0) ? yv(i + s) : 0.0;
+      return lo * yv(i - s) + di * yv(i) + up * yn;
+    };
+    const SrcPt s = source_point(b, yv, row, L.n, L.idxg, L.idxgp, L.idxr, L.iq0, L.nq, so.qw, k, so.aexp[j], so.V + j, nout,
                                  b.t[DEB_HDR_TAU0] - tau);
     const long plane = (long)so.n_units * nout, o = (long)unit * nout + j;
     so.planes[1 * plane + o] = s.S2;
@@ -1636,14 +1766,20 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
     bool keep = (rs != nullptr) || en < 1.0;
     double factor = (rs != nullptr) ? 1.0 : pid.adapt(en, keep, cfg);
     if (keep) {
-      while (isave < nout && tout[isave] <= tnext) {
-        double th = (tout[isave] - tprev) / (tnext - tprev);
-        for (int i = tid; i < n; i += NT) u[i] = y0[i] + th * (W.y1[i] - y0[i]);
-        __syncthreads();
-        if constexpr (SRC) emit_sources<NT>(b, L, k, tout[isave], u, unit, isave, nout, *so, W, tid);
-        else emit_output<NT>(b, L, k, tout[isave], u, yo + (long)isave * wout, prob.return_full, tid);
-        __syncthreads();
-        ++isave;
+      if constexpr (SRC) {  // every output of the step at once
+        int je = isave;
+        while (je < nout && tout[je] <= tnext) ++je;
+        emit_sources<NQ, NT>(b, L, k, tprev, tnext, y0, W.y1, tout, isave, je, unit, nout, *so, W, tid);
+        isave = je;
+      } else {
+        while (isave < nout && tout[isave] <= tnext) {
+          double th = (tout[isave] - tprev) / (tnext - tprev);
+          for (int i = tid; i < n; i += NT) u[i] = y0[i] + th * (W.y1[i] - y0[i]);
+          __syncthreads();
+          emit_output<NT>(b, L, k, tout[isave], u, yo + (long)isave * wout, prob.return_full, tid);
+          __syncthreads();
+          ++isave;
+        }
       }
       __syncthreads();  // every wave has read y0 for the error norm before it is replaced
       for (int i = tid; i < n; i += NT) y0[i] = W.y1[i];
@@ -1841,14 +1977,19 @@ __device__ __forceinline__ void solve_batch(const double *__restrict__ tables, l
           for (int i = tid; i < n; i += NT) { W.y0[i] = ya[i]; W.y1[i] = yb[i]; }
           __syncthreads();
           isv = isave;
-          while (isv < nout && tout[isv] <= tnext) {
-            double th = (tout[isv] - tprev) / (tnext - tprev);
-            for (int i = tid; i < n; i += NT) W.u[i] = W.y0[i] + th * (W.y1[i] - W.y0[i]);
-            __syncthreads();
-            if constexpr (SRC) emit_sources<NT>(b, L, k, tout[isv], W.u, u0 + j, isv, nout, *so, W, tid);
-            else emit_output<NT>(b, L, k, tout[isv], W.u, yo + ((long)j * nout + isv) * wout, prob.return_full, tid);
-            __syncthreads();
-            ++isv;
+          if constexpr (SRC) {
+            while (isv < nout && tout[isv] <= tnext) ++isv;
+            emit_sources<NQ, NT>(b, L, k, tprev, tnext, W.y0, W.y1, tout, isave, isv, u0 + j, nout, *so, W, tid);
+            __syncthreads();  // W.y0, W.y1 are read before the next unit's states replace them
+          } else {
+            while (isv < nout && tout[isv] <= tnext) {
+              double th = (tout[isv] - tprev) / (tnext - tprev);
+              for (int i = tid; i < n; i += NT) W.u[i] = W.y0[i] + th * (W.y1[i] - W.y0[i]);
+              __syncthreads();
+              emit_output<NT>(b, L, k, tout[isv], W.u, yo + ((long)j * nout + isv) * wout, prob.return_full, tid);
+              __syncthreads();
+              ++isv;
+            }
           }
         }
         isave = isv;

@@ -19,19 +19,20 @@ namespace deb {
 constexpr int NV = 10;
 
 // massive neutrinos (perturbations.py:29-85, cmb.py:170-240): momentum integrals of the state and
-// its derivative at one point; Y, P the state and its time derivative, qw the bins
+// its derivative at one point; Y(i), P(i) row i of the state and of its time derivative, qw the bins
 struct NuPt {
   double drhonu, dpnu, fnu, shearnu, rnp, snp;
 };
-__device__ inline NuPt nu_point(const double *Y, const double *P, int iq0, int nq, const double *qw, double a, double amnu, double aH) {
+template <class YA, class PA>
+__device__ inline NuPt nu_point(const YA &Y, const PA &P, int iq0, int nq, const double *qw, double a, double amnu, double aH) {
   NuPt r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int q = 0; q < nq; ++q) {
     const double qq = qw[q], wq = qw[16 + q];
     const double aq = a * amnu / qq;
     const double v = 1.0 / sqrt(1.0 + aq * aq);
     const double vp = -0.5 * aq * aH * v * v * v;
-    const double p0 = Y[iq0 + q], p1 = Y[iq0 + nq + q], p2 = Y[iq0 + 2 * nq + q];
-    const double p0p = P[iq0 + q], p2p = P[iq0 + 2 * nq + q];
+    const double p0 = Y(iq0 + q), p1 = Y(iq0 + nq + q), p2 = Y(iq0 + 2 * nq + q);
+    const double p0p = P(iq0 + q), p2p = P(iq0 + 2 * nq + q);
     r.drhonu += wq * p0 / v;
     r.dpnu += wq * p0 * v;
     r.fnu += wq * p1;
@@ -111,26 +112,27 @@ __device__ inline double se_point(double k, double g, double pol, double chi) {
 }
 
 
-// every output of the fused source chain at one point: Y the (RSA-projected) state, P its time
-// derivative (the right-hand side at Y), a the output scale factor, V the visibility table at
+// every output of the fused source chain at one point: Y(i) row i of the (RSA-projected) state,
+// P(i) row i of its time derivative (the right-hand side at Y), a the output scale factor, V the visibility table at
 // this output time (V[r * vstride], rows as NV above), chi = tau0 - tau; qw: the momentum bins
 // (nodes at [q], weights at [16 + q])
 struct SrcPt {
   double S1, S2, S3, S4, alpha, alphap, SE, pot;  // S1: the algebraic form e^-tau_c (eta' + alpha''); pot: eta + alpha'
 };
-__device__ inline SrcPt source_point(const BgView &b, const double *Y, const double *P, int n, int idxg, int idxgp, int idxr, int iq0, int nq,
+template <class YA, class PA>
+__device__ inline SrcPt source_point(const BgView &b, const YA &Y, const PA &P, int n, int idxg, int idxgp, int idxr, int iq0, int nq,
                                      const double *qw, double k, double a, const double *V, long vstride, double chi) {
   const double opac = V[0 * vstride], opacp = V[1 * vstride], g = V[2 * vstride], g1 = V[3 * vstride], g2 = V[4 * vstride];
   const double aH = V[6 * vstride], rhonu = V[7 * vstride], rho_Q = V[8 * vstride], w_Q = V[9 * vstride];
   const NuPt nu = nu_point(Y, P, iq0, nq, qw, a, b.amnu, aH);
-  const PertPt x{Y[2],           Y[3],           Y[4],           Y[5],     Y[6],     Y[idxg],  Y[idxg + 1], Y[idxg + 2] / 2.0,
-                 P[idxg + 2] / 2.0, Y[idxr], Y[idxr + 1], Y[idxr + 2] / 2.0, P[idxr + 2] / 2.0, Y[n - 2], Y[n - 1]};
+  const PertPt x{Y(2),           Y(3),           Y(4),           Y(5),     Y(6),     Y(idxg),  Y(idxg + 1), Y(idxg + 2) / 2.0,
+                 P(idxg + 2) / 2.0, Y(idxr), Y(idxr + 1), Y(idxr + 2) / 2.0, P(idxr + 2) / 2.0, Y(n - 2), Y(n - 1)};
   const MetricPt mt = metric_point(b, k, a, aH, rhonu, rho_Q, w_Q, x, nu.drhonu, nu.fnu, nu.shearnu, nu.rnp, nu.snp);
-  const double y2[3] = {Y[idxg + 2], Y[idxgp], Y[idxgp + 2]}, p2[3] = {P[idxg + 2], P[idxgp], P[idxgp + 2]};
-  const double p3[3] = {P[idxg + 3], P[idxgp + 1], P[idxgp + 3]};
-  const PolPt pl = pol_point(k, P[idxg + 1], mt.alphap, opac, opacp, y2, p2, p3);
+  const double y2[3] = {Y(idxg + 2), Y(idxgp), Y(idxgp + 2)}, p2[3] = {P(idxg + 2), P(idxgp), P(idxgp + 2)};
+  const double p3[3] = {P(idxg + 3), P(idxgp + 1), P(idxgp + 3)};
+  const PolPt pl = pol_point(k, P(idxg + 1), mt.alphap, opac, opacp, y2, p2, p3);
   SrcPt r;
-  r.S2 = s2_point(k, g, x.deltag, mt.alphap, pl.pol, P[6], pl.polpp);
+  r.S2 = s2_point(k, g, x.deltag, mt.alphap, pl.pol, P(6), pl.polpp);
   r.S3 = s3_point(k, g1, mt.alpha, x.thetab, pl.polp);
   r.S4 = s4_point(k, g2, pl.pol);
   r.alpha = mt.alpha;
