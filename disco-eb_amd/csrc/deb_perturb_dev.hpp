@@ -836,42 +836,59 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 }
 
 // Gauss-Jordan inversion of the N x N row-major matrix A (LDS, N <= 64 known at compile time)
-// by one wavefront, lane r holding row r in registers: no barriers and no LDS traffic between
-// pivots.  Pivot p is the not-yet-used row with the largest |entry| in column p: one wave maximum
-// of a 64-bit key, the magnitude's bit pattern with its six lowest bits replaced by 63 - row, so
-// that ties and near-ties go to the first row; rows are never moved.  Each step replaces column p by the multipliers, in place: after the last step
-// row rmap[p] (the row that pivoted on column p) gives unknown p,
+// by one wavefront, in registers: no barriers and no LDS traffic between pivots.  With G =
+// min(3, 64 / N) column groups, lane g N + r holds columns g CW .. g CW + CW - 1 (CW = ceil(N / G))
+// of row r, so that each pivot step updates CW entries per lane instead of N; the pivot row's
+// entries and each row's pivot-column entry reach the other groups by cross-lane reads.  Pivot p
+// is the not-yet-used row with the largest |entry| in column p: one wave maximum of a 64-bit key
+// over the lanes of p's group, the magnitude's bit pattern with its six lowest bits replaced by
+// 63 - lane, so that ties and near-ties go to the first row; rows are never moved.  Each step
+// replaces column p by the multipliers, in place: after the last step row rmap[p] (the row that
+// pivoted on column p) gives unknown p,
 //   x[p] = sum_c Ainv[rmap[p]][c] b[rmap[c]],
-// so a right-hand side is scattered through pi = rmap^-1.  Ainv may alias A (each lane reads
-// its own row before writing it).
+// so a right-hand side is scattered through pi = rmap^-1.  Every entry is formed by the same
+// operations as with one row per lane.  Ainv may alias A (A is read in full before any write).
 template <int N>
 __device__ __forceinline__ void gauss_jordan_wave(const double *A, double *Ainv, int *rmap, int *pi, int lane) {
   static_assert(N <= WAVE, "one row per lane");
-  double a[N];
+  constexpr int G = (WAVE / N) < 3 ? (WAVE / N) : 3;
+  constexpr int CW = (N + G - 1) / G;
+  const int g = lane / N, r = lane - g * N;
+  const bool live = g < G;  // lanes G N .. 63 hold nothing
+  double a[CW];
 #pragma unroll
-  for (int j = 0; j < N; ++j) a[j] = (lane < N) ? A[lane * N + j] : 0.0;
-  bool used = lane >= N;
+  for (int c = 0; c < CW; ++c) {
+    const int j = g * CW + c;
+    a[c] = (live && j < N) ? A[r * N + j] : 0.0;
+  }
+  wave_sync();  // every lane has read A before any lane writes Ainv (they may alias)
+  bool used = !live;
 #pragma unroll
   for (int p = 0; p < N; ++p) {
-    const unsigned long long mag = (unsigned long long)__double_as_longlong(a[p]) & 0x7fffffffffffffc0ull;
-    const unsigned long long key = used ? 0ull : (mag | (unsigned long long)(WAVE - 1 - lane));
-    const int bi = WAVE - 1 - (int)(wave_max_u64(key) & (WAVE - 1));
-    const double ip = 1.0 / lane_bcast(a[p], bi);
-    const bool me = lane == bi;
-    const double f = a[p];
+    const int gp = p / CW, cp = p - gp * CW;  // the group and slot holding column p
+    const unsigned long long mag = (unsigned long long)__double_as_longlong(a[cp]) & 0x7fffffffffffffc0ull;
+    const unsigned long long key = (used || g != gp) ? 0ull : (mag | (unsigned long long)(WAVE - 1 - lane));
+    const int bl = WAVE - 1 - (int)(wave_max_u64(key) & (WAVE - 1));  // the pivot's lane, in group gp
+    const int bi = bl - gp * N;                                        // its row
+    const double ip = 1.0 / lane_bcast(a[cp], bl);
+    const bool me = live && r == bi;
+    const double f = __shfl(a[cp], gp * N + r);  // this row's entry in column p
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-      if (j == p) continue;
-      const double pj = lane_bcast(a[j], bi) * ip;
-      a[j] = me ? pj : a[j] - f * pj;
+    for (int c = 0; c < CW; ++c) {
+      const double pj = __shfl(a[c], g * N + bi) * ip;  // the pivot row's entry in this column, scaled
+      double v = me ? pj : a[c] - f * pj;
+      if (c == cp && g == gp) v = me ? ip : -f * ip;  // column p becomes the multipliers
+      a[c] = v;
     }
-    a[p] = me ? ip : -f * ip;
     used = used || me;
     if (lane == 0) { rmap[p] = bi; pi[bi] = p; }
   }
-  if (lane < N)
+  if (live)
 #pragma unroll
-    for (int j = 0; j < N; ++j) Ainv[lane * N + j] = a[j];
+    for (int c = 0; c < CW; ++c) {
+      const int j = g * CW + c;
+      if (j < N) Ainv[r * N + j] = a[c];
+    }
 }
 
 // Inverse of a 3x3 matrix by Gauss-Jordan with partial pivoting, in registers.
