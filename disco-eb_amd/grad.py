@@ -550,7 +550,7 @@ def cell_vjp(theta, base, kmodes, weights, *, ellmax=2500, aexp_out=None, n_fftl
     (base["tau_reion"] > 0; then by a central difference like the six).  weights: for l =
     2..ellmax or for every l.
     Returns (L, grad (9,)) as a float and a device tensor, plus C_l when return_cell."""
-    from .background import evolve_background
+    from .background import attach_background, evolve_background
     from .cmb import compute_Cell, compute_sources_fused, compute_theta_ell
     theta = np.asarray(theta, dtype=np.float64).reshape(-1)
     from .cmb import _driver_aexp
@@ -560,8 +560,9 @@ def cell_vjp(theta, base, kmodes, weights, *, ellmax=2500, aexp_out=None, n_fftl
     kk = torch.as_tensor(np.asarray(kmodes, dtype=np.float64), device=dev).reshape(-1).contiguous()
     w = _cell_weights(weights, int(ellmax), dev)
 
-    def chain(th, replay=None):
-        p = evolve_background(param=theta_to_param(th, base), num_thermo=num_thermo)
+    def chain(th, replay=None, p=None):
+        if p is None:
+            p = evolve_background(param=theta_to_param(th, base), num_thermo=num_thermo)
         src = compute_sources_fused(p, kk, aexp, rtol=rtol, atol=atol, isw=isw, step_log=replay is None, replay=replay, **lm)
         tau0 = float(p["tau_of_a_spline"].evaluate(np.array([1.0])).cpu()[0])
         th_l, kout = compute_theta_ell(ellmax, kk, src["tau"], src["S"], tau0, n_fftlog=n_fftlog)
@@ -574,11 +575,17 @@ def cell_vjp(theta, base, kmodes, weights, *, ellmax=2500, aexp_out=None, n_fftl
     mesh, mlen = _accepted_mesh(src0["step_log"], src0["n_accept"], t1)
     del src0
     g = torch.zeros(9, dtype=torch.float64, device=dev)
-    for j in ODE_PARAMS + ((8,) if _reionized(base) else ()):
-        d = rel_step * max(abs(theta[j]), 1e-2)
+    # the shifted cosmologies' backgrounds in one launch (one workgroup each), then their chains
+    cols = ODE_PARAMS + ((8,) if _reionized(base) else ())
+    steps = [rel_step * max(abs(theta[j]), 1e-2) for j in cols]
+    shifted = [th for j, d in zip(cols, steps) for th in _shift_pair(theta, j, d)]
+    tabs = background_tables([theta_to_param(th, base) for th in shifted], num_thermo)
+    for c, (j, d) in enumerate(zip(cols, steps)):
         Ls = []
-        for th in _shift_pair(theta, j, d):
-            _, _, _, _, cl = chain(th, replay=(mesh, mlen))
+        for h in range(2):
+            i = 2 * c + h
+            p = attach_background(theta_to_param(shifted[i], base), tabs, num_thermo, i)
+            _, _, _, _, cl = chain(shifted[i], replay=(mesh, mlen), p=p)
             Ls.append((w * cl).sum())
         g[j] = (Ls[0] - Ls[1]) / (2.0 * d)
     # n_s: d C_l / d n_s = trapezoid over ln k of ln(k / k_p) (k / k_p)^(n_s - 1) Theta_l^2, the cross
