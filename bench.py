@@ -341,6 +341,11 @@ def fisher_workload(n_cosmo=256):
     return thetas, [np.geomspace(1e-4, 2.0, int(n)) for n in nks]
 
 
+def _note(msg):
+    """A progress line on stderr (the JSON result stays the one line on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def measure_fisher(deb, n_cosmo=256, n_latency=16, n_plain=3):
     """The many-cosmology workload end to end: P(k, z = 0) and its 9-column Jacobian for every
     cosmology (adaptive base solves of all units in one launch, the discrete adjoint of all units
@@ -364,6 +369,7 @@ def measure_fisher(deb, n_cosmo=256, n_latency=16, n_plain=3):
     fwd = time.perf_counter() - t
     phantom = int(sum(bool((x == 4).any()) for x in g.split(st)))
     res["forward_graph"] = {"s": fwd, "k_solves_per_s": units / fwd, "phantom_cosmologies": phantom}
+    _note(f"fisher: forward graph {fwd:.2f} s")
     del g
     # forward + Jacobian of every cosmology
     torch.cuda.synchronize()
@@ -375,6 +381,7 @@ def measure_fisher(deb, n_cosmo=256, n_latency=16, n_plain=3):
     res["jacobian"] = {"s": tot, "gradients_per_s": n_cosmo / tot, "k_modes_per_s": units / tot,
                        "status_counts": {str(k): int(v) for k, v in zip(*np.unique(status, return_counts=True))},
                        "phantom_cosmologies": int((status == 4).sum()), "finite": bool(all(torch.isfinite(Js[c]).all() for c in valid))}
+    _note(f"fisher: forward + Jacobian {tot:.2f} s")
     # per-request latency: one cosmology per call
     lat = []
     for c in valid[:n_latency]:
@@ -385,6 +392,7 @@ def measure_fisher(deb, n_cosmo=256, n_latency=16, n_plain=3):
         lat.append(time.perf_counter() - t)
     res["latency_one_per_call"] = {"sample": len(lat), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
                                    "n_k": [int(grids[c].size) for c in valid[:n_latency]]}
+    _note("fisher: latency sample done")
     # the plain form on a sample, scaled by modes
     opts = dict(aexp_out=(1.0,), **LMAX, **SOLVER)
     tp, kp = 0.0, 0
@@ -499,20 +507,26 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         out["copy_calibration_bytes"] = src.numel() * 8
+    _note(f"k-mode solves: {value:.0f}/s")
     if args.grad_modes > 0:
         out["gradient"] = measure_gradient(deb, args.grad_modes)
         out["config"]["gradients_per_s"] = out["gradient"]["gradients_per_s"]
         out["config"]["gradient_method"] = "adjoint"
+        _note("gradient done")
         if args.grad_cosmologies > 0:
             out["gradient_many"] = measure_gradient_many(deb, args.grad_cosmologies, args.grad_modes)
+            _note("gradient of many cosmologies done")
     if args.los:
         out["line_of_sight"] = measure_los(deb)
+        _note("line of sight done")
     if args.cmb_chain:
         out["cmb_chain"] = measure_cmb_chain(deb)
+        _note("CMB chain done")
     if args.fisher:
         out["fisher"] = measure_fisher(deb)
     if args.cpu_sample > 0 and rank == 0:
         out["cpu_baseline"] = cpu_baseline(kmodes, args.cpu_sample, args.cpu_sample_1core)
+        _note("CPU baseline done")
     print(json.dumps(out))
 
 

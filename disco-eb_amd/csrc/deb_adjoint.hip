@@ -895,9 +895,12 @@ __device__ __forceinline__ bool out_rsa(const BgView &b, double k, double tau, d
 }
 
 // ---------------------------------------------------------------------------------------
-// the adjoint of one unit
+// the adjoint of one unit, in two parts with their own launches: REPLAY the forward replay of the
+// mesh, which writes each step's start state and record to the unit's checkpoint slot; otherwise
+// the reverse sweep and the shifted-cosmology terms, which read them.  The replay needs the
+// forward solver's LDS and registers only, so it runs at more units per compute unit.
 // ---------------------------------------------------------------------------------------
-template <int NQ, int NT>
+template <int NQ, int NT, bool REPLAY>
 __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict__ tables, long tstride, const double *__restrict__ shtab,
                                              const double *__restrict__ dtab, const double *__restrict__ dth, const int *__restrict__ unit_cosmo,
                                              const double *__restrict__ unit_k, const double *__restrict__ tau_out,
@@ -940,19 +943,22 @@ __device__ __forceinline__ void adjoint_unit(int unit, const double *__restrict_
   double *ck = ckpt + (long)unit * cstride;
   double *rec0 = ck + (long)(N + 1) * n;  // step i's record at rec0 + i rsz
   // 1. forward replay with checkpoints: the state at every step start and the step's record
-  if (tid == 0) initial_conditions(b, L, t0, k, W.y0);
-  __syncthreads();
-  int gsrc = 0;
-  for (int i = 0; i < N; ++i) {
-    for (int p = tid; p < n; p += NT) ck[(long)i * n + p] = W.y0[p];
-    const double tp = mesh_t(i), tn = mesh_t(i + 1);
-    rodas5_attempt<NQ, NT>(b, L, k, tp, tn - tp, W, tid, gsrc, tn);
-    save_record<NQ, NT>(L, W, rec0 + (long)i * rsz, tid);
-    for (int p = tid; p < n; p += NT) W.y0[p] = W.y1[p];
+  if constexpr (REPLAY) {
+    if (tid == 0) initial_conditions(b, L, t0, k, W.y0);
     __syncthreads();
-    gsrc = (NT > WAVE) ? 1 : 0;
+    int gsrc = 0;
+    for (int i = 0; i < N; ++i) {
+      for (int p = tid; p < n; p += NT) ck[(long)i * n + p] = W.y0[p];
+      const double tp = mesh_t(i), tn = mesh_t(i + 1);
+      rodas5_attempt<NQ, NT>(b, L, k, tp, tn - tp, W, tid, gsrc, tn);
+      save_record<NQ, NT>(L, W, rec0 + (long)i * rsz, tid);
+      for (int p = tid; p < n; p += NT) W.y0[p] = W.y1[p];
+      __syncthreads();
+      gsrc = (NT > WAVE) ? 1 : 0;
+    }
+    for (int p = tid; p < n; p += NT) ck[(long)N * n + p] = W.y0[p];
+    return;
   }
-  for (int p = tid; p < n; p += NT) ck[(long)N * n + p] = W.y0[p];
   // output j lies in the first step whose end reaches it
   // 2. reverse sweep
   for (int p = tid; p < n; p += NT) A.lam[p] = 0.0;
@@ -1330,8 +1336,30 @@ __global__ __launch_bounds__(NT) void adjoint_kernel(const double *__restrict__ 
   if (tid == 0) build_perm(L, W, A.perm64, WAVE);
   __syncthreads();
   const int unit = order ? order[blockIdx.x] : (int)blockIdx.x;
-  adjoint_unit<NQ, NT>(unit, tables, tstride, shtab, dtab, dth, unit_cosmo, unit_k, tau_out, tau_out_sh, nout, mesh, mstride, mesh_len, gbar,
+  adjoint_unit<NQ, NT, false>(unit, tables, tstride, shtab, dtab, dth, unit_cosmo, unit_k, tau_out, tau_out_sh, nout, mesh, mstride, mesh_len, gbar,
                        ckpt, cstride, grad, lam0, tbar, status, L, W, A, tid);
+}
+
+// The forward replay part of the adjoint (adjoint_unit<..., true>): the forward solver's LDS only.
+template <int NQ, int NT>
+__global__ __launch_bounds__(NT) void adjoint_replay_kernel(const double *__restrict__ tables, long tstride, const int *__restrict__ unit_cosmo,
+                                                             const double *__restrict__ unit_k, int n_units, const double *__restrict__ tau_out,
+                                                             int nout, const double *__restrict__ nu_qw, DebProblem prob, long lds_bytes,
+                                                             const double *__restrict__ mesh, long mstride, const int *__restrict__ mesh_len,
+                                                             double *__restrict__ ckpt, long cstride, double *__restrict__ grad,
+                                                             int *__restrict__ status, const int *__restrict__ order) {
+  if ((int)blockIdx.x >= n_units) return;
+  const int tid = threadIdx.x;
+  PLay L;
+  Ws W;
+  if (!setup_solver<NQ, NT>(prob, nu_qw, lds_bytes, L, W, tid)) {
+    for (int u = blockIdx.x * NT + tid; u < n_units; u += gridDim.x * NT) status[u] = 3;
+    return;
+  }
+  AdjWs A{};  // not read by the replay
+  const int unit = order ? order[blockIdx.x] : (int)blockIdx.x;
+  adjoint_unit<NQ, NT, true>(unit, tables, tstride, nullptr, nullptr, nullptr, unit_cosmo, unit_k, tau_out, nullptr, nout, mesh, mstride,
+                             mesh_len, nullptr, ckpt, cstride, grad, nullptr, nullptr, status, L, W, A, tid);
 }
 
 // derivative tables: dtab[c, p] = (shtab[c, 2p] - shtab[c, 2p + 1]) / (2 dth[c, p]), element-wise
@@ -1362,6 +1390,26 @@ extern "C" long deb_adjoint_lds_bytes(const DebProblem *prob, int nt) {
 
 namespace {
 typedef decltype(&deb::adjoint_kernel<0, 256>) AdjFn;
+typedef decltype(&deb::adjoint_replay_kernel<0, 256>) ReplayFn;
+template <int NT> ReplayFn pick_replay_nt(int nq) {
+  switch (nq) {
+    case 3: return deb::adjoint_replay_kernel<3, NT>;
+    case 5: return deb::adjoint_replay_kernel<5, NT>;
+    case 8: return deb::adjoint_replay_kernel<8, NT>;
+    default: return deb::adjoint_replay_kernel<0, NT>;
+  }
+}
+// compute units of the current device (once per device)
+int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int v = 0;
+    cus[dev] = (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ? v : 256;
+  }
+  return cus[dev];
+}
 AdjFn pick_adjoint(int nq) {
   switch (nq) {
     case 3: return deb::adjoint_kernel<3, 256>;
@@ -1382,6 +1430,18 @@ extern "C" int deb_launch_adjoint(const double *d_tables, int num_thermo, const 
   const long total = (long)n_cosmo * deb::NPAR * stride;
   const int blocks = (int)((total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536);
   hipLaunchKernelGGL(deb::dtab_kernel, dim3(blocks), dim3(256), 0, stream, d_shtab, d_dth, n_cosmo, stride, d_dtab);
+  // the forward replay: 256 threads while every unit has a compute unit to itself, else 128 (two
+  // units per compute unit)
+  const long flds = deb::deb_perturb_lds_bytes_dev(*prob);
+  if (flds > 160 * 1024) return -2;
+  const int rnt = (n_units <= device_cus()) ? 256 : 128;
+  ReplayFn rk = (rnt == 256) ? pick_replay_nt<256>(prob->nqmax) : pick_replay_nt<128>(prob->nqmax);
+  if (flds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)rk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds);
+    if (e != hipSuccess) return (int)e;
+  }
+  hipLaunchKernelGGL(rk, dim3(n_units), dim3(rnt), (size_t)flds, stream, d_tables, stride, d_unit_cosmo, d_unit_k, n_units, d_tau_out, nout,
+                     d_nu_qw, *prob, flds, d_mesh, mstride, d_mesh_len, d_ckpt, cstride, d_grad, d_status, d_order);
   const long lds = deb_adjoint_lds_bytes(prob, 256);
   if (lds > 160 * 1024) return -2;
   AdjFn kern = pick_adjoint(prob->nqmax);
