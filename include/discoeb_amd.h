@@ -149,7 +149,9 @@ int deb_evolve_replay(const double *d_tables, int num_thermo, const int *d_unit_
  * d_work: deb_adjoint_workspace_doubles(...) doubles (momentum bins, derivative tables,
  *   checkpoints: per unit the state at every step start and a record of every step's stages and
  *   factorisation, so the reverse sweep does not recompute the steps); d_lam0 (n_units x nvar) and d_tbar (n_units x 2), optional (NULL): the
- *   cotangents of the initial state and of the start and end times.  256 threads per unit.
+ *   cotangents of the initial state and of the start and end times.  Two launches: the forward replay
+ *   (256 threads per unit, or 128 when the units outnumber the compute units) and the reverse sweep
+ *   (256 threads per unit).
  * Replaces the forward-mode derivative jax.jacfwd takes through diffeqsolve (DirectAdjoint,
  * src/discoeb/perturbations.py:904-925) and the Rodas5 step (src/discoeb/ode_integrators_stiff.py:
  * 776-892), with the step mesh held fixed; reverse mode gives all six parameter derivatives of
