@@ -353,7 +353,21 @@ __device__ __forceinline__ void wsolveT(const PLay &L, Ws &W, AdjWs &A, double W
     if (len > 1) {
       w = v[ii] * W.rtd[ii];
       v[ii] = w;
-      for (int j = len - 2; j >= 1; --j) {
+      int j = len - 2;
+      for (; j >= 4; j -= 4) {  // four elements: every operand loaded first, then the serial recurrence
+        const int i0 = ii - s, i1 = ii - 2 * s, i2 = ii - 3 * s, i3 = ii - 4 * s;
+        const double a0 = v[i0], a1 = v[i1], a2 = v[i2], a3 = v[i3];
+        const double l0 = W.tlo[ii], l1 = W.tlo[i0], l2 = W.tlo[i1], l3 = W.tlo[i2];
+        const double r0 = W.rtd[i0], r1 = W.rtd[i1], r2 = W.rtd[i2], r3 = W.rtd[i3];
+        const double w0 = (a0 + l0 * w) * r0, w1 = (a1 + l1 * w0) * r1, w2 = (a2 + l2 * w1) * r2;
+        w = (a3 + l3 * w2) * r3;
+        v[i0] = w0;
+        v[i1] = w1;
+        v[i2] = w2;
+        v[i3] = w;
+        ii = i3;
+      }
+      for (; j >= 1; --j) {
         ii -= s;
         w = (v[ii] + W.tlo[ii + s] * w) * W.rtd[ii];
         v[ii] = w;
@@ -427,7 +441,20 @@ __device__ __forceinline__ void wsolveT(const PLay &L, Ws &W, AdjWs &A, double W
     double x = (A.cb[c] + W.Jh[c * m + pr] * v[W.dmap[pr]]) * W.rtd[h];
     v[h] = x;
     int ii = h;
-    for (int j = 1; j < len; ++j) {
+    int j = 1;
+    for (; j + 3 < len; j += 4) {  // four elements at a time, as above
+      const int i0 = ii + s, i1 = ii + 2 * s, i2 = ii + 3 * s, i3 = ii + 4 * s;
+      const double a0 = v[i0], a1 = v[i1], a2 = v[i2], a3 = v[i3];
+      const double m0 = W.tmul[ii], m1 = W.tmul[i0], m2 = W.tmul[i1], m3 = W.tmul[i2];
+      const double x0 = a0 - m0 * x, x1 = a1 - m1 * x0, x2 = a2 - m2 * x1;
+      x = a3 - m3 * x2;
+      v[i0] = x0;
+      v[i1] = x1;
+      v[i2] = x2;
+      v[i3] = x;
+      ii = i3;
+    }
+    for (; j < len; ++j) {
       const int in = ii + s;
       x = v[in] - W.tmul[ii] * x;
       v[in] = x;
