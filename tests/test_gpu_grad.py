@@ -140,6 +140,25 @@ def test_many_cosmologies_match_one_at_a_time(deb):
     assert not torch.equal(Jm[0], Jm[1])
 
 
+def test_many_units_match_one_at_a_time(deb):
+    """3 cosmologies x 200 k = 600 units: the batched base solve takes its 128-thread schedule (more
+    than 512 units) and the adjoint's forward replay runs two 128-thread units per compute unit
+    (more units than compute units), while one cosmology's 200 units take 256 threads throughout.
+    The P(k) and Jacobian of every cosmology are bitwise those of a separate pk_jacobian call."""
+    print("case: 3 cosmologies x 200 k (600 units), batched adjoint vs one call per cosmology", flush=True)
+    P = base_param()
+    th0 = deb.param_to_theta(P)
+    thetas = np.stack([th0, th0 * (1 + 0.02 * np.array([1, -1, 1, 0, 0, 1, 1, -1, 0])),
+                       th0 * (1 - 0.03 * np.array([1, 1, -1, 0, 0, -1, 1, 1, 0]))])
+    k = deb.k_grid({}, 1e-4, 0.5, 200, "log")
+    pkm, Jm = deb.pk_jacobian_many(thetas, P, k, aexp_out=(1.0,), **LM, **SOLVER)
+    assert pkm.shape == (3, 200, 1) and Jm.shape == (3, 200, 1, 9)
+    assert torch.isfinite(Jm).all()
+    for c in range(3):
+        pk, J = deb.pk_jacobian(thetas[c], P, k, aexp_out=(1.0,), **LM, **SOLVER)
+        assert torch.equal(pkm[c], pk) and torch.equal(Jm[c], J), c
+
+
 @pytest.mark.parametrize("method", ["adjoint", "fd"])
 def test_ragged_grids_match_one_at_a_time(deb, method):
     """Each cosmology on its own k-grid (lengths 5, 3, 6, 4), one of them across the phantom divide
