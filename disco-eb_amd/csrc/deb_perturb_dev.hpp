@@ -1829,20 +1829,21 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   }
 }
 
-// LDS bytes of the forward solver's carve-up (setup_solver): 22 n + m^2 + nch m + 3 m + 3 NRED NQMAX
-// + 21 NQMAX + 24 + NRED^2 doubles and three PBgShared, then m + 5 n + 512 + 4 nch + 4 m + 1 ints and
+// LDS bytes of the forward solver's carve-up (setup_solver): 22 n + m^2 + nch m + 3 m + 3 NRED nq
+// + 21 nq + 24 + NRED^2 doubles and three PBgShared, then m + 5 n + 512 + 4 nch + 4 m + 1 ints and
 // the spline cache.
 __host__ __device__ inline long deb_perturb_lds_bytes_dev(const DebProblem &p) {
   const long n = 7 + (p.lmaxg + 1) + (p.lmaxgp + 1) + (p.lmaxr + 1) + (long)p.nqmax * (p.lmaxnu + 1) + 2;
   const long m = 17 + 3 * (long)p.nqmax, nch = 3 + p.nqmax;
-  const long dbl = n * 22 + m * m + nch * m + 3 * m + 3 * NRED * NQMAX + 21 * NQMAX + 24 + NRED * NRED + 3 * (long)(sizeof(PBgShared) / 8);
+  const long nq = p.nqmax;
+  const long dbl = n * 22 + m * m + nch * m + 3 * m + 3 * NRED * nq + 21 * nq + 24 + NRED * NRED + 3 * (long)(sizeof(PBgShared) / 8);
   const long ints = m + 5 * n + 512 + 4 * nch + 4 * m + 1;
   return dbl * 8 + ints * 4 + (long)sizeof(SplCache);
 }
 
 // Layout of the state vector for the truncations in prob (NQ > 0: compile-time bin count) and
 // the carve-up of the dynamic LDS into the solver workspace, which must match
-// deb_perturb_lds_bytes(): 22 n + m^2 + nch m + 3 m + 3 NRED NQMAX + 21 NQMAX + 24 + NRED^2 doubles and three PBgShared, then
+// deb_perturb_lds_bytes(): 22 n + m^2 + nch m + 3 m + 3 NRED nq + 21 nq + 24 + NRED^2 doubles and three PBgShared, then
 // m + 5 n + 512 + 4 nch + 4 m ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
 template <int NQ, int NT>
 __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const double *__restrict__ nu_qw, long lds_bytes, PLay &L, Ws &W,
@@ -1858,15 +1859,15 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   double *p = smem;
   W.y0 = p; p += n; W.y1 = p; p += n; W.u = p; p += n; W.f = p; p += n; W.J0 = p; p += n; W.dT = p; p += n;
   W.K = p; p += 8 * n; W.tlo = p; p += n; W.tdi = p; p += n; W.tup = p; p += n; W.tmul = p; p += n; W.td = p; p += n;
-  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m; W.P = p; p += NRED * 3 * NQMAX;
+  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m; W.P = p; p += NRED * 3 * L.nq;
   W.tca = p; p += n; W.tcb = p; p += n;
-  W.G = p; p += 2 * (size_t)m; W.Minv = p; p += 9 * NQMAX; W.Q = p; p += 6 * NQMAX; W.rn = p; p += 3 * NQMAX; W.xr = p; p += 24;
+  W.G = p; p += 2 * (size_t)m; W.Minv = p; p += 9 * L.nq; W.Q = p; p += 6 * L.nq; W.rn = p; p += 3 * L.nq; W.xr = p; p += 24;
   W.R = p; p += NRED * NRED;
   W.rtd = p; p += n;
   W.pg = (PBgShared *)p; p += sizeof(PBgShared) / 8;
   W.pgc = (PBgShared *)p; p += sizeof(PBgShared) / 8;
   W.pgn = (PBgShared *)p; p += sizeof(PBgShared) / 8;
-  double *sq = p; p += NQMAX; double *sw = p; p += NQMAX; double *sdl = p; p += NQMAX;
+  double *sq = p; p += L.nq; double *sw = p; p += L.nq; double *sdl = p; p += L.nq;
   int *ip = (int *)p;
   W.dmap = ip; ip += m; W.rk = ip; ip += n; W.chead = ip; ip += nch; W.cstride = ip; ip += nch; W.clen = ip; ip += nch;
   W.cpar = ip; ip += nch; W.piv = ip; ip += m; W.pi = ip; ip += m; W.rmap = ip; ip += m; W.rs = ip; ip += n; W.rbin = ip; ip += n; W.perm = ip; ip += n + 256; W.perm1 = ip; ip += n + 256; W.pch = ip; ip += m;
