@@ -183,10 +183,26 @@ __device__ __forceinline__ void pbg_eval_ad(const BgView &b, const BgD<ND> &bd, 
 // ---------------------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------------------
+// v of the lane CTRL / ROWS name (a DPP move of both halves); a lane the move does not write,
+// and a lane whose source lies outside its row, receives 0
+template <int CTRL, int ROWS> __device__ __forceinline__ double dpp_or_zero(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = (int)(unsigned)u, hi = (int)(unsigned)(u >> 32);
+  const unsigned tlo = (unsigned)__builtin_amdgcn_update_dpp(0, lo, CTRL, ROWS, 0xf, false);
+  const unsigned thi = (unsigned)__builtin_amdgcn_update_dpp(0, hi, CTRL, ROWS, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)thi << 32) | tlo));
+}
+// Wave-wide sum, returned to every lane (all 64 lanes active): an inclusive scan within each row
+// of 16 lanes by row shifts of 1, 2, 4 and 8, then lane 15's total into rows 1 and 3 and lane
+// 31's into rows 2 and 3, so that lane 63 holds the sum, which readlane returns.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
+  v += dpp_or_zero<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_or_zero<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_or_zero<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_or_zero<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_or_zero<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_or_zero<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return lane_bcast(v, WAVE - 1);
 }
 // block-wide sums of K values per thread; red: LDS scratch of (NT / 64) K doubles; every thread
 // receives the totals
