@@ -333,9 +333,8 @@ def compute_sources_fused(param, kmodes, aexp_out, *, lmaxg=11, lmaxgp=11, lmaxr
     Equal to compute_sources_streamed's unfused chain on the same solver.  solver: "per_mode"
     (largest k first) or "batched" (batch_size modes on shared steps; num_k a multiple of it);
     threads: threads per mode or batch (0: 64 per mode, 128 per batch -- with thousands of light
-    modes, each spending most of its time on the 2500 output evaluations, more modes in flight
-    beat wider workgroups: 0.15 s per mode at 64 threads against 0.35 s at 256 for the 3000-mode
-    workload).  param: from evolve_background.  step_log: also return the attempted steps
+    modes, more modes in flight beat wider workgroups: 0.044 s at 64 threads per mode against
+    0.054 s at 128 and 0.092 s at 256 for the 3000-mode, 2500-output workload).  param: from evolve_background.  step_log: also return the attempted steps
     ("step_log": (n_k or n_k / batch_size, max_steps, 3), as evolve_units logs them); replay:
     (mesh, mesh_len) to take each mode's frozen step mesh (per-mode solver only; see evolve_units).
     Returns {"S", "S1".."S4", "SE", "alpha", "alphaprime": (n_k, n_tau), "tau": tau_out,
