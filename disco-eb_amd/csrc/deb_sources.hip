@@ -1,8 +1,9 @@
 // deb_sources.hip -- the CMB chain perturbations -> time derivatives -> source function in one
 // pass.  The per-mode and the batched perturbation kernels run with the source epilogue
-// (solve_unit / solve_batch with SRC = true): at every output time the interpolated state is
-// projected (RSA), the right-hand side is evaluated there and the source terms are formed from
-// the two, so that only the (k, tau) source planes reach HBM -- never the states and their
+// (solve_unit / solve_batch with SRC = true): the outputs of each accepted step are formed one
+// per thread -- the interpolated state projected (RSA), the right-hand side rows the source terms
+// read evaluated there, the source terms formed from the two -- so that only the (k, tau) source
+// planes reach HBM -- never the states and their
 // derivatives, 2 x nvar doubles per point, that the staged chain of the reference
 // (cmb.py:1030-1073: evolve_perturbations_batched with return_full, compute_time_derivatives,
 // compute_source_function) materialises.
