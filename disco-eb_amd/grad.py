@@ -241,7 +241,27 @@ def _setup(thetas, base, kmodes, aexp, lm, solver, num_thermo, rel_step):
     p0s = [theta_to_param(th, base) for th in thetas]
     dev = _lib.device()
     kgrids, ragged = _kgrids(kmodes, C)
-    t0 = background_tables(p0s, num_thermo)
+
+    def shifted_params(bad):
+        steps = np.zeros((C, len(ODE_PARAMS)))
+        params = []
+        for c in range(C):
+            for i, j in enumerate(ODE_PARAMS):
+                d = rel_step * max(abs(thetas[c, j]), 1e-2)
+                steps[c, i] = d
+                # a cosmology on the phantom divide replays nothing: its partners are itself (skipped too)
+                pair = (thetas[c], thetas[c]) if bad[c] else _shift_pair(thetas[c], j, d)
+                for th in pair:
+                    params.append(theta_to_param(th, base))
+        return steps, params
+
+    # the base and the shifted cosmologies' tables in one launch (one workgroup each): the
+    # phantom-divide test the kernels make, (1 + w0)(1 + w0 + wa) <= 0 unless w0 = -1 and wa = 0,
+    # decides the shifts before the base solve reports it
+    bad_pre = np.array([(not _is_lambda(th)) and (1.0 + th[6]) * (1.0 + th[6] + th[7]) <= 0.0 for th in thetas])
+    steps, params = shifted_params(bad_pre)
+    t_all = background_tables(p0s + params, num_thermo)
+    t0, tab = t_all[:C], t_all[C:]
     tau0 = tau_out_all(t0, num_thermo, aexp)
     uc0, uk0, offs = units_for(kgrids, dev)
     U = int(uk0.numel())
@@ -254,17 +274,9 @@ def _setup(thetas, base, kmodes, aexp, lm, solver, num_thermo, rel_step):
     mlen = torch.where(mlen < 1, torch.ones_like(mlen), mlen)
     mesh[:, 0] = torch.where(na < 1, torch.ones_like(mesh[:, 0]), mesh[:, 0])
     del log
-    steps = np.zeros((C, len(ODE_PARAMS)))
-    params = []
-    for c in range(C):
-        for i, j in enumerate(ODE_PARAMS):
-            d = rel_step * max(abs(thetas[c, j]), 1e-2)
-            steps[c, i] = d
-            # a cosmology on the phantom divide replays nothing: its partners are itself (skipped too)
-            pair = (thetas[c], thetas[c]) if bad[c] else _shift_pair(thetas[c], j, d)
-            for th in pair:
-                params.append(theta_to_param(th, base))
-    tab = background_tables(params, num_thermo)
+    if not np.array_equal(bad, bad_pre):  # the solve disagrees with the parameters' test: take its word
+        steps, params = shifted_params(bad)
+        tab = background_tables(params, num_thermo)
     tau = tau_out_all(tab, num_thermo, aexp)
     return dict(p0s=p0s, t0=t0, tau0=tau0, uc0=uc0, uk0=uk0, y=y, bad=bad, mesh=mesh, mlen=mlen, steps=steps, tab=tab, tau=tau,
                 offs=offs, kgrids=kgrids, ragged=ragged)
