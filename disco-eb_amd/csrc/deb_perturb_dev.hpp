@@ -637,6 +637,7 @@ __device__ __forceinline__ void tail_coef(int ch, int l, const PLay &L, double k
 struct Ws {
   double *y0, *y1, *u, *f, *J0, *dT, *K, *S, *Jh, *tlo, *tdi, *tup, *tmul, *td, *xd;
   double *Sinv;         // the buffer holding the dense (or reduced) inverse
+  double *Jd;           // diagonal of the dense block's Jacobian at the current Jacobian point
   int *dmap, *rk, *chead, *cstride, *clen, *cpar, *piv, *pi, *rmap;
   int *rs, *rbin;       // per row: chain stride (tail rows), momentum bin (massive-neutrino rows)
   int *perm;            // RHS row of each thread slot (slot tid + it NT), -1 for none
@@ -1005,54 +1006,67 @@ __device__ __forceinline__ void reduce_dense(const PLay &L, Ws &W, int tid) {
 // one wavefront it is also kept in W.pgc.  With the bordered solve and more than one wavefront,
 // waves 1.. run first(tid - 64) while wave 0 inverts the reduced matrix (g and W00 are set by
 // then).
+// same_point: this attempt retries a rejected one from the same (t0, y0), so the Jacobian
+// pieces the rejected attempt formed are still in LDS (J0, dT, the dense block's entries and its
+// diagonal Jd, Jh, G, the tail coefficients); with the bordered solve only the diagonal, which
+// carries 1/(gamma dt), is formed again, from Jd, and the entries are those a full pass would give.
 template <int MC, int NT, class F>
 __device__ __forceinline__ void factorise(const BgView &b, const PLay &L, double k, double t0, double dt, Ws &W, int tid, double &W00,
-                                          PBg<double> &g, int gsrc, const F &first) {
+                                          PBg<double> &g, int gsrc, const F &first, bool same_point = false) {
   const int n = L.n, m = L.m, nch = L.nch;
   const double gi = 1.0 / (dt * r5::gamma);
-  rhs_dual<NT>(b, L, k, t0, W.y0, W.J0, W.dT, W, tid);
+  const bool reuse = MC > 0 && same_point;
+  if (!reuse) rhs_dual<NT>(b, L, k, t0, W.y0, W.J0, W.dT, W, tid);
   if (gsrc == 0) {
     pbg_eval_lanes(b, L, k, t0, W.y0[0], g, W.sc, tid);
   } else {
     pbg_load(g, gsrc == 1 ? W.pgn : W.pgc, L, tid & (WAVE - 1));
   }
   if (NT > WAVE && gsrc != 2 && tid < WAVE) pbg_store(g, W.pgc, L, tid);
-  // dense-block columns 0..m-1 and chain-head columns m..m+nch-1, one per lane; wave w takes
-  // rows w, w + NR, ... of every column, so a wave evaluates one row formula at a time
-  constexpr int NR = NT / WAVE;
-  const int rg = __builtin_amdgcn_readfirstlane(tid / WAVE);  // uniform: the row formulas branch on scalars
-  for (int col = tid % WAVE; col < m + nch; col += WAVE) {
-    int s = (col < m) ? W.dmap[col] : W.chead[col - m];
-    auto ev = [&](int i) { return i == s ? 1.0 : 0.0; };
-    Met<double> mt;
-    metric<double>(b, L, k, g, ev, mt);
-    auto vsel = [&](int q) { return g.vq_l(L, q); };
-    // the bordered solve reads, of the massive-neutrino rows (dense rows 15 .. 15 + 3 nq - 1),
-    // only each bin's own 3 x 3 block: their other entries are not formed
-    const int nq = L.nq, cq = col - 15;
-    const bool ncol = MC > 0 && col < m && cq >= 0 && cq < 3 * nq;
-    for (int r = rg; r < m; r += NR) {
-      const int rq = r - 15;
-      if (MC > 0 && col < m && rq >= 0 && rq < 3 * nq && !(ncol && rq % nq == cq % nq)) continue;
-      // a hierarchy's l = 3 entry (a chain head) appears only in that hierarchy's l = 2 row
-      if (col >= m && r != W.cpar[col - m]) continue;
-      double v = core_row<double>(__builtin_amdgcn_readfirstlane(W.dmap[r]), b, L, k, g, mt, ev, vsel);
-      if (col < m) W.S[r * m + col] = ((r == col) ? gi : 0.0) - v;  // W_DD = I/(gamma h) - J_DD
-      else W.Jh[(col - m) * m + r] = v;
-    }
-    if (MC > 0 && col < m && rg == 0) {  // h' and eta' per unit of this unknown
-      W.G[2 * col] = mt.hprime;
-      W.G[2 * col + 1] = mt.etaprime;
-    }
-  }
-  // tail coefficients
-  for (int i = tid; i < n; i += NT) {
-    int rk = W.rk[i];
-    if (rk >= 0) {
-      double lo, di, up;
+  if (reuse) {
+    for (int r = tid; r < m; r += NT) W.S[r * m + r] = gi - W.Jd[r];
+  } else {
+    // dense-block columns 0..m-1 and chain-head columns m..m+nch-1, one per lane; wave w takes
+    // rows w, w + NR, ... of every column, so a wave evaluates one row formula at a time
+    constexpr int NR = NT / WAVE;
+    const int rg = __builtin_amdgcn_readfirstlane(tid / WAVE);  // uniform: the row formulas branch on scalars
+    for (int col = tid % WAVE; col < m + nch; col += WAVE) {
+      int s = (col < m) ? W.dmap[col] : W.chead[col - m];
+      auto ev = [&](int i) { return i == s ? 1.0 : 0.0; };
+      Met<double> mt;
+      metric<double>(b, L, k, g, ev, mt);
       auto vsel = [&](int q) { return g.vq_l(L, q); };
-      tail_coef<double>(rk >> 8, rk & 255, L, k, g, W.tca[i], W.tcb[i], vsel, lo, di, up);
-      W.tlo[i] = lo; W.tdi[i] = di; W.tup[i] = up;
+      // the bordered solve reads, of the massive-neutrino rows (dense rows 15 .. 15 + 3 nq - 1),
+      // only each bin's own 3 x 3 block: their other entries are not formed
+      const int nq = L.nq, cq = col - 15;
+      const bool ncol = MC > 0 && col < m && cq >= 0 && cq < 3 * nq;
+      for (int r = rg; r < m; r += NR) {
+        const int rq = r - 15;
+        if (MC > 0 && col < m && rq >= 0 && rq < 3 * nq && !(ncol && rq % nq == cq % nq)) continue;
+        // a hierarchy's l = 3 entry (a chain head) appears only in that hierarchy's l = 2 row
+        if (col >= m && r != W.cpar[col - m]) continue;
+        double v = core_row<double>(__builtin_amdgcn_readfirstlane(W.dmap[r]), b, L, k, g, mt, ev, vsel);
+        if (col < m) {
+          W.S[r * m + col] = ((r == col) ? gi : 0.0) - v;  // W_DD = I/(gamma h) - J_DD
+          if (r == col) W.Jd[r] = v;
+        } else {
+          W.Jh[(col - m) * m + r] = v;
+        }
+      }
+      if (MC > 0 && col < m && rg == 0) {  // h' and eta' per unit of this unknown
+        W.G[2 * col] = mt.hprime;
+        W.G[2 * col + 1] = mt.etaprime;
+      }
+    }
+    // tail coefficients
+    for (int i = tid; i < n; i += NT) {
+      int rk = W.rk[i];
+      if (rk >= 0) {
+        double lo, di, up;
+        auto vsel = [&](int q) { return g.vq_l(L, q); };
+        tail_coef<double>(rk >> 8, rk & 255, L, k, g, W.tca[i], W.tcb[i], vsel, lo, di, up);
+        W.tlo[i] = lo; W.tdi[i] = di; W.tup[i] = up;
+      }
     }
   }
   __syncthreads();
@@ -1593,9 +1607,10 @@ __device__ __forceinline__ double stage_in(int s, double y0i, double ui, const K
 // gsrc: where the Jacobian point's background comes from (see factorise).  tspec >= 0: with
 // more than one wavefront, wavefront 1 also evaluates the background at (tspec, y1[0]) into
 // W.pgn during the last stage solve: the next step's Jacobian point if this step is accepted.
+// same_point: the previous attempt was rejected, so its Jacobian pieces are reused (factorise).
 template <int NQ, int NT>
 __device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, double k, double tprev, double dt, Ws &W, int tid,
-                                              int gsrc = 0, double tspec = -1.0) {
+                                              int gsrc = 0, double tspec = -1.0, bool same_point = false) {
   constexpr int MC = NQ > 0 ? 17 + 3 * NQ : 0;
   // the first stage's right-hand side is formed beside the inversion of the reduced matrix
   constexpr bool FIRST_BESIDE_GJ = MC > 0 && NT > WAVE;
@@ -1649,7 +1664,7 @@ __device__ __forceinline__ int rodas5_attempt(const BgView &b, const PLay &L, do
   auto first = [&](int t) {
     rhs_rows<(NT > WAVE ? NT - WAVE : WAVE)>(b, L, k, g, y0, W.perm1, W, t, emit_stage(0, dt * d1));
   };
-  factorise<MC, NT>(b, L, k, tprev, dt, W, tid, W00, g, gsrc, first);
+  factorise<MC, NT>(b, L, k, tprev, dt, W, tid, W00, g, gsrc, first, same_point);
   for (int s = 0; s < 8; ++s) {
     double *ks = K + s * n;
     // df/dtau weight, selected without a run-time-indexed array (scratch memory)
@@ -1766,12 +1781,13 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   pid.prev_prev_inv = 1.0;
   int isave = 0, steps = 0, na = 0, nr = 0, st = 0;
   int gsrc = 0;  // the next Jacobian point's background: evaluate, W.pgn (accepted), W.pgc (rejected)
+  bool same = false;  // the last attempt was rejected: the next one starts from the same point
   double *K = W.K;
   double *u = W.u, *y0 = W.y0;
   while (tprev < t1) {
     if (steps >= cfg.max_steps || (rs != nullptr && ridx >= rlen)) { st = 1; break; }
     const double dt = tnext - tprev;
-    const int nanflag = rodas5_attempt<NQ, NT>(b, L, k, tprev, dt, W, tid, gsrc, fmin(tnext, t1));
+    const int nanflag = rodas5_attempt<NQ, NT>(b, L, k, tprev, dt, W, tid, gsrc, fmin(tnext, t1), same);
     const double en = sqrt(filtered_sq(k, W.y0, W.y1, K + 7 * n, nanflag, cfg) / 6.0);
     if (step_log != nullptr && tid == 0) {  // (t_prev, t_next, error norm) of every attempted step
       double *lg = step_log + ((long)unit * cfg.max_steps + steps) * 3;
@@ -1808,6 +1824,7 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
     double nt0 = keep ? tnext : tprev;
     tprev = fmin(nt0, t1);
     gsrc = (NT > WAVE) ? (keep ? 1 : 2) : 0;
+    same = !keep;
     double tn = nt0 + dt * factor;
     tnext = (tn > t1 - 1e-10) ? t1 : tn;
     if (rs != nullptr) {
@@ -1829,21 +1846,21 @@ __device__ __forceinline__ void solve_unit(int unit, const double *__restrict__ 
   }
 }
 
-// LDS bytes of the forward solver's carve-up (setup_solver): 22 n + m^2 + nch m + 3 m + 3 NRED nq
+// LDS bytes of the forward solver's carve-up (setup_solver): 22 n + m^2 + nch m + 4 m + 3 NRED nq
 // + 21 nq + 24 + NRED^2 doubles and three PBgShared, then m + 5 n + 512 + 4 nch + 4 m + 1 ints and
 // the spline cache.
 __host__ __device__ inline long deb_perturb_lds_bytes_dev(const DebProblem &p) {
   const long n = 7 + (p.lmaxg + 1) + (p.lmaxgp + 1) + (p.lmaxr + 1) + (long)p.nqmax * (p.lmaxnu + 1) + 2;
   const long m = 17 + 3 * (long)p.nqmax, nch = 3 + p.nqmax;
   const long nq = p.nqmax;
-  const long dbl = n * 22 + m * m + nch * m + 3 * m + 3 * NRED * nq + 21 * nq + 24 + NRED * NRED + 3 * (long)(sizeof(PBgShared) / 8);
+  const long dbl = n * 22 + m * m + nch * m + 4 * m + 3 * NRED * nq + 21 * nq + 24 + NRED * NRED + 3 * (long)(sizeof(PBgShared) / 8);
   const long ints = m + 5 * n + 512 + 4 * nch + 4 * m + 1;
   return dbl * 8 + ints * 4 + (long)sizeof(SplCache);
 }
 
 // Layout of the state vector for the truncations in prob (NQ > 0: compile-time bin count) and
 // the carve-up of the dynamic LDS into the solver workspace, which must match
-// deb_perturb_lds_bytes(): 22 n + m^2 + nch m + 3 m + 3 NRED nq + 21 nq + 24 + NRED^2 doubles and three PBgShared, then
+// deb_perturb_lds_bytes(): 22 n + m^2 + nch m + 4 m + 3 NRED nq + 21 nq + 24 + NRED^2 doubles and three PBgShared, then
 // m + 5 n + 512 + 4 nch + 4 m ints and the spline cache.  Returns false (nothing set up) when that exceeds lds_bytes.
 template <int NQ, int NT>
 __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const double *__restrict__ nu_qw, long lds_bytes, PLay &L, Ws &W,
@@ -1859,7 +1876,7 @@ __device__ __forceinline__ bool setup_solver(const DebProblem &prob, const doubl
   double *p = smem;
   W.y0 = p; p += n; W.y1 = p; p += n; W.u = p; p += n; W.f = p; p += n; W.J0 = p; p += n; W.dT = p; p += n;
   W.K = p; p += 8 * n; W.tlo = p; p += n; W.tdi = p; p += n; W.tup = p; p += n; W.tmul = p; p += n; W.td = p; p += n;
-  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m; W.P = p; p += NRED * 3 * L.nq;
+  W.S = p; p += (size_t)m * m; W.Jh = p; p += (size_t)nch * m; W.xd = p; p += m; W.Jd = p; p += m; W.P = p; p += NRED * 3 * L.nq;
   W.tca = p; p += n; W.tcb = p; p += n;
   W.G = p; p += 2 * (size_t)m; W.Minv = p; p += 9 * L.nq; W.Q = p; p += 6 * L.nq; W.rn = p; p += 3 * L.nq; W.xr = p; p += 24;
   W.R = p; p += NRED * NRED;
