@@ -131,7 +131,10 @@ def test_batched_full_state_cmb_settings(deb, oracle):
 
 def test_batch_of_one_is_the_per_mode_solve(deb):
     """With batch_size 1 the shared-step solver reduces to the per-mode one: same step counts and
-    the same states to rounding (the two kernels inline the same stage code)."""
+    the same states to rounding (the two kernels inline the same stage code).  The per-mode solve
+    retries a rejected step with the Jacobian pieces formed at that point, the shared-step one forms
+    them again, so with rejections present this also checks that the kept pieces are the ones a
+    full pass gives."""
     P = lcdm_param()
     tab = deb.background_tables([P], 256)
     tau = deb.tau_out_all(tab, 256, [0.01, 1.0])
@@ -141,6 +144,7 @@ def test_batch_of_one_is_the_per_mode_solve(deb):
     y1, st1, na1, nr1 = deb.evolve_units(tab, 256, uc, k, tau, **kw)
     yb, stb, nab, nrb = deb.evolve_units_batched(tab, 256, uc, k, tau, batch_size=1, **kw)
     assert torch.equal(stb, st1) and torch.equal(nab, na1) and torch.equal(nrb, nr1)
+    assert int(nr1.sum()) > 0, "no rejected steps: the retry path is not exercised"
     diff = float(((yb - y1).abs() / (y1.abs().amax(dim=(0, 1)) + 1e-300)).max())
     print(f"  batch of one vs per-mode: max scaled diff {diff:.1e} (bit-identical: {torch.equal(yb, y1)})")
     assert diff < 1e-12
